@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Benchmark: elastic2d time steps on one MI355X, with roofline and CPU-baseline figures.
+
+Workload (the BASELINE plain-elastic2d configuration): interior 2000 x 800, dh 5 m, dt 0.4 ms,
+fd_order 8, nbc 50 (padded 2108 x 908 = 1,914,064 cells), seeded random layered model (12
+jittered interfaces, vp U[1800,4500], vs = vp/1.73, Gardner rho, vacuum top row), vertical-force
+Ricker 20 Hz at (1000, 5), 2000 receivers at z = 5.  A step is one full time step (velocity sweep,
+stress sweep, gather) replayed from the captured hipGraph.
+
+value = padded cell-updates per second = N_pad * steps / t, with t the hipEvent time of exactly
+`steps` steps (inputs resident in HBM).  roofline: the dominant kernel's algorithmic bytes per
+launch (velocity sweep 36 B/cell, stress sweep 44 B/cell, FP32) over its average launch time,
+measured with hipEvents on the plan's stream.  cpu_baseline: the C oracle (-O3, OpenMP) on a
+bounded number of steps of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "fomo.jl_amd"), os.path.join(ROOT, "oracle")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+METRIC = "cell-updates/s & % HBM roofline, FD8 2D/3D; learned-ABC+FP16 vs HABC at equal error"
+HBM_PEAK_GBS = 8000.0
+NX, NZ, DH, DT, NT, F0, NBC, ORDER, SEED = 2000, 800, 5.0, 4e-4, 4000, 20.0, 50, 8, 20260101
+BYTES_VEL, BYTES_STR = 36, 44
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(vp, vs, rho, rx, rz, budget_s):
+    """Oracle FP32 elastic2d on the host cores; per-step time from two runs of different length
+    (so setup cancels)."""
+    import numpy as np
+    import oracle
+
+    oracle.lib("f32")
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    threads = min(threads, os.cpu_count() or threads)
+    w = None
+
+    def run(n):
+        nonlocal w
+        from fomo import ricker_wavelet
+        w = ricker_wavelet(F0, DT, n)[None]
+        t = time.perf_counter()
+        oracle.elastic2d(vp, vs, rho, DH, DT, n, sx=[1000], sz=[5], rx=rx, rz=rz, wavelet=w, nbc=NBC,
+                         fd_order=ORDER, source_type=1, nthreads=threads)
+        return time.perf_counter() - t
+
+    t_a = run(4)
+    t_b = run(24)
+    per_step = max((t_b - t_a) / 20.0, 1e-6)
+    n = int(max(20, min(4000, budget_s / per_step)))
+    t_c = run(4 + n)
+    per_step = (t_c - t_a) / n
+    n_pad = (NX + 2 * (NBC + ORDER // 2)) * (NZ + 2 * (NBC + ORDER // 2))
+    _ = np
+    return {"value": n_pad / per_step, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"oracle elastic2d FP32 (C, -O3, OpenMP {threads} threads), same 2000x800 workload, "
+                      f"{n} steps timed (minus a 4-step run for setup), {n * per_step:.1f} s of CPU work"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=NT)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--kernel-steps", type=int, default=200, help="steps for per-kernel event timing")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (shots and slabs would shard "
+              "across GPUs; see DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import fomo
+    import fomo._lib as L
+    from fomo.api import Plan
+    from fomo.models import layered_2d, model_digest
+
+    log("building the seeded 2000x800 layered model")
+    vp, vs, rho = layered_2d(NX, NZ, seed=SEED)
+    rx = np.arange(1, NX + 1)
+    rz = np.full(NX, 5)
+    w = fomo.ricker_wavelet(F0, DT, NT)
+    log("creating the plan (setup, upload, graph capture)")
+    plan = Plan(L.ELASTIC2D, vp, rho, DH, DT, NT, vs=vs, sx=[1000], sz=[5], rx=rx, rz=rz, wavelet=w[None],
+                nbc=NBC, fd_order=ORDER, source_type=L.SRC_FZ, use_graph=True)
+    n_pad = plan.padded[0] * plan.padded[1]
+    log(f"warmup {args.warmup} steps")
+    plan.advance(args.warmup)
+    plan.sync()
+    plan.reset()
+    plan.sync()
+    log(f"timing {args.steps} steps")
+    ms = plan.timed_advance(args.steps)
+    seis = plan.seis(2)
+    finite = bool(np.all(np.isfinite(seis[1])))
+    log(f"{args.steps} steps in {ms:.2f} ms; seismograms finite: {finite}")
+    plan.reset()
+    kt = plan.kernel_times(args.kernel_steps)
+    plan.close()
+    t_vel, t_str, t_rec = kt
+    dom_name, dom_ms, dom_bytes = ("stress_sweep", t_str, BYTES_STR) if t_str >= t_vel else \
+        ("velocity_sweep", t_vel, BYTES_VEL)
+    achieved = dom_bytes * n_pad / (dom_ms * 1e-3) / 1e9
+    log(f"kernel ms: velocity {t_vel:.4f}, stress {t_str:.4f}, gather {t_rec:.4f}")
+
+    value = n_pad * args.steps / (ms * 1e-3)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"elastic2d {NX}x{NZ} interior, padded {plan.padded[0]}x{plan.padded[1]}, fd{ORDER}, "
+                        f"nbc {NBC}, dh {DH} m, dt {DT * 1e3} ms, vertical-force Ricker {F0} Hz, {NX} receivers",
+            "padded_cells": n_pad,
+            "interior_cell_updates_per_s": NX * NZ * args.steps / (ms * 1e-3),
+            "model": f"seeded layered (SplitMix64 seed {SEED}, digest {model_digest(vp, vs, rho)})",
+            "hbm_gbs_algorithmic_per_step": (BYTES_VEL + BYTES_STR) * n_pad / (ms * 1e-3 / args.steps) / 1e9,
+            "kernel_ms": {"velocity_sweep": t_vel, "stress_sweep": t_str, "gather": t_rec},
+            "outputs_finite": finite,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom_name,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_cell": dom_bytes,
+        },
+    }
+    if not args.no_cpu:
+        log("timing the CPU baseline (oracle)")
+        out["cpu_baseline"] = cpu_baseline(vp, vs, rho, rx, rz, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

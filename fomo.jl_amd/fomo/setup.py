@@ -1,0 +1,29 @@
+"""Host-side helpers exported by Fomo.jl, computed by the native library."""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+
+
+def get_fd_coefficients(order: int) -> np.ndarray:
+    """get_fd_coefficients, src/utils/FD_utils.jl:16-19 (raises for unsupported orders)."""
+    out = np.zeros(5, np.float32)
+    rc = _lib.lib().fomo_fd_coefficients(int(order), out.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"Unsupported FD order: {order}")
+    return out[: order // 2].copy()
+
+
+def ricker_wavelet(f0: float, dt: float, nt: int) -> np.ndarray:
+    """ricker_wavelet, src/utils/ricker_wavelet.jl:6-12 (Float32, length nt)."""
+    out = np.zeros(max(int(nt), 1), np.float32)
+    _lib.check(_lib.lib().fomo_ricker_wavelet(float(f0), float(dt), int(nt), out.ctypes.data))
+    return out[: int(nt)].copy()
+
+
+def habc_coefficients(v_ref: float, dt: float, dh: float) -> tuple[float, float, float]:
+    """(q, qt, qxt) of init_habc, src/boundary/habc/habc.jl:21-28."""
+    out = np.zeros(3, np.float32)
+    _lib.check(_lib.lib().fomo_habc_coefficients(float(v_ref), float(dt), float(dh), out.ctypes.data))
+    return float(out[0]), float(out[1]), float(out[2])

@@ -1,0 +1,141 @@
+/*
+ * fomo_hip.h -- C-ABI of the MI355X (gfx950) forward-modeling engine.
+ *
+ * Plain C types only: host pointers to contiguous column-major (x fastest) Float32 / Int32
+ * arrays, sizes as int32, a status code per call and a thread-local error string.  Every call
+ * is blocking unless it says otherwise.  The library never keeps a caller's host pointer after
+ * the call returns.
+ *
+ * Index conventions are the reference's: source / receiver indices are 1-based interior grid
+ * indices (the engine adds the padding `pad = nbc + fd_order/2` itself, as
+ * src/acquisition/source.jl:22-23 and receiver.jl:17-18 do); wavelets are (n_src x nt)
+ * column-major; seismograms are (n_rec x nt) column-major, receiver fastest.
+ *
+ * Citations are to the Fomo.jl sources the entry point replaces.
+ */
+#ifndef FOMO_HIP_H
+#define FOMO_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------------ */
+#define FOMO_OK 0
+#define FOMO_EINVAL 1  /* invalid argument: the Julia binding raises ArgumentError */
+#define FOMO_EHIP 2    /* HIP runtime error */
+#define FOMO_ENOMEM 3  /* device or host allocation failed */
+#define FOMO_EUNSUP 4  /* combination not supported by this build */
+
+/* ---- equation / option enums -------------------------------------------------------------- */
+#define FOMO_ACOUSTIC2D 1
+#define FOMO_ELASTIC2D 2
+#define FOMO_ACOUSTIC3D 3
+#define FOMO_ELASTIC3D 4
+
+#define FOMO_SRC_EXPLOSIVE 0 /* elastic: wavelet added to txx and tzz (reference behaviour) */
+#define FOMO_SRC_FZ 1        /* elastic: vertical force, wavelet added to vz after the velocity HABC */
+
+#define FOMO_STORE_F32 0
+#define FOMO_STORE_F16 1
+#define FOMO_STORE_BF16 2
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *fomo_last_error(void);
+/* Library version string. */
+const char *fomo_version(void);
+/* Number of visible HIP devices (0 when no GPU is present; never fails). */
+int fomo_device_count(void);
+
+/* ---- host-side setup helpers (no GPU needed) ---------------------------------------------- */
+
+/* Staggered FD coefficients for fd_order in {2,4,6,8,10}; writes fd_order/2 floats.
+ * Replaces get_fd_coefficients, src/utils/FD_utils.jl:16-19.  Returns FOMO_EINVAL otherwise. */
+int fomo_fd_coefficients(int32_t fd_order, float *out);
+
+/* Ricker wavelet of length nt.  Replaces ricker_wavelet, src/utils/ricker_wavelet.jl:6-12. */
+int fomo_ricker_wavelet(float f0, float dt, int32_t nt, float *out);
+
+/* Higdon coefficients {q, qt, qxt} for r = v_ref*dt/dh.  Replaces the scalar part of init_habc,
+ * src/boundary/habc/habc.jl:21-28. */
+int fomo_habc_coefficients(float v_ref, float dt, float dh, float *out3);
+
+/* ---- run descriptor ----------------------------------------------------------------------- */
+typedef struct fomo_desc {
+  int32_t eq;            /* FOMO_ACOUSTIC2D ... */
+  int32_t nx, ny, nz;    /* interior grid (ny = 1 for 2D) */
+  int32_t nbc;           /* absorbing layers; pad = nbc + fd_order/2 */
+  int32_t fd_order;      /* 2,4,6,8,10 */
+  int32_t nt;            /* time steps */
+  float dh, dt;          /* grid spacing (m), time step (s) */
+  float v_ref;           /* HABC reference velocity */
+  int32_t n_src, n_rec;
+  int32_t source_type;   /* FOMO_SRC_* (elastic only) */
+  int32_t storage;       /* FOMO_STORE_* (acoustic3d only in this build; 2D is f32) */
+  int32_t snap_interval; /* 0 = no snapshots */
+  int32_t snap_plane;    /* 3D snapshots: 0 = xz, 1 = xy, 2 = yz */
+  int32_t snap_index;    /* 3D snapshots: 1-based interior index of the slice */
+  int32_t use_graph;     /* 1: one hipGraph launch per pair of steps; 0: direct launches */
+  int32_t record_p;      /* acoustic3d: also gather p (extension) */
+  int32_t reserved[7];
+} fomo_desc;
+
+/* ---- one-shot drop-in entry points -------------------------------------------------------- */
+/*
+ * acoustic2d -- replaces Fomo.acoustic2d (src/equations/acoustic2d/acoustic2d.jl:47-192) for
+ * boundary=:habc.  vp, rho: nx*nz.  wavelet: n_src x nt (already repeated per source and, if
+ * requested, scaled by dt, as acoustic2d.jl:91-97 does).  Outputs (n_rec x nt each, any may be
+ * NULL): seis_p, seis_vx, seis_vz.  snaps: (nt/snap_interval) x nx*nz interior p, or NULL.
+ * kernel_time_s: hipEvent time of the step loop (setup, upload and download excluded).
+ */
+int fomo_acoustic2d(const fomo_desc *d, const float *vp, const float *rho, const int32_t *sx, const int32_t *sz,
+                    const float *wavelet, const int32_t *rx, const int32_t *rz, float *seis_p, float *seis_vx,
+                    float *seis_vz, float *snaps, double *kernel_time_s);
+
+/*
+ * elastic2d -- replaces Fomo.elastic2d (src/equations/elastic2d/elastic2d.jl:45-163) for
+ * boundary=:habc.  Outputs seis_vx, seis_vz (n_rec x nt); snaps of interior vz.
+ */
+int fomo_elastic2d(const fomo_desc *d, const float *vp, const float *vs, const float *rho, const int32_t *sx,
+                   const int32_t *sz, const float *wavelet, const int32_t *rx, const int32_t *rz, float *seis_vx,
+                   float *seis_vz, float *snaps, double *kernel_time_s);
+
+/* ---- device-resident plans (bench, batch engine, tests) ----------------------------------- */
+typedef struct fomo_plan fomo_plan;
+
+/* Build a plan: pads the model, derives the medium and the HABC tables, uploads everything,
+ * zeroes the state and (if d->use_graph) captures the step graph.  vs may be NULL for
+ * acoustic equations; sy/ry NULL for 2D. */
+int fomo_plan_create(const fomo_desc *d, const float *vp, const float *vs, const float *rho, const int32_t *sx,
+                     const int32_t *sy, const int32_t *sz, const float *wavelet, const int32_t *rx,
+                     const int32_t *ry, const int32_t *rz, fomo_plan **out);
+/* Zero every field and seismogram, reset the step counter to 0.  Asynchronous. */
+int fomo_plan_reset(fomo_plan *p);
+/* Advance n steps (enqueue only; asynchronous on the plan's stream). */
+int fomo_plan_advance(fomo_plan *p, int32_t n);
+/* Advance n steps bracketed by hipEvents on the plan's stream; blocks; returns milliseconds. */
+int fomo_plan_timed_advance(fomo_plan *p, int32_t n, double *ms);
+int fomo_plan_sync(fomo_plan *p);
+/* Per-kernel timing: runs n steps with direct launches and a hipEvent pair around every launch on
+ * the plan's stream; writes the average milliseconds per launch of {velocity sweep, pressure or
+ * stress sweep, gather} to out3.  The state advances by n steps.  Blocks. */
+int fomo_plan_kernel_times(fomo_plan *p, int32_t n, double *out3);
+/* Steps taken since the last reset. */
+int fomo_plan_steps_done(const fomo_plan *p, int32_t *out);
+/* Copy seismograms to host (n_rec x nt each; NULL skips).  Slots: acoustic {p, vx, vz},
+ * elastic2d {vx, vz, -}, acoustic3d {vx, vy, vz, p}. */
+int fomo_plan_read_seis(fomo_plan *p, float *s0, float *s1, float *s2, float *s3);
+/* Copy the current padded field number `which` (order: acoustic2d p,vx,vz; elastic2d
+ * vx,vz,txx,tzz,txz; acoustic3d p,vx,vy,vz) to host as Float32 (decoded from the storage
+ * precision when reduced-precision storage is used). */
+int fomo_plan_read_field(fomo_plan *p, int32_t which, float *out);
+/* Padded grid dims and pad width. */
+int fomo_plan_dims(const fomo_plan *p, int32_t *nx, int32_t *ny, int32_t *nz, int32_t *pad);
+int fomo_plan_destroy(fomo_plan *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FOMO_HIP_H */

@@ -1,0 +1,576 @@
+/*
+ * oracle/fomo_oracle.c -- CPU restatement of Fomo.jl's staggered-grid forward-modeling time step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: it is imported by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and by nothing else.  The product
+ * path (fomo.jl_amd/) never links or calls it.
+ *
+ * It restates the reference's *host-visible data flow* literally -- in-place updates, the strip
+ * backup into *_old arrays, and the deterministic two-pass Higdon correction through a scratch
+ * vector indexed by the reference's 1-D frame map -- so that it is an independent witness for the
+ * GPU path, which uses a different data flow (ping-pong buffers, HABC fused into the sweep).
+ *
+ * Arithmetic: REAL is float (build liboracle_f32.so) or double (liboracle_f64.so).  The float
+ * build is compiled with -ffp-contract=off and keeps every expression in the reference's Julia
+ * evaluation order, which makes it bit-identical to the reference's uncontracted Float32 updates.
+ *
+ * Indexing: 1-based (i, j[, k]) on the padded grid, x fastest, exactly as the reference's
+ * column-major Julia arrays: element (i,j) lives at (i-1) + (j-1)*nx.
+ *
+ * Reference files followed (paths relative to the Fomo.jl root):
+ *   FD coefficients ........ src/utils/FD_utils.jl:5-11
+ *   padding / buoyancy ..... src/utils/pad_array.jl:8-67, src/utils/pad_array_3d.jl:8-89
+ *   acoustic medium ........ src/equations/acoustic2d/medium.jl:28-55
+ *   elastic medium ......... src/equations/elastic2d/medium.jl:19-74
+ *   HABC coefficients ...... src/boundary/habc/habc.jl:17-44, habc_3d.jl:23-55
+ *   frame map .............. src/boundary/habc/kernels_fused.jl:28-52
+ *   det two-pass HABC ...... src/boundary/habc/kernels_det.jl:22-172
+ *   acoustic sweeps ........ src/equations/acoustic2d/fused_kernels.jl:27-114
+ *   elastic sweeps ......... src/equations/elastic2d/fused_kernels.jl:21-124
+ *   step order ............. src/equations/acoustic2d/loop_graph.jl:28-39,
+ *                            src/equations/elastic2d/loop_graph.jl:30-41
+ *   injection / gather ..... src/acquisition/device_indexed.jl:28-134
+ *   3D acoustic ............ src/equations/acoustic3d/update_velocity.jl:15-52,
+ *                            update_pressure.jl:13-48, acoustic3d.jl:134-185
+ *   3D HABC ................ src/boundary/habc/kernels_3d.jl:117-189 (restated two-pass: every
+ *                            read is the pre-correction value, the intent stated at
+ *                            src/boundary/habc/kernels_det.jl:5-10)
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef REAL
+#define REAL float
+#endif
+
+typedef REAL real;
+
+/* ------------------------------------------------------------------------------------------ */
+/* FD_utils.jl:5-11 -- staggered coefficients, Float32 literals (decimal -> Float64 -> Float32,
+ * as Julia's Float32[...] literal does).                                                       */
+static const double FD_TABLE[5][5] = {
+    {1.0, 0, 0, 0, 0},
+    {1.125, -0.041666667, 0, 0, 0},
+    {1.171875, -0.065104167, 0.0046875, 0, 0},
+    {1.1962890625, -0.079752604167, 0.0095703125, -0.000697544643, 0},
+    {1.2115478515625, -0.089721679687, 0.0138427734375, -0.00176565987723, 0.0001186795166},
+};
+
+int orc_fd_coeffs(int order, float *out) {
+  if (order < 2 || order > 10 || (order & 1)) return -1;
+  int N = order / 2;
+  for (int l = 0; l < N; ++l) out[l] = (float)FD_TABLE[N - 1][l];
+  return N;
+}
+
+static int load_coeffs(int order, real *a) {
+  float tmp[5];
+  int N = orc_fd_coeffs(order, tmp);
+  for (int l = 0; l < N; ++l) a[l] = (real)tmp[l];
+  return N;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* pad_array.jl:8-22 -- edge replication.  The reference first extends along x over all rows,
+ * then copies whole rows along z, which equals clamp-indexing the interior.                    */
+void orc_pad2d(const float *in, int nxi, int nzi, int pad, real *out) {
+  int nx = nxi + 2 * pad, nz = nzi + 2 * pad;
+  for (int j = 1; j <= nz; ++j) {
+    int jj = j - pad;
+    if (jj < 1) jj = 1;
+    if (jj > nzi) jj = nzi;
+    for (int i = 1; i <= nx; ++i) {
+      int ii = i - pad;
+      if (ii < 1) ii = 1;
+      if (ii > nxi) ii = nxi;
+      out[(size_t)(i - 1) + (size_t)(j - 1) * nx] = (real)in[(size_t)(ii - 1) + (size_t)(jj - 1) * nxi];
+    }
+  }
+}
+
+/* pad_array.jl:30-67 harmonic buoyancy with the vacuum special cases */
+static inline real harm_buoy(real r1, real r2) {
+  if (r1 == (real)0 && r2 == (real)0) return (real)0;
+  if (r1 == (real)0) return (real)2 / r2;
+  if (r2 == (real)0) return (real)2 / r1;
+  return (real)2 / (r1 + r2);
+}
+
+void orc_buoyancy2d(const real *rho, int nx, int nz, real *bx, real *bz) {
+#define R(i, j) rho[(size_t)((i)-1) + (size_t)((j)-1) * nx]
+  memset(bx, 0, sizeof(real) * (size_t)nx * nz);
+  memset(bz, 0, sizeof(real) * (size_t)nx * nz);
+  for (int j = 1; j <= nz; ++j)
+    for (int i = 2; i <= nx; ++i) bx[(size_t)(i - 1) + (size_t)(j - 1) * nx] = harm_buoy(R(i - 1, j), R(i, j));
+  for (int j = 1; j <= nz; ++j) bx[(size_t)(j - 1) * nx] = bx[1 + (size_t)(j - 1) * nx];
+  for (int j = 1; j <= nz - 1; ++j)
+    for (int i = 1; i <= nx; ++i) bz[(size_t)(i - 1) + (size_t)(j - 1) * nx] = harm_buoy(R(i, j), R(i, j + 1));
+  for (int i = 1; i <= nx; ++i) bz[(size_t)(i - 1) + (size_t)(nz - 1) * nx] = bz[(size_t)(i - 1) + (size_t)(nz - 2) * nx];
+#undef R
+}
+
+/* elastic2d/medium.jl:56-74 */
+void orc_elastic_params2d(const real *vp, const real *vs, const real *rho, int nx, int nz, real *lam,
+                          real *mu, real *muxz, real *lam2mu) {
+  size_t n = (size_t)nx * nz;
+  for (size_t c = 0; c < n; ++c) {
+    mu[c] = rho[c] * (vs[c] * vs[c]);
+    lam[c] = rho[c] * (vp[c] * vp[c]) - (real)2 * mu[c];
+    lam2mu[c] = lam[c] + (real)2 * mu[c];
+  }
+  memset(muxz, 0, sizeof(real) * n);
+#define MU(i, j) mu[(size_t)((i)-1) + (size_t)((j)-1) * nx]
+  for (int j = 1; j <= nz - 1; ++j)
+    for (int i = 2; i <= nx; ++i) {
+      real m1 = MU(i - 1, j), m2 = MU(i, j), m3 = MU(i - 1, j + 1), m4 = MU(i, j + 1);
+      real v;
+      if (m1 == (real)0 || m2 == (real)0 || m3 == (real)0 || m4 == (real)0)
+        v = (real)0;
+      else
+        v = (real)4 / ((((real)1 / m1 + (real)1 / m2) + (real)1 / m3) + (real)1 / m4);
+      muxz[(size_t)(i - 1) + (size_t)(j - 1) * nx] = v;
+    }
+#undef MU
+  for (int j = 1; j <= nz; ++j) muxz[(size_t)(j - 1) * nx] = muxz[1 + (size_t)(j - 1) * nx];
+  for (int i = 1; i <= nx; ++i) muxz[(size_t)(i - 1) + (size_t)(nz - 1) * nx] = muxz[(size_t)(i - 1) + (size_t)(nz - 2) * nx];
+}
+
+/* habc.jl:21-28: r = v_ref*dt/dh; q = (b(beta+r)-r)/((beta+r)(1-b)); qt = (b(beta+r)-beta)/(...);
+ * qxt = b/(b-1); b = 0.45, beta = 1.  out = {q, qt, qxt}.                                     */
+void orc_habc_coeffs(float v_ref, float dt, float dh, real *out) {
+  real r = ((real)v_ref * (real)dt) / (real)dh;
+  real b = (real)0.45f, beta = (real)1;
+  real one_m_b = (real)1 - b;
+  out[0] = (b * (beta + r) - r) / ((beta + r) * one_m_b);
+  out[1] = (b * (beta + r) - beta) / ((beta + r) * one_m_b);
+  out[2] = b / (b - (real)1);
+}
+
+/* habc.jl:30-34: weight = clamp((dist - off)/(pad-1), 0, 1) computed in Float64, stored as Float32
+ * (the fp64 oracle keeps the Float64 value).  Returned here as a function of dist.           */
+static inline real habc_weight(int dist, double off, int pad) {
+  double w = ((double)dist - off) / (double)(pad - 1);
+  if (w < 0.0) w = 0.0;
+  if (w > 1.0) w = 1.0;
+  return (real)w;
+}
+
+void orc_habc_weights2d(int nx, int nz, int pad, double off, real *w) {
+  for (int j = 1; j <= nz; ++j)
+    for (int i = 1; i <= nx; ++i) {
+      int d = i - 1;
+      if (nx - i < d) d = nx - i;
+      if (j - 1 < d) d = j - 1;
+      if (nz - j < d) d = nz - j;
+      w[(size_t)(i - 1) + (size_t)(j - 1) * nx] = habc_weight(d, off, pad);
+    }
+}
+
+/* kernels_fused.jl:28-52 -- frame map and size */
+long orc_frame_total(int nx, int nz, int nbc) {
+  long Wd = nx - 2, Hc = nz - 2L * nbc - 2;
+  return 2 * Wd * nbc + 2L * nbc * Hc;
+}
+
+void orc_frame_ij(long idx, int nx, int nz, int nbc, int *pi, int *pj) {
+  long Wd = nx - 2, Nh = Wd * nbc;
+  if (idx <= Nh) {
+    long q = (idx - 1) / Wd, r = (idx - 1) % Wd;
+    *pi = (int)(2 + r);
+    *pj = (int)(2 + q);
+  } else if (idx <= 2 * Nh) {
+    long t = idx - Nh - 1;
+    long q = t / Wd, r = t % Wd;
+    *pi = (int)(2 + r);
+    *pj = (int)(nz - nbc + q);
+  } else {
+    long t = idx - 2 * Nh - 1;
+    long q = t / (2L * nbc), r = t % (2L * nbc);
+    *pj = (int)(nbc + 2 + q);
+    *pi = (int)(r < nbc ? 2 + r : nx - nbc + (r - nbc));
+  }
+}
+
+/* kernels_det.jl:22-57 -- Higdon value from pre-correction f and f_old */
+static inline real habc_value(const real *f, const real *fo, real wt, int i, int j, real q, real qt, real qxt,
+                              int nx, int nz, int nbc) {
+#define F(a, b) f[(size_t)((a)-1) + (size_t)((b)-1) * nx]
+#define FO(a, b) fo[(size_t)((a)-1) + (size_t)((b)-1) * nx]
+  int is_left = i <= nbc + 1, is_right = i >= nx - nbc;
+  int is_top = j <= nbc + 1, is_bottom = j >= nz - nbc;
+  int in_x = is_left | is_right, in_z = is_top | is_bottom;
+  real fc = F(i, j);
+  real sx = 0, sz = 0;
+  if (in_x)
+    sx = is_left ? ((-q * F(i + 1, j) - qt * FO(i, j)) - qxt * FO(i + 1, j))
+                 : ((-q * F(i - 1, j) - qt * FO(i, j)) - qxt * FO(i - 1, j));
+  if (in_z)
+    sz = is_top ? ((-q * F(i, j + 1) - qt * FO(i, j)) - qxt * FO(i, j + 1))
+                : ((-q * F(i, j - 1) - qt * FO(i, j)) - qxt * FO(i, j - 1));
+  if (in_x && in_z) return wt * fc + (((real)1 - wt) * (real)0.5f) * (sx + sz);
+  if (in_x) return wt * fc + ((real)1 - wt) * sx;
+  return wt * fc + ((real)1 - wt) * sz;
+#undef F
+#undef FO
+}
+
+/* kernels_det.jl:60-132: pass 1 into scratch, pass 2 back into f.  nf fields share the frame. */
+static void habc_det(int nf, real **f, real **fo, const real **w, real *scr, real q, real qt, real qxt, int nx,
+                     int nz, int nbc) {
+  long total = orc_frame_total(nx, nz, nbc);
+  for (int m = 0; m < nf; ++m) {
+#pragma omp parallel for schedule(static)
+    for (long idx = 1; idx <= total; ++idx) {
+      int i, j;
+      orc_frame_ij(idx, nx, nz, nbc, &i, &j);
+      real wt = w[m][(size_t)(i - 1) + (size_t)(j - 1) * nx];
+      scr[(size_t)m * total + idx - 1] = habc_value(f[m], fo[m], wt, i, j, q, qt, qxt, nx, nz, nbc);
+    }
+  }
+  for (int m = 0; m < nf; ++m) {
+#pragma omp parallel for schedule(static)
+    for (long idx = 1; idx <= total; ++idx) {
+      int i, j;
+      orc_frame_ij(idx, nx, nz, nbc, &i, &j);
+      f[m][(size_t)(i - 1) + (size_t)(j - 1) * nx] = scr[(size_t)m * total + idx - 1];
+    }
+  }
+}
+
+/* fused_kernels.jl:39-45: strip backup condition (nbc here is the HABC width pad-1) */
+static inline int in_strip(int i, int j, int nx, int nz, int nbc) {
+  return (i <= nbc + 2) | (i >= nx - nbc - 1) | (j <= nbc + 2) | (j >= nz - nbc - 1);
+}
+
+static void strip_backup(int nf, real **f, real **fo, int nx, int nz, int nbc) {
+#pragma omp parallel for schedule(static)
+  for (int j = 1; j <= nz; ++j)
+    for (int i = 1; i <= nx; ++i)
+      if (in_strip(i, j, nx, nz, nbc)) {
+        size_t c = (size_t)(i - 1) + (size_t)(j - 1) * nx;
+        for (int m = 0; m < nf; ++m) fo[m][c] = f[m][c];
+      }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Acoustic sweeps, fused_kernels.jl:27-63 and :86-114 (update region M < i <= nx-M, same in j) */
+static void acoustic_velocity(real *vx, real *vz, const real *p, const real *bx, const real *bz, const real *a, int N,
+                              real dtx, real dtz, int nx, int nz) {
+  int M = N;
+#pragma omp parallel for schedule(static)
+  for (int j = M + 1; j <= nz - M; ++j)
+    for (int i = M + 1; i <= nx - M; ++i) {
+      size_t c = (size_t)(i - 1) + (size_t)(j - 1) * nx;
+      real dpdx = 0, dpdz = 0;
+      for (int l = 1; l <= N; ++l) {
+        real cl = a[l - 1];
+        dpdx += cl * (p[c + (l - 1)] - p[c - l]);
+        dpdz += cl * (p[c + (size_t)l * nx] - p[c - (size_t)(l - 1) * nx]);
+      }
+      vx[c] += (bx[c] * dtx) * dpdx;
+      vz[c] += (bz[c] * dtz) * dpdz;
+    }
+}
+
+static void acoustic_pressure(real *p, const real *vx, const real *vz, const real *kappa, const real *a, int N,
+                              real dtx, real dtz, int nx, int nz) {
+  int M = N;
+#pragma omp parallel for schedule(static)
+  for (int j = M + 1; j <= nz - M; ++j)
+    for (int i = M + 1; i <= nx - M; ++i) {
+      size_t c = (size_t)(i - 1) + (size_t)(j - 1) * nx;
+      real dvxdx = 0, dvzdz = 0;
+      for (int l = 1; l <= N; ++l) {
+        real cl = a[l - 1];
+        dvxdx += cl * (vx[c + l] - vx[c - (l - 1)]);
+        dvzdz += cl * (vz[c + (size_t)(l - 1) * nx] - vz[c - (size_t)l * nx]);
+      }
+      p[c] += kappa[c] * (dvxdx * dtx + dvzdz * dtz);
+    }
+}
+
+/* Elastic sweeps, elastic2d/fused_kernels.jl:21-59 and :82-124 */
+static void elastic_velocity(real *vx, real *vz, const real *txx, const real *tzz, const real *txz, const real *bx,
+                             const real *bz, const real *a, int N, real dtx, real dtz, int nx, int nz) {
+  int M = N;
+#pragma omp parallel for schedule(static)
+  for (int j = M + 1; j <= nz - M; ++j)
+    for (int i = M + 1; i <= nx - M; ++i) {
+      size_t c = (size_t)(i - 1) + (size_t)(j - 1) * nx;
+      real dtxxdx = 0, dtxzdz = 0, dtxzdx = 0, dtzzdz = 0;
+      for (int l = 1; l <= N; ++l) {
+        real cl = a[l - 1];
+        dtxxdx += cl * (txx[c + (l - 1)] - txx[c - l]);
+        dtxzdz += cl * (txz[c + (size_t)(l - 1) * nx] - txz[c - (size_t)l * nx]);
+        dtxzdx += cl * (txz[c + l] - txz[c - (l - 1)]);
+        dtzzdz += cl * (tzz[c + (size_t)l * nx] - tzz[c - (size_t)(l - 1) * nx]);
+      }
+      vx[c] += bx[c] * (dtx * dtxxdx + dtz * dtxzdz);
+      vz[c] += bz[c] * (dtx * dtxzdx + dtz * dtzzdz);
+    }
+}
+
+static void elastic_stress(real *txx, real *tzz, real *txz, const real *vx, const real *vz, const real *lam,
+                           const real *lam2mu, const real *muxz, const real *a, int N, real dtx, real dtz, int nx,
+                           int nz) {
+  int M = N;
+#pragma omp parallel for schedule(static)
+  for (int j = M + 1; j <= nz - M; ++j)
+    for (int i = M + 1; i <= nx - M; ++i) {
+      size_t c = (size_t)(i - 1) + (size_t)(j - 1) * nx;
+      real dvxdx = 0, dvzdz = 0, dvxdz = 0, dvzdx = 0;
+      for (int l = 1; l <= N; ++l) {
+        real cl = a[l - 1];
+        dvxdx += cl * (vx[c + l] - vx[c - (l - 1)]);
+        dvzdz += cl * (vz[c + (size_t)(l - 1) * nx] - vz[c - (size_t)l * nx]);
+        dvxdz += cl * (vx[c + (size_t)l * nx] - vx[c - (size_t)(l - 1) * nx]);
+        dvzdx += cl * (vz[c + (l - 1)] - vz[c - l]);
+      }
+      real lv = lam[c], l2m = lam2mu[c];
+      txx[c] += (l2m * dvxdx) * dtx + (lv * dvzdz) * dtz;
+      tzz[c] += (lv * dvxdx) * dtx + (l2m * dvzdz) * dtz;
+      txz[c] += muxz[c] * (dvxdz * dtz + dvzdx * dtx);
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+static void *xcalloc(size_t n, size_t sz) {
+  void *p = calloc(n, sz);
+  return p;
+}
+
+static void set_threads(int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+}
+
+int orc_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* Shared geometry for 2D runs. */
+typedef struct {
+  int nxi, nzi, M, pad, nx, nz, H, N;
+  real a[5];
+  real dtx;
+  real q, qt, qxt;
+} geom2d;
+
+static int geom2d_init(geom2d *g, int nxi, int nzi, int nbc, int fd_order, float dt, float dh, float v_ref,
+                       int habc_width) {
+  g->N = load_coeffs(fd_order, g->a);
+  if (g->N < 0) return -1;
+  g->M = g->N;
+  g->nxi = nxi;
+  g->nzi = nzi;
+  g->pad = nbc + g->M;
+  g->nx = nxi + 2 * g->pad;
+  g->nz = nzi + 2 * g->pad;
+  /* HABC frame width: pad-1 (acoustic2d.jl:127-131, elastic path); the reference's earlier
+   * acoustic path used nbc, which is what example/benchmark/output/report.txt was produced with. */
+  g->H = habc_width > 0 ? habc_width : g->pad - 1;
+  g->dtx = (real)dt / (real)dh;
+  real c[3];
+  orc_habc_coeffs(v_ref, dt, dh, c);
+  g->q = c[0];
+  g->qt = c[1];
+  g->qxt = c[2];
+  return 0;
+}
+
+/*
+ * acoustic2d, boundary = :habc, det two-pass path (loop_graph.jl:28-39 step order).
+ *   inputs  vp, rho: nxi*nzi (x fastest); sx,sz,rx,rz: 1-based interior indices;
+ *           wavelet: n_src x nt column-major (already repeated / dt-scaled by the caller).
+ *   outputs seis_p, seis_vx, seis_vz: n_rec x nt column-major; snaps: n_snap x (nxi*nzi) of p.
+ *   final_state (optional, 3*nx*nz): p, vx, vz after the last step.
+ *   habc_width <= 0 selects the reference's current frame width pad-1.
+ */
+int orc_acoustic2d(int nxi, int nzi, const float *vp, const float *rho, float dh, float dt, int nt, int nbc,
+                   int fd_order, float v_ref, int n_src, const int *sx, const int *sz, const float *wavelet, int n_rec,
+                   const int *rx, const int *rz, real *seis_p, real *seis_vx, real *seis_vz, int snap_interval,
+                   real *snaps, real *final_state, int habc_width, int nthreads) {
+  set_threads(nthreads);
+  geom2d g;
+  if (geom2d_init(&g, nxi, nzi, nbc, fd_order, dt, dh, v_ref, habc_width)) return -1;
+  int nx = g.nx, nz = g.nz;
+  size_t n = (size_t)nx * nz;
+  real *vpp = xcalloc(n, sizeof(real)), *rhop = xcalloc(n, sizeof(real));
+  real *kappa = xcalloc(n, sizeof(real)), *bx = xcalloc(n, sizeof(real)), *bz = xcalloc(n, sizeof(real));
+  real *wvx = xcalloc(n, sizeof(real)), *wvz = xcalloc(n, sizeof(real)), *wtau = xcalloc(n, sizeof(real));
+  real *vx = xcalloc(n, sizeof(real)), *vz = xcalloc(n, sizeof(real)), *p = xcalloc(n, sizeof(real));
+  real *vxo = xcalloc(n, sizeof(real)), *vzo = xcalloc(n, sizeof(real)), *po = xcalloc(n, sizeof(real));
+  long total = orc_frame_total(nx, nz, g.H);
+  real *scr = xcalloc((size_t)(2 * total + 1), sizeof(real));
+  orc_pad2d(vp, nxi, nzi, g.pad, vpp);
+  orc_pad2d(rho, nxi, nzi, g.pad, rhop);
+  for (size_t c = 0; c < n; ++c) kappa[c] = rhop[c] * (vpp[c] * vpp[c]);
+  orc_buoyancy2d(rhop, nx, nz, bx, bz);
+  orc_habc_weights2d(nx, nz, g.pad, 0.0, wvx);
+  orc_habc_weights2d(nx, nz, g.pad, 0.5, wvz);
+  orc_habc_weights2d(nx, nz, g.pad, 0.75, wtau);
+  int snap_idx = 0;
+  for (int it = 1; it <= nt; ++it) {
+    /* A. velocity phase: backup + update, det HABC on (vx, w_vx), (vz, w_vz) */
+    real *fv[2] = {vx, vz}, *fvo[2] = {vxo, vzo};
+    const real *wv[2] = {wvx, wvz};
+    strip_backup(2, fv, fvo, nx, nz, g.H);
+    acoustic_velocity(vx, vz, p, bx, bz, g.a, g.N, g.dtx, g.dtx, nx, nz);
+    habc_det(2, fv, fvo, wv, scr, g.q, g.qt, g.qxt, nx, nz, g.H);
+    /* B. pressure phase */
+    real *fp[1] = {p}, *fpo[1] = {po};
+    const real *wp[1] = {wtau};
+    strip_backup(1, fp, fpo, nx, nz, g.H);
+    acoustic_pressure(p, vx, vz, kappa, g.a, g.N, g.dtx, g.dtx, nx, nz);
+    habc_det(1, fp, fpo, wp, scr, g.q, g.qt, g.qxt, nx, nz, g.H);
+    /* C. raw-add injection after HABC */
+    for (int s = 0; s < n_src; ++s) {
+      size_t c = (size_t)(sx[s] + g.pad - 1) + (size_t)(sz[s] + g.pad - 1) * nx;
+      p[c] += (real)wavelet[(size_t)s + (size_t)n_src * (it - 1)];
+    }
+    /* D. gather */
+    for (int r = 0; r < n_rec; ++r) {
+      size_t c = (size_t)(rx[r] + g.pad - 1) + (size_t)(rz[r] + g.pad - 1) * nx;
+      size_t o = (size_t)r + (size_t)n_rec * (it - 1);
+      if (seis_p) seis_p[o] = p[c];
+      if (seis_vx) seis_vx[o] = vx[c];
+      if (seis_vz) seis_vz[o] = vz[c];
+    }
+    /* E. snapshot of the interior of p */
+    if (snap_interval > 0 && it % snap_interval == 0 && snaps) {
+      real *dst = snaps + (size_t)snap_idx * nxi * nzi;
+      for (int j = 1; j <= nzi; ++j)
+        for (int i = 1; i <= nxi; ++i)
+          dst[(size_t)(i - 1) + (size_t)(j - 1) * nxi] = p[(size_t)(i + g.pad - 1) + (size_t)(j + g.pad - 1) * nx];
+      ++snap_idx;
+    }
+  }
+  if (final_state) {
+    memcpy(final_state, p, n * sizeof(real));
+    memcpy(final_state + n, vx, n * sizeof(real));
+    memcpy(final_state + 2 * n, vz, n * sizeof(real));
+  }
+  free(vpp); free(rhop); free(kappa); free(bx); free(bz); free(wvx); free(wvz); free(wtau);
+  free(vx); free(vz); free(p); free(vxo); free(vzo); free(po); free(scr);
+  return 0;
+}
+
+/*
+ * elastic2d, boundary = :habc, det two-pass path (elastic2d/loop_graph.jl:30-41 step order).
+ *   source_type 0: explosive -- wavelet added to txx and tzz (elastic2d/fused_kernels.jl:147-159);
+ *   source_type 1: vertical force -- wavelet added to vz at the same point of the step (an
+ *                  extension the reference does not have).
+ *   outputs seis_vx, seis_vz (n_rec x nt); snaps of vz; final_state 5*nx*nz (vx,vz,txx,tzz,txz).
+ */
+int orc_elastic2d(int nxi, int nzi, const float *vp, const float *vs, const float *rho, float dh, float dt, int nt,
+                  int nbc, int fd_order, float v_ref, int source_type, int n_src, const int *sx, const int *sz,
+                  const float *wavelet, int n_rec, const int *rx, const int *rz, real *seis_vx, real *seis_vz,
+                  int snap_interval, real *snaps, real *final_state, int habc_width, int nthreads) {
+  set_threads(nthreads);
+  geom2d g;
+  if (geom2d_init(&g, nxi, nzi, nbc, fd_order, dt, dh, v_ref, habc_width)) return -1;
+  int nx = g.nx, nz = g.nz;
+  size_t n = (size_t)nx * nz;
+  real *vpp = xcalloc(n, sizeof(real)), *vsp = xcalloc(n, sizeof(real)), *rhop = xcalloc(n, sizeof(real));
+  real *lam = xcalloc(n, sizeof(real)), *mu = xcalloc(n, sizeof(real)), *muxz = xcalloc(n, sizeof(real));
+  real *l2m = xcalloc(n, sizeof(real)), *bx = xcalloc(n, sizeof(real)), *bz = xcalloc(n, sizeof(real));
+  real *wvx = xcalloc(n, sizeof(real)), *wvz = xcalloc(n, sizeof(real)), *wtau = xcalloc(n, sizeof(real));
+  real *vx = xcalloc(n, sizeof(real)), *vz = xcalloc(n, sizeof(real));
+  real *txx = xcalloc(n, sizeof(real)), *tzz = xcalloc(n, sizeof(real)), *txz = xcalloc(n, sizeof(real));
+  real *vxo = xcalloc(n, sizeof(real)), *vzo = xcalloc(n, sizeof(real));
+  real *txxo = xcalloc(n, sizeof(real)), *tzzo = xcalloc(n, sizeof(real)), *txzo = xcalloc(n, sizeof(real));
+  long total = orc_frame_total(nx, nz, g.H);
+  real *scr = xcalloc((size_t)(3 * total + 1), sizeof(real));
+  orc_pad2d(vp, nxi, nzi, g.pad, vpp);
+  orc_pad2d(vs, nxi, nzi, g.pad, vsp);
+  orc_pad2d(rho, nxi, nzi, g.pad, rhop);
+  orc_buoyancy2d(rhop, nx, nz, bx, bz);
+  orc_elastic_params2d(vpp, vsp, rhop, nx, nz, lam, mu, muxz, l2m);
+  orc_habc_weights2d(nx, nz, g.pad, 0.0, wvx);
+  orc_habc_weights2d(nx, nz, g.pad, 0.5, wvz);
+  orc_habc_weights2d(nx, nz, g.pad, 0.75, wtau);
+  int snap_idx = 0;
+  for (int it = 1; it <= nt; ++it) {
+    real *fv[2] = {vx, vz}, *fvo[2] = {vxo, vzo};
+    const real *wv[2] = {wvx, wvz};
+    strip_backup(2, fv, fvo, nx, nz, g.H);
+    elastic_velocity(vx, vz, txx, tzz, txz, bx, bz, g.a, g.N, g.dtx, g.dtx, nx, nz);
+    habc_det(2, fv, fvo, wv, scr, g.q, g.qt, g.qxt, nx, nz, g.H);
+    real *fs[3] = {txx, tzz, txz}, *fso[3] = {txxo, tzzo, txzo};
+    const real *ws[3] = {wtau, wtau, wtau};
+    strip_backup(3, fs, fso, nx, nz, g.H);
+    elastic_stress(txx, tzz, txz, vx, vz, lam, l2m, muxz, g.a, g.N, g.dtx, g.dtx, nx, nz);
+    habc_det(3, fs, fso, ws, scr, g.q, g.qt, g.qxt, nx, nz, g.H);
+    for (int s = 0; s < n_src; ++s) {
+      size_t c = (size_t)(sx[s] + g.pad - 1) + (size_t)(sz[s] + g.pad - 1) * nx;
+      real wav = (real)wavelet[(size_t)s + (size_t)n_src * (it - 1)];
+      if (source_type == 1) {
+        vz[c] += wav;
+      } else {
+        txx[c] += wav;
+        tzz[c] += wav;
+      }
+    }
+    for (int r = 0; r < n_rec; ++r) {
+      size_t c = (size_t)(rx[r] + g.pad - 1) + (size_t)(rz[r] + g.pad - 1) * nx;
+      size_t o = (size_t)r + (size_t)n_rec * (it - 1);
+      if (seis_vx) seis_vx[o] = vx[c];
+      if (seis_vz) seis_vz[o] = vz[c];
+    }
+    if (snap_interval > 0 && it % snap_interval == 0 && snaps) {
+      real *dst = snaps + (size_t)snap_idx * nxi * nzi;
+      for (int j = 1; j <= nzi; ++j)
+        for (int i = 1; i <= nxi; ++i)
+          dst[(size_t)(i - 1) + (size_t)(j - 1) * nxi] = vz[(size_t)(i + g.pad - 1) + (size_t)(j + g.pad - 1) * nx];
+      ++snap_idx;
+    }
+  }
+  if (final_state) {
+    memcpy(final_state, vx, n * sizeof(real));
+    memcpy(final_state + n, vz, n * sizeof(real));
+    memcpy(final_state + 2 * n, txx, n * sizeof(real));
+    memcpy(final_state + 3 * n, tzz, n * sizeof(real));
+    memcpy(final_state + 4 * n, txz, n * sizeof(real));
+  }
+  free(vpp); free(vsp); free(rhop); free(lam); free(mu); free(muxz); free(l2m); free(bx); free(bz);
+  free(wvx); free(wvz); free(wtau); free(vx); free(vz); free(txx); free(tzz); free(txz);
+  free(vxo); free(vzo); free(txxo); free(tzzo); free(txzo); free(scr);
+  return 0;
+}
+
+/* Single-sweep probes used by the known-answer tests: one velocity update of the acoustic
+ * equations on a caller-given padded state (no HABC), and one HABC two-pass correction. */
+void orc_probe_acoustic_velocity(int nx, int nz, int fd_order, real dtx, real *vx, real *vz, const real *p,
+                                 const real *bx, const real *bz) {
+  real a[5];
+  int N = load_coeffs(fd_order, a);
+  acoustic_velocity(vx, vz, p, bx, bz, a, N, dtx, dtx, nx, nz);
+}
+
+void orc_probe_acoustic_pressure(int nx, int nz, int fd_order, real dtx, real *p, const real *vx, const real *vz,
+                                 const real *kappa) {
+  real a[5];
+  int N = load_coeffs(fd_order, a);
+  acoustic_pressure(p, vx, vz, kappa, a, N, dtx, dtx, nx, nz);
+}
+
+void orc_probe_habc(int nx, int nz, int nbc, real q, real qt, real qxt, real *f, real *fo, const real *w) {
+  long total = orc_frame_total(nx, nz, nbc);
+  real *scr = xcalloc((size_t)total + 1, sizeof(real));
+  real *ff[1] = {f}, *ffo[1] = {fo};
+  const real *ww[1] = {w};
+  habc_det(1, ff, ffo, ww, scr, q, qt, qxt, nx, nz, nbc);
+  free(scr);
+}

@@ -1,0 +1,227 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU oracle.
+
+The FP32 oracle restates the reference's Float32 updates in the reference's evaluation order
+and the engine is compiled with -ffp-contract=off, so the bar here is bit-identity for
+seismograms and final wavefields (stronger than the per-trace relative L2 <= 1e-5 the target
+asks for, which is also asserted).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TRACE_TOL = 1e-5  # per-trace relative L2, FP32
+
+
+def trace_rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.linalg.norm(b, axis=1)
+    num = np.linalg.norm(a - b, axis=1)
+    return np.where(den > 0, num / np.where(den > 0, den, 1), num)
+
+
+def assert_parity(got, want):
+    assert got.shape == want.shape
+    assert np.all(np.isfinite(got))
+    assert trace_rel_l2(got, want).max() <= TRACE_TOL
+    assert np.array_equal(got, want), f"max |diff| = {np.abs(got.astype(np.float64) - want).max()}"
+
+
+def small_model(nx, nz, seed=3, vacuum_top=True):
+    from fomo.models import layered_2d
+    vp, vs, rho = layered_2d(nx, nz, seed=seed, n_interfaces=4, jitter=3.0, vacuum_top=vacuum_top)
+    return vp, vs, rho
+
+
+def geometry(nx, nz):
+    sx = [nx // 2, nx // 3]
+    sz = [5, 9]
+    rx = np.concatenate([np.arange(1, nx + 1, 3), [1, nx, nx // 2]])
+    rz = np.concatenate([np.full(len(np.arange(1, nx + 1, 3)), 4), [nz, 1, nz // 2]])
+    return sx, sz, rx, rz
+
+
+@pytest.mark.parametrize("fd_order", [2, 4, 6, 8, 10])
+def test_acoustic2d_bitwise_vs_oracle(gpu, orc, fd_order):
+    import fomo
+    nx, nz, nt, nbc = 97, 71, 260, 12
+    vp, _, rho = small_model(nx, nz)
+    sx, sz, rx, rz = geometry(nx, nz)
+    dt, dh, f0 = 1e-3, 10.0, 15.0
+    w = fomo.ricker_wavelet(f0, dt, nt)
+    res = fomo.acoustic2d(vp, rho, dh, dt, nt, f0, sx=sx, sz=sz, rx=rx, rz=rz, nbc=nbc, fd_order=fd_order)
+    ref = orc.acoustic2d(vp, rho, dh, dt, nt, sx=sx, sz=sz, rx=rx, rz=rz, wavelet=np.stack([w, w]), nbc=nbc,
+                         fd_order=fd_order)
+    for k in ("seis_p", "seis_vx", "seis_vz"):
+        assert_parity(getattr(res, k), ref[k])
+    assert res.stats.kernel_time_s > 0
+
+
+@pytest.mark.parametrize("fd_order,source_type", [(2, "explosive"), (4, "fz"), (8, "explosive"), (8, "fz"),
+                                                  (10, "explosive")])
+def test_elastic2d_bitwise_vs_oracle(gpu, orc, fd_order, source_type):
+    import fomo
+    nx, nz, nt, nbc = 89, 77, 240, 10
+    vp, vs, rho = small_model(nx, nz, seed=5)
+    sx, sz, rx, rz = geometry(nx, nz)
+    dt, dh, f0 = 5e-4, 5.0, 20.0
+    w = fomo.ricker_wavelet(f0, dt, nt)
+    res = fomo.elastic2d(vp, vs, rho, dh, dt, nt, f0, sx=sx, sz=sz, rx=rx, rz=rz, nbc=nbc, fd_order=fd_order,
+                         source_type=source_type)
+    ref = orc.elastic2d(vp, vs, rho, dh, dt, nt, sx=sx, sz=sz, rx=rx, rz=rz, wavelet=np.stack([w, w]), nbc=nbc,
+                        fd_order=fd_order, source_type=1 if source_type == "fz" else 0)
+    for k in ("seis_vx", "seis_vz"):
+        assert_parity(getattr(res, k), ref[k])
+
+
+def _plan_states(eq, vp, vs, rho, nt, kw, use_graph):
+    from fomo.api import Plan
+    import fomo._lib as L
+    p = Plan(eq, vp, rho, 10.0, 1e-3, nt, vs=vs, use_graph=use_graph, **kw)
+    p.advance(nt)
+    p.sync()
+    nf = 3 if eq == L.ACOUSTIC2D else 5
+    fields = [p.field(k) for k in range(nf)]
+    seis = p.seis(3 if eq == L.ACOUSTIC2D else 2)
+    p.close()
+    return fields, seis
+
+
+@pytest.mark.parametrize("eq", [1, 2])
+def test_final_wavefields_bitwise_and_graph_equals_stream(gpu, orc, eq):
+    """Whole padded state after nt steps equals the oracle bitwise; the hipGraph replay equals
+    direct launches bitwise (verify_det_graph.jl:261-296); reruns are bitwise reproducible
+    (verify_det_graph.jl:152-185).  nt is odd so the tail step outside the graph is covered."""
+    import fomo
+    nx, nz, nt = 70, 66, 151
+    vp, vs, rho = small_model(nx, nz, seed=11)
+    sx, sz, rx, rz = geometry(nx, nz)
+    w = fomo.ricker_wavelet(15.0, 1e-3, nt)
+    kw = dict(sx=sx, sz=sz, rx=rx, rz=rz, wavelet=np.stack([w, w]), nbc=8, fd_order=8)
+    g1, s1 = _plan_states(eq, vp, vs if eq == 2 else None, rho, nt, kw, True)
+    g2, s2 = _plan_states(eq, vp, vs if eq == 2 else None, rho, nt, kw, False)
+    g3, s3 = _plan_states(eq, vp, vs if eq == 2 else None, rho, nt, kw, True)
+    for a, b, c in zip(g1 + s1, g2 + s2, g3 + s3):
+        assert np.array_equal(a, b) and np.array_equal(a, c)
+    if eq == 1:
+        ref = orc.acoustic2d(vp, rho, 10.0, 1e-3, nt, final_state=True, **kw)
+        names = ("p", "vx", "vz")
+    else:
+        ref = orc.elastic2d(vp, vs, rho, 10.0, 1e-3, nt, final_state=True, **kw)
+        names = ("vx", "vz", "txx", "tzz", "txz")
+    for k, name in enumerate(names):
+        assert np.array_equal(g1[k], ref["state"][name]), name
+
+
+def test_snapshots_match_oracle(gpu, orc):
+    import fomo
+    nx, nz, nt = 80, 60, 120
+    vp, vs, rho = small_model(nx, nz, seed=2)
+    sx, sz, rx, rz = geometry(nx, nz)
+    w = fomo.ricker_wavelet(15.0, 1e-3, nt)
+    kw = dict(sx=sx, sz=sz, rx=rx, rz=rz, nbc=10, fd_order=8)
+    res = fomo.acoustic2d(vp, rho, 10.0, 1e-3, nt, 15.0, snap_interval=25, **kw)
+    ref = orc.acoustic2d(vp, rho, 10.0, 1e-3, nt, snap_interval=25, wavelet=np.stack([w, w]), **kw)
+    assert len(res.snaps) == nt // 25 == len(ref["snaps"])
+    for a, b in zip(res.snaps, ref["snaps"]):
+        assert a.shape == (nx, nz)
+        assert np.array_equal(a, b)
+    e = fomo.elastic2d(vp, vs, rho, 10.0, 1e-3, nt, 15.0, snap_interval=40, **kw)
+    er = orc.elastic2d(vp, vs, rho, 10.0, 1e-3, nt, snap_interval=40, wavelet=np.stack([w, w]), **kw)
+    for a, b in zip(e.snaps, er["snaps"]):
+        assert np.array_equal(a, b)
+
+
+def test_edge_cases(gpu, orc):
+    """Thin boundary (nbc=0), sources and receivers on the interior's extreme cells, no receivers."""
+    import fomo
+    nx, nz, nt = 65, 40, 90
+    vp, vs, rho = small_model(nx, nz, seed=9, vacuum_top=False)
+    w = fomo.ricker_wavelet(15.0, 1e-3, nt)
+    kw = dict(sx=[1, nx], sz=[nz, 1], rx=[1, nx, 1, nx], rz=[1, 1, nz, nz], nbc=0, fd_order=4)
+    res = fomo.acoustic2d(vp, rho, 10.0, 1e-3, nt, 15.0, **kw)
+    ref = orc.acoustic2d(vp, rho, 10.0, 1e-3, nt, wavelet=np.stack([w, w]), **kw)
+    for k in ("seis_p", "seis_vx", "seis_vz"):
+        assert np.array_equal(getattr(res, k), ref[k])
+    r0 = fomo.acoustic2d(vp, rho, 10.0, 1e-3, nt, 15.0, sx=[30], sz=[20], rx=[], rz=[], nbc=5)
+    assert r0.seis_p.shape == (0, nt)
+    # duplicate source cells add in order
+    kw2 = dict(sx=[30, 30], sz=[20, 20], rx=[31], rz=[20], nbc=5, fd_order=8)
+    res = fomo.acoustic2d(vp, rho, 10.0, 1e-3, nt, 15.0, **kw2)
+    ref = orc.acoustic2d(vp, rho, 10.0, 1e-3, nt, wavelet=np.stack([w, w]), **kw2)
+    assert np.array_equal(res.seis_p, ref["seis_p"])
+
+
+def test_custom_wavelet_equals_default_and_linearity(gpu):
+    """verify_patch.jl:254-274 on the GPU path."""
+    import fomo
+    nx, nz, nt = 301, 121, 500
+    vp, vs, rho = small_model(nx, nz, seed=4, vacuum_top=False)
+    kw = dict(rx=np.arange(20, 280), rz=np.full(260, 41), nbc=30, fd_order=4)
+    w = fomo.ricker_wavelet(10.0, 1e-3, nt)
+    r0 = fomo.acoustic2d(vp, rho, 10.0, 1e-3, nt, 10.0, sx=[100], sz=[41], **kw)
+    r1 = fomo.acoustic2d(vp, rho, 10.0, 1e-3, nt, 10.0, sx=[100], sz=[41], wavelet=w, **kw)
+    assert np.array_equal(r0.seis_p, r1.seis_p)
+    b = fomo.acoustic2d(vp, rho, 10.0, 1e-3, nt, 10.0, sx=[200], sz=[41], **kw)
+    ab = fomo.acoustic2d(vp, rho, 10.0, 1e-3, nt, 10.0, sx=[100, 200], sz=[41, 41], **kw)
+    r = np.linalg.norm(ab.seis_p - (r0.seis_p + b.seis_p)) / np.linalg.norm(ab.seis_p)
+    assert r < 1e-5
+
+
+def test_mirror_symmetry_gpu(gpu):
+    """verify_patch.jl:198-229 at the reference's size on the GPU path (< 1e-5)."""
+    import fomo
+    from test_oracle_pins import make_model, rel_asym_vx, rel_sym
+    vp, vs, rho = make_model(601, 201)
+    rx = np.arange(150, 453)
+    kw = dict(sx=[301], sz=[41], rx=rx, rz=np.full(len(rx), 41), nbc=30, fd_order=4)
+    r = fomo.acoustic2d(vp, rho, 10.0, 1e-3, 1200, 10.0, **kw)
+    assert rel_sym(r.seis_p) < 1e-5 and rel_sym(r.seis_vz) < 1e-5 and rel_asym_vx(r.seis_vx) < 1e-5
+    e = fomo.elastic2d(vp, vs, rho, 10.0, 1e-3, 1200, 10.0, **kw)
+    assert rel_sym(e.seis_vz) < 1e-5 and rel_asym_vx(e.seis_vx) < 1e-5
+
+
+def test_cfg1_acoustic_full_bitwise(gpu, orc):
+    """Plain acoustic2d 400x300, dh 10, dt 1 ms, nt 2000, fd8, nbc 100, vp 2500 / rho 2000, vacuum
+    top row, Ricker 15 Hz at (200,10), receivers x = 1:2:400 at z = 10 (p, vx, vz)."""
+    import fomo
+    nx, nz, nt = 400, 300, 2000
+    vp = np.full((nx, nz), 2500.0, np.float32)
+    rho = np.full((nx, nz), 2000.0, np.float32)
+    vp[:, 0] = 0.0
+    rho[:, 0] = 0.0
+    rx = np.arange(1, nx + 1, 2)
+    rz = np.full(len(rx), 10)
+    res = fomo.acoustic2d(vp, rho, 10.0, 1e-3, nt, 15.0, sx=[200], sz=[10], rx=rx, rz=rz, nbc=100, fd_order=8,
+                          v_ref=2500.0)
+    w = fomo.ricker_wavelet(15.0, 1e-3, nt)
+    ref = orc.acoustic2d(vp, rho, 10.0, 1e-3, nt, sx=[200], sz=[10], rx=rx, rz=rz, wavelet=w[None], nbc=100,
+                         fd_order=8, v_ref=2500.0)
+    for k in ("seis_p", "seis_vx", "seis_vz"):
+        assert_parity(getattr(res, k), ref[k])
+
+
+def test_cfg2_elastic_full_grid_prefix_bitwise(gpu, orc):
+    """Plain elastic2d at the full 2000x800 grid (dh 5, dt 0.4 ms, fd8, nbc 50, seeded layered
+    model, vacuum top, vertical-force Ricker 20 Hz at (1000,5), receivers at every cell of z=5):
+    the first 300 steps bitwise against the oracle; the full 4000 steps are checked for finite,
+    bounded output."""
+    import fomo
+    from fomo.models import layered_2d
+    nx, nz = 2000, 800
+    vp, vs, rho = layered_2d(nx, nz, seed=20260101)
+    rx = np.arange(1, nx + 1)
+    rz = np.full(nx, 5)
+    kw = dict(sx=[1000], sz=[5], rx=rx, rz=rz, nbc=50, fd_order=8)
+    nt = 300
+    res = fomo.elastic2d(vp, vs, rho, 5.0, 4e-4, nt, 20.0, source_type="fz", **kw)
+    w = fomo.ricker_wavelet(20.0, 4e-4, nt)
+    ref = orc.elastic2d(vp, vs, rho, 5.0, 4e-4, nt, wavelet=w[None], source_type=1, **kw)
+    for k in ("seis_vx", "seis_vz"):
+        assert_parity(getattr(res, k), ref[k])
+    full = fomo.elastic2d(vp, vs, rho, 5.0, 4e-4, 4000, 20.0, source_type="fz", **kw)
+    assert np.all(np.isfinite(full.seis_vz)) and np.all(np.isfinite(full.seis_vx))
+    assert np.array_equal(full.seis_vz[:, :nt], res.seis_vz)
+    peak = np.abs(full.seis_vz).max()
+    assert np.abs(full.seis_vz[:, -200:]).max() < peak

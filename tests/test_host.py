@@ -1,0 +1,89 @@
+"""CPU-side checks of the product library: it loads, exports every declared symbol, and its
+host-side setup helpers agree with the oracle.  No GPU compute is called here."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "fomo_hip.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(fomo_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    import fomo._lib as L
+    lib = L.lib()
+    names = declared_symbols()
+    assert len(names) >= 18
+    for n in names:
+        assert hasattr(lib, n), n
+        assert n in L.SIGNATURES, f"{n} has no ctypes signature"
+    assert lib.fomo_version().decode().startswith("fomo-mi355x")
+
+
+def test_fd_coefficients_match_oracle(orc):
+    import fomo
+    for order in (2, 4, 6, 8, 10):
+        assert np.array_equal(fomo.get_fd_coefficients(order), orc.fd_coeffs(order))
+    with pytest.raises(ValueError):
+        fomo.get_fd_coefficients(12)
+
+
+@pytest.mark.parametrize("f0,dt,nt", [(15.0, 1e-3, 2000), (20.0, 4e-4, 4000), (12.0, 8e-4, 1500),
+                                      (10.0, 1e-3, 1500), (7.3, 1.23e-3, 777)])
+def test_ricker_matches_oracle_bitwise(orc, f0, dt, nt):
+    import fomo
+    assert np.array_equal(fomo.ricker_wavelet(f0, dt, nt), orc.ricker(f0, dt, nt))
+
+
+def test_habc_coefficients_match_oracle(orc):
+    import fomo
+    for args in [(2500, 1e-3, 10), (1800, 4e-4, 5), (2000, 8e-4, 10)]:
+        assert np.array_equal(np.float32(fomo.habc_coefficients(*args)), orc.habc_coeffs(*args, "f32"))
+
+
+def test_validation_errors_raise_argument_error():
+    """verify_patch.jl:263-267: CFL violation and out-of-range geometry throw ArgumentError
+    before any device work."""
+    import fomo
+    vp = np.full((60, 40), 2000.0, np.float32)
+    rho = np.full((60, 40), 2000.0, np.float32)
+    kw = dict(sx=[30], sz=[10], rx=[5, 6], rz=[5, 5], nbc=10, fd_order=4, verbose=False)
+    with pytest.raises(fomo.ArgumentError):
+        fomo.acoustic2d(vp, rho, 10.0, 9e-3, 100, 10.0, **kw)
+    with pytest.raises(fomo.ArgumentError):
+        fomo.elastic2d(vp, vp / 1.73, rho, 10.0, 9e-3, 100, 10.0, **kw)
+    with pytest.raises(fomo.ArgumentError):
+        fomo.acoustic2d(vp, rho, 10.0, 1e-3, 100, 10.0, **{**kw, "sx": [65]})
+    with pytest.raises(fomo.ArgumentError):
+        fomo.acoustic2d(vp, rho, 10.0, 1e-3, 100, 10.0, **{**kw, "rz": [0, 5]})
+    with pytest.raises(fomo.ArgumentError):
+        fomo.acoustic2d(vp, rho, 10.0, 1e-3, 100, 10.0, **kw, boundary="pml")
+    with pytest.raises(fomo.ArgumentError):
+        fomo.acoustic2d(vp, rho, 10.0, 1e-3, 100, 10.0, **kw, wavelet=np.zeros(99))
+
+
+def test_cfl_bound_matches_reference_formula():
+    """dt_max = dh / (vmax sqrt(2) sum|a|): just below passes, just above raises."""
+    from fomo.checks import check_numerics
+    import fomo
+    a = fomo.get_fd_coefficients(8).astype(np.float64)
+    dt_max = 10.0 / (3000.0 * np.sqrt(2.0) * np.abs(a).sum())
+    check_numerics(3000.0, 3000.0, 10.0, dt_max * 0.999, 5.0, 8)
+    with pytest.raises(fomo.ArgumentError):
+        check_numerics(3000.0, 3000.0, 10.0, dt_max * 1.001, 5.0, 8)
+
+
+def test_seeded_models_are_reproducible():
+    from fomo.models import layered_2d, model_digest
+    a = layered_2d(200, 80, seed=7)
+    b = layered_2d(200, 80, seed=7)
+    c = layered_2d(200, 80, seed=8)
+    assert model_digest(*a) == model_digest(*b) != model_digest(*c)
+    vp, vs, rho = a
+    assert np.all(vp[:, 0] == 0) and np.all(rho[:, 0] == 0)
+    assert 1800 <= vp[:, 1:].min() and vp.max() <= 4500
