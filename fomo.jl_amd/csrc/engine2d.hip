@@ -1,0 +1,383 @@
+// engine2d.hip -- 2D staggered-grid sweeps for gfx950 with the deterministic Higdon boundary fused in.
+//
+// One launch per phase (velocity; pressure or stress) over every tile of every shot.  A tile is
+// 64 (x, one wave64 row) by 4*ZPT (z) cells; each thread owns one column of ZPT cells.  The
+// fields the phase reads through FD stencils are staged into LDS with an (N+1)-cell halo; the
+// phase's own fields (time n, ping-pong buffer) and the materials are read once, coalesced.
+//
+// Deterministic HABC without a second pass: a frame cell's Higdon value needs the inward
+// neighbour's *pre-correction* value.  Tiles that touch the frame first compute pre-correction
+// values on the tile plus a one-cell ring into LDS (the halo of N+1 makes the ring's stencils
+// available on-chip), then blend.  Every output is a pure function of read-only buffers, so
+// results do not depend on launch geometry, tile shape or the other shots of a batch.
+//
+// Arithmetic order follows the reference expressions term by term; the library is compiled
+// with -ffp-contract=off, so FP32 results are bit-identical to the reference's uncontracted
+// Float32 updates:
+//   acoustic velocity / pressure ... src/equations/acoustic2d/fused_kernels.jl:47-61, :101-112
+//   elastic velocity / stress ...... src/equations/elastic2d/fused_kernels.jl:39-57, :105-121
+//   Higdon value ................... src/boundary/habc/kernels_det.jl:22-57
+//   injection / gather ............. src/acquisition/device_indexed.jl:28-134 (step index read
+//                                    from a device counter, as the reference's graph path does)
+#include "fomo_common.h"
+
+namespace fomo {
+
+// Resolved pointers of one phase of one shot.
+struct PV {
+  const float* sten[3];
+  const float* old[3];
+  float* out[3];
+  const float* mat[3];
+  const float* lut[3];
+};
+
+template <int PH>
+__device__ __forceinline__ void make_view(const Shot2D& S, int s, PV& V) {
+  const int o = 1 - s;
+  const int L = S.H + 1;
+  if (PH == kAcVel) {
+    V.sten[0] = S.fld[s][0];
+    V.old[0] = S.fld[s][1];
+    V.old[1] = S.fld[s][2];
+    V.out[0] = S.fld[o][1];
+    V.out[1] = S.fld[o][2];
+    V.mat[0] = S.mat[1];
+    V.mat[1] = S.mat[2];
+    V.lut[0] = S.wlut;
+    V.lut[1] = S.wlut + L;
+  } else if (PH == kAcPrs) {
+    V.sten[0] = S.fld[o][1];
+    V.sten[1] = S.fld[o][2];
+    V.old[0] = S.fld[s][0];
+    V.out[0] = S.fld[o][0];
+    V.mat[0] = S.mat[0];
+    V.lut[0] = S.wlut + 3 * L;
+  } else if (PH == kElVel) {
+    V.sten[0] = S.fld[s][2];
+    V.sten[1] = S.fld[s][3];
+    V.sten[2] = S.fld[s][4];
+    V.old[0] = S.fld[s][0];
+    V.old[1] = S.fld[s][1];
+    V.out[0] = S.fld[o][0];
+    V.out[1] = S.fld[o][1];
+    V.mat[0] = S.mat[3];
+    V.mat[1] = S.mat[4];
+    V.lut[0] = S.wlut;
+    V.lut[1] = S.wlut + L;
+  } else {
+    V.sten[0] = S.fld[o][0];
+    V.sten[1] = S.fld[o][1];
+    for (int f = 0; f < 3; ++f) {
+      V.old[f] = S.fld[s][2 + f];
+      V.out[f] = S.fld[o][2 + f];
+      V.mat[f] = S.mat[f];
+      V.lut[f] = S.wlut + 3 * L;
+    }
+  }
+}
+
+// Stencil data staged in LDS: tile k covers global cells [oi, oi+TW) x [oj, oj+TH).
+template <int TW, int TH>
+struct TileRd {
+  const float* t;
+  int oi, oj;
+  __device__ __forceinline__ float operator()(int k, int i, int j) const {
+    return t[k * TW * TH + (j - oj) * TW + (i - oi)];
+  }
+};
+
+// Pre-correction values of all output fields of a phase at padded cell (i, j): the FD update
+// inside the update region N <= i < nx-N, N <= j < nz-N, the old value elsewhere.
+template <int PH, int N>
+struct Phys;
+
+template <int N>
+struct Phys<kAcVel, N> {
+  static constexpr int NF = 2, NS = 1;
+  template <class R>
+  __device__ __forceinline__ static void pre_all(const R& rd, const PV& V, const Shot2D& S, const float* a, int i,
+                                                 int j, float* v) {
+    const size_t c = (size_t)i + (size_t)j * (size_t)S.nx;
+    v[0] = V.old[0][c];
+    v[1] = V.old[1][c];
+    if (i < N || i >= S.nx - N || j < N || j >= S.nz - N) return;
+    float dx = 0.0f, dz = 0.0f;
+#pragma unroll
+    for (int l = 1; l <= N; ++l) {
+      const float cl = a[l - 1];
+      dx += cl * (rd(0, i + l - 1, j) - rd(0, i - l, j));
+      dz += cl * (rd(0, i, j + l) - rd(0, i, j - l + 1));
+    }
+    v[0] = v[0] + (V.mat[0][c] * S.dtx) * dx;
+    v[1] = v[1] + (V.mat[1][c] * S.dtx) * dz;
+  }
+};
+
+template <int N>
+struct Phys<kAcPrs, N> {
+  static constexpr int NF = 1, NS = 2;
+  template <class R>
+  __device__ __forceinline__ static void pre_all(const R& rd, const PV& V, const Shot2D& S, const float* a, int i,
+                                                 int j, float* v) {
+    const size_t c = (size_t)i + (size_t)j * (size_t)S.nx;
+    v[0] = V.old[0][c];
+    if (i < N || i >= S.nx - N || j < N || j >= S.nz - N) return;
+    float dvxdx = 0.0f, dvzdz = 0.0f;
+#pragma unroll
+    for (int l = 1; l <= N; ++l) {
+      const float cl = a[l - 1];
+      dvxdx += cl * (rd(0, i + l, j) - rd(0, i - l + 1, j));
+      dvzdz += cl * (rd(1, i, j + l - 1) - rd(1, i, j - l));
+    }
+    v[0] = v[0] + V.mat[0][c] * (dvxdx * S.dtx + dvzdz * S.dtx);
+  }
+};
+
+template <int N>
+struct Phys<kElVel, N> {
+  static constexpr int NF = 2, NS = 3;
+  template <class R>
+  __device__ __forceinline__ static void pre_all(const R& rd, const PV& V, const Shot2D& S, const float* a, int i,
+                                                 int j, float* v) {
+    const size_t c = (size_t)i + (size_t)j * (size_t)S.nx;
+    v[0] = V.old[0][c];
+    v[1] = V.old[1][c];
+    if (i < N || i >= S.nx - N || j < N || j >= S.nz - N) return;
+    float dtxxdx = 0.0f, dtxzdz = 0.0f, dtxzdx = 0.0f, dtzzdz = 0.0f;
+#pragma unroll
+    for (int l = 1; l <= N; ++l) {
+      const float cl = a[l - 1];
+      dtxxdx += cl * (rd(0, i + l - 1, j) - rd(0, i - l, j));
+      dtxzdz += cl * (rd(2, i, j + l - 1) - rd(2, i, j - l));
+      dtxzdx += cl * (rd(2, i + l, j) - rd(2, i - l + 1, j));
+      dtzzdz += cl * (rd(1, i, j + l) - rd(1, i, j - l + 1));
+    }
+    const float dt = S.dtx;
+    v[0] = v[0] + V.mat[0][c] * (dt * dtxxdx + dt * dtxzdz);
+    v[1] = v[1] + V.mat[1][c] * (dt * dtxzdx + dt * dtzzdz);
+  }
+};
+
+template <int N>
+struct Phys<kElStr, N> {
+  static constexpr int NF = 3, NS = 2;
+  template <class R>
+  __device__ __forceinline__ static void pre_all(const R& rd, const PV& V, const Shot2D& S, const float* a, int i,
+                                                 int j, float* v) {
+    const size_t c = (size_t)i + (size_t)j * (size_t)S.nx;
+    v[0] = V.old[0][c];
+    v[1] = V.old[1][c];
+    v[2] = V.old[2][c];
+    if (i < N || i >= S.nx - N || j < N || j >= S.nz - N) return;
+    float dvxdx = 0.0f, dvzdz = 0.0f, dvxdz = 0.0f, dvzdx = 0.0f;
+#pragma unroll
+    for (int l = 1; l <= N; ++l) {
+      const float cl = a[l - 1];
+      dvxdx += cl * (rd(0, i + l, j) - rd(0, i - l + 1, j));
+      dvzdz += cl * (rd(1, i, j + l - 1) - rd(1, i, j - l));
+      dvxdz += cl * (rd(0, i, j + l) - rd(0, i, j - l + 1));
+      dvzdx += cl * (rd(1, i + l - 1, j) - rd(1, i - l, j));
+    }
+    const float dt = S.dtx;
+    const float lv = V.mat[0][c], l2m = V.mat[1][c];
+    v[0] = v[0] + ((l2m * dvxdx) * dt + (lv * dvzdz) * dt);
+    v[1] = v[1] + ((lv * dvxdx) * dt + (l2m * dvzdz) * dt);
+    v[2] = v[2] + V.mat[2][c] * (dvxdz * dt + dvzdx * dt);
+  }
+};
+
+// Which output fields receive the raw-add source after the HABC.
+template <int PH>
+__device__ __forceinline__ int inj_mask(const Shot2D& S) {
+  if (PH == kAcPrs) return 1;
+  if (PH == kElVel) return S.src_fz ? 2 : 0;
+  if (PH == kElStr) return S.src_fz ? 0 : 3;
+  return 0;
+}
+
+template <int NF>
+__device__ __forceinline__ void inject(const Shot2D& S, int mask, int c, int it, int nt, float* v) {
+  if (mask == 0 || it >= nt) return;
+  for (int s = 0; s < S.n_src; ++s) {
+    if (S.src_c[s] == c) {
+      const float w = S.wav[(size_t)s + (size_t)S.n_src * (size_t)it];
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+        if (mask & (1 << f)) v[f] += w;
+    }
+  }
+}
+
+template <int PH, int N, int ZPT>
+__global__ __launch_bounds__(256) void sweep2d_kernel(const SweepArgs A) {
+  using P = Phys<PH, N>;
+  constexpr int NF = P::NF, NS = P::NS;
+  constexpr int BX = kTileW, BY = 4, BZ = BY * ZPT;
+  constexpr int TW = BX + 2 * N + 2, TH = BZ + 2 * N + 2;
+  constexpr int EW = BX + 2, EH = BZ + 2;
+  __shared__ float tile[NS * TW * TH];
+  __shared__ float pre_s[NF * EW * EH];
+
+  // XCD-aware order: hardware hands block b to XCD b % 8, so each XCD gets a contiguous run of
+  // tiles (gridDim.x is a multiple of 8) and shares halo rows through its own L2.
+  const int per = gridDim.x >> 3;
+  const int w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (w >= A.n_work) return;
+  const int2 wk = A.work[w];
+  const Shot2D& S = A.shots[wk.x];
+  const int nx = S.nx, nz = S.nz, H = S.H;
+  const int tj = wk.y / S.tiles_x;
+  const int i0 = (wk.y - tj * S.tiles_x) * BX, j0 = tj * BZ;
+  PV V;
+  make_view<PH>(S, A.parity, V);
+  const int tid = threadIdx.x + threadIdx.y * BX;
+  const int oi = i0 - N - 1, oj = j0 - N - 1;
+  for (int idx = tid; idx < TW * TH; idx += BX * BY) {
+    const int r = idx / TW, cc = idx - r * TW;
+    const int gi = oi + cc, gj = oj + r;
+    const bool in = gi >= 0 && gi < nx && gj >= 0 && gj < nz;
+    const size_t g = in ? (size_t)gi + (size_t)gj * (size_t)nx : 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) tile[k * TW * TH + idx] = in ? V.sten[k][g] : 0.0f;
+  }
+  __syncthreads();
+  const TileRd<TW, TH> rd{tile, oi, oj};
+  const int it = *A.step;
+  const int mask = inj_mask<PH>(S);
+  const int i = i0 + (int)threadIdx.x;
+  const bool interior = i0 > H && i0 + BX - 1 < nx - 1 - H && j0 > H && j0 + BZ - 1 < nz - 1 - H;
+  if (interior) {
+#pragma unroll
+    for (int k = 0; k < ZPT; ++k) {
+      const int j = j0 + (int)threadIdx.y * ZPT + k;
+      float v[NF];
+      P::pre_all(rd, V, S, A.a, i, j, v);
+      const size_t c = (size_t)i + (size_t)j * (size_t)nx;
+      inject<NF>(S, mask, (int)c, it, A.nt, v);
+#pragma unroll
+      for (int f = 0; f < NF; ++f) V.out[f][c] = v[f];
+    }
+    return;
+  }
+  // Frame tile: pre-correction values on the tile and a one-cell ring.
+  for (int idx = tid; idx < EW * EH; idx += BX * BY) {
+    const int r = idx / EW, cc = idx - r * EW;
+    const int gi = i0 - 1 + cc, gj = j0 - 1 + r;
+    if (gi < 0 || gi >= nx || gj < 0 || gj >= nz) continue;
+    float v[NF];
+    P::pre_all(rd, V, S, A.a, gi, gj, v);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) pre_s[f * EW * EH + idx] = v[f];
+  }
+  __syncthreads();
+  if (i >= nx) return;
+  const float q = S.q, qt = S.qt, qxt = S.qxt;
+#pragma unroll
+  for (int k = 0; k < ZPT; ++k) {
+    const int j = j0 + (int)threadIdx.y * ZPT + k;
+    if (j >= nz) break;
+    const int li = (int)threadIdx.x + 1, lj = (int)threadIdx.y * ZPT + k + 1;
+    const size_t c = (size_t)i + (size_t)j * (size_t)nx;
+    float v[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) v[f] = pre_s[f * EW * EH + lj * EW + li];
+    const int dist = min(min(i, nx - 1 - i), min(j, nz - 1 - j));
+    if (dist >= 1 && dist <= H) {
+      // Region flags (0-based): left i <= H, right i >= nx-1-H, top j <= H, bottom j >= nz-1-H.
+      const bool Lf = i <= H, Rf = i >= nx - 1 - H, Tf = j <= H, Bf = j >= nz - 1 - H;
+      const bool inx = Lf || Rf, inz = Tf || Bf;
+      const int di = Lf ? 1 : -1, dj = Tf ? 1 : -1;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const float* __restrict__ fo = V.old[f];
+        const float foc = fo[c];
+        float sx = 0.0f, sz = 0.0f;
+        if (inx) {
+          const float fn = pre_s[f * EW * EH + lj * EW + li + di];
+          sx = (-q * fn - qt * foc) - qxt * fo[(size_t)(i + di) + (size_t)j * (size_t)nx];
+        }
+        if (inz) {
+          const float fn = pre_s[f * EW * EH + (lj + dj) * EW + li];
+          sz = (-q * fn - qt * foc) - qxt * fo[(size_t)i + (size_t)(j + dj) * (size_t)nx];
+        }
+        const float wt = V.lut[f][dist];
+        if (inx && inz)
+          v[f] = wt * v[f] + ((1.0f - wt) * 0.5f) * (sx + sz);
+        else if (inx)
+          v[f] = wt * v[f] + (1.0f - wt) * sx;
+        else
+          v[f] = wt * v[f] + (1.0f - wt) * sz;
+      }
+    }
+    inject<NF>(S, mask, (int)c, it, A.nt, v);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) V.out[f][c] = v[f];
+  }
+}
+
+__global__ __launch_bounds__(256) void gather2d_kernel(const GatherArgs A) {
+  const int it = *A.step;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < A.n_entries && it < A.nt) {
+    const Shot2D& S = A.shots[A.rec_shot[e]];
+    const size_t c = (size_t)A.rec_c[e];
+    const long long o = A.rec_out[e] + (long long)A.rec_stride[e] * it;
+    for (int k = 0; k < A.nslots; ++k) A.seis[k][o] = S.fld[A.parity_out][A.field_of_slot[k]][c];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned t = atomicAdd(A.done, 1u);
+    if (t == gridDim.x - 1) {
+      atomicExch(A.done, 0u);
+      atomicExch(A.step, it + 1);
+    }
+  }
+}
+
+int tile_rows(int tile_cfg) { return 4 * (tile_cfg == 0 ? 2 : (tile_cfg == 1 ? 4 : 8)); }
+
+template <int PH, int N>
+static hipError_t launch_cfg(int tile_cfg, const SweepArgs& a, hipStream_t s) {
+  const int blocks = (a.n_work + 7) / 8 * 8;
+  const dim3 blk(kTileW, 4);
+  switch (tile_cfg) {
+    case 0: hipLaunchKernelGGL((sweep2d_kernel<PH, N, 2>), dim3(blocks), blk, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((sweep2d_kernel<PH, N, 4>), dim3(blocks), blk, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((sweep2d_kernel<PH, N, 8>), dim3(blocks), blk, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int PH>
+static hipError_t launch_phase(int N, int tile_cfg, const SweepArgs& a, hipStream_t s) {
+  switch (N) {
+    case 1: return launch_cfg<PH, 1>(tile_cfg, a, s);
+    case 2: return launch_cfg<PH, 2>(tile_cfg, a, s);
+    case 3: return launch_cfg<PH, 3>(tile_cfg, a, s);
+    case 4: return launch_cfg<PH, 4>(tile_cfg, a, s);
+    case 5: return launch_cfg<PH, 5>(tile_cfg, a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_sweep2d(int phase, int N, int tile_cfg, const SweepArgs& a, hipStream_t s) {
+  if (a.n_work <= 0) return hipSuccess;
+  switch (phase) {
+    case kAcVel: return launch_phase<kAcVel>(N, tile_cfg, a, s);
+    case kAcPrs: return launch_phase<kAcPrs>(N, tile_cfg, a, s);
+    case kElVel: return launch_phase<kElVel>(N, tile_cfg, a, s);
+    case kElStr: return launch_phase<kElStr>(N, tile_cfg, a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_gather2d(const GatherArgs& a, hipStream_t s) {
+  const int blocks = a.n_entries > 0 ? (a.n_entries + 255) / 256 : 1;
+  hipLaunchKernelGGL(gather2d_kernel, dim3(blocks), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace fomo

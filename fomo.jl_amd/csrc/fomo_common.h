@@ -4,6 +4,11 @@
 // cell (i, j) of a 2D grid lives at i + j*nx.  The reference's 1-based (i_ref, j_ref) is
 // (i+1, j+1).  Each time-dependent field has two buffers (ping-pong): after k steps the state
 // of every field lives in buffer k % 2.
+//
+// A run is a table of *shots*.  A solo run is a table of one shot; a batch (the reference's
+// acoustic2d_batch, or uneven per-shot windows) is a longer table.  Every kernel walks a flat
+// work list of (shot, tile) pairs, so a shot's arithmetic does not depend on the other shots
+// in the table: a batched shot is bit-identical to its solo run.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stddef.h>
@@ -16,43 +21,55 @@ constexpr int kMaxN = 5;  // fd_order/2 for fd_order <= 10
 // Sweep phases.
 enum Phase : int { kAcVel = 0, kAcPrs = 1, kElVel = 2, kElStr = 3 };
 
-// One sweep (one phase of one time step) over a 2D padded grid.  Everything the sweep reads is
-// either a read-only input of this phase or the phase's own fields at time n (old[]), and every
-// write goes to out[] (time n+1): the sweep is a pure function of its inputs.
-struct Sweep2DArgs {
-  const float* rd[3];    // stencil inputs: AcVel {p}; AcPrs {vx,vz}; ElVel {txx,tzz,txz}; ElStr {vx,vz}
-  const float* old[3];   // this phase's fields at time n
-  float* out[3];         // this phase's fields at time n+1
-  const float* mat[3];   // AcVel {bx,bz}; AcPrs {kappa}; ElVel {bx,bz}; ElStr {lam,lam2mu,muxz}
-  const float* wlut[3];  // HABC blend weight as a function of the distance to the grid edge
-  float a[kMaxN];        // staggered FD coefficients
-  float dtx;             // Float32(dt/dh)
-  float q, qt, qxt;      // Higdon coefficients
-  int nx, nz;            // padded dims
-  int H;                 // HABC frame width (pad - 1)
-  // Raw-add injection, applied after the HABC to the fields in inj_mask.
-  int inj_mask;
+// One shot of a 2D run (device-resident table entry).
+struct Shot2D {
+  float* fld[2][5];     // ping-pong fields. acoustic: p, vx, vz; elastic: vx, vz, txx, tzz, txz
+  const float* mat[5];  // acoustic: kappa, bx, bz; elastic: lam, lam2mu, muxz, bx, bz
+  const float* wlut;    // HABC weights by distance to the edge: 4 tables of H+1 (vx, vz, vy, tau)
+  float dtx;            // Float32(dt/dh)
+  float q, qt, qxt;     // Higdon coefficients
+  int nx, nz;           // padded dims
+  int H;                // HABC frame width (pad - 1)
+  int tiles_x;          // tiles along x for the launched tile shape
   int n_src;
-  const int* src_c;      // padded cell index of each source
-  const float* wav;      // n_src x nt, column-major
-  const int* step;       // device step counter (0-based index of the step being taken)
+  const int* src_c;     // padded cell index of each source
+  const float* wav;     // n_src x nt
+  int src_fz;           // elastic: 1 = vertical force into vz, 0 = explosive into txx, tzz
+};
+
+// Arguments of one sweep launch over all shots.
+struct SweepArgs {
+  const Shot2D* shots;
+  const int2* work;     // (shot, tile) pairs
+  int n_work;
+  int parity;           // state parity s: read buffers [s], write buffers [1-s]
+  float a[kMaxN];       // staggered FD coefficients
+  const int* step;      // device step counter (0-based index of the step being taken)
   int nt;
 };
 
-// Receiver gather after the step, then the step counter is advanced by the last block.
-struct RecordArgs {
-  const float* f[4];
-  float* seis[4];
-  int nf;
-  const int* rec_c;      // padded cell index of each receiver
-  int n_rec;
+// Receiver gather after the step over all shots; the last block advances the step counter.
+struct GatherArgs {
+  const Shot2D* shots;
+  const int* rec_shot;  // shot of each receiver entry
+  const int* rec_c;     // padded cell index
+  const long long* rec_out;  // offset of (receiver, step 0) in the seismogram buffer
+  const int* rec_stride;     // step stride (n_rec of the shot)
+  float* seis[3];       // slot buffers (acoustic p,vx,vz; elastic vx,vz)
+  int field_of_slot[3]; // field index gathered into each slot
+  int nslots;
+  int n_entries;
+  int parity_out;       // buffer holding the state after this step
   int nt;
   int* step;
   unsigned* done;
 };
 
-// Launchers (defined in sweep2d.hip).  Return hipSuccess or the launch error.
-hipError_t launch_sweep2d(int phase, int N, const Sweep2DArgs& a, hipStream_t s);
-hipError_t launch_record(const RecordArgs& a, hipStream_t s);
+// Launchers (defined in engine2d.hip).  tile_cfg selects rows per thread (see kTileCfgs).
+hipError_t launch_sweep2d(int phase, int N, int tile_cfg, const SweepArgs& a, hipStream_t s);
+hipError_t launch_gather2d(const GatherArgs& a, hipStream_t s);
+int tile_rows(int tile_cfg);  // rows (z) covered by one tile
+constexpr int kTileW = 64;    // tile width (x): one wave64 row
+constexpr int kNumTileCfgs = 3;
 
 }  // namespace fomo

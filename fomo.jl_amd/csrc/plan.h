@@ -1,0 +1,78 @@
+// plan.h -- host-side plan objects behind the opaque fomo_plan handle.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/fomo_hip.h"
+
+namespace fomo {
+
+int set_error(int code, const std::string& msg);
+
+#define FOMO_HIP_TRY(expr)                                                                    \
+  do {                                                                                        \
+    hipError_t _e = (expr);                                                                   \
+    if (_e != hipSuccess)                                                                     \
+      return ::fomo::set_error(_e == hipErrorOutOfMemory ? FOMO_ENOMEM : FOMO_EHIP,           \
+                               std::string(#expr) + ": " + hipGetErrorString(_e));            \
+  } while (0)
+
+// Common part of every plan: device allocations owned by the plan, its stream, the step
+// counter, and the captured two-step graph.
+struct PlanBase {
+  fomo_desc d{};
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocs;
+  int* step = nullptr;
+  unsigned* done = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  long steps_done = 0;
+
+  virtual ~PlanBase();
+  // Enqueue one step from state parity s; ev (4 events, optional) brackets the launches.
+  virtual hipError_t enqueue_step(int s, hipEvent_t* ev) = 0;
+  virtual int reset() = 0;
+  virtual int read_seis(int shot, float* const* dst) = 0;
+  virtual int read_field(int shot, int which, float* out) = 0;
+  virtual int dims(int shot, int* nx, int* ny, int* nz, int* pad) const = 0;
+  virtual int n_shots() const = 0;
+
+  int advance(int n);
+  int build_graph();
+  template <class T>
+  int alloc(T** ptr, size_t count) {
+    void* v = nullptr;
+    const size_t bytes = (count ? count : 1) * sizeof(T);
+    hipError_t e = hipMalloc(&v, bytes);
+    if (e != hipSuccess)
+      return set_error(e == hipErrorOutOfMemory ? FOMO_ENOMEM : FOMO_EHIP,
+                       "hipMalloc(" + std::to_string(bytes) + " B): " + hipGetErrorString(e));
+    allocs.push_back(v);
+    *ptr = (T*)v;
+    return FOMO_OK;
+  }
+  template <class T>
+  int upload(T** dst, const T* src, size_t count) {
+    int rc = alloc(dst, count);
+    if (rc) return rc;
+    if (count) FOMO_HIP_TRY(hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+    return FOMO_OK;
+  }
+};
+
+// Host-side setup shared by the solvers (fomo_api.hip).
+void pad2d(const float* in, int nxi, int nzi, int pad, int x0, int nx_model, std::vector<float>& out);
+void buoyancy2d(const std::vector<float>& rho, int nx, int nz, std::vector<float>& bx, std::vector<float>& bz);
+void elastic_params2d(const std::vector<float>& vp, const std::vector<float>& vs, const std::vector<float>& rho,
+                      int nx, int nz, std::vector<float>& lam, std::vector<float>& lam2mu, std::vector<float>& muxz);
+void weight_lut(int H, int pad, double off, float* out);
+float min_positive(const float* a, size_t n);
+
+}  // namespace fomo
+
+struct fomo_plan {
+  fomo::PlanBase* impl;
+};

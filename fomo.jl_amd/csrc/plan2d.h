@@ -1,0 +1,45 @@
+// plan2d.h -- 2D plan object (one or many shots).
+#pragma once
+#include <vector>
+
+#include "fomo_common.h"
+#include "plan.h"
+
+namespace fomo {
+
+struct ShotMeta {
+  int nxi, nzi, pad, x0, n_rec;
+  long long seis_off;
+};
+
+struct Plan2D : PlanBase {
+  int N = 0, nfld = 0, nslots = 0, tile_cfg = 1;
+  float coef[kMaxN] = {0};
+  std::vector<Shot2D> shots;
+  std::vector<ShotMeta> meta;
+  Shot2D* d_shots = nullptr;
+  int2* d_work[kNumTileCfgs] = {nullptr};
+  int n_work[kNumTileCfgs] = {0};
+  int n_entries = 0;
+  int* rec_shot = nullptr;
+  int* rec_c = nullptr;
+  long long* rec_out = nullptr;
+  int* rec_stride = nullptr;
+  float* seis[3] = {nullptr};
+  long long seis_total = 0;
+
+  ~Plan2D() override;
+  hipError_t enqueue_step(int s, hipEvent_t* ev) override;
+  int reset() override;
+  int read_seis(int shot, float* const* dst) override;
+  int read_field(int shot, int which, float* out) override;
+  int dims(int shot, int* nx, int* ny, int* nz, int* pad) const override;
+  int n_shots() const override { return (int)shots.size(); }
+  int make_work(int cfg);
+  int autotune();
+};
+
+int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const float* rho, int n_shots,
+                  const fomo_shot* shots, Plan2D** out);
+
+}  // namespace fomo

@@ -1,0 +1,333 @@
+// plan2d.hip -- 2D plans: one or many shots (windows of one model) advanced by shared launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+#include "fomo_common.h"
+#include "plan.h"
+#include "plan2d.h"
+
+namespace fomo {
+
+Plan2D::~Plan2D() = default;
+
+int Plan2D::make_work(int cfg) {
+  if (d_work[cfg]) return FOMO_OK;
+  std::vector<int2> w;
+  const int bz = tile_rows(cfg);
+  for (size_t s = 0; s < shots.size(); ++s) {
+    const int tx = (shots[s].nx + kTileW - 1) / kTileW, tz = (shots[s].nz + bz - 1) / bz;
+    for (int t = 0; t < tx * tz; ++t) w.push_back(make_int2((int)s, t));
+  }
+  n_work[cfg] = (int)w.size();
+  return upload(&d_work[cfg], w.data(), w.size());
+}
+
+hipError_t Plan2D::enqueue_step(int s, hipEvent_t* ev) {
+  SweepArgs a{};
+  a.shots = d_shots;
+  a.work = d_work[tile_cfg];
+  a.n_work = n_work[tile_cfg];
+  a.parity = s;
+  for (int l = 0; l < N; ++l) a.a[l] = coef[l];
+  a.step = step;
+  a.nt = d.nt;
+  GatherArgs g{};
+  g.shots = d_shots;
+  g.rec_shot = rec_shot;
+  g.rec_c = rec_c;
+  g.rec_out = rec_out;
+  g.rec_stride = rec_stride;
+  g.nslots = nslots;
+  for (int k = 0; k < nslots; ++k) {
+    g.seis[k] = seis[k];
+    g.field_of_slot[k] = k;  // acoustic p,vx,vz = fields 0,1,2; elastic vx,vz = fields 0,1
+  }
+  g.n_entries = n_entries;
+  g.parity_out = 1 - s;
+  g.nt = d.nt;
+  g.step = step;
+  g.done = done;
+  const int ph_v = d.eq == FOMO_ACOUSTIC2D ? kAcVel : kElVel;
+  const int ph_s = d.eq == FOMO_ACOUSTIC2D ? kAcPrs : kElStr;
+  hipError_t e = hipSuccess;
+  if (ev) e = hipEventRecord(ev[0], stream);
+  if (e == hipSuccess) e = launch_sweep2d(ph_v, N, tile_cfg, a, stream);
+  if (e == hipSuccess && ev) e = hipEventRecord(ev[1], stream);
+  if (e == hipSuccess) e = launch_sweep2d(ph_s, N, tile_cfg, a, stream);
+  if (e == hipSuccess && ev) e = hipEventRecord(ev[2], stream);
+  if (e == hipSuccess) e = launch_gather2d(g, stream);
+  if (e == hipSuccess && ev) e = hipEventRecord(ev[3], stream);
+  return e;
+}
+
+int Plan2D::reset() {
+  for (auto& sh : shots) {
+    const size_t n = (size_t)sh.nx * sh.nz;
+    for (int b = 0; b < 2; ++b)
+      for (int f = 0; f < nfld; ++f) FOMO_HIP_TRY(hipMemsetAsync(sh.fld[b][f], 0, n * sizeof(float), stream));
+  }
+  for (int k = 0; k < nslots; ++k)
+    if (seis_total) FOMO_HIP_TRY(hipMemsetAsync(seis[k], 0, (size_t)seis_total * sizeof(float), stream));
+  FOMO_HIP_TRY(hipMemsetAsync(step, 0, sizeof(int), stream));
+  FOMO_HIP_TRY(hipMemsetAsync(done, 0, sizeof(unsigned), stream));
+  steps_done = 0;
+  return FOMO_OK;
+}
+
+int Plan2D::read_seis(int shot, float* const* dst) {
+  if (shot < 0 || shot >= (int)shots.size()) return set_error(FOMO_EINVAL, "shot index out of range");
+  FOMO_HIP_TRY(hipStreamSynchronize(stream));
+  const size_t cnt = (size_t)meta[shot].n_rec * d.nt;
+  for (int k = 0; k < nslots; ++k)
+    if (dst[k] && cnt)
+      FOMO_HIP_TRY(hipMemcpy(dst[k], seis[k] + meta[shot].seis_off, cnt * sizeof(float), hipMemcpyDeviceToHost));
+  return FOMO_OK;
+}
+
+int Plan2D::read_field(int shot, int which, float* out) {
+  if (shot < 0 || shot >= (int)shots.size()) return set_error(FOMO_EINVAL, "shot index out of range");
+  if (which < 0 || which >= nfld) return set_error(FOMO_EINVAL, "field index out of range");
+  FOMO_HIP_TRY(hipStreamSynchronize(stream));
+  const Shot2D& sh = shots[shot];
+  FOMO_HIP_TRY(hipMemcpy(out, sh.fld[steps_done % 2][which], (size_t)sh.nx * sh.nz * sizeof(float),
+                         hipMemcpyDeviceToHost));
+  return FOMO_OK;
+}
+
+int Plan2D::dims(int shot, int* nx, int* ny, int* nz, int* pad) const {
+  if (shot < 0 || shot >= (int)shots.size()) return set_error(FOMO_EINVAL, "shot index out of range");
+  if (nx) *nx = shots[shot].nx;
+  if (ny) *ny = 1;
+  if (nz) *nz = shots[shot].nz;
+  if (pad) *pad = meta[shot].pad;
+  return FOMO_OK;
+}
+
+// Times a few direct-launch steps per tile shape and keeps the fastest.  The choice is cached
+// per (equation, order, shot count, padded size) for the life of the process; FOMO_TILE_CFG
+// forces a shape.
+int Plan2D::autotune() {
+  static std::map<std::tuple<int, int, int, long long>, int> cache;
+  long long cells = 0;
+  for (auto& sh : shots) cells += (long long)sh.nx * sh.nz;
+  const auto key = std::make_tuple(d.eq, N, (int)shots.size(), cells);
+  if (const char* env = std::getenv("FOMO_TILE_CFG")) {
+    const int c = std::atoi(env);
+    if (c >= 0 && c < kNumTileCfgs) {
+      tile_cfg = c;
+      return make_work(c);
+    }
+  }
+  auto it = cache.find(key);
+  if (it != cache.end()) {
+    tile_cfg = it->second;
+    return make_work(tile_cfg);
+  }
+  hipEvent_t e0, e1;
+  FOMO_HIP_TRY(hipEventCreate(&e0));
+  FOMO_HIP_TRY(hipEventCreate(&e1));
+  float best = 1e30f;
+  int best_cfg = 1;
+  const int reps = cells > (1 << 22) ? 4 : 10;
+  for (int c = 0; c < kNumTileCfgs; ++c) {
+    int rc = make_work(c);
+    if (rc) return rc;
+    tile_cfg = c;
+    for (int r = 0; r < 2; ++r) FOMO_HIP_TRY(enqueue_step(r & 1, nullptr));
+    FOMO_HIP_TRY(hipEventRecord(e0, stream));
+    for (int r = 0; r < reps; ++r) FOMO_HIP_TRY(enqueue_step(r & 1, nullptr));
+    FOMO_HIP_TRY(hipEventRecord(e1, stream));
+    FOMO_HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    FOMO_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) {
+      best = ms;
+      best_cfg = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  tile_cfg = best_cfg;
+  cache[key] = best_cfg;
+  return reset();
+}
+
+static int check_pts(const char* what, int n, const int32_t* x, const int32_t* z, int nx, int nz) {
+  if (n > 0 && (!x || !z)) return set_error(FOMO_EINVAL, std::string(what) + " arrays must not be NULL");
+  for (int s = 0; s < n; ++s)
+    if (x[s] < 1 || x[s] > nx || z[s] < 1 || z[s] > nz)
+      return set_error(FOMO_EINVAL, std::string(what) + " index out of range: need 1 <= x <= " + std::to_string(nx) +
+                                        ", 1 <= z <= " + std::to_string(nz));
+  return FOMO_OK;
+}
+
+int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const float* rho, int n_shots,
+                  const fomo_shot* sh, Plan2D** out) {
+  *out = nullptr;
+  if (d->eq != FOMO_ACOUSTIC2D && d->eq != FOMO_ELASTIC2D) return set_error(FOMO_EINVAL, "not a 2D equation");
+  if (d->storage != FOMO_STORE_F32) return set_error(FOMO_EUNSUP, "2D solvers store Float32 only");
+  if (n_shots < 1) return set_error(FOMO_EINVAL, "at least one shot is required");
+  if (!vp || !rho || (d->eq == FOMO_ELASTIC2D && !vs)) return set_error(FOMO_EINVAL, "model arrays must not be NULL");
+  if (!sh) return set_error(FOMO_EINVAL, "shot table is NULL");
+  for (int s = 0; s < n_shots; ++s) {
+    if (sh[s].x0 < 1 || sh[s].nx < 1 || sh[s].x0 + sh[s].nx - 1 > d->nx)
+      return set_error(FOMO_EINVAL, "shot " + std::to_string(s) + ": window outside the model");
+    int rc = check_pts("source", sh[s].n_src, sh[s].sx, sh[s].sz, sh[s].nx, d->nz);
+    if (!rc) rc = check_pts("receiver", sh[s].n_rec, sh[s].rx, sh[s].rz, sh[s].nx, d->nz);
+    if (rc) return rc;
+    if (sh[s].n_src > 0 && !sh[s].wavelet) return set_error(FOMO_EINVAL, "wavelet must not be NULL");
+  }
+  Plan2D* p = new Plan2D();
+  p->d = *d;
+  p->N = d->fd_order / 2;
+  fomo_fd_coefficients(d->fd_order, p->coef);
+  p->nfld = d->eq == FOMO_ACOUSTIC2D ? 3 : 5;
+  p->nslots = d->eq == FOMO_ACOUSTIC2D ? 3 : 2;
+  auto bail = [&](int code) {
+    delete p;
+    return code;
+  };
+  hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return bail(set_error(FOMO_EHIP, std::string("stream: ") + hipGetErrorString(e)));
+  const int pad = d->nbc + p->N, H = pad - 1;
+  const float dtx = d->dt / d->dh;
+  // Medium and HABC tables, shared between shots with the same window and v_ref.
+  std::map<std::tuple<int, int, float>, int> medium_of;
+  std::map<const float*, const float*> wav_dev;
+  int rc = FOMO_OK;
+  long long seis_off = 0;
+  std::vector<int> rs, rcell, rstride;
+  std::vector<long long> rout;
+  for (int s = 0; s < n_shots; ++s) {
+    const fomo_shot& S = sh[s];
+    Shot2D x{};
+    ShotMeta m{};
+    m.nxi = S.nx;
+    m.nzi = d->nz;
+    m.pad = pad;
+    m.x0 = S.x0;
+    m.n_rec = S.n_rec;
+    x.nx = S.nx + 2 * pad;
+    x.nz = d->nz + 2 * pad;
+    x.H = H;
+    x.dtx = dtx;
+    x.tiles_x = (x.nx + kTileW - 1) / kTileW;
+    const size_t n = (size_t)x.nx * x.nz;
+    float v_ref = S.v_ref;
+    if (!(v_ref > 0.0f)) {
+      std::vector<float> crop((size_t)S.nx * d->nz);
+      for (int j = 0; j < d->nz; ++j)
+        std::memcpy(&crop[(size_t)j * S.nx], vp + (size_t)(S.x0 - 1) + (size_t)j * d->nx, S.nx * sizeof(float));
+      v_ref = min_positive(crop.data(), crop.size());
+      if (!(v_ref > 0.0f)) return bail(set_error(FOMO_EINVAL, "vp has no positive entry in shot window"));
+    }
+    float hc[3];
+    fomo_habc_coefficients(v_ref, d->dt, d->dh, hc);
+    x.q = hc[0];
+    x.qt = hc[1];
+    x.qxt = hc[2];
+    const auto key = std::make_tuple(S.x0, S.nx, v_ref);
+    auto mit = medium_of.find(key);
+    if (mit != medium_of.end()) {
+      const Shot2D& o = p->shots[mit->second];
+      for (int k = 0; k < 5; ++k) x.mat[k] = o.mat[k];
+      x.wlut = o.wlut;
+    } else {
+      std::vector<float> vpp, rhop, bx, bz;
+      pad2d(vp, S.nx, d->nz, pad, S.x0 - 1, d->nx, vpp);
+      pad2d(rho, S.nx, d->nz, pad, S.x0 - 1, d->nx, rhop);
+      buoyancy2d(rhop, x.nx, x.nz, bx, bz);
+      float* dm[5] = {nullptr};
+      if (d->eq == FOMO_ACOUSTIC2D) {
+        std::vector<float> kappa(n);
+        for (size_t c = 0; c < n; ++c) kappa[c] = rhop[c] * (vpp[c] * vpp[c]);
+        if ((rc = p->upload(&dm[0], kappa.data(), n)) || (rc = p->upload(&dm[1], bx.data(), n)) ||
+            (rc = p->upload(&dm[2], bz.data(), n)))
+          return bail(rc);
+      } else {
+        std::vector<float> vsp, lam, l2m, muxz;
+        pad2d(vs, S.nx, d->nz, pad, S.x0 - 1, d->nx, vsp);
+        elastic_params2d(vpp, vsp, rhop, x.nx, x.nz, lam, l2m, muxz);
+        if ((rc = p->upload(&dm[0], lam.data(), n)) || (rc = p->upload(&dm[1], l2m.data(), n)) ||
+            (rc = p->upload(&dm[2], muxz.data(), n)) || (rc = p->upload(&dm[3], bx.data(), n)) ||
+            (rc = p->upload(&dm[4], bz.data(), n)))
+          return bail(rc);
+      }
+      for (int k = 0; k < 5; ++k) x.mat[k] = dm[k];
+      std::vector<float> lut(4 * (size_t)(H + 1));
+      weight_lut(H, pad, 0.0, lut.data());
+      weight_lut(H, pad, 0.5, lut.data() + (H + 1));
+      weight_lut(H, pad, 0.25, lut.data() + 2 * (H + 1));
+      weight_lut(H, pad, 0.75, lut.data() + 3 * (H + 1));
+      float* dl = nullptr;
+      if ((rc = p->upload(&dl, lut.data(), lut.size()))) return bail(rc);
+      x.wlut = dl;
+      medium_of[key] = s;
+    }
+    for (int b = 0; b < 2; ++b)
+      for (int f = 0; f < p->nfld; ++f)
+        if ((rc = p->alloc(&x.fld[b][f], n))) return bail(rc);
+    // sources
+    x.n_src = S.n_src;
+    x.src_fz = d->source_type == FOMO_SRC_FZ ? 1 : 0;
+    {
+      std::vector<int> sc(std::max(S.n_src, 1), -1);
+      for (int q = 0; q < S.n_src; ++q) sc[q] = (S.sx[q] - 1 + pad) + (S.sz[q] - 1 + pad) * x.nx;
+      int* dsc = nullptr;
+      if ((rc = p->upload(&dsc, sc.data(), sc.size()))) return bail(rc);
+      x.src_c = dsc;
+      if (S.n_src > 0) {
+        auto wit = wav_dev.find(S.wavelet);
+        if (wit != wav_dev.end()) {
+          x.wav = wit->second;
+        } else {
+          float* dw = nullptr;
+          if ((rc = p->upload(&dw, S.wavelet, (size_t)S.n_src * d->nt))) return bail(rc);
+          x.wav = dw;
+          wav_dev[S.wavelet] = dw;
+        }
+      }
+    }
+    // receivers
+    m.seis_off = seis_off;
+    for (int r = 0; r < S.n_rec; ++r) {
+      rs.push_back(s);
+      rcell.push_back((S.rx[r] - 1 + pad) + (S.rz[r] - 1 + pad) * x.nx);
+      rout.push_back(seis_off + r);
+      rstride.push_back(S.n_rec);
+    }
+    seis_off += (long long)S.n_rec * d->nt;
+    p->shots.push_back(x);
+    p->meta.push_back(m);
+  }
+  p->seis_total = seis_off;
+  p->n_entries = (int)rs.size();
+  if ((rc = p->upload(&p->rec_shot, rs.data(), rs.size())) || (rc = p->upload(&p->rec_c, rcell.data(), rcell.size())) ||
+      (rc = p->upload(&p->rec_out, rout.data(), rout.size())) ||
+      (rc = p->upload(&p->rec_stride, rstride.data(), rstride.size())))
+    return bail(rc);
+  for (int k = 0; k < p->nslots; ++k)
+    if ((rc = p->alloc(&p->seis[k], (size_t)std::max<long long>(seis_off, 1)))) return bail(rc);
+  if ((rc = p->upload(&p->d_shots, p->shots.data(), p->shots.size()))) return bail(rc);
+  if ((rc = p->alloc(&p->step, 1)) || (rc = p->alloc(&p->done, 1))) return bail(rc);
+  if ((rc = p->reset())) return bail(rc);
+  if (d->tile_cfg > 0 && d->tile_cfg <= kNumTileCfgs) {
+    p->tile_cfg = d->tile_cfg - 1;
+    if ((rc = p->make_work(p->tile_cfg))) return bail(rc);
+  } else if ((rc = p->autotune())) {
+    return bail(rc);
+  }
+  if (d->use_graph && (rc = p->build_graph())) return bail(rc);
+  e = hipStreamSynchronize(p->stream);
+  if (e != hipSuccess) return bail(set_error(FOMO_EHIP, std::string("setup: ") + hipGetErrorString(e)));
+  *out = p;
+  return FOMO_OK;
+}
+
+}  // namespace fomo

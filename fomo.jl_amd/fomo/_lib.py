@@ -34,7 +34,17 @@ class Desc(C.Structure):
         ("dh", C.c_float), ("dt", C.c_float), ("v_ref", C.c_float),
         ("n_src", C.c_int32), ("n_rec", C.c_int32), ("source_type", C.c_int32), ("storage", C.c_int32),
         ("snap_interval", C.c_int32), ("snap_plane", C.c_int32), ("snap_index", C.c_int32),
-        ("use_graph", C.c_int32), ("record_p", C.c_int32), ("reserved", C.c_int32 * 7),
+        ("use_graph", C.c_int32), ("record_p", C.c_int32), ("tile_cfg", C.c_int32),
+        ("reserved", C.c_int32 * 6),
+    ]
+
+
+class Shot(C.Structure):
+    """struct fomo_shot."""
+    _fields_ = [
+        ("x0", C.c_int32), ("nx", C.c_int32), ("n_src", C.c_int32),
+        ("sx", C.c_void_p), ("sz", C.c_void_p), ("wavelet", C.c_void_p),
+        ("n_rec", C.c_int32), ("rx", C.c_void_p), ("rz", C.c_void_p), ("v_ref", C.c_float),
     ]
 
 
@@ -53,7 +63,16 @@ SIGNATURES = {
                                   C.POINTER(C.c_double)]),
     "fomo_elastic2d": (C.c_int, [C.POINTER(Desc), _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p,
                                  C.POINTER(C.c_double)]),
+    "fomo_acoustic2d_batch": (C.c_int, [C.POINTER(Desc), _p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p,
+                                        C.POINTER(C.c_double)]),
+    "fomo_elastic2d_batch": (C.c_int, [C.POINTER(Desc), _p, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p,
+                                       C.POINTER(C.c_double)]),
     "fomo_plan_create": (C.c_int, [C.POINTER(Desc), _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, C.POINTER(_p)]),
+    "fomo_batch_create": (C.c_int, [C.POINTER(Desc), _p, _p, _p, _i, C.POINTER(Shot), C.POINTER(_p)]),
+    "fomo_plan_n_shots": (C.c_int, [_p, C.POINTER(_i)]),
+    "fomo_plan_read_shot_seis": (C.c_int, [_p, _i, _p, _p, _p, _p]),
+    "fomo_plan_read_shot_field": (C.c_int, [_p, _i, _i, _p]),
+    "fomo_plan_shot_dims": (C.c_int, [_p, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)]),
     "fomo_plan_reset": (C.c_int, [_p]),
     "fomo_plan_advance": (C.c_int, [_p, _i]),
     "fomo_plan_timed_advance": (C.c_int, [_p, _i, C.POINTER(C.c_double)]),

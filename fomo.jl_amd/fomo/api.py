@@ -152,14 +152,95 @@ def elastic2d(vp, vs, rho, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_order=8
     return Elastic2DResult(svx, svz, snap_list, Stats(float(t.value)))
 
 
+Acoustic2DBatchResult = namedtuple("Acoustic2DBatchResult", ["seis_p", "seis_vx", "seis_vz", "stats"])
+Elastic2DBatchResult = namedtuple("Elastic2DBatchResult", ["seis_vx", "seis_vz", "stats"])
+BatchStats = namedtuple("BatchStats", ["kernel_time_s", "n_shots"])
+
+
+def _shot_matrix(sx, sz):
+    """batch.jl:194-198: a vector means one source per shot; a matrix is (n_shots, n_src)."""
+    sxm = np.asarray(sx, dtype=np.int64)
+    szm = np.asarray(sz, dtype=np.int64)
+    if sxm.ndim == 1:
+        sxm = sxm[:, None]
+    if szm.ndim == 1:
+        szm = szm[:, None]
+    if sxm.shape != szm.shape:
+        raise ArgumentError(f"sx and sz differ in size: {sxm.shape} vs {szm.shape}")
+    if sxm.shape[0] < 1:
+        raise ArgumentError("at least one shot is required")
+    return sxm, szm
+
+
+def _batch(eq, vp, vs, rho, dh, dt, nt, f0, sx, sz, rx, rz, nbc, fd_order, v_ref, wavelet,
+           scale_source_by_dt, use_cuda_graph):
+    vp, rho = _f32F(vp), _f32F(rho)
+    if vs is not None:
+        vs = _f32F(vs)
+    nx, nz = vp.shape
+    if nbc < 0 or nt < 0:
+        raise ArgumentError("nbc and nt must be >= 0")
+    if fd_order not in (2, 4, 6, 8, 10):
+        raise ValueError(f"Unsupported FD order: {fd_order}")
+    sxm, szm = _shot_matrix(sx, sz)
+    ns, n_src = sxm.shape
+    for s in range(ns):
+        check_geometry(nx, nz, sxm[s], szm[s], rx, rz)
+    vmin = _default_vref(vp)
+    check_numerics(float(vp.max()), vmin, dh, dt, f0, fd_order)
+    v_ref = vmin if v_ref is None else float(np.float32(v_ref))
+    wm = wavelet_matrix(wavelet, f0, dt, nt, n_src, scale_source_by_dt)
+    rx, rz = _i32(rx), _i32(rz)
+    n_rec = len(rx)
+    # device layout (n_src, n_shots), batch_acq.jl:27-37
+    sxd = np.ascontiguousarray(sxm.T.astype(np.int32))
+    szd = np.ascontiguousarray(szm.T.astype(np.int32))
+    d = _desc(eq, nx, nz, nbc, fd_order, nt, dh, dt, v_ref, n_src, n_rec, 0, use_cuda_graph)
+    nslots = 3 if eq == _lib.ACOUSTIC2D else 2
+    outs = [np.zeros((n_rec, nt, ns), np.float32, order="F") for _ in range(nslots)]
+    t = C.c_double(0.0)
+    if eq == _lib.ACOUSTIC2D:
+        _lib.check(_lib.lib().fomo_acoustic2d_batch(C.byref(d), vp.ctypes.data, rho.ctypes.data, ns, sxd.ctypes.data,
+                                                    szd.ctypes.data, wm.ctypes.data, rx.ctypes.data, rz.ctypes.data,
+                                                    outs[0].ctypes.data, outs[1].ctypes.data, outs[2].ctypes.data,
+                                                    C.byref(t)))
+    else:
+        _lib.check(_lib.lib().fomo_elastic2d_batch(C.byref(d), vp.ctypes.data, vs.ctypes.data, rho.ctypes.data, ns,
+                                                   sxd.ctypes.data, szd.ctypes.data, wm.ctypes.data, rx.ctypes.data,
+                                                   rz.ctypes.data, outs[0].ctypes.data, outs[1].ctypes.data,
+                                                   C.byref(t)))
+    return outs, BatchStats(float(t.value), ns)
+
+
+def acoustic2d_batch(vp, rho, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_order=8, v_ref=None, wavelet=None,
+                     scale_source_by_dt=False, verbose=True, use_cuda_graph=True):
+    """acoustic2d_batch (src/equations/acoustic2d/batch.jl:180-261) -> (seis_p, seis_vx, seis_vz, stats).
+
+    Seismograms are (n_rec, nt, n_shots); each shot is bit-identical to its solo acoustic2d run."""
+    del verbose
+    outs, st = _batch(_lib.ACOUSTIC2D, vp, None, rho, dh, dt, nt, f0, sx, sz, rx, rz, nbc, fd_order, v_ref, wavelet,
+                      scale_source_by_dt, use_cuda_graph)
+    return Acoustic2DBatchResult(outs[0], outs[1], outs[2], st)
+
+
+def elastic2d_batch(vp, vs, rho, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_order=8, v_ref=None, wavelet=None,
+                    scale_source_by_dt=False, verbose=True, use_cuda_graph=True):
+    """elastic2d_batch (src/equations/elastic2d/batch.jl:197-261) -> (seis_vx, seis_vz, stats)."""
+    del verbose
+    outs, st = _batch(_lib.ELASTIC2D, vp, vs, rho, dh, dt, nt, f0, sx, sz, rx, rz, nbc, fd_order, v_ref, wavelet,
+                      scale_source_by_dt, use_cuda_graph)
+    return Elastic2DBatchResult(outs[0], outs[1], st)
+
+
 class Plan:
     """Device-resident run (fomo_plan): setup once, then advance / read / reset.
 
-    Used by the benchmark and the tests; `acoustic2d`/`elastic2d` are one-shot wrappers of the
-    same machinery inside the library."""
+    A solo run is a plan of one shot; `Plan.windows` builds a multi-shot plan whose shots are
+    windows of one model (each bit-identical to its solo run on the cropped model)."""
 
-    def __init__(self, eq, vp, rho, dh, dt, nt, *, sx, sz, rx, rz, wavelet, vs=None, nbc=50, fd_order=8,
-                 v_ref=None, source_type=0, use_graph=True, sy=None, ry=None, storage=0, record_p=False):
+    def __init__(self, eq, vp, rho, dh, dt, nt, *, sx=None, sz=None, rx=None, rz=None, wavelet=None, vs=None,
+                 nbc=50, fd_order=8, v_ref=None, source_type=0, use_graph=True, sy=None, ry=None, storage=0,
+                 record_p=False, tile_cfg=0, shots=None):
         vp, rho = _f32F(vp), _f32F(rho)
         self._keep = [vp, rho]
         if vs is not None:
@@ -168,25 +249,57 @@ class Plan:
         shape = vp.shape
         nx, nz = shape[0], shape[-1]
         ny = shape[1] if len(shape) == 3 else 1
-        v_ref = _default_vref(vp) if v_ref is None else float(np.float32(v_ref))
-        sx, sz, rx, rz = _i32(sx), _i32(sz), _i32(rx), _i32(rz)
-        sy = _i32(sy) if sy is not None else None
-        ry = _i32(ry) if ry is not None else None
-        wm = np.asfortranarray(np.asarray(wavelet, np.float32).reshape(len(sx), nt))
-        self.n_rec, self.nt, self.eq = len(rx), nt, eq
-        d = _desc(eq, nx, nz, nbc, fd_order, nt, dh, dt, v_ref, len(sx), len(rx), 0, use_graph,
-                  source_type=source_type, ny=ny, storage=storage)
-        d.record_p = 1 if record_p else 0
-        h = C.c_void_p()
+        self.nt, self.eq = nt, eq
         ptr = lambda a: None if a is None else a.ctypes.data  # noqa: E731
-        _lib.check(_lib.lib().fomo_plan_create(C.byref(d), vp.ctypes.data, ptr(vs), rho.ctypes.data,
-                                               sx.ctypes.data, ptr(sy), sz.ctypes.data, wm.ctypes.data,
-                                               rx.ctypes.data, ptr(ry), rz.ctypes.data, C.byref(h)))
+        h = C.c_void_p()
+        if shots is None:
+            v_ref = _default_vref(vp) if v_ref is None else float(np.float32(v_ref))
+            sx, sz, rx, rz = _i32(sx), _i32(sz), _i32(rx), _i32(rz)
+            sy = _i32(sy) if sy is not None else None
+            ry = _i32(ry) if ry is not None else None
+            wm = np.asfortranarray(np.asarray(wavelet, np.float32).reshape(len(sx), nt))
+            self._keep += [sx, sz, rx, rz, sy, ry, wm]
+            self.n_rec = [len(rx)]
+            d = _desc(eq, nx, nz, nbc, fd_order, nt, dh, dt, v_ref, len(sx), len(rx), 0, use_graph,
+                      source_type=source_type, ny=ny, storage=storage)
+            d.record_p = 1 if record_p else 0
+            d.tile_cfg = tile_cfg
+            _lib.check(_lib.lib().fomo_plan_create(C.byref(d), vp.ctypes.data, ptr(vs), rho.ctypes.data,
+                                                   sx.ctypes.data, ptr(sy), sz.ctypes.data, wm.ctypes.data,
+                                                   rx.ctypes.data, ptr(ry), rz.ctypes.data, C.byref(h)))
+        else:
+            table = (_lib.Shot * len(shots))()
+            self.n_rec = []
+            for k, s in enumerate(shots):
+                ssx, ssz = _i32(s["sx"]), _i32(s["sz"])
+                srx, srz = _i32(s["rx"]), _i32(s["rz"])
+                w = np.asfortranarray(np.asarray(s["wavelet"], np.float32).reshape(len(ssx), nt))
+                self._keep += [ssx, ssz, srx, srz, w]
+                t = table[k]
+                t.x0, t.nx, t.n_src, t.n_rec = int(s["x0"]), int(s["nx"]), len(ssx), len(srx)
+                t.sx, t.sz, t.wavelet = ssx.ctypes.data, ssz.ctypes.data, w.ctypes.data
+                t.rx, t.rz = srx.ctypes.data, srz.ctypes.data
+                t.v_ref = float(s.get("v_ref") or 0.0)
+                self.n_rec.append(len(srx))
+            d = _desc(eq, nx, nz, nbc, fd_order, nt, dh, dt, 0.0, 0, 0, 0, use_graph, source_type=source_type)
+            d.tile_cfg = tile_cfg
+            _lib.check(_lib.lib().fomo_batch_create(C.byref(d), vp.ctypes.data, ptr(vs), rho.ctypes.data,
+                                                    len(shots), table, C.byref(h)))
         self._h = h
-        a, b, c, p = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
-        _lib.check(_lib.lib().fomo_plan_dims(h, C.byref(a), C.byref(b), C.byref(c), C.byref(p)))
-        self.padded = (a.value, c.value) if ny == 1 else (a.value, b.value, c.value)
-        self.pad = p.value
+        n = C.c_int32()
+        _lib.check(_lib.lib().fomo_plan_n_shots(h, C.byref(n)))
+        self.n_shots = n.value
+        self.padded, self.pads = [], []
+        for s in range(self.n_shots):
+            a, b, c, p = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+            _lib.check(_lib.lib().fomo_plan_shot_dims(h, s, C.byref(a), C.byref(b), C.byref(c), C.byref(p)))
+            self.padded.append((a.value, c.value) if ny == 1 else (a.value, b.value, c.value))
+            self.pads.append(p.value)
+        self.pad = self.pads[0]
+
+    @property
+    def padded_cells(self) -> int:
+        return int(sum(np.prod(p) for p in self.padded))
 
     def reset(self):
         _lib.check(_lib.lib().fomo_plan_reset(self._h))
@@ -208,15 +321,15 @@ class Plan:
         _lib.check(_lib.lib().fomo_plan_kernel_times(self._h, int(n), out))
         return out[0], out[1], out[2]
 
-    def seis(self, nslots: int = 3):
-        outs = [np.zeros((self.n_rec, self.nt), np.float32, order="F") for _ in range(nslots)]
+    def seis(self, nslots: int = 3, shot: int = 0):
+        outs = [np.zeros((self.n_rec[shot], self.nt), np.float32, order="F") for _ in range(nslots)]
         ptrs = [o.ctypes.data for o in outs] + [None] * (4 - nslots)
-        _lib.check(_lib.lib().fomo_plan_read_seis(self._h, *ptrs))
+        _lib.check(_lib.lib().fomo_plan_read_shot_seis(self._h, shot, *ptrs))
         return outs
 
-    def field(self, which: int) -> np.ndarray:
-        out = np.zeros(self.padded, np.float32, order="F")
-        _lib.check(_lib.lib().fomo_plan_read_field(self._h, int(which), out.ctypes.data))
+    def field(self, which: int, shot: int = 0) -> np.ndarray:
+        out = np.zeros(self.padded[shot], np.float32, order="F")
+        _lib.check(_lib.lib().fomo_plan_read_shot_field(self._h, shot, int(which), out.ctypes.data))
         return out
 
     def close(self):

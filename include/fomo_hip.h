@@ -79,8 +79,23 @@ typedef struct fomo_desc {
   int32_t snap_index;    /* 3D snapshots: 1-based interior index of the slice */
   int32_t use_graph;     /* 1: one hipGraph launch per pair of steps; 0: direct launches */
   int32_t record_p;      /* acoustic3d: also gather p (extension) */
-  int32_t reserved[7];
+  int32_t tile_cfg;      /* 0: autotune the tile shape; k > 0: fixed shape k-1 */
+  int32_t reserved[6];
 } fomo_desc;
+
+/* One shot of a batch: a window of the model (full depth) with its own sources and receivers.
+ * The shot is simulated exactly as a solo run on the cropped model would be: padding, medium,
+ * HABC coefficients (from this window's v_ref) and arithmetic are per window. */
+typedef struct fomo_shot {
+  int32_t x0;              /* first interior column of the window in the model (1-based) */
+  int32_t nx;              /* window width in interior columns */
+  int32_t n_src;
+  const int32_t *sx, *sz;  /* 1-based indices within the window */
+  const float *wavelet;    /* n_src x nt, column-major */
+  int32_t n_rec;
+  const int32_t *rx, *rz;  /* 1-based indices within the window */
+  float v_ref;             /* HABC reference velocity; <= 0 selects min(vp > 0) over the window */
+} fomo_shot;
 
 /* ---- one-shot drop-in entry points -------------------------------------------------------- */
 /*
@@ -102,6 +117,22 @@ int fomo_elastic2d(const fomo_desc *d, const float *vp, const float *vs, const f
                    const int32_t *sz, const float *wavelet, const int32_t *rx, const int32_t *rz, float *seis_vx,
                    float *seis_vz, float *snaps, double *kernel_time_s);
 
+/*
+ * acoustic2d_batch -- replaces Fomo.acoustic2d_batch (src/equations/acoustic2d/batch.jl:180-261):
+ * n_shots shots over the shared model and receiver spread; d->n_src sources per shot.
+ * sx, sz: n_src x n_shots (column-major, as init_batched_source stores them, batch_acq.jl:27-37);
+ * wavelet: n_src x nt shared by all shots; outputs n_rec x nt x n_shots.  Each shot is
+ * bit-identical to a solo fomo_acoustic2d run.
+ */
+int fomo_acoustic2d_batch(const fomo_desc *d, const float *vp, const float *rho, int32_t n_shots, const int32_t *sx,
+                          const int32_t *sz, const float *wavelet, const int32_t *rx, const int32_t *rz,
+                          float *seis_p, float *seis_vx, float *seis_vz, double *kernel_time_s);
+
+/* elastic2d_batch -- replaces Fomo.elastic2d_batch (src/equations/elastic2d/batch.jl:197-261). */
+int fomo_elastic2d_batch(const fomo_desc *d, const float *vp, const float *vs, const float *rho, int32_t n_shots,
+                         const int32_t *sx, const int32_t *sz, const float *wavelet, const int32_t *rx,
+                         const int32_t *rz, float *seis_vx, float *seis_vz, double *kernel_time_s);
+
 /* ---- device-resident plans (bench, batch engine, tests) ----------------------------------- */
 typedef struct fomo_plan fomo_plan;
 
@@ -111,6 +142,19 @@ typedef struct fomo_plan fomo_plan;
 int fomo_plan_create(const fomo_desc *d, const float *vp, const float *vs, const float *rho, const int32_t *sx,
                      const int32_t *sy, const int32_t *sz, const float *wavelet, const int32_t *rx,
                      const int32_t *ry, const int32_t *rz, fomo_plan **out);
+/* Build a multi-shot plan: every shot is a window of the model (fomo_shot).  All shots advance
+ * together, one launch per phase over all tiles of all shots.  An extension: the reference's
+ * batch shares one grid between shots (fomo_acoustic2d_batch covers that form). */
+int fomo_batch_create(const fomo_desc *d, const float *vp, const float *vs, const float *rho, int32_t n_shots,
+                      const fomo_shot *shots, fomo_plan **out);
+/* Number of shots of a plan. */
+int fomo_plan_n_shots(const fomo_plan *p, int32_t *out);
+/* Seismograms of one shot (n_rec_shot x nt each; NULL skips). */
+int fomo_plan_read_shot_seis(fomo_plan *p, int32_t shot, float *s0, float *s1, float *s2, float *s3);
+/* Current padded field `which` of one shot. */
+int fomo_plan_read_shot_field(fomo_plan *p, int32_t shot, int32_t which, float *out);
+/* Padded dims and pad width of one shot. */
+int fomo_plan_shot_dims(const fomo_plan *p, int32_t shot, int32_t *nx, int32_t *ny, int32_t *nz, int32_t *pad);
 /* Zero every field and seismogram, reset the step counter to 0.  Asynchronous. */
 int fomo_plan_reset(fomo_plan *p);
 /* Advance n steps (enqueue only; asynchronous on the plan's stream). */
