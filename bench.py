@@ -67,6 +67,38 @@ def cpu_baseline(vp, vs, rho, rx, rz, budget_s):
                       f"{n} steps timed (minus a 4-step run for setup), {n * per_step:.1f} s of CPU work"}
 
 
+def probe(args):
+    """HBM-bound 2D probe (not the headline line): homogeneous 8192 x 8192 interior, fd8, nbc 50,
+    acoustic2d and elastic2d; prints per-kernel times and algorithmic GB/s."""
+    import numpy as np
+
+    import fomo
+    import fomo._lib as L
+    from fomo.api import Plan
+    n = args.probe_n
+    vp = np.full((n, n), 3000.0, np.float32)
+    vs = np.full((n, n), 1700.0, np.float32)
+    rho = np.full((n, n), 2200.0, np.float32)
+    nt = args.steps
+    w = fomo.ricker_wavelet(15.0, 1e-3, nt)
+    res = {}
+    for eq, name, bv, bs in ((L.ACOUSTIC2D, "acoustic2d", 28, 20), (L.ELASTIC2D, "elastic2d", BYTES_VEL, BYTES_STR)):
+        plan = Plan(eq, vp, rho, 10.0, 1e-3, nt, vs=vs if eq == L.ELASTIC2D else None, sx=[n // 2], sz=[n // 2],
+                    rx=[1, n], rz=[1, n], wavelet=w[None], nbc=50, fd_order=8)
+        cells = plan.padded_cells
+        plan.advance(10)
+        plan.sync()
+        ms = plan.timed_advance(nt)
+        tv, ts, tr = plan.kernel_times(max(10, nt // 4))
+        plan.close()
+        res[name] = {"padded_cells": cells, "ms_per_step": ms / nt, "gcell_per_s": cells * nt / ms / 1e6,
+                     "step_gbs": (bv + bs) * cells * nt / ms / 1e6,
+                     "vel_ms": tv, "vel_gbs": bv * cells / tv / 1e6, "prs_ms": ts, "prs_gbs": bs * cells / ts / 1e6,
+                     "gather_ms": tr}
+        log(f"{name}: {res[name]}")
+    print(json.dumps({"probe": f"{n}x{n} interior, fd8, nbc 50", **res}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,11 +107,17 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=200, help="steps for per-kernel event timing")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe", action="store_true", help="HBM-bound 2D probe instead of the headline line")
+    ap.add_argument("--probe-n", type=int, default=8192)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (shots and slabs would shard "
               "across GPUs; see DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+
+    if args.probe:
+        probe(args)
+        return
 
     import numpy as np
 
@@ -96,7 +134,8 @@ def main():
     log("creating the plan (setup, upload, graph capture)")
     plan = Plan(L.ELASTIC2D, vp, rho, DH, DT, NT, vs=vs, sx=[1000], sz=[5], rx=rx, rz=rz, wavelet=w[None],
                 nbc=NBC, fd_order=ORDER, source_type=L.SRC_FZ, use_graph=True)
-    n_pad = plan.padded[0] * plan.padded[1]
+    n_pad = plan.padded_cells
+    pnx, pnz = plan.padded[0]
     log(f"warmup {args.warmup} steps")
     plan.advance(args.warmup)
     plan.sync()
@@ -131,7 +170,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"elastic2d {NX}x{NZ} interior, padded {plan.padded[0]}x{plan.padded[1]}, fd{ORDER}, "
+            "workload": f"elastic2d {NX}x{NZ} interior, padded {pnx}x{pnz}, fd{ORDER}, "
                         f"nbc {NBC}, dh {DH} m, dt {DT * 1e3} ms, vertical-force Ricker {F0} Hz, {NX} receivers",
             "padded_cells": n_pad,
             "interior_cell_updates_per_s": NX * NZ * args.steps / (ms * 1e-3),

@@ -1,15 +1,17 @@
 // engine2d.hip -- 2D staggered-grid sweeps for gfx950 with the deterministic Higdon boundary fused in.
 //
-// One launch per phase (velocity; pressure or stress) over every tile of every shot.  A tile is
-// 64 (x, one wave64 row) by 4*ZPT (z) cells; each thread owns one column of ZPT cells.  The
-// fields the phase reads through FD stencils are staged into LDS with an (N+1)-cell halo; the
-// phase's own fields (time n, ping-pong buffer) and the materials are read once, coalesced.
+// One launch per phase (velocity; pressure or stress) over a work list spanning every shot.  The
+// user's interior of a shot (the padded grid minus the pad, which is exactly the absorbing frame
+// plus the outer ring) is cut into strips 64 columns wide (one wave64 row) that march down z; the
+// frame is cut into 64 x 8 tiles.  The fields a phase reads through FD stencils are staged in
+// LDS; the phase's own fields (time n, ping-pong buffer) and the materials are read once,
+// coalesced.
 //
 // Deterministic HABC without a second pass: a frame cell's Higdon value needs the inward
-// neighbour's *pre-correction* value.  Tiles that touch the frame first compute pre-correction
-// values on the tile plus a one-cell ring into LDS (the halo of N+1 makes the ring's stencils
-// available on-chip), then blend.  Every output is a pure function of read-only buffers, so
-// results do not depend on launch geometry, tile shape or the other shots of a batch.
+// neighbour's *pre-correction* value.  Frame tiles first compute pre-correction values on the
+// tile plus a one-cell ring into LDS (an (N+1)-cell halo makes the ring's stencils available
+// on-chip), then blend.  Every output is a pure function of read-only buffers, so results do not
+// depend on launch geometry, strip length or the other shots of a batch.
 //
 // Arithmetic order follows the reference expressions term by term; the library is compiled
 // with -ffp-contract=off, so FP32 results are bit-identical to the reference's uncontracted
@@ -209,31 +211,34 @@ __device__ __forceinline__ void inject(const Shot2D& S, int mask, int c, int it,
   }
 }
 
-template <int PH, int N, int ZPT>
-__global__ __launch_bounds__(256) void sweep2d_kernel(const SweepArgs A) {
+// Stencil data in a circular window of 16 rows (N <= 6): row j lives in slot (j - jb) & 15.
+template <int RW>
+struct WinRd {
+  const float* t;
+  int oi, jb;
+  __device__ __forceinline__ float operator()(int k, int i, int j) const {
+    return t[k * 16 * RW + ((j - jb) & 15) * RW + (i - oi)];
+  }
+};
+
+// Frame tile: at most 64 x kFrameRows cells of the absorbing frame.  Stencil inputs are staged
+// with an (N+1) halo; pre-correction values of the tile plus a one-cell ring go to LDS; each
+// frame cell then blends with its inward neighbour's pre-correction value.
+template <int PH, int N>
+__device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, const Shot2D& S, const PV& V,
+                                           float* lds) {
   using P = Phys<PH, N>;
   constexpr int NF = P::NF, NS = P::NS;
-  constexpr int BX = kTileW, BY = 4, BZ = BY * ZPT;
+  constexpr int BX = kTileW, BZ = kFrameRows, ZPT = BZ / 4;
   constexpr int TW = BX + 2 * N + 2, TH = BZ + 2 * N + 2;
   constexpr int EW = BX + 2, EH = BZ + 2;
-  __shared__ float tile[NS * TW * TH];
-  __shared__ float pre_s[NF * EW * EH];
-
-  // XCD-aware order: hardware hands block b to XCD b % 8, so each XCD gets a contiguous run of
-  // tiles (gridDim.x is a multiple of 8) and shares halo rows through its own L2.
-  const int per = gridDim.x >> 3;
-  const int w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  if (w >= A.n_work) return;
-  const int2 wk = A.work[w];
-  const Shot2D& S = A.shots[wk.x];
+  float* tile = lds;
+  float* pre_s = lds + NS * TW * TH;
   const int nx = S.nx, nz = S.nz, H = S.H;
-  const int tj = wk.y / S.tiles_x;
-  const int i0 = (wk.y - tj * S.tiles_x) * BX, j0 = tj * BZ;
-  PV V;
-  make_view<PH>(S, A.parity, V);
+  const int i0 = W.i0, j0 = W.j0;
   const int tid = threadIdx.x + threadIdx.y * BX;
   const int oi = i0 - N - 1, oj = j0 - N - 1;
-  for (int idx = tid; idx < TW * TH; idx += BX * BY) {
+  for (int idx = tid; idx < TW * TH; idx += BX * 4) {
     const int r = idx / TW, cc = idx - r * TW;
     const int gi = oi + cc, gj = oj + r;
     const bool in = gi >= 0 && gi < nx && gj >= 0 && gj < nz;
@@ -243,41 +248,27 @@ __global__ __launch_bounds__(256) void sweep2d_kernel(const SweepArgs A) {
   }
   __syncthreads();
   const TileRd<TW, TH> rd{tile, oi, oj};
-  const int it = *A.step;
-  const int mask = inj_mask<PH>(S);
-  const int i = i0 + (int)threadIdx.x;
-  const bool interior = i0 > H && i0 + BX - 1 < nx - 1 - H && j0 > H && j0 + BZ - 1 < nz - 1 - H;
-  if (interior) {
-#pragma unroll
-    for (int k = 0; k < ZPT; ++k) {
-      const int j = j0 + (int)threadIdx.y * ZPT + k;
-      float v[NF];
-      P::pre_all(rd, V, S, A.a, i, j, v);
-      const size_t c = (size_t)i + (size_t)j * (size_t)nx;
-      inject<NF>(S, mask, (int)c, it, A.nt, v);
-#pragma unroll
-      for (int f = 0; f < NF; ++f) V.out[f][c] = v[f];
-    }
-    return;
-  }
-  // Frame tile: pre-correction values on the tile and a one-cell ring.
-  for (int idx = tid; idx < EW * EH; idx += BX * BY) {
+  for (int idx = tid; idx < EW * EH; idx += BX * 4) {
     const int r = idx / EW, cc = idx - r * EW;
     const int gi = i0 - 1 + cc, gj = j0 - 1 + r;
-    if (gi < 0 || gi >= nx || gj < 0 || gj >= nz) continue;
+    if (gi < 0 || gi >= nx || gj < 0 || gj >= nz || cc > W.w + 1 || r > W.h + 1) continue;
     float v[NF];
     P::pre_all(rd, V, S, A.a, gi, gj, v);
 #pragma unroll
     for (int f = 0; f < NF; ++f) pre_s[f * EW * EH + idx] = v[f];
   }
   __syncthreads();
-  if (i >= nx) return;
+  if ((int)threadIdx.x >= W.w) return;
+  const int i = i0 + (int)threadIdx.x;
+  const int it = *A.step;
+  const int mask = inj_mask<PH>(S);
   const float q = S.q, qt = S.qt, qxt = S.qxt;
 #pragma unroll
   for (int k = 0; k < ZPT; ++k) {
-    const int j = j0 + (int)threadIdx.y * ZPT + k;
-    if (j >= nz) break;
-    const int li = (int)threadIdx.x + 1, lj = (int)threadIdx.y * ZPT + k + 1;
+    const int jl = (int)threadIdx.y * ZPT + k;
+    if (jl >= W.h) break;
+    const int j = j0 + jl;
+    const int li = (int)threadIdx.x + 1, lj = jl + 1;
     const size_t c = (size_t)i + (size_t)j * (size_t)nx;
     float v[NF];
 #pragma unroll
@@ -316,6 +307,96 @@ __global__ __launch_bounds__(256) void sweep2d_kernel(const SweepArgs A) {
   }
 }
 
+// Interior strip: at most 64 columns by W.h rows of the user's interior (no frame cell, every
+// cell in the update region).  The block walks down z four rows at a time (one row per wave)
+// through a 16-row circular LDS window of the stencil inputs; the rows of the next step are
+// loaded into registers while the current rows are computed.
+template <int PH, int N>
+__device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D& W, const Shot2D& S, const PV& V,
+                                               float* lds) {
+  using P = Phys<PH, N>;
+  constexpr int NF = P::NF, NS = P::NS;
+  constexpr int BX = kTileW, RW = BX + 2 * N;
+  static_assert(4 + 2 * N <= 16, "window of 16 rows");
+  const int nx = S.nx, nz = S.nz;
+  const int i0 = W.i0, j0 = W.j0, jend = W.j0 + W.h;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int oi = i0 - N, jb = j0 - N;
+  // columns loaded by this lane: oi + tx and, for tx < 2N, oi + 64 + tx
+  const int ca = oi + tx, cb = oi + BX + tx;
+  const bool has_b = tx < 2 * N;
+  const bool ina = ca >= 0 && ca < nx, inb = has_b && cb >= 0 && cb < nx;
+  auto load_row = [&](int j, float (&ra)[NS], float (&rb)[NS]) {
+    const bool inj = j >= 0 && j < nz;
+    const size_t rowo = (size_t)(inj ? j : 0) * (size_t)nx;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      ra[k] = (inj && ina) ? V.sten[k][rowo + ca] : 0.0f;
+      rb[k] = (inj && inb) ? V.sten[k][rowo + cb] : 0.0f;
+    }
+  };
+  auto store_row = [&](int j, const float (&ra)[NS], const float (&rb)[NS]) {
+    const int slot = (j - jb) & 15;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      lds[k * 16 * RW + slot * RW + tx] = ra[k];
+      if (has_b) lds[k * 16 * RW + slot * RW + BX + tx] = rb[k];
+    }
+  };
+  // initial window: rows j0-N .. j0+3+N
+  for (int r = ty; r < 4 + 2 * N; r += 4) {
+    float ra[NS], rb[NS];
+    load_row(jb + r, ra, rb);
+    store_row(jb + r, ra, rb);
+  }
+  __syncthreads();
+  const WinRd<RW> rd{lds, oi, jb};
+  const int it = *A.step;
+  const int mask = inj_mask<PH>(S);
+  const int i = i0 + tx;
+  const bool col_ok = tx < W.w;
+  for (int jr = j0; jr < jend; jr += 4) {
+    const bool more = jr + 4 < jend;
+    const int jn = jr + 4 + N + ty;
+    float na[NS], nb[NS];
+    if (more) load_row(jn, na, nb);
+    const int j = jr + ty;
+    if (col_ok && j < jend) {
+      float v[NF];
+      P::pre_all(rd, V, S, A.a, i, j, v);
+      const size_t c = (size_t)i + (size_t)j * (size_t)nx;
+      inject<NF>(S, mask, (int)c, it, A.nt, v);
+#pragma unroll
+      for (int f = 0; f < NF; ++f) V.out[f][c] = v[f];
+    }
+    __syncthreads();
+    if (more) store_row(jn, na, nb);
+    __syncthreads();
+  }
+}
+
+// The host lays the work list out per XCD: hardware hands block b to XCD b % 8, and the list
+// gives each XCD a contiguous, cost-balanced run of the grid so that halo rows stay in its L2.
+// Padding entries have kind -1.
+template <int PH, int N>
+__global__ __launch_bounds__(256) void sweep2d_kernel(const SweepArgs A) {
+  using P = Phys<PH, N>;
+  constexpr int NF = P::NF, NS = P::NS;
+  constexpr int FT = NS * (kTileW + 2 * N + 2) * (kFrameRows + 2 * N + 2) + NF * (kTileW + 2) * (kFrameRows + 2);
+  constexpr int IS = NS * 16 * (kTileW + 2 * N);
+  __shared__ float lds[FT > IS ? FT : IS];
+  if ((int)blockIdx.x >= A.n_work) return;
+  const Work2D W = A.work[blockIdx.x];
+  if (W.kind < 0) return;
+  const Shot2D& S = A.shots[W.shot];
+  PV V;
+  make_view<PH>(S, A.parity, V);
+  if (W.kind == 0)
+    frame_tile<PH, N>(A, W, S, V, lds);
+  else
+    interior_strip<PH, N>(A, W, S, V, lds);
+}
+
 __global__ __launch_bounds__(256) void gather2d_kernel(const GatherArgs A) {
   const int it = *A.step;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -336,18 +417,12 @@ __global__ __launch_bounds__(256) void gather2d_kernel(const GatherArgs A) {
   }
 }
 
-int tile_rows(int tile_cfg) { return 4 * (tile_cfg == 0 ? 2 : (tile_cfg == 1 ? 4 : 8)); }
+int strip_rows(int tile_cfg) { return tile_cfg == 0 ? 16 : (tile_cfg == 1 ? 32 : 64); }
 
 template <int PH, int N>
 static hipError_t launch_cfg(int tile_cfg, const SweepArgs& a, hipStream_t s) {
-  const int blocks = (a.n_work + 7) / 8 * 8;
-  const dim3 blk(kTileW, 4);
-  switch (tile_cfg) {
-    case 0: hipLaunchKernelGGL((sweep2d_kernel<PH, N, 2>), dim3(blocks), blk, 0, s, a); break;
-    case 1: hipLaunchKernelGGL((sweep2d_kernel<PH, N, 4>), dim3(blocks), blk, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((sweep2d_kernel<PH, N, 8>), dim3(blocks), blk, 0, s, a); break;
-    default: return hipErrorInvalidValue;
-  }
+  (void)tile_cfg;  // the strip length lives in the work list
+  hipLaunchKernelGGL((sweep2d_kernel<PH, N>), dim3(a.n_work), dim3(kTileW, 4), 0, s, a);
   return hipGetLastError();
 }
 

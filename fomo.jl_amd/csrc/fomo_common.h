@@ -30,17 +30,23 @@ struct Shot2D {
   float q, qt, qxt;     // Higdon coefficients
   int nx, nz;           // padded dims
   int H;                // HABC frame width (pad - 1)
-  int tiles_x;          // tiles along x for the launched tile shape
   int n_src;
   const int* src_c;     // padded cell index of each source
   const float* wav;     // n_src x nt
   int src_fz;           // elastic: 1 = vertical force into vz, 0 = explosive into txx, tzz
 };
 
+// One unit of sweep work.  kind 0: a frame tile (at most kTileW x kFrameRows cells of the
+// absorbing frame, blended on-chip); kind 1: an interior strip (at most kTileW columns of the
+// user's interior, marched down z).
+struct Work2D {
+  int shot, kind, i0, j0, w, h;
+};
+
 // Arguments of one sweep launch over all shots.
 struct SweepArgs {
   const Shot2D* shots;
-  const int2* work;     // (shot, tile) pairs
+  const Work2D* work;
   int n_work;
   int parity;           // state parity s: read buffers [s], write buffers [1-s]
   float a[kMaxN];       // staggered FD coefficients
@@ -65,11 +71,12 @@ struct GatherArgs {
   unsigned* done;
 };
 
-// Launchers (defined in engine2d.hip).  tile_cfg selects rows per thread (see kTileCfgs).
+// Launchers (defined in engine2d.hip).  tile_cfg selects the interior strip length.
 hipError_t launch_sweep2d(int phase, int N, int tile_cfg, const SweepArgs& a, hipStream_t s);
 hipError_t launch_gather2d(const GatherArgs& a, hipStream_t s);
-int tile_rows(int tile_cfg);  // rows (z) covered by one tile
-constexpr int kTileW = 64;    // tile width (x): one wave64 row
+int strip_rows(int tile_cfg);   // rows (z) of one interior strip
+constexpr int kTileW = 64;      // tile / strip width (x): one wave64 row
+constexpr int kFrameRows = 8;   // rows of a frame tile
 constexpr int kNumTileCfgs = 3;
 
 }  // namespace fomo

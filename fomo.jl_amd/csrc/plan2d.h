@@ -18,7 +18,7 @@ struct Plan2D : PlanBase {
   std::vector<Shot2D> shots;
   std::vector<ShotMeta> meta;
   Shot2D* d_shots = nullptr;
-  int2* d_work[kNumTileCfgs] = {nullptr};
+  Work2D* d_work[kNumTileCfgs] = {nullptr};
   int n_work[kNumTileCfgs] = {0};
   int n_entries = 0;
   int* rec_shot = nullptr;

@@ -16,16 +16,55 @@ namespace fomo {
 
 Plan2D::~Plan2D() = default;
 
+// Work list of one launch: frame tiles (64 x kFrameRows) over the four pad bands and interior
+// strips (64 x strip_rows) over the user's interior, for every shot, sorted by (shot, row,
+// column).  The list is split into 8 contiguous runs of about equal cost, one per XCD, and
+// interleaved so that block b (which the hardware places on XCD b % 8) takes the next item of
+// run b % 8.
 int Plan2D::make_work(int cfg) {
   if (d_work[cfg]) return FOMO_OK;
-  std::vector<int2> w;
-  const int bz = tile_rows(cfg);
+  std::vector<Work2D> items;
+  const int zs = strip_rows(cfg);
+  auto add_rect = [&](int s, int kind, int x0, int x1, int z0, int z1, int th) {
+    for (int j = z0; j < z1; j += th)
+      for (int i = x0; i < x1; i += kTileW)
+        items.push_back(Work2D{s, kind, i, j, std::min(kTileW, x1 - i), std::min(th, z1 - j)});
+  };
   for (size_t s = 0; s < shots.size(); ++s) {
-    const int tx = (shots[s].nx + kTileW - 1) / kTileW, tz = (shots[s].nz + bz - 1) / bz;
-    for (int t = 0; t < tx * tz; ++t) w.push_back(make_int2((int)s, t));
+    const int nx = shots[s].nx, nz = shots[s].nz, pad = meta[s].pad;
+    const int xe = pad + meta[s].nxi, ze = pad + meta[s].nzi;
+    add_rect((int)s, 0, 0, nx, 0, pad, kFrameRows);       // top band
+    add_rect((int)s, 0, 0, pad, pad, ze, kFrameRows);     // left band
+    add_rect((int)s, 1, pad, xe, pad, ze, zs);            // interior
+    add_rect((int)s, 0, xe, nx, pad, ze, kFrameRows);     // right band
+    add_rect((int)s, 0, 0, nx, ze, nz, kFrameRows);       // bottom band
   }
-  n_work[cfg] = (int)w.size();
-  return upload(&d_work[cfg], w.data(), w.size());
+  std::stable_sort(items.begin(), items.end(), [](const Work2D& a, const Work2D& b) {
+    if (a.shot != b.shot) return a.shot < b.shot;
+    if (a.j0 != b.j0) return a.j0 < b.j0;
+    return a.i0 < b.i0;
+  });
+  // frame cells cost about twice an interior cell (ring recompute, blend)
+  std::vector<double> cost(items.size());
+  double total = 0.0;
+  for (size_t k = 0; k < items.size(); ++k) {
+    cost[k] = (double)items[k].w * items[k].h * (items[k].kind == 0 ? 2.0 : 1.0);
+    total += cost[k];
+  }
+  std::vector<std::vector<Work2D>> runs(8);
+  double acc = 0.0;
+  for (size_t k = 0; k < items.size(); ++k) {
+    const int r = std::min(7, (int)(8.0 * (acc + 0.5 * cost[k]) / std::max(total, 1.0)));
+    runs[r].push_back(items[k]);
+    acc += cost[k];
+  }
+  size_t len = 0;
+  for (auto& r : runs) len = std::max(len, r.size());
+  std::vector<Work2D> flat(8 * len, Work2D{0, -1, 0, 0, 0, 0});
+  for (int r = 0; r < 8; ++r)
+    for (size_t k = 0; k < runs[r].size(); ++k) flat[k * 8 + r] = runs[r][k];
+  n_work[cfg] = (int)flat.size();
+  return upload(&d_work[cfg], flat.data(), flat.size());
 }
 
 hipError_t Plan2D::enqueue_step(int s, hipEvent_t* ev) {
@@ -217,7 +256,6 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
     x.nz = d->nz + 2 * pad;
     x.H = H;
     x.dtx = dtx;
-    x.tiles_x = (x.nx + kTileW - 1) / kTileW;
     const size_t n = (size_t)x.nx * x.nz;
     float v_ref = S.v_ref;
     if (!(v_ref > 0.0f)) {

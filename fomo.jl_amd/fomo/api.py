@@ -192,9 +192,9 @@ def _batch(eq, vp, vs, rho, dh, dt, nt, f0, sx, sz, rx, rz, nbc, fd_order, v_ref
     wm = wavelet_matrix(wavelet, f0, dt, nt, n_src, scale_source_by_dt)
     rx, rz = _i32(rx), _i32(rz)
     n_rec = len(rx)
-    # device layout (n_src, n_shots), batch_acq.jl:27-37
-    sxd = np.ascontiguousarray(sxm.T.astype(np.int32))
-    szd = np.ascontiguousarray(szm.T.astype(np.int32))
+    # device layout (n_src, n_shots) column-major (batch_acq.jl:27-37) = (n_shots, n_src) row-major
+    sxd = np.ascontiguousarray(sxm.astype(np.int32))
+    szd = np.ascontiguousarray(szm.astype(np.int32))
     d = _desc(eq, nx, nz, nbc, fd_order, nt, dh, dt, v_ref, n_src, n_rec, 0, use_cuda_graph)
     nslots = 3 if eq == _lib.ACOUSTIC2D else 2
     outs = [np.zeros((n_rec, nt, ns), np.float32, order="F") for _ in range(nslots)]

@@ -470,8 +470,8 @@ int orc_acoustic2d(int nxi, int nzi, const float *vp, const float *rho, float dh
 /*
  * elastic2d, boundary = :habc, det two-pass path (elastic2d/loop_graph.jl:30-41 step order).
  *   source_type 0: explosive -- wavelet added to txx and tzz (elastic2d/fused_kernels.jl:147-159);
- *   source_type 1: vertical force -- wavelet added to vz at the same point of the step (an
- *                  extension the reference does not have).
+ *   source_type 1: vertical force -- wavelet added to vz at the end of the velocity phase,
+ *                  after its HABC (an extension the reference does not have).
  *   outputs seis_vx, seis_vz (n_rec x nt); snaps of vz; final_state 5*nx*nz (vx,vz,txx,tzz,txz).
  */
 int orc_elastic2d(int nxi, int nzi, const float *vp, const float *vs, const float *rho, float dh, float dt, int nt,
@@ -508,6 +508,12 @@ int orc_elastic2d(int nxi, int nzi, const float *vp, const float *vs, const floa
     strip_backup(2, fv, fvo, nx, nz, g.H);
     elastic_velocity(vx, vz, txx, tzz, txz, bx, bz, g.a, g.N, g.dtx, g.dtx, nx, nz);
     habc_det(2, fv, fvo, wv, scr, g.q, g.qt, g.qxt, nx, nz, g.H);
+    /* vertical force (extension): raw add into vz at the end of the velocity phase */
+    if (source_type == 1)
+      for (int s = 0; s < n_src; ++s) {
+        size_t c = (size_t)(sx[s] + g.pad - 1) + (size_t)(sz[s] + g.pad - 1) * nx;
+        vz[c] += (real)wavelet[(size_t)s + (size_t)n_src * (it - 1)];
+      }
     real *fs[3] = {txx, tzz, txz}, *fso[3] = {txxo, tzzo, txzo};
     const real *ws[3] = {wtau, wtau, wtau};
     strip_backup(3, fs, fso, nx, nz, g.H);
@@ -516,9 +522,7 @@ int orc_elastic2d(int nxi, int nzi, const float *vp, const float *vs, const floa
     for (int s = 0; s < n_src; ++s) {
       size_t c = (size_t)(sx[s] + g.pad - 1) + (size_t)(sz[s] + g.pad - 1) * nx;
       real wav = (real)wavelet[(size_t)s + (size_t)n_src * (it - 1)];
-      if (source_type == 1) {
-        vz[c] += wav;
-      } else {
+      if (source_type != 1) {
         txx[c] += wav;
         tzz[c] += wav;
       }
