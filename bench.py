@@ -99,6 +99,40 @@ def probe(args):
     print(json.dumps({"probe": f"{n}x{n} interior, fd8, nbc 50", **res}), flush=True)
 
 
+def probe3d(args):
+    """acoustic3d at the BASELINE 3D size (512^3 interior, nbc 32, fd8; 584^3 padded) for each
+    storage precision: per-kernel times and algorithmic GB/s (64 B/cell FP32, 32 B/cell 16-bit)."""
+    import numpy as np
+
+    import fomo
+    import fomo._lib as L
+    from fomo.api import Plan
+    from fomo.models import layered_3d
+    n = args.probe_n3
+    log(f"building the seeded {n}^3 layered model")
+    vp, rho = layered_3d(n, n, n, seed=SEED)
+    nt = args.steps
+    w = fomo.ricker_wavelet(12.0, 8e-4, nt)
+    rx = np.arange(1, n + 1, 2)
+    gx, gy = np.meshgrid(rx, rx, indexing="ij")
+    res = {}
+    for storage, name in ((L.STORE_F32, "f32"), (L.STORE_F16, "f16"), (L.STORE_BF16, "bf16")):
+        eb = 4 if storage == L.STORE_F32 else 2
+        plan = Plan(L.ACOUSTIC3D, vp, rho, 10.0, 8e-4, nt, sx=[n // 2], sy=[n // 2], sz=[1], rx=gx.ravel(),
+                    ry=gy.ravel(), rz=np.ones(gx.size, int), wavelet=w[None], nbc=32, fd_order=8, storage=storage)
+        cells = plan.padded_cells
+        plan.advance(4)
+        plan.sync()
+        ms = plan.timed_advance(nt)
+        tv, tp, tr = plan.kernel_times(max(4, nt // 4))
+        plan.close()
+        res[name] = {"padded_cells": cells, "ms_per_step": ms / nt, "gcell_per_s": cells * nt / ms / 1e6,
+                     "step_gbs": 16 * eb * cells * nt / ms / 1e6, "vel_ms": tv, "vel_gbs": 10 * eb * cells / tv / 1e6,
+                     "prs_ms": tp, "prs_gbs": 6 * eb * cells / tp / 1e6, "gather_ms": tr}
+        log(f"{name}: {res[name]}")
+    print(json.dumps({"probe3d": f"{n}^3 interior, fd8, nbc 32", **res}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +143,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe", action="store_true", help="HBM-bound 2D probe instead of the headline line")
     ap.add_argument("--probe-n", type=int, default=8192)
+    ap.add_argument("--probe3d", action="store_true", help="acoustic3d 512^3 probe (f32/f16/bf16)")
+    ap.add_argument("--probe-n3", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (shots and slabs would shard "
@@ -117,6 +153,9 @@ def main():
 
     if args.probe:
         probe(args)
+        return
+    if args.probe3d:
+        probe3d(args)
         return
 
     import numpy as np

@@ -561,8 +561,3 @@ extern "C" int fomo_elastic2d_batch(const fomo_desc* d, const float* vp, const f
   return run_batch(d, vp, vs, rho, n_shots, sx, sz, wavelet, rx, rz, seis_vx, seis_vz, nullptr, kernel_time_s);
 }
 
-// 3D plans are not part of this build yet.
-int fomo_plan_create3d(const fomo_desc*, const float*, const float*, const float*, const int32_t*, const int32_t*,
-                       const int32_t*, const float*, const int32_t*, const int32_t*, const int32_t*, fomo_plan**) {
-  return fail(FOMO_EUNSUP, "3D solvers are not provided by this build");
-}

@@ -1,0 +1,465 @@
+// plan3d.hip -- 3D acoustic plan: setup (padding, medium, HABC tables, storage scales), work
+// list, step enqueue, snapshots and the acoustic3d one-shot entry point.
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <thread>
+#include <type_traits>
+#include <vector>
+
+#include "engine3d.h"
+#include "plan.h"
+
+namespace fomo {
+
+static void parallel_for(long long n, const std::function<void(long long, long long)>& fn) {
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < (1 << 20) || hw == 1) {
+    fn(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  const long long chunk = (n + hw - 1) / hw;
+  for (unsigned t = 0; t < hw; ++t) {
+    const long long a = t * chunk, b = std::min(n, a + chunk);
+    if (a < b) th.emplace_back(fn, a, b);
+  }
+  for (auto& x : th) x.join();
+}
+
+struct Plan3D : PlanBase {
+  int N = 0, storage = 0, nxi = 0, nyi = 0, nzi = 0, pad = 0, elem = 4;
+  float coef[kMaxN] = {0};
+  Shot3D shot{};
+  Shot3D* d_shot = nullptr;
+  Work3D* d_work = nullptr;
+  int n_work = 0;
+  long long* rec_c = nullptr;
+  float* seis[4] = {nullptr};
+  int nslots = 3;
+  float* scratch = nullptr;  // slice / decode staging
+  size_t scratch_n = 0;
+
+  hipError_t enqueue_step(int s, hipEvent_t* ev) override {
+    Sweep3DArgs a{};
+    a.shot = d_shot;
+    a.work = d_work;
+    a.n_work = n_work;
+    a.parity = s;
+    for (int l = 0; l < N; ++l) a.a[l] = coef[l];
+    a.step = step;
+    a.nt = d.nt;
+    Gather3DArgs g{};
+    g.shot = d_shot;
+    g.rec_c = rec_c;
+    const int fields[4] = {1, 2, 3, 0};
+    for (int k = 0; k < nslots; ++k) {
+      g.seis[k] = seis[k];
+      g.slot_field[k] = fields[k];
+    }
+    g.nslots = nslots;
+    g.n_rec = d.n_rec;
+    g.parity_out = 1 - s;
+    g.nt = d.nt;
+    g.step = step;
+    g.done = done;
+    hipError_t e = hipSuccess;
+    if (ev) e = hipEventRecord(ev[0], stream);
+    if (e == hipSuccess) e = launch_sweep3d(storage, 0, N, a, stream);
+    if (e == hipSuccess && ev) e = hipEventRecord(ev[1], stream);
+    if (e == hipSuccess) e = launch_sweep3d(storage, 1, N, a, stream);
+    if (e == hipSuccess && ev) e = hipEventRecord(ev[2], stream);
+    if (e == hipSuccess) e = launch_gather3d(storage, g, stream);
+    if (e == hipSuccess && ev) e = hipEventRecord(ev[3], stream);
+    return e;
+  }
+
+  int reset() override {
+    const size_t n = (size_t)shot.nx * shot.ny * shot.nz;
+    for (int b = 0; b < 2; ++b)
+      for (int f = 0; f < 4; ++f) FOMO_HIP_TRY(hipMemsetAsync(shot.fld[b][f], 0, n * elem, stream));
+    const size_t ns = (size_t)d.n_rec * d.nt;
+    for (int k = 0; k < nslots; ++k)
+      if (ns) FOMO_HIP_TRY(hipMemsetAsync(seis[k], 0, ns * sizeof(float), stream));
+    FOMO_HIP_TRY(hipMemsetAsync(step, 0, sizeof(int), stream));
+    FOMO_HIP_TRY(hipMemsetAsync(done, 0, sizeof(unsigned), stream));
+    steps_done = 0;
+    return FOMO_OK;
+  }
+
+  int read_seis(int sh, float* const* dst) override {
+    if (sh != 0) return set_error(FOMO_EINVAL, "shot index out of range");
+    FOMO_HIP_TRY(hipStreamSynchronize(stream));
+    const size_t cnt = (size_t)d.n_rec * d.nt;
+    for (int k = 0; k < nslots; ++k)
+      if (dst[k] && cnt) FOMO_HIP_TRY(hipMemcpy(dst[k], seis[k], cnt * sizeof(float), hipMemcpyDeviceToHost));
+    return FOMO_OK;
+  }
+
+  int ensure_scratch(size_t n) {
+    if (scratch_n >= n) return FOMO_OK;
+    if (scratch) {
+      (void)hipStreamSynchronize(stream);
+      auto it = std::find(allocs.begin(), allocs.end(), (void*)scratch);
+      if (it != allocs.end()) allocs.erase(it);
+      (void)hipFree(scratch);
+      scratch = nullptr;
+      scratch_n = 0;
+    }
+    int rc = alloc(&scratch, n);
+    if (!rc) scratch_n = n;
+    return rc;
+  }
+
+  int read_field(int sh, int which, float* out) override {
+    if (sh != 0) return set_error(FOMO_EINVAL, "shot index out of range");
+    if (which < 0 || which >= 4) return set_error(FOMO_EINVAL, "field index out of range");
+    const size_t n = (size_t)shot.nx * shot.ny * shot.nz;
+    int rc = ensure_scratch(n);
+    if (rc) return rc;
+    FOMO_HIP_TRY(launch_decode3d(storage, shot.fld[steps_done % 2][which], scratch, n, shot.fdec[which], stream));
+    FOMO_HIP_TRY(hipMemcpyAsync(out, scratch, n * sizeof(float), hipMemcpyDeviceToHost, stream));
+    FOMO_HIP_TRY(hipStreamSynchronize(stream));
+    return FOMO_OK;
+  }
+
+  // Interior slice of p (plane 0 xz, 1 xy, 2 yz; idx1 = 1-based interior index) into host dst.
+  int snapshot(float* dst, int plane, int idx1) {
+    const size_t n = plane == 0 ? (size_t)nxi * nzi : (plane == 1 ? (size_t)nxi * nyi : (size_t)nyi * nzi);
+    int rc = ensure_scratch(n);
+    if (rc) return rc;
+    FOMO_HIP_TRY(launch_slice3d(storage, shot.fld[steps_done % 2][0], scratch, shot.nx, shot.ny, shot.nz, pad, plane,
+                                idx1 - 1 + pad, shot.fdec[0], stream));
+    FOMO_HIP_TRY(hipMemcpyAsync(dst, scratch, n * sizeof(float), hipMemcpyDeviceToHost, stream));
+    return FOMO_OK;
+  }
+
+  int dims(int sh, int* nx, int* ny, int* nz, int* pd) const override {
+    if (sh != 0) return set_error(FOMO_EINVAL, "shot index out of range");
+    if (nx) *nx = shot.nx;
+    if (ny) *ny = shot.ny;
+    if (nz) *nz = shot.nz;
+    if (pd) *pd = pad;
+    return FOMO_OK;
+  }
+  int n_shots() const override { return 1; }
+};
+
+// Work list: frame boxes (64 x 4 cells of one plane) over the six pad slabs and interior column
+// tiles (64 x kTileY3) marched over z-segments of `kseg` planes; ordered by (z, y, x) and split
+// into 8 cost-balanced contiguous runs, one per XCD (block b runs on XCD b % 8).
+static int make_work3d(Plan3D* p, int kseg) {
+  std::vector<Work3D> items;
+  const int nx = p->shot.nx, ny = p->shot.ny, nz = p->shot.nz, pad = p->pad;
+  const int xe = pad + p->nxi, ye = pad + p->nyi, ze = pad + p->nzi;
+  auto frame_rect = [&](int x0, int x1, int y0, int y1, int k) {
+    for (int j = y0; j < y1; j += 4)
+      for (int i = x0; i < x1; i += kTileW)
+        items.push_back(Work3D{0, i, j, k, std::min(kTileW, x1 - i), std::min(4, y1 - j), 1});
+  };
+  for (int k = 0; k < nz; ++k) {
+    if (k < pad || k >= ze) {
+      frame_rect(0, nx, 0, ny, k);
+    } else {
+      frame_rect(0, nx, 0, pad, k);
+      frame_rect(0, nx, ye, ny, k);
+      frame_rect(0, pad, pad, ye, k);
+      frame_rect(xe, nx, pad, ye, k);
+    }
+  }
+  for (int k = pad; k < ze; k += kseg)
+    for (int j = pad; j < ye; j += kTileY3)
+      for (int i = pad; i < xe; i += kTileW)
+        items.push_back(Work3D{1, i, j, k, std::min(kTileW, xe - i), std::min(kTileY3, ye - j), std::min(kseg, ze - k)});
+  std::stable_sort(items.begin(), items.end(), [](const Work3D& a, const Work3D& b) {
+    if (a.k0 != b.k0) return a.k0 < b.k0;
+    if (a.j0 != b.j0) return a.j0 < b.j0;
+    return a.i0 < b.i0;
+  });
+  std::vector<double> cost(items.size());
+  double total = 0.0;
+  for (size_t q = 0; q < items.size(); ++q) {
+    cost[q] = (double)items[q].w * items[q].hy * items[q].hz * (items[q].kind == 0 ? 3.0 : 1.0);
+    total += cost[q];
+  }
+  std::vector<std::vector<Work3D>> runs(8);
+  double acc = 0.0;
+  for (size_t q = 0; q < items.size(); ++q) {
+    const int r = std::min(7, (int)(8.0 * (acc + 0.5 * cost[q]) / std::max(total, 1.0)));
+    runs[r].push_back(items[q]);
+    acc += cost[q];
+  }
+  size_t len = 0;
+  for (auto& r : runs) len = std::max(len, r.size());
+  std::vector<Work3D> flat(8 * len, Work3D{-1, 0, 0, 0, 0, 0, 0});
+  for (int r = 0; r < 8; ++r)
+    for (size_t q = 0; q < runs[r].size(); ++q) flat[q * 8 + r] = runs[r][q];
+  if (flat.size() > (size_t)0x7fffffff) return set_error(FOMO_EUNSUP, "grid too large for one launch");
+  p->n_work = (int)flat.size();
+  return p->upload(&p->d_work, flat.data(), flat.size());
+}
+
+template <class T>
+static int upload_stored(Plan3D* p, void** dst, const std::vector<float>& h, float enc) {
+  if (p->storage == FOMO_STORE_F32) {
+    float* d = nullptr;
+    int rc = p->upload(&d, h.data(), h.size());
+    *dst = d;
+    return rc;
+  }
+  std::vector<T> tmp(h.size());
+  parallel_for((long long)h.size(), [&](long long a, long long b) {
+    for (long long c = a; c < b; ++c) {
+      if constexpr (std::is_same<T, __half>::value)
+        tmp[c] = __float2half_rn(h[c] * enc);
+      else
+        tmp[c] = __float2bfloat16(h[c] * enc);
+    }
+  });
+  T* d = nullptr;
+  int rc = p->upload(&d, tmp.data(), tmp.size());
+  *dst = d;
+  return rc;
+}
+
+static int upload_any(Plan3D* p, void** dst, const std::vector<float>& h, float enc) {
+  if (p->storage == FOMO_STORE_F16) return upload_stored<__half>(p, dst, h, enc);
+  if (p->storage == FOMO_STORE_BF16) return upload_stored<__hip_bfloat16>(p, dst, h, enc);
+  return upload_stored<float>(p, dst, h, enc);
+}
+
+}  // namespace fomo
+
+using namespace fomo;
+
+int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, const float* rho, const int32_t* sx,
+                       const int32_t* sy, const int32_t* sz, const float* wavelet, const int32_t* rx,
+                       const int32_t* ry, const int32_t* rz, fomo_plan** out) {
+  (void)vs;
+  *out = nullptr;
+  if (d->eq != FOMO_ACOUSTIC3D) return set_error(FOMO_EUNSUP, "elastic3d is not provided by this build");
+  if (d->storage < FOMO_STORE_F32 || d->storage > FOMO_STORE_BF16) return set_error(FOMO_EINVAL, "unknown storage");
+  if (!vp || !rho) return set_error(FOMO_EINVAL, "model arrays must not be NULL");
+  if (d->n_src > 0 && (!sx || !sy || !sz || !wavelet)) return set_error(FOMO_EINVAL, "source arrays must not be NULL");
+  if (d->n_rec > 0 && (!rx || !ry || !rz)) return set_error(FOMO_EINVAL, "receiver arrays must not be NULL");
+  for (int s = 0; s < d->n_src; ++s)
+    if (sx[s] < 1 || sx[s] > d->nx || sy[s] < 1 || sy[s] > d->ny || sz[s] < 1 || sz[s] > d->nz)
+      return set_error(FOMO_EINVAL, "source index out of range");
+  for (int r = 0; r < d->n_rec; ++r)
+    if (rx[r] < 1 || rx[r] > d->nx || ry[r] < 1 || ry[r] > d->ny || rz[r] < 1 || rz[r] > d->nz)
+      return set_error(FOMO_EINVAL, "receiver index out of range");
+  Plan3D* p = new Plan3D();
+  auto bail = [&](int code) {
+    delete p;
+    return code;
+  };
+  p->d = *d;
+  p->N = d->fd_order / 2;
+  p->storage = d->storage;
+  p->elem = d->storage == FOMO_STORE_F32 ? 4 : 2;
+  fomo_fd_coefficients(d->fd_order, p->coef);
+  p->nxi = d->nx;
+  p->nyi = d->ny;
+  p->nzi = d->nz;
+  p->pad = d->nbc + p->N;
+  const int pad = p->pad;
+  Shot3D& S = p->shot;
+  S.nx = d->nx + 2 * pad;
+  S.ny = d->ny + 2 * pad;
+  S.nz = d->nz + 2 * pad;
+  S.H = pad - 1;
+  S.pad = pad;
+  S.dtdh = d->dt / d->dh;
+  hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return bail(set_error(FOMO_EHIP, std::string("stream: ") + hipGetErrorString(e)));
+  const size_t sy_ = (size_t)S.nx, sz_ = (size_t)S.nx * S.ny, n = sz_ * S.nz;
+  const size_t ixy = (size_t)d->nx * d->ny;
+  // v_ref: min(vp > 0) over the interior unless given
+  float v_ref = d->v_ref;
+  if (!(v_ref > 0.0f)) v_ref = min_positive(vp, ixy * d->nz);
+  float hc[3];
+  fomo_habc_coefficients(v_ref, d->dt, d->dh, hc);
+  S.q = hc[0];
+  S.qt = hc[1];
+  S.qxt = hc[2];
+  // medium: pad_array_3d.jl:8-29 (clamp indexing), kappa = rho*vp^2, buoyancy pad_array_3d.jl:37-89
+  std::vector<float> rhop(n), kappa(n);
+  parallel_for(S.nz, [&](long long k0, long long k1) {
+    for (long long k = k0; k < k1; ++k) {
+      const int kk = std::min(std::max((int)k - pad, 0), d->nz - 1);
+      for (int j = 0; j < S.ny; ++j) {
+        const int jj = std::min(std::max(j - pad, 0), d->ny - 1);
+        for (int i = 0; i < S.nx; ++i) {
+          const int ii = std::min(std::max(i - pad, 0), d->nx - 1);
+          const size_t src = (size_t)ii + (size_t)jj * d->nx + (size_t)kk * ixy;
+          const size_t c = (size_t)i + j * sy_ + k * sz_;
+          rhop[c] = rho[src];
+          kappa[c] = rho[src] * (vp[src] * vp[src]);
+        }
+      }
+    }
+  });
+  auto hb = [](float r1, float r2) {
+    if (r1 == 0.0f && r2 == 0.0f) return 0.0f;
+    if (r1 == 0.0f) return 2.0f / r2;
+    if (r2 == 0.0f) return 2.0f / r1;
+    return 2.0f / (r1 + r2);
+  };
+  float kmax = 0.0f, bmax = 0.0f, zmin = INFINITY;
+  for (size_t c = 0; c < n; c += 1) kmax = std::max(kmax, std::fabs(kappa[c]));
+  // impedance floor for the velocity scale: min rho*vp over cells with vp > 0
+  for (size_t c = 0; c < ixy * d->nz; ++c)
+    if (vp[c] > 0.0f && rho[c] > 0.0f) zmin = std::min(zmin, rho[c] * vp[c]);
+  float wmax = 0.0f;
+  for (size_t c = 0; c < (size_t)d->n_src * d->nt; ++c) wmax = std::max(wmax, std::fabs(wavelet[c]));
+  const bool scaled = d->storage != FOMO_STORE_F32;
+  int rc;
+  {
+    void* m0 = nullptr;
+    const float kdec = scaled && kmax > 0.0f ? kmax : 1.0f;
+    if ((rc = upload_any(p, &m0, kappa, 1.0f / kdec))) return bail(rc);
+    S.mat[0] = m0;
+    S.mdec[0] = scaled ? kdec : 1.0f;
+    std::vector<float>().swap(kappa);
+  }
+  for (int ax = 0; ax < 3; ++ax) {
+    std::vector<float> b(n, 0.0f);
+    parallel_for(S.nz, [&](long long k0, long long k1) {
+      for (long long k = k0; k < k1; ++k)
+        for (int j = 0; j < S.ny; ++j)
+          for (int i = 0; i < S.nx; ++i) {
+            const size_t c = (size_t)i + j * sy_ + k * sz_;
+            if (ax == 0) b[c] = i >= 1 ? hb(rhop[c - 1], rhop[c]) : 0.0f;
+            else if (ax == 1) b[c] = j >= 1 ? hb(rhop[c - sy_], rhop[c]) : 0.0f;
+            else b[c] = k + 1 < S.nz ? hb(rhop[c], rhop[c + sz_]) : 0.0f;
+          }
+    });
+    // edge copies: buoy_vx[1,:,:] = buoy_vx[2,:,:]; buoy_vy[:,1,:] = buoy_vy[:,2,:]; buoy_vz[:,:,nz] = buoy_vz[:,:,nz-1]
+    if (ax == 0)
+      for (int k = 0; k < S.nz; ++k)
+        for (int j = 0; j < S.ny; ++j) b[j * sy_ + k * sz_] = b[1 + j * sy_ + k * sz_];
+    if (ax == 1)
+      for (int k = 0; k < S.nz; ++k)
+        for (int i = 0; i < S.nx; ++i) b[i + k * sz_] = b[i + sy_ + k * sz_];
+    if (ax == 2)
+      for (int j = 0; j < S.ny; ++j)
+        for (int i = 0; i < S.nx; ++i) b[i + j * sy_ + (S.nz - 1) * sz_] = b[i + j * sy_ + (S.nz - 2) * sz_];
+    float bm = 0.0f;
+    for (size_t c = 0; c < n; ++c) bm = std::max(bm, b[c]);
+    bmax = std::max(bmax, bm);
+    const float bdec = scaled && bm > 0.0f ? bm : 1.0f;
+    void* m = nullptr;
+    if ((rc = upload_any(p, &m, b, 1.0f / bdec))) return bail(rc);
+    S.mat[1 + ax] = m;
+    S.mdec[1 + ax] = scaled ? bdec : 1.0f;
+  }
+  std::vector<float>().swap(rhop);
+  // field scales: p in units of the peak source amplitude, velocities in p/(min impedance)
+  const float pdec = scaled ? (wmax > 0.0f ? wmax : 1.0f) : 1.0f;
+  const float vdec = scaled ? (std::isfinite(zmin) && zmin > 0.0f ? pdec / zmin : pdec) : 1.0f;
+  S.fdec[0] = pdec;
+  S.fenc[0] = 1.0f / pdec;
+  for (int f = 1; f < 4; ++f) {
+    S.fdec[f] = vdec;
+    S.fenc[f] = 1.0f / vdec;
+  }
+  for (int b = 0; b < 2; ++b)
+    for (int f = 0; f < 4; ++f) {
+      char* ptr = nullptr;
+      if ((rc = p->alloc(&ptr, n * p->elem))) return bail(rc);
+      S.fld[b][f] = ptr;
+    }
+  {
+    std::vector<float> lut(4 * (size_t)(S.H + 1));
+    weight_lut(S.H, pad, 0.0, lut.data());
+    weight_lut(S.H, pad, 0.5, lut.data() + (S.H + 1));
+    weight_lut(S.H, pad, 0.25, lut.data() + 2 * (S.H + 1));
+    weight_lut(S.H, pad, 0.75, lut.data() + 3 * (S.H + 1));
+    float* dl = nullptr;
+    if ((rc = p->upload(&dl, lut.data(), lut.size()))) return bail(rc);
+    S.wlut = dl;
+  }
+  {
+    S.n_src = d->n_src;
+    std::vector<long long> sc(std::max(d->n_src, 1), -1);
+    for (int q = 0; q < d->n_src; ++q)
+      sc[q] = (long long)(sx[q] - 1 + pad) + (long long)(sy[q] - 1 + pad) * (long long)sy_ +
+              (long long)(sz[q] - 1 + pad) * (long long)sz_;
+    long long* dsc = nullptr;
+    if ((rc = p->upload(&dsc, sc.data(), sc.size()))) return bail(rc);
+    S.src_c = dsc;
+    float* dw = nullptr;
+    if ((rc = p->upload(&dw, wavelet ? wavelet : (const float*)nullptr, wavelet ? (size_t)d->n_src * d->nt : 0)))
+      return bail(rc);
+    S.wav = dw;
+    std::vector<long long> rcell(std::max(d->n_rec, 1), 0);
+    for (int r = 0; r < d->n_rec; ++r)
+      rcell[r] = (long long)(rx[r] - 1 + pad) + (long long)(ry[r] - 1 + pad) * (long long)sy_ +
+                 (long long)(rz[r] - 1 + pad) * (long long)sz_;
+    if ((rc = p->upload(&p->rec_c, rcell.data(), rcell.size()))) return bail(rc);
+    p->nslots = d->record_p ? 4 : 3;
+    for (int k = 0; k < p->nslots; ++k)
+      if ((rc = p->alloc(&p->seis[k], (size_t)d->n_rec * d->nt))) return bail(rc);
+  }
+  if ((rc = p->upload(&p->d_shot, &p->shot, 1))) return bail(rc);
+  if ((rc = p->alloc(&p->step, 1)) || (rc = p->alloc(&p->done, 1))) return bail(rc);
+  if ((rc = make_work3d(p, d->tile_cfg > 0 ? 16 << (d->tile_cfg - 1) : 64))) return bail(rc);
+  if ((rc = p->reset())) return bail(rc);
+  if (d->use_graph && (rc = p->build_graph())) return bail(rc);
+  e = hipStreamSynchronize(p->stream);
+  if (e != hipSuccess) return bail(set_error(FOMO_EHIP, std::string("setup: ") + hipGetErrorString(e)));
+  *out = new fomo_plan{p};
+  return FOMO_OK;
+}
+
+extern "C" int fomo_acoustic3d(const fomo_desc* d, const float* vp, const float* rho, const int32_t* sx,
+                               const int32_t* sy, const int32_t* sz, const float* wavelet, const int32_t* rx,
+                               const int32_t* ry, const int32_t* rz, float* seis_vx, float* seis_vy, float* seis_vz,
+                               float* seis_p, float* snaps, double* kernel_time_s) {
+  if (!d || d->eq != FOMO_ACOUSTIC3D) return set_error(FOMO_EINVAL, "descriptor eq must be FOMO_ACOUSTIC3D");
+  fomo_desc dd = *d;
+  dd.record_p = seis_p ? 1 : 0;
+  fomo_plan* h = nullptr;
+  int rc = fomo_plan_create(&dd, vp, nullptr, rho, sx, sy, sz, wavelet, rx, ry, rz, &h);
+  if (rc) return rc;
+  Plan3D* p = static_cast<Plan3D*>(h->impl);
+  auto finish = [&](int code) {
+    fomo_plan_destroy(h);
+    return code;
+  };
+  if (d->nt > 0) {
+    if ((rc = p->advance(1)) || (rc = fomo_plan_sync(h)) || (rc = p->reset())) return finish(rc);
+  }
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return finish(set_error(FOMO_EHIP, "hipEventCreate failed"));
+  (void)hipEventRecord(e0, p->stream);
+  if (d->snap_interval > 0 && snaps) {
+    const int plane = d->snap_plane;
+    const size_t per = plane == 0 ? (size_t)d->nx * d->nz : (plane == 1 ? (size_t)d->nx * d->ny : (size_t)d->ny * d->nz);
+    const int idx = d->snap_index > 0 ? d->snap_index : (plane == 0 ? d->ny / 2 : (plane == 1 ? d->nz / 2 : d->nx / 2));
+    const int ns = d->nt / d->snap_interval;
+    for (int k = 0; k < ns; ++k) {
+      if ((rc = p->advance(d->snap_interval)) || (rc = p->snapshot(snaps + (size_t)k * per, plane, idx)))
+        return finish(rc);
+    }
+    if ((rc = p->advance(d->nt - ns * d->snap_interval))) return finish(rc);
+  } else if ((rc = p->advance(d->nt))) {
+    return finish(rc);
+  }
+  (void)hipEventRecord(e1, p->stream);
+  if (hipEventSynchronize(e1) != hipSuccess) return finish(set_error(FOMO_EHIP, "step loop failed"));
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (kernel_time_s) *kernel_time_s = ms * 1e-3;
+  float* dst[4] = {seis_vx, seis_vy, seis_vz, seis_p};
+  rc = p->read_seis(0, dst);
+  return finish(rc);
+}

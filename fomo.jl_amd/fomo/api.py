@@ -232,6 +232,70 @@ def elastic2d_batch(vp, vs, rho, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_o
     return Elastic2DBatchResult(outs[0], outs[1], st)
 
 
+Acoustic3DResult = namedtuple("Acoustic3DResult", ["seis_vx", "seis_vy", "seis_vz", "snaps"])
+Acoustic3DExtended = namedtuple("Acoustic3DExtended", ["seis_vx", "seis_vy", "seis_vz", "snaps", "seis_p", "stats"])
+_PLANES = {"xz": 0, "xy": 1, "yz": 2}
+_STORAGE = {"f32": _lib.STORE_F32, "float32": _lib.STORE_F32, "f16": _lib.STORE_F16, "float16": _lib.STORE_F16,
+            "bf16": _lib.STORE_BF16, "bfloat16": _lib.STORE_BF16}
+
+
+def storage_code(storage) -> int:
+    key = _sym(storage).lower()
+    if key not in _STORAGE:
+        raise ArgumentError(f"storage must be :f32, :f16 or :bf16, got {storage}")
+    return _STORAGE[key]
+
+
+def acoustic3d(vp, rho, dh, dt, nt, f0, *, sx, sy, sz, rx, ry, rz, nbc=50, fd_order=8, snap_interval=0,
+               snap_plane="xz", snap_index=0, v_ref=None, storage="f32", wavelet=None, record_p=False,
+               use_cuda_graph=True, extended=False):
+    """acoustic3d (src/equations/acoustic3d/acoustic3d.jl:31-129) -> (seis_vx, seis_vy, seis_vz, snaps).
+
+    Extensions: `storage` (:f32, :f16, :bf16 storage of wavefields and materials, FP32 arithmetic),
+    `wavelet` (custom source time function), `record_p` and `extended=True` (also return seis_p and
+    stats).  Every source gets the wavelet (the reference reshapes one wavelet to 1 x nt)."""
+    vp, rho = _f32F(vp), _f32F(rho)
+    if vp.ndim != 3 or vp.shape != rho.shape:
+        raise ArgumentError("vp and rho must be 3D arrays of the same size")
+    if fd_order not in (2, 4, 6, 8, 10):
+        raise ValueError(f"Unsupported FD order: {fd_order}")
+    if nbc < 0 or nt < 0 or snap_interval < 0:
+        raise ArgumentError("nbc, nt and snap_interval must be >= 0")
+    nx, ny, nz = vp.shape
+    plane = _PLANES.get(_sym(snap_plane))
+    if plane is None:
+        raise ArgumentError(f"snap_plane must be :xz, :xy or :yz, got {snap_plane}")
+    if snap_index == 0:
+        snap_index = ny // 2  # acoustic3d.jl:53-55 (the reference uses ny/2 for every plane)
+    from .checks import check_geometry_3d
+    check_geometry_3d(nx, ny, nz, sx, sy, sz, rx, ry, rz)
+    v_ref = _default_vref(vp) if v_ref is None else float(np.float32(v_ref))
+    sx, sy, sz, rx, ry, rz = (_i32(x) for x in (sx, sy, sz, rx, ry, rz))
+    w = ricker_wavelet(f0, dt, nt) if wavelet is None else np.asarray(wavelet, np.float64).astype(np.float32)
+    if len(w) != nt:
+        raise ArgumentError(f"wavelet length {len(w)} != nt={nt}")
+    wm = np.asfortranarray(np.repeat(w[None, :], len(sx), axis=0))
+    d = _desc(_lib.ACOUSTIC3D, nx, nz, nbc, fd_order, nt, dh, dt, v_ref, len(sx), len(rx), snap_interval,
+              use_cuda_graph, ny=ny, storage=storage_code(storage))
+    d.snap_plane, d.snap_index = plane, snap_index
+    n_rec = len(rx)
+    outs = [np.zeros((n_rec, nt), np.float32, order="F") for _ in range(4)]
+    want_p = record_p or extended
+    nsnap = nt // snap_interval if snap_interval > 0 else 0
+    a_n, b_n = {0: (nx, nz), 1: (nx, ny), 2: (ny, nz)}[plane]
+    snaps = np.zeros((nsnap, b_n, a_n), np.float32) if nsnap else None
+    t = C.c_double(0.0)
+    _lib.check(_lib.lib().fomo_acoustic3d(C.byref(d), vp.ctypes.data, rho.ctypes.data, sx.ctypes.data,
+                                          sy.ctypes.data, sz.ctypes.data, wm.ctypes.data, rx.ctypes.data,
+                                          ry.ctypes.data, rz.ctypes.data, outs[0].ctypes.data, outs[1].ctypes.data,
+                                          outs[2].ctypes.data, outs[3].ctypes.data if want_p else None,
+                                          None if snaps is None else snaps.ctypes.data, C.byref(t)))
+    snap_list = [s.T for s in snaps] if snaps is not None else []
+    if extended:
+        return Acoustic3DExtended(outs[0], outs[1], outs[2], snap_list, outs[3], Stats(float(t.value)))
+    return Acoustic3DResult(outs[0], outs[1], outs[2], snap_list)
+
+
 class Plan:
     """Device-resident run (fomo_plan): setup once, then advance / read / reset.
 

@@ -118,6 +118,20 @@ int fomo_elastic2d(const fomo_desc *d, const float *vp, const float *vs, const f
                    float *seis_vz, float *snaps, double *kernel_time_s);
 
 /*
+ * acoustic3d -- replaces Fomo.acoustic3d (src/equations/acoustic3d/acoustic3d.jl:31-129).
+ * vp, rho: nx*ny*nz (x fastest).  Sources (sx,sy,sz) with wavelet n_src x nt; receivers
+ * (rx,ry,rz).  Outputs n_rec x nt: seis_vx, seis_vy, seis_vz, and seis_p when not NULL (an
+ * extension: the reference records only velocities).  snaps: (nt/snap_interval) slices of p in
+ * d->snap_plane (0 xz, 1 xy, 2 yz) at d->snap_index (1-based interior index; 0 = middle).
+ * d->storage selects Float32, Float16 or BFloat16 storage of wavefields and materials
+ * (arithmetic stays Float32).  The Higdon boundary is the deterministic two-pass form.
+ */
+int fomo_acoustic3d(const fomo_desc *d, const float *vp, const float *rho, const int32_t *sx, const int32_t *sy,
+                    const int32_t *sz, const float *wavelet, const int32_t *rx, const int32_t *ry, const int32_t *rz,
+                    float *seis_vx, float *seis_vy, float *seis_vz, float *seis_p, float *snaps,
+                    double *kernel_time_s);
+
+/*
  * acoustic2d_batch -- replaces Fomo.acoustic2d_batch (src/equations/acoustic2d/batch.jl:180-261):
  * n_shots shots over the shared model and receiver spread; d->n_src sources per shot.
  * sx, sz: n_src x n_shots (column-major, as init_batched_source stores them, batch_acq.jl:27-37);

@@ -1,0 +1,119 @@
+"""GPU acoustic3d: bit-identity with the FP32 oracle, graph == stream, snapshots, and the error
+of Float16 / BFloat16 storage against the FP64 oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def model3d(nx, ny, nz, seed):
+    from fomo.models import layered_3d
+    return layered_3d(nx, ny, nz, seed=seed, n_interfaces=3, jitter=3.0, n_blobs=2, sigma=6.0)
+
+
+def geom3d(nx, ny, nz):
+    sx, sy, sz = [nx // 2, nx // 3], [ny // 2, ny // 2 + 3], [6, nz // 2]
+    rx = np.concatenate([np.arange(1, nx + 1, 4), [1, nx, nx]])
+    ry = np.concatenate([np.full(len(np.arange(1, nx + 1, 4)), ny // 2), [1, ny, 1]])
+    rz = np.concatenate([np.full(len(np.arange(1, nx + 1, 4)), 3), [1, nz, nz]])
+    return sx, sy, sz, rx, ry, rz
+
+
+def run_plan(vp, rho, nt, w, geo, nbc, order, storage=0, use_graph=True):
+    import fomo._lib as L
+    from fomo.api import Plan
+    sx, sy, sz, rx, ry, rz = geo
+    p = Plan(L.ACOUSTIC3D, vp, rho, 10.0, 1e-3, nt, sx=sx, sy=sy, sz=sz, rx=rx, ry=ry, rz=rz,
+             wavelet=np.stack([w] * len(sx)), nbc=nbc, fd_order=order, storage=storage, record_p=True,
+             use_graph=use_graph)
+    p.advance(nt)
+    p.sync()
+    seis = p.seis(4)
+    state = [p.field(k) for k in range(4)]
+    p.close()
+    return seis, state
+
+
+@pytest.mark.parametrize("order", [2, 4, 8])
+def test_acoustic3d_bitwise_vs_oracle(gpu, orc, order):
+    import fomo
+    nx, ny, nz, nt, nbc = 45, 38, 41, 70, 5
+    vp, rho = model3d(nx, ny, nz, 3)
+    geo = geom3d(nx, ny, nz)
+    w = fomo.ricker_wavelet(15.0, 1e-3, nt)
+    seis, state = run_plan(vp, rho, nt, w, geo, nbc, order)
+    sx, sy, sz, rx, ry, rz = geo
+    ref = orc.acoustic3d(vp, rho, 10.0, 1e-3, nt, sx=sx, sy=sy, sz=sz, rx=rx, ry=ry, rz=rz,
+                         wavelet=np.stack([w, w]), nbc=nbc, fd_order=order, final_state=True)
+    for got, key in zip(seis, ("seis_vx", "seis_vy", "seis_vz", "seis_p")):
+        assert np.all(np.isfinite(got))
+        assert np.array_equal(got, ref[key]), (key, np.abs(got - ref[key]).max())
+    for got, key in zip(state, ("p", "vx", "vy", "vz")):
+        assert np.array_equal(got, ref["state"][key]), key
+
+
+def test_acoustic3d_graph_equals_stream_and_api(gpu, orc):
+    import fomo
+    nx, ny, nz, nt = 40, 33, 36, 51
+    vp, rho = model3d(nx, ny, nz, 4)
+    geo = geom3d(nx, ny, nz)
+    w = fomo.ricker_wavelet(15.0, 1e-3, nt)
+    a, sa = run_plan(vp, rho, nt, w, geo, 6, 8, use_graph=True)
+    b, sb = run_plan(vp, rho, nt, w, geo, 6, 8, use_graph=False)
+    c, sc = run_plan(vp, rho, nt, w, geo, 6, 8, use_graph=True)
+    for x, y, z in zip(a + sa, b + sb, c + sc):
+        assert np.array_equal(x, y) and np.array_equal(x, z)
+    sx, sy, sz, rx, ry, rz = geo
+    r = fomo.acoustic3d(vp, rho, 10.0, 1e-3, nt, 15.0, sx=sx, sy=sy, sz=sz, rx=rx, ry=ry, rz=rz, nbc=6,
+                        fd_order=8)
+    assert len(r) == 4
+    for x, y in zip(r[:3], a[:3]):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("plane", ["xz", "xy", "yz"])
+def test_acoustic3d_snapshots(gpu, orc, plane):
+    import fomo
+    nx, ny, nz, nt = 34, 30, 28, 40
+    vp, rho = model3d(nx, ny, nz, 5)
+    geo = geom3d(nx, ny, nz)
+    sx, sy, sz, rx, ry, rz = geo
+    r = fomo.acoustic3d(vp, rho, 10.0, 1e-3, nt, 15.0, sx=sx, sy=sy, sz=sz, rx=rx, ry=ry, rz=rz, nbc=4,
+                        fd_order=4, snap_interval=20, snap_plane=plane, snap_index=7)
+    assert len(r.snaps) == 2
+    w = fomo.ricker_wavelet(15.0, 1e-3, nt)
+    ref = orc.acoustic3d(vp, rho, 10.0, 1e-3, nt, sx=sx, sy=sy, sz=sz, rx=rx, ry=ry, rz=rz,
+                         wavelet=np.stack([w, w]), nbc=4, fd_order=4, final_state=True)
+    pad = 4 + 2
+    p = ref["state"]["p"][pad:-pad, pad:-pad, pad:-pad]
+    want = {"xz": p[:, 6, :], "xy": p[:, :, 6], "yz": p[6, :, :]}[plane]
+    assert r.snaps[-1].shape == want.shape
+    assert np.array_equal(r.snaps[-1], want)
+
+
+@pytest.mark.parametrize("storage,tol", [(1, 2e-2), (2, 1e-1)])
+def test_reduced_precision_storage_error(gpu, orc, storage, tol):
+    """Float16 / BFloat16 storage with per-array scaling, FP32 arithmetic: per-trace relative L2
+    against the FP64 oracle (reported; bound set loosely, the figure is what matters)."""
+    import fomo
+    nx, ny, nz, nt, nbc = 48, 44, 46, 200, 8
+    vp, rho = model3d(nx, ny, nz, 6)
+    geo = geom3d(nx, ny, nz)
+    w = fomo.ricker_wavelet(15.0, 1e-3, nt)
+    seis, _ = run_plan(vp, rho, nt, w, geo, nbc, 8, storage=storage)
+    sx, sy, sz, rx, ry, rz = geo
+    ref = orc.acoustic3d(vp, rho, 10.0, 1e-3, nt, sx=sx, sy=sy, sz=sz, rx=rx, ry=ry, rz=rz,
+                         wavelet=np.stack([w, w]), nbc=nbc, fd_order=8, precision="f64")
+    f32, _ = run_plan(vp, rho, nt, w, geo, nbc, 8, storage=0)
+    errs = {}
+    for got, g32, key in zip(seis, f32, ("seis_vx", "seis_vy", "seis_vz", "seis_p")):
+        r = ref[key]
+        den = np.linalg.norm(r, axis=1)
+        ok = den > 1e-6 * den.max()
+        e = np.linalg.norm(got.astype(np.float64) - r, axis=1)[ok] / den[ok]
+        e32 = np.linalg.norm(g32.astype(np.float64) - r, axis=1)[ok] / den[ok]
+        errs[key] = (float(np.median(e)), float(e.max()), float(e32.max()))
+    print(f"storage {storage}: per-trace rel L2 vs FP64 (median, max, fp32 max) {errs}")
+    for key, (med, mx, m32) in errs.items():
+        assert m32 < 1e-4
+        assert mx < tol, (key, mx)
