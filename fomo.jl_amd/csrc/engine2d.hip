@@ -32,11 +32,18 @@ struct PV {
   float* out[3];
   const float* mat[3];
   const float* lut[3];
+  const float* alpha[3];  // learned corrections of the distance-table weight (NULL: table only)
+  float* sens[3];         // f_pde - S at frame cells while training (NULL: not recorded)
 };
 
 template <int PH>
 __device__ __forceinline__ void make_view(const Shot2D& S, int s, PV& V) {
   const int o = 1 - s;
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    V.alpha[f] = nullptr;
+    V.sens[f] = nullptr;
+  }
   const int L = S.H + 1;
   if (PH == kAcVel) {
     V.sten[0] = S.fld[s][0];
@@ -48,6 +55,9 @@ __device__ __forceinline__ void make_view(const Shot2D& S, int s, PV& V) {
     V.mat[1] = S.mat[2];
     V.lut[0] = S.wlut;
     V.lut[1] = S.wlut + L;
+    V.alpha[0] = V.alpha[1] = S.alpha[0];
+    V.sens[0] = S.sens[0];
+    V.sens[1] = S.sens[1];
   } else if (PH == kAcPrs) {
     V.sten[0] = S.fld[o][1];
     V.sten[1] = S.fld[o][2];
@@ -55,6 +65,8 @@ __device__ __forceinline__ void make_view(const Shot2D& S, int s, PV& V) {
     V.out[0] = S.fld[o][0];
     V.mat[0] = S.mat[0];
     V.lut[0] = S.wlut + 3 * L;
+    V.alpha[0] = S.alpha[1];
+    V.sens[0] = S.sens[2];
   } else if (PH == kElVel) {
     V.sten[0] = S.fld[s][2];
     V.sten[1] = S.fld[s][3];
@@ -292,7 +304,8 @@ __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, 
           const float fn = pre_s[f * EW * EH + (lj + dj) * EW + li];
           sz = (-q * fn - qt * foc) - qxt * fo[(size_t)i + (size_t)(j + dj) * (size_t)nx];
         }
-        const float wt = V.lut[f][dist];
+        const float wt = V.alpha[f] ? fminf(fmaxf(V.lut[f][dist] + V.alpha[f][c], 0.0f), 1.0f) : V.lut[f][dist];
+        if (V.sens[f]) V.sens[f][c] = v[f] - (inx && inz ? 0.5f * (sx + sz) : (inx ? sx : sz));
         if (inx && inz)
           v[f] = wt * v[f] + ((1.0f - wt) * 0.5f) * (sx + sz);
         else if (inx)

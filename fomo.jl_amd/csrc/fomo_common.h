@@ -34,6 +34,11 @@ struct Shot2D {
   const int* src_c;     // padded cell index of each source
   const float* wav;     // n_src x nt
   int src_fz;           // elastic: 1 = vertical force into vz, 0 = explosive into txx, tzz
+  // Learned boundary (acoustic only; NULL otherwise): per-cell corrections of the blend weights of
+  // the velocity and pressure fields at frame cells, and, while training, the sensitivities
+  // f_pde - S of vx, vz, p.
+  const float* alpha[2];
+  float* sens[3];
 };
 
 // One unit of sweep work.  kind 0: a frame tile (at most kTileW x kFrameRows cells of the

@@ -1,0 +1,62 @@
+// learned2d.h -- learned absorbing boundary for 2D acoustic runs: a per-frame-cell MLP that
+// predicts the Higdon blend weights.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fomo_common.h"
+
+namespace fomo {
+
+// MLP shape: 76 inputs (5x5 patches of p, vx, vz, each scaled by its patch max-abs, plus the depth
+// in the layer dist/H), two hidden layers of 32 (ReLU), 2 outputs z.  The outputs are bounded
+// corrections to the distance-ramp blend weights: corr = kCorrMax * tanh(z), and the frame blend
+// uses clamp(w(dist) + corr, 0, 1) -- corr_v for vx and vz, corr_p for p.  Zero output weights
+// give exactly the plain Higdon boundary.
+constexpr int kMlpIn = 76, kMlpH = 32, kMlpOut = 2;
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kMlpH * kMlpIn;   // W1 is [32][76] row-major
+constexpr int kOffW2 = kOffB1 + kMlpH;
+constexpr int kOffB2 = kOffW2 + kMlpH * kMlpH;    // W2 [32][32]
+constexpr int kOffW3 = kOffB2 + kMlpH;
+constexpr int kOffB3 = kOffW3 + kMlpOut * kMlpH;  // W3 [2][32]
+constexpr int kMlpParams = kOffB3 + kMlpOut;      // 3586
+constexpr float kCorrMax = 0.25f;
+
+struct MlpArgs {
+  const void* w_bf16;   // kMlpParams BFloat16 weights
+  const int* cells;     // frame cells (padded index) of shot 0
+  int n_cells;
+  int nx, nz, H;
+  const float* p;       // state at time n (read buffer)
+  const float* vx;
+  const float* vz;
+  float* alpha_v;       // full-grid arrays of corrections, written at frame cells
+  float* alpha_p;
+};
+
+struct MlpTrainArgs {
+  MlpArgs m;
+  // student after the step (time n+1) and the frame sensitivities f_pde - S written by the sweeps
+  const float* p1;
+  const float* vx1;
+  const float* vz1;
+  const float* sens_p;
+  const float* sens_vx;
+  const float* sens_vz;
+  // teacher state at time n+1 and the offset of the student grid inside the teacher grid
+  const float* tp;
+  const float* tvx;
+  const float* tvz;
+  int tnx, off_x, off_z;
+  float inv_scale_p, inv_scale_v;
+  float* partial;       // [gridDim.x][kMlpParams + 1] gradient (+ loss) partial sums, accumulated
+};
+
+hipError_t launch_mlp_forward(const MlpArgs& a, hipStream_t s);
+hipError_t launch_mlp_train(const MlpTrainArgs& a, int blocks, hipStream_t s);
+hipError_t launch_reduce_partials(const float* partial, int blocks, int width, float* out, hipStream_t s);
+// Unit entry: forward/backward on given features (n x 76) and output gradients (n x 2).
+hipError_t launch_mlp_eval(const void* w_bf16, const float* feats, const float* galpha, int n, float* alpha,
+                           float* partial, int blocks, hipStream_t s);
+
+}  // namespace fomo

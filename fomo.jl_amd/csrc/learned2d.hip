@@ -1,0 +1,274 @@
+// learned2d.hip -- per-frame-cell MLP for the learned absorbing boundary: inference, training
+// (forward + backward with deterministic gradient reduction) and a unit entry for gradient checks.
+//
+// Inference writes the two blend-weight corrections of every frame cell; the 2D frame tiles then
+// blend f = a*f_pde + (1-a)*S with a = clamp(w(dist) + corr, 0, 1).  Training compares the
+// student's new state at frame cells with a wide-boundary teacher run at the same physical cells
+// (one-step loss L = sum ((f - f_teacher)/scale)^2), so that dL/dcorr = 2 (f - f_teacher)/scale^2
+// * (f_pde - S) (inside the clamp), with f_pde - S written by the sweep.
+#include <hip/hip_bf16.h>
+
+#include "learned2d.h"
+
+namespace fomo {
+
+__device__ __forceinline__ float corrf(float z) { return kCorrMax * tanhf(z); }
+
+// Load all weights into LDS as Float32 (BF16 storage, FP32 accumulate).
+__device__ __forceinline__ void load_weights(const void* w_bf16, float* W) {
+  const __hip_bfloat16* w = (const __hip_bfloat16*)w_bf16;
+  for (int e = threadIdx.x; e < kMlpParams; e += blockDim.x) W[e] = __bfloat162float(w[e]);
+  __syncthreads();
+}
+
+// Features of frame cell c: 5x5 patches (clamped to the grid) of p, vx, vz, p divided by its
+// patch max-abs and vx, vz by theirs (no squares: precursor values ~1e-20 stay representable),
+// then dist/H.
+__device__ __forceinline__ void gather_features(const float* p, const float* vx, const float* vz, int nx, int nz,
+                                                int H, int c, float* x) {
+  const int i = c % nx, j = c / nx;
+  float mp = 0.0f, mv = 0.0f;
+#pragma unroll
+  for (int dy = -2; dy <= 2; ++dy) {
+    const int jj = min(max(j + dy, 0), nz - 1);
+#pragma unroll
+    for (int dx = -2; dx <= 2; ++dx) {
+      const int ii = min(max(i + dx, 0), nx - 1);
+      const int q = (dy + 2) * 5 + (dx + 2);
+      const size_t g = (size_t)ii + (size_t)jj * (size_t)nx;
+      const float a = p[g], b = vx[g], d = vz[g];
+      x[q] = a;
+      x[25 + q] = b;
+      x[50 + q] = d;
+      mp = fmaxf(mp, fabsf(a));
+      mv = fmaxf(mv, fmaxf(fabsf(b), fabsf(d)));
+    }
+  }
+  const float sp = mp + 1e-30f, sv = mv + 1e-30f;
+#pragma unroll
+  for (int q = 0; q < 25; ++q) x[q] = x[q] / sp;
+#pragma unroll
+  for (int q = 25; q < 75; ++q) x[q] = x[q] / sv;
+  const int dist = min(min(i, nx - 1 - i), min(j, nz - 1 - j));
+  x[75] = (float)dist / (float)H;
+}
+
+__device__ __forceinline__ void mlp_fwd(const float* W, const float* x, float* h1, float* h2, float* z) {
+#pragma unroll 4
+  for (int o = 0; o < kMlpH; ++o) {
+    float acc = W[kOffB1 + o];
+    const float* w = W + kOffW1 + o * kMlpIn;
+#pragma unroll
+    for (int k = 0; k < kMlpIn; ++k) acc += w[k] * x[k];
+    h1[o] = fmaxf(acc, 0.0f);
+  }
+#pragma unroll 4
+  for (int o = 0; o < kMlpH; ++o) {
+    float acc = W[kOffB2 + o];
+    const float* w = W + kOffW2 + o * kMlpH;
+#pragma unroll
+    for (int k = 0; k < kMlpH; ++k) acc += w[k] * h1[k];
+    h2[o] = fmaxf(acc, 0.0f);
+  }
+#pragma unroll
+  for (int o = 0; o < kMlpOut; ++o) {
+    float acc = W[kOffB3 + o];
+    const float* w = W + kOffW3 + o * kMlpH;
+#pragma unroll
+    for (int k = 0; k < kMlpH; ++k) acc += w[k] * h2[k];
+    z[o] = acc;
+  }
+}
+
+// Backward of one cell given dL/dcorr; leaves g_z, g_h2, g_h1 for the weight-gradient GEMMs.
+// corr = kCorrMax tanh(z): dcorr/dz = kCorrMax (1 - (corr/kCorrMax)^2).
+__device__ __forceinline__ void mlp_bwd(const float* W, const float* h1, const float* h2, const float* corr,
+                                        const float* ga, float* gz, float* gh2, float* gh1) {
+#pragma unroll
+  for (int k = 0; k < kMlpOut; ++k) {
+    const float t = corr[k] * (1.0f / kCorrMax);
+    gz[k] = ga[k] * (kCorrMax * (1.0f - t * t));
+  }
+#pragma unroll 4
+  for (int o = 0; o < kMlpH; ++o) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kMlpOut; ++k) acc += W[kOffW3 + k * kMlpH + o] * gz[k];
+    gh2[o] = h2[o] > 0.0f ? acc : 0.0f;
+  }
+#pragma unroll 4
+  for (int o = 0; o < kMlpH; ++o) {
+    float acc = 0.0f;
+#pragma unroll 8
+    for (int m = 0; m < kMlpH; ++m) acc += W[kOffW2 + m * kMlpH + o] * gh2[m];
+    gh1[o] = h1[o] > 0.0f ? acc : 0.0f;
+  }
+}
+
+__global__ __launch_bounds__(256) void mlp_forward_kernel(const MlpArgs A) {
+  __shared__ float W[kMlpParams];
+  load_weights(A.w_bf16, W);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= A.n_cells) return;
+  const int c = A.cells[t];
+  float x[kMlpIn], h1[kMlpH], h2[kMlpH], z[kMlpOut];
+  gather_features(A.p, A.vx, A.vz, A.nx, A.nz, A.H, c, x);
+  mlp_fwd(W, x, h1, h2, z);
+  A.alpha_v[c] = corrf(z[0]);
+  A.alpha_p[c] = corrf(z[1]);
+}
+
+// Per-chunk LDS layout for the weight-gradient GEMMs (32 cells per chunk, 64 threads).
+constexpr int kChunk = 32;
+constexpr int kTrainThreads = 64;
+constexpr int kGradW = kMlpParams + 1;  // gradients + loss
+
+struct ChunkLds {
+  float x[kChunk][kMlpIn];
+  float h1[kChunk][kMlpH];
+  float h2[kChunk][kMlpH];
+  float gh1[kChunk][kMlpH];
+  float gh2[kChunk][kMlpH];
+  float gz[kChunk][kMlpOut];
+  float loss[kChunk];
+};
+
+// Accumulate gradient entry e over the chunk's cells in a fixed order.
+__device__ __forceinline__ float grad_entry(const ChunkLds& L, int e) {
+  float acc = 0.0f;
+  if (e < kOffB1) {
+    const int o = e / kMlpIn, k = e - o * kMlpIn;
+    for (int q = 0; q < kChunk; ++q) acc += L.gh1[q][o] * L.x[q][k];
+  } else if (e < kOffW2) {
+    const int o = e - kOffB1;
+    for (int q = 0; q < kChunk; ++q) acc += L.gh1[q][o];
+  } else if (e < kOffB2) {
+    const int m = (e - kOffW2) / kMlpH, o = (e - kOffW2) - m * kMlpH;
+    for (int q = 0; q < kChunk; ++q) acc += L.gh2[q][m] * L.h1[q][o];
+  } else if (e < kOffW3) {
+    const int m = e - kOffB2;
+    for (int q = 0; q < kChunk; ++q) acc += L.gh2[q][m];
+  } else if (e < kOffB3) {
+    const int k = (e - kOffW3) / kMlpH, o = (e - kOffW3) - k * kMlpH;
+    for (int q = 0; q < kChunk; ++q) acc += L.gz[q][k] * L.h2[q][o];
+  } else if (e < kMlpParams) {
+    const int k = e - kOffB3;
+    for (int q = 0; q < kChunk; ++q) acc += L.gz[q][k];
+  } else if (e == kMlpParams) {
+    for (int q = 0; q < kChunk; ++q) acc += L.loss[q];
+  }
+  return acc;
+}
+
+// One block walks chunks blockIdx.x, blockIdx.x + gridDim.x, ... of 32 cells (threads 0..31 take
+// one cell each), then all 64 threads form the weight-gradient sums of the chunk in a fixed order.
+// The per-block partial is accumulated into partial[blockIdx.x] (the host clears it).
+template <bool FROM_FIELDS>
+__device__ void train_blocks(const MlpTrainArgs* T, const void* w_bf16, const float* feats, const float* galpha,
+                             int n, float* alpha_out, float* partial) {
+  __shared__ float W[kMlpParams];
+  __shared__ ChunkLds L;
+  __shared__ float G[kGradW];
+  for (int e = threadIdx.x; e < kGradW; e += blockDim.x) G[e] = 0.0f;
+  load_weights(w_bf16, W);
+  const int nchunks = (n + kChunk - 1) / kChunk;
+  for (int ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const int q = threadIdx.x;
+    if (q < kChunk) {
+      const int t = ch * kChunk + q;
+      float x[kMlpIn], h1[kMlpH], h2[kMlpH], z[kMlpOut], al[kMlpOut], ga[kMlpOut] = {0.0f, 0.0f};
+      float gz[kMlpOut] = {0.0f, 0.0f}, gh1[kMlpH], gh2[kMlpH];
+      float loss = 0.0f;
+      if (t < n) {
+        if (FROM_FIELDS) {
+          const MlpArgs& A = T->m;
+          const int c = A.cells[t];
+          gather_features(A.p, A.vx, A.vz, A.nx, A.nz, A.H, c, x);
+          const int i = c % A.nx, j = c / A.nx;
+          const size_t tc = (size_t)(i + T->off_x) + (size_t)(j + T->off_z) * (size_t)T->tnx;
+          const float rvx = (T->vx1[c] - T->tvx[tc]) * T->inv_scale_v;
+          const float rvz = (T->vz1[c] - T->tvz[tc]) * T->inv_scale_v;
+          const float rp = (T->p1[c] - T->tp[tc]) * T->inv_scale_p;
+          ga[0] = 2.0f * T->inv_scale_v * (rvx * T->sens_vx[c] + rvz * T->sens_vz[c]);
+          ga[1] = 2.0f * T->inv_scale_p * rp * T->sens_p[c];
+          loss = rvx * rvx + rvz * rvz + rp * rp;
+        } else {
+#pragma unroll
+          for (int k = 0; k < kMlpIn; ++k) x[k] = feats[(size_t)t * kMlpIn + k];
+          ga[0] = galpha[2 * (size_t)t];
+          ga[1] = galpha[2 * (size_t)t + 1];
+        }
+        mlp_fwd(W, x, h1, h2, z);
+        al[0] = corrf(z[0]);
+        al[1] = corrf(z[1]);
+        if (!FROM_FIELDS && alpha_out) {
+          alpha_out[2 * (size_t)t] = al[0];
+          alpha_out[2 * (size_t)t + 1] = al[1];
+        }
+        mlp_bwd(W, h1, h2, al, ga, gz, gh2, gh1);
+      } else {
+#pragma unroll
+        for (int k = 0; k < kMlpIn; ++k) x[k] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < kMlpH; ++k) h1[k] = h2[k] = gh1[k] = gh2[k] = 0.0f;
+      }
+#pragma unroll
+      for (int k = 0; k < kMlpIn; ++k) L.x[q][k] = x[k];
+#pragma unroll
+      for (int k = 0; k < kMlpH; ++k) {
+        L.h1[q][k] = h1[k];
+        L.h2[q][k] = h2[k];
+        L.gh1[q][k] = gh1[k];
+        L.gh2[q][k] = gh2[k];
+      }
+      L.gz[q][0] = gz[0];
+      L.gz[q][1] = gz[1];
+      L.loss[q] = loss;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kGradW; e += blockDim.x) G[e] += grad_entry(L, e);
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < kGradW; e += blockDim.x) partial[(size_t)blockIdx.x * kGradW + e] += G[e];
+}
+
+__global__ __launch_bounds__(64) void mlp_train_kernel(const MlpTrainArgs T) {
+  train_blocks<true>(&T, T.m.w_bf16, nullptr, nullptr, T.m.n_cells, nullptr, T.partial);
+}
+
+__global__ __launch_bounds__(64) void mlp_eval_kernel(const void* w, const float* feats, const float* galpha, int n,
+                                                      float* alpha, float* partial) {
+  train_blocks<false>(nullptr, w, feats, galpha, n, alpha, partial);
+}
+
+__global__ void reduce_partials_kernel(const float* partial, int blocks, int width, float* out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= width) return;
+  float acc = 0.0f;
+  for (int b = 0; b < blocks; ++b) acc += partial[(size_t)b * width + e];
+  out[e] = acc;
+}
+
+hipError_t launch_mlp_forward(const MlpArgs& a, hipStream_t s) {
+  if (a.n_cells <= 0) return hipSuccess;
+  hipLaunchKernelGGL(mlp_forward_kernel, dim3((a.n_cells + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_train(const MlpTrainArgs& a, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(mlp_train_kernel, dim3(blocks), dim3(kTrainThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_partials(const float* partial, int blocks, int width, float* out, hipStream_t s) {
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3((width + 255) / 256), dim3(256), 0, s, partial, blocks, width, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_eval(const void* w_bf16, const float* feats, const float* galpha, int n, float* alpha,
+                           float* partial, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(mlp_eval_kernel, dim3(blocks), dim3(kTrainThreads), 0, s, w_bf16, feats, galpha, n, alpha, partial);
+  return hipGetLastError();
+}
+
+}  // namespace fomo

@@ -27,6 +27,16 @@ struct Plan2D : PlanBase {
   int* rec_stride = nullptr;
   float* seis[3] = {nullptr};
   long long seis_total = 0;
+  // learned boundary (shot 0 of an acoustic plan)
+  bool learned = false, training = false;
+  int* d_cells = nullptr;
+  int n_cells = 0;
+  void* w_bf16 = nullptr;
+  float* alpha[2] = {nullptr, nullptr};
+  float* sens[3] = {nullptr, nullptr, nullptr};
+  float* partial = nullptr;
+  float* grad = nullptr;
+  int train_blocks = 0;
 
   ~Plan2D() override;
   hipError_t enqueue_step(int s, hipEvent_t* ev) override;

@@ -11,6 +11,7 @@
 #include "fomo_common.h"
 #include "plan.h"
 #include "plan2d.h"
+#include "learned2d.h"
 
 namespace fomo {
 
@@ -96,6 +97,22 @@ hipError_t Plan2D::enqueue_step(int s, hipEvent_t* ev) {
   const int ph_s = d.eq == FOMO_ACOUSTIC2D ? kAcPrs : kElStr;
   hipError_t e = hipSuccess;
   if (ev) e = hipEventRecord(ev[0], stream);
+  if (e == hipSuccess && learned) {
+    // blend weights of every frame cell from the state at time n
+    MlpArgs m{};
+    m.w_bf16 = w_bf16;
+    m.cells = d_cells;
+    m.n_cells = n_cells;
+    m.nx = shots[0].nx;
+    m.nz = shots[0].nz;
+    m.H = shots[0].H;
+    m.p = shots[0].fld[s][0];
+    m.vx = shots[0].fld[s][1];
+    m.vz = shots[0].fld[s][2];
+    m.alpha_v = alpha[0];
+    m.alpha_p = alpha[1];
+    e = launch_mlp_forward(m, stream);
+  }
   if (e == hipSuccess) e = launch_sweep2d(ph_v, N, tile_cfg, a, stream);
   if (e == hipSuccess && ev) e = hipEventRecord(ev[1], stream);
   if (e == hipSuccess) e = launch_sweep2d(ph_s, N, tile_cfg, a, stream);

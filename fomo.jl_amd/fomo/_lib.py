@@ -85,6 +85,13 @@ SIGNATURES = {
     "fomo_plan_read_field": (C.c_int, [_p, _i, _p]),
     "fomo_plan_dims": (C.c_int, [_p, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)]),
     "fomo_plan_destroy": (C.c_int, [_p]),
+    "fomo_learned_n_params": (C.c_int, []),
+    "fomo_learned_enable": (C.c_int, [_p, _p, _i]),
+    "fomo_learned_set_weights": (C.c_int, [_p, _p]),
+    "fomo_learned_train_step": (C.c_int, [_p, _p, _i, _i, _f, _f]),
+    "fomo_learned_grad": (C.c_int, [_p, _p, C.POINTER(C.c_double)]),
+    "fomo_learned_alpha": (C.c_int, [_p, C.POINTER(_i), _p, _p, _p]),
+    "fomo_mlp_eval": (C.c_int, [_p, _i, _p, _p, _p, _p]),
 }
 
 _lib = None

@@ -1,0 +1,210 @@
+"""Learned thin absorbing boundary for acoustic2d: initialisation, training and evaluation.
+
+The network (76 -> 32 -> 32 -> 2) predicts, for every frame cell and step, bounded corrections
+(0.25 tanh) to the Higdon blend weights of the velocity fields and of p, from max-abs-scaled 5x5
+patches of p, vx, vz and the depth in the layer; the blend weight is clamp(ramp + correction, 0, 1).
+Training data come from the simulator itself: a *student* run on the model with a
+thin boundary (nbc = 16) and a *teacher* run on the same model edge-extended by `extend` cells with
+a wide boundary (nbc = 200) advance together; at every step the one-step loss at the student's
+frame cells against the teacher's field at the same physical cells is differentiated through the
+MLP by the engine's own forward/backward kernels (fomo_learned_train_step), and Adam updates the
+weights on the host.  The plain distance-ramp HABC stays the default, the initial state of the
+network and the yardstick of reflection error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+from . import _lib
+from .api import Plan
+
+N_IN, N_H, N_OUT = 76, 32, 2
+N_PARAMS = N_H * N_IN + N_H + N_H * N_H + N_H + N_OUT * N_H + N_OUT
+WEIGHTS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "learned_weights.npy")
+
+
+def _split(w):
+    o = 0
+    out = []
+    for shape in ((N_H, N_IN), (N_H,), (N_H, N_H), (N_H,), (N_OUT, N_H), (N_OUT,)):
+        n = int(np.prod(shape))
+        out.append(w[o:o + n].reshape(shape))
+        o += n
+    return out
+
+
+def initial_weights(seed: int = 0) -> np.ndarray:
+    """Random hidden layers and zero output layer: the corrections start at exactly 0, so the
+    learned boundary starts as the plain distance-ramp Higdon boundary (habc.jl:30-34)."""
+    rng = np.random.default_rng(seed)
+    W1 = rng.normal(0, 1.0 / math.sqrt(N_IN), (N_H, N_IN))
+    b1 = np.full(N_H, 0.05)
+    W2 = rng.normal(0, 1.0 / math.sqrt(N_H), (N_H, N_H))
+    b2 = np.full(N_H, 0.05)
+    W3 = np.zeros((N_OUT, N_H))
+    b3 = np.zeros(N_OUT)
+    return np.concatenate([W1.ravel(), b1, W2.ravel(), b2, W3.ravel(), b3]).astype(np.float32)
+
+
+def load_weights() -> np.ndarray:
+    """The trained weights shipped with the package."""
+    if not os.path.exists(WEIGHTS_PATH):
+        raise FileNotFoundError(f"{WEIGHTS_PATH} is missing: train with fomo.learned.train() first")
+    w = np.load(WEIGHTS_PATH, allow_pickle=False).astype(np.float32)
+    if w.shape != (N_PARAMS,):
+        raise ValueError(f"learned weights have shape {w.shape}, expected ({N_PARAMS},)")
+    return w
+
+
+def enable(plan: Plan, weights, train: bool = False) -> None:
+    w = np.ascontiguousarray(np.asarray(weights, np.float32))
+    _lib.check(_lib.lib().fomo_learned_enable(plan._h, w.ctypes.data, 1 if train else 0))
+
+
+def set_weights(plan: Plan, weights) -> None:
+    w = np.ascontiguousarray(np.asarray(weights, np.float32))
+    _lib.check(_lib.lib().fomo_learned_set_weights(plan._h, w.ctypes.data))
+
+
+def alphas(plan: Plan):
+    n = C.c_int32()
+    _lib.check(_lib.lib().fomo_learned_alpha(plan._h, C.byref(n), None, None, None))
+    cells = np.zeros(n.value, np.int32)
+    av = np.zeros(n.value, np.float32)
+    ap = np.zeros(n.value, np.float32)
+    _lib.check(_lib.lib().fomo_learned_alpha(plan._h, C.byref(n), cells.ctypes.data, av.ctypes.data,
+                                             ap.ctypes.data))
+    return cells, av, ap
+
+
+def mlp_eval(weights, feats, galpha):
+    """GPU forward/backward of the MLP on given features: (alpha (n,2), summed gradient)."""
+    w = np.ascontiguousarray(np.asarray(weights, np.float32))
+    f = np.ascontiguousarray(np.asarray(feats, np.float32))
+    g = np.ascontiguousarray(np.asarray(galpha, np.float32))
+    n = f.shape[0]
+    alpha = np.zeros((n, 2), np.float32)
+    grad = np.zeros(N_PARAMS, np.float32)
+    _lib.check(_lib.lib().fomo_mlp_eval(w.ctypes.data, n, f.ctypes.data, g.ctypes.data, alpha.ctypes.data,
+                                        grad.ctypes.data))
+    return alpha, grad
+
+
+def edge_extend(m, L):
+    ix = np.clip(np.arange(1, m.shape[0] + 2 * L + 1) - L, 1, m.shape[0]) - 1
+    iz = np.clip(np.arange(1, m.shape[1] + 2 * L + 1) - L, 1, m.shape[1]) - 1
+    return np.asfortranarray(m[np.ix_(ix, iz)])
+
+
+class Adam:
+    def __init__(self, w, lr=2e-3, b1=0.9, b2=0.999, eps=1e-8):
+        self.w = np.asarray(w, np.float64).copy()
+        self.m = np.zeros_like(self.w)
+        self.v = np.zeros_like(self.w)
+        self.t = 0
+        self.lr, self.b1, self.b2, self.eps = lr, b1, b2, eps
+
+    def step(self, g):
+        self.t += 1
+        self.m = self.b1 * self.m + (1 - self.b1) * g
+        self.v = self.b2 * self.v + (1 - self.b2) * g * g
+        mh = self.m / (1 - self.b1 ** self.t)
+        vh = self.v / (1 - self.b2 ** self.t)
+        self.w -= self.lr * mh / (np.sqrt(vh) + self.eps)
+        return self.w.astype(np.float32)
+
+
+def training_model(seed: int, nx: int, nz: int):
+    from .models import layered_2d
+    vp, vs, rho = layered_2d(nx, nz, seed=seed, n_interfaces=4, jitter=0.08 * nz, vacuum_top=False)
+    return vp, rho
+
+
+def train(n_models: int = 512, nx: int = 240, nz: int = 160, nt: int = 500, nbc: int = 16,
+          teacher_nbc: int = 200, extend: int = 60, update_every: int = 25, lr: float = 5e-4, seed: int = 1,
+          weights=None, log=None, clip: float = 1.0):
+    """Train the learned boundary on `n_models` seeded random layered models.  Returns (weights,
+    history of per-model mean losses)."""
+    dh, dt, f0 = 10.0, 8e-4, 15.0
+    w = initial_weights() if weights is None else np.asarray(weights, np.float32)
+    opt = Adam(w, lr=lr)
+    from .setup import ricker_wavelet
+    wav = ricker_wavelet(f0, dt, nt)
+    scale_p = float(np.abs(wav).max())
+    history = []
+    g = np.zeros(N_PARAMS, np.float32)
+    for m in range(n_models):
+        vp, rho = training_model(seed * 100003 + m, nx, nz)
+        rng = np.random.default_rng(seed * 7919 + m)
+        sx = int(rng.integers(nx // 5, 4 * nx // 5))
+        sz = int(rng.integers(nz // 5, 4 * nz // 5))
+        z_min = float((rho * vp).min())
+        scale_v = scale_p / z_min
+        rx = np.arange(1, nx + 1)
+        rz = np.full(nx, 1)
+        stu = Plan(_lib.ACOUSTIC2D, vp, rho, dh, dt, nt, sx=[sx], sz=[sz], rx=rx, rz=rz, wavelet=wav[None],
+                   nbc=nbc, fd_order=8, use_graph=False, tile_cfg=2)
+        tea = Plan(_lib.ACOUSTIC2D, edge_extend(vp, extend), edge_extend(rho, extend), dh, dt, nt,
+                   sx=[sx + extend], sz=[sz + extend], rx=rx + extend, rz=rz + extend, wavelet=wav[None],
+                   nbc=teacher_nbc, fd_order=8, use_graph=False, v_ref=float(vp.min()), tile_cfg=2)
+        enable(stu, w, train=True)
+        off = extend + tea.pad - stu.pad
+        losses = []
+        for t in range(nt):
+            _lib.check(_lib.lib().fomo_learned_train_step(stu._h, tea._h, off, off, scale_p, scale_v))
+            if (t + 1) % update_every == 0:
+                loss = C.c_double(0.0)
+                _lib.check(_lib.lib().fomo_learned_grad(stu._h, g.ctypes.data, C.byref(loss)))
+                gn = g.astype(np.float64) / update_every
+                norm = float(np.linalg.norm(gn))
+                if norm > clip:
+                    gn *= clip / norm
+                w = opt.step(gn)
+                set_weights(stu, w)
+                losses.append(loss.value / update_every)
+        stu.close()
+        tea.close()
+        history.append(float(np.mean(losses)) if losses else 0.0)
+        if log and (m % 16 == 0 or m == n_models - 1):
+            log(f"model {m}: mean one-step loss {history[-1]:.4e}")
+    return w, history
+
+
+def reflection_db(nx, nz, nt, vp, rho, sx, sz, rx, rz, *, boundaries, dh=10.0, dt=8e-4, f0=15.0, extend=None):
+    """Reflection energy 10 log10(|u - u_ref|^2 / |u_ref|^2) of seis_p at the receivers for each
+    boundary in `boundaries` ((name, nbc, weights-or-None) tuples), with u_ref from a boundary-free
+    domain edge-extended by `extend` cells (the reflection of its own boundary arrives after the
+    record ends), as in test/verify_patch.jl:233-249.  Also returns per-trace relative L2."""
+    from .setup import ricker_wavelet
+    wav = ricker_wavelet(f0, dt, nt)
+    if extend is None:
+        vmax = float(vp.max())
+        extend = int(math.ceil(vmax * nt * dt / dh / 2.0)) + 20
+    ref = Plan(_lib.ACOUSTIC2D, edge_extend(vp, extend), edge_extend(rho, extend), dh, dt, nt, sx=[sx + extend],
+               sz=[sz + extend], rx=np.asarray(rx) + extend, rz=np.asarray(rz) + extend, wavelet=wav[None], nbc=16,
+               fd_order=8, v_ref=float(vp.min()))
+    ref.advance(nt)
+    u_ref = ref.seis(3)[0].astype(np.float64)
+    ref.close()
+    out = {}
+    for name, nbc, weights in boundaries:
+        p = Plan(_lib.ACOUSTIC2D, vp, rho, dh, dt, nt, sx=[sx], sz=[sz], rx=rx, rz=rz, wavelet=wav[None], nbc=nbc,
+                 fd_order=8)
+        if weights is not None:
+            enable(p, weights)
+        ms = p.timed_advance(nt)
+        u = p.seis(3)[0].astype(np.float64)
+        cells = p.padded_cells
+        p.close()
+        err = np.linalg.norm(u - u_ref) ** 2 / np.linalg.norm(u_ref) ** 2
+        den = np.linalg.norm(u_ref, axis=1)
+        ok = den > 1e-6 * den.max()
+        tr = np.linalg.norm(u - u_ref, axis=1)[ok] / den[ok]
+        out[name] = {"reflection_db": 10.0 * math.log10(max(err, 1e-300)), "trace_rel_l2_median": float(np.median(tr)),
+                     "trace_rel_l2_max": float(tr.max()), "ms": ms, "padded_cells": cells,
+                     "shots_per_s": 1000.0 / ms}
+    return out

@@ -193,6 +193,31 @@ int fomo_plan_read_field(fomo_plan *p, int32_t which, float *out);
 int fomo_plan_dims(const fomo_plan *p, int32_t *nx, int32_t *ny, int32_t *nz, int32_t *pad);
 int fomo_plan_destroy(fomo_plan *p);
 
+/* ---- learned absorbing boundary (extension; acoustic2d single-shot plans) ----------------- */
+/* The Higdon blend weight of every frame cell comes from a small MLP run per frame cell
+ * (76 inputs: max-abs-scaled 5x5 patches of p, vx, vz and dist/H; 32-32 ReLU; 2 sigmoid outputs:
+ * the weights of the velocity fields and of p).  Weights are stored as BFloat16, arithmetic is
+ * Float32.  The reference has no counterpart; its distance-ramp weights (habc.jl:30-34) remain
+ * the default and the fallback. */
+int fomo_learned_n_params(void);
+/* Attach the learned boundary to plan p with weights (fomo_learned_n_params() floats); train != 0
+ * also allocates the training buffers. */
+int fomo_learned_enable(fomo_plan *p, const float *weights, int32_t train);
+int fomo_learned_set_weights(fomo_plan *p, const float *weights);
+/* Advance teacher and student one step each and accumulate the gradient of the one-step loss
+ * sum(((f_student - f_teacher)/scale)^2) over the student's frame cells (student cell (i,j) =
+ * teacher cell (i+off_x, j+off_z)).  Asynchronous. */
+int fomo_learned_train_step(fomo_plan *student, fomo_plan *teacher, int32_t off_x, int32_t off_z, float scale_p,
+                            float scale_v);
+/* Accumulated weight gradients (n_params floats) and loss since the last call; clears them. */
+int fomo_learned_grad(fomo_plan *p, float *grad, double *loss);
+/* Frame cells (padded indices) and their current blend weights. */
+int fomo_learned_alpha(fomo_plan *p, int32_t *n, int32_t *cells, float *alpha_v, float *alpha_p);
+/* MLP forward + backward on given features (n x 76, row-major) and dL/dalpha (n x 2): writes
+ * alpha (n x 2) and the summed weight gradients (for gradient checks). */
+int fomo_mlp_eval(const float *weights, int32_t n, const float *feats, const float *galpha, float *alpha,
+                  float *grad);
+
 #ifdef __cplusplus
 }
 #endif

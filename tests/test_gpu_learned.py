@@ -1,0 +1,84 @@
+"""GPU learned boundary: the MLP kernels against the numpy oracle (forward, gradients, feature
+gather on a live wavefield), and the learned plan's basic behaviour."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mlp_kernels_match_oracle(gpu):
+    import mlp
+    from fomo.learned import mlp_eval
+    rng = np.random.default_rng(5)
+    w = rng.normal(0, 0.25, mlp.N_PARAMS).astype(np.float32)
+    n = 203  # ragged: not a multiple of the 32-cell chunk
+    x = rng.normal(0, 1, (n, mlp.N_IN)).astype(np.float32)
+    x[:, 75] = rng.uniform(0, 1, n)
+    ga = rng.normal(0, 1, (n, 2)).astype(np.float32)
+    alpha, grad = mlp_eval(w, x, ga)
+    wb = mlp.bf16_round(w)
+    a_ref, cache = mlp.forward(wb, x.astype(np.float64))
+    g_ref = mlp.backward(wb, cache, ga.astype(np.float64))
+    assert np.allclose(alpha, a_ref, rtol=2e-5, atol=2e-6)
+    scale = np.abs(g_ref).max()
+    assert np.abs(grad - g_ref).max() <= 2e-5 * scale
+    # determinism of the reduction
+    alpha2, grad2 = mlp_eval(w, x, ga)
+    assert np.array_equal(grad, grad2) and np.array_equal(alpha, alpha2)
+
+
+def test_learned_inference_on_live_wavefield(gpu):
+    """Blend-weight corrections computed by the engine from the state at time n equal the oracle's
+    MLP on features gathered from the same state."""
+    import mlp
+    import fomo
+    import fomo._lib as L
+    from fomo.api import Plan
+    from fomo.learned import alphas, enable, initial_weights
+    nx, nz, nt = 120, 90, 80
+    vp = np.full((nx, nz), 2500.0, np.float32)
+    vp[:, 45:] = 3000.0
+    rho = np.full((nx, nz), 2100.0, np.float32)
+    w = fomo.ricker_wavelet(20.0, 1e-3, nt)
+    p = Plan(L.ACOUSTIC2D, vp, rho, 10.0, 1e-3, nt, sx=[60], sz=[20], rx=[1], rz=[1], wavelet=w[None], nbc=16,
+             fd_order=8, use_graph=False)
+    rng = np.random.default_rng(2)
+    weights = initial_weights(4)
+    weights[-66:] = rng.normal(0, 0.5, 66).astype(np.float32)  # non-zero output layer
+    enable(p, weights)
+    p.advance(60)
+    p.sync()
+    state = [p.field(k) for k in range(3)]  # p, vx, vz at time 60
+    p.advance(1)
+    cells, av, ap = alphas(p)
+    x = mlp.features(state[0], state[1], state[2], cells, 19)
+    a_ref, _ = mlp.forward(mlp.bf16_round(weights), x)
+    assert np.allclose(av, a_ref[:, 0], rtol=1e-4, atol=1e-5)
+    assert np.allclose(ap, a_ref[:, 1], rtol=1e-4, atol=1e-5)
+    p.close()
+
+
+def test_zero_correction_is_plain_habc_bitwise(gpu):
+    """With the initial weights (zero output layer) the learned boundary reproduces the plain
+    HABC bit for bit: clamp(w + 0, 0, 1) = w."""
+    import fomo
+    import fomo._lib as L
+    from fomo.api import Plan
+    from fomo.learned import enable, initial_weights
+    nx, nz, nt = 200, 140, 300
+    vp = np.full((nx, nz), 2000.0, np.float32)
+    vp[:, 70:] = 2600.0
+    rho = np.full((nx, nz), 2000.0, np.float32)
+    w = fomo.ricker_wavelet(15.0, 8e-4, nt)
+    rx = np.arange(1, nx + 1, 3)
+    out = []
+    for learned in (False, True):
+        p = Plan(L.ACOUSTIC2D, vp, rho, 10.0, 8e-4, nt, sx=[100], sz=[30], rx=rx, rz=np.full(len(rx), 2),
+                 wavelet=w[None], nbc=16, fd_order=8)
+        if learned:
+            enable(p, initial_weights(7))
+        p.advance(nt)
+        out.append(p.seis(3) + [p.field(k) for k in range(3)])
+        p.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
