@@ -32,8 +32,12 @@ struct PlanBase {
   long steps_done = 0;
 
   virtual ~PlanBase();
-  // Enqueue one step from state parity s; ev (4 events, optional) brackets the launches.
-  virtual hipError_t enqueue_step(int s, hipEvent_t* ev) = 0;
+  // Enqueue one part of a step from state parity s: 0 velocity phase, 1 pressure/stress phase,
+  // 2 receiver gather (which also bumps the device step counter).  Parts 0 and 1 read only the
+  // parity-s buffers and the step counter besides what they write, so repeating one is idempotent.
+  virtual hipError_t enqueue_part(int part, int s) = 0;
+  // Enqueue one step; ev (4 events, optional) brackets the three parts.
+  hipError_t enqueue_step(int s, hipEvent_t* ev);
   virtual int reset() = 0;
   virtual int read_seis(int shot, float* const* dst) = 0;
   virtual int read_field(int shot, int which, float* out) = 0;

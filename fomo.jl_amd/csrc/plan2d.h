@@ -39,7 +39,7 @@ struct Plan2D : PlanBase {
   int train_blocks = 0;
 
   ~Plan2D() override;
-  hipError_t enqueue_step(int s, hipEvent_t* ev) override;
+  hipError_t enqueue_part(int part, int s) override;
   int reset() override;
   int read_seis(int shot, float* const* dst) override;
   int read_field(int shot, int which, float* out) override;

@@ -68,7 +68,26 @@ int Plan2D::make_work(int cfg) {
   return upload(&d_work[cfg], flat.data(), flat.size());
 }
 
-hipError_t Plan2D::enqueue_step(int s, hipEvent_t* ev) {
+hipError_t Plan2D::enqueue_part(int part, int s) {
+  if (part == 2) {
+    GatherArgs g{};
+    g.shots = d_shots;
+    g.rec_shot = rec_shot;
+    g.rec_c = rec_c;
+    g.rec_out = rec_out;
+    g.rec_stride = rec_stride;
+    g.nslots = nslots;
+    for (int k = 0; k < nslots; ++k) {
+      g.seis[k] = seis[k];
+      g.field_of_slot[k] = k;  // acoustic p,vx,vz = fields 0,1,2; elastic vx,vz = fields 0,1
+    }
+    g.n_entries = n_entries;
+    g.parity_out = 1 - s;
+    g.nt = d.nt;
+    g.step = step;
+    g.done = done;
+    return launch_gather2d(g, stream);
+  }
   SweepArgs a{};
   a.shots = d_shots;
   a.work = d_work[tile_cfg];
@@ -77,28 +96,9 @@ hipError_t Plan2D::enqueue_step(int s, hipEvent_t* ev) {
   for (int l = 0; l < N; ++l) a.a[l] = coef[l];
   a.step = step;
   a.nt = d.nt;
-  GatherArgs g{};
-  g.shots = d_shots;
-  g.rec_shot = rec_shot;
-  g.rec_c = rec_c;
-  g.rec_out = rec_out;
-  g.rec_stride = rec_stride;
-  g.nslots = nslots;
-  for (int k = 0; k < nslots; ++k) {
-    g.seis[k] = seis[k];
-    g.field_of_slot[k] = k;  // acoustic p,vx,vz = fields 0,1,2; elastic vx,vz = fields 0,1
-  }
-  g.n_entries = n_entries;
-  g.parity_out = 1 - s;
-  g.nt = d.nt;
-  g.step = step;
-  g.done = done;
-  const int ph_v = d.eq == FOMO_ACOUSTIC2D ? kAcVel : kElVel;
-  const int ph_s = d.eq == FOMO_ACOUSTIC2D ? kAcPrs : kElStr;
-  hipError_t e = hipSuccess;
-  if (ev) e = hipEventRecord(ev[0], stream);
-  if (e == hipSuccess && learned) {
-    // blend weights of every frame cell from the state at time n
+  if (part == 1) return launch_sweep2d(d.eq == FOMO_ACOUSTIC2D ? kAcPrs : kElStr, N, tile_cfg, a, stream);
+  if (learned) {
+    // blend-weight corrections of every frame cell from the state at time n
     MlpArgs m{};
     m.w_bf16 = w_bf16;
     m.cells = d_cells;
@@ -111,15 +111,10 @@ hipError_t Plan2D::enqueue_step(int s, hipEvent_t* ev) {
     m.vz = shots[0].fld[s][2];
     m.alpha_v = alpha[0];
     m.alpha_p = alpha[1];
-    e = launch_mlp_forward(m, stream);
+    const hipError_t e = launch_mlp_forward(m, stream);
+    if (e != hipSuccess) return e;
   }
-  if (e == hipSuccess) e = launch_sweep2d(ph_v, N, tile_cfg, a, stream);
-  if (e == hipSuccess && ev) e = hipEventRecord(ev[1], stream);
-  if (e == hipSuccess) e = launch_sweep2d(ph_s, N, tile_cfg, a, stream);
-  if (e == hipSuccess && ev) e = hipEventRecord(ev[2], stream);
-  if (e == hipSuccess) e = launch_gather2d(g, stream);
-  if (e == hipSuccess && ev) e = hipEventRecord(ev[3], stream);
-  return e;
+  return launch_sweep2d(d.eq == FOMO_ACOUSTIC2D ? kAcVel : kElVel, N, tile_cfg, a, stream);
 }
 
 int Plan2D::reset() {

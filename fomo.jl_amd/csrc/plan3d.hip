@@ -45,7 +45,24 @@ struct Plan3D : PlanBase {
   float* scratch = nullptr;  // slice / decode staging
   size_t scratch_n = 0;
 
-  hipError_t enqueue_step(int s, hipEvent_t* ev) override {
+  hipError_t enqueue_part(int part, int s) override {
+    if (part == 2) {
+      Gather3DArgs g{};
+      g.shot = d_shot;
+      g.rec_c = rec_c;
+      const int fields[4] = {1, 2, 3, 0};
+      for (int k = 0; k < nslots; ++k) {
+        g.seis[k] = seis[k];
+        g.slot_field[k] = fields[k];
+      }
+      g.nslots = nslots;
+      g.n_rec = d.n_rec;
+      g.parity_out = 1 - s;
+      g.nt = d.nt;
+      g.step = step;
+      g.done = done;
+      return launch_gather3d(storage, g, stream);
+    }
     Sweep3DArgs a{};
     a.shot = d_shot;
     a.work = d_work;
@@ -54,29 +71,7 @@ struct Plan3D : PlanBase {
     for (int l = 0; l < N; ++l) a.a[l] = coef[l];
     a.step = step;
     a.nt = d.nt;
-    Gather3DArgs g{};
-    g.shot = d_shot;
-    g.rec_c = rec_c;
-    const int fields[4] = {1, 2, 3, 0};
-    for (int k = 0; k < nslots; ++k) {
-      g.seis[k] = seis[k];
-      g.slot_field[k] = fields[k];
-    }
-    g.nslots = nslots;
-    g.n_rec = d.n_rec;
-    g.parity_out = 1 - s;
-    g.nt = d.nt;
-    g.step = step;
-    g.done = done;
-    hipError_t e = hipSuccess;
-    if (ev) e = hipEventRecord(ev[0], stream);
-    if (e == hipSuccess) e = launch_sweep3d(storage, 0, N, a, stream);
-    if (e == hipSuccess && ev) e = hipEventRecord(ev[1], stream);
-    if (e == hipSuccess) e = launch_sweep3d(storage, 1, N, a, stream);
-    if (e == hipSuccess && ev) e = hipEventRecord(ev[2], stream);
-    if (e == hipSuccess) e = launch_gather3d(storage, g, stream);
-    if (e == hipSuccess && ev) e = hipEventRecord(ev[3], stream);
-    return e;
+    return launch_sweep3d(storage, part, N, a, stream);
   }
 
   int reset() override {

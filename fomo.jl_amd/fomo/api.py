@@ -380,7 +380,8 @@ class Plan:
         _lib.check(_lib.lib().fomo_plan_sync(self._h))
 
     def kernel_times(self, n: int) -> tuple[float, float, float]:
-        """Average ms per launch of (velocity sweep, pressure/stress sweep, gather) over n steps."""
+        """Average ms per launch of (velocity sweep, pressure/stress sweep, gather): each sweep over
+        n back-to-back launches, the gather over n ordinary steps (the state advances n steps)."""
         out = (C.c_double * 3)()
         _lib.check(_lib.lib().fomo_plan_kernel_times(self._h, int(n), out))
         return out[0], out[1], out[2]

@@ -169,7 +169,7 @@ def train(n_models: int = 512, nx: int = 240, nz: int = 160, nt: int = 500, nbc:
         stu.close()
         tea.close()
         history.append(float(np.mean(losses)) if losses else 0.0)
-        if log and (m % 16 == 0 or m == n_models - 1):
+        if log and (m % max(1, n_models // 16) == 0 or m == n_models - 1):
             log(f"model {m}: mean one-step loss {history[-1]:.4e}")
     return w, history
 
