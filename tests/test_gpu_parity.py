@@ -134,7 +134,8 @@ def test_snapshots_match_oracle(gpu, orc):
 
 
 def test_edge_cases(gpu, orc):
-    """Thin boundary (nbc=0), sources and receivers on the interior's extreme cells, no receivers."""
+    """Thin boundary (nbc=0), sources and receivers on the interior's extreme cells, no receivers,
+    duplicate source cells, nt = 1 and an odd step count."""
     import fomo
     nx, nz, nt = 65, 40, 90
     vp, vs, rho = small_model(nx, nz, seed=9, vacuum_top=False)
@@ -151,6 +152,13 @@ def test_edge_cases(gpu, orc):
     res = fomo.acoustic2d(vp, rho, 10.0, 1e-3, nt, 15.0, **kw2)
     ref = orc.acoustic2d(vp, rho, 10.0, 1e-3, nt, wavelet=np.stack([w, w]), **kw2)
     assert np.array_equal(res.seis_p, ref["seis_p"])
+    # one step, and an odd step count (the two-step graph plus a direct tail launch)
+    for n in (1, 7):
+        wn = fomo.ricker_wavelet(15.0, 1e-3, n)
+        kw3 = dict(sx=[30], sz=[20], rx=[30, 31], rz=[20, 21], nbc=5, fd_order=8)
+        res = fomo.acoustic2d(vp, rho, 10.0, 1e-3, n, 15.0, **kw3)
+        ref = orc.acoustic2d(vp, rho, 10.0, 1e-3, n, wavelet=wn[None], **kw3)
+        assert np.array_equal(res.seis_p, ref["seis_p"]) and res.seis_p.shape == (2, n)
 
 
 def test_custom_wavelet_equals_default_and_linearity(gpu):
