@@ -116,10 +116,13 @@ def probe3d(args):
     rx = np.arange(1, n + 1, 2)
     gx, gy = np.meshgrid(rx, rx, indexing="ij")
     res = {}
-    for storage, name in ((L.STORE_F32, "f32"), (L.STORE_F16, "f16"), (L.STORE_BF16, "bf16")):
+    codes = {"f32": L.STORE_F32, "f16": L.STORE_F16, "bf16": L.STORE_BF16}
+    for name in args.probe_storage.split(","):
+        storage = codes[name]
         eb = 4 if storage == L.STORE_F32 else 2
         plan = Plan(L.ACOUSTIC3D, vp, rho, 10.0, 8e-4, nt, sx=[n // 2], sy=[n // 2], sz=[1], rx=gx.ravel(),
-                    ry=gy.ravel(), rz=np.ones(gx.size, int), wavelet=w[None], nbc=32, fd_order=8, storage=storage)
+                    ry=gy.ravel(), rz=np.ones(gx.size, int), wavelet=w[None], nbc=32, fd_order=8, storage=storage,
+                    tile_cfg=args.probe_tile)
         cells = plan.padded_cells
         plan.advance(4)
         plan.sync()
@@ -145,6 +148,8 @@ def main():
     ap.add_argument("--probe-n", type=int, default=8192)
     ap.add_argument("--probe3d", action="store_true", help="acoustic3d 512^3 probe (f32/f16/bf16)")
     ap.add_argument("--probe-n3", type=int, default=512)
+    ap.add_argument("--probe-storage", default="f32,f16,bf16", help="storages of the 3D probe")
+    ap.add_argument("--probe-tile", type=int, default=0, help="3D z-segment choice (0 default, k: 16<<(k-1) planes)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (shots and slabs would shard "

@@ -27,13 +27,13 @@ namespace fomo {
 
 // Resolved pointers of one phase of one shot.
 struct PV {
-  const float* sten[3];
-  const float* old[3];
-  float* out[3];
-  const float* mat[3];
-  const float* lut[3];
-  const float* alpha[3];  // learned corrections of the distance-table weight (NULL: table only)
-  float* sens[3];         // f_pde - S at frame cells while training (NULL: not recorded)
+  gp<const float> sten[3];
+  gp<const float> old[3];
+  gp<float> out[3];
+  gp<const float> mat[3];
+  gp<const float> lut[3];
+  gp<const float> alpha[3];  // learned corrections of the distance-table weight (NULL: table only)
+  gp<float> sens[3];         // f_pde - S at frame cells while training (NULL: not recorded)
 };
 
 template <int PH>
@@ -46,47 +46,47 @@ __device__ __forceinline__ void make_view(const Shot2D& S, int s, PV& V) {
   }
   const int L = S.H + 1;
   if (PH == kAcVel) {
-    V.sten[0] = S.fld[s][0];
-    V.old[0] = S.fld[s][1];
-    V.old[1] = S.fld[s][2];
-    V.out[0] = S.fld[o][1];
-    V.out[1] = S.fld[o][2];
-    V.mat[0] = S.mat[1];
-    V.mat[1] = S.mat[2];
-    V.lut[0] = S.wlut;
-    V.lut[1] = S.wlut + L;
-    V.alpha[0] = V.alpha[1] = S.alpha[0];
-    V.sens[0] = S.sens[0];
-    V.sens[1] = S.sens[1];
+    V.sten[0] = G(S.fld[s][0]);
+    V.old[0] = G(S.fld[s][1]);
+    V.old[1] = G(S.fld[s][2]);
+    V.out[0] = G(S.fld[o][1]);
+    V.out[1] = G(S.fld[o][2]);
+    V.mat[0] = G(S.mat[1]);
+    V.mat[1] = G(S.mat[2]);
+    V.lut[0] = G(S.wlut);
+    V.lut[1] = G(S.wlut + L);
+    V.alpha[0] = V.alpha[1] = G((const float*)S.alpha[0]);
+    V.sens[0] = G(S.sens[0]);
+    V.sens[1] = G(S.sens[1]);
   } else if (PH == kAcPrs) {
-    V.sten[0] = S.fld[o][1];
-    V.sten[1] = S.fld[o][2];
-    V.old[0] = S.fld[s][0];
-    V.out[0] = S.fld[o][0];
-    V.mat[0] = S.mat[0];
-    V.lut[0] = S.wlut + 3 * L;
-    V.alpha[0] = S.alpha[1];
-    V.sens[0] = S.sens[2];
+    V.sten[0] = G(S.fld[o][1]);
+    V.sten[1] = G(S.fld[o][2]);
+    V.old[0] = G(S.fld[s][0]);
+    V.out[0] = G(S.fld[o][0]);
+    V.mat[0] = G(S.mat[0]);
+    V.lut[0] = G(S.wlut + 3 * L);
+    V.alpha[0] = G((const float*)S.alpha[1]);
+    V.sens[0] = G(S.sens[2]);
   } else if (PH == kElVel) {
-    V.sten[0] = S.fld[s][2];
-    V.sten[1] = S.fld[s][3];
-    V.sten[2] = S.fld[s][4];
-    V.old[0] = S.fld[s][0];
-    V.old[1] = S.fld[s][1];
-    V.out[0] = S.fld[o][0];
-    V.out[1] = S.fld[o][1];
-    V.mat[0] = S.mat[3];
-    V.mat[1] = S.mat[4];
-    V.lut[0] = S.wlut;
-    V.lut[1] = S.wlut + L;
+    V.sten[0] = G(S.fld[s][2]);
+    V.sten[1] = G(S.fld[s][3]);
+    V.sten[2] = G(S.fld[s][4]);
+    V.old[0] = G(S.fld[s][0]);
+    V.old[1] = G(S.fld[s][1]);
+    V.out[0] = G(S.fld[o][0]);
+    V.out[1] = G(S.fld[o][1]);
+    V.mat[0] = G(S.mat[3]);
+    V.mat[1] = G(S.mat[4]);
+    V.lut[0] = G(S.wlut);
+    V.lut[1] = G(S.wlut + L);
   } else {
-    V.sten[0] = S.fld[o][0];
-    V.sten[1] = S.fld[o][1];
+    V.sten[0] = G(S.fld[o][0]);
+    V.sten[1] = G(S.fld[o][1]);
     for (int f = 0; f < 3; ++f) {
-      V.old[f] = S.fld[s][2 + f];
-      V.out[f] = S.fld[o][2 + f];
-      V.mat[f] = S.mat[f];
-      V.lut[f] = S.wlut + 3 * L;
+      V.old[f] = G(S.fld[s][2 + f]);
+      V.out[f] = G(S.fld[o][2 + f]);
+      V.mat[f] = G(S.mat[f]);
+      V.lut[f] = G(S.wlut + 3 * L);
     }
   }
 }
@@ -214,8 +214,8 @@ template <int NF>
 __device__ __forceinline__ void inject(const Shot2D& S, int mask, int c, int it, int nt, float* v) {
   if (mask == 0 || it >= nt) return;
   for (int s = 0; s < S.n_src; ++s) {
-    if (S.src_c[s] == c) {
-      const float w = S.wav[(size_t)s + (size_t)S.n_src * (size_t)it];
+    if (G(S.src_c)[s] == c) {
+      const float w = G(S.wav)[(size_t)s + (size_t)S.n_src * (size_t)it];
 #pragma unroll
       for (int f = 0; f < NF; ++f)
         if (mask & (1 << f)) v[f] += w;
@@ -293,7 +293,7 @@ __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, 
       const int di = Lf ? 1 : -1, dj = Tf ? 1 : -1;
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
-        const float* __restrict__ fo = V.old[f];
+        const gp<const float> fo = V.old[f];
         const float foc = fo[c];
         float sx = 0.0f, sz = 0.0f;
         if (inx) {
@@ -417,7 +417,7 @@ __global__ __launch_bounds__(256) void gather2d_kernel(const GatherArgs A) {
     const Shot2D& S = A.shots[A.rec_shot[e]];
     const size_t c = (size_t)A.rec_c[e];
     const long long o = A.rec_out[e] + (long long)A.rec_stride[e] * it;
-    for (int k = 0; k < A.nslots; ++k) A.seis[k][o] = S.fld[A.parity_out][A.field_of_slot[k]][c];
+    for (int k = 0; k < A.nslots; ++k) G(A.seis[k])[o] = G(S.fld[A.parity_out][A.field_of_slot[k]])[c];
   }
   __syncthreads();
   if (threadIdx.x == 0) {

@@ -1,4 +1,4 @@
-// engine3d.h -- 3D acoustic engine: shared declarations (host + device).
+// engine3d.h -- 3D engine (acoustic3d, elastic3d): shared declarations (host + device).
 #pragma once
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
@@ -9,14 +9,19 @@
 
 namespace fomo {
 
-// Fields: 0 p, 1 vx, 2 vy, 3 vz (ping-pong).  Materials: 0 kappa, 1 bx, 2 by, 3 bz.  Storage is
-// Float32, Float16 or BFloat16; arithmetic is always Float32.  A reduced-precision value v is
-// stored as RN(v * enc) and read back as float(stored) * dec, one (dec, enc) pair per array.
+// Fields (ping-pong):  acoustic3d 0 p, 1 vx, 2 vy, 3 vz;
+//                      elastic3d  0 vx, 1 vy, 2 vz, 3 txx, 4 tyy, 5 tzz, 6 txy, 7 txz, 8 tyz.
+// Materials:           acoustic3d 0 kappa, 1 bx, 2 by, 3 bz;
+//                      elastic3d  0 bx, 1 by, 2 bz, 3 lam, 4 lam2mu, 5 mu_txy, 6 mu_txz, 7 mu_tyz.
+// Storage is Float32, Float16 or BFloat16; arithmetic is always Float32.  A reduced-precision
+// value v is stored as RN(v * enc) and read back as float(stored) * dec, one pair per array.
+constexpr int kMaxF3 = 9, kMaxM3 = 8;
+
 struct Shot3D {
-  void* fld[2][4];
-  const void* mat[4];
-  float fdec[4], fenc[4];
-  float mdec[4];
+  void* fld[2][kMaxF3];
+  const void* mat[kMaxM3];
+  float fdec[kMaxF3], fenc[kMaxF3];
+  float mdec[kMaxM3];
   const float* wlut;  // HABC weight tables by distance: vx, vz, vy, tau (H+1 each)
   float dtdh, q, qt, qxt;
   int nx, ny, nz, H, pad;
@@ -25,16 +30,23 @@ struct Shot3D {
   const float* wav;
 };
 
-// kind 0: frame box (x up to 64, y up to 4, one plane), cell by cell; kind 1: interior column
-// tile (x up to 64, y up to kTileY3) marched down z over hz planes.
+// A column tile marched down z.  Output cells: x in [i0, i0+w), y in [j0, j0+hy), z in
+// [k0, k0+hz).  ring = 1: the tile also computes pre-correction values on a one-cell ring in x
+// and y (w <= 62, hy <= rows-2), which the Higdon blend of x/y frame cells reads; ring = 0: no
+// cell of the footprint is an x/y frame cell (w <= 64, hy <= rows).  Core items (the user's
+// interior in x, y and z) go to the core list, all others to the frame list.  kind < 0: padding.
 struct Work3D {
-  int kind, i0, j0, k0, w, hy, hz;
+  int kind, i0, j0, k0, w, hy, hz, ring;
 };
 
-constexpr int kTileY3 = 16;
+// Rows of a tile (4 thread rows x rows-per-thread): 8 for both equations (2 rows per thread keep
+// the per-thread state small enough for 4 waves per SIMD in the acoustic sweeps).
+constexpr int kRowsAc3 = 8, kRowsEl3 = 8;
 
+// The shot descriptor travels by value in the kernel arguments (constant, so its pointers and
+// scales stay in scalar registers or are reloaded from the argument segment).
 struct Sweep3DArgs {
-  const Shot3D* shot;
+  Shot3D shot;
   const Work3D* work;
   int n_work;
   int parity;
@@ -44,9 +56,9 @@ struct Sweep3DArgs {
 };
 
 struct Gather3DArgs {
-  const Shot3D* shot;
+  Shot3D shot;
   const long long* rec_c;
-  float* seis[4];  // slots: vx, vy, vz, p
+  float* seis[4];
   int slot_field[4];
   int nslots;
   int n_rec;
@@ -56,8 +68,11 @@ struct Gather3DArgs {
   unsigned* done;
 };
 
-// storage: FOMO_STORE_F32 / F16 / BF16.  phase 0: velocity, 1: pressure.
-hipError_t launch_sweep3d(int storage, int phase, int N, const Sweep3DArgs& a, hipStream_t s);
+// eq: FOMO_ACOUSTIC3D / FOMO_ELASTIC3D; storage: FOMO_STORE_F32 / F16 / BF16; phase 0 velocity,
+// 1 pressure or stress.  Two launches: the core items (user interior, no frame cell) and the
+// frame items (everything else); they write disjoint cells and read only read-only buffers.
+hipError_t launch_sweep3d(int eq, int storage, int phase, int N, const Sweep3DArgs& core, const Sweep3DArgs& frame,
+                          hipStream_t s);
 hipError_t launch_gather3d(int storage, const Gather3DArgs& a, hipStream_t s);
 hipError_t launch_decode3d(int storage, const void* src, float* dst, size_t n, float dec, hipStream_t s);
 hipError_t launch_slice3d(int storage, const void* src, float* dst, int nx, int ny, int nz, int pad, int plane,

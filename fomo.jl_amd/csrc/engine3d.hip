@@ -1,359 +1,518 @@
-// engine3d.hip -- 3D acoustic sweeps for gfx950 with the deterministic Higdon boundary fused in.
+// engine3d.hip -- 3D staggered-grid sweeps (acoustic3d, elastic3d) for gfx950 with the
+// deterministic Higdon boundary fused in.
 //
 // Padded cube, x fastest, then y, then z; cell (i,j,k) at i + j*nx + k*nx*ny.  One launch per
-// phase over a work list with two kinds of items:
-//   * interior column tiles (64 x 16 cells in x-y, the user's interior only) that march down z.
-//     The field read through the z-stencil lives in a per-thread register queue, the planes read
-//     through x/y stencils are staged in LDS with their halos; every other array is read once.
-//   * frame boxes (64 x 4 cells of one plane) covering the six pad slabs.  Each thread computes
-//     its cell's pre-correction values and, for the Higdon blend, those of its inward neighbours
-//     (recomputed from cached global reads), so the result needs no second pass.
+// phase over a work list of column tiles (64 x rows cells in x-y) that march down z.  A tile
+// keeps, per thread, the fields read through z-stencils in register queues, stages the planes
+// read through x/y stencils in LDS with their halos, and reads every other array once.
+//
+// Higdon boundary without a second pass.  A frame cell's blend needs the pre-correction values
+// of its inward neighbours in x, y and z.  The march runs one plane behind: at plane t it
+// computes the pre-correction values of plane t, then finalises plane t-1, whose z-neighbours
+// t-2, t (registers) and x/y-neighbours (an LDS copy of plane t-1) are all on chip.  Tiles whose
+// footprint touches the x/y frame also compute a one-cell ring of pre-correction values around
+// their output cells (ring tiles); the other tiles need no ring.  Every output is a pure function
+// of the read-only buffers (ping-pong), so the result does not depend on the tiling.
+//
 // Storage is Float32, Float16 or BFloat16 (per-array scale); arithmetic is Float32.  With Float32
-// storage and -ffp-contract=off the result is bit-identical to the reference's Float32 updates:
-//   velocity ... src/equations/acoustic3d/update_velocity.jl:31-48
-//   pressure ... src/equations/acoustic3d/update_pressure.jl:29-45
-//   Higdon ..... src/boundary/habc/kernels_3d.jl:131-186, evaluated on pre-correction values
-//                (the race-free form, as src/boundary/habc/kernels_det.jl:5-10 describes for 2D)
+// storage and -ffp-contract=off the result is bit-identical to the reference's Float32 updates,
+// evaluated term by term in the reference's order:
+//   acoustic velocity / pressure ... src/equations/acoustic3d/update_velocity.jl:31-48,
+//                                    src/equations/acoustic3d/update_pressure.jl:29-45
+//   elastic velocity / stress ...... src/equations/elastic3d/update_velocity.jl:30-82,
+//                                    src/equations/elastic3d/update_stress.jl:34-92
+//   Higdon ......................... src/boundary/habc/kernels_3d.jl:131-186 on pre-correction
+//                                    values (the race-free form of src/boundary/habc/kernels_det.jl:5-10)
+//   injection ...................... src/acquisition/inject_source_3d.jl:5-17 (after the HABC)
 #include <algorithm>
 
 #include "engine3d.h"
 
 namespace fomo {
 
+// Element c of a global array of E: with a 32-bit index the byte offset is formed in 32 bits,
+// so the access compiles to a scalar base plus a 32-bit vector offset.
+template <class E, class I>
+__device__ __forceinline__ E gget(const void* p, I c) {
+  if constexpr (sizeof(I) == 4)
+    return *(gp<const E>)(G((const char*)p) + (unsigned)(c * (unsigned)sizeof(E)));
+  else
+    return G((const E*)p)[c];
+}
+template <class E, class I>
+__device__ __forceinline__ void gput(void* p, I c, E v) {
+  if constexpr (sizeof(I) == 4)
+    *(gp<E>)(G((char*)p) + (unsigned)(c * (unsigned)sizeof(E))) = v;
+  else
+    G((E*)p)[c] = v;
+}
+
 template <class T>
 struct Cvt;
 template <>
 struct Cvt<float> {
-  __device__ __forceinline__ static float ld(const void* p, size_t c, float) { return ((const float*)p)[c]; }
-  __device__ __forceinline__ static void st(void* p, size_t c, float v, float) { ((float*)p)[c] = v; }
+  template <class I>
+  __device__ __forceinline__ static float ld(const void* p, I c, float) {
+    return gget<float>(p, c);
+  }
+  template <class I>
+  __device__ __forceinline__ static void st(void* p, I c, float v, float) {
+    gput<float>(p, c, v);
+  }
 };
 template <>
 struct Cvt<__half> {
-  __device__ __forceinline__ static float ld(const void* p, size_t c, float dec) {
-    return __half2float(((const __half*)p)[c]) * dec;
+  template <class I>
+  __device__ __forceinline__ static float ld(const void* p, I c, float dec) {
+    return __half2float(__ushort_as_half(gget<unsigned short>(p, c))) * dec;
   }
-  __device__ __forceinline__ static void st(void* p, size_t c, float v, float enc) {
-    ((__half*)p)[c] = __float2half_rn(v * enc);
+  template <class I>
+  __device__ __forceinline__ static void st(void* p, I c, float v, float enc) {
+    gput<unsigned short>(p, c, __half_as_ushort(__float2half_rn(v * enc)));
   }
 };
 template <>
 struct Cvt<__hip_bfloat16> {
-  __device__ __forceinline__ static float ld(const void* p, size_t c, float dec) {
-    return __bfloat162float(((const __hip_bfloat16*)p)[c]) * dec;
+  template <class I>
+  __device__ __forceinline__ static float ld(const void* p, I c, float dec) {
+    return __bfloat162float(__ushort_as_bfloat16(gget<unsigned short>(p, c))) * dec;
   }
-  __device__ __forceinline__ static void st(void* p, size_t c, float v, float enc) {
-    ((__hip_bfloat16*)p)[c] = __float2bfloat16(v * enc);
-  }
-};
-
-struct G3 {
-  int nx, ny, nz;
-  size_t sy, sz;
-  __device__ __forceinline__ size_t at(int i, int j, int k) const {
-    return (size_t)i + (size_t)j * sy + (size_t)k * sz;
-  }
-  __device__ __forceinline__ bool in(int i, int j, int k) const {
-    return i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz;
+  template <class I>
+  __device__ __forceinline__ static void st(void* p, I c, float v, float enc) {
+    gput<unsigned short>(p, c, __bfloat16_as_ushort(__float2bfloat16(v * enc)));
   }
 };
 
-// --- cell-by-cell physics on global memory (frame boxes) ------------------------------------
-template <int N, class T>
-struct Cell3D {
-  const Shot3D& S;
-  const float* a;
-  int s;  // read parity
-  G3 g;
-  __device__ __forceinline__ bool upd(int i, int j, int k) const {
-    return i >= N && i < g.nx - N && j >= N && j < g.ny - N && k >= N && k < g.nz - N;
-  }
-  __device__ __forceinline__ float fld(int buf, int f, size_t c) const {
-    return Cvt<T>::ld(S.fld[buf][f], c, S.fdec[f]);
-  }
-  __device__ __forceinline__ float mat(int m, size_t c) const { return Cvt<T>::ld(S.mat[m], c, S.mdec[m]); }
-  // pre-correction velocity component f (1 vx, 2 vy, 3 vz) at (i,j,k)
-  __device__ float pre_v(int f, int i, int j, int k) const {
-    const size_t c = g.at(i, j, k);
-    const float o = fld(s, f, c);
-    if (!upd(i, j, k)) return o;
-    float d = 0.0f;
-#pragma unroll
-    for (int l = 1; l <= N; ++l) {
-      float hi, lo;
-      if (f == 1) {
-        hi = fld(s, 0, c + (l - 1));
-        lo = fld(s, 0, c - l);
-      } else if (f == 2) {
-        hi = fld(s, 0, c + (size_t)(l - 1) * g.sy);
-        lo = fld(s, 0, c - (size_t)l * g.sy);
-      } else {
-        hi = fld(s, 0, c + (size_t)l * g.sz);
-        lo = fld(s, 0, c - (size_t)(l - 1) * g.sz);
-      }
-      d += a[l - 1] * (hi - lo);
-    }
-    return o + (mat(f, c) * S.dtdh) * d;
-  }
-  // pre-correction pressure at (i,j,k): reads the new velocities (buffer 1-s)
-  __device__ float pre_p(int i, int j, int k) const {
-    const int o_ = 1 - s;
-    const size_t c = g.at(i, j, k);
-    const float o = fld(s, 0, c);
-    if (!upd(i, j, k)) return o;
-    float dx = 0.0f, dy = 0.0f, dz = 0.0f;
-#pragma unroll
-    for (int l = 1; l <= N; ++l) {
-      const float cl = a[l - 1];
-      dx += cl * (fld(o_, 1, c + l) - fld(o_, 1, c - (l - 1)));
-      dy += cl * (fld(o_, 2, c + (size_t)l * g.sy) - fld(o_, 2, c - (size_t)(l - 1) * g.sy));
-      dz += cl * (fld(o_, 3, c + (size_t)(l - 1) * g.sz) - fld(o_, 3, c - (size_t)l * g.sz));
-    }
-    return o + (mat(0, c) * S.dtdh) * ((dx + dy) + dz);
-  }
-};
-
-// Higdon blend (kernels_3d.jl:131-186) of field f at frame cell (i,j,k): directional terms in
-// x, y, z order summed from 0, then wt*f + ((1-wt)*sum)/n_dirs.  pre(i,j,k) gives the
-// pre-correction value; old values come from the read buffer.
-template <class PreFn, class T>
-__device__ __forceinline__ float higdon3d(const Shot3D& S, const G3& g, int s, int f, int lut, int i, int j, int k,
-                                         float fc, const PreFn& pre) {
-  const int H = S.H;
-  const bool il = i <= H, ir = i >= g.nx - 1 - H, jf = j <= H, jb = j >= g.ny - 1 - H;
-  const bool kt = k <= H, kb = k >= g.nz - 1 - H;
-  const size_t c = g.at(i, j, k);
-  const float foc = Cvt<T>::ld(S.fld[s][f], c, S.fdec[f]);
-  float sum = 0.0f;
-  int nd = 0;
-  if (il || ir) {
-    const int ii = il ? i + 1 : i - 1;
-    sum += (-S.q * pre(ii, j, k) - S.qt * foc) - S.qxt * Cvt<T>::ld(S.fld[s][f], g.at(ii, j, k), S.fdec[f]);
-    ++nd;
-  }
-  if (jf || jb) {
-    const int jj = jf ? j + 1 : j - 1;
-    sum += (-S.q * pre(i, jj, k) - S.qt * foc) - S.qxt * Cvt<T>::ld(S.fld[s][f], g.at(i, jj, k), S.fdec[f]);
-    ++nd;
-  }
-  if (kt || kb) {
-    const int kk = kt ? k + 1 : k - 1;
-    sum += (-S.q * pre(i, j, kk) - S.qt * foc) - S.qxt * Cvt<T>::ld(S.fld[s][f], g.at(i, j, kk), S.fdec[f]);
-    ++nd;
-  }
-  const int dist = min(min(min(i, g.nx - 1 - i), min(j, g.ny - 1 - j)), min(k, g.nz - 1 - k));
-  const float wt = S.wlut[lut * (H + 1) + dist];
-  return wt * fc + ((1.0f - wt) * sum) / (float)nd;
+// Base of plane k of a stored array (uniform across the block, so loads address it as a scalar
+// base plus a 32-bit in-plane offset).
+template <class T>
+__device__ __forceinline__ const void* plane_of(const void* p, int k, size_t plane_elems) {
+  return (const void*)((const T*)p + (size_t)k * plane_elems);
 }
+template <class T>
+__device__ __forceinline__ void* plane_of(void* p, int k, size_t plane_elems) {
+  return (void*)((T*)p + (size_t)k * plane_elems);
+}
+
+enum Phase3 { kA3Vel = 0, kA3Prs = 1, kE3Vel = 2, kE3Str = 3 };
+
+// Per phase: NF output fields, NS staged planes, NQ z-queues, YPT rows per thread, whether the
+// stencil inputs are this step's new velocities (buffer 1-s), and the field ids involved.
+template <int PH>
+struct P3;
+template <>
+struct P3<kA3Vel> {
+  static constexpr int NF = 3, NS = 1, NQ = 1, YPT = 2, NM = 3;
+  __device__ static constexpr int mat_id(int k) { return 1 + k; }  // bx by bz
+  static constexpr bool kNew = false;
+  __device__ static constexpr int out_f(int k) { return 1 + k; }                       // vx vy vz
+  __device__ static constexpr int lut(int k) { return k == 0 ? 0 : (k == 1 ? 2 : 1); }  // vx vy vz tables
+  __device__ static constexpr bool inj(int) { return false; }
+  __device__ static constexpr int st_f(int) { return 0; }  // p
+  __device__ static constexpr bool st_xh(int) { return true; }
+  __device__ static constexpr bool st_yh(int) { return true; }
+  __device__ static constexpr int st_q(int) { return 0; }  // own column from queue 0
+  __device__ static constexpr int q_f(int) { return 0; }   // p
+  __device__ static constexpr bool q_back(int) { return false; }
+};
+template <>
+struct P3<kA3Prs> {
+  static constexpr int NF = 1, NS = 2, NQ = 1, YPT = 2, NM = 1;
+  __device__ static constexpr int mat_id(int) { return 0; }  // kappa
+  static constexpr bool kNew = true;
+  __device__ static constexpr int out_f(int) { return 0; }
+  __device__ static constexpr int lut(int) { return 3; }
+  __device__ static constexpr bool inj(int) { return true; }
+  __device__ static constexpr int st_f(int s) { return s == 0 ? 1 : 2; }  // vx (x), vy (y)
+  __device__ static constexpr bool st_xh(int s) { return s == 0; }
+  __device__ static constexpr bool st_yh(int s) { return s == 1; }
+  __device__ static constexpr int st_q(int) { return -1; }
+  __device__ static constexpr int q_f(int) { return 3; }  // vz
+  __device__ static constexpr bool q_back(int) { return true; }
+};
+template <>
+struct P3<kE3Vel> {
+  static constexpr int NF = 3, NS = 5, NQ = 3, YPT = 2, NM = 3;
+  __device__ static constexpr int mat_id(int k) { return k; }  // bx by bz
+  static constexpr bool kNew = false;
+  __device__ static constexpr int out_f(int k) { return k; }  // vx vy vz
+  __device__ static constexpr int lut(int k) { return k == 0 ? 0 : (k == 1 ? 2 : 1); }
+  __device__ static constexpr bool inj(int) { return false; }
+  // staged: txx (x), txy (x, y), tyy (y), txz (x), tyz (y)
+  __device__ static constexpr int st_f(int s) { return s == 0 ? 3 : (s == 1 ? 6 : (s == 2 ? 4 : (s == 3 ? 7 : 8))); }
+  __device__ static constexpr bool st_xh(int s) { return s == 0 || s == 1 || s == 3; }
+  __device__ static constexpr bool st_yh(int s) { return s == 1 || s == 2 || s == 4; }
+  __device__ static constexpr int st_q(int s) { return s == 3 ? 1 : (s == 4 ? 2 : -1); }
+  // queues: tzz (forward), txz (backward), tyz (backward)
+  __device__ static constexpr int q_f(int q) { return q == 0 ? 5 : (q == 1 ? 7 : 8); }
+  __device__ static constexpr bool q_back(int q) { return q != 0; }
+};
+template <>
+struct P3<kE3Str> {
+  static constexpr int NF = 6, NS = 3, NQ = 3, YPT = 2, NM = 5;
+  __device__ static constexpr int mat_id(int k) { return 3 + k; }  // lam lam2mu mu_txy mu_txz mu_tyz
+  static constexpr bool kNew = true;
+  __device__ static constexpr int out_f(int k) { return 3 + k; }  // txx tyy tzz txy txz tyz
+  __device__ static constexpr int lut(int) { return 3; }
+  __device__ static constexpr bool inj(int k) { return k < 3; }  // explosive: txx, tyy, tzz
+  __device__ static constexpr int st_f(int s) { return s; }      // vx, vy, vz (x and y halos)
+  __device__ static constexpr bool st_xh(int) { return true; }
+  __device__ static constexpr bool st_yh(int) { return true; }
+  __device__ static constexpr int st_q(int s) { return s; }
+  // queues: vx (forward), vy (forward), vz (backward)
+  __device__ static constexpr int q_f(int q) { return q; }
+  __device__ static constexpr bool q_back(int q) { return q == 2; }
+};
+
+static_assert(4 * P3<kA3Vel>::YPT == kRowsAc3 && 4 * P3<kA3Prs>::YPT == kRowsAc3, "acoustic tile rows");
+static_assert(4 * P3<kE3Vel>::YPT == kRowsEl3 && 4 * P3<kE3Str>::YPT == kRowsEl3, "elastic tile rows");
+
+// LDS of a tile: the staged planes (with halos) and a ring of three planes of pre-correction
+// values of the outputs (planes t-2, t-1, t).
+template <int PH, int N>
+struct Lds3 {
+  static constexpr int ROWS = 4 * P3<PH>::YPT, PW = kTileW + 2 * N, PR = ROWS + 2 * N;
+  static constexpr int STG = P3<PH>::NS * PR * PW, SLOT = P3<PH>::NF * ROWS * kTileW;
+  static constexpr int SIZE = STG + 3 * SLOT;
+};
 
 __device__ __forceinline__ void inject3d(const Shot3D& S, size_t c, int it, int nt, float& v) {
   if (it >= nt) return;
   for (int q = 0; q < S.n_src; ++q)
-    if ((size_t)S.src_c[q] == c) v += S.wav[(size_t)q + (size_t)S.n_src * (size_t)it];
+    if ((size_t)G(S.src_c)[q] == c) v += G(S.wav)[(size_t)q + (size_t)S.n_src * (size_t)it];
 }
 
-template <int N, class T>
-__device__ void frame_box(const Sweep3DArgs& A, const Work3D& W, int phase) {
-  const Shot3D& S = *A.shot;
-  const G3 g{S.nx, S.ny, S.nz, (size_t)S.nx, (size_t)S.nx * S.ny};
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  if (tx >= W.w || ty >= W.hy) return;
-  const int i = W.i0 + tx, j = W.j0 + ty, k = W.k0;
-  const int s = A.parity, o = 1 - s;
-  const Cell3D<N, T> C{S, A.a, s, g};
-  const size_t c = g.at(i, j, k);
-  const int dist = min(min(min(i, g.nx - 1 - i), min(j, g.ny - 1 - j)), min(k, g.nz - 1 - k));
-  const bool frame = dist >= 1 && dist <= S.H;
-  if (phase == 0) {
-    // vx, vy, vz with weight tables vx (0), vy (2), vz (1)
-    const int luts[4] = {0, 0, 2, 1};
-#pragma unroll
-    for (int f = 1; f <= 3; ++f) {
-      float v = C.pre_v(f, i, j, k);
-      if (frame) {
-        auto pre = [&](int a_, int b_, int c_) { return C.pre_v(f, a_, b_, c_); };
-        v = higdon3d<decltype(pre), T>(S, g, s, f, luts[f], i, j, k, v, pre);
-      }
-      Cvt<T>::st(S.fld[o][f], c, v, S.fenc[f]);
-    }
-  } else {
-    float v = C.pre_p(i, j, k);
-    if (frame) {
-      auto pre = [&](int a_, int b_, int c_) { return C.pre_p(a_, b_, c_); };
-      v = higdon3d<decltype(pre), T>(S, g, s, 0, 3, i, j, k, v, pre);
-    }
-    inject3d(S, c, *A.step, A.nt, v);
-    Cvt<T>::st(S.fld[o][0], c, v, S.fenc[0]);
-  }
-}
-
-// --- interior column tiles -------------------------------------------------------------------
-// Velocity: p through x, y (LDS plane with halo N) and z (register queue of 2N planes).
-template <int N, class T>
-__device__ void interior_vel(const Sweep3DArgs& A, const Work3D& W, float* lds) {
-  constexpr int TY = kTileY3, YPT = TY / 4, PW = kTileW + 2 * N;
-  const Shot3D& S = *A.shot;
-  const G3 g{S.nx, S.ny, S.nz, (size_t)S.nx, (size_t)S.nx * S.ny};
+// LAG = true: frame march (one plane behind, ring tiles, Higdon blend, bounds-checked loads).
+// LAG = false: core march over the user's interior (no frame cell anywhere in the item): every
+// cell is updated and stored at once, and every stencil read lies inside the grid.
+template <int PH, int N, class T, bool LAG>
+__device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, float* lds) {
+  using P = P3<PH>;
+  using L = Lds3<PH, N>;
+  constexpr int YPT = P::YPT, ROWS = L::ROWS, NF = P::NF, NS = P::NS, NQ = P::NQ, NM = P::NM;
+  constexpr int PW = L::PW, PR = L::PR, HYR = (2 * N + 3) / 4;
+  float* stg = lds;
+  float* ring = lds + L::STG;  // slot t % 3 holds the pre-correction values of plane t
+  const Shot3D& S = A.shot;
+  const int nx = S.nx, ny = S.ny, nz = S.nz, H = S.H;
+  const size_t sz = (size_t)nx * ny;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int s = A.parity, o = 1 - s;
-  const int i = W.i0 + tx;
+  const int sb = P::kNew ? o : s;
+  const int pi0 = W.i0 - W.ring, pj0 = W.j0 - W.ring;
+  const int i = pi0 + tx;
   const float* __restrict__ a = A.a;
-  float q[YPT][2 * N];  // planes k-N+1 .. k+N
-  auto ldp = [&](int ii, int jj, int kk) -> float {
-    return g.in(ii, jj, kk) ? Cvt<T>::ld(S.fld[s][0], g.at(ii, jj, kk), S.fdec[0]) : 0.0f;
-  };
-#pragma unroll
-  for (int r = 0; r < YPT; ++r)
-#pragma unroll
-    for (int m = 0; m < 2 * N; ++m) q[r][m] = ldp(i, W.j0 + ty + 4 * r, W.k0 - N + 1 + m);
-  const float dtdh = S.dtdh;
-  for (int k = W.k0; k < W.k0 + W.hz; ++k) {
-    // plane k into LDS: own columns from the queue, x-halos and y-halo rows from global
-#pragma unroll
-    for (int r = 0; r < YPT; ++r) {
-      const int jl = ty + 4 * r;
-      lds[(jl + N) * PW + tx + N] = q[r][N - 1];
-      if (tx < N) {
-        lds[(jl + N) * PW + tx] = ldp(W.i0 - N + tx, W.j0 + jl, k);
-        lds[(jl + N) * PW + kTileW + N + tx] = ldp(W.i0 + kTileW + tx, W.j0 + jl, k);
-      }
-    }
-    for (int hr = ty; hr < 2 * N; hr += 4) {
-      const int jl = hr < N ? hr - N : TY + (hr - N);
-      lds[(jl + N) * PW + tx + N] = ldp(i, W.j0 + jl, k);
-    }
-    __syncthreads();
-    const bool col_ok = tx < W.w;
-#pragma unroll
-    for (int r = 0; r < YPT; ++r) {
-      const int jl = ty + 4 * r;
-      if (!col_ok || jl >= W.hy) continue;
-      const int j = W.j0 + jl;
-      const size_t c = g.at(i, j, k);
-      float dx = 0.0f, dy = 0.0f, dz = 0.0f;
-      const float* row = lds + (jl + N) * PW + tx + N;
-#pragma unroll
-      for (int l = 1; l <= N; ++l) {
-        const float cl = a[l - 1];
-        dx += cl * (row[l - 1] - row[-l]);
-        dy += cl * (row[(l - 1) * PW] - row[-l * PW]);
-        dz += cl * (q[r][N - 1 + l] - q[r][N - l]);
-      }
-      const float vx = Cvt<T>::ld(S.fld[s][1], c, S.fdec[1]) + (Cvt<T>::ld(S.mat[1], c, S.mdec[1]) * dtdh) * dx;
-      const float vy = Cvt<T>::ld(S.fld[s][2], c, S.fdec[2]) + (Cvt<T>::ld(S.mat[2], c, S.mdec[2]) * dtdh) * dy;
-      const float vz = Cvt<T>::ld(S.fld[s][3], c, S.fdec[3]) + (Cvt<T>::ld(S.mat[3], c, S.mdec[3]) * dtdh) * dz;
-      Cvt<T>::st(S.fld[o][1], c, vx, S.fenc[1]);
-      Cvt<T>::st(S.fld[o][2], c, vy, S.fenc[2]);
-      Cvt<T>::st(S.fld[o][3], c, vz, S.fenc[3]);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < YPT; ++r) {
-#pragma unroll
-      for (int m = 0; m < 2 * N - 1; ++m) q[r][m] = q[r][m + 1];
-      q[r][2 * N - 1] = ldp(i, W.j0 + ty + 4 * r, k + 1 + N);
-    }
-  }
-}
-
-// Pressure: vx through x (LDS rows with x-halo), vy through y (LDS with y-halo rows), vz through
-// z (register queue of planes k-N .. k+N-1).
-template <int N, class T>
-__device__ void interior_prs(const Sweep3DArgs& A, const Work3D& W, float* lds) {
-  constexpr int TY = kTileY3, YPT = TY / 4, PWX = kTileW + 2 * N;
-  const Shot3D& S = *A.shot;
-  const G3 g{S.nx, S.ny, S.nz, (size_t)S.nx, (size_t)S.nx * S.ny};
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  const int s = A.parity, o = 1 - s;
-  const int i = W.i0 + tx;
-  const float* __restrict__ a = A.a;
-  float* ldx = lds;                 // TY rows x PWX
-  float* ldy = lds + TY * PWX;      // PHY rows x 64
-  auto ldv = [&](int f, int ii, int jj, int kk) -> float {
-    return g.in(ii, jj, kk) ? Cvt<T>::ld(S.fld[o][f], g.at(ii, jj, kk), S.fdec[f]) : 0.0f;
-  };
-  float q[YPT][2 * N];  // vz planes k-N .. k+N-1
-#pragma unroll
-  for (int r = 0; r < YPT; ++r)
-#pragma unroll
-    for (int m = 0; m < 2 * N; ++m) q[r][m] = ldv(3, i, W.j0 + ty + 4 * r, W.k0 - N + m);
   const float dtdh = S.dtdh;
   const int it = *A.step;
-  for (int k = W.k0; k < W.k0 + W.hz; ++k) {
+  const bool i_upd = i >= N && i < nx - N;
+
+  // loads of field f at (ii, jj) of plane kk: scalar plane base + 32-bit in-plane offset
+  auto ld = [&](int buf, int f, int ii, int jj, int kk) -> float {
+    const unsigned off = (unsigned)(ii + jj * nx);
+    if constexpr (!LAG) return Cvt<T>::ld(plane_of<T>(S.fld[buf][f], kk, sz), off, S.fdec[f]);
+    return (ii >= 0 && ii < nx && jj >= 0 && jj < ny && kk >= 0 && kk < nz)
+               ? Cvt<T>::ld(plane_of<T>(S.fld[buf][f], kk, sz), off, S.fdec[f])
+               : 0.0f;
+  };
+
+  // z-queues: window of 2N planes starting at t-N (backward stencils) or t-N+1 (forward)
+  float q[NQ][YPT][2 * N];
+  const int t0 = LAG ? W.k0 - 1 : W.k0;
+#pragma unroll
+  for (int qq = 0; qq < NQ; ++qq) {
+    const int lo = P::q_back(qq) ? -N : -N + 1;
+#pragma unroll
+    for (int r = 0; r < YPT; ++r)
+#pragma unroll
+      for (int m = 0; m < 2 * N; ++m) q[qq][r][m] = ld(sb, P::q_f(qq), i, pj0 + ty + 4 * r, t0 + lo + m);
+  }
+
+  // Registers prefetched one plane ahead: own-column values of staged fields that have no queue,
+  // x-halo values (lanes tx < N), y-halo rows, old values of the outputs and the materials.
+  float sv[NS][YPT], hx[NS][YPT][2], hy[NS][HYR], oldF[YPT][NF], matF[YPT][NM], qn[NQ][YPT];
+  auto prefetch_stage = [&](int tp) {
+#pragma unroll
+    for (int sp = 0; sp < NS; ++sp) {
+      const int f = P::st_f(sp);
+#pragma unroll
+      for (int r = 0; r < YPT; ++r) {
+        const int j = pj0 + ty + 4 * r;
+        if (P::st_q(sp) < 0) sv[sp][r] = ld(sb, f, i, j, tp);
+        if (P::st_xh(sp)) {
+          hx[sp][r][0] = tx < N ? ld(sb, f, pi0 - N + tx, j, tp) : 0.0f;
+          hx[sp][r][1] = tx < N ? ld(sb, f, pi0 + kTileW + tx, j, tp) : 0.0f;
+        }
+      }
+      if (P::st_yh(sp)) {
+#pragma unroll
+        for (int m = 0; m < HYR; ++m) {
+          const int hr = ty + 4 * m;
+          const int jl = hr < N ? hr - N : ROWS + (hr - N);
+          hy[sp][m] = hr < 2 * N ? ld(sb, f, i, pj0 + jl, tp) : 0.0f;
+        }
+      }
+    }
+  };
+  auto prefetch_cells = [&](int tp) {
+    const bool k_upd = tp >= N && tp < nz - N;
 #pragma unroll
     for (int r = 0; r < YPT; ++r) {
-      const int jl = ty + 4 * r;
-      const int j = W.j0 + jl;
-      ldx[jl * PWX + tx + N] = ldv(1, i, j, k);
-      if (tx < N) {
-        ldx[jl * PWX + tx] = ldv(1, W.i0 - N + tx, j, k);
-        ldx[jl * PWX + kTileW + N + tx] = ldv(1, W.i0 + kTileW + tx, j, k);
-      }
-      ldy[(jl + N) * kTileW + tx] = ldv(2, i, j, k);
+      const int j = pj0 + ty + 4 * r;
+      const bool valid = !LAG || (i < nx && j < ny);
+      const bool upd = !LAG || (i_upd && k_upd && j >= N && j < ny - N);
+      const unsigned co = (unsigned)(i + j * nx);
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+        oldF[r][f] = valid ? Cvt<T>::ld(plane_of<T>(S.fld[s][P::out_f(f)], tp, sz), co, S.fdec[P::out_f(f)]) : 0.0f;
+#pragma unroll
+      for (int k = 0; k < NM; ++k)
+        matF[r][k] = upd ? Cvt<T>::ld(plane_of<T>(S.mat[P::mat_id(k)], tp, sz), co, S.mdec[P::mat_id(k)]) : 0.0f;
     }
-    for (int hr = ty; hr < 2 * N; hr += 4) {
-      const int jl = hr < N ? hr - N : TY + (hr - N);
-      ldy[(jl + N) * kTileW + tx] = ldv(2, i, W.j0 + jl, k);
+  };
+
+  float preN[YPT][NF];
+
+  prefetch_stage(t0);
+  prefetch_cells(t0);
+  const int t_end = LAG ? W.k0 + W.hz : W.k0 + W.hz - 1;
+  for (int t = t0; t <= t_end; ++t) {
+    // 1. stage the x/y-stencil planes of level t from the prefetched registers
+#pragma unroll
+    for (int sp = 0; sp < NS; ++sp) {
+      float* pl = stg + sp * PR * PW;
+#pragma unroll
+      for (int r = 0; r < YPT; ++r) {
+        const int jl = ty + 4 * r;
+        const int qs = P::st_q(sp) < 0 ? 0 : P::st_q(sp);
+        const float v = P::st_q(sp) >= 0 ? q[qs][r][P::q_back(qs) ? N : N - 1] : sv[sp][r];
+        pl[(jl + N) * PW + tx + N] = v;
+        if (P::st_xh(sp) && tx < N) {
+          pl[(jl + N) * PW + tx] = hx[sp][r][0];
+          pl[(jl + N) * PW + kTileW + N + tx] = hx[sp][r][1];
+        }
+      }
+      if (P::st_yh(sp)) {
+#pragma unroll
+        for (int m = 0; m < HYR; ++m) {
+          const int hr = ty + 4 * m;
+          if (hr < 2 * N) {
+            const int jl = hr < N ? hr - N : ROWS + (hr - N);
+            pl[(jl + N) * PW + tx + N] = hy[sp][m];
+          }
+        }
+      }
+    }
+    // 2. issue the staging loads of level t+1 (and the queue entry that completes its window)
+    if (t < t_end) {
+      prefetch_stage(t + 1);
+#pragma unroll
+      for (int qq = 0; qq < NQ; ++qq) {
+        const int lo = P::q_back(qq) ? -N : -N + 1;
+#pragma unroll
+        for (int r = 0; r < YPT; ++r) qn[qq][r] = ld(sb, P::q_f(qq), i, pj0 + ty + 4 * r, t + 1 + lo + 2 * N - 1);
+      }
     }
     __syncthreads();
-    const bool col_ok = tx < W.w;
+
+    // 3. pre-correction values of level t for this thread's cells
+    const bool k_upd = t >= N && t < nz - N;
 #pragma unroll
     for (int r = 0; r < YPT; ++r) {
-      const int jl = ty + 4 * r;
-      if (!col_ok || jl >= W.hy) continue;
-      const int j = W.j0 + jl;
-      const size_t c = g.at(i, j, k);
-      float dx = 0.0f, dy = 0.0f, dz = 0.0f;
-      const float* rx = ldx + jl * PWX + tx + N;
-      const float* ry = ldy + (jl + N) * kTileW + tx;
+      const int jl = ty + 4 * r, j = pj0 + jl;
+      const bool upd = !LAG || (i_upd && k_upd && j >= N && j < ny - N);
+      const int row = (jl + N) * PW + tx + N;
+      if constexpr (PH == kA3Vel) {
+        const float* P0 = stg + row;
+        float dx = 0.0f, dy = 0.0f, dz = 0.0f;
 #pragma unroll
-      for (int l = 1; l <= N; ++l) {
-        const float cl = a[l - 1];
-        dx += cl * (rx[l] - rx[-(l - 1)]);
-        dy += cl * (ry[l * kTileW] - ry[-(l - 1) * kTileW]);
-        dz += cl * (q[r][N + l - 1] - q[r][N - l]);
+        for (int l = 1; l <= N; ++l) {
+          const float cl = a[l - 1];
+          dx += cl * (P0[l - 1] - P0[-l]);
+          dy += cl * (P0[(l - 1) * PW] - P0[-l * PW]);
+          dz += cl * (q[0][r][N - 1 + l] - q[0][r][N - l]);
+        }
+        preN[r][0] = oldF[r][0] + (matF[r][0] * dtdh) * dx;
+        preN[r][1] = oldF[r][1] + (matF[r][1] * dtdh) * dy;
+        preN[r][2] = oldF[r][2] + (matF[r][2] * dtdh) * dz;
+      } else if constexpr (PH == kA3Prs) {
+        const float* VX = stg + row;
+        const float* VY = stg + PR * PW + row;
+        float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+#pragma unroll
+        for (int l = 1; l <= N; ++l) {
+          const float cl = a[l - 1];
+          dx += cl * (VX[l] - VX[-(l - 1)]);
+          dy += cl * (VY[l * PW] - VY[-(l - 1) * PW]);
+          dz += cl * (q[0][r][N + l - 1] - q[0][r][N - l]);
+        }
+        preN[r][0] = oldF[r][0] + (matF[r][0] * dtdh) * ((dx + dy) + dz);
+      } else if constexpr (PH == kE3Vel) {
+        const float* TXX = stg + row;
+        const float* TXY = stg + PR * PW + row;
+        const float* TYY = stg + 2 * PR * PW + row;
+        const float* TXZ = stg + 3 * PR * PW + row;
+        const float* TYZ = stg + 4 * PR * PW + row;
+        float dtxxdx = 0.0f, dtxydy = 0.0f, dtxzdz = 0.0f, dtxydx = 0.0f, dtyydy = 0.0f, dtyzdz = 0.0f;
+        float dtxzdx = 0.0f, dtyzdy = 0.0f, dtzzdz = 0.0f;
+#pragma unroll
+        for (int l = 1; l <= N; ++l) {
+          const float cl = a[l - 1];
+          dtxxdx += cl * (TXX[l - 1] - TXX[-l]);
+          dtyydy += cl * (TYY[(l - 1) * PW] - TYY[-l * PW]);
+          dtzzdz += cl * (q[0][r][N - 1 + l] - q[0][r][N - l]);
+          dtxydy += cl * (TXY[l * PW] - TXY[-(l - 1) * PW]);
+          dtxydx += cl * (TXY[l] - TXY[-(l - 1)]);
+          dtxzdz += cl * (q[1][r][N + l - 1] - q[1][r][N - l]);
+          dtxzdx += cl * (TXZ[l] - TXZ[-(l - 1)]);
+          dtyzdz += cl * (q[2][r][N + l - 1] - q[2][r][N - l]);
+          dtyzdy += cl * (TYZ[l * PW] - TYZ[-(l - 1) * PW]);
+        }
+        preN[r][0] = oldF[r][0] + (matF[r][0] * dtdh) * ((dtxxdx + dtxydy) + dtxzdz);
+        preN[r][1] = oldF[r][1] + (matF[r][1] * dtdh) * ((dtxydx + dtyydy) + dtyzdz);
+        preN[r][2] = oldF[r][2] + (matF[r][2] * dtdh) * ((dtxzdx + dtyzdy) + dtzzdz);
+      } else {
+        const float* VX = stg + row;
+        const float* VY = stg + PR * PW + row;
+        const float* VZ = stg + 2 * PR * PW + row;
+        float dvxdx = 0.0f, dvydy = 0.0f, dvzdz = 0.0f, dvxdy = 0.0f, dvydx = 0.0f, dvxdz = 0.0f;
+        float dvzdx = 0.0f, dvydz = 0.0f, dvzdy = 0.0f;
+#pragma unroll
+        for (int l = 1; l <= N; ++l) {
+          const float cl = a[l - 1];
+          dvxdx += cl * (VX[l] - VX[-(l - 1)]);
+          dvydy += cl * (VY[l * PW] - VY[-(l - 1) * PW]);
+          dvzdz += cl * (q[2][r][N + l - 1] - q[2][r][N - l]);
+          dvxdy += cl * (VX[(l - 1) * PW] - VX[-l * PW]);
+          dvydx += cl * (VY[l - 1] - VY[-l]);
+          dvxdz += cl * (q[0][r][N - 1 + l] - q[0][r][N - l]);
+          dvzdx += cl * (VZ[l - 1] - VZ[-l]);
+          dvydz += cl * (q[1][r][N - 1 + l] - q[1][r][N - l]);
+          dvzdy += cl * (VZ[(l - 1) * PW] - VZ[-l * PW]);
+        }
+        const float lv = matF[r][0], l2 = matF[r][1];
+        preN[r][0] = oldF[r][0] + ((l2 * dvxdx + lv * dvydy) + lv * dvzdz) * dtdh;
+        preN[r][1] = oldF[r][1] + ((lv * dvxdx + l2 * dvydy) + lv * dvzdz) * dtdh;
+        preN[r][2] = oldF[r][2] + ((lv * dvxdx + lv * dvydy) + l2 * dvzdz) * dtdh;
+        preN[r][3] = oldF[r][3] + (matF[r][2] * (dvxdy + dvydx)) * dtdh;
+        preN[r][4] = oldF[r][4] + (matF[r][3] * (dvxdz + dvzdx)) * dtdh;
+        preN[r][5] = oldF[r][5] + (matF[r][4] * (dvydz + dvzdy)) * dtdh;
       }
-      float p = Cvt<T>::ld(S.fld[s][0], c, S.fdec[0]) + (Cvt<T>::ld(S.mat[0], c, S.mdec[0]) * dtdh) * ((dx + dy) + dz);
-      inject3d(S, c, it, A.nt, p);
-      Cvt<T>::st(S.fld[o][0], c, p, S.fenc[0]);
+      if (!upd) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) preN[r][f] = oldF[r][f];
+      }
     }
+    if constexpr (!LAG) {
+      // core march: no frame cell, store at once (after the next plane's loads are issued)
+      if (t < t_end) prefetch_cells(t + 1);
+      const bool col_out = tx < W.w;
+#pragma unroll
+      for (int r = 0; r < YPT; ++r) {
+        const int jl = ty + 4 * r;
+        if (!col_out || jl >= W.hy) continue;
+        const unsigned co = (unsigned)(i + (pj0 + jl) * nx);
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          float v = preN[r][f];
+          if (P::inj(f)) inject3d(S, (size_t)co + (size_t)t * sz, it, A.nt, v);
+          Cvt<T>::st(plane_of<T>(S.fld[o][P::out_f(f)], t, sz), co, v, S.fenc[P::out_f(f)]);
+        }
+      }
+      __syncthreads();
+    } else {
+    // publish plane t's pre-correction values; then load the old values and materials of t+1
+    {
+      float* slot = ring + (t % 3) * L::SLOT;
+#pragma unroll
+      for (int r = 0; r < YPT; ++r)
+#pragma unroll
+        for (int f = 0; f < NF; ++f) slot[(f * ROWS + ty + 4 * r) * kTileW + tx] = preN[r][f];
+    }
+    if (t < t_end) prefetch_cells(t + 1);
     __syncthreads();
+
+    // 4. finalise plane t-1: Higdon blend of frame cells, injection, store
+    if (t > W.k0) {
+      const int kk = t - 1;
+      const bool kt = kk <= H, kb = kk >= nz - 1 - H;
+      const bool il = i <= H, ir = i >= nx - 1 - H;
+      const bool col_out = tx >= W.ring && tx < W.ring + W.w;
+      const float* cur = ring + ((t + 2) % 3) * L::SLOT;   // plane t-1
+      const float* prv = ring + ((t + 1) % 3) * L::SLOT;   // plane t-2
 #pragma unroll
-    for (int r = 0; r < YPT; ++r) {
+      for (int r = 0; r < YPT; ++r) {
+        const int jl = ty + 4 * r, j = pj0 + jl;
+        if (!col_out || jl < W.ring || jl >= W.ring + W.hy) continue;
+        const unsigned co = (unsigned)(i + j * nx);
+        const bool jf = j <= H, jb = j >= ny - 1 - H;
+        const int dist = min(min(min(i, nx - 1 - i), min(j, ny - 1 - j)), min(kk, nz - 1 - kk));
+        const bool frame = dist <= H;
 #pragma unroll
-      for (int m = 0; m < 2 * N - 1; ++m) q[r][m] = q[r][m + 1];
-      q[r][2 * N - 1] = ldv(3, i, W.j0 + ty + 4 * r, k + N);
+        for (int f = 0; f < NF; ++f) {
+          const int F = P::out_f(f);
+          const int e = (f * ROWS + jl) * kTileW + tx;
+          float v = cur[e];
+          if (frame) {
+            const float foc = ld(s, F, i, j, kk);
+            float sum = 0.0f;
+            int nd = 0;
+            if (il || ir) {
+              const int d = il ? 1 : -1;
+              sum += (-S.q * cur[e + d] - S.qt * foc) - S.qxt * ld(s, F, i + d, j, kk);
+              ++nd;
+            }
+            if (jf || jb) {
+              const int d = jf ? 1 : -1;
+              sum += (-S.q * cur[e + d * kTileW] - S.qt * foc) - S.qxt * ld(s, F, i, j + d, kk);
+              ++nd;
+            }
+            if (kt || kb) {
+              const float pz = kt ? preN[r][f] : prv[e];
+              sum += (-S.q * pz - S.qt * foc) - S.qxt * ld(s, F, i, j, kt ? kk + 1 : kk - 1);
+              ++nd;
+            }
+            const float wt = G(S.wlut)[P::lut(f) * (H + 1) + dist];
+            v = wt * v + ((1.0f - wt) * sum) / (float)nd;
+          }
+          if (P::inj(f)) inject3d(S, (size_t)co + (size_t)kk * sz, it, A.nt, v);
+          Cvt<T>::st(plane_of<T>(S.fld[o][F], kk, sz), co, v, S.fenc[F]);
+        }
+      }
     }
+    }  // LAG
+
+    // 5. advance the queues to the window of t+1
+#pragma unroll
+    for (int qq = 0; qq < NQ; ++qq)
+#pragma unroll
+      for (int r = 0; r < YPT; ++r) {
+#pragma unroll
+        for (int m = 0; m < 2 * N - 1; ++m) q[qq][r][m] = q[qq][r][m + 1];
+        q[qq][r][2 * N - 1] = qn[qq][r];
+      }
   }
 }
 
-template <int PHASE, int N, class T>
+template <int PH, int N, class T, bool LAG>
 __global__ __launch_bounds__(256) void sweep3d_kernel(const Sweep3DArgs A) {
-  constexpr int LV = (kTileW + 2 * N) * (kTileY3 + 2 * N);
-  constexpr int LP = kTileY3 * (kTileW + 2 * N) + (kTileY3 + 2 * N) * kTileW;
-  __shared__ float lds[LV > LP ? LV : LP];
+  __shared__ float lds[LAG ? Lds3<PH, N>::SIZE : Lds3<PH, N>::STG];
   if ((int)blockIdx.x >= A.n_work) return;
   const Work3D W = A.work[blockIdx.x];
   if (W.kind < 0) return;
-  if (W.kind == 0) {
-    frame_box<N, T>(A, W, PHASE);
-  } else if (PHASE == 0) {
-    interior_vel<N, T>(A, W, lds);
-  } else {
-    interior_prs<N, T>(A, W, lds);
-  }
+  march3d<PH, N, T, LAG>(A, W, lds);
 }
 
 template <class T>
 __global__ __launch_bounds__(256) void gather3d_kernel(const Gather3DArgs A) {
   const int it = *A.step;
-  const Shot3D& S = *A.shot;
+  const Shot3D& S = A.shot;
   if (it < A.nt) {
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < A.n_rec; r += gridDim.x * blockDim.x) {
-      const size_t c = (size_t)A.rec_c[r];
+      const size_t c = (size_t)G(A.rec_c)[r];
       const size_t o = (size_t)r + (size_t)A.n_rec * (size_t)it;
       for (int k = 0; k < A.nslots; ++k) {
         const int f = A.slot_field[k];
-        A.seis[k][o] = Cvt<T>::ld(S.fld[A.parity_out][f], c, S.fdec[f]);
+        G(A.seis[k])[o] = Cvt<T>::ld(S.fld[A.parity_out][f], c, S.fdec[f]);
       }
     }
   }
@@ -430,31 +589,41 @@ hipError_t launch_slice3d(int storage, const void* src, float* dst, int nx, int 
   return hipGetLastError();
 }
 
-template <int PHASE, class T>
+template <int PH, class T, bool LAG>
 static hipError_t launch_n(int N, const Sweep3DArgs& a, hipStream_t s) {
+  if (a.n_work <= 0) return hipSuccess;
   const dim3 blk(kTileW, 4);
   switch (N) {
-    case 1: hipLaunchKernelGGL((sweep3d_kernel<PHASE, 1, T>), dim3(a.n_work), blk, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((sweep3d_kernel<PHASE, 2, T>), dim3(a.n_work), blk, 0, s, a); break;
-    case 3: hipLaunchKernelGGL((sweep3d_kernel<PHASE, 3, T>), dim3(a.n_work), blk, 0, s, a); break;
-    case 4: hipLaunchKernelGGL((sweep3d_kernel<PHASE, 4, T>), dim3(a.n_work), blk, 0, s, a); break;
-    case 5: hipLaunchKernelGGL((sweep3d_kernel<PHASE, 5, T>), dim3(a.n_work), blk, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((sweep3d_kernel<PH, 1, T, LAG>), dim3(a.n_work), blk, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((sweep3d_kernel<PH, 2, T, LAG>), dim3(a.n_work), blk, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((sweep3d_kernel<PH, 3, T, LAG>), dim3(a.n_work), blk, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((sweep3d_kernel<PH, 4, T, LAG>), dim3(a.n_work), blk, 0, s, a); break;
+    case 5: hipLaunchKernelGGL((sweep3d_kernel<PH, 5, T, LAG>), dim3(a.n_work), blk, 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-template <class T>
-static hipError_t launch_t(int phase, int N, const Sweep3DArgs& a, hipStream_t s) {
-  return phase == 0 ? launch_n<0, T>(N, a, s) : launch_n<1, T>(N, a, s);
+template <int PH, class T>
+static hipError_t launch_ph(int N, const Sweep3DArgs& core, const Sweep3DArgs& frame, hipStream_t s) {
+  hipError_t e = launch_n<PH, T, false>(N, core, s);
+  if (e == hipSuccess) e = launch_n<PH, T, true>(N, frame, s);
+  return e;
 }
 
-hipError_t launch_sweep3d(int storage, int phase, int N, const Sweep3DArgs& a, hipStream_t s) {
-  if (a.n_work <= 0) return hipSuccess;
+template <class T>
+static hipError_t launch_t(int eq, int phase, int N, const Sweep3DArgs& c, const Sweep3DArgs& f, hipStream_t s) {
+  if (eq == FOMO_ACOUSTIC3D) return phase == 0 ? launch_ph<kA3Vel, T>(N, c, f, s) : launch_ph<kA3Prs, T>(N, c, f, s);
+  return phase == 0 ? launch_ph<kE3Vel, T>(N, c, f, s) : launch_ph<kE3Str, T>(N, c, f, s);
+}
+
+hipError_t launch_sweep3d(int eq, int storage, int phase, int N, const Sweep3DArgs& core, const Sweep3DArgs& frame,
+                          hipStream_t s) {
+  if (eq != FOMO_ACOUSTIC3D && eq != FOMO_ELASTIC3D) return hipErrorInvalidValue;
   switch (storage) {
-    case FOMO_STORE_F32: return launch_t<float>(phase, N, a, s);
-    case FOMO_STORE_F16: return launch_t<__half>(phase, N, a, s);
-    case FOMO_STORE_BF16: return launch_t<__hip_bfloat16>(phase, N, a, s);
+    case FOMO_STORE_F32: return launch_t<float>(eq, phase, N, core, frame, s);
+    case FOMO_STORE_F16: return launch_t<__half>(eq, phase, N, core, frame, s);
+    case FOMO_STORE_BF16: return launch_t<__hip_bfloat16>(eq, phase, N, core, frame, s);
     default: return hipErrorInvalidValue;
   }
 }

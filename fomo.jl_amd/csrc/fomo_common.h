@@ -18,6 +18,17 @@ namespace fomo {
 
 constexpr int kMaxN = 5;  // fd_order/2 for fd_order <= 10
 
+// Pointers tagged as device global memory.  A plain pointer taken from a table in memory is a
+// generic pointer, and accesses through it compile to flat_* instructions, whose completion is
+// counted together with LDS traffic (a wait for an LDS read then also waits for HBM); accesses
+// through these compile to global_* instructions.
+template <class T>
+using gp = T __attribute__((address_space(1)))*;
+template <class T>
+__device__ __forceinline__ gp<T> G(T* p) {
+  return (gp<T>)p;
+}
+
 // Sweep phases.
 enum Phase : int { kAcVel = 0, kAcPrs = 1, kElVel = 2, kElStr = 3 };
 

@@ -1,5 +1,5 @@
-// plan3d.hip -- 3D acoustic plan: setup (padding, medium, HABC tables, storage scales), work
-// list, step enqueue, snapshots and the acoustic3d one-shot entry point.
+// plan3d.hip -- 3D plans (acoustic3d, elastic3d): setup (padding, media, HABC tables, storage
+// scales), work list, step enqueue, snapshots and the acoustic3d / elastic3d one-shot entry points.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
@@ -34,11 +34,12 @@ static void parallel_for(long long n, const std::function<void(long long, long l
 
 struct Plan3D : PlanBase {
   int N = 0, storage = 0, nxi = 0, nyi = 0, nzi = 0, pad = 0, elem = 4;
+  int nf = 4;       // fields: acoustic 4, elastic 9
+  int snap_f = 0;   // field of the snapshots: acoustic p, elastic vz
   float coef[kMaxN] = {0};
   Shot3D shot{};
-  Shot3D* d_shot = nullptr;
-  Work3D* d_work = nullptr;
-  int n_work = 0;
+  Work3D* d_work[2] = {nullptr, nullptr};  // core items, frame items
+  int n_work[2] = {0, 0};
   long long* rec_c = nullptr;
   float* seis[4] = {nullptr};
   int nslots = 3;
@@ -48,9 +49,10 @@ struct Plan3D : PlanBase {
   hipError_t enqueue_part(int part, int s) override {
     if (part == 2) {
       Gather3DArgs g{};
-      g.shot = d_shot;
+      g.shot = shot;
       g.rec_c = rec_c;
-      const int fields[4] = {1, 2, 3, 0};
+      const bool ac = d.eq == FOMO_ACOUSTIC3D;
+      const int fields[4] = {ac ? 1 : 0, ac ? 2 : 1, ac ? 3 : 2, 0};  // vx, vy, vz (, p)
       for (int k = 0; k < nslots; ++k) {
         g.seis[k] = seis[k];
         g.slot_field[k] = fields[k];
@@ -64,20 +66,23 @@ struct Plan3D : PlanBase {
       return launch_gather3d(storage, g, stream);
     }
     Sweep3DArgs a{};
-    a.shot = d_shot;
-    a.work = d_work;
-    a.n_work = n_work;
+    a.shot = shot;
     a.parity = s;
     for (int l = 0; l < N; ++l) a.a[l] = coef[l];
     a.step = step;
     a.nt = d.nt;
-    return launch_sweep3d(storage, part, N, a, stream);
+    Sweep3DArgs f = a;
+    a.work = d_work[0];
+    a.n_work = n_work[0];
+    f.work = d_work[1];
+    f.n_work = n_work[1];
+    return launch_sweep3d(d.eq, storage, part, N, a, f, stream);
   }
 
   int reset() override {
     const size_t n = (size_t)shot.nx * shot.ny * shot.nz;
     for (int b = 0; b < 2; ++b)
-      for (int f = 0; f < 4; ++f) FOMO_HIP_TRY(hipMemsetAsync(shot.fld[b][f], 0, n * elem, stream));
+      for (int f = 0; f < nf; ++f) FOMO_HIP_TRY(hipMemsetAsync(shot.fld[b][f], 0, n * elem, stream));
     const size_t ns = (size_t)d.n_rec * d.nt;
     for (int k = 0; k < nslots; ++k)
       if (ns) FOMO_HIP_TRY(hipMemsetAsync(seis[k], 0, ns * sizeof(float), stream));
@@ -113,7 +118,7 @@ struct Plan3D : PlanBase {
 
   int read_field(int sh, int which, float* out) override {
     if (sh != 0) return set_error(FOMO_EINVAL, "shot index out of range");
-    if (which < 0 || which >= 4) return set_error(FOMO_EINVAL, "field index out of range");
+    if (which < 0 || which >= nf) return set_error(FOMO_EINVAL, "field index out of range");
     const size_t n = (size_t)shot.nx * shot.ny * shot.nz;
     int rc = ensure_scratch(n);
     if (rc) return rc;
@@ -123,13 +128,14 @@ struct Plan3D : PlanBase {
     return FOMO_OK;
   }
 
-  // Interior slice of p (plane 0 xz, 1 xy, 2 yz; idx1 = 1-based interior index) into host dst.
+  // Interior slice of the snapshot field (acoustic p, elastic vz; plane 0 xz, 1 xy, 2 yz; idx1 =
+  // 1-based interior index) into host dst.
   int snapshot(float* dst, int plane, int idx1) {
     const size_t n = plane == 0 ? (size_t)nxi * nzi : (plane == 1 ? (size_t)nxi * nyi : (size_t)nyi * nzi);
     int rc = ensure_scratch(n);
     if (rc) return rc;
-    FOMO_HIP_TRY(launch_slice3d(storage, shot.fld[steps_done % 2][0], scratch, shot.nx, shot.ny, shot.nz, pad, plane,
-                                idx1 - 1 + pad, shot.fdec[0], stream));
+    FOMO_HIP_TRY(launch_slice3d(storage, shot.fld[steps_done % 2][snap_f], scratch, shot.nx, shot.ny, shot.nz, pad,
+                                plane, idx1 - 1 + pad, shot.fdec[snap_f], stream));
     FOMO_HIP_TRY(hipMemcpyAsync(dst, scratch, n * sizeof(float), hipMemcpyDeviceToHost, stream));
     return FOMO_OK;
   }
@@ -145,32 +151,13 @@ struct Plan3D : PlanBase {
   int n_shots() const override { return 1; }
 };
 
-// Work list: frame boxes (64 x 4 cells of one plane) over the six pad slabs and interior column
-// tiles (64 x kTileY3) marched over z-segments of `kseg` planes; ordered by (z, y, x) and split
-// into 8 cost-balanced contiguous runs, one per XCD (block b runs on XCD b % 8).
-static int make_work3d(Plan3D* p, int kseg) {
-  std::vector<Work3D> items;
-  const int nx = p->shot.nx, ny = p->shot.ny, nz = p->shot.nz, pad = p->pad;
-  const int xe = pad + p->nxi, ye = pad + p->nyi, ze = pad + p->nzi;
-  auto frame_rect = [&](int x0, int x1, int y0, int y1, int k) {
-    for (int j = y0; j < y1; j += 4)
-      for (int i = x0; i < x1; i += kTileW)
-        items.push_back(Work3D{0, i, j, k, std::min(kTileW, x1 - i), std::min(4, y1 - j), 1});
-  };
-  for (int k = 0; k < nz; ++k) {
-    if (k < pad || k >= ze) {
-      frame_rect(0, nx, 0, ny, k);
-    } else {
-      frame_rect(0, nx, 0, pad, k);
-      frame_rect(0, nx, ye, ny, k);
-      frame_rect(0, pad, pad, ye, k);
-      frame_rect(xe, nx, pad, ye, k);
-    }
-  }
-  for (int k = pad; k < ze; k += kseg)
-    for (int j = pad; j < ye; j += kTileY3)
-      for (int i = pad; i < xe; i += kTileW)
-        items.push_back(Work3D{1, i, j, k, std::min(kTileW, xe - i), std::min(kTileY3, ye - j), std::min(kseg, ze - k)});
+// Work lists over the cells 1..n-2 of each axis (the outer ring is never written).  The x-y
+// plane is cut into core tiles (64 x rows over the user's interior, no x/y frame cell) and ring
+// tiles (62 x (rows-2) over the pad bands).  Core tiles over the user's z range form the core
+// list; ring tiles over all of z and core tiles over the top and bottom pad slabs form the frame
+// list.  Each column is marched over z-segments of about `kseg` planes.  Each list is ordered by
+// (z, y, x) and split into 8 cost-balanced contiguous runs, one per XCD (block b runs on XCD b % 8).
+static int upload_work(Plan3D* p, int which, std::vector<Work3D>& items, int rows) {
   std::stable_sort(items.begin(), items.end(), [](const Work3D& a, const Work3D& b) {
     if (a.k0 != b.k0) return a.k0 < b.k0;
     if (a.j0 != b.j0) return a.j0 < b.j0;
@@ -179,7 +166,7 @@ static int make_work3d(Plan3D* p, int kseg) {
   std::vector<double> cost(items.size());
   double total = 0.0;
   for (size_t q = 0; q < items.size(); ++q) {
-    cost[q] = (double)items[q].w * items[q].hy * items[q].hz * (items[q].kind == 0 ? 3.0 : 1.0);
+    cost[q] = (double)kTileW * rows * (items[q].hz + 2);
     total += cost[q];
   }
   std::vector<std::vector<Work3D>> runs(8);
@@ -191,12 +178,51 @@ static int make_work3d(Plan3D* p, int kseg) {
   }
   size_t len = 0;
   for (auto& r : runs) len = std::max(len, r.size());
-  std::vector<Work3D> flat(8 * len, Work3D{-1, 0, 0, 0, 0, 0, 0});
+  std::vector<Work3D> flat(8 * len, Work3D{-1, 0, 0, 0, 0, 0, 0, 0});
   for (int r = 0; r < 8; ++r)
     for (size_t q = 0; q < runs[r].size(); ++q) flat[q * 8 + r] = runs[r][q];
   if (flat.size() > (size_t)0x7fffffff) return set_error(FOMO_EUNSUP, "grid too large for one launch");
-  p->n_work = (int)flat.size();
-  return p->upload(&p->d_work, flat.data(), flat.size());
+  p->n_work[which] = (int)flat.size();
+  return p->upload(&p->d_work[which], flat.data(), flat.size());
+}
+
+static int make_work3d(Plan3D* p, int kseg, int rows) {
+  const int nx = p->shot.nx, ny = p->shot.ny, nz = p->shot.nz, pad = p->pad;
+  std::vector<Work3D> core_fp, ring_fp;
+  auto ring_rect = [&](int x0, int x1, int y0, int y1) {
+    for (int j = y0; j < y1; j += rows - 2)
+      for (int i = x0; i < x1; i += kTileW - 2)
+        ring_fp.push_back(Work3D{1, i, j, 0, std::min(kTileW - 2, x1 - i), std::min(rows - 2, y1 - j), 0, 1});
+  };
+  for (int j = pad; j < ny - pad; j += rows)
+    for (int i = pad; i < nx - pad; i += kTileW)
+      core_fp.push_back(Work3D{1, i, j, 0, std::min(kTileW, nx - pad - i), std::min(rows, ny - pad - j), 0, 0});
+  ring_rect(1, nx - 1, 1, pad);
+  ring_rect(1, pad, pad, ny - pad);
+  ring_rect(nx - pad, nx - 1, pad, ny - pad);
+  ring_rect(1, nx - 1, ny - pad, ny - 1);
+  // z-segments of [z0, z1) of about kseg planes each
+  auto segs = [&](int z0, int z1, const std::vector<Work3D>& fp, std::vector<Work3D>& out) {
+    const int nk = z1 - z0;
+    if (nk <= 0) return;
+    const int nseg = std::max(1, (nk + kseg - 1) / kseg);
+    const int hz = (nk + nseg - 1) / nseg;
+    for (int k = z0; k < z1; k += hz)
+      for (const Work3D& f : fp) {
+        Work3D w = f;
+        w.k0 = k;
+        w.hz = std::min(hz, z1 - k);
+        out.push_back(w);
+      }
+  };
+  std::vector<Work3D> core, frame;
+  segs(pad, nz - pad, core_fp, core);
+  segs(1, pad, core_fp, frame);
+  segs(nz - pad, nz - 1, core_fp, frame);
+  segs(1, nz - 1, ring_fp, frame);
+  int rc = upload_work(p, 0, core, rows);
+  if (!rc) rc = upload_work(p, 1, frame, rows);
+  return rc;
 }
 
 template <class T>
@@ -235,11 +261,12 @@ using namespace fomo;
 int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, const float* rho, const int32_t* sx,
                        const int32_t* sy, const int32_t* sz, const float* wavelet, const int32_t* rx,
                        const int32_t* ry, const int32_t* rz, fomo_plan** out) {
-  (void)vs;
   *out = nullptr;
-  if (d->eq != FOMO_ACOUSTIC3D) return set_error(FOMO_EUNSUP, "elastic3d is not provided by this build");
+  const bool elastic = d->eq == FOMO_ELASTIC3D;
+  if (d->eq != FOMO_ACOUSTIC3D && !elastic) return set_error(FOMO_EINVAL, "not a 3D equation");
   if (d->storage < FOMO_STORE_F32 || d->storage > FOMO_STORE_BF16) return set_error(FOMO_EINVAL, "unknown storage");
-  if (!vp || !rho) return set_error(FOMO_EINVAL, "model arrays must not be NULL");
+  if (!vp || !rho || (elastic && !vs)) return set_error(FOMO_EINVAL, "model arrays must not be NULL");
+  if (elastic && d->record_p) return set_error(FOMO_EINVAL, "elastic3d has no pressure field to record");
   if (d->n_src > 0 && (!sx || !sy || !sz || !wavelet)) return set_error(FOMO_EINVAL, "source arrays must not be NULL");
   if (d->n_rec > 0 && (!rx || !ry || !rz)) return set_error(FOMO_EINVAL, "receiver arrays must not be NULL");
   for (int s = 0; s < d->n_src; ++s)
@@ -257,6 +284,8 @@ int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, con
   p->N = d->fd_order / 2;
   p->storage = d->storage;
   p->elem = d->storage == FOMO_STORE_F32 ? 4 : 2;
+  p->nf = elastic ? 9 : 4;
+  p->snap_f = elastic ? 2 : 0;
   fomo_fd_coefficients(d->fd_order, p->coef);
   p->nxi = d->nx;
   p->nyi = d->ny;
@@ -282,46 +311,53 @@ int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, con
   S.q = hc[0];
   S.qt = hc[1];
   S.qxt = hc[2];
-  // medium: pad_array_3d.jl:8-29 (clamp indexing), kappa = rho*vp^2, buoyancy pad_array_3d.jl:37-89
-  std::vector<float> rhop(n), kappa(n);
-  parallel_for(S.nz, [&](long long k0, long long k1) {
-    for (long long k = k0; k < k1; ++k) {
-      const int kk = std::min(std::max((int)k - pad, 0), d->nz - 1);
-      for (int j = 0; j < S.ny; ++j) {
-        const int jj = std::min(std::max(j - pad, 0), d->ny - 1);
-        for (int i = 0; i < S.nx; ++i) {
-          const int ii = std::min(std::max(i - pad, 0), d->nx - 1);
-          const size_t src = (size_t)ii + (size_t)jj * d->nx + (size_t)kk * ixy;
-          const size_t c = (size_t)i + j * sy_ + k * sz_;
-          rhop[c] = rho[src];
-          kappa[c] = rho[src] * (vp[src] * vp[src]);
+  const bool scaled = d->storage != FOMO_STORE_F32;
+  int rc;
+  // Upload one material array (index m) with its storage scale (max |value| when scaled).
+  auto put_mat = [&](int m, const std::vector<float>& h) -> int {
+    float mx = 0.0f;
+    for (size_t c = 0; c < h.size(); ++c) mx = std::max(mx, std::fabs(h[c]));
+    const float dec = scaled && mx > 0.0f ? mx : 1.0f;
+    void* dm = nullptr;
+    int r2 = upload_any(p, &dm, h, 1.0f / dec);
+    if (r2) return r2;
+    S.mat[m] = dm;
+    S.mdec[m] = dec;
+    return FOMO_OK;
+  };
+  // padding: pad_array_3d.jl:8-29 (clamp indexing)
+  auto pad3 = [&](const float* in, std::vector<float>& o) {
+    o.assign(n, 0.0f);
+    parallel_for(S.nz, [&](long long k0, long long k1) {
+      for (long long k = k0; k < k1; ++k) {
+        const int kk = std::min(std::max((int)k - pad, 0), d->nz - 1);
+        for (int j = 0; j < S.ny; ++j) {
+          const int jj = std::min(std::max(j - pad, 0), d->ny - 1);
+          for (int i = 0; i < S.nx; ++i) {
+            const int ii = std::min(std::max(i - pad, 0), d->nx - 1);
+            o[(size_t)i + j * sy_ + k * sz_] = in[(size_t)ii + (size_t)jj * d->nx + (size_t)kk * ixy];
+          }
         }
       }
-    }
-  });
+    });
+  };
+  std::vector<float> rhop;
+  pad3(rho, rhop);
+  const int mb = elastic ? 0 : 1;  // index of bx among the materials
+  if (!elastic) {
+    // kappa = rho * vp^2 (acoustic3d/medium.jl:40-62)
+    std::vector<float> vpp, kappa(n);
+    pad3(vp, vpp);
+    for (size_t c = 0; c < n; ++c) kappa[c] = rhop[c] * (vpp[c] * vpp[c]);
+    if ((rc = put_mat(0, kappa))) return bail(rc);
+  }
+  // staggered buoyancy, pad_array_3d.jl:37-89
   auto hb = [](float r1, float r2) {
     if (r1 == 0.0f && r2 == 0.0f) return 0.0f;
     if (r1 == 0.0f) return 2.0f / r2;
     if (r2 == 0.0f) return 2.0f / r1;
     return 2.0f / (r1 + r2);
   };
-  float kmax = 0.0f, bmax = 0.0f, zmin = INFINITY;
-  for (size_t c = 0; c < n; c += 1) kmax = std::max(kmax, std::fabs(kappa[c]));
-  // impedance floor for the velocity scale: min rho*vp over cells with vp > 0
-  for (size_t c = 0; c < ixy * d->nz; ++c)
-    if (vp[c] > 0.0f && rho[c] > 0.0f) zmin = std::min(zmin, rho[c] * vp[c]);
-  float wmax = 0.0f;
-  for (size_t c = 0; c < (size_t)d->n_src * d->nt; ++c) wmax = std::max(wmax, std::fabs(wavelet[c]));
-  const bool scaled = d->storage != FOMO_STORE_F32;
-  int rc;
-  {
-    void* m0 = nullptr;
-    const float kdec = scaled && kmax > 0.0f ? kmax : 1.0f;
-    if ((rc = upload_any(p, &m0, kappa, 1.0f / kdec))) return bail(rc);
-    S.mat[0] = m0;
-    S.mdec[0] = scaled ? kdec : 1.0f;
-    std::vector<float>().swap(kappa);
-  }
   for (int ax = 0; ax < 3; ++ax) {
     std::vector<float> b(n, 0.0f);
     parallel_for(S.nz, [&](long long k0, long long k1) {
@@ -344,27 +380,80 @@ int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, con
     if (ax == 2)
       for (int j = 0; j < S.ny; ++j)
         for (int i = 0; i < S.nx; ++i) b[i + j * sy_ + (S.nz - 1) * sz_] = b[i + j * sy_ + (S.nz - 2) * sz_];
-    float bm = 0.0f;
-    for (size_t c = 0; c < n; ++c) bm = std::max(bm, b[c]);
-    bmax = std::max(bmax, bm);
-    const float bdec = scaled && bm > 0.0f ? bm : 1.0f;
-    void* m = nullptr;
-    if ((rc = upload_any(p, &m, b, 1.0f / bdec))) return bail(rc);
-    S.mat[1 + ax] = m;
-    S.mdec[1 + ax] = scaled ? bdec : 1.0f;
+    if ((rc = put_mat(mb + ax, b))) return bail(rc);
+  }
+  if (elastic) {
+    // elastic3d/medium.jl:76-123: mu = rho vs^2, lam = rho vp^2 - 2 mu, lam2mu = lam + 2 mu, and the
+    // four-point harmonic means of mu at txy, txz, tyz (0 if any of the four is 0), edge-copied.
+    std::vector<float> vpp, vsp;
+    pad3(vp, vpp);
+    pad3(vs, vsp);
+    std::vector<float> mu(n), lam(n), l2m(n);
+    for (size_t c = 0; c < n; ++c) {
+      mu[c] = rhop[c] * (vsp[c] * vsp[c]);
+      lam[c] = rhop[c] * (vpp[c] * vpp[c]) - 2.0f * mu[c];
+      l2m[c] = lam[c] + 2.0f * mu[c];
+    }
+    std::vector<float>().swap(vpp);
+    std::vector<float>().swap(vsp);
+    if ((rc = put_mat(3, lam)) || (rc = put_mat(4, l2m))) return bail(rc);
+    std::vector<float>().swap(lam);
+    std::vector<float>().swap(l2m);
+    auto h4 = [](float m1, float m2, float m3, float m4) {
+      if (m1 == 0.0f || m2 == 0.0f || m3 == 0.0f || m4 == 0.0f) return 0.0f;
+      return 4.0f / (((1.0f / m1 + 1.0f / m2) + 1.0f / m3) + 1.0f / m4);
+    };
+    const int NX = S.nx, NY = S.ny, NZ = S.nz;
+    for (int which = 0; which < 3; ++which) {
+      std::vector<float> m(n, 0.0f);
+      parallel_for(NZ, [&](long long k0, long long k1) {
+        for (long long k = k0; k < k1; ++k)
+          for (int j = 0; j < NY; ++j)
+            for (int i = 0; i < NX; ++i) {
+              const size_t c = (size_t)i + j * sy_ + k * sz_;
+              if (which == 0 && i >= 1 && j >= 1)
+                m[c] = h4(mu[c - 1 - sy_], mu[c - sy_], mu[c - 1], mu[c]);
+              else if (which == 1 && i >= 1 && k + 1 < NZ)
+                m[c] = h4(mu[c - 1], mu[c], mu[c - 1 + sz_], mu[c + sz_]);
+              else if (which == 2 && j >= 1 && k + 1 < NZ)
+                m[c] = h4(mu[c - sy_], mu[c], mu[c - sy_ + sz_], mu[c + sz_]);
+            }
+      });
+      if (which == 0) {  // mu_txy[1,:,:] = mu_txy[2,:,:]; mu_txy[:,1,:] = mu_txy[:,2,:]
+        for (int k = 0; k < NZ; ++k)
+          for (int j = 0; j < NY; ++j) m[j * sy_ + k * sz_] = m[1 + j * sy_ + k * sz_];
+        for (int k = 0; k < NZ; ++k)
+          for (int i = 0; i < NX; ++i) m[i + k * sz_] = m[i + sy_ + k * sz_];
+      } else if (which == 1) {  // mu_txz[1,:,:] = mu_txz[2,:,:]; mu_txz[:,:,nz] = mu_txz[:,:,nz-1]
+        for (int k = 0; k < NZ; ++k)
+          for (int j = 0; j < NY; ++j) m[j * sy_ + k * sz_] = m[1 + j * sy_ + k * sz_];
+        for (int j = 0; j < NY; ++j)
+          for (int i = 0; i < NX; ++i) m[i + j * sy_ + (NZ - 1) * sz_] = m[i + j * sy_ + (NZ - 2) * sz_];
+      } else {  // mu_tyz[:,1,:] = mu_tyz[:,2,:]; mu_tyz[:,:,nz] = mu_tyz[:,:,nz-1]
+        for (int k = 0; k < NZ; ++k)
+          for (int i = 0; i < NX; ++i) m[i + k * sz_] = m[i + sy_ + k * sz_];
+        for (int j = 0; j < NY; ++j)
+          for (int i = 0; i < NX; ++i) m[i + j * sy_ + (NZ - 1) * sz_] = m[i + j * sy_ + (NZ - 2) * sz_];
+      }
+      if ((rc = put_mat(5 + which, m))) return bail(rc);
+    }
   }
   std::vector<float>().swap(rhop);
-  // field scales: p in units of the peak source amplitude, velocities in p/(min impedance)
-  const float pdec = scaled ? (wmax > 0.0f ? wmax : 1.0f) : 1.0f;
-  const float vdec = scaled ? (std::isfinite(zmin) && zmin > 0.0f ? pdec / zmin : pdec) : 1.0f;
-  S.fdec[0] = pdec;
-  S.fenc[0] = 1.0f / pdec;
-  for (int f = 1; f < 4; ++f) {
-    S.fdec[f] = vdec;
-    S.fenc[f] = 1.0f / vdec;
+  // field scales: p / stresses in units of the peak source amplitude, velocities in that over the
+  // smallest impedance rho*vp (over cells with vp, rho > 0)
+  float zmin = INFINITY, wmax = 0.0f;
+  for (size_t c = 0; c < ixy * d->nz; ++c)
+    if (vp[c] > 0.0f && rho[c] > 0.0f) zmin = std::min(zmin, rho[c] * vp[c]);
+  for (size_t c = 0; c < (size_t)d->n_src * d->nt; ++c) wmax = std::max(wmax, std::fabs(wavelet[c]));
+  const float sdec = scaled ? (wmax > 0.0f ? wmax : 1.0f) : 1.0f;
+  const float vdec = scaled ? (std::isfinite(zmin) && zmin > 0.0f ? sdec / zmin : sdec) : 1.0f;
+  for (int f = 0; f < p->nf; ++f) {
+    const bool vel = elastic ? f < 3 : f >= 1;
+    S.fdec[f] = vel ? vdec : sdec;
+    S.fenc[f] = 1.0f / S.fdec[f];
   }
   for (int b = 0; b < 2; ++b)
-    for (int f = 0; f < 4; ++f) {
+    for (int f = 0; f < p->nf; ++f) {
       char* ptr = nullptr;
       if ((rc = p->alloc(&ptr, n * p->elem))) return bail(rc);
       S.fld[b][f] = ptr;
@@ -401,9 +490,9 @@ int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, con
     for (int k = 0; k < p->nslots; ++k)
       if ((rc = p->alloc(&p->seis[k], (size_t)d->n_rec * d->nt))) return bail(rc);
   }
-  if ((rc = p->upload(&p->d_shot, &p->shot, 1))) return bail(rc);
   if ((rc = p->alloc(&p->step, 1)) || (rc = p->alloc(&p->done, 1))) return bail(rc);
-  if ((rc = make_work3d(p, d->tile_cfg > 0 ? 16 << (d->tile_cfg - 1) : 64))) return bail(rc);
+  const int kseg = d->tile_cfg > 0 ? 16 << (d->tile_cfg - 1) : 64;
+  if ((rc = make_work3d(p, kseg, elastic ? kRowsEl3 : kRowsAc3))) return bail(rc);
   if ((rc = p->reset())) return bail(rc);
   if (d->use_graph && (rc = p->build_graph())) return bail(rc);
   e = hipStreamSynchronize(p->stream);
@@ -412,15 +501,13 @@ int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, con
   return FOMO_OK;
 }
 
-extern "C" int fomo_acoustic3d(const fomo_desc* d, const float* vp, const float* rho, const int32_t* sx,
-                               const int32_t* sy, const int32_t* sz, const float* wavelet, const int32_t* rx,
-                               const int32_t* ry, const int32_t* rz, float* seis_vx, float* seis_vy, float* seis_vz,
-                               float* seis_p, float* snaps, double* kernel_time_s) {
-  if (!d || d->eq != FOMO_ACOUSTIC3D) return set_error(FOMO_EINVAL, "descriptor eq must be FOMO_ACOUSTIC3D");
-  fomo_desc dd = *d;
-  dd.record_p = seis_p ? 1 : 0;
+// Shared one-shot driver of acoustic3d and elastic3d: warm-up step, reset, timed loop with
+// snapshots, seismogram download.
+static int run3d(const fomo_desc* d, const float* vp, const float* vs, const float* rho, const int32_t* sx,
+                 const int32_t* sy, const int32_t* sz, const float* wavelet, const int32_t* rx, const int32_t* ry,
+                 const int32_t* rz, float* const* seis, float* snaps, double* kernel_time_s) {
   fomo_plan* h = nullptr;
-  int rc = fomo_plan_create(&dd, vp, nullptr, rho, sx, sy, sz, wavelet, rx, ry, rz, &h);
+  int rc = fomo_plan_create(d, vp, vs, rho, sx, sy, sz, wavelet, rx, ry, rz, &h);
   if (rc) return rc;
   Plan3D* p = static_cast<Plan3D*>(h->impl);
   auto finish = [&](int code) {
@@ -454,7 +541,28 @@ extern "C" int fomo_acoustic3d(const fomo_desc* d, const float* vp, const float*
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (kernel_time_s) *kernel_time_s = ms * 1e-3;
-  float* dst[4] = {seis_vx, seis_vy, seis_vz, seis_p};
-  rc = p->read_seis(0, dst);
+  rc = p->read_seis(0, seis);
   return finish(rc);
+}
+
+extern "C" int fomo_acoustic3d(const fomo_desc* d, const float* vp, const float* rho, const int32_t* sx,
+                               const int32_t* sy, const int32_t* sz, const float* wavelet, const int32_t* rx,
+                               const int32_t* ry, const int32_t* rz, float* seis_vx, float* seis_vy, float* seis_vz,
+                               float* seis_p, float* snaps, double* kernel_time_s) {
+  if (!d || d->eq != FOMO_ACOUSTIC3D) return set_error(FOMO_EINVAL, "descriptor eq must be FOMO_ACOUSTIC3D");
+  fomo_desc dd = *d;
+  dd.record_p = seis_p ? 1 : 0;
+  float* dst[4] = {seis_vx, seis_vy, seis_vz, seis_p};
+  return run3d(&dd, vp, nullptr, rho, sx, sy, sz, wavelet, rx, ry, rz, dst, snaps, kernel_time_s);
+}
+
+extern "C" int fomo_elastic3d(const fomo_desc* d, const float* vp, const float* vs, const float* rho,
+                              const int32_t* sx, const int32_t* sy, const int32_t* sz, const float* wavelet,
+                              const int32_t* rx, const int32_t* ry, const int32_t* rz, float* seis_vx,
+                              float* seis_vy, float* seis_vz, float* snaps, double* kernel_time_s) {
+  if (!d || d->eq != FOMO_ELASTIC3D) return set_error(FOMO_EINVAL, "descriptor eq must be FOMO_ELASTIC3D");
+  fomo_desc dd = *d;
+  dd.record_p = 0;
+  float* dst[4] = {seis_vx, seis_vy, seis_vz, nullptr};
+  return run3d(&dd, vp, vs, rho, sx, sy, sz, wavelet, rx, ry, rz, dst, snaps, kernel_time_s);
 }

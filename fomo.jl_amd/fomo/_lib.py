@@ -65,6 +65,8 @@ SIGNATURES = {
                                  C.POINTER(C.c_double)]),
     "fomo_acoustic3d": (C.c_int, [C.POINTER(Desc), _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p,
                                   C.POINTER(C.c_double)]),
+    "fomo_elastic3d": (C.c_int, [C.POINTER(Desc), _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p,
+                                 C.POINTER(C.c_double)]),
     "fomo_acoustic2d_batch": (C.c_int, [C.POINTER(Desc), _p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p,
                                         C.POINTER(C.c_double)]),
     "fomo_elastic2d_batch": (C.c_int, [C.POINTER(Desc), _p, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p,

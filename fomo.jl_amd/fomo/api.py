@@ -246,17 +246,15 @@ def storage_code(storage) -> int:
     return _STORAGE[key]
 
 
-def acoustic3d(vp, rho, dh, dt, nt, f0, *, sx, sy, sz, rx, ry, rz, nbc=50, fd_order=8, snap_interval=0,
-               snap_plane="xz", snap_index=0, v_ref=None, storage="f32", wavelet=None, record_p=False,
-               use_cuda_graph=True, extended=False):
-    """acoustic3d (src/equations/acoustic3d/acoustic3d.jl:31-129) -> (seis_vx, seis_vy, seis_vz, snaps).
-
-    Extensions: `storage` (:f32, :f16, :bf16 storage of wavefields and materials, FP32 arithmetic),
-    `wavelet` (custom source time function), `record_p` and `extended=True` (also return seis_p and
-    stats).  Every source gets the wavelet (the reference reshapes one wavelet to 1 x nt)."""
+def _run3d(eq, vp, vs, rho, dh, dt, nt, f0, sx, sy, sz, rx, ry, rz, nbc, fd_order, snap_interval, snap_plane,
+           snap_index, v_ref, storage, wavelet, want_p, use_cuda_graph):
     vp, rho = _f32F(vp), _f32F(rho)
     if vp.ndim != 3 or vp.shape != rho.shape:
         raise ArgumentError("vp and rho must be 3D arrays of the same size")
+    if vs is not None:
+        vs = _f32F(vs)
+        if vs.shape != vp.shape:
+            raise ArgumentError("vp, vs and rho must be 3D arrays of the same size")
     if fd_order not in (2, 4, 6, 8, 10):
         raise ValueError(f"Unsupported FD order: {fd_order}")
     if nbc < 0 or nt < 0 or snap_interval < 0:
@@ -266,7 +264,7 @@ def acoustic3d(vp, rho, dh, dt, nt, f0, *, sx, sy, sz, rx, ry, rz, nbc=50, fd_or
     if plane is None:
         raise ArgumentError(f"snap_plane must be :xz, :xy or :yz, got {snap_plane}")
     if snap_index == 0:
-        snap_index = ny // 2  # acoustic3d.jl:53-55 (the reference uses ny/2 for every plane)
+        snap_index = ny // 2  # acoustic3d.jl:53-55, elastic3d.jl:49-51 (ny/2 for every plane)
     from .checks import check_geometry_3d
     check_geometry_3d(nx, ny, nz, sx, sy, sz, rx, ry, rz)
     v_ref = _default_vref(vp) if v_ref is None else float(np.float32(v_ref))
@@ -275,25 +273,63 @@ def acoustic3d(vp, rho, dh, dt, nt, f0, *, sx, sy, sz, rx, ry, rz, nbc=50, fd_or
     if len(w) != nt:
         raise ArgumentError(f"wavelet length {len(w)} != nt={nt}")
     wm = np.asfortranarray(np.repeat(w[None, :], len(sx), axis=0))
-    d = _desc(_lib.ACOUSTIC3D, nx, nz, nbc, fd_order, nt, dh, dt, v_ref, len(sx), len(rx), snap_interval,
+    d = _desc(eq, nx, nz, nbc, fd_order, nt, dh, dt, v_ref, len(sx), len(rx), snap_interval,
               use_cuda_graph, ny=ny, storage=storage_code(storage))
     d.snap_plane, d.snap_index = plane, snap_index
     n_rec = len(rx)
     outs = [np.zeros((n_rec, nt), np.float32, order="F") for _ in range(4)]
-    want_p = record_p or extended
     nsnap = nt // snap_interval if snap_interval > 0 else 0
     a_n, b_n = {0: (nx, nz), 1: (nx, ny), 2: (ny, nz)}[plane]
     snaps = np.zeros((nsnap, b_n, a_n), np.float32) if nsnap else None
     t = C.c_double(0.0)
-    _lib.check(_lib.lib().fomo_acoustic3d(C.byref(d), vp.ctypes.data, rho.ctypes.data, sx.ctypes.data,
-                                          sy.ctypes.data, sz.ctypes.data, wm.ctypes.data, rx.ctypes.data,
-                                          ry.ctypes.data, rz.ctypes.data, outs[0].ctypes.data, outs[1].ctypes.data,
-                                          outs[2].ctypes.data, outs[3].ctypes.data if want_p else None,
-                                          None if snaps is None else snaps.ctypes.data, C.byref(t)))
+    common = (sx.ctypes.data, sy.ctypes.data, sz.ctypes.data, wm.ctypes.data, rx.ctypes.data, ry.ctypes.data,
+              rz.ctypes.data, outs[0].ctypes.data, outs[1].ctypes.data, outs[2].ctypes.data)
+    tail = (None if snaps is None else snaps.ctypes.data, C.byref(t))
+    if eq == _lib.ACOUSTIC3D:
+        _lib.check(_lib.lib().fomo_acoustic3d(C.byref(d), vp.ctypes.data, rho.ctypes.data, *common,
+                                              outs[3].ctypes.data if want_p else None, *tail))
+    else:
+        _lib.check(_lib.lib().fomo_elastic3d(C.byref(d), vp.ctypes.data, vs.ctypes.data, rho.ctypes.data, *common,
+                                             *tail))
     snap_list = [s.T for s in snaps] if snaps is not None else []
+    return outs, snap_list, float(t.value)
+
+
+def acoustic3d(vp, rho, dh, dt, nt, f0, *, sx, sy, sz, rx, ry, rz, nbc=50, fd_order=8, snap_interval=0,
+               snap_plane="xz", snap_index=0, v_ref=None, storage="f32", wavelet=None, record_p=False,
+               use_cuda_graph=True, extended=False):
+    """acoustic3d (src/equations/acoustic3d/acoustic3d.jl:31-129) -> (seis_vx, seis_vy, seis_vz, snaps).
+
+    Extensions: `storage` (:f32, :f16, :bf16 storage of wavefields and materials, FP32 arithmetic),
+    `wavelet` (custom source time function), `record_p` and `extended=True` (also return seis_p and
+    stats).  Every source gets the wavelet (the reference reshapes one wavelet to 1 x nt)."""
+    outs, snap_list, t = _run3d(_lib.ACOUSTIC3D, vp, None, rho, dh, dt, nt, f0, sx, sy, sz, rx, ry, rz, nbc,
+                                fd_order, snap_interval, snap_plane, snap_index, v_ref, storage, wavelet,
+                                record_p or extended, use_cuda_graph)
     if extended:
-        return Acoustic3DExtended(outs[0], outs[1], outs[2], snap_list, outs[3], Stats(float(t.value)))
+        return Acoustic3DExtended(outs[0], outs[1], outs[2], snap_list, outs[3], Stats(t))
     return Acoustic3DResult(outs[0], outs[1], outs[2], snap_list)
+
+
+Elastic3DResult = namedtuple("Elastic3DResult", ["seis_vx", "seis_vy", "seis_vz", "snaps"])
+Elastic3DExtended = namedtuple("Elastic3DExtended", ["seis_vx", "seis_vy", "seis_vz", "snaps", "stats"])
+
+
+def elastic3d(vp, vs, rho, dh, dt, nt, f0, *, sx, sy, sz, rx, ry, rz, nbc=50, fd_order=8, snap_interval=0,
+              snap_plane="xz", snap_index=0, v_ref=None, storage="f32", wavelet=None, use_cuda_graph=True,
+              extended=False):
+    """elastic3d (src/equations/elastic3d/elastic3d.jl:31-117) -> (seis_vx, seis_vy, seis_vz, snaps).
+
+    Explosive source (the wavelet is added to txx, tyy and tzz after the stress HABC), snapshots of
+    vz.  Extensions as acoustic3d: `storage`, `wavelet`, `extended=True` (adds stats)."""
+    if vs is None:
+        raise ArgumentError("elastic3d needs vs")
+    outs, snap_list, t = _run3d(_lib.ELASTIC3D, vp, vs, rho, dh, dt, nt, f0, sx, sy, sz, rx, ry, rz, nbc,
+                                fd_order, snap_interval, snap_plane, snap_index, v_ref, storage, wavelet, False,
+                                use_cuda_graph)
+    if extended:
+        return Elastic3DExtended(outs[0], outs[1], outs[2], snap_list, Stats(t))
+    return Elastic3DResult(outs[0], outs[1], outs[2], snap_list)
 
 
 class Plan:

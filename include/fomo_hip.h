@@ -73,7 +73,7 @@ typedef struct fomo_desc {
   float v_ref;           /* HABC reference velocity */
   int32_t n_src, n_rec;
   int32_t source_type;   /* FOMO_SRC_* (elastic only) */
-  int32_t storage;       /* FOMO_STORE_* (acoustic3d only in this build; 2D is f32) */
+  int32_t storage;       /* FOMO_STORE_* (acoustic3d and elastic3d; 2D is f32) */
   int32_t snap_interval; /* 0 = no snapshots */
   int32_t snap_plane;    /* 3D snapshots: 0 = xz, 1 = xy, 2 = yz */
   int32_t snap_index;    /* 3D snapshots: 1-based interior index of the slice */
@@ -130,6 +130,17 @@ int fomo_acoustic3d(const fomo_desc *d, const float *vp, const float *rho, const
                     const int32_t *sz, const float *wavelet, const int32_t *rx, const int32_t *ry, const int32_t *rz,
                     float *seis_vx, float *seis_vy, float *seis_vz, float *seis_p, float *snaps,
                     double *kernel_time_s);
+
+/*
+ * elastic3d -- replaces Fomo.elastic3d (src/equations/elastic3d/elastic3d.jl:31-117).
+ * vp, vs, rho: nx*ny*nz.  Explosive source: the wavelet row of each source is added to txx, tyy
+ * and tzz after the stress HABC (elastic3d.jl:143-146).  Outputs n_rec x nt: seis_vx, seis_vy,
+ * seis_vz.  snaps: slices of vz as in fomo_acoustic3d.  d->storage as in fomo_acoustic3d.
+ */
+int fomo_elastic3d(const fomo_desc *d, const float *vp, const float *vs, const float *rho, const int32_t *sx,
+                   const int32_t *sy, const int32_t *sz, const float *wavelet, const int32_t *rx, const int32_t *ry,
+                   const int32_t *rz, float *seis_vx, float *seis_vy, float *seis_vz, float *snaps,
+                   double *kernel_time_s);
 
 /*
  * acoustic2d_batch -- replaces Fomo.acoustic2d_batch (src/equations/acoustic2d/batch.jl:180-261):

@@ -247,3 +247,184 @@ int orc_acoustic3d(int nxi, int nyi, int nzi, const float *vp, const float *rho,
   free(fo); free(fpre); free(vxo); free(vyo); free(vzo);
   return 0;
 }
+
+/* ------------------------------------------------------------------------------------------------
+ * elastic3d
+ *   medium ................... src/equations/elastic3d/medium.jl:29-123
+ *   velocity ................. src/equations/elastic3d/update_velocity.jl:17-84
+ *   stress ................... src/equations/elastic3d/update_stress.jl:21-94
+ *   HABC fields and weights .. src/boundary/habc/kernels_3d.jl:204-229 (vx/vy/vz own tables,
+ *                              all six stresses the tau table)
+ *   step order ............... src/equations/elastic3d/elastic3d.jl:122-162 (explosive source
+ *                              added to txx, tyy, tzz after the stress HABC; vx, vy, vz recorded)
+ */
+
+static inline real hmean4(real m1, real m2, real m3, real m4) {
+  if (m1 == (real)0 || m2 == (real)0 || m3 == (real)0 || m4 == (real)0) return (real)0;
+  return (real)4 / ((((real)1 / m1 + (real)1 / m2) + (real)1 / m3) + (real)1 / m4);
+}
+
+/* lam, lam2mu, mu_txy, mu_txz, mu_tyz on the padded cube (medium.jl:76-123). */
+void orc_elastic_params3d(const real *vp, const real *vs, const real *rho, int nx, int ny, int nz, real *lam,
+                          real *lam2mu, real *mu_txy, real *mu_txz, real *mu_tyz) {
+  size_t n = (size_t)nx * ny * nz;
+  real *mu = malloc(n * sizeof(real));
+  for (size_t c = 0; c < n; ++c) {
+    mu[c] = rho[c] * (vs[c] * vs[c]);
+    lam[c] = rho[c] * (vp[c] * vp[c]) - (real)2 * mu[c];
+    lam2mu[c] = lam[c] + (real)2 * mu[c];
+    mu_txy[c] = mu_txz[c] = mu_tyz[c] = (real)0;
+  }
+  for (int k = 1; k <= nz; ++k)
+    for (int j = 2; j <= ny; ++j)
+      for (int i = 2; i <= nx; ++i)
+        mu_txy[ID3(i, j, k)] = hmean4(mu[ID3(i - 1, j - 1, k)], mu[ID3(i, j - 1, k)], mu[ID3(i - 1, j, k)],
+                                      mu[ID3(i, j, k)]);
+  for (int k = 1; k <= nz; ++k)
+    for (int j = 1; j <= ny; ++j) mu_txy[ID3(1, j, k)] = mu_txy[ID3(2, j, k)];
+  for (int k = 1; k <= nz; ++k)
+    for (int i = 1; i <= nx; ++i) mu_txy[ID3(i, 1, k)] = mu_txy[ID3(i, 2, k)];
+  for (int k = 1; k <= nz - 1; ++k)
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 2; i <= nx; ++i)
+        mu_txz[ID3(i, j, k)] = hmean4(mu[ID3(i - 1, j, k)], mu[ID3(i, j, k)], mu[ID3(i - 1, j, k + 1)],
+                                      mu[ID3(i, j, k + 1)]);
+  for (int k = 1; k <= nz; ++k)
+    for (int j = 1; j <= ny; ++j) mu_txz[ID3(1, j, k)] = mu_txz[ID3(2, j, k)];
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) mu_txz[ID3(i, j, nz)] = mu_txz[ID3(i, j, nz - 1)];
+  for (int k = 1; k <= nz - 1; ++k)
+    for (int j = 2; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i)
+        mu_tyz[ID3(i, j, k)] = hmean4(mu[ID3(i, j - 1, k)], mu[ID3(i, j, k)], mu[ID3(i, j - 1, k + 1)],
+                                      mu[ID3(i, j, k + 1)]);
+  for (int k = 1; k <= nz; ++k)
+    for (int i = 1; i <= nx; ++i) mu_tyz[ID3(i, 1, k)] = mu_tyz[ID3(i, 2, k)];
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) mu_tyz[ID3(i, j, nz)] = mu_tyz[ID3(i, j, nz - 1)];
+  free(mu);
+}
+
+/*
+ * elastic3d (boundary HABC) -- inputs vp, vs, rho: nxi*nyi*nzi; outputs seis_{vx,vy,vz} n_rec x nt
+ * (any may be NULL); final_state (optional): vx, vy, vz, txx, tyy, tzz, txy, txz, tyz.
+ */
+int orc_elastic3d(int nxi, int nyi, int nzi, const float *vp, const float *vs, const float *rho, float dh, float dt,
+                  int nt, int nbc, int fd_order, float v_ref, int n_src, const int *sx, const int *sy, const int *sz,
+                  const float *wavelet, int n_rec, const int *rx, const int *ry, const int *rz, real *seis_vx,
+                  real *seis_vy, real *seis_vz, real *final_state, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+  float af[5];
+  int N = orc_fd_coeffs(fd_order, af);
+  if (N < 0) return -1;
+  real a[5];
+  for (int l = 0; l < N; ++l) a[l] = (real)af[l];
+  int M = N, pad = nbc + M, H = pad - 1;
+  int nx = nxi + 2 * pad, ny = nyi + 2 * pad, nz = nzi + 2 * pad;
+  size_t n = (size_t)nx * ny * nz;
+  const size_t sy_ = (size_t)nx, sz_ = (size_t)nx * ny;
+  real *arr[32];
+  for (int q = 0; q < 29; ++q) {
+    arr[q] = calloc(n, sizeof(real));
+    if (!arr[q]) return -2;
+  }
+  real *vpp = arr[0], *vsp = arr[1], *rhop = arr[2];
+  real *bx = arr[3], *by = arr[4], *bz = arr[5], *lam = arr[6], *l2m = arr[7], *mxy = arr[8], *mxz = arr[9],
+       *myz = arr[10];
+  real *F[9], *FO[9];
+  for (int f = 0; f < 9; ++f) {
+    F[f] = arr[11 + f];
+    FO[f] = arr[20 + f];
+  }
+  real *vx = F[0], *vy = F[1], *vz = F[2], *txx = F[3], *tyy = F[4], *tzz = F[5], *txy = F[6], *txz = F[7],
+       *tyz = F[8];
+  orc_pad3d(vp, nxi, nyi, nzi, pad, vpp);
+  orc_pad3d(vs, nxi, nyi, nzi, pad, vsp);
+  orc_pad3d(rho, nxi, nyi, nzi, pad, rhop);
+  orc_buoyancy3d(rhop, nx, ny, nz, bx, by, bz);
+  orc_elastic_params3d(vpp, vsp, rhop, nx, ny, nz, lam, l2m, mxy, mxz, myz);
+  real *fpre = vpp; /* reused as the pre-correction copy */
+  real hc[3];
+  orc_habc_coeffs(v_ref, dt, dh, hc);
+  real q = hc[0], qt = hc[1], qxt = hc[2];
+  real dtdh = (real)dt / (real)dh;
+  const double voff[3] = {0.0, 0.25, 0.5};
+  for (int it = 1; it <= nt; ++it) {
+    for (int f = 0; f < 3; ++f) backup3(FO[f], F[f], nx, ny, nz, H);
+#pragma omp parallel for schedule(static)
+    for (int k = M + 1; k <= nz - M; ++k)
+      for (int j = M + 1; j <= ny - M; ++j)
+        for (int i = M + 1; i <= nx - M; ++i) {
+          size_t c = ID3(i, j, k);
+          real dtxxdx = 0, dtxydy = 0, dtxzdz = 0, dtxydx = 0, dtyydy = 0, dtyzdz = 0, dtxzdx = 0, dtyzdy = 0,
+               dtzzdz = 0;
+          for (int l = 1; l <= N; ++l) {
+            real cl = a[l - 1];
+            dtxxdx += cl * (txx[c + (l - 1)] - txx[c - l]);
+            dtyydy += cl * (tyy[c + (l - 1) * sy_] - tyy[c - l * sy_]);
+            dtzzdz += cl * (tzz[c + l * sz_] - tzz[c - (l - 1) * sz_]);
+            dtxydy += cl * (txy[c + l * sy_] - txy[c - (l - 1) * sy_]);
+            dtxydx += cl * (txy[c + l] - txy[c - (l - 1)]);
+            dtxzdz += cl * (txz[c + (l - 1) * sz_] - txz[c - l * sz_]);
+            dtxzdx += cl * (txz[c + l] - txz[c - (l - 1)]);
+            dtyzdz += cl * (tyz[c + (l - 1) * sz_] - tyz[c - l * sz_]);
+            dtyzdy += cl * (tyz[c + l * sy_] - tyz[c - (l - 1) * sy_]);
+          }
+          vx[c] += (bx[c] * dtdh) * ((dtxxdx + dtxydy) + dtxzdz);
+          vy[c] += (by[c] * dtdh) * ((dtxydx + dtyydy) + dtyzdz);
+          vz[c] += (bz[c] * dtdh) * ((dtxzdx + dtyzdy) + dtzzdz);
+        }
+    for (int f = 0; f < 3; ++f) {
+      memcpy(fpre, F[f], n * sizeof(real));
+      habc3d(F[f], fpre, FO[f], nx, ny, nz, H, pad, voff[f], q, qt, qxt);
+    }
+
+    for (int f = 3; f < 9; ++f) backup3(FO[f], F[f], nx, ny, nz, H);
+#pragma omp parallel for schedule(static)
+    for (int k = M + 1; k <= nz - M; ++k)
+      for (int j = M + 1; j <= ny - M; ++j)
+        for (int i = M + 1; i <= nx - M; ++i) {
+          size_t c = ID3(i, j, k);
+          real dvxdx = 0, dvydy = 0, dvzdz = 0, dvxdy = 0, dvydx = 0, dvxdz = 0, dvzdx = 0, dvydz = 0, dvzdy = 0;
+          for (int l = 1; l <= N; ++l) {
+            real cl = a[l - 1];
+            dvxdx += cl * (vx[c + l] - vx[c - (l - 1)]);
+            dvydy += cl * (vy[c + l * sy_] - vy[c - (l - 1) * sy_]);
+            dvzdz += cl * (vz[c + (l - 1) * sz_] - vz[c - l * sz_]);
+            dvxdy += cl * (vx[c + (l - 1) * sy_] - vx[c - l * sy_]);
+            dvydx += cl * (vy[c + (l - 1)] - vy[c - l]);
+            dvxdz += cl * (vx[c + l * sz_] - vx[c - (l - 1) * sz_]);
+            dvzdx += cl * (vz[c + (l - 1)] - vz[c - l]);
+            dvydz += cl * (vy[c + l * sz_] - vy[c - (l - 1) * sz_]);
+            dvzdy += cl * (vz[c + (l - 1) * sy_] - vz[c - l * sy_]);
+          }
+          real lv = lam[c], l2 = l2m[c];
+          txx[c] += ((l2 * dvxdx + lv * dvydy) + lv * dvzdz) * dtdh;
+          tyy[c] += ((lv * dvxdx + l2 * dvydy) + lv * dvzdz) * dtdh;
+          tzz[c] += ((lv * dvxdx + lv * dvydy) + l2 * dvzdz) * dtdh;
+          txy[c] += (mxy[c] * (dvxdy + dvydx)) * dtdh;
+          txz[c] += (mxz[c] * (dvxdz + dvzdx)) * dtdh;
+          tyz[c] += (myz[c] * (dvydz + dvzdy)) * dtdh;
+        }
+    for (int f = 3; f < 9; ++f) {
+      memcpy(fpre, F[f], n * sizeof(real));
+      habc3d(F[f], fpre, FO[f], nx, ny, nz, H, pad, 0.75, q, qt, qxt);
+    }
+    for (int f = 3; f < 6; ++f)
+      for (int s = 0; s < n_src; ++s)
+        F[f][ID3(sx[s] + pad, sy[s] + pad, sz[s] + pad)] += (real)wavelet[(size_t)s + (size_t)n_src * (it - 1)];
+    for (int r = 0; r < n_rec; ++r) {
+      size_t c = ID3(rx[r] + pad, ry[r] + pad, rz[r] + pad);
+      size_t o = (size_t)r + (size_t)n_rec * (it - 1);
+      if (seis_vx) seis_vx[o] = vx[c];
+      if (seis_vy) seis_vy[o] = vy[c];
+      if (seis_vz) seis_vz[o] = vz[c];
+    }
+  }
+  if (final_state)
+    for (int f = 0; f < 9; ++f) memcpy(final_state + (size_t)f * n, F[f], n * sizeof(real));
+  for (int q2 = 0; q2 < 29; ++q2) free(arr[q2]);
+  return 0;
+}

@@ -48,6 +48,11 @@ def lib(precision: str = "f32") -> C.CDLL:
         L.orc_acoustic3d.restype = _i
         L.orc_acoustic3d.argtypes = [_i, _i, _i, _p, _p, _f, _f, _i, _i, _i, _f, _i, _p, _p, _p, _p, _i, _p,
                                      _p, _p, _p, _p, _p, _p, _p, _i]
+        L.orc_elastic3d.restype = _i
+        L.orc_elastic3d.argtypes = [_i, _i, _i, _p, _p, _p, _f, _f, _i, _i, _i, _f, _i, _p, _p, _p, _p, _i, _p,
+                                    _p, _p, _p, _p, _p, _p, _i]
+        L.orc_elastic_params3d.restype = None
+        L.orc_elastic_params3d.argtypes = [_p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _p]
         L.orc_fd_coeffs.restype = _i
         L.orc_fd_coeffs.argtypes = [_i, _p]
         L.orc_habc_coeffs.restype = None
@@ -319,3 +324,42 @@ def acoustic3d(vp, rho, dh, dt, nt, *, sx, sy, sz, rx, ry, rz, wavelet, nbc=50, 
     if final_state:
         out["state"] = {k: fs[m * n:(m + 1) * n].reshape(shp, order="F") for m, k in enumerate(("p", "vx", "vy", "vz"))}
     return out
+
+
+def elastic3d(vp, vs, rho, dh, dt, nt, *, sx, sy, sz, rx, ry, rz, wavelet, nbc=50, fd_order=8, v_ref=None,
+              precision="f32", nthreads=0, final_state=False):
+    """elastic3d with the two-pass HABC (fomo_oracle3d.c:orc_elastic3d)."""
+    vp, vs, rho = _f32F(vp), _f32F(vs), _f32F(rho)
+    nx, ny, nz = vp.shape
+    if v_ref is None:
+        v_ref = float(vp[vp > 0].min())
+    sx, sy, sz, rx, ry, rz = (_i32(a) for a in (sx, sy, sz, rx, ry, rz))
+    w = np.asfortranarray(np.asarray(wavelet, np.float32).reshape(len(sx), nt))
+    nrec = len(rx)
+    dt_ = _dtype(precision)
+    svx, svy, svz = (np.zeros((nrec, nt), dt_, order="F") for _ in range(3))
+    pad = nbc + fd_order // 2
+    shp = (nx + 2 * pad, ny + 2 * pad, nz + 2 * pad)
+    n = shp[0] * shp[1] * shp[2]
+    fs = np.zeros(9 * n, dt_) if final_state else None
+    rc = lib(precision).orc_elastic3d(nx, ny, nz, vp.ctypes.data, vs.ctypes.data, rho.ctypes.data, dh, dt, nt, nbc,
+                                      fd_order, v_ref, len(sx), sx.ctypes.data, sy.ctypes.data, sz.ctypes.data,
+                                      w.ctypes.data, nrec, rx.ctypes.data, ry.ctypes.data, rz.ctypes.data,
+                                      svx.ctypes.data, svy.ctypes.data, svz.ctypes.data, _ptr(fs), nthreads)
+    if rc != 0:
+        raise ValueError(f"oracle elastic3d failed ({rc})")
+    out = dict(seis_vx=svx, seis_vy=svy, seis_vz=svz)
+    if final_state:
+        names = ("vx", "vy", "vz", "txx", "tyy", "tzz", "txy", "txz", "tyz")
+        out["state"] = {k: fs[m * n:(m + 1) * n].reshape(shp, order="F") for m, k in enumerate(names)}
+    return out
+
+
+def elastic_params3d(vp, vs, rho, precision="f32"):
+    """lam, lam2mu, mu_txy, mu_txz, mu_tyz of padded (nx, ny, nz) arrays."""
+    dt_ = _dtype(precision)
+    vp, vs, rho = (np.asfortranarray(np.asarray(a, dt_)) for a in (vp, vs, rho))
+    outs = [np.zeros(vp.shape, dt_, order="F") for _ in range(5)]
+    lib(precision).orc_elastic_params3d(vp.ctypes.data, vs.ctypes.data, rho.ctypes.data, *vp.shape,
+                                        *(o.ctypes.data for o in outs))
+    return outs

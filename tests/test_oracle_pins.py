@@ -341,3 +341,81 @@ def test_oracle3d_reduces_to_2d_for_y_invariant_field(orc):
     assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-12
     assert np.linalg.norm(r3["seis_vx"] - r2["seis_vx"]) / np.linalg.norm(r2["seis_vx"]) < 1e-12
     assert np.abs(r3["seis_vy"]).max() == 0.0
+
+
+def test_elastic_params3d(orc):
+    """elastic3d/medium.jl:76-123: homogeneous -> every harmonic mean equals mu and lam2mu = rho vp^2;
+    one zero-mu cell zeroes exactly the four-point means that include it (and no edge copy moves
+    them)."""
+    shp = (6, 5, 4)
+    vp, vs, rho = np.full(shp, 3000.0), np.full(shp, 1700.0), np.full(shp, 2100.0)
+    vs[2, 2, 1] = 0.0
+    lam, l2m, mxy, mxz, myz = orc.elastic_params3d(vp, vs, rho, "f64")
+    mu = 2100.0 * 1700.0 ** 2
+    assert np.allclose(l2m, 2100.0 * 3000.0 ** 2, rtol=1e-15)
+    assert lam[2, 2, 1] == 2100.0 * 3000.0 ** 2 and lam[0, 0, 0] == pytest.approx(2100.0 * 3000.0 ** 2 - 2 * mu)
+    want_xy, want_xz, want_yz = (np.full(shp, mu) for _ in range(3))
+    for i in (2, 3):
+        for j in (2, 3):
+            want_xy[i, j, 1] = 0.0
+    for i in (2, 3):
+        for k in (0, 1):
+            want_xz[i, 2, k] = 0.0
+    for j in (2, 3):
+        for k in (0, 1):
+            want_yz[2, j, k] = 0.0
+    for got, want in ((mxy, want_xy), (mxz, want_xz), (myz, want_yz)):
+        assert np.allclose(got, want, rtol=1e-14, atol=0.0)
+
+
+def test_oracle_elastic3d_reduces_to_elastic2d(orc):
+    """A y-invariant model and a line of identical explosive sources on every y cell: within the
+    numerical domain of dependence of the y faces, vx and vz of the 3D oracle equal the
+    independently pinned 2D elastic oracle, and vy stays exactly zero (every y-difference of a
+    y-invariant field vanishes)."""
+    nx, nz, ny, nt = 40, 36, 60, 6
+    vp2 = np.full((nx, nz), 2000.0, np.float32)
+    vp2[:, 12:] = 2600.0
+    vs2 = (vp2 / 1.7).astype(np.float32)
+    rho2 = np.full((nx, nz), 2000.0, np.float32)
+    rho2[:, 14:] = 2300.0
+    rep = lambda a: np.repeat(a[:, None, :], ny, axis=1)  # noqa: E731
+    w = np.linspace(1.0, 2.0, nt).astype(np.float32)
+    rx = np.arange(8, 33)
+    rz = np.full(len(rx), 11)
+    r2 = orc.elastic2d(vp2, vs2, rho2, 10.0, 1e-3, nt, sx=[20], sz=[10], rx=rx, rz=rz, wavelet=w[None], nbc=2,
+                       fd_order=4, precision="f64")
+    ys = np.arange(1, ny + 1)
+    r3 = orc.elastic3d(rep(vp2), rep(vs2), rep(rho2), 10.0, 1e-3, nt, sx=np.full(ny, 20), sy=ys, sz=np.full(ny, 10),
+                       rx=rx, ry=np.full(len(rx), ny // 2), rz=rz, wavelet=np.repeat(w[None], ny, axis=0), nbc=2,
+                       fd_order=4, precision="f64")
+    for k in ("seis_vx", "seis_vz"):
+        assert np.abs(r2[k]).max() > 0
+        assert np.linalg.norm(r3[k] - r2[k]) / np.linalg.norm(r2[k]) < 1e-12
+    assert np.abs(r3["seis_vy"]).max() == 0.0
+
+
+def test_oracle_elastic3d_symmetries_and_precision(orc):
+    """Model symmetric in x and y, explosive source on the axis: vz mirror-symmetric and vx
+    antisymmetric along x; and the x<->y exchange maps vx on an x line to vy on the y line.  The
+    exchange is exact except for the Higdon weight offsets (vx 0, vy 0.25), which leave ~3e-8; a
+    one-cell shift fails at O(1).  FP32 tracks FP64."""
+    n, nz, nt = 31, 30, 80
+    vp = np.full((n, n, nz), 2500.0, np.float32)
+    vp[:, :, 15:] = 3100.0
+    vs = (vp / 1.8).astype(np.float32)
+    rho = np.full((n, n, nz), 2100.0, np.float32)
+    w = orc.ricker(15.0, 1e-3, nt)[None]
+    line = np.arange(1, n + 1)
+    kw = dict(sx=[16], sy=[16], sz=[8], wavelet=w, nbc=6, fd_order=8)
+    a = orc.elastic3d(vp, vs, rho, 10.0, 1e-3, nt, precision="f64", rx=line, ry=np.full(n, 16), rz=np.full(n, 5), **kw)
+    b = orc.elastic3d(vp, vs, rho, 10.0, 1e-3, nt, precision="f64", rx=np.full(n, 16), ry=line, rz=np.full(n, 5), **kw)
+    assert rel_sym(a["seis_vz"]) < 1e-6 and rel_asym_vx(a["seis_vx"]) < 1e-6
+    assert rel_asym_vx(b["seis_vy"]) < 1e-6
+    ref = np.linalg.norm(a["seis_vx"])
+    assert np.linalg.norm(a["seis_vx"] - b["seis_vy"]) / ref < 1e-6
+    assert np.linalg.norm(a["seis_vz"] - b["seis_vz"]) / np.linalg.norm(a["seis_vz"]) < 1e-6
+    assert np.linalg.norm(a["seis_vx"][1:] - b["seis_vy"][:-1]) / ref > 1e-2
+    a32 = orc.elastic3d(vp, vs, rho, 10.0, 1e-3, nt, precision="f32", rx=line, ry=np.full(n, 16), rz=np.full(n, 5),
+                        **kw)
+    assert np.linalg.norm(a32["seis_vz"] - a["seis_vz"]) / np.linalg.norm(a["seis_vz"]) < 1e-5
