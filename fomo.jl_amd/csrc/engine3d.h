@@ -31,10 +31,11 @@ struct Shot3D {
 };
 
 // A column tile marched down z.  Output cells: x in [i0, i0+w), y in [j0, j0+hy), z in
-// [k0, k0+hz).  ring = 1: the tile also computes pre-correction values on a one-cell ring in x
-// and y (w <= 62, hy <= rows-2), which the Higdon blend of x/y frame cells reads; ring = 0: no
-// cell of the footprint is an x/y frame cell (w <= 64, hy <= rows).  Core items (the user's
-// interior in x, y and z) go to the core list, all others to the frame list.  kind < 0: padding.
+// [k0, k0+hz).  ring bit 0 (x) / bit 1 (y): the tile also computes pre-correction values one
+// cell beyond its outputs in x (w <= 62) / y (hy <= rows-2), which the Higdon blend of x / y frame
+// cells reads; without the bit no output cell is an x / y frame cell (w <= 64, hy <= rows).  Core
+// items (the user's interior in x, y and z) go to the core list, all others to the frame list.
+// kind < 0: padding.
 struct Work3D {
   int kind, i0, j0, k0, w, hy, hz, ring;
 };

@@ -167,13 +167,13 @@ struct P3<kE3Str> {
 static_assert(4 * P3<kA3Vel>::YPT == kRowsAc3 && 4 * P3<kA3Prs>::YPT == kRowsAc3, "acoustic tile rows");
 static_assert(4 * P3<kE3Vel>::YPT == kRowsEl3 && 4 * P3<kE3Str>::YPT == kRowsEl3, "elastic tile rows");
 
-// LDS of a tile: the staged planes (with halos) and a ring of three planes of pre-correction
-// values of the outputs (planes t-2, t-1, t).
+// LDS of a tile: the staged planes (with halos) and, for the frame march, two-plane rings of the
+// pre-correction values and of the old values of the outputs (planes t-1 and t).
 template <int PH, int N>
 struct Lds3 {
   static constexpr int ROWS = 4 * P3<PH>::YPT, PW = kTileW + 2 * N, PR = ROWS + 2 * N;
   static constexpr int STG = P3<PH>::NS * PR * PW, SLOT = P3<PH>::NF * ROWS * kTileW;
-  static constexpr int SIZE = STG + 3 * SLOT;
+  static constexpr int SIZE = STG + 4 * SLOT;
 };
 
 __device__ __forceinline__ void inject3d(const Shot3D& S, size_t c, int it, int nt, float& v) {
@@ -192,14 +192,15 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
   constexpr int YPT = P::YPT, ROWS = L::ROWS, NF = P::NF, NS = P::NS, NQ = P::NQ, NM = P::NM;
   constexpr int PW = L::PW, PR = L::PR, HYR = (2 * N + 3) / 4;
   float* stg = lds;
-  float* ring = lds + L::STG;  // slot t % 3 holds the pre-correction values of plane t
+  float* ring = lds + L::STG;  // pre-correction values of plane t in slot t % 2, old values in slot 2 + t % 2
   const Shot3D& S = A.shot;
   const int nx = S.nx, ny = S.ny, nz = S.nz, H = S.H;
   const size_t sz = (size_t)nx * ny;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int s = A.parity, o = 1 - s;
   const int sb = P::kNew ? o : s;
-  const int pi0 = W.i0 - W.ring, pj0 = W.j0 - W.ring;
+  const int rx = W.ring & 1, ry = (W.ring >> 1) & 1;
+  const int pi0 = W.i0 - rx, pj0 = W.j0 - ry;
   const int i = pi0 + tx;
   const float* __restrict__ a = A.a;
   const float dtdh = S.dtdh;
@@ -270,7 +271,7 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
     }
   };
 
-  float preN[YPT][NF];
+  float preN[YPT][NF], oldT[YPT][NF], preP[YPT][NF], oldP[YPT][NF];
 
   prefetch_stage(t0);
   prefetch_cells(t0);
@@ -420,13 +421,19 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
       }
       __syncthreads();
     } else {
-    // publish plane t's pre-correction values; then load the old values and materials of t+1
+    // publish plane t's pre-correction and old values; then load the old values and materials of t+1
     {
-      float* slot = ring + (t % 3) * L::SLOT;
+      float* sp_ = ring + (t & 1) * L::SLOT;
+      float* so_ = ring + (2 + (t & 1)) * L::SLOT;
 #pragma unroll
       for (int r = 0; r < YPT; ++r)
 #pragma unroll
-        for (int f = 0; f < NF; ++f) slot[(f * ROWS + ty + 4 * r) * kTileW + tx] = preN[r][f];
+        for (int f = 0; f < NF; ++f) {
+          const int e = (f * ROWS + ty + 4 * r) * kTileW + tx;
+          sp_[e] = preN[r][f];
+          so_[e] = oldF[r][f];
+          oldT[r][f] = oldF[r][f];
+        }
     }
     if (t < t_end) prefetch_cells(t + 1);
     __syncthreads();
@@ -436,13 +443,13 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
       const int kk = t - 1;
       const bool kt = kk <= H, kb = kk >= nz - 1 - H;
       const bool il = i <= H, ir = i >= nx - 1 - H;
-      const bool col_out = tx >= W.ring && tx < W.ring + W.w;
-      const float* cur = ring + ((t + 2) % 3) * L::SLOT;   // plane t-1
-      const float* prv = ring + ((t + 1) % 3) * L::SLOT;   // plane t-2
+      const bool col_out = tx >= rx && tx < rx + W.w;
+      const float* curp = ring + ((t - 1) & 1) * L::SLOT;        // pre-correction values of plane t-1
+      const float* curo = ring + (2 + ((t - 1) & 1)) * L::SLOT;  // old values of plane t-1
 #pragma unroll
       for (int r = 0; r < YPT; ++r) {
         const int jl = ty + 4 * r, j = pj0 + jl;
-        if (!col_out || jl < W.ring || jl >= W.ring + W.hy) continue;
+        if (!col_out || jl < ry || jl >= ry + W.hy) continue;
         const unsigned co = (unsigned)(i + j * nx);
         const bool jf = j <= H, jb = j >= ny - 1 - H;
         const int dist = min(min(min(i, nx - 1 - i), min(j, ny - 1 - j)), min(kk, nz - 1 - kk));
@@ -451,24 +458,25 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
         for (int f = 0; f < NF; ++f) {
           const int F = P::out_f(f);
           const int e = (f * ROWS + jl) * kTileW + tx;
-          float v = cur[e];
+          float v = curp[e];
           if (frame) {
-            const float foc = ld(s, F, i, j, kk);
+            const float foc = curo[e];
             float sum = 0.0f;
             int nd = 0;
             if (il || ir) {
               const int d = il ? 1 : -1;
-              sum += (-S.q * cur[e + d] - S.qt * foc) - S.qxt * ld(s, F, i + d, j, kk);
+              sum += (-S.q * curp[e + d] - S.qt * foc) - S.qxt * curo[e + d];
               ++nd;
             }
             if (jf || jb) {
-              const int d = jf ? 1 : -1;
-              sum += (-S.q * cur[e + d * kTileW] - S.qt * foc) - S.qxt * ld(s, F, i, j + d, kk);
+              const int d = jf ? kTileW : -kTileW;
+              sum += (-S.q * curp[e + d] - S.qt * foc) - S.qxt * curo[e + d];
               ++nd;
             }
             if (kt || kb) {
-              const float pz = kt ? preN[r][f] : prv[e];
-              sum += (-S.q * pz - S.qt * foc) - S.qxt * ld(s, F, i, j, kt ? kk + 1 : kk - 1);
+              const float pz = kt ? preN[r][f] : preP[r][f];
+              const float oz = kt ? oldT[r][f] : oldP[r][f];
+              sum += (-S.q * pz - S.qt * foc) - S.qxt * oz;
               ++nd;
             }
             const float wt = G(S.wlut)[P::lut(f) * (H + 1) + dist];
@@ -478,6 +486,19 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
           Cvt<T>::st(plane_of<T>(S.fld[o][F], kk, sz), co, v, S.fenc[F]);
         }
       }
+    }
+    // own values of plane t-1 become the z-neighbours below the next output plane
+    {
+      const float* curp = ring + ((t - 1) & 1) * L::SLOT;
+      const float* curo = ring + (2 + ((t - 1) & 1)) * L::SLOT;
+#pragma unroll
+      for (int r = 0; r < YPT; ++r)
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          const int e = (f * ROWS + ty + 4 * r) * kTileW + tx;
+          preP[r][f] = curp[e];
+          oldP[r][f] = curo[e];
+        }
     }
     }  // LAG
 

@@ -189,18 +189,20 @@ static int upload_work(Plan3D* p, int which, std::vector<Work3D>& items, int row
 static int make_work3d(Plan3D* p, int kseg, int rows) {
   const int nx = p->shot.nx, ny = p->shot.ny, nz = p->shot.nz, pad = p->pad;
   std::vector<Work3D> core_fp, ring_fp;
-  auto ring_rect = [&](int x0, int x1, int y0, int y1) {
-    for (int j = y0; j < y1; j += rows - 2)
-      for (int i = x0; i < x1; i += kTileW - 2)
-        ring_fp.push_back(Work3D{1, i, j, 0, std::min(kTileW - 2, x1 - i), std::min(rows - 2, y1 - j), 0, 1});
+  // rx / ry: the rectangle holds x / y frame cells (then the tile computes a ring in that direction)
+  auto ring_rect = [&](int x0, int x1, int y0, int y1, int rx, int ry) {
+    const int tw = kTileW - 2 * rx, th = rows - 2 * ry;
+    for (int j = y0; j < y1; j += th)
+      for (int i = x0; i < x1; i += tw)
+        ring_fp.push_back(Work3D{1, i, j, 0, std::min(tw, x1 - i), std::min(th, y1 - j), 0, rx | (ry << 1)});
   };
   for (int j = pad; j < ny - pad; j += rows)
     for (int i = pad; i < nx - pad; i += kTileW)
       core_fp.push_back(Work3D{1, i, j, 0, std::min(kTileW, nx - pad - i), std::min(rows, ny - pad - j), 0, 0});
-  ring_rect(1, nx - 1, 1, pad);
-  ring_rect(1, pad, pad, ny - pad);
-  ring_rect(nx - pad, nx - 1, pad, ny - pad);
-  ring_rect(1, nx - 1, ny - pad, ny - 1);
+  ring_rect(1, nx - 1, 1, pad, 1, 1);
+  ring_rect(1, pad, pad, ny - pad, 1, 0);
+  ring_rect(nx - pad, nx - 1, pad, ny - pad, 1, 0);
+  ring_rect(1, nx - 1, ny - pad, ny - 1, 1, 1);
   // z-segments of [z0, z1) of about kseg planes each
   auto segs = [&](int z0, int z1, const std::vector<Work3D>& fp, std::vector<Work3D>& out) {
     const int nk = z1 - z0;
