@@ -99,6 +99,16 @@ def probe(args):
     print(json.dumps({"probe": f"{n}x{n} interior, fd8, nbc 50", **res}), flush=True)
 
 
+def stream_copy_gbs(nbytes=2 << 30, iters=20):
+    """Practical HBM ceiling: float4 streaming copy of `nbytes` (read + write counted)."""
+    import ctypes as C
+
+    import fomo._lib as L
+    ms = C.c_double(0.0)
+    L.check(L.lib().fomo_stream_copy(int(nbytes), int(iters), C.byref(ms)))
+    return 2.0 * nbytes / (ms.value * 1e-3) / 1e9, ms.value
+
+
 def probe3d(args):
     """acoustic3d at the BASELINE 3D size (512^3 interior, nbc 32, fd8; 584^3 padded) for each
     storage precision: per-kernel times and algorithmic GB/s (64 B/cell FP32, 32 B/cell 16-bit)."""
@@ -133,6 +143,9 @@ def probe3d(args):
                      "step_gbs": 16 * eb * cells * nt / ms / 1e6, "vel_ms": tv, "vel_gbs": 10 * eb * cells / tv / 1e6,
                      "prs_ms": tp, "prs_gbs": 6 * eb * cells / tp / 1e6, "gather_ms": tr}
         log(f"{name}: {res[name]}")
+    gbs, cms = stream_copy_gbs()
+    res["stream_copy"] = {"bytes": 2 << 30, "ms": cms, "gbs": gbs}
+    log(f"stream copy: {gbs:.0f} GB/s")
     print(json.dumps({"probe3d": f"{n}^3 interior, fd8, nbc 32", **res}), flush=True)
 
 
@@ -149,6 +162,7 @@ def main():
     ap.add_argument("--probe3d", action="store_true", help="acoustic3d 512^3 probe (f32/f16/bf16)")
     ap.add_argument("--probe-n3", type=int, default=512)
     ap.add_argument("--probe-storage", default="f32,f16,bf16", help="storages of the 3D probe")
+    ap.add_argument("--stream-copy", action="store_true", help="only the HBM streaming-copy calibration")
     ap.add_argument("--probe-tile", type=int, default=0, help="3D z-segment choice (0 default, k: 16<<(k-1) planes)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -161,6 +175,10 @@ def main():
         return
     if args.probe3d:
         probe3d(args)
+        return
+    if args.stream_copy:
+        gbs, ms = stream_copy_gbs(1 << 30, 4)
+        print(json.dumps({"stream_copy": {"bytes": 1 << 30, "ms": ms, "gbs": gbs}}), flush=True)
         return
 
     import numpy as np

@@ -40,9 +40,10 @@ struct Work3D {
   int kind, i0, j0, k0, w, hy, hz, ring;
 };
 
-// Rows of a tile (4 thread rows x rows-per-thread): 8 for both equations (2 rows per thread keep
-// the per-thread state small enough for 4 waves per SIMD in the acoustic sweeps).
-constexpr int kRowsAc3 = 8, kRowsEl3 = 8;
+// Rows of a tile (thread rows x rows per thread): acoustic 8 x 2 (512-thread blocks; 2 rows per
+// thread keep the per-thread state small enough for 4 waves per SIMD, 16 rows halve the share of
+// y-halo reads), elastic 4 x 2.
+constexpr int kRowsAc3 = 16, kRowsEl3 = 8;
 
 // The shot descriptor travels by value in the kernel arguments (constant, so its pointers and
 // scales stay in scalar registers or are reloaded from the argument segment).

@@ -31,55 +31,66 @@
 namespace fomo {
 
 // Element c of a global array of E: with a 32-bit index the byte offset is formed in 32 bits,
-// so the access compiles to a scalar base plus a 32-bit vector offset.
-template <class E, class I>
+// so the access compiles to a scalar base plus a 32-bit vector offset.  NT: streaming hint
+// (measured slower for every array of the sweeps, which re-read their inputs through L2).
+template <class E, bool NT = false, class I>
 __device__ __forceinline__ E gget(const void* p, I c) {
+  gp<const E> a;
   if constexpr (sizeof(I) == 4)
-    return *(gp<const E>)(G((const char*)p) + (unsigned)(c * (unsigned)sizeof(E)));
+    a = (gp<const E>)(G((const char*)p) + (unsigned)(c * (unsigned)sizeof(E)));
   else
-    return G((const E*)p)[c];
+    a = G((const E*)p) + c;
+  if constexpr (NT)
+    return __builtin_nontemporal_load(a);
+  else
+    return *a;
 }
-template <class E, class I>
+template <class E, bool NT = false, class I>
 __device__ __forceinline__ void gput(void* p, I c, E v) {
+  gp<E> a;
   if constexpr (sizeof(I) == 4)
-    *(gp<E>)(G((char*)p) + (unsigned)(c * (unsigned)sizeof(E))) = v;
+    a = (gp<E>)(G((char*)p) + (unsigned)(c * (unsigned)sizeof(E)));
   else
-    G((E*)p)[c] = v;
+    a = G((E*)p) + c;
+  if constexpr (NT)
+    __builtin_nontemporal_store(v, a);
+  else
+    *a = v;
 }
 
 template <class T>
 struct Cvt;
 template <>
 struct Cvt<float> {
-  template <class I>
+  template <bool NT = false, class I>
   __device__ __forceinline__ static float ld(const void* p, I c, float) {
-    return gget<float>(p, c);
+    return gget<float, NT>(p, c);
   }
-  template <class I>
+  template <bool NT = false, class I>
   __device__ __forceinline__ static void st(void* p, I c, float v, float) {
-    gput<float>(p, c, v);
+    gput<float, NT>(p, c, v);
   }
 };
 template <>
 struct Cvt<__half> {
-  template <class I>
+  template <bool NT = false, class I>
   __device__ __forceinline__ static float ld(const void* p, I c, float dec) {
-    return __half2float(__ushort_as_half(gget<unsigned short>(p, c))) * dec;
+    return __half2float(__ushort_as_half(gget<unsigned short, NT>(p, c))) * dec;
   }
-  template <class I>
+  template <bool NT = false, class I>
   __device__ __forceinline__ static void st(void* p, I c, float v, float enc) {
-    gput<unsigned short>(p, c, __half_as_ushort(__float2half_rn(v * enc)));
+    gput<unsigned short, NT>(p, c, __half_as_ushort(__float2half_rn(v * enc)));
   }
 };
 template <>
 struct Cvt<__hip_bfloat16> {
-  template <class I>
+  template <bool NT = false, class I>
   __device__ __forceinline__ static float ld(const void* p, I c, float dec) {
-    return __bfloat162float(__ushort_as_bfloat16(gget<unsigned short>(p, c))) * dec;
+    return __bfloat162float(__ushort_as_bfloat16(gget<unsigned short, NT>(p, c))) * dec;
   }
-  template <class I>
+  template <bool NT = false, class I>
   __device__ __forceinline__ static void st(void* p, I c, float v, float enc) {
-    gput<unsigned short>(p, c, __bfloat16_as_ushort(__float2bfloat16(v * enc)));
+    gput<unsigned short, NT>(p, c, __bfloat16_as_ushort(__float2bfloat16(v * enc)));
   }
 };
 
@@ -96,13 +107,14 @@ __device__ __forceinline__ void* plane_of(void* p, int k, size_t plane_elems) {
 
 enum Phase3 { kA3Vel = 0, kA3Prs = 1, kE3Vel = 2, kE3Str = 3 };
 
-// Per phase: NF output fields, NS staged planes, NQ z-queues, YPT rows per thread, whether the
-// stencil inputs are this step's new velocities (buffer 1-s), and the field ids involved.
+// Per phase: NF output fields, NS staged planes, NQ z-queues, NM materials, TR thread rows of the
+// block (64 x TR threads), YPT rows per thread (a tile is 64 x TR*YPT cells), whether the stencil
+// inputs are this step's new velocities (buffer 1-s), and the field ids involved.
 template <int PH>
 struct P3;
 template <>
 struct P3<kA3Vel> {
-  static constexpr int NF = 3, NS = 1, NQ = 1, YPT = 2, NM = 3;
+  static constexpr int NF = 3, NS = 1, NQ = 1, YPT = 2, NM = 3, TR = 8;
   __device__ static constexpr int mat_id(int k) { return 1 + k; }  // bx by bz
   static constexpr bool kNew = false;
   __device__ static constexpr int out_f(int k) { return 1 + k; }                       // vx vy vz
@@ -117,7 +129,7 @@ struct P3<kA3Vel> {
 };
 template <>
 struct P3<kA3Prs> {
-  static constexpr int NF = 1, NS = 2, NQ = 1, YPT = 2, NM = 1;
+  static constexpr int NF = 1, NS = 2, NQ = 1, YPT = 2, NM = 1, TR = 8;
   __device__ static constexpr int mat_id(int) { return 0; }  // kappa
   static constexpr bool kNew = true;
   __device__ static constexpr int out_f(int) { return 0; }
@@ -132,7 +144,7 @@ struct P3<kA3Prs> {
 };
 template <>
 struct P3<kE3Vel> {
-  static constexpr int NF = 3, NS = 5, NQ = 3, YPT = 2, NM = 3;
+  static constexpr int NF = 3, NS = 5, NQ = 3, YPT = 2, NM = 3, TR = 4;
   __device__ static constexpr int mat_id(int k) { return k; }  // bx by bz
   static constexpr bool kNew = false;
   __device__ static constexpr int out_f(int k) { return k; }  // vx vy vz
@@ -149,7 +161,7 @@ struct P3<kE3Vel> {
 };
 template <>
 struct P3<kE3Str> {
-  static constexpr int NF = 6, NS = 3, NQ = 3, YPT = 2, NM = 5;
+  static constexpr int NF = 6, NS = 3, NQ = 3, YPT = 2, NM = 5, TR = 4;
   __device__ static constexpr int mat_id(int k) { return 3 + k; }  // lam lam2mu mu_txy mu_txz mu_tyz
   static constexpr bool kNew = true;
   __device__ static constexpr int out_f(int k) { return 3 + k; }  // txx tyy tzz txy txz tyz
@@ -164,16 +176,20 @@ struct P3<kE3Str> {
   __device__ static constexpr bool q_back(int q) { return q == 2; }
 };
 
-static_assert(4 * P3<kA3Vel>::YPT == kRowsAc3 && 4 * P3<kA3Prs>::YPT == kRowsAc3, "acoustic tile rows");
-static_assert(4 * P3<kE3Vel>::YPT == kRowsEl3 && 4 * P3<kE3Str>::YPT == kRowsEl3, "elastic tile rows");
+static_assert(P3<kA3Vel>::TR * P3<kA3Vel>::YPT == kRowsAc3 && P3<kA3Prs>::TR * P3<kA3Prs>::YPT == kRowsAc3,
+              "acoustic tile rows");
+static_assert(P3<kE3Vel>::TR * P3<kE3Vel>::YPT == kRowsEl3 && P3<kE3Str>::TR * P3<kE3Str>::YPT == kRowsEl3,
+              "elastic tile rows");
 
-// LDS of a tile: the staged planes (with halos) and, for the frame march, two-plane rings of the
-// pre-correction values and of the old values of the outputs (planes t-1 and t).
+// LDS of a tile: the staged planes (with halos) of two consecutive levels and, for the frame
+// march, two-plane rings of the pre-correction values and of the old values of the outputs
+// (planes t-1 and t).  Level t uses staging buffer t % 2; with both buffers and both rings
+// alternating, one block barrier per plane orders every LDS write after the reads it replaces.
 template <int PH, int N>
 struct Lds3 {
-  static constexpr int ROWS = 4 * P3<PH>::YPT, PW = kTileW + 2 * N, PR = ROWS + 2 * N;
+  static constexpr int ROWS = P3<PH>::TR * P3<PH>::YPT, PW = kTileW + 2 * N, PR = ROWS + 2 * N;
   static constexpr int STG = P3<PH>::NS * PR * PW, SLOT = P3<PH>::NF * ROWS * kTileW;
-  static constexpr int SIZE = STG + 4 * SLOT;
+  static constexpr int CORE = 2 * STG, SIZE = 2 * STG + 4 * SLOT;
 };
 
 __device__ __forceinline__ void inject3d(const Shot3D& S, size_t c, int it, int nt, float& v) {
@@ -190,9 +206,8 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
   using P = P3<PH>;
   using L = Lds3<PH, N>;
   constexpr int YPT = P::YPT, ROWS = L::ROWS, NF = P::NF, NS = P::NS, NQ = P::NQ, NM = P::NM;
-  constexpr int PW = L::PW, PR = L::PR, HYR = (2 * N + 3) / 4;
-  float* stg = lds;
-  float* ring = lds + L::STG;  // pre-correction values of plane t in slot t % 2, old values in slot 2 + t % 2
+  constexpr int TR = P::TR, PW = L::PW, PR = L::PR, HYR = (2 * N + TR - 1) / TR;
+  float* ring = lds + 2 * L::STG;  // pre-correction values of plane t in slot t % 2, old values in slot 2 + t % 2
   const Shot3D& S = A.shot;
   const int nx = S.nx, ny = S.ny, nz = S.nz, H = S.H;
   const size_t sz = (size_t)nx * ny;
@@ -225,7 +240,7 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
 #pragma unroll
     for (int r = 0; r < YPT; ++r)
 #pragma unroll
-      for (int m = 0; m < 2 * N; ++m) q[qq][r][m] = ld(sb, P::q_f(qq), i, pj0 + ty + 4 * r, t0 + lo + m);
+      for (int m = 0; m < 2 * N; ++m) q[qq][r][m] = ld(sb, P::q_f(qq), i, pj0 + ty + TR * r, t0 + lo + m);
   }
 
   // Registers prefetched one plane ahead: own-column values of staged fields that have no queue,
@@ -237,7 +252,7 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
       const int f = P::st_f(sp);
 #pragma unroll
       for (int r = 0; r < YPT; ++r) {
-        const int j = pj0 + ty + 4 * r;
+        const int j = pj0 + ty + TR * r;
         if (P::st_q(sp) < 0) sv[sp][r] = ld(sb, f, i, j, tp);
         if (P::st_xh(sp)) {
           hx[sp][r][0] = tx < N ? ld(sb, f, pi0 - N + tx, j, tp) : 0.0f;
@@ -247,7 +262,7 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
       if (P::st_yh(sp)) {
 #pragma unroll
         for (int m = 0; m < HYR; ++m) {
-          const int hr = ty + 4 * m;
+          const int hr = ty + TR * m;
           const int jl = hr < N ? hr - N : ROWS + (hr - N);
           hy[sp][m] = hr < 2 * N ? ld(sb, f, i, pj0 + jl, tp) : 0.0f;
         }
@@ -258,7 +273,7 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
     const bool k_upd = tp >= N && tp < nz - N;
 #pragma unroll
     for (int r = 0; r < YPT; ++r) {
-      const int j = pj0 + ty + 4 * r;
+      const int j = pj0 + ty + TR * r;
       const bool valid = !LAG || (i < nx && j < ny);
       const bool upd = !LAG || (i_upd && k_upd && j >= N && j < ny - N);
       const unsigned co = (unsigned)(i + j * nx);
@@ -267,7 +282,8 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
         oldF[r][f] = valid ? Cvt<T>::ld(plane_of<T>(S.fld[s][P::out_f(f)], tp, sz), co, S.fdec[P::out_f(f)]) : 0.0f;
 #pragma unroll
       for (int k = 0; k < NM; ++k)
-        matF[r][k] = upd ? Cvt<T>::ld(plane_of<T>(S.mat[P::mat_id(k)], tp, sz), co, S.mdec[P::mat_id(k)]) : 0.0f;
+        matF[r][k] =
+            upd ? Cvt<T>::ld(plane_of<T>(S.mat[P::mat_id(k)], tp, sz), co, S.mdec[P::mat_id(k)]) : 0.0f;
     }
   };
 
@@ -277,13 +293,14 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
   prefetch_cells(t0);
   const int t_end = LAG ? W.k0 + W.hz : W.k0 + W.hz - 1;
   for (int t = t0; t <= t_end; ++t) {
+    float* stg = lds + (t & 1) * L::STG;
     // 1. stage the x/y-stencil planes of level t from the prefetched registers
 #pragma unroll
     for (int sp = 0; sp < NS; ++sp) {
       float* pl = stg + sp * PR * PW;
 #pragma unroll
       for (int r = 0; r < YPT; ++r) {
-        const int jl = ty + 4 * r;
+        const int jl = ty + TR * r;
         const int qs = P::st_q(sp) < 0 ? 0 : P::st_q(sp);
         const float v = P::st_q(sp) >= 0 ? q[qs][r][P::q_back(qs) ? N : N - 1] : sv[sp][r];
         pl[(jl + N) * PW + tx + N] = v;
@@ -295,7 +312,7 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
       if (P::st_yh(sp)) {
 #pragma unroll
         for (int m = 0; m < HYR; ++m) {
-          const int hr = ty + 4 * m;
+          const int hr = ty + TR * m;
           if (hr < 2 * N) {
             const int jl = hr < N ? hr - N : ROWS + (hr - N);
             pl[(jl + N) * PW + tx + N] = hy[sp][m];
@@ -310,7 +327,7 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
       for (int qq = 0; qq < NQ; ++qq) {
         const int lo = P::q_back(qq) ? -N : -N + 1;
 #pragma unroll
-        for (int r = 0; r < YPT; ++r) qn[qq][r] = ld(sb, P::q_f(qq), i, pj0 + ty + 4 * r, t + 1 + lo + 2 * N - 1);
+        for (int r = 0; r < YPT; ++r) qn[qq][r] = ld(sb, P::q_f(qq), i, pj0 + ty + TR * r, t + 1 + lo + 2 * N - 1);
       }
     }
     __syncthreads();
@@ -319,7 +336,7 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
     const bool k_upd = t >= N && t < nz - N;
 #pragma unroll
     for (int r = 0; r < YPT; ++r) {
-      const int jl = ty + 4 * r, j = pj0 + jl;
+      const int jl = ty + TR * r, j = pj0 + jl;
       const bool upd = !LAG || (i_upd && k_upd && j >= N && j < ny - N);
       const int row = (jl + N) * PW + tx + N;
       if constexpr (PH == kA3Vel) {
@@ -409,7 +426,7 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
       const bool col_out = tx < W.w;
 #pragma unroll
       for (int r = 0; r < YPT; ++r) {
-        const int jl = ty + 4 * r;
+        const int jl = ty + TR * r;
         if (!col_out || jl >= W.hy) continue;
         const unsigned co = (unsigned)(i + (pj0 + jl) * nx);
 #pragma unroll
@@ -419,7 +436,6 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
           Cvt<T>::st(plane_of<T>(S.fld[o][P::out_f(f)], t, sz), co, v, S.fenc[P::out_f(f)]);
         }
       }
-      __syncthreads();
     } else {
     // publish plane t's pre-correction and old values; then load the old values and materials of t+1
     {
@@ -429,16 +445,15 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
       for (int r = 0; r < YPT; ++r)
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
-          const int e = (f * ROWS + ty + 4 * r) * kTileW + tx;
+          const int e = (f * ROWS + ty + TR * r) * kTileW + tx;
           sp_[e] = preN[r][f];
           so_[e] = oldF[r][f];
           oldT[r][f] = oldF[r][f];
         }
     }
     if (t < t_end) prefetch_cells(t + 1);
-    __syncthreads();
 
-    // 4. finalise plane t-1: Higdon blend of frame cells, injection, store
+    // 4. finalise plane t-1 (its rings were written before this level's barrier): Higdon blend of frame cells, injection, store
     if (t > W.k0) {
       const int kk = t - 1;
       const bool kt = kk <= H, kb = kk >= nz - 1 - H;
@@ -448,7 +463,7 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
       const float* curo = ring + (2 + ((t - 1) & 1)) * L::SLOT;  // old values of plane t-1
 #pragma unroll
       for (int r = 0; r < YPT; ++r) {
-        const int jl = ty + 4 * r, j = pj0 + jl;
+        const int jl = ty + TR * r, j = pj0 + jl;
         if (!col_out || jl < ry || jl >= ry + W.hy) continue;
         const unsigned co = (unsigned)(i + j * nx);
         const bool jf = j <= H, jb = j >= ny - 1 - H;
@@ -495,7 +510,7 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
       for (int r = 0; r < YPT; ++r)
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
-          const int e = (f * ROWS + ty + 4 * r) * kTileW + tx;
+          const int e = (f * ROWS + ty + TR * r) * kTileW + tx;
           preP[r][f] = curp[e];
           oldP[r][f] = curo[e];
         }
@@ -515,8 +530,8 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
 }
 
 template <int PH, int N, class T, bool LAG>
-__global__ __launch_bounds__(256) void sweep3d_kernel(const Sweep3DArgs A) {
-  __shared__ float lds[LAG ? Lds3<PH, N>::SIZE : Lds3<PH, N>::STG];
+__global__ __launch_bounds__(kTileW * P3<PH>::TR) void sweep3d_kernel(const Sweep3DArgs A) {
+  __shared__ float lds[LAG ? Lds3<PH, N>::SIZE : Lds3<PH, N>::CORE];
   if ((int)blockIdx.x >= A.n_work) return;
   const Work3D W = A.work[blockIdx.x];
   if (W.kind < 0) return;
@@ -613,7 +628,7 @@ hipError_t launch_slice3d(int storage, const void* src, float* dst, int nx, int 
 template <int PH, class T, bool LAG>
 static hipError_t launch_n(int N, const Sweep3DArgs& a, hipStream_t s) {
   if (a.n_work <= 0) return hipSuccess;
-  const dim3 blk(kTileW, 4);
+  const dim3 blk(kTileW, P3<PH>::TR);
   switch (N) {
     case 1: hipLaunchKernelGGL((sweep3d_kernel<PH, 1, T, LAG>), dim3(a.n_work), blk, 0, s, a); break;
     case 2: hipLaunchKernelGGL((sweep3d_kernel<PH, 2, T, LAG>), dim3(a.n_work), blk, 0, s, a); break;

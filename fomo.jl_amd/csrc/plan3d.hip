@@ -493,7 +493,7 @@ int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, con
       if ((rc = p->alloc(&p->seis[k], (size_t)d->n_rec * d->nt))) return bail(rc);
   }
   if ((rc = p->alloc(&p->step, 1)) || (rc = p->alloc(&p->done, 1))) return bail(rc);
-  const int kseg = d->tile_cfg > 0 ? 16 << (d->tile_cfg - 1) : 64;
+  const int kseg = d->tile_cfg > 0 ? 16 << (d->tile_cfg - 1) : 32;
   if ((rc = make_work3d(p, kseg, elastic ? kRowsEl3 : kRowsAc3))) return bail(rc);
   if ((rc = p->reset())) return bail(rc);
   if (d->use_graph && (rc = p->build_graph())) return bail(rc);

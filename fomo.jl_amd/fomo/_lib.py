@@ -56,6 +56,7 @@ SIGNATURES = {
     "fomo_last_error": (C.c_char_p, []),
     "fomo_version": (C.c_char_p, []),
     "fomo_device_count": (C.c_int, []),
+    "fomo_stream_copy": (C.c_int, [C.c_int64, _i, C.POINTER(C.c_double)]),
     "fomo_fd_coefficients": (C.c_int, [_i, _p]),
     "fomo_ricker_wavelet": (C.c_int, [_f, _f, _i, _p]),
     "fomo_habc_coefficients": (C.c_int, [_f, _f, _f, _p]),

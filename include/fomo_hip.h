@@ -49,6 +49,11 @@ const char *fomo_version(void);
 /* Number of visible HIP devices (0 when no GPU is present; never fails). */
 int fomo_device_count(void);
 
+/* HBM calibration (no counterpart in the reference; a measurement helper for the roofline): copies
+ * `bytes` between two device buffers `iters` times with a float4 streaming kernel and writes the
+ * average milliseconds per copy.  Copy bandwidth = 2 * bytes / time. */
+int fomo_stream_copy(int64_t bytes, int32_t iters, double *ms_per_copy);
+
 /* ---- host-side setup helpers (no GPU needed) ---------------------------------------------- */
 
 /* Staggered FD coefficients for fd_order in {2,4,6,8,10}; writes fd_order/2 floats.
