@@ -44,6 +44,9 @@ struct Work3D {
 // thread keep the per-thread state small enough for 4 waves per SIMD, 16 rows halve the share of
 // y-halo reads), elastic 4 x 2.
 constexpr int kRowsAc3 = 16, kRowsEl3 = 8;
+// Core tiles (user interior, no frame logic): acoustic 16 x 2 rows (1024-thread blocks, y-halo
+// reads a quarter of the plane), elastic as its frame tiles.
+constexpr int kRowsAc3Core = 32, kRowsEl3Core = 8;
 
 // The shot descriptor travels by value in the kernel arguments (constant, so its pointers and
 // scales stay in scalar registers or are reloaded from the argument segment).

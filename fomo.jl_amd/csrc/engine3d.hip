@@ -107,14 +107,15 @@ __device__ __forceinline__ void* plane_of(void* p, int k, size_t plane_elems) {
 
 enum Phase3 { kA3Vel = 0, kA3Prs = 1, kE3Vel = 2, kE3Str = 3 };
 
-// Per phase: NF output fields, NS staged planes, NQ z-queues, NM materials, TR thread rows of the
-// block (64 x TR threads), YPT rows per thread (a tile is 64 x TR*YPT cells), whether the stencil
+// Per phase: NF output fields, NS staged planes, NQ z-queues, NM materials, TR / TRC thread rows of
+// the frame / core blocks (64 x TR threads), YPT rows per thread (a tile is 64 x TR*YPT cells),
+// whether the stencil
 // inputs are this step's new velocities (buffer 1-s), and the field ids involved.
 template <int PH>
 struct P3;
 template <>
 struct P3<kA3Vel> {
-  static constexpr int NF = 3, NS = 1, NQ = 1, YPT = 2, NM = 3, TR = 8;
+  static constexpr int NF = 3, NS = 1, NQ = 1, YPT = 2, NM = 3, TR = 8, TRC = 16;
   __device__ static constexpr int mat_id(int k) { return 1 + k; }  // bx by bz
   static constexpr bool kNew = false;
   __device__ static constexpr int out_f(int k) { return 1 + k; }                       // vx vy vz
@@ -129,7 +130,7 @@ struct P3<kA3Vel> {
 };
 template <>
 struct P3<kA3Prs> {
-  static constexpr int NF = 1, NS = 2, NQ = 1, YPT = 2, NM = 1, TR = 8;
+  static constexpr int NF = 1, NS = 2, NQ = 1, YPT = 2, NM = 1, TR = 8, TRC = 16;
   __device__ static constexpr int mat_id(int) { return 0; }  // kappa
   static constexpr bool kNew = true;
   __device__ static constexpr int out_f(int) { return 0; }
@@ -144,7 +145,7 @@ struct P3<kA3Prs> {
 };
 template <>
 struct P3<kE3Vel> {
-  static constexpr int NF = 3, NS = 5, NQ = 3, YPT = 2, NM = 3, TR = 4;
+  static constexpr int NF = 3, NS = 5, NQ = 3, YPT = 2, NM = 3, TR = 4, TRC = 4;
   __device__ static constexpr int mat_id(int k) { return k; }  // bx by bz
   static constexpr bool kNew = false;
   __device__ static constexpr int out_f(int k) { return k; }  // vx vy vz
@@ -161,7 +162,7 @@ struct P3<kE3Vel> {
 };
 template <>
 struct P3<kE3Str> {
-  static constexpr int NF = 6, NS = 3, NQ = 3, YPT = 2, NM = 5, TR = 4;
+  static constexpr int NF = 6, NS = 3, NQ = 3, YPT = 2, NM = 5, TR = 4, TRC = 4;
   __device__ static constexpr int mat_id(int k) { return 3 + k; }  // lam lam2mu mu_txy mu_txz mu_tyz
   static constexpr bool kNew = true;
   __device__ static constexpr int out_f(int k) { return 3 + k; }  // txx tyy tzz txy txz tyz
@@ -176,18 +177,22 @@ struct P3<kE3Str> {
   __device__ static constexpr bool q_back(int q) { return q == 2; }
 };
 
-static_assert(P3<kA3Vel>::TR * P3<kA3Vel>::YPT == kRowsAc3 && P3<kA3Prs>::TR * P3<kA3Prs>::YPT == kRowsAc3,
-              "acoustic tile rows");
-static_assert(P3<kE3Vel>::TR * P3<kE3Vel>::YPT == kRowsEl3 && P3<kE3Str>::TR * P3<kE3Str>::YPT == kRowsEl3,
-              "elastic tile rows");
+template <int PH, bool LAG>
+constexpr int kTR = LAG ? P3<PH>::TR : P3<PH>::TRC;  // thread rows of a block
+template <int PH, bool LAG>
+constexpr int kRows = kTR<PH, LAG> * P3<PH>::YPT;  // tile rows
+static_assert(kRows<kA3Vel, true> == kRowsAc3 && kRows<kA3Prs, true> == kRowsAc3, "acoustic frame tile rows");
+static_assert(kRows<kA3Vel, false> == kRowsAc3Core && kRows<kA3Prs, false> == kRowsAc3Core, "acoustic core tile rows");
+static_assert(kRows<kE3Vel, true> == kRowsEl3 && kRows<kE3Str, true> == kRowsEl3, "elastic frame tile rows");
+static_assert(kRows<kE3Vel, false> == kRowsEl3Core && kRows<kE3Str, false> == kRowsEl3Core, "elastic core tile rows");
 
 // LDS of a tile: the staged planes (with halos) of two consecutive levels and, for the frame
 // march, two-plane rings of the pre-correction values and of the old values of the outputs
 // (planes t-1 and t).  Level t uses staging buffer t % 2; with both buffers and both rings
 // alternating, one block barrier per plane orders every LDS write after the reads it replaces.
-template <int PH, int N>
+template <int PH, int N, bool LAG>
 struct Lds3 {
-  static constexpr int ROWS = P3<PH>::TR * P3<PH>::YPT, PW = kTileW + 2 * N, PR = ROWS + 2 * N;
+  static constexpr int ROWS = kRows<PH, LAG>, PW = kTileW + 2 * N, PR = ROWS + 2 * N;
   static constexpr int STG = P3<PH>::NS * PR * PW, SLOT = P3<PH>::NF * ROWS * kTileW;
   static constexpr int CORE = 2 * STG, SIZE = 2 * STG + 4 * SLOT;
 };
@@ -204,9 +209,9 @@ __device__ __forceinline__ void inject3d(const Shot3D& S, size_t c, int it, int 
 template <int PH, int N, class T, bool LAG>
 __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, float* lds) {
   using P = P3<PH>;
-  using L = Lds3<PH, N>;
+  using L = Lds3<PH, N, LAG>;
   constexpr int YPT = P::YPT, ROWS = L::ROWS, NF = P::NF, NS = P::NS, NQ = P::NQ, NM = P::NM;
-  constexpr int TR = P::TR, PW = L::PW, PR = L::PR, HYR = (2 * N + TR - 1) / TR;
+  constexpr int TR = kTR<PH, LAG>, PW = L::PW, PR = L::PR, HYR = (2 * N + TR - 1) / TR;
   float* ring = lds + 2 * L::STG;  // pre-correction values of plane t in slot t % 2, old values in slot 2 + t % 2
   const Shot3D& S = A.shot;
   const int nx = S.nx, ny = S.ny, nz = S.nz, H = S.H;
@@ -530,8 +535,8 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
 }
 
 template <int PH, int N, class T, bool LAG>
-__global__ __launch_bounds__(kTileW * P3<PH>::TR) void sweep3d_kernel(const Sweep3DArgs A) {
-  __shared__ float lds[LAG ? Lds3<PH, N>::SIZE : Lds3<PH, N>::CORE];
+__global__ __launch_bounds__((kTileW * kTR<PH, LAG>)) void sweep3d_kernel(const Sweep3DArgs A) {
+  __shared__ float lds[LAG ? Lds3<PH, N, LAG>::SIZE : Lds3<PH, N, LAG>::CORE];
   if ((int)blockIdx.x >= A.n_work) return;
   const Work3D W = A.work[blockIdx.x];
   if (W.kind < 0) return;
@@ -628,7 +633,7 @@ hipError_t launch_slice3d(int storage, const void* src, float* dst, int nx, int 
 template <int PH, class T, bool LAG>
 static hipError_t launch_n(int N, const Sweep3DArgs& a, hipStream_t s) {
   if (a.n_work <= 0) return hipSuccess;
-  const dim3 blk(kTileW, P3<PH>::TR);
+  const dim3 blk(kTileW, kTR<PH, LAG>);
   switch (N) {
     case 1: hipLaunchKernelGGL((sweep3d_kernel<PH, 1, T, LAG>), dim3(a.n_work), blk, 0, s, a); break;
     case 2: hipLaunchKernelGGL((sweep3d_kernel<PH, 2, T, LAG>), dim3(a.n_work), blk, 0, s, a); break;

@@ -186,7 +186,7 @@ static int upload_work(Plan3D* p, int which, std::vector<Work3D>& items, int row
   return p->upload(&p->d_work[which], flat.data(), flat.size());
 }
 
-static int make_work3d(Plan3D* p, int kseg, int rows) {
+static int make_work3d(Plan3D* p, int kseg, int rows, int core_rows) {
   const int nx = p->shot.nx, ny = p->shot.ny, nz = p->shot.nz, pad = p->pad;
   std::vector<Work3D> core_fp, ring_fp;
   // rx / ry: the rectangle holds x / y frame cells (then the tile computes a ring in that direction)
@@ -196,9 +196,14 @@ static int make_work3d(Plan3D* p, int kseg, int rows) {
       for (int i = x0; i < x1; i += tw)
         ring_fp.push_back(Work3D{1, i, j, 0, std::min(tw, x1 - i), std::min(th, y1 - j), 0, rx | (ry << 1)});
   };
+  // core footprints for the core list (core_rows) and for the frame list's top/bottom slabs (rows)
+  std::vector<Work3D> slab_fp;
+  for (int j = pad; j < ny - pad; j += core_rows)
+    for (int i = pad; i < nx - pad; i += kTileW)
+      core_fp.push_back(Work3D{1, i, j, 0, std::min(kTileW, nx - pad - i), std::min(core_rows, ny - pad - j), 0, 0});
   for (int j = pad; j < ny - pad; j += rows)
     for (int i = pad; i < nx - pad; i += kTileW)
-      core_fp.push_back(Work3D{1, i, j, 0, std::min(kTileW, nx - pad - i), std::min(rows, ny - pad - j), 0, 0});
+      slab_fp.push_back(Work3D{1, i, j, 0, std::min(kTileW, nx - pad - i), std::min(rows, ny - pad - j), 0, 0});
   ring_rect(1, nx - 1, 1, pad, 1, 1);
   ring_rect(1, pad, pad, ny - pad, 1, 0);
   ring_rect(nx - pad, nx - 1, pad, ny - pad, 1, 0);
@@ -219,10 +224,10 @@ static int make_work3d(Plan3D* p, int kseg, int rows) {
   };
   std::vector<Work3D> core, frame;
   segs(pad, nz - pad, core_fp, core);
-  segs(1, pad, core_fp, frame);
-  segs(nz - pad, nz - 1, core_fp, frame);
+  segs(1, pad, slab_fp, frame);
+  segs(nz - pad, nz - 1, slab_fp, frame);
   segs(1, nz - 1, ring_fp, frame);
-  int rc = upload_work(p, 0, core, rows);
+  int rc = upload_work(p, 0, core, core_rows);
   if (!rc) rc = upload_work(p, 1, frame, rows);
   return rc;
 }
@@ -494,7 +499,7 @@ int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, con
   }
   if ((rc = p->alloc(&p->step, 1)) || (rc = p->alloc(&p->done, 1))) return bail(rc);
   const int kseg = d->tile_cfg > 0 ? 16 << (d->tile_cfg - 1) : 32;
-  if ((rc = make_work3d(p, kseg, elastic ? kRowsEl3 : kRowsAc3))) return bail(rc);
+  if ((rc = make_work3d(p, kseg, elastic ? kRowsEl3 : kRowsAc3, elastic ? kRowsEl3Core : kRowsAc3Core))) return bail(rc);
   if ((rc = p->reset())) return bail(rc);
   if (d->use_graph && (rc = p->build_graph())) return bail(rc);
   e = hipStreamSynchronize(p->stream);
