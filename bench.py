@@ -25,6 +25,10 @@ for p in (os.path.join(ROOT, "fomo.jl_amd"), os.path.join(ROOT, "oracle")):
         sys.path.insert(0, p)
 
 METRIC = "cell-updates/s & % HBM roofline, FD8 2D/3D; learned-ABC+FP16 vs HABC at equal error"
+# HBM-side bytes per launch of the two sweeps at this workload, measured with rocprofv3 --pmc
+# (FETCH_SIZE scaled x2.0 and WRITE_SIZE x1.0, the factors the same counters read on a streaming
+# copy of known size); profiles/elastic2d_2000x800_fd8_pmc_traffic.md holds the readings.
+PMC_TRAFFIC = {"velocity_sweep": 77.0e6, "stress_sweep": 89.9e6}
 HBM_PEAK_GBS = 8000.0
 NX, NZ, DH, DT, NT, F0, NBC, ORDER, SEED = 2000, 800, 5.0, 4e-4, 4000, 20.0, 50, 8, 20260101
 BYTES_VEL, BYTES_STR = 36, 44
@@ -250,7 +254,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": PMC_TRAFFIC[dom_name],
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, calibrated on a "
+                            "streaming copy; profiles/elastic2d_2000x800_fd8_pmc_traffic.md)",
+            "algorithmic_bytes_per_launch": dom_bytes * n_pad,
             "bytes_per_cell": dom_bytes,
         },
     }
