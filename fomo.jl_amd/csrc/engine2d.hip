@@ -286,7 +286,12 @@ __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, 
 #pragma unroll
     for (int f = 0; f < NF; ++f) v[f] = pre_s[f * EW * EH + lj * EW + li];
     const int dist = min(min(i, nx - 1 - i), min(j, nz - 1 - j));
-    if (dist >= 1 && dist <= H) {
+    if (S.damp) {
+      // Cerjan sponge, src/boundary/sponge.jl:56-78: field *= damp after the update
+      const float dm = G(S.damp)[dist];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) v[f] = v[f] * dm;
+    } else if (dist >= 1 && dist <= H) {
       // Region flags (0-based): left i <= H, right i >= nx-1-H, top j <= H, bottom j >= nz-1-H.
       const bool Lf = i <= H, Rf = i >= nx - 1 - H, Tf = j <= H, Bf = j >= nz - 1 - H;
       const bool inx = Lf || Rf, inz = Tf || Bf;

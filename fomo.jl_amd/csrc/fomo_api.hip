@@ -216,6 +216,19 @@ void weight_lut(int H, int pad, double off, float* out) {
   }
 }
 
+// exp(-(factor*(nbc-d))^2) for d < nbc, 1 otherwise: factor*(nbc-d) and its square in Float32 as
+// sponge.jl:44-46 evaluates them, the exponential in double rounded once to Float32.
+void sponge_lut(int nbc, int pad, float factor, float* out) {
+  for (int d = 0; d < pad; ++d) {
+    if (d < nbc) {
+      const float r = factor * (float)(nbc - d);
+      out[d] = (float)std::exp((double)(-r * r));
+    } else {
+      out[d] = 1.0f;
+    }
+  }
+}
+
 float min_positive(const float* a, size_t n) {
   float m = INFINITY;
   for (size_t c = 0; c < n; ++c)
@@ -289,6 +302,13 @@ static int validate_common(const fomo_desc* d) {
   if (d->n_src < 0 || d->n_rec < 0) return fail(FOMO_EINVAL, "n_src and n_rec must be >= 0");
   if (d->snap_interval < 0) return fail(FOMO_EINVAL, "snap_interval must be >= 0");
   if (!(d->dh > 0.0f) || !(d->dt > 0.0f)) return fail(FOMO_EINVAL, "dh and dt must be > 0");
+  if (d->boundary != FOMO_BOUNDARY_HABC && d->boundary != FOMO_BOUNDARY_SPONGE)
+    return fail(FOMO_EINVAL, "boundary must be FOMO_BOUNDARY_HABC or FOMO_BOUNDARY_SPONGE");
+  if (d->boundary == FOMO_BOUNDARY_SPONGE) {
+    if (!(d->sponge_factor > 0.0f)) return fail(FOMO_EINVAL, "sponge_factor must be > 0");
+    if (d->eq != FOMO_ACOUSTIC2D && d->eq != FOMO_ELASTIC2D)
+      return fail(FOMO_EUNSUP, "the sponge boundary is 2D only (as in the reference)");
+  }
   if (fomo_device_count() < 1) return fail(FOMO_EHIP, "no HIP device visible");
   return FOMO_OK;
 }

@@ -50,6 +50,9 @@ struct Shot2D {
   // f_pde - S of vx, vz, p.
   const float* alpha[2];
   float* sens[3];
+  // Cerjan sponge instead of the Higdon blend (NULL: HABC): damping factor by distance to the
+  // padded edge, pad entries (1 at distances >= nbc).
+  const float* damp;
 };
 
 // One unit of sweep work.  kind 0: a frame tile (at most kTileW x kFrameRows cells of the

@@ -34,6 +34,7 @@ extern "C" int fomo_learned_enable(fomo_plan* p, const float* weights, int32_t t
   if (q->shots.size() != 1) return set_error(FOMO_EUNSUP, "the learned boundary runs on single-shot plans");
   if (!weights) return set_error(FOMO_EINVAL, "weights must not be NULL");
   if (q->learned) return set_error(FOMO_EINVAL, "the learned boundary is already enabled");
+  if (q->d.boundary != FOMO_BOUNDARY_HABC) return set_error(FOMO_EINVAL, "the learned boundary replaces the HABC");
   Shot2D& S = q->shots[0];
   const size_t n = (size_t)S.nx * S.nz;
   std::vector<int> cells;

@@ -73,6 +73,8 @@ void buoyancy2d(const std::vector<float>& rho, int nx, int nz, std::vector<float
 void elastic_params2d(const std::vector<float>& vp, const std::vector<float>& vs, const std::vector<float>& rho,
                       int nx, int nz, std::vector<float>& lam, std::vector<float>& lam2mu, std::vector<float>& muxz);
 void weight_lut(int H, int pad, double off, float* out);
+// Cerjan sponge factors by distance d = 0..pad-1 to the padded edge (src/boundary/sponge.jl:30-50).
+void sponge_lut(int nbc, int pad, float factor, float* out);
 float min_positive(const float* a, size_t n);
 
 }  // namespace fomo

@@ -248,6 +248,14 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
   if (e != hipSuccess) return bail(set_error(FOMO_EHIP, std::string("stream: ") + hipGetErrorString(e)));
   const int pad = d->nbc + p->N, H = pad - 1;
   const float dtx = d->dt / d->dh;
+  // Cerjan sponge factors by distance, shared by every shot
+  float* d_damp = nullptr;
+  if (d->boundary == FOMO_BOUNDARY_SPONGE) {
+    std::vector<float> lut(pad);
+    sponge_lut(d->nbc, pad, d->sponge_factor, lut.data());
+    int rc0 = p->upload(&d_damp, lut.data(), lut.size());
+    if (rc0) return bail(rc0);
+  }
   // Medium and HABC tables, shared between shots with the same window and v_ref.
   std::map<std::tuple<int, int, float>, int> medium_of;
   std::map<const float*, const float*> wav_dev;
@@ -268,6 +276,7 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
     x.nz = d->nz + 2 * pad;
     x.H = H;
     x.dtx = dtx;
+    x.damp = d_damp;
     const size_t n = (size_t)x.nx * x.nz;
     float v_ref = S.v_ref;
     if (!(v_ref > 0.0f)) {

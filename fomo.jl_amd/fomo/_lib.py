@@ -16,6 +16,7 @@ FOMO_OK, FOMO_EINVAL, FOMO_EHIP, FOMO_ENOMEM, FOMO_EUNSUP = 0, 1, 2, 3, 4
 ACOUSTIC2D, ELASTIC2D, ACOUSTIC3D, ELASTIC3D = 1, 2, 3, 4
 SRC_EXPLOSIVE, SRC_FZ = 0, 1
 STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
+BOUNDARY_HABC, BOUNDARY_SPONGE = 0, 1
 
 
 class ArgumentError(ValueError):
@@ -35,7 +36,7 @@ class Desc(C.Structure):
         ("n_src", C.c_int32), ("n_rec", C.c_int32), ("source_type", C.c_int32), ("storage", C.c_int32),
         ("snap_interval", C.c_int32), ("snap_plane", C.c_int32), ("snap_index", C.c_int32),
         ("use_graph", C.c_int32), ("record_p", C.c_int32), ("tile_cfg", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("boundary", C.c_int32), ("sponge_factor", C.c_float), ("reserved", C.c_int32 * 4),
     ]
 
 

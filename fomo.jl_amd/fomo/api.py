@@ -56,8 +56,10 @@ def wavelet_matrix(wavelet, f0, dt, nt, n_src, scale_source_by_dt) -> np.ndarray
 
 
 def _desc(eq, nx, nz, nbc, fd_order, nt, dh, dt, v_ref, n_src, n_rec, snap_interval, use_graph,
-          source_type=0, ny=1, storage=0) -> Desc:
+          source_type=0, ny=1, storage=0, boundary="habc", sponge_factor=0.015) -> Desc:
     d = Desc()
+    d.boundary = _lib.BOUNDARY_SPONGE if _sym(boundary) == "sponge" else _lib.BOUNDARY_HABC
+    d.sponge_factor = float(np.float32(sponge_factor))
     d.eq, d.nx, d.ny, d.nz = eq, nx, ny, nz
     d.nbc, d.fd_order, d.nt = nbc, fd_order, nt
     d.dh, d.dt, d.v_ref = dh, dt, v_ref
@@ -71,8 +73,6 @@ def _desc(eq, nx, nz, nbc, fd_order, nt, dh, dt, v_ref, n_src, n_rec, snap_inter
 def _common_checks(boundary, nbc, fd_order, nt, snap_interval):
     if _sym(boundary) not in ("habc", "sponge"):
         raise ArgumentError(f"boundary must be :habc or :sponge, got {boundary}")
-    if _sym(boundary) == "sponge":
-        raise _lib.FomoError("boundary=:sponge is not provided by this engine (HABC only)")
     if nbc < 0 or nt < 0 or snap_interval < 0:
         raise ArgumentError("nbc, nt and snap_interval must be >= 0")
     if fd_order not in (2, 4, 6, 8, 10):
@@ -82,8 +82,10 @@ def _common_checks(boundary, nbc, fd_order, nt, snap_interval):
 def acoustic2d(vp, rho, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_order=8, snap_interval=0,
                boundary="habc", v_ref=None, sponge_factor=0.015, wavelet=None, scale_source_by_dt=False,
                use_cuda_graph=True, verbose=True):
-    """acoustic2d(vp, rho, dh, dt, nt, f0; sx, sz, rx, rz, ...) -> (seis_p, seis_vx, seis_vz, snaps, stats)."""
-    del sponge_factor, verbose
+    """acoustic2d(vp, rho, dh, dt, nt, f0; sx, sz, rx, rz, ...) -> (seis_p, seis_vx, seis_vz, snaps, stats).
+
+    boundary = :habc (deterministic two-pass Higdon) or :sponge (Cerjan, `sponge_factor`)."""
+    del verbose
     _common_checks(boundary, nbc, fd_order, nt, snap_interval)
     vp, rho = _f32F(vp), _f32F(rho)
     if vp.ndim != 2 or vp.shape != rho.shape:
@@ -99,7 +101,7 @@ def acoustic2d(vp, rho, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_order=8, s
     wm = wavelet_matrix(wavelet, f0, dt, nt, len(sx), scale_source_by_dt)
     n_rec = len(rx)
     d = _desc(_lib.ACOUSTIC2D, nx, nz, nbc, fd_order, nt, dh, dt, v_ref, len(sx), n_rec, snap_interval,
-              use_cuda_graph)
+              use_cuda_graph, boundary=boundary, sponge_factor=sponge_factor)
     sp, svx, svz = (np.zeros((n_rec, nt), np.float32, order="F") for _ in range(3))
     nsnap = nt // snap_interval if snap_interval > 0 else 0
     snaps = np.zeros((nsnap, nz, nx), np.float32) if nsnap else None
@@ -119,7 +121,7 @@ def elastic2d(vp, vs, rho, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_order=8
 
     `source_type` (extension, default :explosive = the reference's τxx+τzz source); :fz adds the
     wavelet to vz at the end of the velocity phase (a vertical force)."""
-    del sponge_factor, verbose
+    del verbose
     _common_checks(boundary, nbc, fd_order, nt, snap_interval)
     st = _sym(source_type)
     if st not in ("explosive", "fz"):
@@ -139,7 +141,8 @@ def elastic2d(vp, vs, rho, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_order=8
     wm = wavelet_matrix(wavelet, f0, dt, nt, len(sx), scale_source_by_dt)
     n_rec = len(rx)
     d = _desc(_lib.ELASTIC2D, nx, nz, nbc, fd_order, nt, dh, dt, v_ref, len(sx), n_rec, snap_interval,
-              use_cuda_graph, source_type=_lib.SRC_FZ if st == "fz" else _lib.SRC_EXPLOSIVE)
+              use_cuda_graph, source_type=_lib.SRC_FZ if st == "fz" else _lib.SRC_EXPLOSIVE, boundary=boundary,
+              sponge_factor=sponge_factor)
     svx, svz = (np.zeros((n_rec, nt), np.float32, order="F") for _ in range(2))
     nsnap = nt // snap_interval if snap_interval > 0 else 0
     snaps = np.zeros((nsnap, nz, nx), np.float32) if nsnap else None

@@ -100,6 +100,24 @@ def layered_3d(nx: int, ny: int, nz: int, *, seed: int, n_interfaces: int = 12, 
     return vp, rho
 
 
+def patch_model(nx: int, nz: int):
+    """The reference's absorption-test model (test/verify_patch.jl:16-30): 2000 m/s over 3000 m/s
+    below a cosine interface at depth 100 +- 20, plus a 400 m/s Gaussian anomaly at depth 60;
+    vs = vp/1.73, Gardner rho.  Returns (vp, vs, rho) as Float32 (nx, nz) arrays."""
+    xc = (nx + 1) / 2
+    vp = np.full((nx, nz), 2000.0, np.float32)
+    for i in range(1, nx + 1):
+        zint = 100 + int(np.rint(20 * math.cos(2 * math.pi * (i - xc) / nx)))
+        vp[i - 1, zint - 1:] = 3000.0
+    xg = (np.arange(1, nx + 1) - xc) / 30
+    zg = (np.arange(1, nz + 1) - 60) / 15
+    g = np.exp(-(xg[:, None] ** 2 + zg[None, :] ** 2)).astype(np.float32)  # Float32.(exp.(...))
+    vp = (vp + np.float32(400.0) * g).astype(np.float32)
+    vs = (vp / np.float32(1.73)).astype(np.float32)
+    rho = (310.0 * vp.astype(np.float64) ** 0.25).astype(np.float32)
+    return vp, vs, rho
+
+
 def model_digest(*arrays) -> str:
     h = hashlib.sha256()
     for a in arrays:

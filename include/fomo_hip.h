@@ -38,6 +38,9 @@ extern "C" {
 #define FOMO_SRC_EXPLOSIVE 0 /* elastic: wavelet added to txx and tzz (reference behaviour) */
 #define FOMO_SRC_FZ 1        /* elastic: vertical force, wavelet added to vz after the velocity HABC */
 
+#define FOMO_BOUNDARY_HABC 0   /* deterministic two-pass Higdon (reference default) */
+#define FOMO_BOUNDARY_SPONGE 1 /* Cerjan sponge, src/boundary/sponge.jl (2D solo and batch runs) */
+
 #define FOMO_STORE_F32 0
 #define FOMO_STORE_F16 1
 #define FOMO_STORE_BF16 2
@@ -85,7 +88,9 @@ typedef struct fomo_desc {
   int32_t use_graph;     /* 1: one hipGraph launch per pair of steps; 0: direct launches */
   int32_t record_p;      /* acoustic3d: also gather p (extension) */
   int32_t tile_cfg;      /* 0: autotune the tile shape; k > 0: fixed shape k-1 */
-  int32_t reserved[6];
+  int32_t boundary;      /* FOMO_BOUNDARY_HABC or FOMO_BOUNDARY_SPONGE (2D) */
+  float sponge_factor;   /* Cerjan factor (reference default 0.015); used with FOMO_BOUNDARY_SPONGE */
+  int32_t reserved[4];
 } fomo_desc;
 
 /* One shot of a batch: a window of the model (full depth) with its own sources and receivers.
@@ -104,8 +109,8 @@ typedef struct fomo_shot {
 
 /* ---- one-shot drop-in entry points -------------------------------------------------------- */
 /*
- * acoustic2d -- replaces Fomo.acoustic2d (src/equations/acoustic2d/acoustic2d.jl:47-192) for
- * boundary=:habc.  vp, rho: nx*nz.  wavelet: n_src x nt (already repeated per source and, if
+ * acoustic2d -- replaces Fomo.acoustic2d (src/equations/acoustic2d/acoustic2d.jl:47-192);
+ * d->boundary selects :habc or :sponge (acoustic2d.jl:82-88, src/boundary/sponge.jl:30-78).  vp, rho: nx*nz.  wavelet: n_src x nt (already repeated per source and, if
  * requested, scaled by dt, as acoustic2d.jl:91-97 does).  Outputs (n_rec x nt each, any may be
  * NULL): seis_p, seis_vx, seis_vz.  snaps: (nt/snap_interval) x nx*nz interior p, or NULL.
  * kernel_time_s: hipEvent time of the step loop (setup, upload and download excluded).
@@ -115,8 +120,8 @@ int fomo_acoustic2d(const fomo_desc *d, const float *vp, const float *rho, const
                     float *seis_vz, float *snaps, double *kernel_time_s);
 
 /*
- * elastic2d -- replaces Fomo.elastic2d (src/equations/elastic2d/elastic2d.jl:45-163) for
- * boundary=:habc.  Outputs seis_vx, seis_vz (n_rec x nt); snaps of interior vz.
+ * elastic2d -- replaces Fomo.elastic2d (src/equations/elastic2d/elastic2d.jl:45-163);
+ * d->boundary selects :habc or :sponge (elastic2d.jl:82-87, 180-195).  Outputs seis_vx, seis_vz (n_rec x nt); snaps of interior vz.
  */
 int fomo_elastic2d(const fomo_desc *d, const float *vp, const float *vs, const float *rho, const int32_t *sx,
                    const int32_t *sz, const float *wavelet, const int32_t *rx, const int32_t *rz, float *seis_vx,

@@ -390,6 +390,32 @@ static int geom2d_init(geom2d *g, int nxi, int nzi, int nbc, int fd_order, float
   return 0;
 }
 
+/* Cerjan sponge (src/boundary/sponge.jl:30-50): damp = exp(-(factor*(nbc-d))^2) for d < nbc, d the
+ * distance to the nearest padded edge (1-based min(i-1, nx-i, j-1, nz-j)), 1 elsewhere.  The product
+ * factor*(nbc-d) and its square are Float32 as in the reference; exp is evaluated in double and
+ * rounded once (Julia's Float32 exp agrees to the last ulp, which no test can pin). */
+void orc_sponge_damp2d(int nx, int nz, int nbc, float factor, real *damp) {
+  for (int j = 1; j <= nz; ++j)
+    for (int i = 1; i <= nx; ++i) {
+      int d = i - 1;
+      if (nx - i < d) d = nx - i;
+      if (j - 1 < d) d = j - 1;
+      if (nz - j < d) d = nz - j;
+      real v = (real)1;
+      if (d < nbc) {
+        float r = factor * (float)(nbc - d);
+        v = (real)(float)exp((double)(-r * r));
+      }
+      damp[(size_t)(i - 1) + (size_t)(j - 1) * nx] = v;
+    }
+}
+
+/* sponge.jl:56-78: field .*= damp over the whole padded grid */
+static void sponge_apply(real *f, const real *damp, size_t n) {
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < (long)n; ++c) f[c] = f[c] * damp[c];
+}
+
 /*
  * acoustic2d, boundary = :habc, det two-pass path (loop_graph.jl:28-39 step order).
  *   inputs  vp, rho: nxi*nzi (x fastest); sx,sz,rx,rz: 1-based interior indices;
@@ -397,11 +423,13 @@ static int geom2d_init(geom2d *g, int nxi, int nzi, int nbc, int fd_order, float
  *   outputs seis_p, seis_vx, seis_vz: n_rec x nt column-major; snaps: n_snap x (nxi*nzi) of p.
  *   final_state (optional, 3*nx*nz): p, vx, vz after the last step.
  *   habc_width <= 0 selects the reference's current frame width pad-1.
+ *   sponge_factor > 0 selects boundary = :sponge instead (acoustic2d.jl:213-228: update, then
+ *   multiply vx, vz / p by the damping profile; no HABC).
  */
 int orc_acoustic2d(int nxi, int nzi, const float *vp, const float *rho, float dh, float dt, int nt, int nbc,
                    int fd_order, float v_ref, int n_src, const int *sx, const int *sz, const float *wavelet, int n_rec,
                    const int *rx, const int *rz, real *seis_p, real *seis_vx, real *seis_vz, int snap_interval,
-                   real *snaps, real *final_state, int habc_width, int nthreads) {
+                   real *snaps, real *final_state, int habc_width, float sponge_factor, int nthreads) {
   set_threads(nthreads);
   geom2d g;
   if (geom2d_init(&g, nxi, nzi, nbc, fd_order, dt, dh, v_ref, habc_width)) return -1;
@@ -421,20 +449,31 @@ int orc_acoustic2d(int nxi, int nzi, const float *vp, const float *rho, float dh
   orc_habc_weights2d(nx, nz, g.pad, 0.0, wvx);
   orc_habc_weights2d(nx, nz, g.pad, 0.5, wvz);
   orc_habc_weights2d(nx, nz, g.pad, 0.75, wtau);
+  const int sponge = sponge_factor > 0.0f;
+  real *damp = sponge ? xcalloc(n, sizeof(real)) : NULL;
+  if (sponge) orc_sponge_damp2d(nx, nz, nbc, sponge_factor, damp);
   int snap_idx = 0;
   for (int it = 1; it <= nt; ++it) {
-    /* A. velocity phase: backup + update, det HABC on (vx, w_vx), (vz, w_vz) */
+    /* A. velocity phase: backup + update, det HABC on (vx, w_vx), (vz, w_vz) -- or the sponge */
     real *fv[2] = {vx, vz}, *fvo[2] = {vxo, vzo};
     const real *wv[2] = {wvx, wvz};
-    strip_backup(2, fv, fvo, nx, nz, g.H);
+    if (!sponge) strip_backup(2, fv, fvo, nx, nz, g.H);
     acoustic_velocity(vx, vz, p, bx, bz, g.a, g.N, g.dtx, g.dtx, nx, nz);
-    habc_det(2, fv, fvo, wv, scr, g.q, g.qt, g.qxt, nx, nz, g.H);
+    if (sponge) {
+      sponge_apply(vx, damp, n);
+      sponge_apply(vz, damp, n);
+    } else {
+      habc_det(2, fv, fvo, wv, scr, g.q, g.qt, g.qxt, nx, nz, g.H);
+    }
     /* B. pressure phase */
     real *fp[1] = {p}, *fpo[1] = {po};
     const real *wp[1] = {wtau};
-    strip_backup(1, fp, fpo, nx, nz, g.H);
+    if (!sponge) strip_backup(1, fp, fpo, nx, nz, g.H);
     acoustic_pressure(p, vx, vz, kappa, g.a, g.N, g.dtx, g.dtx, nx, nz);
-    habc_det(1, fp, fpo, wp, scr, g.q, g.qt, g.qxt, nx, nz, g.H);
+    if (sponge)
+      sponge_apply(p, damp, n);
+    else
+      habc_det(1, fp, fpo, wp, scr, g.q, g.qt, g.qxt, nx, nz, g.H);
     /* C. raw-add injection after HABC */
     for (int s = 0; s < n_src; ++s) {
       size_t c = (size_t)(sx[s] + g.pad - 1) + (size_t)(sz[s] + g.pad - 1) * nx;
@@ -463,7 +502,7 @@ int orc_acoustic2d(int nxi, int nzi, const float *vp, const float *rho, float dh
     memcpy(final_state + 2 * n, vz, n * sizeof(real));
   }
   free(vpp); free(rhop); free(kappa); free(bx); free(bz); free(wvx); free(wvz); free(wtau);
-  free(vx); free(vz); free(p); free(vxo); free(vzo); free(po); free(scr);
+  free(vx); free(vz); free(p); free(vxo); free(vzo); free(po); free(scr); free(damp);
   return 0;
 }
 
@@ -473,11 +512,14 @@ int orc_acoustic2d(int nxi, int nzi, const float *vp, const float *rho, float dh
  *   source_type 1: vertical force -- wavelet added to vz at the end of the velocity phase,
  *                  after its HABC (an extension the reference does not have).
  *   outputs seis_vx, seis_vz (n_rec x nt); snaps of vz; final_state 5*nx*nz (vx,vz,txx,tzz,txz).
+ *   sponge_factor > 0 selects boundary = :sponge (elastic2d.jl:180-195: vx, vz and txx, tzz, txz
+ *   multiplied by the damping profile after their updates).
  */
 int orc_elastic2d(int nxi, int nzi, const float *vp, const float *vs, const float *rho, float dh, float dt, int nt,
                   int nbc, int fd_order, float v_ref, int source_type, int n_src, const int *sx, const int *sz,
                   const float *wavelet, int n_rec, const int *rx, const int *rz, real *seis_vx, real *seis_vz,
-                  int snap_interval, real *snaps, real *final_state, int habc_width, int nthreads) {
+                  int snap_interval, real *snaps, real *final_state, int habc_width, float sponge_factor,
+                  int nthreads) {
   set_threads(nthreads);
   geom2d g;
   if (geom2d_init(&g, nxi, nzi, nbc, fd_order, dt, dh, v_ref, habc_width)) return -1;
@@ -501,13 +543,21 @@ int orc_elastic2d(int nxi, int nzi, const float *vp, const float *vs, const floa
   orc_habc_weights2d(nx, nz, g.pad, 0.0, wvx);
   orc_habc_weights2d(nx, nz, g.pad, 0.5, wvz);
   orc_habc_weights2d(nx, nz, g.pad, 0.75, wtau);
+  const int sponge = sponge_factor > 0.0f;
+  real *damp = sponge ? xcalloc(n, sizeof(real)) : NULL;
+  if (sponge) orc_sponge_damp2d(nx, nz, nbc, sponge_factor, damp);
   int snap_idx = 0;
   for (int it = 1; it <= nt; ++it) {
     real *fv[2] = {vx, vz}, *fvo[2] = {vxo, vzo};
     const real *wv[2] = {wvx, wvz};
-    strip_backup(2, fv, fvo, nx, nz, g.H);
+    if (!sponge) strip_backup(2, fv, fvo, nx, nz, g.H);
     elastic_velocity(vx, vz, txx, tzz, txz, bx, bz, g.a, g.N, g.dtx, g.dtx, nx, nz);
-    habc_det(2, fv, fvo, wv, scr, g.q, g.qt, g.qxt, nx, nz, g.H);
+    if (sponge) {
+      sponge_apply(vx, damp, n);
+      sponge_apply(vz, damp, n);
+    } else {
+      habc_det(2, fv, fvo, wv, scr, g.q, g.qt, g.qxt, nx, nz, g.H);
+    }
     /* vertical force (extension): raw add into vz at the end of the velocity phase */
     if (source_type == 1)
       for (int s = 0; s < n_src; ++s) {
@@ -516,9 +566,15 @@ int orc_elastic2d(int nxi, int nzi, const float *vp, const float *vs, const floa
       }
     real *fs[3] = {txx, tzz, txz}, *fso[3] = {txxo, tzzo, txzo};
     const real *ws[3] = {wtau, wtau, wtau};
-    strip_backup(3, fs, fso, nx, nz, g.H);
+    if (!sponge) strip_backup(3, fs, fso, nx, nz, g.H);
     elastic_stress(txx, tzz, txz, vx, vz, lam, l2m, muxz, g.a, g.N, g.dtx, g.dtx, nx, nz);
-    habc_det(3, fs, fso, ws, scr, g.q, g.qt, g.qxt, nx, nz, g.H);
+    if (sponge) {
+      sponge_apply(txx, damp, n);
+      sponge_apply(tzz, damp, n);
+      sponge_apply(txz, damp, n);
+    } else {
+      habc_det(3, fs, fso, ws, scr, g.q, g.qt, g.qxt, nx, nz, g.H);
+    }
     for (int s = 0; s < n_src; ++s) {
       size_t c = (size_t)(sx[s] + g.pad - 1) + (size_t)(sz[s] + g.pad - 1) * nx;
       real wav = (real)wavelet[(size_t)s + (size_t)n_src * (it - 1)];
@@ -550,7 +606,7 @@ int orc_elastic2d(int nxi, int nzi, const float *vp, const float *vs, const floa
   }
   free(vpp); free(vsp); free(rhop); free(lam); free(mu); free(muxz); free(l2m); free(bx); free(bz);
   free(wvx); free(wvz); free(wtau); free(vx); free(vz); free(txx); free(tzz); free(txz);
-  free(vxo); free(vzo); free(txxo); free(tzzo); free(txzo); free(scr);
+  free(vxo); free(vzo); free(txxo); free(tzzo); free(txzo); free(scr); free(damp);
   return 0;
 }
 

@@ -41,10 +41,10 @@ def lib(precision: str = "f32") -> C.CDLL:
         real = _f if precision == "f32" else C.c_double
         L.orc_acoustic2d.restype = _i
         L.orc_acoustic2d.argtypes = [_i, _i, _p, _p, _f, _f, _i, _i, _i, _f, _i, _p, _p, _p, _i, _p, _p,
-                                     _p, _p, _p, _i, _p, _p, _i, _i]
+                                     _p, _p, _p, _i, _p, _p, _i, _f, _i]
         L.orc_elastic2d.restype = _i
         L.orc_elastic2d.argtypes = [_i, _i, _p, _p, _p, _f, _f, _i, _i, _i, _f, _i, _i, _p, _p, _p, _i, _p, _p,
-                                    _p, _p, _i, _p, _p, _i, _i]
+                                    _p, _p, _i, _p, _p, _i, _f, _i]
         L.orc_acoustic3d.restype = _i
         L.orc_acoustic3d.argtypes = [_i, _i, _i, _p, _p, _f, _f, _i, _i, _i, _f, _i, _p, _p, _p, _p, _i, _p,
                                      _p, _p, _p, _p, _p, _p, _p, _i]
@@ -53,6 +53,8 @@ def lib(precision: str = "f32") -> C.CDLL:
                                     _p, _p, _p, _p, _p, _p, _i]
         L.orc_elastic_params3d.restype = None
         L.orc_elastic_params3d.argtypes = [_p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _p]
+        L.orc_sponge_damp2d.restype = None
+        L.orc_sponge_damp2d.argtypes = [_i, _i, _i, _f, _p]
         L.orc_fd_coeffs.restype = _i
         L.orc_fd_coeffs.argtypes = [_i, _p]
         L.orc_habc_coeffs.restype = None
@@ -239,7 +241,7 @@ def ricker(f0: float, dt: float, nt: int) -> np.ndarray:
 # full runs
 
 def acoustic2d(vp, rho, dh, dt, nt, *, sx, sz, rx, rz, wavelet, nbc=50, fd_order=8, v_ref=None,
-               snap_interval=0, precision="f32", nthreads=0, final_state=False, habc_width=0):
+               snap_interval=0, precision="f32", nthreads=0, final_state=False, habc_width=0, sponge_factor=0.0):
     """acoustic2d (HABC, det path).  wavelet: (n_src, nt).  Returns dict of outputs."""
     vp, rho = _f32F(vp), _f32F(rho)
     nx, nz = vp.shape
@@ -257,7 +259,7 @@ def acoustic2d(vp, rho, dh, dt, nt, *, sx, sz, rx, rz, wavelet, nbc=50, fd_order
     rc = lib(precision).orc_acoustic2d(nx, nz, vp.ctypes.data, rho.ctypes.data, dh, dt, nt, nbc, fd_order, v_ref,
                                        len(sx), sx.ctypes.data, sz.ctypes.data, w.ctypes.data, nrec, rx.ctypes.data,
                                        rz.ctypes.data, sp.ctypes.data, svx.ctypes.data, svz.ctypes.data,
-                                       snap_interval, _ptr(snaps), _ptr(fs), habc_width, nthreads)
+                                       snap_interval, _ptr(snaps), _ptr(fs), habc_width, sponge_factor, nthreads)
     if rc != 0:
         raise ValueError(f"oracle acoustic2d failed ({rc})")
     out = dict(seis_p=sp, seis_vx=svx, seis_vz=svz,
@@ -270,7 +272,8 @@ def acoustic2d(vp, rho, dh, dt, nt, *, sx, sz, rx, rz, wavelet, nbc=50, fd_order
 
 
 def elastic2d(vp, vs, rho, dh, dt, nt, *, sx, sz, rx, rz, wavelet, nbc=50, fd_order=8, v_ref=None,
-              source_type=0, snap_interval=0, precision="f32", nthreads=0, final_state=False, habc_width=0):
+              source_type=0, snap_interval=0, precision="f32", nthreads=0, final_state=False, habc_width=0,
+              sponge_factor=0.0):
     vp, vs, rho = _f32F(vp), _f32F(vs), _f32F(rho)
     nx, nz = vp.shape
     if v_ref is None:
@@ -287,7 +290,8 @@ def elastic2d(vp, vs, rho, dh, dt, nt, *, sx, sz, rx, rz, wavelet, nbc=50, fd_or
     rc = lib(precision).orc_elastic2d(nx, nz, vp.ctypes.data, vs.ctypes.data, rho.ctypes.data, dh, dt, nt, nbc,
                                       fd_order, v_ref, source_type, len(sx), sx.ctypes.data, sz.ctypes.data,
                                       w.ctypes.data, nrec, rx.ctypes.data, rz.ctypes.data, svx.ctypes.data,
-                                      svz.ctypes.data, snap_interval, _ptr(snaps), _ptr(fs), habc_width, nthreads)
+                                      svz.ctypes.data, snap_interval, _ptr(snaps), _ptr(fs), habc_width, sponge_factor,
+                                      nthreads)
     if rc != 0:
         raise ValueError(f"oracle elastic2d failed ({rc})")
     out = dict(seis_vx=svx, seis_vz=svz, snaps=[s.T for s in snaps] if snaps is not None else [])
@@ -363,3 +367,10 @@ def elastic_params3d(vp, vs, rho, precision="f32"):
     lib(precision).orc_elastic_params3d(vp.ctypes.data, vs.ctypes.data, rho.ctypes.data, *vp.shape,
                                         *(o.ctypes.data for o in outs))
     return outs
+
+
+def sponge_damp2d(nx: int, nz: int, nbc: int, factor: float = 0.015, precision: str = "f32") -> np.ndarray:
+    """Cerjan damping profile on the padded (nx, nz) grid (src/boundary/sponge.jl:30-50)."""
+    out = np.zeros((nx, nz), _dtype(precision), order="F")
+    lib(precision).orc_sponge_damp2d(nx, nz, nbc, factor, out.ctypes.data)
+    return out

@@ -233,3 +233,65 @@ def test_cfg2_elastic_full_grid_prefix_bitwise(gpu, orc):
     assert np.array_equal(full.seis_vz[:, :nt], res.seis_vz)
     peak = np.abs(full.seis_vz).max()
     assert np.abs(full.seis_vz[:, -200:]).max() < peak
+
+
+# ---------------------------------------------------------------------------------------------
+# boundary = :sponge (src/boundary/sponge.jl, acoustic2d.jl:213-228, elastic2d.jl:180-195)
+
+@pytest.mark.parametrize("fd_order", [4, 8])
+def test_sponge_acoustic2d_bitwise_vs_oracle(gpu, orc, fd_order):
+    import fomo
+    nx, nz, nt, nbc = 97, 71, 260, 12
+    vp, _, rho = small_model(nx, nz)
+    sx, sz, rx, rz = geometry(nx, nz)
+    w = fomo.ricker_wavelet(15.0, 1e-3, nt)
+    res = fomo.acoustic2d(vp, rho, 10.0, 1e-3, nt, 15.0, sx=sx, sz=sz, rx=rx, rz=rz, nbc=nbc, fd_order=fd_order,
+                          boundary="sponge", snap_interval=100)
+    ref = orc.acoustic2d(vp, rho, 10.0, 1e-3, nt, sx=sx, sz=sz, rx=rx, rz=rz, wavelet=np.stack([w, w]), nbc=nbc,
+                         fd_order=fd_order, sponge_factor=0.015, snap_interval=100)
+    for k in ("seis_p", "seis_vx", "seis_vz"):
+        assert_parity(getattr(res, k), ref[k])
+    assert np.array_equal(np.stack(res.snaps), ref["snaps"])
+    # a different factor is honoured (and changes the result)
+    res2 = fomo.acoustic2d(vp, rho, 10.0, 1e-3, nt, 15.0, sx=sx, sz=sz, rx=rx, rz=rz, nbc=nbc, fd_order=fd_order,
+                           boundary="sponge", sponge_factor=0.025)
+    ref2 = orc.acoustic2d(vp, rho, 10.0, 1e-3, nt, sx=sx, sz=sz, rx=rx, rz=rz, wavelet=np.stack([w, w]), nbc=nbc,
+                          fd_order=fd_order, sponge_factor=0.025)
+    assert_parity(res2.seis_p, ref2["seis_p"])
+    assert not np.array_equal(res2.seis_p, res.seis_p)
+
+
+@pytest.mark.parametrize("source_type", ["explosive", "fz"])
+def test_sponge_elastic2d_bitwise_vs_oracle(gpu, orc, source_type):
+    import fomo
+    nx, nz, nt, nbc = 89, 77, 240, 10
+    vp, vs, rho = small_model(nx, nz, seed=5)
+    sx, sz, rx, rz = geometry(nx, nz)
+    w = fomo.ricker_wavelet(20.0, 5e-4, nt)
+    res = fomo.elastic2d(vp, vs, rho, 5.0, 5e-4, nt, 20.0, sx=sx, sz=sz, rx=rx, rz=rz, nbc=nbc, fd_order=8,
+                         source_type=source_type, boundary="sponge")
+    ref = orc.elastic2d(vp, vs, rho, 5.0, 5e-4, nt, sx=sx, sz=sz, rx=rx, rz=rz, wavelet=np.stack([w, w]), nbc=nbc,
+                        fd_order=8, source_type=1 if source_type == "fz" else 0, sponge_factor=0.015)
+    for k in ("seis_vx", "seis_vz"):
+        assert_parity(getattr(res, k), ref[k])
+
+
+def test_sponge_mirror_symmetry_and_calibration_gpu(gpu):
+    """verify_patch.jl:57-73 with boundary=:sponge (causally shielded mirror geometry, < 1e-5) and the
+    reference's calibrated sponge residual 2.87e-2 (verify_patch.jl:9-11) on the GPU path."""
+    import fomo
+    from fomo.learned import edge_extend
+    from test_oracle_pins import make_model, rel_asym_vx, rel_sym
+    vp, vs, rho = make_model(601, 201)
+    r = fomo.acoustic2d(vp, rho, 10.0, 1e-3, 1200, 10.0, sx=[301], sz=[41], rx=np.arange(150, 453),
+                        rz=np.full(303, 41), nbc=30, fd_order=4, boundary="sponge")
+    assert rel_sym(r.seis_p) < 1e-5 and rel_sym(r.seis_vz) < 1e-5 and rel_asym_vx(r.seis_vx) < 1e-5
+    nxr, nzr, ntr, L = 301, 201, 1500, 270
+    vp2, _, rho2 = make_model(nxr, nzr)
+    rx2 = np.arange(1, nxr + 1)
+    h = fomo.acoustic2d(vp2, rho2, 10.0, 1e-3, ntr, 10.0, sx=[151], sz=[41], rx=rx2, rz=np.full(nxr, 41), nbc=20,
+                        fd_order=4, boundary="sponge")
+    ref = fomo.acoustic2d(edge_extend(vp2, L), edge_extend(rho2, L), 10.0, 1e-3, ntr, 10.0, sx=[151 + L],
+                          sz=[41 + L], rx=rx2 + L, rz=np.full(nxr, 41 + L), nbc=20, fd_order=4, boundary="sponge")
+    resid = np.linalg.norm(h.seis_p - ref.seis_p) / np.linalg.norm(ref.seis_p)
+    assert abs(resid - 2.87e-2) < 0.005e-2, resid

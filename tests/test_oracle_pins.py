@@ -241,6 +241,43 @@ def test_habc_residual_matches_reference_calibration(orc, precision):
     assert abs(resid2 - 1.245e-2) > 0.05e-2
 
 
+def test_sponge_profile_closed_form(orc):
+    """src/boundary/sponge.jl:30-50: damp = exp(-(0.015 (nbc - d))^2) for d < nbc (d the distance to the
+    padded edge), 1 elsewhere; FP64 closed form at every distance."""
+    nx, nz, nbc = 40, 33, 12
+    d = orc.sponge_damp2d(nx, nz, nbc, 0.015, "f64")
+    for (i, j) in ((0, 0), (5, 20), (11, 11), (12, 20), (20, 16), (39, 32), (20, 2)):
+        dist = min(i, nx - 1 - i, j, nz - 1 - j)
+        r = np.float32(0.015) * np.float32(nbc - dist)
+        want = math.exp(-float(r * r)) if dist < nbc else 1.0
+        assert d[i, j] == pytest.approx(want, rel=1e-7), (i, j)
+    assert d.min() == pytest.approx(math.exp(-(0.015 * 12) ** 2), rel=1e-6) and d.max() == 1.0
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_sponge_residual_matches_reference_calibration(orc, precision):
+    """test/verify_patch.jl:9-11 records the CPU-f64 calibration of the same absorption experiment with
+    a Cerjan sponge of the same width: 2.87e-2 (and the experiment's boundary-free reference domain is
+    itself sponge-bounded, verify_patch.jl:84-87, which leaves the HABC residual at 1.245e-2)."""
+    nxr, nzr, ntr, L = 301, 201, 1500, 270
+    vp2, _, rho2 = make_model(nxr, nzr)
+    w = orc.ricker(10.0, 1e-3, ntr)[None, :]
+    rx2 = np.arange(1, nxr + 1)
+    kw = dict(wavelet=w, nbc=20, fd_order=4, precision=precision)
+    ref = orc.acoustic2d(edge_extend(vp2, L), edge_extend(rho2, L), 10.0, 1e-3, ntr, sx=[151 + L], sz=[41 + L],
+                         rx=rx2 + L, rz=np.full(nxr, 41 + L), sponge_factor=0.015, **kw)["seis_p"]
+    h = orc.acoustic2d(vp2, rho2, 10.0, 1e-3, ntr, sx=[151], sz=[41], rx=rx2, rz=np.full(nxr, 41),
+                       sponge_factor=0.015, **kw)["seis_p"]
+    resid = np.linalg.norm(h - ref) / np.linalg.norm(ref)
+    assert abs(resid - 2.87e-2) < 0.005e-2, resid
+    hh = orc.acoustic2d(vp2, rho2, 10.0, 1e-3, ntr, sx=[151], sz=[41], rx=rx2, rz=np.full(nxr, 41), **kw)["seis_p"]
+    assert abs(np.linalg.norm(hh - ref) / np.linalg.norm(ref) - 1.245e-2) < 0.0006e-2
+    # planted mistake: a stronger damping factor moves the residual well away
+    h2 = orc.acoustic2d(vp2, rho2, 10.0, 1e-3, ntr, sx=[151], sz=[41], rx=rx2, rz=np.full(nxr, 41),
+                        sponge_factor=0.02, **kw)["seis_p"]
+    assert abs(np.linalg.norm(h2 - ref) / np.linalg.norm(ref) - 2.87e-2) > 0.1e-2
+
+
 @pytest.mark.slow
 def test_reference_report_late_energy(orc):
     """example/benchmark/output/report.txt: acoustic2d 401x401 homogeneous, nbc=50, fd8, nt=1500,
