@@ -70,9 +70,10 @@ def _desc(eq, nx, nz, nbc, fd_order, nt, dh, dt, v_ref, n_src, n_rec, snap_inter
     return d
 
 
-def _common_checks(boundary, nbc, fd_order, nt, snap_interval):
-    if _sym(boundary) not in ("habc", "sponge"):
-        raise ArgumentError(f"boundary must be :habc or :sponge, got {boundary}")
+def _common_checks(boundary, nbc, fd_order, nt, snap_interval, learned_ok=False):
+    allowed = ("habc", "sponge", "learned") if learned_ok else ("habc", "sponge")
+    if _sym(boundary) not in allowed:
+        raise ArgumentError(f"boundary must be one of {', '.join(':' + a for a in allowed)}, got {boundary}")
     if nbc < 0 or nt < 0 or snap_interval < 0:
         raise ArgumentError("nbc, nt and snap_interval must be >= 0")
     if fd_order not in (2, 4, 6, 8, 10):
@@ -81,12 +82,14 @@ def _common_checks(boundary, nbc, fd_order, nt, snap_interval):
 
 def acoustic2d(vp, rho, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_order=8, snap_interval=0,
                boundary="habc", v_ref=None, sponge_factor=0.015, wavelet=None, scale_source_by_dt=False,
-               use_cuda_graph=True, verbose=True):
+               use_cuda_graph=True, verbose=True, learned_weights=None):
     """acoustic2d(vp, rho, dh, dt, nt, f0; sx, sz, rx, rz, ...) -> (seis_p, seis_vx, seis_vz, snaps, stats).
 
-    boundary = :habc (deterministic two-pass Higdon) or :sponge (Cerjan, `sponge_factor`)."""
+    boundary = :habc (deterministic two-pass Higdon), :sponge (Cerjan, `sponge_factor`) or :learned
+    (an extension: the Higdon frame with MLP-corrected blend weights, fomo.learned, trained for thin
+    layers such as nbc = 16; `learned_weights` overrides the shipped network)."""
     del verbose
-    _common_checks(boundary, nbc, fd_order, nt, snap_interval)
+    _common_checks(boundary, nbc, fd_order, nt, snap_interval, learned_ok=True)
     vp, rho = _f32F(vp), _f32F(rho)
     if vp.ndim != 2 or vp.shape != rho.shape:
         raise ArgumentError("vp and rho must be matrices of the same size")
@@ -100,6 +103,9 @@ def acoustic2d(vp, rho, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_order=8, s
         raise ArgumentError("source / receiver coordinate vectors differ in length")
     wm = wavelet_matrix(wavelet, f0, dt, nt, len(sx), scale_source_by_dt)
     n_rec = len(rx)
+    if _sym(boundary) == "learned":
+        return _acoustic2d_learned(vp, rho, dh, dt, nt, sx, sz, rx, rz, wm, nbc, fd_order, v_ref, snap_interval,
+                                   use_cuda_graph, learned_weights)
     d = _desc(_lib.ACOUSTIC2D, nx, nz, nbc, fd_order, nt, dh, dt, v_ref, len(sx), n_rec, snap_interval,
               use_cuda_graph, boundary=boundary, sponge_factor=sponge_factor)
     sp, svx, svz = (np.zeros((n_rec, nt), np.float32, order="F") for _ in range(3))
@@ -112,6 +118,34 @@ def acoustic2d(vp, rho, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_order=8, s
                                           None if snaps is None else snaps.ctypes.data, C.byref(t)))
     snap_list = [s.T for s in snaps] if snaps is not None else []
     return Acoustic2DResult(sp, svx, svz, snap_list, Stats(float(t.value)))
+
+
+def _acoustic2d_learned(vp, rho, dh, dt, nt, sx, sz, rx, rz, wm, nbc, fd_order, v_ref, snap_interval, use_graph,
+                        weights):
+    """boundary = :learned through a device-resident plan: enable the network, run the loop (timed
+    with hipEvents), copy snapshots of the interior of p between segments, read the seismograms."""
+    from . import learned
+    w = learned.load_weights() if weights is None else np.asarray(weights, np.float32)
+    p = Plan(_lib.ACOUSTIC2D, vp, rho, dh, dt, nt, sx=sx, sz=sz, rx=rx, rz=rz, wavelet=wm, nbc=nbc,
+             fd_order=fd_order, v_ref=v_ref, use_graph=use_graph)
+    try:
+        learned.enable(p, w)
+        snaps = []
+        ms = 0.0
+        if snap_interval > 0:
+            pad = p.pad
+            for _ in range(nt // snap_interval):
+                ms += p.timed_advance(snap_interval)
+                snaps.append(np.array(p.field(0)[pad:-pad, pad:-pad]))
+            rest = nt - (nt // snap_interval) * snap_interval
+            if rest:
+                ms += p.timed_advance(rest)
+        else:
+            ms = p.timed_advance(nt)
+        sp, svx, svz = p.seis(3)
+    finally:
+        p.close()
+    return Acoustic2DResult(sp, svx, svz, snaps, Stats(ms * 1e-3))
 
 
 def elastic2d(vp, vs, rho, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_order=8, snap_interval=0,

@@ -82,3 +82,29 @@ def test_zero_correction_is_plain_habc_bitwise(gpu):
         p.close()
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b)
+
+
+def test_shipped_network_reduces_reflection(gpu):
+    """The shipped weights, in the reference's absorption experiment (test/verify_patch.jl:76-91, its
+    model, source and receivers; FD8): at nbc = 16 the residual against the boundary-free reference
+    is <= -39.5 dB and at least 2 dB below the plain HABC of the same width (-37.6 dB); the API path
+    acoustic2d(boundary="learned") gives the same traces as the plan it is built on."""
+    import fomo
+    from fomo import learned
+    vp, rho, sx, sz, rx, rz = learned.patch_geometry()
+    w = learned.load_weights()
+    res = learned.reflection_residual(vp, rho, sx=sx, sz=sz, rx=rx, rz=rz, nt=1500, L=270,
+                                      boundaries=[("habc16", 16, None), ("learned16", 16, w)], judge="none")
+    print({k: round(v["reflection_db"], 2) for k, v in res.items()})
+    assert res["learned16"]["reflection_db"] <= -39.5
+    assert res["learned16"]["reflection_db"] <= res["habc16"]["reflection_db"] - 2.0
+    r = fomo.acoustic2d(vp, rho, 10.0, 1e-3, 300, 10.0, sx=[sx], sz=[sz], rx=rx, rz=rz, nbc=16, boundary="learned")
+    from fomo.api import Plan
+    import fomo._lib as L
+    p = Plan(L.ACOUSTIC2D, vp, rho, 10.0, 1e-3, 300, sx=[sx], sz=[sz], rx=rx, rz=rz,
+             wavelet=fomo.ricker_wavelet(10.0, 1e-3, 300)[None], nbc=16)
+    learned.enable(p, w)
+    p.advance(300)
+    seis = p.seis(3)
+    p.close()
+    assert np.array_equal(r.seis_p, seis[0]) and np.array_equal(r.seis_vz, seis[2])
