@@ -103,6 +103,45 @@ def probe(args):
     print(json.dumps({"probe": f"{n}x{n} interior, fd8, nbc 50", **res}), flush=True)
 
 
+def learned_bench(args):
+    """cfg5: acoustic2d 2400x1200, dh 10 m, dt 0.8 ms, FD8, seeded layered model; per-step time of
+    plain HABC at nbc 100 and of the learned boundary at nbc 16 (MLP inference included), and the
+    reflection of both in the reference's absorption experiment (test/verify_patch.jl:76-91)."""
+    import numpy as np
+
+    import fomo
+    import fomo._lib as L
+    from fomo import learned
+    from fomo.api import Plan
+    from fomo.models import layered_2d
+    nx, nz, dt, nt = 2400, 1200, 8e-4, args.steps
+    vp, _, rho = layered_2d(nx, nz, seed=SEED, vacuum_top=False)
+    w = fomo.ricker_wavelet(15.0, dt, max(nt, 2))
+    rx = np.arange(1, nx + 1, 2)
+    weights = learned.load_weights()
+    res = {}
+    for name, nbc, use in (("habc_nbc100", 100, False), ("learned_nbc16", 16, True), ("habc_nbc16", 16, False)):
+        p = Plan(L.ACOUSTIC2D, vp, rho, 10.0, dt, nt, sx=[nx // 2], sz=[50], rx=rx, rz=np.full(len(rx), 10),
+                 wavelet=w[None, :nt], nbc=nbc, fd_order=8)
+        if use:
+            learned.enable(p, weights)
+        p.advance(min(20, nt))
+        p.reset()
+        ms = p.timed_advance(nt)
+        res[name] = {"padded_cells": p.padded_cells, "ms_per_step": ms / nt}
+        p.close()
+        log(f"{name}: {res[name]}")
+    res["learned_speedup_vs_habc_nbc100"] = res["habc_nbc100"]["ms_per_step"] / res["learned_nbc16"]["ms_per_step"]
+    vp2, rho2, sx, sz, rx2, rz2 = learned.patch_geometry()
+    refl = learned.reflection_residual(vp2, rho2, sx=sx, sz=sz, rx=rx2, rz=rz2, nt=1500, L=270, judge="none",
+                                       boundaries=[("habc_nbc100", 100, None), ("learned_nbc16", 16, weights),
+                                                   ("habc_nbc16", 16, None)])
+    for k, v in refl.items():
+        res[k]["reflection_db"] = v["reflection_db"]
+    print(json.dumps({"learned": f"acoustic2d {nx}x{nz}, fd8, dt 0.8 ms, {nt} steps; reflection from the "
+                                 "reference's absorption experiment (301x201, 1500 steps)", **res}), flush=True)
+
+
 def stream_copy_gbs(nbytes=2 << 30, iters=20):
     """Practical HBM ceiling: float4 streaming copy of `nbytes` (read + write counted)."""
     import ctypes as C
@@ -167,6 +206,7 @@ def main():
     ap.add_argument("--probe-n3", type=int, default=512)
     ap.add_argument("--probe-storage", default="f32,f16,bf16", help="storages of the 3D probe")
     ap.add_argument("--stream-copy", action="store_true", help="only the HBM streaming-copy calibration")
+    ap.add_argument("--learned", action="store_true", help="learned nbc-16 boundary vs plain HABC nbc 100 (cfg5)")
     ap.add_argument("--probe-tile", type=int, default=0, help="3D z-segment choice (0 default, k: 16<<(k-1) planes)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -179,6 +219,9 @@ def main():
         return
     if args.probe3d:
         probe3d(args)
+        return
+    if args.learned:
+        learned_bench(args)
         return
     if args.stream_copy:
         gbs, ms = stream_copy_gbs(1 << 30, 4)

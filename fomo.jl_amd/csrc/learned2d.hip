@@ -1,12 +1,14 @@
 // learned2d.hip -- per-frame-cell MLP for the learned absorbing boundary: inference, training
 // (forward + backward with deterministic gradient reduction) and a unit entry for gradient checks.
 //
-// Inference writes the two blend-weight corrections of every frame cell; the 2D frame tiles then
+// Inference (matrix cores, BF16 operands) writes the two blend-weight corrections of every frame cell; the 2D frame tiles then
 // blend f = a*f_pde + (1-a)*S with a = clamp(w(dist) + corr, 0, 1).  Training compares the
 // student's new state at frame cells with a wide-boundary teacher run at the same physical cells
 // (one-step loss L = sum ((f - f_teacher)/scale)^2), so that dL/dcorr = 2 (f - f_teacher)/scale^2
 // * (f_pde - S) (inside the clamp), with f_pde - S written by the sweep.
 #include <hip/hip_bf16.h>
+
+#include <algorithm>
 
 #include "learned2d.h"
 
@@ -105,17 +107,163 @@ __device__ __forceinline__ void mlp_bwd(const float* W, const float* h1, const f
   }
 }
 
-__global__ __launch_bounds__(256) void mlp_forward_kernel(const MlpArgs A) {
-  __shared__ float W[kMlpParams];
-  load_weights(A.w_bf16, W);
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= A.n_cells) return;
-  const int c = A.cells[t];
-  float x[kMlpIn], h1[kMlpH], h2[kMlpH], z[kMlpOut];
-  gather_features(A.p, A.vx, A.vz, A.nx, A.nz, A.H, c, x);
-  mlp_fwd(W, x, h1, h2, z);
-  A.alpha_v[c] = corrf(z[0]);
-  A.alpha_p[c] = corrf(z[1]);
+// Inference on the matrix cores.  A wave takes 64 frame cells per pass: every lane gathers the 76
+// features of one cell (FP32, scaled as above), rounds them to BF16 into LDS, and the three layers
+// run as 32-cell x 32-unit tiles of v_mfma_f32_32x32x16_bf16 (K = 80 for layer 1, 76 features and
+// 4 zeros; K = 32 for layers 2 and 3; layer 3 has 2 live output columns).  Weights stay in
+// registers as BF16 B-fragments for the whole kernel; accumulation is FP32; the hidden activations
+// are rounded to BF16 between layers.  Fragment layouts (32x32x16): A lane l holds row l%32,
+// k = 8*(l/32) + 0..7 of a 16-wide k block; B lane l holds column l%32, the same k; D register r
+// of lane l holds row 8*(r/4) + 4*(l/32) + r%4, column l%32.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+constexpr int kK1 = 80;                 // layer-1 k padded to 5 blocks of 16
+constexpr int kInferWaves = 4;
+constexpr int kInferCells = 64;         // cells per wave pass
+
+// Features for inference: as gather_features, with the patch scale applied as a multiply by its
+// reciprocal (within one FP32 ulp of the division, far below the BF16 rounding that follows) and
+// 32-bit offsets from the scalar field bases.
+__device__ __forceinline__ void infer_features(const float* p, const float* vx, const float* vz, int nx, int nz,
+                                               int H, int c, float* x) {
+  const int i = c % nx, j = c / nx;
+  float mp = 0.0f, mv = 0.0f;
+#pragma unroll
+  for (int dy = -2; dy <= 2; ++dy) {
+    const int jj = min(max(j + dy, 0), nz - 1);
+#pragma unroll
+    for (int dx = -2; dx <= 2; ++dx) {
+      const int ii = min(max(i + dx, 0), nx - 1);
+      const int q = (dy + 2) * 5 + (dx + 2);
+      const unsigned g = (unsigned)(ii + jj * nx);
+      const float a = G(p)[g], b = G(vx)[g], d = G(vz)[g];
+      x[q] = a;
+      x[25 + q] = b;
+      x[50 + q] = d;
+      mp = fmaxf(mp, fabsf(a));
+      mv = fmaxf(mv, fmaxf(fabsf(b), fabsf(d)));
+    }
+  }
+  const float rp = 1.0f / (mp + 1e-30f), rv = 1.0f / (mv + 1e-30f);
+#pragma unroll
+  for (int q = 0; q < 25; ++q) x[q] *= rp;
+#pragma unroll
+  for (int q = 25; q < 75; ++q) x[q] *= rv;
+  const int dist = min(min(i, nx - 1 - i), min(j, nz - 1 - j));
+  x[75] = (float)dist / (float)H;
+}
+
+__device__ __forceinline__ __bf16 wbf(const unsigned short* w, int e) {
+  const unsigned short u = G(w)[e];
+  return __builtin_bit_cast(__bf16, u);
+}
+
+__global__ __launch_bounds__(64 * kInferWaves) void mlp_infer_kernel(const MlpArgs A) {
+  __shared__ __bf16 feats[kInferWaves][kInferCells * kK1];
+  __shared__ __bf16 hbuf[kInferWaves][kInferCells * kMlpH];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int col = lane & 31, half = lane >> 5;
+  const unsigned short* w = (const unsigned short*)A.w_bf16;
+  bf16x8 b1[5], b2[2], b3[2];
+#pragma unroll
+  for (int kb = 0; kb < 5; ++kb)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int k = kb * 16 + half * 8 + j;
+      b1[kb][j] = k < kMlpIn ? wbf(w, kOffW1 + col * kMlpIn + k) : (__bf16)0.0f;
+    }
+#pragma unroll
+  for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int k = kb * 16 + half * 8 + j;
+      b2[kb][j] = wbf(w, kOffW2 + col * kMlpH + k);
+      b3[kb][j] = col < kMlpOut ? wbf(w, kOffW3 + col * kMlpH + k) : (__bf16)0.0f;
+    }
+  const float bias1 = (float)wbf(w, kOffB1 + col), bias2 = (float)wbf(w, kOffB2 + col);
+  const float bias3 = col < kMlpOut ? (float)wbf(w, kOffB3 + col) : 0.0f;
+  __bf16* F = feats[wv];
+  __bf16* Hb = hbuf[wv];
+  const int per_block = kInferWaves * kInferCells;
+  // every wave of a block runs the same number of passes, so the block barriers are uniform
+  for (int g0 = blockIdx.x * per_block; g0 < A.n_cells; g0 += gridDim.x * per_block) {
+    const int g = g0 + wv * kInferCells;
+    {
+      const int e = g + lane;
+      float x[kK1];
+      infer_features(A.p, A.vx, A.vz, A.nx, A.nz, A.H, G(A.cells)[e < A.n_cells ? e : 0], x);
+#pragma unroll
+      for (int k = kMlpIn; k < kK1; ++k) x[k] = 0.0f;
+#pragma unroll
+      for (int k = 0; k < kK1; k += 8) {
+        bf16x8 v;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = (__bf16)x[k + u];
+        *(bf16x8*)&F[lane * kK1 + k] = v;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int grp = 0; grp < 2; ++grp) {
+      const int rb = grp * 32;  // first cell of the 32-row tile
+      f32x16 acc;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+#pragma unroll
+      for (int kb = 0; kb < 5; ++kb) {
+        const bf16x8 a = *(const bf16x8*)&F[(rb + col) * kK1 + kb * 16 + half * 8];
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1[kb], acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = 8 * (r / 4) + 4 * half + (r % 4);
+        Hb[(rb + row) * kMlpH + col] = (__bf16)fmaxf(acc[r] + bias1, 0.0f);
+      }
+    }
+    __syncthreads();
+    f32x16 h2[2];
+#pragma unroll
+    for (int grp = 0; grp < 2; ++grp) {
+      const int rb = grp * 32;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) h2[grp][r] = 0.0f;
+#pragma unroll
+      for (int kb = 0; kb < 2; ++kb) {
+        const bf16x8 a = *(const bf16x8*)&Hb[(rb + col) * kMlpH + kb * 16 + half * 8];
+        h2[grp] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b2[kb], h2[grp], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int grp = 0; grp < 2; ++grp)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = 8 * (r / 4) + 4 * half + (r % 4);
+        Hb[(grp * 32 + row) * kMlpH + col] = (__bf16)fmaxf(h2[grp][r] + bias2, 0.0f);
+      }
+    __syncthreads();
+#pragma unroll
+    for (int grp = 0; grp < 2; ++grp) {
+      const int rb = grp * 32;
+      f32x16 z;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) z[r] = 0.0f;
+#pragma unroll
+      for (int kb = 0; kb < 2; ++kb) {
+        const bf16x8 a = *(const bf16x8*)&Hb[(rb + col) * kMlpH + kb * 16 + half * 8];
+        z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b3[kb], z, 0, 0, 0);
+      }
+      if (col < kMlpOut) {
+        float* out = col == 0 ? A.alpha_v : A.alpha_p;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int e = g + rb + 8 * (r / 4) + 4 * half + (r % 4);
+          if (e < A.n_cells) G(out)[G(A.cells)[e]] = corrf(z[r] + bias3);
+        }
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // Per-chunk LDS layout for the weight-gradient GEMMs (32 cells per chunk, 64 threads).
@@ -251,7 +399,9 @@ __global__ void reduce_partials_kernel(const float* partial, int blocks, int wid
 
 hipError_t launch_mlp_forward(const MlpArgs& a, hipStream_t s) {
   if (a.n_cells <= 0) return hipSuccess;
-  hipLaunchKernelGGL(mlp_forward_kernel, dim3((a.n_cells + 255) / 256), dim3(256), 0, s, a);
+  const int per_block = kInferWaves * kInferCells;
+  const int blocks = std::min(4096, (a.n_cells + per_block - 1) / per_block);
+  hipLaunchKernelGGL(mlp_infer_kernel, dim3(blocks), dim3(64 * kInferWaves), 0, s, a);
   return hipGetLastError();
 }
 

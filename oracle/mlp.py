@@ -71,6 +71,18 @@ def forward(w, x):
     return corr, (x, h1, h2, corr)
 
 
+def forward_bf16(w, x):
+    """The inference path of the sweeps (matrix cores): features and the hidden activations are
+    rounded to BFloat16 before each layer, every layer accumulates in floating point (here FP64) from
+    BFloat16 weights; returns the corrections (n x 2)."""
+    m = unpack(w)
+    h = bf16_round(np.asarray(x, np.float64))
+    h1 = bf16_round(np.maximum(h @ m["W1"].T + m["b1"], 0.0))
+    h2 = bf16_round(np.maximum(h1 @ m["W2"].T + m["b2"], 0.0))
+    z = h2 @ m["W3"].T + m["b3"]
+    return CORR_MAX * np.tanh(z)
+
+
 def backward(w, cache, galpha):
     """Summed weight gradients for dL/dcorr = galpha (n x 2)."""
     m = unpack(w)

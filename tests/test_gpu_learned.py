@@ -28,8 +28,8 @@ def test_mlp_kernels_match_oracle(gpu):
 
 
 def test_learned_inference_on_live_wavefield(gpu):
-    """Blend-weight corrections computed by the engine from the state at time n equal the oracle's
-    MLP on features gathered from the same state."""
+    """Blend-weight corrections computed by the engine (matrix-core inference) from the state at time n
+    equal the oracle's BF16 inference path on features gathered from the same state."""
     import mlp
     import fomo
     import fomo._lib as L
@@ -52,9 +52,15 @@ def test_learned_inference_on_live_wavefield(gpu):
     p.advance(1)
     cells, av, ap = alphas(p)
     x = mlp.features(state[0], state[1], state[2], cells, 19)
-    a_ref, _ = mlp.forward(mlp.bf16_round(weights), x)
-    assert np.allclose(av, a_ref[:, 0], rtol=1e-4, atol=1e-5)
-    assert np.allclose(ap, a_ref[:, 1], rtol=1e-4, atol=1e-5)
+    a_ref = mlp.forward_bf16(mlp.bf16_round(weights), x)
+    # BF16 activations: a hidden unit that lands on a rounding boundary may round the other way
+    # (one BF16 ulp, 2^-8 relative); bound the worst cell and require the typical one to be exact
+    # to FP32 accumulation order.
+    for got, want in ((av, a_ref[:, 0]), (ap, a_ref[:, 1])):
+        err = np.abs(got - want)
+        assert err.max() < 2e-3 and np.median(err) < 1e-5, (err.max(), np.median(err))
+    a32, _ = mlp.forward(mlp.bf16_round(weights), x)
+    assert np.abs(av - a32[:, 0]).max() < 5e-3  # BF16 inference stays close to the FP32 network
     p.close()
 
 
