@@ -120,9 +120,11 @@ def learned_bench(args):
     rx = np.arange(1, nx + 1, 2)
     weights = learned.load_weights()
     res = {}
-    for name, nbc, use in (("habc_nbc100", 100, False), ("learned_nbc16", 16, True), ("habc_nbc16", 16, False)):
+    rows = (("habc_nbc100", 100, False, "f32"), ("learned_nbc16", 16, True, "f32"), ("habc_nbc16", 16, False, "f32"),
+            ("learned_nbc16_f16", 16, True, "f16"), ("habc_nbc100_f16", 100, False, "f16"))
+    for name, nbc, use, storage in rows:
         p = Plan(L.ACOUSTIC2D, vp, rho, 10.0, dt, nt, sx=[nx // 2], sz=[50], rx=rx, rz=np.full(len(rx), 10),
-                 wavelet=w[None, :nt], nbc=nbc, fd_order=8)
+                 wavelet=w[None, :nt], nbc=nbc, fd_order=8, storage=storage)
         if use:
             learned.enable(p, weights)
         p.advance(min(20, nt))
@@ -131,13 +133,17 @@ def learned_bench(args):
         res[name] = {"padded_cells": p.padded_cells, "ms_per_step": ms / nt}
         p.close()
         log(f"{name}: {res[name]}")
-    res["learned_speedup_vs_habc_nbc100"] = res["habc_nbc100"]["ms_per_step"] / res["learned_nbc16"]["ms_per_step"]
+    for name in ("learned_nbc16", "learned_nbc16_f16"):
+        res[name + "_speedup_vs_habc_nbc100"] = res["habc_nbc100"]["ms_per_step"] / res[name]["ms_per_step"]
     vp2, rho2, sx, sz, rx2, rz2 = learned.patch_geometry()
-    refl = learned.reflection_residual(vp2, rho2, sx=sx, sz=sz, rx=rx2, rz=rz2, nt=1500, L=270, judge="none",
+    refl = learned.reflection_residual(vp2, rho2, sx=sx, sz=sz, rx=rx2, rz=rz2, nt=1500, L=270, judge="learned_nbc16",
                                        boundaries=[("habc_nbc100", 100, None), ("learned_nbc16", 16, weights),
-                                                   ("habc_nbc16", 16, None)])
+                                                   ("habc_nbc16", 16, None), ("learned_nbc16_f16", 16, weights, "f16"),
+                                                   ("habc_nbc100_f16", 100, None, "f16")])
     for k, v in refl.items():
         res[k]["reflection_db"] = v["reflection_db"]
+        if k == "learned_nbc16_f16":
+            res[k]["trace_rel_l2_vs_learned_nbc16"] = v["vs_learned_nbc16"]
     print(json.dumps({"learned": f"acoustic2d {nx}x{nz}, fd8, dt 0.8 ms, {nt} steps; reflection from the "
                                  "reference's absorption experiment (301x201, 1500 steps)", **res}), flush=True)
 

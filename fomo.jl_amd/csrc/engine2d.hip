@@ -21,16 +21,20 @@
 //   Higdon value ................... src/boundary/habc/kernels_det.jl:22-57
 //   injection / gather ............. src/acquisition/device_indexed.jl:28-134 (step index read
 //                                    from a device counter, as the reference's graph path does)
+#include <algorithm>
+
 #include "fomo_common.h"
+#include "storage.h"
 
 namespace fomo {
 
 // Resolved pointers of one phase of one shot.
 struct PV {
-  gp<const float> sten[3];
-  gp<const float> old[3];
-  gp<float> out[3];
-  gp<const float> mat[3];
+  const void* sten[3];
+  const void* old[3];
+  void* out[3];
+  const void* mat[3];
+  float sdec[3], odec[3], oenc[3], mdec[3];  // scales of the stored arrays above
   gp<const float> lut[3];
   gp<const float> alpha[3];  // learned corrections of the distance-table weight (NULL: table only)
   gp<float> sens[3];         // f_pde - S at frame cells while training (NULL: not recorded)
@@ -44,48 +48,56 @@ __device__ __forceinline__ void make_view(const Shot2D& S, int s, PV& V) {
     V.alpha[f] = nullptr;
     V.sens[f] = nullptr;
   }
+  auto sten = [&](int k, int buf, int f) {
+    V.sten[k] = S.fld[buf][f];
+    V.sdec[k] = S.fdec[f];
+  };
+  auto old_out = [&](int k, int f) {
+    V.old[k] = S.fld[s][f];
+    V.odec[k] = S.fdec[f];
+    V.out[k] = S.fld[o][f];
+    V.oenc[k] = S.fenc[f];
+  };
+  auto mat = [&](int k, int m) {
+    V.mat[k] = S.mat[m];
+    V.mdec[k] = S.mdec[m];
+  };
   const int L = S.H + 1;
   if (PH == kAcVel) {
-    V.sten[0] = G(S.fld[s][0]);
-    V.old[0] = G(S.fld[s][1]);
-    V.old[1] = G(S.fld[s][2]);
-    V.out[0] = G(S.fld[o][1]);
-    V.out[1] = G(S.fld[o][2]);
-    V.mat[0] = G(S.mat[1]);
-    V.mat[1] = G(S.mat[2]);
+    sten(0, s, 0);
+    old_out(0, 1);
+    old_out(1, 2);
+    mat(0, 1);
+    mat(1, 2);
     V.lut[0] = G(S.wlut);
     V.lut[1] = G(S.wlut + L);
     V.alpha[0] = V.alpha[1] = G((const float*)S.alpha[0]);
     V.sens[0] = G(S.sens[0]);
     V.sens[1] = G(S.sens[1]);
   } else if (PH == kAcPrs) {
-    V.sten[0] = G(S.fld[o][1]);
-    V.sten[1] = G(S.fld[o][2]);
-    V.old[0] = G(S.fld[s][0]);
-    V.out[0] = G(S.fld[o][0]);
-    V.mat[0] = G(S.mat[0]);
+    sten(0, o, 1);
+    sten(1, o, 2);
+    old_out(0, 0);
+    mat(0, 0);
     V.lut[0] = G(S.wlut + 3 * L);
     V.alpha[0] = G((const float*)S.alpha[1]);
     V.sens[0] = G(S.sens[2]);
   } else if (PH == kElVel) {
-    V.sten[0] = G(S.fld[s][2]);
-    V.sten[1] = G(S.fld[s][3]);
-    V.sten[2] = G(S.fld[s][4]);
-    V.old[0] = G(S.fld[s][0]);
-    V.old[1] = G(S.fld[s][1]);
-    V.out[0] = G(S.fld[o][0]);
-    V.out[1] = G(S.fld[o][1]);
-    V.mat[0] = G(S.mat[3]);
-    V.mat[1] = G(S.mat[4]);
+    sten(0, s, 2);
+    sten(1, s, 3);
+    sten(2, s, 4);
+    old_out(0, 0);
+    old_out(1, 1);
+    mat(0, 3);
+    mat(1, 4);
     V.lut[0] = G(S.wlut);
     V.lut[1] = G(S.wlut + L);
   } else {
-    V.sten[0] = G(S.fld[o][0]);
-    V.sten[1] = G(S.fld[o][1]);
+    sten(0, o, 0);
+    sten(1, o, 1);
     for (int f = 0; f < 3; ++f) {
-      V.old[f] = G(S.fld[s][2 + f]);
-      V.out[f] = G(S.fld[o][2 + f]);
-      V.mat[f] = G(S.mat[f]);
+      old_out(f, 2 + f);
+      mat(f, f);
       V.lut[f] = G(S.wlut + 3 * L);
     }
   }
@@ -103,18 +115,18 @@ struct TileRd {
 
 // Pre-correction values of all output fields of a phase at padded cell (i, j): the FD update
 // inside the update region N <= i < nx-N, N <= j < nz-N, the old value elsewhere.
-template <int PH, int N>
+template <int PH, int N, class T>
 struct Phys;
 
-template <int N>
-struct Phys<kAcVel, N> {
+template <int N, class T>
+struct Phys<kAcVel, N, T> {
   static constexpr int NF = 2, NS = 1;
   template <class R>
   __device__ __forceinline__ static void pre_all(const R& rd, const PV& V, const Shot2D& S, const float* a, int i,
                                                  int j, float* v) {
-    const size_t c = (size_t)i + (size_t)j * (size_t)S.nx;
-    v[0] = V.old[0][c];
-    v[1] = V.old[1][c];
+    const unsigned c = (unsigned)(i + j * S.nx);
+    v[0] = Cvt<T>::ld(V.old[0], c, V.odec[0]);
+    v[1] = Cvt<T>::ld(V.old[1], c, V.odec[1]);
     if (i < N || i >= S.nx - N || j < N || j >= S.nz - N) return;
     float dx = 0.0f, dz = 0.0f;
 #pragma unroll
@@ -123,19 +135,19 @@ struct Phys<kAcVel, N> {
       dx += cl * (rd(0, i + l - 1, j) - rd(0, i - l, j));
       dz += cl * (rd(0, i, j + l) - rd(0, i, j - l + 1));
     }
-    v[0] = v[0] + (V.mat[0][c] * S.dtx) * dx;
-    v[1] = v[1] + (V.mat[1][c] * S.dtx) * dz;
+    v[0] = v[0] + (Cvt<T>::ld(V.mat[0], c, V.mdec[0]) * S.dtx) * dx;
+    v[1] = v[1] + (Cvt<T>::ld(V.mat[1], c, V.mdec[1]) * S.dtx) * dz;
   }
 };
 
-template <int N>
-struct Phys<kAcPrs, N> {
+template <int N, class T>
+struct Phys<kAcPrs, N, T> {
   static constexpr int NF = 1, NS = 2;
   template <class R>
   __device__ __forceinline__ static void pre_all(const R& rd, const PV& V, const Shot2D& S, const float* a, int i,
                                                  int j, float* v) {
-    const size_t c = (size_t)i + (size_t)j * (size_t)S.nx;
-    v[0] = V.old[0][c];
+    const unsigned c = (unsigned)(i + j * S.nx);
+    v[0] = Cvt<T>::ld(V.old[0], c, V.odec[0]);
     if (i < N || i >= S.nx - N || j < N || j >= S.nz - N) return;
     float dvxdx = 0.0f, dvzdz = 0.0f;
 #pragma unroll
@@ -144,19 +156,19 @@ struct Phys<kAcPrs, N> {
       dvxdx += cl * (rd(0, i + l, j) - rd(0, i - l + 1, j));
       dvzdz += cl * (rd(1, i, j + l - 1) - rd(1, i, j - l));
     }
-    v[0] = v[0] + V.mat[0][c] * (dvxdx * S.dtx + dvzdz * S.dtx);
+    v[0] = v[0] + Cvt<T>::ld(V.mat[0], c, V.mdec[0]) * (dvxdx * S.dtx + dvzdz * S.dtx);
   }
 };
 
-template <int N>
-struct Phys<kElVel, N> {
+template <int N, class T>
+struct Phys<kElVel, N, T> {
   static constexpr int NF = 2, NS = 3;
   template <class R>
   __device__ __forceinline__ static void pre_all(const R& rd, const PV& V, const Shot2D& S, const float* a, int i,
                                                  int j, float* v) {
-    const size_t c = (size_t)i + (size_t)j * (size_t)S.nx;
-    v[0] = V.old[0][c];
-    v[1] = V.old[1][c];
+    const unsigned c = (unsigned)(i + j * S.nx);
+    v[0] = Cvt<T>::ld(V.old[0], c, V.odec[0]);
+    v[1] = Cvt<T>::ld(V.old[1], c, V.odec[1]);
     if (i < N || i >= S.nx - N || j < N || j >= S.nz - N) return;
     float dtxxdx = 0.0f, dtxzdz = 0.0f, dtxzdx = 0.0f, dtzzdz = 0.0f;
 #pragma unroll
@@ -168,21 +180,21 @@ struct Phys<kElVel, N> {
       dtzzdz += cl * (rd(1, i, j + l) - rd(1, i, j - l + 1));
     }
     const float dt = S.dtx;
-    v[0] = v[0] + V.mat[0][c] * (dt * dtxxdx + dt * dtxzdz);
-    v[1] = v[1] + V.mat[1][c] * (dt * dtxzdx + dt * dtzzdz);
+    v[0] = v[0] + Cvt<T>::ld(V.mat[0], c, V.mdec[0]) * (dt * dtxxdx + dt * dtxzdz);
+    v[1] = v[1] + Cvt<T>::ld(V.mat[1], c, V.mdec[1]) * (dt * dtxzdx + dt * dtzzdz);
   }
 };
 
-template <int N>
-struct Phys<kElStr, N> {
+template <int N, class T>
+struct Phys<kElStr, N, T> {
   static constexpr int NF = 3, NS = 2;
   template <class R>
   __device__ __forceinline__ static void pre_all(const R& rd, const PV& V, const Shot2D& S, const float* a, int i,
                                                  int j, float* v) {
-    const size_t c = (size_t)i + (size_t)j * (size_t)S.nx;
-    v[0] = V.old[0][c];
-    v[1] = V.old[1][c];
-    v[2] = V.old[2][c];
+    const unsigned c = (unsigned)(i + j * S.nx);
+    v[0] = Cvt<T>::ld(V.old[0], c, V.odec[0]);
+    v[1] = Cvt<T>::ld(V.old[1], c, V.odec[1]);
+    v[2] = Cvt<T>::ld(V.old[2], c, V.odec[2]);
     if (i < N || i >= S.nx - N || j < N || j >= S.nz - N) return;
     float dvxdx = 0.0f, dvzdz = 0.0f, dvxdz = 0.0f, dvzdx = 0.0f;
 #pragma unroll
@@ -194,10 +206,10 @@ struct Phys<kElStr, N> {
       dvzdx += cl * (rd(1, i + l - 1, j) - rd(1, i - l, j));
     }
     const float dt = S.dtx;
-    const float lv = V.mat[0][c], l2m = V.mat[1][c];
+    const float lv = Cvt<T>::ld(V.mat[0], c, V.mdec[0]), l2m = Cvt<T>::ld(V.mat[1], c, V.mdec[1]);
     v[0] = v[0] + ((l2m * dvxdx) * dt + (lv * dvzdz) * dt);
     v[1] = v[1] + ((lv * dvxdx) * dt + (l2m * dvzdz) * dt);
-    v[2] = v[2] + V.mat[2][c] * (dvxdz * dt + dvzdx * dt);
+    v[2] = v[2] + Cvt<T>::ld(V.mat[2], c, V.mdec[2]) * (dvxdz * dt + dvzdx * dt);
   }
 };
 
@@ -236,10 +248,10 @@ struct WinRd {
 // Frame tile: at most 64 x kFrameRows cells of the absorbing frame.  Stencil inputs are staged
 // with an (N+1) halo; pre-correction values of the tile plus a one-cell ring go to LDS; each
 // frame cell then blends with its inward neighbour's pre-correction value.
-template <int PH, int N>
+template <int PH, int N, class T>
 __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, const Shot2D& S, const PV& V,
                                            float* lds) {
-  using P = Phys<PH, N>;
+  using P = Phys<PH, N, T>;
   constexpr int NF = P::NF, NS = P::NS;
   constexpr int BX = kTileW, BZ = kFrameRows, ZPT = BZ / 4;
   constexpr int TW = BX + 2 * N + 2, TH = BZ + 2 * N + 2;
@@ -254,9 +266,9 @@ __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, 
     const int r = idx / TW, cc = idx - r * TW;
     const int gi = oi + cc, gj = oj + r;
     const bool in = gi >= 0 && gi < nx && gj >= 0 && gj < nz;
-    const size_t g = in ? (size_t)gi + (size_t)gj * (size_t)nx : 0;
+    const unsigned g = in ? (unsigned)(gi + gj * nx) : 0u;
 #pragma unroll
-    for (int k = 0; k < NS; ++k) tile[k * TW * TH + idx] = in ? V.sten[k][g] : 0.0f;
+    for (int k = 0; k < NS; ++k) tile[k * TW * TH + idx] = in ? Cvt<T>::ld(V.sten[k], g, V.sdec[k]) : 0.0f;
   }
   __syncthreads();
   const TileRd<TW, TH> rd{tile, oi, oj};
@@ -281,7 +293,7 @@ __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, 
     if (jl >= W.h) break;
     const int j = j0 + jl;
     const int li = (int)threadIdx.x + 1, lj = jl + 1;
-    const size_t c = (size_t)i + (size_t)j * (size_t)nx;
+    const unsigned c = (unsigned)(i + j * nx);
     float v[NF];
 #pragma unroll
     for (int f = 0; f < NF; ++f) v[f] = pre_s[f * EW * EH + lj * EW + li];
@@ -298,16 +310,17 @@ __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, 
       const int di = Lf ? 1 : -1, dj = Tf ? 1 : -1;
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
-        const gp<const float> fo = V.old[f];
-        const float foc = fo[c];
+        const void* fo = V.old[f];
+        const float od = V.odec[f];
+        const float foc = Cvt<T>::ld(fo, c, od);
         float sx = 0.0f, sz = 0.0f;
         if (inx) {
           const float fn = pre_s[f * EW * EH + lj * EW + li + di];
-          sx = (-q * fn - qt * foc) - qxt * fo[(size_t)(i + di) + (size_t)j * (size_t)nx];
+          sx = (-q * fn - qt * foc) - qxt * Cvt<T>::ld(fo, (unsigned)(i + di + j * nx), od);
         }
         if (inz) {
           const float fn = pre_s[f * EW * EH + (lj + dj) * EW + li];
-          sz = (-q * fn - qt * foc) - qxt * fo[(size_t)i + (size_t)(j + dj) * (size_t)nx];
+          sz = (-q * fn - qt * foc) - qxt * Cvt<T>::ld(fo, (unsigned)(i + (j + dj) * nx), od);
         }
         const float wt = V.alpha[f] ? fminf(fmaxf(V.lut[f][dist] + V.alpha[f][c], 0.0f), 1.0f) : V.lut[f][dist];
         if (V.sens[f]) V.sens[f][c] = v[f] - (inx && inz ? 0.5f * (sx + sz) : (inx ? sx : sz));
@@ -321,7 +334,7 @@ __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, 
     }
     inject<NF>(S, mask, (int)c, it, A.nt, v);
 #pragma unroll
-    for (int f = 0; f < NF; ++f) V.out[f][c] = v[f];
+    for (int f = 0; f < NF; ++f) Cvt<T>::st(V.out[f], c, v[f], V.oenc[f]);
   }
 }
 
@@ -329,10 +342,10 @@ __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, 
 // cell in the update region).  The block walks down z four rows at a time (one row per wave)
 // through a 16-row circular LDS window of the stencil inputs; the rows of the next step are
 // loaded into registers while the current rows are computed.
-template <int PH, int N>
+template <int PH, int N, class T>
 __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D& W, const Shot2D& S, const PV& V,
                                                float* lds) {
-  using P = Phys<PH, N>;
+  using P = Phys<PH, N, T>;
   constexpr int NF = P::NF, NS = P::NS;
   constexpr int BX = kTileW, RW = BX + 2 * N;
   static_assert(4 + 2 * N <= 16, "window of 16 rows");
@@ -346,11 +359,11 @@ __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D&
   const bool ina = ca >= 0 && ca < nx, inb = has_b && cb >= 0 && cb < nx;
   auto load_row = [&](int j, float (&ra)[NS], float (&rb)[NS]) {
     const bool inj = j >= 0 && j < nz;
-    const size_t rowo = (size_t)(inj ? j : 0) * (size_t)nx;
+    const unsigned rowo = (unsigned)((inj ? j : 0) * nx);
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-      ra[k] = (inj && ina) ? V.sten[k][rowo + ca] : 0.0f;
-      rb[k] = (inj && inb) ? V.sten[k][rowo + cb] : 0.0f;
+      ra[k] = (inj && ina) ? Cvt<T>::ld(V.sten[k], rowo + (unsigned)ca, V.sdec[k]) : 0.0f;
+      rb[k] = (inj && inb) ? Cvt<T>::ld(V.sten[k], rowo + (unsigned)cb, V.sdec[k]) : 0.0f;
     }
   };
   auto store_row = [&](int j, const float (&ra)[NS], const float (&rb)[NS]) {
@@ -382,10 +395,10 @@ __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D&
     if (col_ok && j < jend) {
       float v[NF];
       P::pre_all(rd, V, S, A.a, i, j, v);
-      const size_t c = (size_t)i + (size_t)j * (size_t)nx;
+      const unsigned c = (unsigned)(i + j * nx);
       inject<NF>(S, mask, (int)c, it, A.nt, v);
 #pragma unroll
-      for (int f = 0; f < NF; ++f) V.out[f][c] = v[f];
+      for (int f = 0; f < NF; ++f) Cvt<T>::st(V.out[f], c, v[f], V.oenc[f]);
     }
     __syncthreads();
     if (more) store_row(jn, na, nb);
@@ -396,9 +409,9 @@ __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D&
 // The host lays the work list out per XCD: hardware hands block b to XCD b % 8, and the list
 // gives each XCD a contiguous, cost-balanced run of the grid so that halo rows stay in its L2.
 // Padding entries have kind -1.
-template <int PH, int N>
+template <int PH, int N, class T>
 __global__ __launch_bounds__(256) void sweep2d_kernel(const SweepArgs A) {
-  using P = Phys<PH, N>;
+  using P = Phys<PH, N, T>;
   constexpr int NF = P::NF, NS = P::NS;
   constexpr int FT = NS * (kTileW + 2 * N + 2) * (kFrameRows + 2 * N + 2) + NF * (kTileW + 2) * (kFrameRows + 2);
   constexpr int IS = NS * 16 * (kTileW + 2 * N);
@@ -410,19 +423,23 @@ __global__ __launch_bounds__(256) void sweep2d_kernel(const SweepArgs A) {
   PV V;
   make_view<PH>(S, A.parity, V);
   if (W.kind == 0)
-    frame_tile<PH, N>(A, W, S, V, lds);
+    frame_tile<PH, N, T>(A, W, S, V, lds);
   else
-    interior_strip<PH, N>(A, W, S, V, lds);
+    interior_strip<PH, N, T>(A, W, S, V, lds);
 }
 
+template <class T>
 __global__ __launch_bounds__(256) void gather2d_kernel(const GatherArgs A) {
   const int it = *A.step;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e < A.n_entries && it < A.nt) {
     const Shot2D& S = A.shots[A.rec_shot[e]];
-    const size_t c = (size_t)A.rec_c[e];
+    const unsigned c = (unsigned)A.rec_c[e];
     const long long o = A.rec_out[e] + (long long)A.rec_stride[e] * it;
-    for (int k = 0; k < A.nslots; ++k) G(A.seis[k])[o] = G(S.fld[A.parity_out][A.field_of_slot[k]])[c];
+    for (int k = 0; k < A.nslots; ++k) {
+      const int f = A.field_of_slot[k];
+      G(A.seis[k])[o] = Cvt<T>::ld(S.fld[A.parity_out][f], c, S.fdec[f]);
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -435,41 +452,77 @@ __global__ __launch_bounds__(256) void gather2d_kernel(const GatherArgs A) {
   }
 }
 
+template <class T>
+__global__ void decode2d_kernel(const void* src, float* dst, size_t n, float dec) {
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x)
+    dst[c] = Cvt<T>::ld(src, c, dec);
+}
+
 int strip_rows(int tile_cfg) { return tile_cfg == 0 ? 16 : (tile_cfg == 1 ? 32 : 64); }
 
-template <int PH, int N>
-static hipError_t launch_cfg(int tile_cfg, const SweepArgs& a, hipStream_t s) {
-  (void)tile_cfg;  // the strip length lives in the work list
-  hipLaunchKernelGGL((sweep2d_kernel<PH, N>), dim3(a.n_work), dim3(kTileW, 4), 0, s, a);
+template <int PH, int N, class T>
+static hipError_t launch_cfg(const SweepArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((sweep2d_kernel<PH, N, T>), dim3(a.n_work), dim3(kTileW, 4), 0, s, a);
   return hipGetLastError();
 }
 
-template <int PH>
-static hipError_t launch_phase(int N, int tile_cfg, const SweepArgs& a, hipStream_t s) {
+template <int PH, class T>
+static hipError_t launch_phase(int N, const SweepArgs& a, hipStream_t s) {
   switch (N) {
-    case 1: return launch_cfg<PH, 1>(tile_cfg, a, s);
-    case 2: return launch_cfg<PH, 2>(tile_cfg, a, s);
-    case 3: return launch_cfg<PH, 3>(tile_cfg, a, s);
-    case 4: return launch_cfg<PH, 4>(tile_cfg, a, s);
-    case 5: return launch_cfg<PH, 5>(tile_cfg, a, s);
+    case 1: return launch_cfg<PH, 1, T>(a, s);
+    case 2: return launch_cfg<PH, 2, T>(a, s);
+    case 3: return launch_cfg<PH, 3, T>(a, s);
+    case 4: return launch_cfg<PH, 4, T>(a, s);
+    case 5: return launch_cfg<PH, 5, T>(a, s);
     default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t launch_sweep2d(int phase, int N, int tile_cfg, const SweepArgs& a, hipStream_t s) {
-  if (a.n_work <= 0) return hipSuccess;
+template <class T>
+static hipError_t launch_t2(int phase, int N, const SweepArgs& a, hipStream_t s) {
   switch (phase) {
-    case kAcVel: return launch_phase<kAcVel>(N, tile_cfg, a, s);
-    case kAcPrs: return launch_phase<kAcPrs>(N, tile_cfg, a, s);
-    case kElVel: return launch_phase<kElVel>(N, tile_cfg, a, s);
-    case kElStr: return launch_phase<kElStr>(N, tile_cfg, a, s);
+    case kAcVel: return launch_phase<kAcVel, T>(N, a, s);
+    case kAcPrs: return launch_phase<kAcPrs, T>(N, a, s);
+    case kElVel: return launch_phase<kElVel, T>(N, a, s);
+    case kElStr: return launch_phase<kElStr, T>(N, a, s);
     default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t launch_gather2d(const GatherArgs& a, hipStream_t s) {
+// tile_cfg only shapes the work list (the strip length lives in each item).
+hipError_t launch_sweep2d(int storage, int phase, int N, int tile_cfg, const SweepArgs& a, hipStream_t s) {
+  (void)tile_cfg;
+  if (a.n_work <= 0) return hipSuccess;
+  switch (storage) {
+    case FOMO_STORE_F32: return launch_t2<float>(phase, N, a, s);
+    case FOMO_STORE_F16: return launch_t2<__half>(phase, N, a, s);
+    case FOMO_STORE_BF16: return launch_t2<__hip_bfloat16>(phase, N, a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_gather2d(int storage, const GatherArgs& a, hipStream_t s) {
   const int blocks = a.n_entries > 0 ? (a.n_entries + 255) / 256 : 1;
-  hipLaunchKernelGGL(gather2d_kernel, dim3(blocks), dim3(256), 0, s, a);
+  switch (storage) {
+    case FOMO_STORE_F32: hipLaunchKernelGGL(gather2d_kernel<float>, dim3(blocks), dim3(256), 0, s, a); break;
+    case FOMO_STORE_F16: hipLaunchKernelGGL(gather2d_kernel<__half>, dim3(blocks), dim3(256), 0, s, a); break;
+    case FOMO_STORE_BF16: hipLaunchKernelGGL(gather2d_kernel<__hip_bfloat16>, dim3(blocks), dim3(256), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_decode(int storage, const void* src, float* dst, size_t n, float dec, hipStream_t s) {
+  const int blocks = (int)std::min<size_t>(4096, (n + 255) / 256);
+  if (blocks == 0) return hipSuccess;
+  switch (storage) {
+    case FOMO_STORE_F32: hipLaunchKernelGGL(decode2d_kernel<float>, dim3(blocks), dim3(256), 0, s, src, dst, n, dec); break;
+    case FOMO_STORE_F16: hipLaunchKernelGGL(decode2d_kernel<__half>, dim3(blocks), dim3(256), 0, s, src, dst, n, dec); break;
+    case FOMO_STORE_BF16:
+      hipLaunchKernelGGL(decode2d_kernel<__hip_bfloat16>, dim3(blocks), dim3(256), 0, s, src, dst, n, dec);
+      break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -5,6 +5,8 @@
 // -ffp-contract=off for host and device alike.  The time loop is a plan object holding all device
 // state; every step is two sweeps and one gather, and pairs of steps are replayed from one
 // captured hipGraph.
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -214,6 +216,30 @@ void weight_lut(int H, int pad, double off, float* out) {
     w = std::min(std::max(w, 0.0), 1.0);
     out[d] = (float)w;
   }
+}
+
+int upload_stored(PlanBase* p, int storage, void** dst, const float* h, size_t n, float enc) {
+  if (storage == FOMO_STORE_F32) {
+    float* d = nullptr;
+    const int rc = p->upload(&d, h, n);
+    *dst = d;
+    return rc;
+  }
+  std::vector<unsigned short> tmp(n);
+  for (size_t c = 0; c < n; ++c) {
+    const float v = h[c] * enc;
+    tmp[c] = storage == FOMO_STORE_F16 ? __half_as_ushort(__float2half_rn(v)) : __bfloat16_as_ushort(__float2bfloat16(v));
+  }
+  unsigned short* d = nullptr;
+  const int rc = p->upload(&d, tmp.data(), n);
+  *dst = d;
+  return rc;
+}
+
+float max_abs(const float* v, size_t n) {
+  float m = 0.0f;
+  for (size_t c = 0; c < n; ++c) m = std::max(m, std::fabs(v[c]));
+  return m;
 }
 
 // exp(-(factor*(nbc-d))^2) for d < nbc, 1 otherwise: factor*(nbc-d) and its square in Float32 as
@@ -509,7 +535,14 @@ static int timed_run(fomo_plan* p, const fomo_desc* d, float* snaps, double* ker
     const int ns = d->nt / d->snap_interval;
     for (int k = 0; k < ns; ++k) {
       if ((rc = b->advance(d->snap_interval))) return rc;
-      const float* src = sh.fld[b->steps_done % 2][snap_field] + (size_t)pad + (size_t)pad * sh.nx;
+      const float* base = nullptr;
+      if (p2->storage == FOMO_STORE_F32) {
+        base = (const float*)sh.fld[b->steps_done % 2][snap_field];
+      } else {
+        if ((rc = p2->decode_field(0, snap_field))) return rc;
+        base = p2->scratch;
+      }
+      const float* src = base + (size_t)pad + (size_t)pad * sh.nx;
       HIP_TRY(hipMemcpy2DAsync(snaps + (size_t)k * d->nx * d->nz, (size_t)d->nx * sizeof(float), src,
                                (size_t)sh.nx * sizeof(float), (size_t)d->nx * sizeof(float), d->nz,
                                hipMemcpyDeviceToHost, b->stream));

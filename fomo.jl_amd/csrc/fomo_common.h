@@ -14,6 +14,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "../../include/fomo_hip.h"
+
 namespace fomo {
 
 constexpr int kMaxN = 5;  // fd_order/2 for fd_order <= 10
@@ -33,9 +35,12 @@ __device__ __forceinline__ gp<T> G(T* p) {
 enum Phase : int { kAcVel = 0, kAcPrs = 1, kElVel = 2, kElStr = 3 };
 
 // One shot of a 2D run (device-resident table entry).
+// Fields and materials are stored as the plan's storage type (Float32, Float16 or BFloat16); a
+// 16-bit value v is stored as RN(v * enc) and read back as float(stored) * dec.
 struct Shot2D {
-  float* fld[2][5];     // ping-pong fields. acoustic: p, vx, vz; elastic: vx, vz, txx, tzz, txz
-  const float* mat[5];  // acoustic: kappa, bx, bz; elastic: lam, lam2mu, muxz, bx, bz
+  void* fld[2][5];      // ping-pong fields. acoustic: p, vx, vz; elastic: vx, vz, txx, tzz, txz
+  const void* mat[5];   // acoustic: kappa, bx, bz; elastic: lam, lam2mu, muxz, bx, bz
+  float fdec[5], fenc[5], mdec[5];
   const float* wlut;    // HABC weights by distance to the edge: 4 tables of H+1 (vx, vz, vy, tau)
   float dtx;            // Float32(dt/dh)
   float q, qt, qxt;     // Higdon coefficients
@@ -90,9 +95,12 @@ struct GatherArgs {
   unsigned* done;
 };
 
-// Launchers (defined in engine2d.hip).  tile_cfg selects the interior strip length.
-hipError_t launch_sweep2d(int phase, int N, int tile_cfg, const SweepArgs& a, hipStream_t s);
-hipError_t launch_gather2d(const GatherArgs& a, hipStream_t s);
+// Launchers (defined in engine2d.hip).  tile_cfg selects the interior strip length; storage is
+// FOMO_STORE_F32 / F16 / BF16.
+hipError_t launch_sweep2d(int storage, int phase, int N, int tile_cfg, const SweepArgs& a, hipStream_t s);
+hipError_t launch_gather2d(int storage, const GatherArgs& a, hipStream_t s);
+// Decode n stored values (storage type, scale dec) to Float32.
+hipError_t launch_decode(int storage, const void* src, float* dst, size_t n, float dec, hipStream_t s);
 int strip_rows(int tile_cfg);   // rows (z) of one interior strip
 constexpr int kTileW = 64;      // tile / strip width (x): one wave64 row
 constexpr int kFrameRows = 8;   // rows of a frame tile

@@ -27,9 +27,11 @@ struct MlpArgs {
   const int* cells;     // frame cells (padded index) of shot 0
   int n_cells;
   int nx, nz, H;
-  const float* p;       // state at time n (read buffer)
-  const float* vx;
-  const float* vz;
+  const void* p;        // state at time n (read buffer), stored as `storage` with scales dec
+  const void* vx;
+  const void* vz;
+  float dec[3];
+  int storage;          // FOMO_STORE_*: inference reads any storage, training needs Float32
   float* alpha_v;       // full-grid arrays of corrections, written at frame cells
   float* alpha_p;
 };

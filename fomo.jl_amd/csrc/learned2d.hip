@@ -11,6 +11,7 @@
 #include <algorithm>
 
 #include "learned2d.h"
+#include "storage.h"
 
 namespace fomo {
 
@@ -124,8 +125,9 @@ constexpr int kInferCells = 64;         // cells per wave pass
 // Features for inference: as gather_features, with the patch scale applied as a multiply by its
 // reciprocal (within one FP32 ulp of the division, far below the BF16 rounding that follows) and
 // 32-bit offsets from the scalar field bases.
-__device__ __forceinline__ void infer_features(const float* p, const float* vx, const float* vz, int nx, int nz,
-                                               int H, int c, float* x) {
+template <class T>
+__device__ __forceinline__ void infer_features(const MlpArgs& A, int c, float* x) {
+  const int nx = A.nx, nz = A.nz, H = A.H;
   const int i = c % nx, j = c / nx;
   float mp = 0.0f, mv = 0.0f;
 #pragma unroll
@@ -136,7 +138,7 @@ __device__ __forceinline__ void infer_features(const float* p, const float* vx, 
       const int ii = min(max(i + dx, 0), nx - 1);
       const int q = (dy + 2) * 5 + (dx + 2);
       const unsigned g = (unsigned)(ii + jj * nx);
-      const float a = G(p)[g], b = G(vx)[g], d = G(vz)[g];
+      const float a = Cvt<T>::ld(A.p, g, A.dec[0]), b = Cvt<T>::ld(A.vx, g, A.dec[1]), d = Cvt<T>::ld(A.vz, g, A.dec[2]);
       x[q] = a;
       x[25 + q] = b;
       x[50 + q] = d;
@@ -158,6 +160,7 @@ __device__ __forceinline__ __bf16 wbf(const unsigned short* w, int e) {
   return __builtin_bit_cast(__bf16, u);
 }
 
+template <class T>
 __global__ __launch_bounds__(64 * kInferWaves) void mlp_infer_kernel(const MlpArgs A) {
   __shared__ __bf16 feats[kInferWaves][kInferCells * kK1];
   __shared__ __bf16 hbuf[kInferWaves][kInferCells * kMlpH];
@@ -191,7 +194,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void mlp_infer_kernel(const MlpAr
     {
       const int e = g + lane;
       float x[kK1];
-      infer_features(A.p, A.vx, A.vz, A.nx, A.nz, A.H, G(A.cells)[e < A.n_cells ? e : 0], x);
+      infer_features<T>(A, G(A.cells)[e < A.n_cells ? e : 0], x);
 #pragma unroll
       for (int k = kMlpIn; k < kK1; ++k) x[k] = 0.0f;
 #pragma unroll
@@ -331,7 +334,7 @@ __device__ void train_blocks(const MlpTrainArgs* T, const void* w_bf16, const fl
         if (FROM_FIELDS) {
           const MlpArgs& A = T->m;
           const int c = A.cells[t];
-          gather_features(A.p, A.vx, A.vz, A.nx, A.nz, A.H, c, x);
+          gather_features((const float*)A.p, (const float*)A.vx, (const float*)A.vz, A.nx, A.nz, A.H, c, x);
           const int i = c % A.nx, j = c / A.nx;
           const size_t tc = (size_t)(i + T->off_x) + (size_t)(j + T->off_z) * (size_t)T->tnx;
           const float rvx = (T->vx1[c] - T->tvx[tc]) * T->inv_scale_v;
@@ -401,7 +404,14 @@ hipError_t launch_mlp_forward(const MlpArgs& a, hipStream_t s) {
   if (a.n_cells <= 0) return hipSuccess;
   const int per_block = kInferWaves * kInferCells;
   const int blocks = std::min(4096, (a.n_cells + per_block - 1) / per_block);
-  hipLaunchKernelGGL(mlp_infer_kernel, dim3(blocks), dim3(64 * kInferWaves), 0, s, a);
+  switch (a.storage) {
+    case FOMO_STORE_F32: hipLaunchKernelGGL(mlp_infer_kernel<float>, dim3(blocks), dim3(64 * kInferWaves), 0, s, a); break;
+    case FOMO_STORE_F16: hipLaunchKernelGGL(mlp_infer_kernel<__half>, dim3(blocks), dim3(64 * kInferWaves), 0, s, a); break;
+    case FOMO_STORE_BF16:
+      hipLaunchKernelGGL(mlp_infer_kernel<__hip_bfloat16>, dim3(blocks), dim3(64 * kInferWaves), 0, s, a);
+      break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

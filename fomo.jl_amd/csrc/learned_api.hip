@@ -93,6 +93,8 @@ extern "C" int fomo_learned_train_step(fomo_plan* student, fomo_plan* teacher, i
   Plan2D* t = acoustic_plan(teacher);
   if (!q || !q->training) return set_error(FOMO_EINVAL, "student plan is not in training mode");
   if (!t || t->shots.size() != 1) return set_error(FOMO_EINVAL, "teacher must be a single-shot acoustic2d plan");
+  if (q->storage != FOMO_STORE_F32 || t->storage != FOMO_STORE_F32)
+    return set_error(FOMO_EINVAL, "training needs Float32 storage in the student and teacher plans");
   const Shot2D& S = q->shots[0];
   const Shot2D& T = t->shots[0];
   if (off_x < 0 || off_z < 0 || S.nx + off_x > T.nx || S.nz + off_z > T.nz)
@@ -118,16 +120,18 @@ extern "C" int fomo_learned_train_step(fomo_plan* student, fomo_plan* teacher, i
   a.m.p = S.fld[s][0];
   a.m.vx = S.fld[s][1];
   a.m.vz = S.fld[s][2];
-  a.p1 = S.fld[1 - s][0];
-  a.vx1 = S.fld[1 - s][1];
-  a.vz1 = S.fld[1 - s][2];
+  for (int f = 0; f < 3; ++f) a.m.dec[f] = 1.0f;
+  a.m.storage = FOMO_STORE_F32;
+  a.p1 = (const float*)S.fld[1 - s][0];
+  a.vx1 = (const float*)S.fld[1 - s][1];
+  a.vz1 = (const float*)S.fld[1 - s][2];
   a.sens_vx = q->sens[0];
   a.sens_vz = q->sens[1];
   a.sens_p = q->sens[2];
   const int to = 1 - ts;
-  a.tp = T.fld[to][0];
-  a.tvx = T.fld[to][1];
-  a.tvz = T.fld[to][2];
+  a.tp = (const float*)T.fld[to][0];
+  a.tvx = (const float*)T.fld[to][1];
+  a.tvz = (const float*)T.fld[to][2];
   a.tnx = T.nx;
   a.off_x = off_x;
   a.off_z = off_z;

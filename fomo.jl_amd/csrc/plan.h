@@ -73,6 +73,10 @@ void buoyancy2d(const std::vector<float>& rho, int nx, int nz, std::vector<float
 void elastic_params2d(const std::vector<float>& vp, const std::vector<float>& vs, const std::vector<float>& rho,
                       int nx, int nz, std::vector<float>& lam, std::vector<float>& lam2mu, std::vector<float>& muxz);
 void weight_lut(int H, int pad, double off, float* out);
+// Upload n Float32 values as the storage type (16-bit: RN(v * enc)) into a new plan allocation.
+int upload_stored(PlanBase* p, int storage, void** dst, const float* h, size_t n, float enc);
+// Max |v| over n values (0 if none).
+float max_abs(const float* v, size_t n);
 // Cerjan sponge factors by distance d = 0..pad-1 to the padded edge (src/boundary/sponge.jl:30-50).
 void sponge_lut(int nbc, int pad, float factor, float* out);
 float min_positive(const float* a, size_t n);

@@ -14,6 +14,9 @@ struct ShotMeta {
 
 struct Plan2D : PlanBase {
   int N = 0, nfld = 0, nslots = 0, tile_cfg = 1;
+  int storage = FOMO_STORE_F32, elem = 4;  // storage of fields and materials
+  float* scratch = nullptr;                // Float32 staging of one decoded field
+  size_t scratch_n = 0;
   float coef[kMaxN] = {0};
   std::vector<Shot2D> shots;
   std::vector<ShotMeta> meta;
@@ -46,6 +49,8 @@ struct Plan2D : PlanBase {
   int dims(int shot, int* nx, int* ny, int* nz, int* pad) const override;
   int n_shots() const override { return (int)shots.size(); }
   int make_work(int cfg);
+  // Decode the current field `which` of a shot into `scratch` (asynchronous on the stream).
+  int decode_field(int shot, int which);
   int autotune();
 };
 

@@ -86,7 +86,7 @@ hipError_t Plan2D::enqueue_part(int part, int s) {
     g.nt = d.nt;
     g.step = step;
     g.done = done;
-    return launch_gather2d(g, stream);
+    return launch_gather2d(storage, g, stream);
   }
   SweepArgs a{};
   a.shots = d_shots;
@@ -96,7 +96,7 @@ hipError_t Plan2D::enqueue_part(int part, int s) {
   for (int l = 0; l < N; ++l) a.a[l] = coef[l];
   a.step = step;
   a.nt = d.nt;
-  if (part == 1) return launch_sweep2d(d.eq == FOMO_ACOUSTIC2D ? kAcPrs : kElStr, N, tile_cfg, a, stream);
+  if (part == 1) return launch_sweep2d(storage, d.eq == FOMO_ACOUSTIC2D ? kAcPrs : kElStr, N, tile_cfg, a, stream);
   if (learned) {
     // blend-weight corrections of every frame cell from the state at time n
     MlpArgs m{};
@@ -109,19 +109,21 @@ hipError_t Plan2D::enqueue_part(int part, int s) {
     m.p = shots[0].fld[s][0];
     m.vx = shots[0].fld[s][1];
     m.vz = shots[0].fld[s][2];
+    for (int f = 0; f < 3; ++f) m.dec[f] = shots[0].fdec[f];
+    m.storage = storage;
     m.alpha_v = alpha[0];
     m.alpha_p = alpha[1];
     const hipError_t e = launch_mlp_forward(m, stream);
     if (e != hipSuccess) return e;
   }
-  return launch_sweep2d(d.eq == FOMO_ACOUSTIC2D ? kAcVel : kElVel, N, tile_cfg, a, stream);
+  return launch_sweep2d(storage, d.eq == FOMO_ACOUSTIC2D ? kAcVel : kElVel, N, tile_cfg, a, stream);
 }
 
 int Plan2D::reset() {
   for (auto& sh : shots) {
     const size_t n = (size_t)sh.nx * sh.nz;
     for (int b = 0; b < 2; ++b)
-      for (int f = 0; f < nfld; ++f) FOMO_HIP_TRY(hipMemsetAsync(sh.fld[b][f], 0, n * sizeof(float), stream));
+      for (int f = 0; f < nfld; ++f) FOMO_HIP_TRY(hipMemsetAsync(sh.fld[b][f], 0, n * elem, stream));
   }
   for (int k = 0; k < nslots; ++k)
     if (seis_total) FOMO_HIP_TRY(hipMemsetAsync(seis[k], 0, (size_t)seis_total * sizeof(float), stream));
@@ -141,13 +143,39 @@ int Plan2D::read_seis(int shot, float* const* dst) {
   return FOMO_OK;
 }
 
+int Plan2D::decode_field(int shot, int which) {
+  const Shot2D& sh = shots[shot];
+  const size_t n = (size_t)sh.nx * sh.nz;
+  if (scratch_n < n) {
+    if (scratch) {
+      FOMO_HIP_TRY(hipStreamSynchronize(stream));
+      auto it = std::find(allocs.begin(), allocs.end(), (void*)scratch);
+      if (it != allocs.end()) allocs.erase(it);
+      (void)hipFree(scratch);
+      scratch = nullptr;
+      scratch_n = 0;
+    }
+    const int rc = alloc(&scratch, n);
+    if (rc) return rc;
+    scratch_n = n;
+  }
+  FOMO_HIP_TRY(launch_decode(storage, sh.fld[steps_done % 2][which], scratch, n, sh.fdec[which], stream));
+  return FOMO_OK;
+}
+
 int Plan2D::read_field(int shot, int which, float* out) {
   if (shot < 0 || shot >= (int)shots.size()) return set_error(FOMO_EINVAL, "shot index out of range");
   if (which < 0 || which >= nfld) return set_error(FOMO_EINVAL, "field index out of range");
-  FOMO_HIP_TRY(hipStreamSynchronize(stream));
   const Shot2D& sh = shots[shot];
-  FOMO_HIP_TRY(hipMemcpy(out, sh.fld[steps_done % 2][which], (size_t)sh.nx * sh.nz * sizeof(float),
-                         hipMemcpyDeviceToHost));
+  const size_t n = (size_t)sh.nx * sh.nz;
+  const void* src = sh.fld[steps_done % 2][which];
+  if (storage != FOMO_STORE_F32) {
+    const int rc = decode_field(shot, which);
+    if (rc) return rc;
+    src = scratch;
+  }
+  FOMO_HIP_TRY(hipStreamSynchronize(stream));
+  FOMO_HIP_TRY(hipMemcpy(out, src, n * sizeof(float), hipMemcpyDeviceToHost));
   return FOMO_OK;
 }
 
@@ -222,7 +250,6 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
                   const fomo_shot* sh, Plan2D** out) {
   *out = nullptr;
   if (d->eq != FOMO_ACOUSTIC2D && d->eq != FOMO_ELASTIC2D) return set_error(FOMO_EINVAL, "not a 2D equation");
-  if (d->storage != FOMO_STORE_F32) return set_error(FOMO_EUNSUP, "2D solvers store Float32 only");
   if (n_shots < 1) return set_error(FOMO_EINVAL, "at least one shot is required");
   if (!vp || !rho || (d->eq == FOMO_ELASTIC2D && !vs)) return set_error(FOMO_EINVAL, "model arrays must not be NULL");
   if (!sh) return set_error(FOMO_EINVAL, "shot table is NULL");
@@ -234,8 +261,11 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
     if (rc) return rc;
     if (sh[s].n_src > 0 && !sh[s].wavelet) return set_error(FOMO_EINVAL, "wavelet must not be NULL");
   }
+  if (d->storage < FOMO_STORE_F32 || d->storage > FOMO_STORE_BF16) return set_error(FOMO_EINVAL, "unknown storage");
   Plan2D* p = new Plan2D();
   p->d = *d;
+  p->storage = d->storage;
+  p->elem = d->storage == FOMO_STORE_F32 ? 4 : 2;
   p->N = d->fd_order / 2;
   fomo_fd_coefficients(d->fd_order, p->coef);
   p->nfld = d->eq == FOMO_ACOUSTIC2D ? 3 : 5;
@@ -291,31 +321,70 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
     x.q = hc[0];
     x.qt = hc[1];
     x.qxt = hc[2];
+    // storage scales.  Sources into p or stresses: p / stresses in units of the shot's peak source
+    // amplitude, velocities in that over the smallest impedance rho*vp of the window (cells with
+    // vp, rho > 0).  Vertical force into vz: velocities in units of the peak amplitude, stresses
+    // in that times the largest impedance.
+    const bool scaled = p->storage != FOMO_STORE_F32;
+    {
+      float wmax = 0.0f, zmin = INFINITY, zmax = 0.0f;
+      if (S.wavelet) wmax = max_abs(S.wavelet, (size_t)S.n_src * d->nt);
+      for (int j = 0; j < d->nz; ++j)
+        for (int i = 0; i < S.nx; ++i) {
+          const size_t c = (size_t)(S.x0 - 1 + i) + (size_t)j * d->nx;
+          if (vp[c] > 0.0f && rho[c] > 0.0f) {
+            zmin = std::min(zmin, rho[c] * vp[c]);
+            zmax = std::max(zmax, rho[c] * vp[c]);
+          }
+        }
+      const float amp = scaled && wmax > 0.0f ? wmax : 1.0f;
+      const bool has_z = std::isfinite(zmin) && zmin > 0.0f;
+      const bool force = d->eq == FOMO_ELASTIC2D && d->source_type == FOMO_SRC_FZ;
+      float sdec = amp, vdec = amp;
+      if (scaled && has_z) {
+        if (force)
+          sdec = amp * zmax;
+        else
+          vdec = amp / zmin;
+      }
+      if (!scaled) sdec = vdec = 1.0f;
+      for (int f = 0; f < 5; ++f) {
+        const bool vel = d->eq == FOMO_ACOUSTIC2D ? (f == 1 || f == 2) : (f < 2);
+        x.fdec[f] = vel ? vdec : sdec;
+        x.fenc[f] = 1.0f / x.fdec[f];
+        x.mdec[f] = 1.0f;
+      }
+    }
     const auto key = std::make_tuple(S.x0, S.nx, v_ref);
     auto mit = medium_of.find(key);
     if (mit != medium_of.end()) {
       const Shot2D& o = p->shots[mit->second];
-      for (int k = 0; k < 5; ++k) x.mat[k] = o.mat[k];
+      for (int k = 0; k < 5; ++k) {
+        x.mat[k] = o.mat[k];
+        x.mdec[k] = o.mdec[k];
+      }
       x.wlut = o.wlut;
     } else {
       std::vector<float> vpp, rhop, bx, bz;
       pad2d(vp, S.nx, d->nz, pad, S.x0 - 1, d->nx, vpp);
       pad2d(rho, S.nx, d->nz, pad, S.x0 - 1, d->nx, rhop);
       buoyancy2d(rhop, x.nx, x.nz, bx, bz);
-      float* dm[5] = {nullptr};
+      void* dm[5] = {nullptr};
+      // material m, stored with scale max |value| when 16-bit
+      auto put = [&](int m, const std::vector<float>& h) -> int {
+        const float mx = max_abs(h.data(), h.size());
+        x.mdec[m] = scaled && mx > 0.0f ? mx : 1.0f;
+        return upload_stored(p, p->storage, &dm[m], h.data(), h.size(), 1.0f / x.mdec[m]);
+      };
       if (d->eq == FOMO_ACOUSTIC2D) {
         std::vector<float> kappa(n);
         for (size_t c = 0; c < n; ++c) kappa[c] = rhop[c] * (vpp[c] * vpp[c]);
-        if ((rc = p->upload(&dm[0], kappa.data(), n)) || (rc = p->upload(&dm[1], bx.data(), n)) ||
-            (rc = p->upload(&dm[2], bz.data(), n)))
-          return bail(rc);
+        if ((rc = put(0, kappa)) || (rc = put(1, bx)) || (rc = put(2, bz))) return bail(rc);
       } else {
         std::vector<float> vsp, lam, l2m, muxz;
         pad2d(vs, S.nx, d->nz, pad, S.x0 - 1, d->nx, vsp);
         elastic_params2d(vpp, vsp, rhop, x.nx, x.nz, lam, l2m, muxz);
-        if ((rc = p->upload(&dm[0], lam.data(), n)) || (rc = p->upload(&dm[1], l2m.data(), n)) ||
-            (rc = p->upload(&dm[2], muxz.data(), n)) || (rc = p->upload(&dm[3], bx.data(), n)) ||
-            (rc = p->upload(&dm[4], bz.data(), n)))
+        if ((rc = put(0, lam)) || (rc = put(1, l2m)) || (rc = put(2, muxz)) || (rc = put(3, bx)) || (rc = put(4, bz)))
           return bail(rc);
       }
       for (int k = 0; k < 5; ++k) x.mat[k] = dm[k];
@@ -331,7 +400,11 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
     }
     for (int b = 0; b < 2; ++b)
       for (int f = 0; f < p->nfld; ++f)
-        if ((rc = p->alloc(&x.fld[b][f], n))) return bail(rc);
+      {
+        char* fp = nullptr;
+        if ((rc = p->alloc(&fp, n * p->elem))) return bail(rc);
+        x.fld[b][f] = fp;
+      }
     // sources
     x.n_src = S.n_src;
     x.src_fz = d->source_type == FOMO_SRC_FZ ? 1 : 0;

@@ -1,0 +1,88 @@
+// storage.h -- loads and stores of stored arrays (Float32, or Float16 / BFloat16 with a per-array
+// scale: a value v is stored as RN(v * enc) and read back as float(stored) * dec); arithmetic is
+// always Float32.
+#pragma once
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "fomo_common.h"
+
+namespace fomo {
+
+// Element c of a global array of E: with a 32-bit index the byte offset is formed in 32 bits,
+// so the access compiles to a scalar base plus a 32-bit vector offset.  NT: streaming hint
+// (measured slower for every array of the sweeps, which re-read their inputs through L2).
+template <class E, bool NT = false, class I>
+__device__ __forceinline__ E gget(const void* p, I c) {
+  gp<const E> a;
+  if constexpr (sizeof(I) == 4)
+    a = (gp<const E>)(G((const char*)p) + (unsigned)(c * (unsigned)sizeof(E)));
+  else
+    a = G((const E*)p) + c;
+  if constexpr (NT)
+    return __builtin_nontemporal_load(a);
+  else
+    return *a;
+}
+template <class E, bool NT = false, class I>
+__device__ __forceinline__ void gput(void* p, I c, E v) {
+  gp<E> a;
+  if constexpr (sizeof(I) == 4)
+    a = (gp<E>)(G((char*)p) + (unsigned)(c * (unsigned)sizeof(E)));
+  else
+    a = G((E*)p) + c;
+  if constexpr (NT)
+    __builtin_nontemporal_store(v, a);
+  else
+    *a = v;
+}
+
+template <class T>
+struct Cvt;
+template <>
+struct Cvt<float> {
+  template <bool NT = false, class I>
+  __device__ __forceinline__ static float ld(const void* p, I c, float) {
+    return gget<float, NT>(p, c);
+  }
+  template <bool NT = false, class I>
+  __device__ __forceinline__ static void st(void* p, I c, float v, float) {
+    gput<float, NT>(p, c, v);
+  }
+};
+template <>
+struct Cvt<__half> {
+  template <bool NT = false, class I>
+  __device__ __forceinline__ static float ld(const void* p, I c, float dec) {
+    return __half2float(__ushort_as_half(gget<unsigned short, NT>(p, c))) * dec;
+  }
+  template <bool NT = false, class I>
+  __device__ __forceinline__ static void st(void* p, I c, float v, float enc) {
+    gput<unsigned short, NT>(p, c, __half_as_ushort(__float2half_rn(v * enc)));
+  }
+};
+template <>
+struct Cvt<__hip_bfloat16> {
+  template <bool NT = false, class I>
+  __device__ __forceinline__ static float ld(const void* p, I c, float dec) {
+    return __bfloat162float(__ushort_as_bfloat16(gget<unsigned short, NT>(p, c))) * dec;
+  }
+  template <bool NT = false, class I>
+  __device__ __forceinline__ static void st(void* p, I c, float v, float enc) {
+    gput<unsigned short, NT>(p, c, __bfloat16_as_ushort(__float2bfloat16(v * enc)));
+  }
+};
+
+// Base of plane k of a stored array (uniform across the block, so loads address it as a scalar
+// base plus a 32-bit in-plane offset).
+template <class T>
+__device__ __forceinline__ const void* plane_of(const void* p, int k, size_t plane_elems) {
+  return (const void*)((const T*)p + (size_t)k * plane_elems);
+}
+template <class T>
+__device__ __forceinline__ void* plane_of(void* p, int k, size_t plane_elems) {
+  return (void*)((T*)p + (size_t)k * plane_elems);
+}
+
+}  // namespace fomo

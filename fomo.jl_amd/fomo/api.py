@@ -82,12 +82,14 @@ def _common_checks(boundary, nbc, fd_order, nt, snap_interval, learned_ok=False)
 
 def acoustic2d(vp, rho, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_order=8, snap_interval=0,
                boundary="habc", v_ref=None, sponge_factor=0.015, wavelet=None, scale_source_by_dt=False,
-               use_cuda_graph=True, verbose=True, learned_weights=None):
+               use_cuda_graph=True, verbose=True, learned_weights=None, storage="f32"):
     """acoustic2d(vp, rho, dh, dt, nt, f0; sx, sz, rx, rz, ...) -> (seis_p, seis_vx, seis_vz, snaps, stats).
 
     boundary = :habc (deterministic two-pass Higdon), :sponge (Cerjan, `sponge_factor`) or :learned
     (an extension: the Higdon frame with MLP-corrected blend weights, fomo.learned, trained for thin
-    layers such as nbc = 16; `learned_weights` overrides the shipped network)."""
+    layers such as nbc = 16; `learned_weights` overrides the shipped network).
+    `storage` (extension): :f32 (default, bit-identical to the reference), :f16 or :bf16 storage of
+    wavefields and materials with per-field scales; arithmetic stays Float32."""
     del verbose
     _common_checks(boundary, nbc, fd_order, nt, snap_interval, learned_ok=True)
     vp, rho = _f32F(vp), _f32F(rho)
@@ -105,9 +107,9 @@ def acoustic2d(vp, rho, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_order=8, s
     n_rec = len(rx)
     if _sym(boundary) == "learned":
         return _acoustic2d_learned(vp, rho, dh, dt, nt, sx, sz, rx, rz, wm, nbc, fd_order, v_ref, snap_interval,
-                                   use_cuda_graph, learned_weights)
+                                   use_cuda_graph, learned_weights, storage)
     d = _desc(_lib.ACOUSTIC2D, nx, nz, nbc, fd_order, nt, dh, dt, v_ref, len(sx), n_rec, snap_interval,
-              use_cuda_graph, boundary=boundary, sponge_factor=sponge_factor)
+              use_cuda_graph, boundary=boundary, sponge_factor=sponge_factor, storage=storage_code(storage))
     sp, svx, svz = (np.zeros((n_rec, nt), np.float32, order="F") for _ in range(3))
     nsnap = nt // snap_interval if snap_interval > 0 else 0
     snaps = np.zeros((nsnap, nz, nx), np.float32) if nsnap else None
@@ -121,13 +123,13 @@ def acoustic2d(vp, rho, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_order=8, s
 
 
 def _acoustic2d_learned(vp, rho, dh, dt, nt, sx, sz, rx, rz, wm, nbc, fd_order, v_ref, snap_interval, use_graph,
-                        weights):
+                        weights, storage="f32"):
     """boundary = :learned through a device-resident plan: enable the network, run the loop (timed
     with hipEvents), copy snapshots of the interior of p between segments, read the seismograms."""
     from . import learned
     w = learned.load_weights() if weights is None else np.asarray(weights, np.float32)
     p = Plan(_lib.ACOUSTIC2D, vp, rho, dh, dt, nt, sx=sx, sz=sz, rx=rx, rz=rz, wavelet=wm, nbc=nbc,
-             fd_order=fd_order, v_ref=v_ref, use_graph=use_graph)
+             fd_order=fd_order, v_ref=v_ref, use_graph=use_graph, storage=storage)
     try:
         learned.enable(p, w)
         snaps = []
@@ -150,11 +152,11 @@ def _acoustic2d_learned(vp, rho, dh, dt, nt, sx, sz, rx, rz, wm, nbc, fd_order, 
 
 def elastic2d(vp, vs, rho, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_order=8, snap_interval=0,
               boundary="habc", v_ref=None, sponge_factor=0.015, wavelet=None, scale_source_by_dt=False,
-              use_cuda_graph=True, verbose=True, source_type="explosive"):
+              use_cuda_graph=True, verbose=True, source_type="explosive", storage="f32"):
     """elastic2d(vp, vs, rho, dh, dt, nt, f0; ...) -> (seis_vx, seis_vz, snaps, stats).
 
     `source_type` (extension, default :explosive = the reference's τxx+τzz source); :fz adds the
-    wavelet to vz at the end of the velocity phase (a vertical force)."""
+    wavelet to vz at the end of the velocity phase (a vertical force).  `storage` as acoustic2d."""
     del verbose
     _common_checks(boundary, nbc, fd_order, nt, snap_interval)
     st = _sym(source_type)
@@ -176,7 +178,7 @@ def elastic2d(vp, vs, rho, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_order=8
     n_rec = len(rx)
     d = _desc(_lib.ELASTIC2D, nx, nz, nbc, fd_order, nt, dh, dt, v_ref, len(sx), n_rec, snap_interval,
               use_cuda_graph, source_type=_lib.SRC_FZ if st == "fz" else _lib.SRC_EXPLOSIVE, boundary=boundary,
-              sponge_factor=sponge_factor)
+              sponge_factor=sponge_factor, storage=storage_code(storage))
     svx, svz = (np.zeros((n_rec, nt), np.float32, order="F") for _ in range(2))
     nsnap = nt // snap_interval if snap_interval > 0 else 0
     snaps = np.zeros((nsnap, nz, nx), np.float32) if nsnap else None
@@ -210,7 +212,7 @@ def _shot_matrix(sx, sz):
 
 
 def _batch(eq, vp, vs, rho, dh, dt, nt, f0, sx, sz, rx, rz, nbc, fd_order, v_ref, wavelet,
-           scale_source_by_dt, use_cuda_graph):
+           scale_source_by_dt, use_cuda_graph, storage="f32"):
     vp, rho = _f32F(vp), _f32F(rho)
     if vs is not None:
         vs = _f32F(vs)
@@ -232,7 +234,8 @@ def _batch(eq, vp, vs, rho, dh, dt, nt, f0, sx, sz, rx, rz, nbc, fd_order, v_ref
     # device layout (n_src, n_shots) column-major (batch_acq.jl:27-37) = (n_shots, n_src) row-major
     sxd = np.ascontiguousarray(sxm.astype(np.int32))
     szd = np.ascontiguousarray(szm.astype(np.int32))
-    d = _desc(eq, nx, nz, nbc, fd_order, nt, dh, dt, v_ref, n_src, n_rec, 0, use_cuda_graph)
+    d = _desc(eq, nx, nz, nbc, fd_order, nt, dh, dt, v_ref, n_src, n_rec, 0, use_cuda_graph,
+              storage=storage_code(storage))
     nslots = 3 if eq == _lib.ACOUSTIC2D else 2
     outs = [np.zeros((n_rec, nt, ns), np.float32, order="F") for _ in range(nslots)]
     t = C.c_double(0.0)
@@ -250,22 +253,22 @@ def _batch(eq, vp, vs, rho, dh, dt, nt, f0, sx, sz, rx, rz, nbc, fd_order, v_ref
 
 
 def acoustic2d_batch(vp, rho, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_order=8, v_ref=None, wavelet=None,
-                     scale_source_by_dt=False, verbose=True, use_cuda_graph=True):
+                     scale_source_by_dt=False, verbose=True, use_cuda_graph=True, storage="f32"):
     """acoustic2d_batch (src/equations/acoustic2d/batch.jl:180-261) -> (seis_p, seis_vx, seis_vz, stats).
 
     Seismograms are (n_rec, nt, n_shots); each shot is bit-identical to its solo acoustic2d run."""
     del verbose
     outs, st = _batch(_lib.ACOUSTIC2D, vp, None, rho, dh, dt, nt, f0, sx, sz, rx, rz, nbc, fd_order, v_ref, wavelet,
-                      scale_source_by_dt, use_cuda_graph)
+                      scale_source_by_dt, use_cuda_graph, storage)
     return Acoustic2DBatchResult(outs[0], outs[1], outs[2], st)
 
 
 def elastic2d_batch(vp, vs, rho, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_order=8, v_ref=None, wavelet=None,
-                    scale_source_by_dt=False, verbose=True, use_cuda_graph=True):
+                    scale_source_by_dt=False, verbose=True, use_cuda_graph=True, storage="f32"):
     """elastic2d_batch (src/equations/elastic2d/batch.jl:197-261) -> (seis_vx, seis_vz, stats)."""
     del verbose
     outs, st = _batch(_lib.ELASTIC2D, vp, vs, rho, dh, dt, nt, f0, sx, sz, rx, rz, nbc, fd_order, v_ref, wavelet,
-                      scale_source_by_dt, use_cuda_graph)
+                      scale_source_by_dt, use_cuda_graph, storage)
     return Elastic2DBatchResult(outs[0], outs[1], st)
 
 
@@ -277,6 +280,10 @@ _STORAGE = {"f32": _lib.STORE_F32, "float32": _lib.STORE_F32, "f16": _lib.STORE_
 
 
 def storage_code(storage) -> int:
+    if isinstance(storage, (int, np.integer)):
+        if int(storage) not in _STORAGE.values():
+            raise ArgumentError(f"unknown storage code {storage}")
+        return int(storage)
     key = _sym(storage).lower()
     if key not in _STORAGE:
         raise ArgumentError(f"storage must be :f32, :f16 or :bf16, got {storage}")
@@ -398,7 +405,7 @@ class Plan:
             self._keep += [sx, sz, rx, rz, sy, ry, wm]
             self.n_rec = [len(rx)]
             d = _desc(eq, nx, nz, nbc, fd_order, nt, dh, dt, v_ref, len(sx), len(rx), 0, use_graph,
-                      source_type=source_type, ny=ny, storage=storage)
+                      source_type=source_type, ny=ny, storage=storage_code(storage))
             d.record_p = 1 if record_p else 0
             d.tile_cfg = tile_cfg
             _lib.check(_lib.lib().fomo_plan_create(C.byref(d), vp.ctypes.data, ptr(vs), rho.ctypes.data,
@@ -418,7 +425,8 @@ class Plan:
                 t.rx, t.rz = srx.ctypes.data, srz.ctypes.data
                 t.v_ref = float(s.get("v_ref") or 0.0)
                 self.n_rec.append(len(srx))
-            d = _desc(eq, nx, nz, nbc, fd_order, nt, dh, dt, 0.0, 0, 0, 0, use_graph, source_type=source_type)
+            d = _desc(eq, nx, nz, nbc, fd_order, nt, dh, dt, 0.0, 0, 0, 0, use_graph, source_type=source_type,
+                      storage=storage_code(storage))
             d.tile_cfg = tile_cfg
             _lib.check(_lib.lib().fomo_batch_create(C.byref(d), vp.ctypes.data, ptr(vs), rho.ctypes.data,
                                                     len(shots), table, C.byref(h)))

@@ -179,7 +179,7 @@ def reflection_residual(vp, rho, *, sx, sz, rx, rz, nt, boundaries, dh=10.0, dt=
     """The reference's absorption measure (test/verify_patch.jl:76-91): resid = ||u - u_ref|| / ||u_ref||
     over seis_p at all receivers, with u_ref from the model edge-extended by L cells (reflections of its
     own boundary arrive after the record ends), also given as 20 log10(resid) dB.  `boundaries` is a
-    list of (name, nbc, weights-or-None).  Also reports per-trace relative L2 against u_ref and against
+    list of (name, nbc, weights-or-None[, storage]).  Also reports per-trace relative L2 against u_ref and against
     the boundary named `judge`, and the time of the step loop."""
     from .setup import ricker_wavelet
     wav = ricker_wavelet(f0, dt, nt)
@@ -187,9 +187,9 @@ def reflection_residual(vp, rho, *, sx, sz, rx, rz, nt, boundaries, dh=10.0, dt=
         L = int(math.ceil(float(vp.max()) * nt * dt / dh / 2.0)) + 20
     rx, rz = np.asarray(rx), np.asarray(rz)
 
-    def run(v, r, nbc, sx_, sz_, rx_, rz_, weights=None, v_ref=None):
+    def run(v, r, nbc, sx_, sz_, rx_, rz_, weights=None, v_ref=None, storage="f32"):
         p = Plan(_lib.ACOUSTIC2D, v, r, dh, dt, nt, sx=[sx_], sz=[sz_], rx=rx_, rz=rz_, wavelet=wav[None], nbc=nbc,
-                 fd_order=fd_order, v_ref=v_ref)
+                 fd_order=fd_order, v_ref=v_ref, storage=storage)
         if weights is not None:
             enable(p, weights)
         ms = p.timed_advance(nt)
@@ -200,7 +200,8 @@ def reflection_residual(vp, rho, *, sx, sz, rx, rz, nt, boundaries, dh=10.0, dt=
 
     u_ref, _, _ = run(edge_extend(vp, L), edge_extend(rho, L), 20, sx + L, sz + L, rx + L, rz + L,
                       v_ref=float(vp.min()))
-    runs = {name: run(vp, rho, nbc, sx, sz, rx, rz, weights) for name, nbc, weights in boundaries}
+    runs = {b[0]: run(vp, rho, b[1], sx, sz, rx, rz, b[2], storage=b[3] if len(b) > 3 else "f32")
+            for b in boundaries}
     den = np.linalg.norm(u_ref, axis=1)
     ok = den > 1e-3 * den.max()
     out = {}

@@ -65,6 +65,10 @@ def test_validation_errors_raise_argument_error():
         fomo.acoustic2d(vp, rho, 10.0, 1e-3, 100, 10.0, **kw, boundary="pml")
     with pytest.raises(fomo.ArgumentError):
         fomo.acoustic2d(vp, rho, 10.0, 1e-3, 100, 10.0, **kw, wavelet=np.zeros(99))
+    with pytest.raises(fomo.ArgumentError):
+        fomo.acoustic2d(vp, rho, 10.0, 1e-3, 100, 10.0, **kw, storage="f8")
+    with pytest.raises(fomo.ArgumentError):
+        fomo.elastic2d(vp, vp / 1.73, rho, 10.0, 1e-3, 100, 10.0, **kw, storage=7)
 
 
 def test_cfl_bound_matches_reference_formula():
