@@ -113,21 +113,20 @@ struct TileRd {
   }
 };
 
-// Pre-correction values of all output fields of a phase at padded cell (i, j): the FD update
-// inside the update region N <= i < nx-N, N <= j < nz-N, the old value elsewhere.
+// Phase physics.  NF output fields, NS stencil fields, NM materials.  `upd` is the FD update at
+// padded cell (i, j) inside the update region N <= i < nx-N, N <= j < nz-N, from the stencil
+// reader rd, the decoded old values o and materials m; outside that region the pre-correction
+// value is the old value.  Loads are done by the callers, so a block can issue all of its loads
+// before it computes anything.
 template <int PH, int N, class T>
 struct Phys;
 
 template <int N, class T>
 struct Phys<kAcVel, N, T> {
-  static constexpr int NF = 2, NS = 1;
+  static constexpr int NF = 2, NS = 1, NM = 2;
   template <class R>
-  __device__ __forceinline__ static void pre_all(const R& rd, const PV& V, const Shot2D& S, const float* a, int i,
-                                                 int j, float* v) {
-    const unsigned c = (unsigned)(i + j * S.nx);
-    v[0] = Cvt<T>::ld(V.old[0], c, V.odec[0]);
-    v[1] = Cvt<T>::ld(V.old[1], c, V.odec[1]);
-    if (i < N || i >= S.nx - N || j < N || j >= S.nz - N) return;
+  __device__ __forceinline__ static void upd(const R& rd, float dtx, const float* a, int i, int j,
+                                             const float* o, const float* m, float* v) {
     float dx = 0.0f, dz = 0.0f;
 #pragma unroll
     for (int l = 1; l <= N; ++l) {
@@ -135,20 +134,17 @@ struct Phys<kAcVel, N, T> {
       dx += cl * (rd(0, i + l - 1, j) - rd(0, i - l, j));
       dz += cl * (rd(0, i, j + l) - rd(0, i, j - l + 1));
     }
-    v[0] = v[0] + (Cvt<T>::ld(V.mat[0], c, V.mdec[0]) * S.dtx) * dx;
-    v[1] = v[1] + (Cvt<T>::ld(V.mat[1], c, V.mdec[1]) * S.dtx) * dz;
+    v[0] = o[0] + (m[0] * dtx) * dx;
+    v[1] = o[1] + (m[1] * dtx) * dz;
   }
 };
 
 template <int N, class T>
 struct Phys<kAcPrs, N, T> {
-  static constexpr int NF = 1, NS = 2;
+  static constexpr int NF = 1, NS = 2, NM = 1;
   template <class R>
-  __device__ __forceinline__ static void pre_all(const R& rd, const PV& V, const Shot2D& S, const float* a, int i,
-                                                 int j, float* v) {
-    const unsigned c = (unsigned)(i + j * S.nx);
-    v[0] = Cvt<T>::ld(V.old[0], c, V.odec[0]);
-    if (i < N || i >= S.nx - N || j < N || j >= S.nz - N) return;
+  __device__ __forceinline__ static void upd(const R& rd, float dtx, const float* a, int i, int j,
+                                             const float* o, const float* m, float* v) {
     float dvxdx = 0.0f, dvzdz = 0.0f;
 #pragma unroll
     for (int l = 1; l <= N; ++l) {
@@ -156,20 +152,16 @@ struct Phys<kAcPrs, N, T> {
       dvxdx += cl * (rd(0, i + l, j) - rd(0, i - l + 1, j));
       dvzdz += cl * (rd(1, i, j + l - 1) - rd(1, i, j - l));
     }
-    v[0] = v[0] + Cvt<T>::ld(V.mat[0], c, V.mdec[0]) * (dvxdx * S.dtx + dvzdz * S.dtx);
+    v[0] = o[0] + m[0] * (dvxdx * dtx + dvzdz * dtx);
   }
 };
 
 template <int N, class T>
 struct Phys<kElVel, N, T> {
-  static constexpr int NF = 2, NS = 3;
+  static constexpr int NF = 2, NS = 3, NM = 2;
   template <class R>
-  __device__ __forceinline__ static void pre_all(const R& rd, const PV& V, const Shot2D& S, const float* a, int i,
-                                                 int j, float* v) {
-    const unsigned c = (unsigned)(i + j * S.nx);
-    v[0] = Cvt<T>::ld(V.old[0], c, V.odec[0]);
-    v[1] = Cvt<T>::ld(V.old[1], c, V.odec[1]);
-    if (i < N || i >= S.nx - N || j < N || j >= S.nz - N) return;
+  __device__ __forceinline__ static void upd(const R& rd, float dtx, const float* a, int i, int j,
+                                             const float* o, const float* m, float* v) {
     float dtxxdx = 0.0f, dtxzdz = 0.0f, dtxzdx = 0.0f, dtzzdz = 0.0f;
 #pragma unroll
     for (int l = 1; l <= N; ++l) {
@@ -179,23 +171,18 @@ struct Phys<kElVel, N, T> {
       dtxzdx += cl * (rd(2, i + l, j) - rd(2, i - l + 1, j));
       dtzzdz += cl * (rd(1, i, j + l) - rd(1, i, j - l + 1));
     }
-    const float dt = S.dtx;
-    v[0] = v[0] + Cvt<T>::ld(V.mat[0], c, V.mdec[0]) * (dt * dtxxdx + dt * dtxzdz);
-    v[1] = v[1] + Cvt<T>::ld(V.mat[1], c, V.mdec[1]) * (dt * dtxzdx + dt * dtzzdz);
+    const float dt = dtx;
+    v[0] = o[0] + m[0] * (dt * dtxxdx + dt * dtxzdz);
+    v[1] = o[1] + m[1] * (dt * dtxzdx + dt * dtzzdz);
   }
 };
 
 template <int N, class T>
 struct Phys<kElStr, N, T> {
-  static constexpr int NF = 3, NS = 2;
+  static constexpr int NF = 3, NS = 2, NM = 3;
   template <class R>
-  __device__ __forceinline__ static void pre_all(const R& rd, const PV& V, const Shot2D& S, const float* a, int i,
-                                                 int j, float* v) {
-    const unsigned c = (unsigned)(i + j * S.nx);
-    v[0] = Cvt<T>::ld(V.old[0], c, V.odec[0]);
-    v[1] = Cvt<T>::ld(V.old[1], c, V.odec[1]);
-    v[2] = Cvt<T>::ld(V.old[2], c, V.odec[2]);
-    if (i < N || i >= S.nx - N || j < N || j >= S.nz - N) return;
+  __device__ __forceinline__ static void upd(const R& rd, float dtx, const float* a, int i, int j,
+                                             const float* o, const float* m, float* v) {
     float dvxdx = 0.0f, dvzdz = 0.0f, dvxdz = 0.0f, dvzdx = 0.0f;
 #pragma unroll
     for (int l = 1; l <= N; ++l) {
@@ -205,13 +192,22 @@ struct Phys<kElStr, N, T> {
       dvxdz += cl * (rd(0, i, j + l) - rd(0, i, j - l + 1));
       dvzdx += cl * (rd(1, i + l - 1, j) - rd(1, i - l, j));
     }
-    const float dt = S.dtx;
-    const float lv = Cvt<T>::ld(V.mat[0], c, V.mdec[0]), l2m = Cvt<T>::ld(V.mat[1], c, V.mdec[1]);
-    v[0] = v[0] + ((l2m * dvxdx) * dt + (lv * dvzdz) * dt);
-    v[1] = v[1] + ((lv * dvxdx) * dt + (l2m * dvzdz) * dt);
-    v[2] = v[2] + Cvt<T>::ld(V.mat[2], c, V.mdec[2]) * (dvxdz * dt + dvzdx * dt);
+    const float dt = dtx;
+    const float lv = m[0], l2m = m[1];
+    v[0] = o[0] + ((l2m * dvxdx) * dt + (lv * dvzdz) * dt);
+    v[1] = o[1] + ((lv * dvxdx) * dt + (l2m * dvzdz) * dt);
+    v[2] = o[2] + m[2] * (dvxdz * dt + dvzdx * dt);
   }
 };
+
+// Old values and materials of cell c (decoded).
+template <class P, class T>
+__device__ __forceinline__ void load_om(const PV& V, unsigned c, float* o, float* m) {
+#pragma unroll
+  for (int f = 0; f < P::NF; ++f) o[f] = Cvt<T>::ld(V.old[f], c, V.odec[f]);
+#pragma unroll
+  for (int k = 0; k < P::NM; ++k) m[k] = Cvt<T>::ld(V.mat[k], c, V.mdec[k]);
+}
 
 // Which output fields receive the raw-add source after the HABC.
 template <int PH>
@@ -235,58 +231,123 @@ __device__ __forceinline__ void inject(const Shot2D& S, int mask, int c, int it,
   }
 }
 
-// Stencil data in a circular window of 16 rows (N <= 6): row j lives in slot (j - jb) & 15.
-template <int RW>
+// Stencil data in a circular window of RS rows: row j lives in slot (j - jb) & (RS - 1).
+template <int RW, int RS>
 struct WinRd {
   const float* t;
   int oi, jb;
   __device__ __forceinline__ float operator()(int k, int i, int j) const {
-    return t[k * 16 * RW + ((j - jb) & 15) * RW + (i - oi)];
+    return t[k * RS * RW + ((j - jb) & (RS - 1)) * RW + (i - oi)];
   }
 };
 
-// Frame tile: at most 64 x kFrameRows cells of the absorbing frame.  Stencil inputs are staged
-// with an (N+1) halo; pre-correction values of the tile plus a one-cell ring go to LDS; each
-// frame cell then blends with its inward neighbour's pre-correction value.
+// Frame tile: at most 64 x kFrameRows cells of the absorbing frame.  Every global load of the tile
+// is issued at once: the stencil fields with an (N+1) halo, old values and materials of the tile
+// plus a one-cell ring, and the old values each frame cell's Higdon term needs.  Pre-correction
+// values of the tile and ring then go to LDS, and each frame cell blends with its inward
+// neighbour's pre-correction value.
 template <int PH, int N, class T>
 __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, const Shot2D& S, const PV& V,
-                                           float* lds) {
+                                           float* lds, int it) {
   using P = Phys<PH, N, T>;
-  constexpr int NF = P::NF, NS = P::NS;
-  constexpr int BX = kTileW, BZ = kFrameRows, ZPT = BZ / 4;
+  constexpr int NF = P::NF, NS = P::NS, NM = P::NM;
+  constexpr int BX = kTileW, BZ = kFrameRows, ZPT = BZ / 4, NTH = BX * 4;
   constexpr int TW = BX + 2 * N + 2, TH = BZ + 2 * N + 2;
   constexpr int EW = BX + 2, EH = BZ + 2;
+  constexpr int KS = (TW * TH + NTH - 1) / NTH;  // staged cells per thread
+  constexpr int KR = (EW * EH + NTH - 1) / NTH;  // tile + ring cells per thread
   float* tile = lds;
   float* pre_s = lds + NS * TW * TH;
   const int nx = S.nx, nz = S.nz, H = S.H;
+  const float dtx = S.dtx, q = S.q, qt = S.qt, qxt = S.qxt;
+  const float* damp = S.damp;
   const int i0 = W.i0, j0 = W.j0;
   const int tid = threadIdx.x + threadIdx.y * BX;
   const int oi = i0 - N - 1, oj = j0 - N - 1;
-  for (int idx = tid; idx < TW * TH; idx += BX * 4) {
+  // 1. issue every load
+  float st[KS][NS];
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const int idx = tid + k * NTH;
     const int r = idx / TW, cc = idx - r * TW;
     const int gi = oi + cc, gj = oj + r;
-    const bool in = gi >= 0 && gi < nx && gj >= 0 && gj < nz;
+    const bool in = idx < TW * TH && gi >= 0 && gi < nx && gj >= 0 && gj < nz;
     const unsigned g = in ? (unsigned)(gi + gj * nx) : 0u;
+    // Unconditional load of a valid cell, zeroed by a multiply outside the grid: a load whose
+    // value is selected would be moved under a branch and waited for there.  (Cells outside the
+    // grid never enter an update; zeroing them only keeps the staged tile clean.)
+    const float zm = in ? 1.0f : 0.0f;
 #pragma unroll
-    for (int k = 0; k < NS; ++k) tile[k * TW * TH + idx] = in ? Cvt<T>::ld(V.sten[k], g, V.sdec[k]) : 0.0f;
+    for (int f = 0; f < NS; ++f) st[k][f] = Cvt<T>::ld(V.sten[f], g, V.sdec[f]) * zm;
   }
-  __syncthreads();
-  const TileRd<TW, TH> rd{tile, oi, oj};
-  for (int idx = tid; idx < EW * EH; idx += BX * 4) {
+  float ro[KR][NF], rm[KR][NM];
+#pragma unroll
+  for (int k = 0; k < KR; ++k) {
+    const int idx = tid + k * NTH;
     const int r = idx / EW, cc = idx - r * EW;
     const int gi = i0 - 1 + cc, gj = j0 - 1 + r;
-    if (gi < 0 || gi >= nx || gj < 0 || gj >= nz || cc > W.w + 1 || r > W.h + 1) continue;
+    const bool ok = idx < EW * EH && gi >= 0 && gi < nx && gj >= 0 && gj < nz && cc <= W.w + 1 && r <= W.h + 1;
+    const unsigned g = ok ? (unsigned)(gi + gj * nx) : 0u;
+    load_om<P, T>(V, g, ro[k], rm[k]);
+  }
+  const int i = i0 + (int)threadIdx.x;
+  const bool lane_ok = (int)threadIdx.x < W.w;
+  // old values at the cell and at its inward x / z neighbours, and the blend weights
+  float fo[ZPT][NF], fx[ZPT][NF], fz[ZPT][NF], wt[ZPT][NF], dm[ZPT];
+#pragma unroll
+  for (int k = 0; k < ZPT; ++k) {
+    const int jl = (int)threadIdx.y * ZPT + k;
+    const int j = j0 + jl;
+    const bool ok = lane_ok && jl < W.h;
+    const int dist = ok ? min(min(i, nx - 1 - i), min(j, nz - 1 - j)) : 0;
+    const bool Lf = i <= H, Rf = i >= nx - 1 - H, Tf = j <= H, Bf = j >= nz - 1 - H;
+    const bool blend = ok && !damp && dist >= 1 && dist <= H;
+    const bool inx = blend && (Lf || Rf), inz = blend && (Tf || Bf);
+    const unsigned c = ok ? (unsigned)(i + j * nx) : 0u;
+    const unsigned cx = inx ? (unsigned)(i + (Lf ? 1 : -1) + j * nx) : c;
+    const unsigned cz = inz ? (unsigned)(i + (j + (Tf ? 1 : -1)) * nx) : c;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      fo[k][f] = Cvt<T>::ld(V.old[f], c, V.odec[f]);
+      fx[k][f] = Cvt<T>::ld(V.old[f], cx, V.odec[f]);
+      fz[k][f] = Cvt<T>::ld(V.old[f], cz, V.odec[f]);
+      wt[k][f] = blend ? (V.alpha[f] ? fminf(fmaxf(V.lut[f][dist] + V.alpha[f][c], 0.0f), 1.0f) : V.lut[f][dist])
+                       : 1.0f;
+    }
+    dm[k] = (ok && damp) ? G(damp)[dist] : 1.0f;
+  }
+  // 2. stage the stencil fields
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const int idx = tid + k * NTH;
+    if (idx < TW * TH) {
+#pragma unroll
+      for (int f = 0; f < NS; ++f) tile[f * TW * TH + idx] = st[k][f];
+    }
+  }
+  __syncthreads();
+  // 3. pre-correction values of the tile and its ring
+  const TileRd<TW, TH> rd{tile, oi, oj};
+#pragma unroll
+  for (int k = 0; k < KR; ++k) {
+    const int idx = tid + k * NTH;
+    const int r = idx / EW, cc = idx - r * EW;
+    const int gi = i0 - 1 + cc, gj = j0 - 1 + r;
+    if (idx >= EW * EH || gi < 0 || gi >= nx || gj < 0 || gj >= nz || cc > W.w + 1 || r > W.h + 1) continue;
     float v[NF];
-    P::pre_all(rd, V, S, A.a, gi, gj, v);
+    if (gi < N || gi >= nx - N || gj < N || gj >= nz - N) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) v[f] = ro[k][f];
+    } else {
+      P::upd(rd, dtx, A.a, gi, gj, ro[k], rm[k], v);
+    }
 #pragma unroll
     for (int f = 0; f < NF; ++f) pre_s[f * EW * EH + idx] = v[f];
   }
   __syncthreads();
-  if ((int)threadIdx.x >= W.w) return;
-  const int i = i0 + (int)threadIdx.x;
-  const int it = *A.step;
-  const int mask = inj_mask<PH>(S);
-  const float q = S.q, qt = S.qt, qxt = S.qxt;
+  // 4. blend and store (no source lies in the frame: sources are validated against the user's
+  // interior, which interior strips cover)
+  if (!lane_ok) return;
 #pragma unroll
   for (int k = 0; k < ZPT; ++k) {
     const int jl = (int)threadIdx.y * ZPT + k;
@@ -298,11 +359,10 @@ __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, 
 #pragma unroll
     for (int f = 0; f < NF; ++f) v[f] = pre_s[f * EW * EH + lj * EW + li];
     const int dist = min(min(i, nx - 1 - i), min(j, nz - 1 - j));
-    if (S.damp) {
+    if (damp) {
       // Cerjan sponge, src/boundary/sponge.jl:56-78: field *= damp after the update
-      const float dm = G(S.damp)[dist];
 #pragma unroll
-      for (int f = 0; f < NF; ++f) v[f] = v[f] * dm;
+      for (int f = 0; f < NF; ++f) v[f] = v[f] * dm[k];
     } else if (dist >= 1 && dist <= H) {
       // Region flags (0-based): left i <= H, right i >= nx-1-H, top j <= H, bottom j >= nz-1-H.
       const bool Lf = i <= H, Rf = i >= nx - 1 - H, Tf = j <= H, Bf = j >= nz - 1 - H;
@@ -310,29 +370,26 @@ __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, 
       const int di = Lf ? 1 : -1, dj = Tf ? 1 : -1;
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
-        const void* fo = V.old[f];
-        const float od = V.odec[f];
-        const float foc = Cvt<T>::ld(fo, c, od);
+        const float foc = fo[k][f];
         float sx = 0.0f, sz = 0.0f;
         if (inx) {
           const float fn = pre_s[f * EW * EH + lj * EW + li + di];
-          sx = (-q * fn - qt * foc) - qxt * Cvt<T>::ld(fo, (unsigned)(i + di + j * nx), od);
+          sx = (-q * fn - qt * foc) - qxt * fx[k][f];
         }
         if (inz) {
           const float fn = pre_s[f * EW * EH + (lj + dj) * EW + li];
-          sz = (-q * fn - qt * foc) - qxt * Cvt<T>::ld(fo, (unsigned)(i + (j + dj) * nx), od);
+          sz = (-q * fn - qt * foc) - qxt * fz[k][f];
         }
-        const float wt = V.alpha[f] ? fminf(fmaxf(V.lut[f][dist] + V.alpha[f][c], 0.0f), 1.0f) : V.lut[f][dist];
+        const float w = wt[k][f];
         if (V.sens[f]) V.sens[f][c] = v[f] - (inx && inz ? 0.5f * (sx + sz) : (inx ? sx : sz));
         if (inx && inz)
-          v[f] = wt * v[f] + ((1.0f - wt) * 0.5f) * (sx + sz);
+          v[f] = w * v[f] + ((1.0f - w) * 0.5f) * (sx + sz);
         else if (inx)
-          v[f] = wt * v[f] + (1.0f - wt) * sx;
+          v[f] = w * v[f] + (1.0f - w) * sx;
         else
-          v[f] = wt * v[f] + (1.0f - wt) * sz;
+          v[f] = w * v[f] + (1.0f - w) * sz;
       }
     }
-    inject<NF>(S, mask, (int)c, it, A.nt, v);
 #pragma unroll
     for (int f = 0; f < NF; ++f) Cvt<T>::st(V.out[f], c, v[f], V.oenc[f]);
   }
@@ -340,92 +397,139 @@ __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, 
 
 // Interior strip: at most 64 columns by W.h rows of the user's interior (no frame cell, every
 // cell in the update region).  The block walks down z four rows at a time (one row per wave)
-// through a 16-row circular LDS window of the stencil inputs; the rows of the next step are
-// loaded into registers while the current rows are computed.
+// through a circular LDS window of RS rows holding the stencil inputs.  While the current rows
+// are computed, the stencil rows, old values and materials of the next four rows are in flight,
+// so each block waits on one round of loads per four rows; the window is large enough that the
+// rows being written never overlap the rows being read, so one barrier per four rows suffices.
+template <int N>
+struct StripRing {
+  static constexpr int RS = (8 + 2 * N) <= 16 ? 16 : 32;
+};
+
 template <int PH, int N, class T>
 __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D& W, const Shot2D& S, const PV& V,
-                                               float* lds) {
+                                               float* lds, int it) {
   using P = Phys<PH, N, T>;
-  constexpr int NF = P::NF, NS = P::NS;
-  constexpr int BX = kTileW, RW = BX + 2 * N;
-  static_assert(4 + 2 * N <= 16, "window of 16 rows");
+  constexpr int NF = P::NF, NS = P::NS, NM = P::NM;
+  constexpr int BX = kTileW, RW = BX + 2 * N, RS = StripRing<N>::RS;
+  constexpr int W0 = 4 + 2 * N, R0 = (W0 + 3) / 4;  // initial window rows, per thread row
   const int nx = S.nx, nz = S.nz;
+  const float dtx = S.dtx;
   const int i0 = W.i0, j0 = W.j0, jend = W.j0 + W.h;
   const int tx = threadIdx.x, ty = threadIdx.y;
+  // Rows of this strip where this lane's column holds a source, scanned before any other load so
+  // that the row loop never waits on the source table.
+  const int mask = inj_mask<PH>(S);
+  int src_lo = 1 << 30, src_hi = -1;
+  if (mask && it < A.nt) {
+    const int n_src = S.n_src;
+    const int* src_c = S.src_c;
+    const int ic = i0 + tx;
+    for (int q = 0; q < n_src; ++q) {
+      const int cs = G(src_c)[q];
+      const int js = cs / nx, is = cs - js * nx;
+      if (is == ic && js >= j0 && js < jend) {
+        src_lo = min(src_lo, js);
+        src_hi = max(src_hi, js);
+      }
+    }
+  }
   const int oi = i0 - N, jb = j0 - N;
   // columns loaded by this lane: oi + tx and, for tx < 2N, oi + 64 + tx
   const int ca = oi + tx, cb = oi + BX + tx;
   const bool has_b = tx < 2 * N;
   const bool ina = ca >= 0 && ca < nx, inb = has_b && cb >= 0 && cb < nx;
+  // Unconditional loads of valid cells; cells outside the grid are zeroed by a multiply when the
+  // row is written to LDS (see frame_tile), so nothing between the loads and that write waits.
   auto load_row = [&](int j, float (&ra)[NS], float (&rb)[NS]) {
     const bool inj = j >= 0 && j < nz;
     const unsigned rowo = (unsigned)((inj ? j : 0) * nx);
+    const unsigned ga = rowo + (unsigned)(ina ? ca : 0), gb = rowo + (unsigned)(inb ? cb : 0);
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-      ra[k] = (inj && ina) ? Cvt<T>::ld(V.sten[k], rowo + (unsigned)ca, V.sdec[k]) : 0.0f;
-      rb[k] = (inj && inb) ? Cvt<T>::ld(V.sten[k], rowo + (unsigned)cb, V.sdec[k]) : 0.0f;
+      ra[k] = Cvt<T>::ld(V.sten[k], ga, V.sdec[k]);
+      rb[k] = Cvt<T>::ld(V.sten[k], gb, V.sdec[k]);
     }
   };
   auto store_row = [&](int j, const float (&ra)[NS], const float (&rb)[NS]) {
-    const int slot = (j - jb) & 15;
+    const bool inj = j >= 0 && j < nz;
+    const float za = (inj && ina) ? 1.0f : 0.0f, zb = (inj && inb) ? 1.0f : 0.0f;
+    const int slot = (j - jb) & (RS - 1);
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-      lds[k * 16 * RW + slot * RW + tx] = ra[k];
-      if (has_b) lds[k * 16 * RW + slot * RW + BX + tx] = rb[k];
+      lds[k * RS * RW + slot * RW + tx] = ra[k] * za;
+      if (has_b) lds[k * RS * RW + slot * RW + BX + tx] = rb[k] * zb;
     }
   };
-  // initial window: rows j0-N .. j0+3+N
-  for (int r = ty; r < 4 + 2 * N; r += 4) {
-    float ra[NS], rb[NS];
-    load_row(jb + r, ra, rb);
-    store_row(jb + r, ra, rb);
-  }
-  __syncthreads();
-  const WinRd<RW> rd{lds, oi, jb};
-  const int it = *A.step;
-  const int mask = inj_mask<PH>(S);
   const int i = i0 + tx;
   const bool col_ok = tx < W.w;
+  // initial window (rows j0-N .. j0+3+N) and the old values / materials of the first rows
+  {
+    float ia[R0][NS], ib[R0][NS];
+#pragma unroll
+    for (int k = 0; k < R0; ++k) load_row(ty + 4 * k < W0 ? jb + ty + 4 * k : -1, ia[k], ib[k]);
+#pragma unroll
+    for (int k = 0; k < R0; ++k)
+      if (ty + 4 * k < W0) store_row(jb + ty + 4 * k, ia[k], ib[k]);
+  }
+  float o[NF], m[NM];
+  {
+    const int j = j0 + ty;
+    const unsigned c = (col_ok && j < jend) ? (unsigned)(i + j * nx) : 0u;
+    load_om<P, T>(V, c, o, m);
+  }
+  __syncthreads();
+  const WinRd<RW, RS> rd{lds, oi, jb};
   for (int jr = j0; jr < jend; jr += 4) {
     const bool more = jr + 4 < jend;
     const int jn = jr + 4 + N + ty;
-    float na[NS], nb[NS];
-    if (more) load_row(jn, na, nb);
     const int j = jr + ty;
+    float na[NS], nb[NS], no[NF], nm[NM];
+    if (more) {
+      load_row(jn, na, nb);
+      const unsigned cn = (col_ok && j + 4 < jend) ? (unsigned)(i + (j + 4) * nx) : 0u;
+      load_om<P, T>(V, cn, no, nm);
+    }
     if (col_ok && j < jend) {
       float v[NF];
-      P::pre_all(rd, V, S, A.a, i, j, v);
+      P::upd(rd, dtx, A.a, i, j, o, m, v);
       const unsigned c = (unsigned)(i + j * nx);
-      inject<NF>(S, mask, (int)c, it, A.nt, v);
+      if (j >= src_lo && j <= src_hi) inject<NF>(S, mask, (int)c, it, A.nt, v);
 #pragma unroll
       for (int f = 0; f < NF; ++f) Cvt<T>::st(V.out[f], c, v[f], V.oenc[f]);
     }
-    __syncthreads();
-    if (more) store_row(jn, na, nb);
+    if (!more) break;
+    store_row(jn, na, nb);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) o[f] = no[f];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) m[k] = nm[k];
     __syncthreads();
   }
 }
 
 // The host lays the work list out per XCD: hardware hands block b to XCD b % 8, and the list
 // gives each XCD a contiguous, cost-balanced run of the grid so that halo rows stay in its L2.
-// Padding entries have kind -1.
+// Padding entries have kind -1.  The step index is read before the first barrier, so every
+// thread of every block has read it before any block can finish.
 template <int PH, int N, class T>
 __global__ __launch_bounds__(256) void sweep2d_kernel(const SweepArgs A) {
   using P = Phys<PH, N, T>;
   constexpr int NF = P::NF, NS = P::NS;
   constexpr int FT = NS * (kTileW + 2 * N + 2) * (kFrameRows + 2 * N + 2) + NF * (kTileW + 2) * (kFrameRows + 2);
-  constexpr int IS = NS * 16 * (kTileW + 2 * N);
+  constexpr int IS = NS * StripRing<N>::RS * (kTileW + 2 * N);
   __shared__ float lds[FT > IS ? FT : IS];
   if ((int)blockIdx.x >= A.n_work) return;
   const Work2D W = A.work[blockIdx.x];
   if (W.kind < 0) return;
+  const int it = *A.step;
   const Shot2D& S = A.shots[W.shot];
   PV V;
   make_view<PH>(S, A.parity, V);
   if (W.kind == 0)
-    frame_tile<PH, N, T>(A, W, S, V, lds);
+    frame_tile<PH, N, T>(A, W, S, V, lds, it);
   else
-    interior_strip<PH, N, T>(A, W, S, V, lds);
+    interior_strip<PH, N, T>(A, W, S, V, lds, it);
 }
 
 template <class T>
