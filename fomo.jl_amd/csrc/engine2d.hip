@@ -508,10 +508,20 @@ __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D&
   }
 }
 
+// Receiver entry e: the value of every slot's field in buffer [parity] into column col
+// (src/acquisition/device_indexed.jl:100-134).
+template <class T>
+__device__ __forceinline__ void gather_entry(const GatherArgs& g, int e, int col, int parity) {
+  if (e >= g.n_entries || col < 0 || col >= g.nt) return;
+  const Shot2D& S = g.shots[g.rec_shot[e]];
+  const unsigned c = (unsigned)g.rec_c[e];
+  const long long o = g.rec_out[e] + (long long)g.rec_stride[e] * col;
+  for (int k = 0; k < g.nslots; ++k) G(g.seis[k])[o] = Cvt<T>::ld(S.fld[parity][k], c, S.fdec[k]);
+}
+
 // The host lays the work list out per XCD: hardware hands block b to XCD b % 8, and the list
 // gives each XCD a contiguous, cost-balanced run of the grid so that halo rows stay in its L2.
-// Padding entries have kind -1.  The step index is read before the first barrier, so every
-// thread of every block has read it before any block can finish.
+// Padding entries have kind -1.  Blocks past the work list record the previous step's receivers.
 template <int PH, int N, class T>
 __global__ __launch_bounds__(256) void sweep2d_kernel(const SweepArgs A) {
   using P = Phys<PH, N, T>;
@@ -519,10 +529,14 @@ __global__ __launch_bounds__(256) void sweep2d_kernel(const SweepArgs A) {
   constexpr int FT = NS * (kTileW + 2 * N + 2) * (kFrameRows + 2 * N + 2) + NF * (kTileW + 2) * (kFrameRows + 2);
   constexpr int IS = NS * StripRing<N>::RS * (kTileW + 2 * N);
   __shared__ float lds[FT > IS ? FT : IS];
-  if ((int)blockIdx.x >= A.n_work) return;
+  const int it = *A.step + A.koff;
+  if ((int)blockIdx.x >= A.n_work) {
+    const int e = ((int)blockIdx.x - A.n_work) * kGatherBlock + (int)(threadIdx.x + threadIdx.y * kTileW);
+    gather_entry<T>(A.g, e, it - 1, A.parity);
+    return;
+  }
   const Work2D W = A.work[blockIdx.x];
   if (W.kind < 0) return;
-  const int it = *A.step;
   const Shot2D& S = A.shots[W.shot];
   PV V;
   make_view<PH>(S, A.parity, V);
@@ -533,27 +547,8 @@ __global__ __launch_bounds__(256) void sweep2d_kernel(const SweepArgs A) {
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void gather2d_kernel(const GatherArgs A) {
-  const int it = *A.step;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < A.n_entries && it < A.nt) {
-    const Shot2D& S = A.shots[A.rec_shot[e]];
-    const unsigned c = (unsigned)A.rec_c[e];
-    const long long o = A.rec_out[e] + (long long)A.rec_stride[e] * it;
-    for (int k = 0; k < A.nslots; ++k) {
-      const int f = A.field_of_slot[k];
-      G(A.seis[k])[o] = Cvt<T>::ld(S.fld[A.parity_out][f], c, S.fdec[f]);
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned t = atomicAdd(A.done, 1u);
-    if (t == gridDim.x - 1) {
-      atomicExch(A.done, 0u);
-      atomicExch(A.step, it + 1);
-    }
-  }
+__global__ __launch_bounds__(256) void gather2d_kernel(const GatherArgs g, const int* step, int koff, int parity) {
+  gather_entry<T>(g, (int)(blockIdx.x * blockDim.x + threadIdx.x), *step + koff - 1, parity);
 }
 
 template <class T>
@@ -566,7 +561,8 @@ int strip_rows(int tile_cfg) { return tile_cfg == 0 ? 16 : (tile_cfg == 1 ? 32 :
 
 template <int PH, int N, class T>
 static hipError_t launch_cfg(const SweepArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((sweep2d_kernel<PH, N, T>), dim3(a.n_work), dim3(kTileW, 4), 0, s, a);
+  static_assert(kTileW * 4 == kGatherBlock, "one receiver entry per thread of a gather block");
+  hipLaunchKernelGGL((sweep2d_kernel<PH, N, T>), dim3(a.n_work + a.n_gather), dim3(kTileW, 4), 0, s, a);
   return hipGetLastError();
 }
 
@@ -596,7 +592,7 @@ static hipError_t launch_t2(int phase, int N, const SweepArgs& a, hipStream_t s)
 // tile_cfg only shapes the work list (the strip length lives in each item).
 hipError_t launch_sweep2d(int storage, int phase, int N, int tile_cfg, const SweepArgs& a, hipStream_t s) {
   (void)tile_cfg;
-  if (a.n_work <= 0) return hipSuccess;
+  if (a.n_work + a.n_gather <= 0) return hipSuccess;
   switch (storage) {
     case FOMO_STORE_F32: return launch_t2<float>(phase, N, a, s);
     case FOMO_STORE_F16: return launch_t2<__half>(phase, N, a, s);
@@ -605,12 +601,20 @@ hipError_t launch_sweep2d(int storage, int phase, int N, int tile_cfg, const Swe
   }
 }
 
-hipError_t launch_gather2d(int storage, const GatherArgs& a, hipStream_t s) {
-  const int blocks = a.n_entries > 0 ? (a.n_entries + 255) / 256 : 1;
+hipError_t launch_gather2d(int storage, const GatherArgs& g, const int* step, int koff, int parity, hipStream_t s) {
+  if (g.n_entries <= 0) return hipSuccess;
+  const int blocks = (g.n_entries + kGatherBlock - 1) / kGatherBlock;
   switch (storage) {
-    case FOMO_STORE_F32: hipLaunchKernelGGL(gather2d_kernel<float>, dim3(blocks), dim3(256), 0, s, a); break;
-    case FOMO_STORE_F16: hipLaunchKernelGGL(gather2d_kernel<__half>, dim3(blocks), dim3(256), 0, s, a); break;
-    case FOMO_STORE_BF16: hipLaunchKernelGGL(gather2d_kernel<__hip_bfloat16>, dim3(blocks), dim3(256), 0, s, a); break;
+    case FOMO_STORE_F32:
+      hipLaunchKernelGGL(gather2d_kernel<float>, dim3(blocks), dim3(kGatherBlock), 0, s, g, step, koff, parity);
+      break;
+    case FOMO_STORE_F16:
+      hipLaunchKernelGGL(gather2d_kernel<__half>, dim3(blocks), dim3(kGatherBlock), 0, s, g, step, koff, parity);
+      break;
+    case FOMO_STORE_BF16:
+      hipLaunchKernelGGL(gather2d_kernel<__hip_bfloat16>, dim3(blocks), dim3(kGatherBlock), 0, s, g, step, koff,
+                         parity);
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

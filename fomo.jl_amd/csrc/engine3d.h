@@ -56,7 +56,8 @@ struct Sweep3DArgs {
   int n_work;
   int parity;
   float a[kMaxN];
-  const int* step;
+  const int* step;  // device step counter; the step index is *step + koff
+  int koff;
   int nt;
 };
 
@@ -67,10 +68,10 @@ struct Gather3DArgs {
   int slot_field[4];
   int nslots;
   int n_rec;
-  int parity_out;
+  int parity_out;  // buffer holding the state after the step
   int nt;
-  int* step;
-  unsigned* done;
+  const int* step;  // the state after step *step + koff goes into that column
+  int koff;
 };
 
 // eq: FOMO_ACOUSTIC3D / FOMO_ELASTIC3D; storage: FOMO_STORE_F32 / F16 / BF16; phase 0 velocity,

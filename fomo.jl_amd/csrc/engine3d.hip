@@ -150,7 +150,7 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
   const int i = pi0 + tx;
   const float* __restrict__ a = A.a;
   const float dtdh = S.dtdh;
-  const int it = *A.step;
+  const int it = *A.step + A.koff;
   const bool i_upd = i >= N && i < nx - N;
 
   // loads of field f at (ii, jj) of plane kk: scalar plane base + 32-bit in-plane offset
@@ -471,7 +471,7 @@ __global__ __launch_bounds__((kTileW * kTR<PH, LAG>)) void sweep3d_kernel(const 
 
 template <class T>
 __global__ __launch_bounds__(256) void gather3d_kernel(const Gather3DArgs A) {
-  const int it = *A.step;
+  const int it = *A.step + A.koff;
   const Shot3D& S = A.shot;
   if (it < A.nt) {
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < A.n_rec; r += gridDim.x * blockDim.x) {
@@ -481,15 +481,6 @@ __global__ __launch_bounds__(256) void gather3d_kernel(const Gather3DArgs A) {
         const int f = A.slot_field[k];
         G(A.seis[k])[o] = Cvt<T>::ld(S.fld[A.parity_out][f], c, S.fdec[f]);
       }
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned t = atomicAdd(A.done, 1u);
-    if (t == gridDim.x - 1) {
-      atomicExch(A.done, 0u);
-      atomicExch(A.step, it + 1);
     }
   }
 }

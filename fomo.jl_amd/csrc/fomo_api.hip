@@ -271,11 +271,18 @@ PlanBase::~PlanBase() {
   if (stream) (void)hipStreamDestroy(stream);
 }
 
-// Two steps (parity 0 then 1) are captured into one graph; a replay advances two steps.
+__global__ void tick_kernel(int* step, int m) {
+  if (threadIdx.x == 0) *step += m;
+}
+
+// kGraphSteps steps (parities 0, 1, 0, ... offsets 0 .. kGraphSteps-1) and their counter tick
+// are captured into one graph; a replay advances kGraphSteps steps from an even step.
 int PlanBase::build_graph() {
+  const long saved = steps_done;
+  steps_done = 0;
   FOMO_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-  hipError_t e0 = enqueue_step(0, nullptr);
-  hipError_t e1 = e0 == hipSuccess ? enqueue_step(1, nullptr) : e0;
+  hipError_t e1 = run_steps(kGraphSteps);
+  steps_done = saved;
   hipGraph_t g = nullptr;
   hipError_t ec = hipStreamEndCapture(stream, &g);
   if (e1 != hipSuccess || ec != hipSuccess) {
@@ -287,29 +294,43 @@ int PlanBase::build_graph() {
   return FOMO_OK;
 }
 
-hipError_t PlanBase::enqueue_step(int s, hipEvent_t* ev) {
+hipError_t PlanBase::run_steps(int m, hipEvent_t* ev) {
   hipError_t e = hipSuccess;
-  for (int part = 0; part < 3 && e == hipSuccess; ++part) {
-    if (ev) e = hipEventRecord(ev[part], stream);
-    if (e == hipSuccess) e = enqueue_part(part, s);
+  const bool per_step_gather = !gather_in_sweep();
+  for (int k = 0; k < m && e == hipSuccess; ++k) {
+    const int s = (int)((steps_done + k) % 2);
+    for (int part = 0; part < (per_step_gather ? 3 : 2) && e == hipSuccess; ++part) {
+      if (ev && k == 0 && part < 2) e = hipEventRecord(ev[part], stream);
+      if (e == hipSuccess) e = enqueue_part(part, s, k);
+    }
+    if (e == hipSuccess && ev && k == 0) e = hipEventRecord(ev[2], stream);
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(tick_kernel, dim3(1), dim3(1), 0, stream, step, m);
+    e = hipGetLastError();
   }
   if (e == hipSuccess && ev) e = hipEventRecord(ev[3], stream);
+  if (e == hipSuccess) steps_done += m;
   return e;
 }
 
 int PlanBase::advance(int n) {
   if (n < 0) return set_error(FOMO_EINVAL, "step count must be >= 0");
+  if (n == 0) return FOMO_OK;
   while (n > 0) {
-    if (gexec && (steps_done % 2) == 0 && n >= 2) {
+    if (gexec && (steps_done % 2) == 0 && n >= kGraphSteps) {
       FOMO_HIP_TRY(hipGraphLaunch(gexec, stream));
-      steps_done += 2;
-      n -= 2;
+      steps_done += kGraphSteps;
+      n -= kGraphSteps;
     } else {
-      FOMO_HIP_TRY(enqueue_step((int)(steps_done % 2), nullptr));
-      steps_done += 1;
-      n -= 1;
+      // without a graph: all at once; with one: a single step to reach an even step, or the
+      // tail shorter than a replay
+      const int m = (gexec && (steps_done % 2)) ? 1 : (gexec ? std::min(n, kGraphSteps - 1) : n);
+      FOMO_HIP_TRY(run_steps(m));
+      n -= m;
     }
   }
+  if (gather_in_sweep()) FOMO_HIP_TRY(enqueue_part(2, (int)(steps_done % 2), 0));
   return FOMO_OK;
 }
 
@@ -419,34 +440,31 @@ extern "C" int fomo_plan_kernel_times(fomo_plan* p, int32_t n, double* out3) {
   if (!out3) return fail(FOMO_EINVAL, "NULL argument");
   if (n < 1) return fail(FOMO_EINVAL, "step count must be >= 1");
   PlanBase* b = p->impl;
-  hipEvent_t ev[4];
-  for (int k = 0; k < 4; ++k) HIP_TRY(hipEventCreate(&ev[k]));
-  double acc[3] = {0.0, 0.0, 0.0};
+  hipEvent_t ev[2];
+  for (int k = 0; k < 2; ++k) HIP_TRY(hipEventCreate(&ev[k]));
   // The two sweeps: n back-to-back launches of the same (idempotent) phase between two events.
   const int s = (int)(b->steps_done % 2);
   for (int part = 0; part < 2; ++part) {
-    HIP_TRY(b->enqueue_part(part, s));  // warm
+    HIP_TRY(b->enqueue_part(part, s, 0));  // warm
     HIP_TRY(hipEventRecord(ev[0], b->stream));
-    for (int t = 0; t < n; ++t) HIP_TRY(b->enqueue_part(part, s));
+    for (int t = 0; t < n; ++t) HIP_TRY(b->enqueue_part(part, s, 0));
     HIP_TRY(hipEventRecord(ev[1], b->stream));
     HIP_TRY(hipEventSynchronize(ev[1]));
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
     out3[part] = ms / n;
   }
-  // The gather bumps the step counter: time it inside n ordinary steps.
-  for (int t = 0; t < n; ++t) {
-    HIP_TRY(b->enqueue_step((int)(b->steps_done % 2), ev));
-    b->steps_done += 1;
-    HIP_TRY(hipEventSynchronize(ev[3]));
-    for (int k = 0; k < 3; ++k) {
-      float ms = 0.0f;
-      HIP_TRY(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
-      acc[k] += ms;
-    }
-  }
-  for (int k = 0; k < 4; ++k) (void)hipEventDestroy(ev[k]);
-  out3[2] = acc[2] / n;
+  // Everything else per step (receiver gather, counter ticks, launch gaps): n ordinary steps
+  // through advance, less the two sweeps.
+  HIP_TRY(hipEventRecord(ev[0], b->stream));
+  int rc = b->advance(n);
+  HIP_TRY(hipEventRecord(ev[1], b->stream));
+  HIP_TRY(hipEventSynchronize(ev[1]));
+  float ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+  for (int k = 0; k < 2; ++k) (void)hipEventDestroy(ev[k]);
+  if (rc) return rc;
+  out3[2] = std::max(0.0, (double)ms / n - out3[0] - out3[1]);
   return FOMO_OK;
 }
 

@@ -67,38 +67,42 @@ struct Work2D {
   int shot, kind, i0, j0, w, h;
 };
 
-// Arguments of one sweep launch over all shots.
-struct SweepArgs {
-  const Shot2D* shots;
-  const Work2D* work;
-  int n_work;
-  int parity;           // state parity s: read buffers [s], write buffers [1-s]
-  float a[kMaxN];       // staggered FD coefficients
-  const int* step;      // device step counter (0-based index of the step being taken)
-  int nt;
-};
-
-// Receiver gather after the step over all shots; the last block advances the step counter.
+// Receiver tables over all shots.  Slot k records field k (acoustic p, vx, vz; elastic vx, vz)
+// of the state after step t into column t.
 struct GatherArgs {
   const Shot2D* shots;
   const int* rec_shot;  // shot of each receiver entry
   const int* rec_c;     // padded cell index
   const long long* rec_out;  // offset of (receiver, step 0) in the seismogram buffer
   const int* rec_stride;     // step stride (n_rec of the shot)
-  float* seis[3];       // slot buffers (acoustic p,vx,vz; elastic vx,vz)
-  int field_of_slot[3]; // field index gathered into each slot
+  float* seis[3];       // slot buffers
   int nslots;
   int n_entries;
-  int parity_out;       // buffer holding the state after this step
   int nt;
-  int* step;
-  unsigned* done;
+};
+constexpr int kGatherBlock = 256;  // receiver entries per gather block
+
+// Arguments of one sweep launch over all shots.  The step index is *step + koff.
+struct SweepArgs {
+  const Shot2D* shots;
+  const Work2D* work;
+  int n_work;
+  int parity;           // state parity s: read buffers [s], write buffers [1-s]
+  float a[kMaxN];       // staggered FD coefficients
+  const int* step;      // device step counter
+  int koff;             // offset of this launch's step from the counter
+  int nt;
+  // Velocity phase only: blocks n_work .. n_work + n_gather - 1 record the state held in buffer
+  // [s] (the state after step it - 1) into column it - 1.
+  GatherArgs g;
+  int n_gather;
 };
 
 // Launchers (defined in engine2d.hip).  tile_cfg selects the interior strip length; storage is
 // FOMO_STORE_F32 / F16 / BF16.
 hipError_t launch_sweep2d(int storage, int phase, int N, int tile_cfg, const SweepArgs& a, hipStream_t s);
-hipError_t launch_gather2d(int storage, const GatherArgs& a, hipStream_t s);
+// Record the state held in buffer [parity] into column (*step + koff - 1).
+hipError_t launch_gather2d(int storage, const GatherArgs& g, const int* step, int koff, int parity, hipStream_t s);
 // Decode n stored values (storage type, scale dec) to Float32.
 hipError_t launch_decode(int storage, const void* src, float* dst, size_t n, float dec, hipStream_t s);
 int strip_rows(int tile_cfg);   // rows (z) of one interior strip

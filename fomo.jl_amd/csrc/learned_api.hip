@@ -101,14 +101,12 @@ extern "C" int fomo_learned_train_step(fomo_plan* student, fomo_plan* teacher, i
     return set_error(FOMO_EINVAL, "student grid does not fit in the teacher grid at this offset");
   if (!(scale_p > 0.0f) || !(scale_v > 0.0f)) return set_error(FOMO_EINVAL, "scales must be > 0");
   const int ts = (int)(t->steps_done % 2);
-  FOMO_HIP_TRY(t->enqueue_step(ts, nullptr));
-  t->steps_done += 1;
+  FOMO_HIP_TRY(t->run_steps(1));
   hipEvent_t ev;
   FOMO_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   FOMO_HIP_TRY(hipEventRecord(ev, t->stream));
   const int s = (int)(q->steps_done % 2);
-  FOMO_HIP_TRY(q->enqueue_step(s, nullptr));
-  q->steps_done += 1;
+  FOMO_HIP_TRY(q->run_steps(1));
   FOMO_HIP_TRY(hipStreamWaitEvent(q->stream, ev, 0));
   MlpTrainArgs a{};
   a.m.w_bf16 = q->w_bf16;

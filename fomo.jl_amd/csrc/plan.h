@@ -19,31 +19,44 @@ int set_error(int code, const std::string& msg);
                                std::string(#expr) + ": " + hipGetErrorString(_e));            \
   } while (0)
 
+// Steps per captured graph (even: a replay starts and ends on parity 0).
+constexpr int kGraphSteps = 16;
+
 // Common part of every plan: device allocations owned by the plan, its stream, the step
-// counter, and the captured two-step graph.
+// counter, and the captured multi-step graph.
+//
+// Step indexing: kernels take the index of their step as (*step + koff), with koff a launch
+// argument; the counter moves only by tick launches at the end of a run of steps.  A captured
+// graph of kGraphSteps steps therefore holds every step's offset in its nodes and needs one
+// counter update per replay instead of one per step.
 struct PlanBase {
   fomo_desc d{};
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
   int* step = nullptr;
-  unsigned* done = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   long steps_done = 0;
 
   virtual ~PlanBase();
-  // Enqueue one part of a step from state parity s: 0 velocity phase, 1 pressure/stress phase,
-  // 2 receiver gather (which also bumps the device step counter).  Parts 0 and 1 read only the
-  // parity-s buffers and the step counter besides what they write, so repeating one is idempotent.
-  virtual hipError_t enqueue_part(int part, int s) = 0;
-  // Enqueue one step; ev (4 events, optional) brackets the three parts.
-  hipError_t enqueue_step(int s, hipEvent_t* ev);
+  // Enqueue one part of the step with index *step + koff from state parity s: 0 velocity phase,
+  // 1 pressure/stress phase, 2 receiver gather.  Parts 0 and 1 read only the parity-s buffers and
+  // the step counter besides what they write, so repeating one is idempotent.
+  virtual hipError_t enqueue_part(int part, int s, int koff) = 0;
+  // True when the velocity phase also records the previous step's receivers; part 2 then only
+  // records the latest state (once, at the end of a run of steps).
+  virtual bool gather_in_sweep() const { return false; }
+  // Enqueue m >= 1 steps from the current state and the counter tick; steps_done += m.  ev
+  // (optional, 4 events) brackets the sweeps of the first step: ev[0] velocity ev[1] stress ev[2].
+  hipError_t run_steps(int m, hipEvent_t* ev = nullptr);
   virtual int reset() = 0;
   virtual int read_seis(int shot, float* const* dst) = 0;
   virtual int read_field(int shot, int which, float* out) = 0;
   virtual int dims(int shot, int* nx, int* ny, int* nz, int* pad) const = 0;
   virtual int n_shots() const = 0;
 
+  // Advance n steps (graph replays where they fit), then record the latest state's receivers when
+  // the sweeps record them one step late.
   int advance(int n);
   int build_graph();
   template <class T>

@@ -42,7 +42,8 @@ struct Plan2D : PlanBase {
   int train_blocks = 0;
 
   ~Plan2D() override;
-  hipError_t enqueue_part(int part, int s) override;
+  hipError_t enqueue_part(int part, int s, int koff) override;
+  bool gather_in_sweep() const override { return true; }
   int reset() override;
   int read_seis(int shot, float* const* dst) override;
   int read_field(int shot, int which, float* out) override;

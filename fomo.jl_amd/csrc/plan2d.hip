@@ -68,26 +68,18 @@ int Plan2D::make_work(int cfg) {
   return upload(&d_work[cfg], flat.data(), flat.size());
 }
 
-hipError_t Plan2D::enqueue_part(int part, int s) {
-  if (part == 2) {
-    GatherArgs g{};
-    g.shots = d_shots;
-    g.rec_shot = rec_shot;
-    g.rec_c = rec_c;
-    g.rec_out = rec_out;
-    g.rec_stride = rec_stride;
-    g.nslots = nslots;
-    for (int k = 0; k < nslots; ++k) {
-      g.seis[k] = seis[k];
-      g.field_of_slot[k] = k;  // acoustic p,vx,vz = fields 0,1,2; elastic vx,vz = fields 0,1
-    }
-    g.n_entries = n_entries;
-    g.parity_out = 1 - s;
-    g.nt = d.nt;
-    g.step = step;
-    g.done = done;
-    return launch_gather2d(storage, g, stream);
-  }
+hipError_t Plan2D::enqueue_part(int part, int s, int koff) {
+  GatherArgs g{};
+  g.shots = d_shots;
+  g.rec_shot = rec_shot;
+  g.rec_c = rec_c;
+  g.rec_out = rec_out;
+  g.rec_stride = rec_stride;
+  g.nslots = nslots;
+  for (int k = 0; k < nslots; ++k) g.seis[k] = seis[k];
+  g.n_entries = n_entries;
+  g.nt = d.nt;
+  if (part == 2) return launch_gather2d(storage, g, step, koff, s, stream);
   SweepArgs a{};
   a.shots = d_shots;
   a.work = d_work[tile_cfg];
@@ -95,7 +87,12 @@ hipError_t Plan2D::enqueue_part(int part, int s) {
   a.parity = s;
   for (int l = 0; l < N; ++l) a.a[l] = coef[l];
   a.step = step;
+  a.koff = koff;
   a.nt = d.nt;
+  if (part == 0) {
+    a.g = g;
+    a.n_gather = (n_entries + kGatherBlock - 1) / kGatherBlock;
+  }
   if (part == 1) return launch_sweep2d(storage, d.eq == FOMO_ACOUSTIC2D ? kAcPrs : kElStr, N, tile_cfg, a, stream);
   if (learned) {
     // blend-weight corrections of every frame cell from the state at time n
@@ -128,7 +125,6 @@ int Plan2D::reset() {
   for (int k = 0; k < nslots; ++k)
     if (seis_total) FOMO_HIP_TRY(hipMemsetAsync(seis[k], 0, (size_t)seis_total * sizeof(float), stream));
   FOMO_HIP_TRY(hipMemsetAsync(step, 0, sizeof(int), stream));
-  FOMO_HIP_TRY(hipMemsetAsync(done, 0, sizeof(unsigned), stream));
   steps_done = 0;
   return FOMO_OK;
 }
@@ -218,9 +214,9 @@ int Plan2D::autotune() {
     int rc = make_work(c);
     if (rc) return rc;
     tile_cfg = c;
-    for (int r = 0; r < 2; ++r) FOMO_HIP_TRY(enqueue_step(r & 1, nullptr));
+    FOMO_HIP_TRY(run_steps(2));
     FOMO_HIP_TRY(hipEventRecord(e0, stream));
-    for (int r = 0; r < reps; ++r) FOMO_HIP_TRY(enqueue_step(r & 1, nullptr));
+    FOMO_HIP_TRY(run_steps(reps));
     FOMO_HIP_TRY(hipEventRecord(e1, stream));
     FOMO_HIP_TRY(hipEventSynchronize(e1));
     float ms = 0.0f;
@@ -447,7 +443,7 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
   for (int k = 0; k < p->nslots; ++k)
     if ((rc = p->alloc(&p->seis[k], (size_t)std::max<long long>(seis_off, 1)))) return bail(rc);
   if ((rc = p->upload(&p->d_shots, p->shots.data(), p->shots.size()))) return bail(rc);
-  if ((rc = p->alloc(&p->step, 1)) || (rc = p->alloc(&p->done, 1))) return bail(rc);
+  if ((rc = p->alloc(&p->step, 1))) return bail(rc);
   if ((rc = p->reset())) return bail(rc);
   if (d->tile_cfg > 0 && d->tile_cfg <= kNumTileCfgs) {
     p->tile_cfg = d->tile_cfg - 1;

@@ -46,7 +46,7 @@ struct Plan3D : PlanBase {
   float* scratch = nullptr;  // slice / decode staging
   size_t scratch_n = 0;
 
-  hipError_t enqueue_part(int part, int s) override {
+  hipError_t enqueue_part(int part, int s, int koff) override {
     if (part == 2) {
       Gather3DArgs g{};
       g.shot = shot;
@@ -62,7 +62,7 @@ struct Plan3D : PlanBase {
       g.parity_out = 1 - s;
       g.nt = d.nt;
       g.step = step;
-      g.done = done;
+      g.koff = koff;
       return launch_gather3d(storage, g, stream);
     }
     Sweep3DArgs a{};
@@ -70,6 +70,7 @@ struct Plan3D : PlanBase {
     a.parity = s;
     for (int l = 0; l < N; ++l) a.a[l] = coef[l];
     a.step = step;
+    a.koff = koff;
     a.nt = d.nt;
     Sweep3DArgs f = a;
     a.work = d_work[0];
@@ -87,7 +88,6 @@ struct Plan3D : PlanBase {
     for (int k = 0; k < nslots; ++k)
       if (ns) FOMO_HIP_TRY(hipMemsetAsync(seis[k], 0, ns * sizeof(float), stream));
     FOMO_HIP_TRY(hipMemsetAsync(step, 0, sizeof(int), stream));
-    FOMO_HIP_TRY(hipMemsetAsync(done, 0, sizeof(unsigned), stream));
     steps_done = 0;
     return FOMO_OK;
   }
@@ -497,7 +497,7 @@ int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, con
     for (int k = 0; k < p->nslots; ++k)
       if ((rc = p->alloc(&p->seis[k], (size_t)d->n_rec * d->nt))) return bail(rc);
   }
-  if ((rc = p->alloc(&p->step, 1)) || (rc = p->alloc(&p->done, 1))) return bail(rc);
+  if ((rc = p->alloc(&p->step, 1))) return bail(rc);
   const int kseg = d->tile_cfg > 0 ? 16 << (d->tile_cfg - 1) : 32;
   if ((rc = make_work3d(p, kseg, elastic ? kRowsEl3 : kRowsAc3, elastic ? kRowsEl3Core : kRowsAc3Core))) return bail(rc);
   if ((rc = p->reset())) return bail(rc);

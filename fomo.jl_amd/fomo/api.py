@@ -461,8 +461,9 @@ class Plan:
         _lib.check(_lib.lib().fomo_plan_sync(self._h))
 
     def kernel_times(self, n: int) -> tuple[float, float, float]:
-        """Average ms per launch of (velocity sweep, pressure/stress sweep, gather): each sweep over
-        n back-to-back launches, the gather over n ordinary steps (the state advances n steps)."""
+        """Average ms per launch of (velocity sweep, pressure/stress sweep) over n back-to-back
+        launches each, and the rest of an ordinary step's time (gather, counter updates, gaps)
+        over n steps (the state advances n steps)."""
         out = (C.c_double * 3)()
         _lib.check(_lib.lib().fomo_plan_kernel_times(self._h, int(n), out))
         return out[0], out[1], out[2]

@@ -199,9 +199,10 @@ int fomo_plan_timed_advance(fomo_plan *p, int32_t n, double *ms);
 int fomo_plan_sync(fomo_plan *p);
 /* Per-kernel timing on the plan's stream with hipEvents: the velocity and the pressure/stress sweep
  * are each launched n times back to back from the current state (repeating a sweep rewrites the
- * same values), the gather is timed inside n ordinary steps; writes the average milliseconds per
- * launch of {velocity sweep, pressure or stress sweep, gather} to out3.  The state advances by n
- * steps.  Blocks. */
+ * same values); then n ordinary steps are timed.  Writes the average milliseconds per launch of
+ * the velocity sweep and of the pressure/stress sweep, and the rest of a step's time (receiver
+ * gather, step-counter updates, gaps between launches), to out3.  The state advances by n steps.
+ * Blocks. */
 int fomo_plan_kernel_times(fomo_plan *p, int32_t n, double *out3);
 /* Steps taken since the last reset. */
 int fomo_plan_steps_done(const fomo_plan *p, int32_t *out);
