@@ -292,29 +292,32 @@ __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, 
   }
   const int i = i0 + (int)threadIdx.x;
   const bool lane_ok = (int)threadIdx.x < W.w;
-  // old values at the cell and at its inward x / z neighbours, and the blend weights
-  float fo[ZPT][NF], fx[ZPT][NF], fz[ZPT][NF], wt[ZPT][NF], dm[ZPT];
+  // Old values at the cell and at its inward x / z neighbours, the distance-table weights, the
+  // learned corrections and the sponge factors.  Every load has a valid index and no lane-dependent
+  // condition (such a load would be waited for in its own branch, draining all loads in flight).
+  float fo[ZPT][NF], fx[ZPT][NF], fz[ZPT][NF], lw[ZPT][NF], aw[ZPT][NF], dm[ZPT];
 #pragma unroll
   for (int k = 0; k < ZPT; ++k) {
     const int jl = (int)threadIdx.y * ZPT + k;
     const int j = j0 + jl;
     const bool ok = lane_ok && jl < W.h;
-    const int dist = ok ? min(min(i, nx - 1 - i), min(j, nz - 1 - j)) : 0;
+    const int dist = ok ? min(min(i, nx - 1 - i), min(j, nz - 1 - j)) : 0;  // < pad in a frame tile
     const bool Lf = i <= H, Rf = i >= nx - 1 - H, Tf = j <= H, Bf = j >= nz - 1 - H;
     const bool blend = ok && !damp && dist >= 1 && dist <= H;
     const bool inx = blend && (Lf || Rf), inz = blend && (Tf || Bf);
     const unsigned c = ok ? (unsigned)(i + j * nx) : 0u;
     const unsigned cx = inx ? (unsigned)(i + (Lf ? 1 : -1) + j * nx) : c;
     const unsigned cz = inz ? (unsigned)(i + (j + (Tf ? 1 : -1)) * nx) : c;
+    const int dl = min(dist, H);
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       fo[k][f] = Cvt<T>::ld(V.old[f], c, V.odec[f]);
       fx[k][f] = Cvt<T>::ld(V.old[f], cx, V.odec[f]);
       fz[k][f] = Cvt<T>::ld(V.old[f], cz, V.odec[f]);
-      wt[k][f] = blend ? (V.alpha[f] ? fminf(fmaxf(V.lut[f][dist] + V.alpha[f][c], 0.0f), 1.0f) : V.lut[f][dist])
-                       : 1.0f;
+      lw[k][f] = V.lut[f][dl];
+      aw[k][f] = V.alpha[f] ? V.alpha[f][c] : 0.0f;
     }
-    dm[k] = (ok && damp) ? G(damp)[dist] : 1.0f;
+    dm[k] = damp ? G(damp)[dist] : 1.0f;
   }
   // 2. stage the stencil fields
 #pragma unroll
@@ -380,7 +383,7 @@ __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, 
           const float fn = pre_s[f * EW * EH + (lj + dj) * EW + li];
           sz = (-q * fn - qt * foc) - qxt * fz[k][f];
         }
-        const float w = wt[k][f];
+        const float w = V.alpha[f] ? fminf(fmaxf(lw[k][f] + aw[k][f], 0.0f), 1.0f) : lw[k][f];
         if (V.sens[f]) V.sens[f][c] = v[f] - (inx && inz ? 0.5f * (sx + sz) : (inx ? sx : sz));
         if (inx && inz)
           v[f] = w * v[f] + ((1.0f - w) * 0.5f) * (sx + sz);
@@ -396,14 +399,16 @@ __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, 
 }
 
 // Interior strip: at most 64 columns by W.h rows of the user's interior (no frame cell, every
-// cell in the update region).  The block walks down z four rows at a time (one row per wave)
-// through a circular LDS window of RS rows holding the stencil inputs.  While the current rows
-// are computed, the stencil rows, old values and materials of the next four rows are in flight,
-// so each block waits on one round of loads per four rows; the window is large enough that the
-// rows being written never overlap the rows being read, so one barrier per four rows suffices.
+// cell in the update region).  The block walks down z in groups of eight rows (two rows per
+// wave, four rows apart) through a circular LDS window of RS rows holding the stencil inputs.
+// While a group is computed, the stencil rows, old values and materials of the next group are in
+// flight (sixteen to twenty loads per lane), so each block waits on one round of loads per eight
+// rows.  The window is large enough that the rows being written never overlap the rows being
+// read, so one barrier per group suffices.
+constexpr int kStripGroup = 8;
 template <int N>
 struct StripRing {
-  static constexpr int RS = (8 + 2 * N) <= 16 ? 16 : 32;
+  static constexpr int RS = (2 * kStripGroup + 2 * N) <= 32 ? 32 : 64;
 };
 
 template <int PH, int N, class T>
@@ -411,8 +416,8 @@ __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D&
                                                float* lds, int it) {
   using P = Phys<PH, N, T>;
   constexpr int NF = P::NF, NS = P::NS, NM = P::NM;
-  constexpr int BX = kTileW, RW = BX + 2 * N, RS = StripRing<N>::RS;
-  constexpr int W0 = 4 + 2 * N, R0 = (W0 + 3) / 4;  // initial window rows, per thread row
+  constexpr int BX = kTileW, RW = BX + 2 * N, RS = StripRing<N>::RS, GR = kStripGroup, RPT = GR / 4;
+  constexpr int W0 = GR + 2 * N, R0 = (W0 + 3) / 4;  // initial window rows, per thread row
   const int nx = S.nx, nz = S.nz;
   const float dtx = S.dtx;
   const int i0 = W.i0, j0 = W.j0, jend = W.j0 + W.h;
@@ -463,47 +468,59 @@ __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D&
   };
   const int i = i0 + tx;
   const bool col_ok = tx < W.w;
-  // initial window (rows j0-N .. j0+3+N) and the old values / materials of the first rows
+  const int n_grp = (W.h + GR - 1) / GR;
+  // Loads of group k: the window rows it adds (j0 + GR*k + N + ty + 4r) and the old values and
+  // materials of its rows j0 + GR*k + ty + 4r.
+  float ra[RPT][NS], rb[RPT][NS], no[RPT][NF], nm[RPT][NM];
+  auto issue = [&](int k) {
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      load_row(j0 + GR * k + N + ty + 4 * r, ra[r], rb[r]);
+      const int j = j0 + GR * k + ty + 4 * r;
+      load_om<P, T>(V, (col_ok && j < jend) ? (unsigned)(i + j * nx) : 0u, no[r], nm[r]);
+    }
+  };
+  float co[RPT][NF], cm[RPT][NM];
   {
+    // initial window (rows j0-N .. j0+GR-1+N) and the old values / materials of group 0
     float ia[R0][NS], ib[R0][NS];
 #pragma unroll
     for (int k = 0; k < R0; ++k) load_row(ty + 4 * k < W0 ? jb + ty + 4 * k : -1, ia[k], ib[k]);
 #pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const int j = j0 + ty + 4 * r;
+      load_om<P, T>(V, (col_ok && j < jend) ? (unsigned)(i + j * nx) : 0u, co[r], cm[r]);
+    }
+#pragma unroll
     for (int k = 0; k < R0; ++k)
       if (ty + 4 * k < W0) store_row(jb + ty + 4 * k, ia[k], ib[k]);
   }
-  float o[NF], m[NM];
-  {
-    const int j = j0 + ty;
-    const unsigned c = (col_ok && j < jend) ? (unsigned)(i + j * nx) : 0u;
-    load_om<P, T>(V, c, o, m);
-  }
   __syncthreads();
   const WinRd<RW, RS> rd{lds, oi, jb};
-  for (int jr = j0; jr < jend; jr += 4) {
-    const bool more = jr + 4 < jend;
-    const int jn = jr + 4 + N + ty;
-    const int j = jr + ty;
-    float na[NS], nb[NS], no[NF], nm[NM];
-    if (more) {
-      load_row(jn, na, nb);
-      const unsigned cn = (col_ok && j + 4 < jend) ? (unsigned)(i + (j + 4) * nx) : 0u;
-      load_om<P, T>(V, cn, no, nm);
-    }
-    if (col_ok && j < jend) {
-      float v[NF];
-      P::upd(rd, dtx, A.a, i, j, o, m, v);
-      const unsigned c = (unsigned)(i + j * nx);
-      if (j >= src_lo && j <= src_hi) inject<NF>(S, mask, (int)c, it, A.nt, v);
+  for (int k = 0; k < n_grp; ++k) {
+    const bool more = k + 1 < n_grp;
+    if (more) issue(k + 1);
 #pragma unroll
-      for (int f = 0; f < NF; ++f) Cvt<T>::st(V.out[f], c, v[f], V.oenc[f]);
+    for (int r = 0; r < RPT; ++r) {
+      const int j = j0 + GR * k + ty + 4 * r;
+      if (col_ok && j < jend) {
+        float v[NF];
+        P::upd(rd, dtx, A.a, i, j, co[r], cm[r], v);
+        const unsigned c = (unsigned)(i + j * nx);
+        if (j >= src_lo && j <= src_hi) inject<NF>(S, mask, (int)c, it, A.nt, v);
+#pragma unroll
+        for (int f = 0; f < NF; ++f) Cvt<T>::st(V.out[f], c, v[f], V.oenc[f]);
+      }
     }
     if (!more) break;
-    store_row(jn, na, nb);
 #pragma unroll
-    for (int f = 0; f < NF; ++f) o[f] = no[f];
+    for (int r = 0; r < RPT; ++r) {
+      store_row(j0 + GR * (k + 1) + N + ty + 4 * r, ra[r], rb[r]);
 #pragma unroll
-    for (int k = 0; k < NM; ++k) m[k] = nm[k];
+      for (int f = 0; f < NF; ++f) co[r][f] = no[r][f];
+#pragma unroll
+      for (int q = 0; q < NM; ++q) cm[r][q] = nm[r][q];
+    }
     __syncthreads();
   }
 }

@@ -22,10 +22,20 @@ constexpr int kOffB3 = kOffW3 + kMlpOut * kMlpH;  // W3 [2][32]
 constexpr int kMlpParams = kOffB3 + kMlpOut;      // 3586
 constexpr float kCorrMax = 0.25f;
 
+// Inference tile: w x h frame cells from (x0, z0); w <= 64, w * h <= kInferTileCells and the
+// staged patch region (w + 4) x (h + 4) fits kInferStage cells.
+struct InferTile {
+  int x0, z0, w, h;
+};
+constexpr int kInferTileCells = 256;
+constexpr int kInferStage = 544;  // (64 + 4) x (4 + 4)
+
 struct MlpArgs {
   const void* w_bf16;   // kMlpParams BFloat16 weights
-  const int* cells;     // frame cells (padded index) of shot 0
+  const int* cells;     // frame cells (padded index) of shot 0 (training)
   int n_cells;
+  const InferTile* tiles;  // the same cells as inference tiles
+  int n_tiles;
   int nx, nz, H;
   const void* p;        // state at time n (read buffer), stored as `storage` with scales dec
   const void* vx;

@@ -9,6 +9,7 @@
 #include <hip/hip_bf16.h>
 
 #include <algorithm>
+#include <utility>
 
 #include "learned2d.h"
 #include "storage.h"
@@ -118,154 +119,219 @@ __device__ __forceinline__ void mlp_bwd(const float* W, const float* h1, const f
 // of lane l holds row 8*(r/4) + 4*(l/32) + r%4, column l%32.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
-constexpr int kK1 = 80;                 // layer-1 k padded to 5 blocks of 16
 constexpr int kInferWaves = 4;
 constexpr int kInferCells = 64;         // cells per wave pass
-
-// Features for inference: as gather_features, with the patch scale applied as a multiply by its
-// reciprocal (within one FP32 ulp of the division, far below the BF16 rounding that follows) and
-// 32-bit offsets from the scalar field bases.
-template <class T>
-__device__ __forceinline__ void infer_features(const MlpArgs& A, int c, float* x) {
-  const int nx = A.nx, nz = A.nz, H = A.H;
-  const int i = c % nx, j = c / nx;
-  float mp = 0.0f, mv = 0.0f;
-#pragma unroll
-  for (int dy = -2; dy <= 2; ++dy) {
-    const int jj = min(max(j + dy, 0), nz - 1);
-#pragma unroll
-    for (int dx = -2; dx <= 2; ++dx) {
-      const int ii = min(max(i + dx, 0), nx - 1);
-      const int q = (dy + 2) * 5 + (dx + 2);
-      const unsigned g = (unsigned)(ii + jj * nx);
-      const float a = Cvt<T>::ld(A.p, g, A.dec[0]), b = Cvt<T>::ld(A.vx, g, A.dec[1]), d = Cvt<T>::ld(A.vz, g, A.dec[2]);
-      x[q] = a;
-      x[25 + q] = b;
-      x[50 + q] = d;
-      mp = fmaxf(mp, fabsf(a));
-      mv = fmaxf(mv, fmaxf(fabsf(b), fabsf(d)));
-    }
-  }
-  const float rp = 1.0f / (mp + 1e-30f), rv = 1.0f / (mv + 1e-30f);
-#pragma unroll
-  for (int q = 0; q < 25; ++q) x[q] *= rp;
-#pragma unroll
-  for (int q = 25; q < 75; ++q) x[q] *= rv;
-  const int dist = min(min(i, nx - 1 - i), min(j, nz - 1 - j));
-  x[75] = (float)dist / (float)H;
-}
 
 __device__ __forceinline__ __bf16 wbf(const unsigned short* w, int e) {
   const unsigned short u = G(w)[e];
   return __builtin_bit_cast(__bf16, u);
 }
 
+// Feature k of a cell (k < 25 p, k < 50 vx, k < 75 vz at patch offset q = k mod 25, row-major
+// 5 x 5 from the cell's patch corner; 75 the layer depth; 76.. zero) as a constant offset into the
+// staged region (field-major, kInferStage cells per field, row pitch sw) and a scale selector.
+struct FeatK {
+  int field_off, dy, dx;  // field * kInferStage, patch row and column
+  bool pscale, depth, zero;
+};
+__device__ constexpr FeatK feat_k(int k) {
+  if (k > 75) return FeatK{0, 0, 0, false, false, true};
+  if (k == 75) return FeatK{0, 0, 0, false, true, false};
+  const int f = k < 25 ? 0 : (k < 50 ? 1 : 2), q = k - 25 * f;
+  return FeatK{f * kInferStage, q / 5, q % 5, f == 0, false, false};
+}
+
+// The 8 BF16 features of an MFMA A fragment: features K0 .. K0+7 in the lower half-wave,
+// K0+8 .. K0+15 in the upper one.  The half-wave selects an address and a scale, so each feature
+// is one LDS read and no branch.  sc: 1/max|p|, 1/max|v|, depth of the lane's cell.
+template <int K0>
+__device__ __forceinline__ bf16x8 frag8(bool upper, const float* stg, int base, int sw, const float* sc) {
+  bf16x8 a;
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const FeatK l = feat_k(K0 + u), h = feat_k(K0 + 8 + u);
+    const int off = upper ? h.field_off + h.dy * sw + h.dx : l.field_off + l.dy * sw + l.dx;
+    const bool ps = upper ? h.pscale : l.pscale, dp = upper ? h.depth : l.depth, zr = upper ? h.zero : l.zero;
+    float x = stg[base + off] * (ps ? sc[0] : sc[1]);
+    x = dp ? sc[2] : x;
+    x = zr ? 0.0f : x;
+    a[u] = (__bf16)x;
+  }
+  return a;
+}
+
+// Inference over one tile of frame cells: the tile's patch region of p, vx, vz is staged in LDS
+// once (each value serves up to 25 cells), the features of each MFMA A fragment are formed from
+// it (the patch scale applied as a multiply by its reciprocal, within one FP32 ulp of the
+// division and far below the BF16 rounding that follows), and the three layers run on BF16 MFMA
+// with FP32 accumulation.
 template <class T>
 __global__ __launch_bounds__(64 * kInferWaves) void mlp_infer_kernel(const MlpArgs A) {
-  __shared__ __bf16 feats[kInferWaves][kInferCells * kK1];
+  static_assert(kInferWaves * kInferCells == kInferTileCells, "one tile cell per thread");
+  // staged p, vx, vz of the tile's patch region; per-cell feature scales; hidden activations
+  __shared__ float stg[3][kInferStage];
+  __shared__ float scl[kInferTileCells][3];  // 1/max|p|, 1/max|v|, dist/H
   __shared__ __bf16 hbuf[kInferWaves][kInferCells * kMlpH];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __shared__ unsigned short wsh[kMlpParams];
+  __shared__ float zout[kInferTileCells][kMlpOut];  // layer-3 outputs (pre-activation)
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
   const int col = lane & 31, half = lane >> 5;
-  const unsigned short* w = (const unsigned short*)A.w_bf16;
+  const InferTile tl = A.tiles[blockIdx.x];
+  const int nx = A.nx, nz = A.nz, SW = tl.w + 4, n_stage = SW * (tl.h + 4);
+  // 1. stage the patch region (cells clamped to the grid, as the features define): every load is
+  // issued before the first LDS write
+  constexpr int KS = (kInferStage + 64 * kInferWaves - 1) / (64 * kInferWaves);
+  float sv[KS][3];
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const int idx = min(tid + k * 64 * kInferWaves, n_stage - 1);
+    const int r = idx / SW, cc = idx - r * SW;
+    const int ii = min(max(tl.x0 - 2 + cc, 0), nx - 1), jj = min(max(tl.z0 - 2 + r, 0), nz - 1);
+    const unsigned g = (unsigned)(ii + jj * nx);
+    sv[k][0] = Cvt<T>::ld(A.p, g, A.dec[0]);
+    sv[k][1] = Cvt<T>::ld(A.vx, g, A.dec[1]);
+    sv[k][2] = Cvt<T>::ld(A.vz, g, A.dec[2]);
+  }
+  // the weights, staged in LDS with one round of unconditional loads (a load under a
+  // lane-dependent condition would be waited for on its own)
+  {
+    constexpr int KW = (kMlpParams + 64 * kInferWaves - 1) / (64 * kInferWaves);
+    const unsigned short* w = (const unsigned short*)A.w_bf16;
+    unsigned short wr[KW];
+#pragma unroll
+    for (int k = 0; k < KW; ++k) wr[k] = G(w)[min(tid + k * 64 * kInferWaves, kMlpParams - 1)];
+#pragma unroll
+    for (int k = 0; k < KW; ++k)
+      if (tid + k * 64 * kInferWaves < kMlpParams) wsh[tid + k * 64 * kInferWaves] = wr[k];
+  }
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const int idx = tid + k * 64 * kInferWaves;
+    if (idx < n_stage) {
+      stg[0][idx] = sv[k][0];
+      stg[1][idx] = sv[k][1];
+      stg[2][idx] = sv[k][2];
+    }
+  }
+  __syncthreads();
+  auto wl = [&](int e) { return __builtin_bit_cast(__bf16, wsh[e]); };
   bf16x8 b1[5], b2[2], b3[2];
 #pragma unroll
   for (int kb = 0; kb < 5; ++kb)
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int k = kb * 16 + half * 8 + j;
-      b1[kb][j] = k < kMlpIn ? wbf(w, kOffW1 + col * kMlpIn + k) : (__bf16)0.0f;
+      b1[kb][j] = k < kMlpIn ? wl(kOffW1 + col * kMlpIn + k) : (__bf16)0.0f;
     }
 #pragma unroll
   for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int k = kb * 16 + half * 8 + j;
-      b2[kb][j] = wbf(w, kOffW2 + col * kMlpH + k);
-      b3[kb][j] = col < kMlpOut ? wbf(w, kOffW3 + col * kMlpH + k) : (__bf16)0.0f;
+      b2[kb][j] = wl(kOffW2 + col * kMlpH + k);
+      b3[kb][j] = col < kMlpOut ? wl(kOffW3 + col * kMlpH + k) : (__bf16)0.0f;
     }
-  const float bias1 = (float)wbf(w, kOffB1 + col), bias2 = (float)wbf(w, kOffB2 + col);
-  const float bias3 = col < kMlpOut ? (float)wbf(w, kOffB3 + col) : 0.0f;
-  __bf16* F = feats[wv];
-  __bf16* Hb = hbuf[wv];
-  const int per_block = kInferWaves * kInferCells;
-  // every wave of a block runs the same number of passes, so the block barriers are uniform
-  for (int g0 = blockIdx.x * per_block; g0 < A.n_cells; g0 += gridDim.x * per_block) {
-    const int g = g0 + wv * kInferCells;
-    {
-      const int e = g + lane;
-      float x[kK1];
-      infer_features<T>(A, G(A.cells)[e < A.n_cells ? e : 0], x);
+  const float bias1 = (float)wl(kOffB1 + col), bias2 = (float)wl(kOffB2 + col);
+  const float bias3 = col < kMlpOut ? (float)wl(kOffB3 + col) : 0.0f;
+  __syncthreads();
+  // 2. per-cell scales: max-abs of the 5x5 patches (p; vx and vz together) and the layer depth
+  const int n_tc = tl.w * tl.h;
+  {
+    float mp = 0.0f, mv = 0.0f, dh = 0.0f;
+    if (tid < n_tc) {
+      const int li = tid % tl.w, lj = tid / tl.w;
 #pragma unroll
-      for (int k = kMlpIn; k < kK1; ++k) x[k] = 0.0f;
+      for (int dy = 0; dy < 5; ++dy)
 #pragma unroll
-      for (int k = 0; k < kK1; k += 8) {
-        bf16x8 v;
-#pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = (__bf16)x[k + u];
-        *(bf16x8*)&F[lane * kK1 + k] = v;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int grp = 0; grp < 2; ++grp) {
-      const int rb = grp * 32;  // first cell of the 32-row tile
-      f32x16 acc;
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-#pragma unroll
-      for (int kb = 0; kb < 5; ++kb) {
-        const bf16x8 a = *(const bf16x8*)&F[(rb + col) * kK1 + kb * 16 + half * 8];
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1[kb], acc, 0, 0, 0);
-      }
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int row = 8 * (r / 4) + 4 * half + (r % 4);
-        Hb[(rb + row) * kMlpH + col] = (__bf16)fmaxf(acc[r] + bias1, 0.0f);
-      }
-    }
-    __syncthreads();
-    f32x16 h2[2];
-#pragma unroll
-    for (int grp = 0; grp < 2; ++grp) {
-      const int rb = grp * 32;
-#pragma unroll
-      for (int r = 0; r < 16; ++r) h2[grp][r] = 0.0f;
-#pragma unroll
-      for (int kb = 0; kb < 2; ++kb) {
-        const bf16x8 a = *(const bf16x8*)&Hb[(rb + col) * kMlpH + kb * 16 + half * 8];
-        h2[grp] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b2[kb], h2[grp], 0, 0, 0);
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int grp = 0; grp < 2; ++grp)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int row = 8 * (r / 4) + 4 * half + (r % 4);
-        Hb[(grp * 32 + row) * kMlpH + col] = (__bf16)fmaxf(h2[grp][r] + bias2, 0.0f);
-      }
-    __syncthreads();
-#pragma unroll
-    for (int grp = 0; grp < 2; ++grp) {
-      const int rb = grp * 32;
-      f32x16 z;
-#pragma unroll
-      for (int r = 0; r < 16; ++r) z[r] = 0.0f;
-#pragma unroll
-      for (int kb = 0; kb < 2; ++kb) {
-        const bf16x8 a = *(const bf16x8*)&Hb[(rb + col) * kMlpH + kb * 16 + half * 8];
-        z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b3[kb], z, 0, 0, 0);
-      }
-      if (col < kMlpOut) {
-        float* out = col == 0 ? A.alpha_v : A.alpha_p;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int e = g + rb + 8 * (r / 4) + 4 * half + (r % 4);
-          if (e < A.n_cells) G(out)[G(A.cells)[e]] = corrf(z[r] + bias3);
+        for (int dx = 0; dx < 5; ++dx) {
+          const int o = (lj + dy) * SW + li + dx;
+          mp = fmaxf(mp, fabsf(stg[0][o]));
+          mv = fmaxf(mv, fmaxf(fabsf(stg[1][o]), fabsf(stg[2][o])));
         }
-      }
+      const int i = tl.x0 + li, j = tl.z0 + lj;
+      const int dist = min(min(i, nx - 1 - i), min(j, nz - 1 - j));
+      dh = (float)dist / (float)A.H;
     }
-    __syncthreads();
+    scl[tid][0] = 1.0f / (mp + 1e-30f);
+    scl[tid][1] = 1.0f / (mv + 1e-30f);
+    scl[tid][2] = dh;
+  }
+  __syncthreads();
+  __bf16* Hb = hbuf[wv];
+  // 3. layer 1 (features built from the staged region), per wave 64 cells in two 32-row groups
+#pragma unroll
+  for (int grp = 0; grp < 2; ++grp) {
+    const int rb = grp * 32;
+    const int t = wv * kInferCells + rb + col;
+    const bool tv = t < n_tc;
+    const int lj = tv ? t / tl.w : 0, li = tv ? t - lj * tl.w : 0;
+    const int base = lj * SW + li;
+    const float sc[3] = {tv ? scl[t][0] : 0.0f, tv ? scl[t][1] : 0.0f, tv ? scl[t][2] : 0.0f};
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+    auto kblock = [&](auto kbc) {
+      constexpr int kb = decltype(kbc)::value;
+      const bf16x8 a = frag8<kb * 16>(half != 0, &stg[0][0], base, SW, sc);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1[kb], acc, 0, 0, 0);
+    };
+    kblock(std::integral_constant<int, 0>{});
+    kblock(std::integral_constant<int, 1>{});
+    kblock(std::integral_constant<int, 2>{});
+    kblock(std::integral_constant<int, 3>{});
+    kblock(std::integral_constant<int, 4>{});
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int row = 8 * (r / 4) + 4 * half + (r % 4);
+      Hb[(rb + row) * kMlpH + col] = (__bf16)fmaxf(acc[r] + bias1, 0.0f);
+    }
+  }
+  __syncthreads();
+  f32x16 h2[2];
+#pragma unroll
+  for (int grp = 0; grp < 2; ++grp) {
+    const int rb = grp * 32;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) h2[grp][r] = 0.0f;
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+      const bf16x8 a = *(const bf16x8*)&Hb[(rb + col) * kMlpH + kb * 16 + half * 8];
+      h2[grp] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b2[kb], h2[grp], 0, 0, 0);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int grp = 0; grp < 2; ++grp)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int row = 8 * (r / 4) + 4 * half + (r % 4);
+      Hb[(grp * 32 + row) * kMlpH + col] = (__bf16)fmaxf(h2[grp][r] + bias2, 0.0f);
+    }
+  __syncthreads();
+#pragma unroll
+  for (int grp = 0; grp < 2; ++grp) {
+    const int rb = grp * 32;
+    f32x16 z;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) z[r] = 0.0f;
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+      const bf16x8 a = *(const bf16x8*)&Hb[(rb + col) * kMlpH + kb * 16 + half * 8];
+      z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b3[kb], z, 0, 0, 0);
+    }
+    if (col < kMlpOut) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) zout[wv * kInferCells + rb + 8 * (r / 4) + 4 * half + (r % 4)][col] = z[r] + bias3;
+    }
+  }
+  __syncthreads();
+  // 4. one cell per thread: the bounded corrections at the cell's padded index
+  if (tid < n_tc) {
+    const int lj = tid / tl.w, li = tid - lj * tl.w;
+    const int c = tl.x0 + li + (tl.z0 + lj) * nx;
+    G(A.alpha_v)[c] = corrf(zout[tid][0]);
+    G(A.alpha_p)[c] = corrf(zout[tid][1]);
   }
 }
 
@@ -402,8 +468,8 @@ __global__ void reduce_partials_kernel(const float* partial, int blocks, int wid
 
 hipError_t launch_mlp_forward(const MlpArgs& a, hipStream_t s) {
   if (a.n_cells <= 0) return hipSuccess;
-  const int per_block = kInferWaves * kInferCells;
-  const int blocks = std::min(4096, (a.n_cells + per_block - 1) / per_block);
+  const int blocks = a.n_tiles;
+  if (blocks <= 0) return hipErrorInvalidValue;
   switch (a.storage) {
     case FOMO_STORE_F32: hipLaunchKernelGGL(mlp_infer_kernel<float>, dim3(blocks), dim3(64 * kInferWaves), 0, s, a); break;
     case FOMO_STORE_F16: hipLaunchKernelGGL(mlp_infer_kernel<__half>, dim3(blocks), dim3(64 * kInferWaves), 0, s, a); break;

@@ -43,9 +43,40 @@ extern "C" int fomo_learned_enable(fomo_plan* p, const float* weights, int32_t t
       const int d = std::min(std::min(i, S.nx - 1 - i), std::min(j, S.nz - 1 - j));
       if (d >= 1 && d <= S.H) cells.push_back(i + j * S.nx);
     }
+  // Inference tiles over the four bands of the frame (cells with 1 <= dist <= H): top and bottom
+  // bands (rows 1..H and nz-1-H..nz-2, columns 1..nx-2) in 64 x 4 tiles, left and right bands
+  // (columns 1..H and nx-1-H..nx-2, rows H+1..nz-2-H) in tiles as tall as 256 cells allow.
+  std::vector<InferTile> tiles;
+  auto add_band = [&](int xa, int xb, int za, int zb) {  // inclusive-exclusive ranges
+    for (int x = xa; x < xb; x += 64) {
+      const int w = std::min(64, xb - x);
+      const int hs = std::max(1, std::min(16, kInferTileCells / w));
+      for (int z = za; z < zb; z += hs) tiles.push_back(InferTile{x, z, w, std::min(hs, zb - z)});
+    }
+  };
+  {
+    const int H = S.H, nx = S.nx, nz = S.nz;
+    add_band(1, nx - 1, 1, std::min(H + 1, nz - 1));
+    add_band(1, nx - 1, std::max(nz - 1 - H, H + 1), nz - 1);
+    add_band(1, std::min(H + 1, nx - 1), H + 1, nz - 1 - H);
+    add_band(std::max(nx - 1 - H, H + 1), nx - 1, H + 1, nz - 1 - H);
+    size_t covered = 0;
+    for (const auto& t : tiles) {
+      if ((t.w + 4) * (t.h + 4) > kInferStage || t.w * t.h > kInferTileCells)
+        return set_error(FOMO_EINVAL, "inference tile exceeds its staging area");
+      covered += (size_t)t.w * t.h;
+    }
+    if (covered != cells.size()) return set_error(FOMO_EINVAL, "inference tiles do not cover the frame cells");
+  }
   int rc;
   if ((rc = q->upload(&q->d_cells, cells.data(), cells.size()))) return rc;
   q->n_cells = (int)cells.size();
+  {
+    InferTile* dt = nullptr;
+    if ((rc = q->upload(&dt, tiles.data(), tiles.size()))) return rc;
+    q->d_itiles = dt;
+    q->n_itiles = (int)tiles.size();
+  }
   __hip_bfloat16* wb = nullptr;
   if ((rc = q->alloc(&wb, kMlpParams))) return rc;
   q->w_bf16 = wb;

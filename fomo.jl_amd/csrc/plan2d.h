@@ -34,6 +34,8 @@ struct Plan2D : PlanBase {
   bool learned = false, training = false;
   int* d_cells = nullptr;
   int n_cells = 0;
+  void* d_itiles = nullptr;  // InferTile list over the same cells
+  int n_itiles = 0;
   void* w_bf16 = nullptr;
   float* alpha[2] = {nullptr, nullptr};
   float* sens[3] = {nullptr, nullptr, nullptr};

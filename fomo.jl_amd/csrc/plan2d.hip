@@ -100,6 +100,8 @@ hipError_t Plan2D::enqueue_part(int part, int s, int koff) {
     m.w_bf16 = w_bf16;
     m.cells = d_cells;
     m.n_cells = n_cells;
+    m.tiles = (const InferTile*)d_itiles;
+    m.n_tiles = n_itiles;
     m.nx = shots[0].nx;
     m.nz = shots[0].nz;
     m.H = shots[0].H;
