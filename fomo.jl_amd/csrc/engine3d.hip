@@ -153,13 +153,19 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
   const int it = *A.step + A.koff;
   const bool i_upd = i >= N && i < nx - N;
 
-  // loads of field f at (ii, jj) of plane kk: scalar plane base + 32-bit in-plane offset
+  // Loads of field f at (ii, jj) of plane kk: scalar plane base + 32-bit in-plane offset.  Frame
+  // tiles reach outside the grid (halos, the ring); there the cell is clamped to the grid instead
+  // of the load being skipped: a load under a per-lane condition is waited for in its own branch,
+  // and values of cells outside the grid enter no stored result (those cells are not stored, and
+  // every stencil of an update cell stays inside the grid).
   auto ld = [&](int buf, int f, int ii, int jj, int kk) -> float {
+    if constexpr (LAG) {
+      ii = min(max(ii, 0), nx - 1);
+      jj = min(max(jj, 0), ny - 1);
+      kk = min(max(kk, 0), nz - 1);
+    }
     const unsigned off = (unsigned)(ii + jj * nx);
-    if constexpr (!LAG) return Cvt<T>::ld(plane_of<T>(S.fld[buf][f], kk, sz), off, S.fdec[f]);
-    return (ii >= 0 && ii < nx && jj >= 0 && jj < ny && kk >= 0 && kk < nz)
-               ? Cvt<T>::ld(plane_of<T>(S.fld[buf][f], kk, sz), off, S.fdec[f])
-               : 0.0f;
+    return Cvt<T>::ld(plane_of<T>(S.fld[buf][f], kk, sz), off, S.fdec[f]);
   };
 
   // z-queues: window of 2N planes starting at t-N (backward stencils) or t-N+1 (forward)
@@ -185,36 +191,36 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
       for (int r = 0; r < YPT; ++r) {
         const int j = pj0 + ty + TR * r;
         if (P::st_q(sp) < 0) sv[sp][r] = ld(sb, f, i, j, tp);
-        if (P::st_xh(sp)) {
-          hx[sp][r][0] = tx < N ? ld(sb, f, pi0 - N + tx, j, tp) : 0.0f;
-          hx[sp][r][1] = tx < N ? ld(sb, f, pi0 + kTileW + tx, j, tp) : 0.0f;
+        if (P::st_xh(sp)) {  // used by lanes tx < N; the others load a clamped cell
+          hx[sp][r][0] = ld(sb, f, max(pi0 - N + tx, 0), j, tp);
+          hx[sp][r][1] = ld(sb, f, min(pi0 + kTileW + tx, nx - 1), j, tp);
         }
       }
       if (P::st_yh(sp)) {
 #pragma unroll
         for (int m = 0; m < HYR; ++m) {
           const int hr = ty + TR * m;
-          const int jl = hr < N ? hr - N : ROWS + (hr - N);
-          hy[sp][m] = hr < 2 * N ? ld(sb, f, i, pj0 + jl, tp) : 0.0f;
+          const int jl = hr < N ? hr - N : ROWS + (hr - N);  // used by rows hr < 2N
+          hy[sp][m] = ld(sb, f, min(i, nx - 1), min(max(pj0 + jl, 0), ny - 1), tp);
         }
       }
     }
   };
   auto prefetch_cells = [&](int tp) {
-    const bool k_upd = tp >= N && tp < nz - N;
 #pragma unroll
     for (int r = 0; r < YPT; ++r) {
       const int j = pj0 + ty + TR * r;
-      const bool valid = !LAG || (i < nx && j < ny);
-      const bool upd = !LAG || (i_upd && k_upd && j >= N && j < ny - N);
-      const unsigned co = (unsigned)(i + j * nx);
+      // cells outside the grid load a clamped cell (see ld); materials of cells outside the update
+      // region are loaded too and not used
+      const int ic = LAG ? min(max(i, 0), nx - 1) : i, jc = LAG ? min(max(j, 0), ny - 1) : j;
+      const int tc = LAG ? min(max(tp, 0), nz - 1) : tp;
+      const unsigned co = (unsigned)(ic + jc * nx);
 #pragma unroll
       for (int f = 0; f < NF; ++f)
-        oldF[r][f] = valid ? Cvt<T>::ld(plane_of<T>(S.fld[s][P::out_f(f)], tp, sz), co, S.fdec[P::out_f(f)]) : 0.0f;
+        oldF[r][f] = Cvt<T>::ld(plane_of<T>(S.fld[s][P::out_f(f)], tc, sz), co, S.fdec[P::out_f(f)]);
 #pragma unroll
       for (int k = 0; k < NM; ++k)
-        matF[r][k] =
-            upd ? Cvt<T>::ld(plane_of<T>(S.mat[P::mat_id(k)], tp, sz), co, S.mdec[P::mat_id(k)]) : 0.0f;
+        matF[r][k] = Cvt<T>::ld(plane_of<T>(S.mat[P::mat_id(k)], tc, sz), co, S.mdec[P::mat_id(k)]);
     }
   };
 
