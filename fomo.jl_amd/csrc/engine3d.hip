@@ -226,6 +226,24 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
 
   float preN[YPT][NF], oldT[YPT][NF], preP[YPT][NF], oldP[YPT][NF];
 
+  // Core items: does a source lie in this thread's column and rows within the item's planes?
+  // Scanned once, so the plane loop does not wait on the source table.
+  bool src_here = false;
+  if constexpr (!LAG) {
+    bool inj_any = false;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) inj_any = inj_any || P::inj(f);
+    if (inj_any && it < A.nt) {
+      for (int q = 0; q < S.n_src; ++q) {
+        const long long cs = (long long)G(S.src_c)[q];
+        const long long ks = cs / (long long)sz, rem = cs - ks * (long long)sz;
+        const int js = (int)(rem / nx), is = (int)(rem - (long long)js * nx);
+        if (is != i || ks < W.k0 || ks >= W.k0 + W.hz) continue;
+#pragma unroll
+        for (int r = 0; r < YPT; ++r) src_here = src_here || js == pj0 + ty + TR * r;
+      }
+    }
+  }
   prefetch_stage(t0);
   prefetch_cells(t0);
   const int t_end = LAG ? W.k0 + W.hz : W.k0 + W.hz - 1;
@@ -369,7 +387,7 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
           float v = preN[r][f];
-          if (P::inj(f)) inject3d(S, (size_t)co + (size_t)t * sz, it, A.nt, v);
+          if (P::inj(f) && src_here) inject3d(S, (size_t)co + (size_t)t * sz, it, A.nt, v);
           Cvt<T>::st(plane_of<T>(S.fld[o][P::out_f(f)], t, sz), co, v, S.fenc[P::out_f(f)]);
         }
       }
@@ -411,6 +429,8 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
           const int F = P::out_f(f);
           const int e = (f * ROWS + jl) * kTileW + tx;
           float v = curp[e];
+          // loaded for every cell (clamped distance), not under the per-lane frame condition
+          const float wt = G(S.wlut)[P::lut(f) * (H + 1) + min(dist, H)];
           if (frame) {
             const float foc = curo[e];
             float sum = 0.0f;
@@ -431,10 +451,9 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
               sum += (-S.q * pz - S.qt * foc) - S.qxt * oz;
               ++nd;
             }
-            const float wt = G(S.wlut)[P::lut(f) * (H + 1) + dist];
             v = wt * v + ((1.0f - wt) * sum) / (float)nd;
           }
-          if (P::inj(f)) inject3d(S, (size_t)co + (size_t)kk * sz, it, A.nt, v);
+          // no injection here: sources are validated against the user's interior (core items)
           Cvt<T>::st(plane_of<T>(S.fld[o][F], kk, sz), co, v, S.fenc[F]);
         }
       }
