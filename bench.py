@@ -73,7 +73,8 @@ def cpu_baseline(vp, vs, rho, rx, rz, budget_s):
 
 def probe(args):
     """HBM-bound 2D probe (not the headline line): homogeneous 8192 x 8192 interior, fd8, nbc 50,
-    acoustic2d and elastic2d; prints per-kernel times and algorithmic GB/s."""
+    acoustic2d, elastic2d and scalar2d (one sweep per step: read u^n, u^(n-1), c2, write u^(n+1) =
+    16 B/cell); prints per-kernel times and algorithmic GB/s."""
     import numpy as np
 
     import fomo
@@ -86,7 +87,8 @@ def probe(args):
     nt = args.steps
     w = fomo.ricker_wavelet(15.0, 1e-3, nt)
     res = {}
-    for eq, name, bv, bs in ((L.ACOUSTIC2D, "acoustic2d", 28, 20), (L.ELASTIC2D, "elastic2d", BYTES_VEL, BYTES_STR)):
+    for eq, name, bv, bs in ((L.ACOUSTIC2D, "acoustic2d", 28, 20), (L.ELASTIC2D, "elastic2d", BYTES_VEL, BYTES_STR),
+                             (L.SCALAR2D, "scalar2d", 16, 0)):
         plan = Plan(eq, vp, rho, 10.0, 1e-3, nt, vs=vs if eq == L.ELASTIC2D else None, sx=[n // 2], sz=[n // 2],
                     rx=[1, n], rz=[1, n], wavelet=w[None], nbc=50, fd_order=8)
         cells = plan.padded_cells
@@ -97,8 +99,8 @@ def probe(args):
         plan.close()
         res[name] = {"padded_cells": cells, "ms_per_step": ms / nt, "gcell_per_s": cells * nt / ms / 1e6,
                      "step_gbs": (bv + bs) * cells * nt / ms / 1e6,
-                     "vel_ms": tv, "vel_gbs": bv * cells / tv / 1e6, "prs_ms": ts, "prs_gbs": bs * cells / ts / 1e6,
-                     "gather_ms": tr}
+                     "vel_ms": tv, "vel_gbs": bv * cells / tv / 1e6, "prs_ms": ts,
+                     "prs_gbs": bs * cells / ts / 1e6 if bs else 0.0, "gather_ms": tr}
         log(f"{name}: {res[name]}")
     print(json.dumps({"probe": f"{n}x{n} interior, fd8, nbc 50", **res}), flush=True)
 

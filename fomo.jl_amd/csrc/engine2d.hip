@@ -41,7 +41,7 @@ struct PV {
 };
 
 template <int PH>
-__device__ __forceinline__ void make_view(const Shot2D& S, int s, PV& V) {
+__device__ __forceinline__ void make_view(const Shot2D& S, int s, int it, PV& V) {
   const int o = 1 - s;
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
@@ -92,7 +92,7 @@ __device__ __forceinline__ void make_view(const Shot2D& S, int s, PV& V) {
     mat(1, 4);
     V.lut[0] = G(S.wlut);
     V.lut[1] = G(S.wlut + L);
-  } else {
+  } else if (PH == kElStr) {
     sten(0, o, 0);
     sten(1, o, 1);
     for (int f = 0; f < 3; ++f) {
@@ -100,6 +100,20 @@ __device__ __forceinline__ void make_view(const Shot2D& S, int s, PV& V) {
       mat(f, f);
       V.lut[f] = G(S.wlut + 3 * L);
     }
+  } else {
+    // scalar leapfrog, step it: u^it is read (stencil and old value), u^{it-1} enters as the second
+    // per-cell input, u^{it+1} is written; buffers rotate by it mod 3
+    const int cur = it % 3, prv = (it + 2) % 3, nxt = (it + 1) % 3;
+    V.sten[0] = S.fld[0][cur];
+    V.sdec[0] = S.fdec[0];
+    V.old[0] = S.fld[0][cur];
+    V.odec[0] = S.fdec[0];
+    V.out[0] = S.fld[0][nxt];
+    V.oenc[0] = S.fenc[0];
+    mat(0, 0);
+    V.mat[1] = S.fld[0][prv];
+    V.mdec[1] = S.fdec[0];
+    V.lut[0] = G(S.wlut + 3 * L);
   }
 }
 
@@ -200,6 +214,26 @@ struct Phys<kElStr, N, T> {
   }
 };
 
+// Second-order scalar (constant-density) equation, src/equations/scalar2d/scalar2d.jl:33-55:
+// u^{n+1} = (2 u - u^{n-1}) + c2 * lap, lap = (2 c0) u + sum_l d2_l (((u[i+l] + u[i-l]) +
+// u[j+l]) + u[j-l]); o[0] = u^n at the cell, m = {c2, u^{n-1}}; c0 is passed in dtx's place.
+template <int N, class T>
+struct Phys<kScal, N, T> {
+  static constexpr int NF = 1, NS = 1, NM = 2;
+  template <class R>
+  __device__ __forceinline__ static void upd(const R& rd, float c0, const float* a, int i, int j,
+                                             const float* o, const float* m, float* v) {
+    const float uc = o[0];
+    float lap = (2.0f * c0) * uc;
+#pragma unroll
+    for (int l = 1; l <= N; ++l) {
+      const float cl = a[l - 1];
+      lap += cl * (((rd(0, i + l, j) + rd(0, i - l, j)) + rd(0, i, j + l)) + rd(0, i, j - l));
+    }
+    v[0] = (2.0f * uc - m[1]) + m[0] * lap;
+  }
+};
+
 // Old values and materials of cell c (decoded).
 template <class P, class T>
 __device__ __forceinline__ void load_om(const PV& V, unsigned c, float* o, float* m) {
@@ -215,6 +249,7 @@ __device__ __forceinline__ int inj_mask(const Shot2D& S) {
   if (PH == kAcPrs) return 1;
   if (PH == kElVel) return S.src_fz ? 2 : 0;
   if (PH == kElStr) return S.src_fz ? 0 : 3;
+  if (PH == kScal) return 1;
   return 0;
 }
 
@@ -342,7 +377,7 @@ __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, 
 #pragma unroll
       for (int f = 0; f < NF; ++f) v[f] = ro[k][f];
     } else {
-      P::upd(rd, dtx, A.a, gi, gj, ro[k], rm[k], v);
+      P::upd(rd, PH == kScal ? A.c0 : dtx, A.a, gi, gj, ro[k], rm[k], v);
     }
 #pragma unroll
     for (int f = 0; f < NF; ++f) pre_s[f * EW * EH + idx] = v[f];
@@ -512,7 +547,7 @@ __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D&
       const int j = j0 + GR * k + ty + 4 * r;
       if (col_ok && j < jend) {
         float v[NF];
-        P::upd(rd, dtx, A.a, i, j, co[r], cm[r], v);
+        P::upd(rd, PH == kScal ? A.c0 : dtx, A.a, i, j, co[r], cm[r], v);
         const unsigned c = (unsigned)(i + j * nx);
         if (j >= src_lo && j <= src_hi) inject<NF>(S, mask, (int)c, it, A.nt, v);
 #pragma unroll
@@ -543,6 +578,10 @@ __device__ __forceinline__ void gather_entry(const GatherArgs& g, int e, int col
   const Shot2D& S = g.shots[g.rec_shot[e]];
   const unsigned c = (unsigned)g.rec_c[e];
   const long long o = g.rec_out[e] + (long long)g.rec_stride[e] * col;
+  if (S.nbuf == 3) {  // scalar: u after step col is u^{col+1}
+    G(g.seis[0])[o] = Cvt<T>::ld(S.fld[0][(col + 1) % 3], c, S.fdec[0]);
+    return;
+  }
   for (int k = 0; k < g.nslots; ++k) G(g.seis[k])[o] = Cvt<T>::ld(S.fld[parity][k], c, S.fdec[k]);
 }
 
@@ -566,7 +605,7 @@ __global__ __launch_bounds__(256) void sweep2d_kernel(const SweepArgs A) {
   if (W.kind < 0) return;
   const Shot2D& S = A.shots[W.shot];
   PV V;
-  make_view<PH>(S, A.parity, V);
+  make_view<PH>(S, A.parity, it, V);
   if (W.kind == 0)
     frame_tile<PH, N, T>(A, W, S, V, lds, it);
   else
@@ -612,6 +651,7 @@ static hipError_t launch_t2(int phase, int N, const SweepArgs& a, hipStream_t s)
     case kAcPrs: return launch_phase<kAcPrs, T>(N, a, s);
     case kElVel: return launch_phase<kElVel, T>(N, a, s);
     case kElStr: return launch_phase<kElStr, T>(N, a, s);
+    case kScal: return launch_phase<kScal, T>(N, a, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -63,6 +63,27 @@ extern "C" int fomo_fd_coefficients(int32_t fd_order, float* out) {
   return FOMO_OK;
 }
 
+// Centred second-derivative weights, src/utils/FD_centered.jl:31-47: Julia parses each ratio in
+// Float64 and rounds it to Float32; so do the double ratios below.
+static const double kD2[5][5] = {
+    {1.0, 0, 0, 0, 0},
+    {4.0 / 3.0, -1.0 / 12.0, 0, 0, 0},
+    {3.0 / 2.0, -3.0 / 20.0, 1.0 / 90.0, 0, 0},
+    {8.0 / 5.0, -1.0 / 5.0, 8.0 / 315.0, -1.0 / 560.0, 0},
+    {5.0 / 3.0, -5.0 / 21.0, 5.0 / 126.0, -5.0 / 1008.0, 1.0 / 3150.0},
+};
+static const double kD2C[5] = {-2.0, -5.0 / 2.0, -49.0 / 18.0, -205.0 / 72.0, -5269.0 / 1800.0};
+
+extern "C" int fomo_centered_d2(int32_t fd_order, float* c0, float* d2) {
+  if (fd_order < 2 || fd_order > 10 || (fd_order & 1))
+    return fail(FOMO_EINVAL, "Unsupported FD order: " + std::to_string(fd_order) + " (centered d2)");
+  if (!c0 || !d2) return fail(FOMO_EINVAL, "NULL argument");
+  const int N = fd_order / 2;
+  *c0 = (float)kD2C[N - 1];
+  for (int l = 0; l < N; ++l) d2[l] = (float)kD2[N - 1][l];
+  return FOMO_OK;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Ricker wavelet, src/utils/ricker_wavelet.jl:6-12:
 //   t = (0:nt-1) .* dt; tau = t .- 1.5f0/f0; u = pi*f0 .* tau; w = (1 - 2u^2) exp(-u^2).
@@ -374,6 +395,20 @@ extern "C" int fomo_batch_create(const fomo_desc* d, const float* vp, const floa
   return FOMO_OK;
 }
 
+// A single-shot 2D plan of any 2D equation (acoustic2d, elastic2d, scalar2d).
+static int create_single2d(const fomo_desc* d, const float* vp, const float* vs, const float* rho, const fomo_shot* s,
+                           fomo_plan** out) {
+  if (d->eq == FOMO_SCALAR2D) {
+    if (d->boundary != FOMO_BOUNDARY_HABC) return fail(FOMO_EUNSUP, "scalar2d uses the HABC (as in the reference)");
+    Plan2D* p = nullptr;
+    const int rc = create_plan2d(d, vp, vs, rho, 1, s, &p);
+    if (rc) return rc;
+    *out = new fomo_plan{p};
+    return FOMO_OK;
+  }
+  return fomo_batch_create(d, vp, vs, rho, 1, s, out);
+}
+
 int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, const float* rho, const int32_t* sx,
                        const int32_t* sy, const int32_t* sz, const float* wavelet, const int32_t* rx,
                        const int32_t* ry, const int32_t* rz, fomo_plan** out);
@@ -387,7 +422,7 @@ extern "C" int fomo_plan_create(const fomo_desc* d, const float* vp, const float
   if (rc) return rc;
   if (d->eq == FOMO_ACOUSTIC3D || d->eq == FOMO_ELASTIC3D)
     return fomo_plan_create3d(d, vp, vs, rho, sx, sy, sz, wavelet, rx, ry, rz, out);
-  if (d->eq != FOMO_ACOUSTIC2D && d->eq != FOMO_ELASTIC2D)
+  if (d->eq != FOMO_ACOUSTIC2D && d->eq != FOMO_ELASTIC2D && d->eq != FOMO_SCALAR2D)
     return fail(FOMO_EINVAL, "unknown equation " + std::to_string(d->eq));
   fomo_shot s{};
   s.x0 = 1;
@@ -400,7 +435,7 @@ extern "C" int fomo_plan_create(const fomo_desc* d, const float* vp, const float
   s.rx = rx;
   s.rz = rz;
   s.v_ref = d->v_ref;
-  return fomo_batch_create(d, vp, vs, rho, 1, &s, out);
+  return create_single2d(d, vp, vs, rho, &s, out);
 }
 
 #define PLAN_OR_FAIL(p)                                         \
@@ -586,6 +621,14 @@ extern "C" int fomo_elastic2d(const fomo_desc* d, const float* vp, const float* 
                               float* seis_vx, float* seis_vz, float* snaps, double* kernel_time_s) {
   if (!d || d->eq != FOMO_ELASTIC2D) return fail(FOMO_EINVAL, "descriptor eq must be FOMO_ELASTIC2D");
   return run_2d(d, vp, vs, rho, sx, sz, wavelet, rx, rz, seis_vx, seis_vz, nullptr, snaps, kernel_time_s);
+}
+
+extern "C" int fomo_scalar2d(const fomo_desc* d, const float* vp, const int32_t* sx, const int32_t* sz,
+                             const float* wavelet, const int32_t* rx, const int32_t* rz, float* seis_u,
+                             double* kernel_time_s) {
+  if (!d || d->eq != FOMO_SCALAR2D) return fail(FOMO_EINVAL, "descriptor must have eq = FOMO_SCALAR2D");
+  if (d->snap_interval != 0) return fail(FOMO_EINVAL, "scalar2d takes no snapshots (as in the reference)");
+  return run_2d(d, vp, nullptr, nullptr, sx, sz, wavelet, rx, rz, seis_u, nullptr, nullptr, nullptr, kernel_time_s);
 }
 
 // Reference batch form: every shot spans the whole model and shares the receivers and wavelet.

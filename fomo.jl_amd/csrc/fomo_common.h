@@ -32,9 +32,12 @@ __device__ __forceinline__ gp<T> G(T* p) {
 }
 
 // Sweep phases.
-enum Phase : int { kAcVel = 0, kAcPrs = 1, kElVel = 2, kElStr = 3 };
+enum Phase : int { kAcVel = 0, kAcPrs = 1, kElVel = 2, kElStr = 3, kScal = 4 };
 
-// One shot of a 2D run (device-resident table entry).
+// One shot of a 2D run (device-resident table entry).  Velocity-stress equations keep each field
+// in two buffers (ping-pong, fld[s][f]); the second-order scalar equation keeps u in three buffers
+// fld[0][0..2] and the state after k steps, u^k, lives in fld[0][k % 3] (u^{k-1} in the previous
+// one), so the leapfrog reads u^k and u^{k-1} and writes the third buffer.
 // Fields and materials are stored as the plan's storage type (Float32, Float16 or BFloat16); a
 // 16-bit value v is stored as RN(v * enc) and read back as float(stored) * dec.
 struct Shot2D {
@@ -62,6 +65,7 @@ struct Shot2D {
   // (elastic) as Float32 into snap[k] (x fastest), k = steps / snap_every - 1 < snap_n.
   float* snap;
   int snap_every, snap_n;
+  int nbuf;             // 2: ping-pong fields; 3: scalar u^{k+1}, u^k, u^{k-1} rotation
 };
 
 // One unit of sweep work.  kind 0: a frame tile (at most kTileW x kFrameRows cells of the
@@ -96,6 +100,7 @@ struct SweepArgs {
   const int* step;      // device step counter
   int koff;             // offset of this launch's step from the counter
   int nt;
+  float c0;             // scalar equation: centre coefficient of the second derivative
   // Velocity phase only: blocks n_work .. n_work + n_gather - 1 record the state held in buffer
   // [s] (the state after step it - 1) into column it - 1.
   GatherArgs g;

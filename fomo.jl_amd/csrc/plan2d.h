@@ -17,7 +17,8 @@ struct Plan2D : PlanBase {
   int storage = FOMO_STORE_F32, elem = 4;  // storage of fields and materials
   float* scratch = nullptr;                // Float32 staging of one decoded field
   size_t scratch_n = 0;
-  float coef[kMaxN] = {0};
+  float coef[kMaxN] = {0};  // staggered first-derivative (scalar: centred second-derivative) weights
+  float c0 = 0.0f;           // scalar: centre weight of the second derivative
   std::vector<Shot2D> shots;
   std::vector<ShotMeta> meta;
   Shot2D* d_shots = nullptr;
@@ -54,6 +55,8 @@ struct Plan2D : PlanBase {
   int make_work(int cfg);
   // Decode the current field `which` of a shot into `scratch` (asynchronous on the stream).
   int decode_field(int shot, int which);
+  // Buffer holding the current state of field `which` of a shot.
+  const void* current(int shot, int which) const;
   int autotune();
 };
 

@@ -80,6 +80,7 @@ hipError_t Plan2D::enqueue_part(int part, int s, int koff) {
   g.n_entries = n_entries;
   g.nt = d.nt;
   if (part == 2) return launch_gather2d(storage, g, step, koff, s, stream);
+  if (part == 1 && d.eq == FOMO_SCALAR2D) return hipSuccess;  // one phase per step
   SweepArgs a{};
   a.shots = d_shots;
   a.work = d_work[tile_cfg];
@@ -89,6 +90,7 @@ hipError_t Plan2D::enqueue_part(int part, int s, int koff) {
   a.step = step;
   a.koff = koff;
   a.nt = d.nt;
+  a.c0 = c0;
   if (part == 0) {
     a.g = g;
     a.n_gather = (n_entries + kGatherBlock - 1) / kGatherBlock;
@@ -115,14 +117,16 @@ hipError_t Plan2D::enqueue_part(int part, int s, int koff) {
     const hipError_t e = launch_mlp_forward(m, stream);
     if (e != hipSuccess) return e;
   }
-  return launch_sweep2d(storage, d.eq == FOMO_ACOUSTIC2D ? kAcVel : kElVel, N, tile_cfg, a, stream);
+  const int ph = d.eq == FOMO_ACOUSTIC2D ? kAcVel : (d.eq == FOMO_ELASTIC2D ? kElVel : kScal);
+  return launch_sweep2d(storage, ph, N, tile_cfg, a, stream);
 }
 
 int Plan2D::reset() {
   for (auto& sh : shots) {
     const size_t n = (size_t)sh.nx * sh.nz;
     for (int b = 0; b < 2; ++b)
-      for (int f = 0; f < nfld; ++f) FOMO_HIP_TRY(hipMemsetAsync(sh.fld[b][f], 0, n * elem, stream));
+      for (int f = 0; f < nfld; ++f)
+        if (sh.fld[b][f]) FOMO_HIP_TRY(hipMemsetAsync(sh.fld[b][f], 0, n * elem, stream));
   }
   for (int k = 0; k < nslots; ++k)
     if (seis_total) FOMO_HIP_TRY(hipMemsetAsync(seis[k], 0, (size_t)seis_total * sizeof(float), stream));
@@ -157,8 +161,14 @@ int Plan2D::decode_field(int shot, int which) {
     if (rc) return rc;
     scratch_n = n;
   }
-  FOMO_HIP_TRY(launch_decode(storage, sh.fld[steps_done % 2][which], scratch, n, sh.fdec[which], stream));
+  FOMO_HIP_TRY(launch_decode(storage, current(shot, which), scratch, n, sh.fdec[which], stream));
   return FOMO_OK;
+}
+
+const void* Plan2D::current(int shot, int which) const {
+  const Shot2D& sh = shots[shot];
+  if (sh.nbuf == 3) return sh.fld[0][steps_done % 3];  // scalar u^k
+  return sh.fld[steps_done % 2][which];
 }
 
 int Plan2D::read_field(int shot, int which, float* out) {
@@ -166,7 +176,7 @@ int Plan2D::read_field(int shot, int which, float* out) {
   if (which < 0 || which >= nfld) return set_error(FOMO_EINVAL, "field index out of range");
   const Shot2D& sh = shots[shot];
   const size_t n = (size_t)sh.nx * sh.nz;
-  const void* src = sh.fld[steps_done % 2][which];
+  const void* src = current(shot, which);
   if (storage != FOMO_STORE_F32) {
     const int rc = decode_field(shot, which);
     if (rc) return rc;
@@ -247,9 +257,12 @@ static int check_pts(const char* what, int n, const int32_t* x, const int32_t* z
 int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const float* rho, int n_shots,
                   const fomo_shot* sh, Plan2D** out) {
   *out = nullptr;
-  if (d->eq != FOMO_ACOUSTIC2D && d->eq != FOMO_ELASTIC2D) return set_error(FOMO_EINVAL, "not a 2D equation");
+  if (d->eq != FOMO_ACOUSTIC2D && d->eq != FOMO_ELASTIC2D && d->eq != FOMO_SCALAR2D)
+    return set_error(FOMO_EINVAL, "not a 2D equation");
+  const bool scalar = d->eq == FOMO_SCALAR2D;
   if (n_shots < 1) return set_error(FOMO_EINVAL, "at least one shot is required");
-  if (!vp || !rho || (d->eq == FOMO_ELASTIC2D && !vs)) return set_error(FOMO_EINVAL, "model arrays must not be NULL");
+  if (!vp || (!scalar && !rho) || (d->eq == FOMO_ELASTIC2D && !vs))
+    return set_error(FOMO_EINVAL, "model arrays must not be NULL");
   if (!sh) return set_error(FOMO_EINVAL, "shot table is NULL");
   for (int s = 0; s < n_shots; ++s) {
     if (sh[s].x0 < 1 || sh[s].nx < 1 || sh[s].x0 + sh[s].nx - 1 > d->nx)
@@ -265,9 +278,12 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
   p->storage = d->storage;
   p->elem = d->storage == FOMO_STORE_F32 ? 4 : 2;
   p->N = d->fd_order / 2;
-  fomo_fd_coefficients(d->fd_order, p->coef);
-  p->nfld = d->eq == FOMO_ACOUSTIC2D ? 3 : 5;
-  p->nslots = d->eq == FOMO_ACOUSTIC2D ? 3 : 2;
+  if (scalar)
+    fomo_centered_d2(d->fd_order, &p->c0, p->coef);
+  else
+    fomo_fd_coefficients(d->fd_order, p->coef);
+  p->nfld = d->eq == FOMO_ELASTIC2D ? 5 : 3;  // scalar: the three u buffers
+  p->nslots = d->eq == FOMO_ACOUSTIC2D ? 3 : (scalar ? 1 : 2);
   auto bail = [&](int code) {
     delete p;
     return code;
@@ -338,7 +354,7 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
       for (int j = 0; j < d->nz; ++j)
         for (int i = 0; i < S.nx; ++i) {
           const size_t c = (size_t)(S.x0 - 1 + i) + (size_t)j * d->nx;
-          if (vp[c] > 0.0f && rho[c] > 0.0f) {
+          if (!scalar && vp[c] > 0.0f && rho[c] > 0.0f) {
             zmin = std::min(zmin, rho[c] * vp[c]);
             zmax = std::max(zmax, rho[c] * vp[c]);
           }
@@ -373,8 +389,10 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
     } else {
       std::vector<float> vpp, rhop, bx, bz;
       pad2d(vp, S.nx, d->nz, pad, S.x0 - 1, d->nx, vpp);
-      pad2d(rho, S.nx, d->nz, pad, S.x0 - 1, d->nx, rhop);
-      buoyancy2d(rhop, x.nx, x.nz, bx, bz);
+      if (!scalar) {
+        pad2d(rho, S.nx, d->nz, pad, S.x0 - 1, d->nx, rhop);
+        buoyancy2d(rhop, x.nx, x.nz, bx, bz);
+      }
       void* dm[5] = {nullptr};
       // material m, stored with scale max |value| when 16-bit
       auto put = [&](int m, const std::vector<float>& h) -> int {
@@ -382,7 +400,15 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
         x.mdec[m] = scaled && mx > 0.0f ? mx : 1.0f;
         return upload_stored(p, p->storage, &dm[m], h.data(), h.size(), 1.0f / x.mdec[m]);
       };
-      if (d->eq == FOMO_ACOUSTIC2D) {
+      if (scalar) {
+        // c2 = (v * Float32(dt/dh))^2 on the edge-padded model, scalar2d.jl:126-128
+        std::vector<float> c2(n);
+        for (size_t c = 0; c < n; ++c) {
+          const float t = vpp[c] * dtx;
+          c2[c] = t * t;
+        }
+        if ((rc = put(0, c2))) return bail(rc);
+      } else if (d->eq == FOMO_ACOUSTIC2D) {
         std::vector<float> kappa(n);
         for (size_t c = 0; c < n; ++c) kappa[c] = rhop[c] * (vpp[c] * vpp[c]);
         if ((rc = put(0, kappa)) || (rc = put(1, bx)) || (rc = put(2, bz))) return bail(rc);
@@ -404,7 +430,8 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
       x.wlut = dl;
       medium_of[key] = s;
     }
-    for (int b = 0; b < 2; ++b)
+    x.nbuf = scalar ? 3 : 2;
+    for (int b = 0; b < (scalar ? 1 : 2); ++b)
       for (int f = 0; f < p->nfld; ++f)
       {
         char* fp = nullptr;

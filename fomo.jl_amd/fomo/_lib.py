@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "libfomo_hip.so"))
 
 FOMO_OK, FOMO_EINVAL, FOMO_EHIP, FOMO_ENOMEM, FOMO_EUNSUP = 0, 1, 2, 3, 4
-ACOUSTIC2D, ELASTIC2D, ACOUSTIC3D, ELASTIC3D = 1, 2, 3, 4
+ACOUSTIC2D, ELASTIC2D, ACOUSTIC3D, ELASTIC3D, SCALAR2D = 1, 2, 3, 4, 5
 SRC_EXPLOSIVE, SRC_FZ = 0, 1
 STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
 BOUNDARY_HABC, BOUNDARY_SPONGE = 0, 1
@@ -59,6 +59,8 @@ SIGNATURES = {
     "fomo_device_count": (C.c_int, []),
     "fomo_stream_copy": (C.c_int, [C.c_int64, _i, C.POINTER(C.c_double)]),
     "fomo_fd_coefficients": (C.c_int, [_i, _p]),
+    "fomo_centered_d2": (C.c_int, [_i, _p, _p]),
+    "fomo_scalar2d": (C.c_int, [C.POINTER(Desc), _p, _p, _p, _p, _p, _p, _p, C.POINTER(C.c_double)]),
     "fomo_ricker_wavelet": (C.c_int, [_f, _f, _i, _p]),
     "fomo_habc_coefficients": (C.c_int, [_f, _f, _f, _p]),
     "fomo_acoustic2d": (C.c_int, [C.POINTER(Desc), _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p,

@@ -16,12 +16,13 @@ import numpy as np
 
 from . import _lib
 from ._lib import ArgumentError, Desc
-from .checks import check_geometry, check_numerics
+from .checks import check_geometry, check_numerics, check_numerics_centered
 from .setup import ricker_wavelet
 
 Stats = namedtuple("Stats", ["kernel_time_s"])
 Acoustic2DResult = namedtuple("Acoustic2DResult", ["seis_p", "seis_vx", "seis_vz", "snaps", "stats"])
 Elastic2DResult = namedtuple("Elastic2DResult", ["seis_vx", "seis_vz", "snaps", "stats"])
+Scalar2DResult = namedtuple("Scalar2DResult", ["seis_u", "stats"])
 
 
 def _sym(x) -> str:
@@ -189,6 +190,58 @@ def elastic2d(vp, vs, rho, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_order=8
                                          None if snaps is None else snaps.ctypes.data, C.byref(t)))
     snap_list = [s.T for s in snaps] if snaps is not None else []
     return Elastic2DResult(svx, svz, snap_list, Stats(float(t.value)))
+
+
+def scalar2d(vp, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_order=8, v_ref=None, wavelet=None,
+             source_scale="v2dt2", verbose=True, use_cuda_graph=True, storage="f32"):
+    """scalar2d(vp, dh, dt, nt, f0; sx, sz, rx, rz, ...) -> (seis_u, stats).
+
+    src/equations/scalar2d/scalar2d.jl:113-189: second-order constant-density equation with the
+    deterministic HABC.  source_scale :v2dt2 (default) multiplies each source's wavelet by
+    vp(source)^2 dt^2, :dt by dt, :none leaves it (scalar2d.jl:139-146).  Extensions as acoustic2d:
+    `use_cuda_graph`, `storage`."""
+    del verbose
+    scale = _sym(source_scale)
+    if scale not in ("v2dt2", "dt", "none"):
+        raise ArgumentError(f"source_scale must be :v2dt2, :dt or :none, got {source_scale}")
+    if nbc < 0 or nt < 0:
+        raise ArgumentError("nbc and nt must be >= 0")
+    if fd_order not in (2, 4, 6, 8, 10):
+        raise ValueError(f"Unsupported FD order: {fd_order} (centered d2)")
+    vp = _f32F(vp)
+    if vp.ndim != 2:
+        raise ArgumentError("vp must be a matrix")
+    nx, nz = vp.shape
+    check_geometry(nx, nz, sx, sz, rx, rz)
+    vmin = _default_vref(vp)
+    check_numerics_centered(float(vp.max()), vmin, dh, dt, f0, fd_order)
+    v_ref = vmin if v_ref is None else float(np.float32(v_ref))
+    sx, sz, rx, rz = _i32(sx), _i32(sz), _i32(rx), _i32(rz)
+    if len(sx) != len(sz) or len(rx) != len(rz):
+        raise ArgumentError("source / receiver coordinate vectors differ in length")
+    if wavelet is None:
+        wav = ricker_wavelet(f0, dt, nt)
+    else:
+        wav = np.asarray(wavelet, np.float32).ravel()
+        if len(wav) != nt:
+            raise ArgumentError(f"wavelet length {len(wav)} != nt={nt}")
+    wm = np.repeat(wav[None, :nt], len(sx), axis=0).astype(np.float32)
+    f32 = np.float32
+    if scale == "v2dt2":
+        for q in range(len(sx)):
+            v = f32(vp[sx[q] - 1, sz[q] - 1])
+            wm[q] = wm[q] * f32(f32(v * v) * f32(f32(dt) * f32(dt)))
+    elif scale == "dt":
+        wm = (wm * f32(dt)).astype(np.float32)
+    wm = np.asfortranarray(wm)
+    n_rec = len(rx)
+    d = _desc(_lib.SCALAR2D, nx, nz, nbc, fd_order, nt, dh, dt, v_ref, len(sx), n_rec, 0, use_cuda_graph,
+              storage=storage_code(storage))
+    su = np.zeros((n_rec, nt), np.float32, order="F")
+    t = C.c_double(0.0)
+    _lib.check(_lib.lib().fomo_scalar2d(C.byref(d), vp.ctypes.data, sx.ctypes.data, sz.ctypes.data, wm.ctypes.data,
+                                        rx.ctypes.data, rz.ctypes.data, su.ctypes.data, C.byref(t)))
+    return Scalar2DResult(su, Stats(float(t.value)))
 
 
 Acoustic2DBatchResult = namedtuple("Acoustic2DBatchResult", ["seis_p", "seis_vx", "seis_vz", "stats"])

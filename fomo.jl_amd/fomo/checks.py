@@ -6,7 +6,7 @@ import warnings
 import numpy as np
 
 from ._lib import ArgumentError
-from .setup import get_fd_coefficients
+from .setup import get_centered_d2, get_fd_coefficients
 
 
 def check_geometry(nx: int, nz: int, sx, sz, rx, rz) -> None:
@@ -48,5 +48,22 @@ def check_numerics(vmax: float, vmin_disp: float, dh: float, dt: float, f0: floa
     dt_max = np.float32(np.float32(dh) / np.float32(np.float32(np.float32(vmax) * root) * s))
     if not np.float32(dt) <= dt_max:
         raise ArgumentError(f"CFL unstable: dt={dt} > dt_max={float(dt_max):.4g} "
+                            f"(vmax={vmax}, dh={dh}, fd_order={fd_order})")
+    warn_dispersion(vmin_disp, f0, dh, ppw_min)
+
+
+def check_numerics_centered(vmax: float, vmin_disp: float, dh: float, dt: float, f0: float, fd_order: int,
+                            ppw_min: float = 4.0) -> None:
+    """_check_numerics_centered, src/utils/checks.jl:61-72, in Float32: lam_max = |c0| + 2 sum|d2_l|,
+    dt <= 2 dh / (vmax sqrt(2 lam_max))."""
+    c0, d2 = get_centered_d2(fd_order)
+    s = np.float32(0.0)
+    for c in d2:
+        s = np.float32(s + np.float32(abs(c)))
+    lam_max = np.float32(np.float32(abs(c0)) + np.float32(np.float32(2.0) * s))
+    root = np.float32(np.sqrt(np.float32(np.float32(2.0) * lam_max)))
+    dt_max = np.float32(np.float32(np.float32(2.0) * np.float32(dh)) / np.float32(np.float32(vmax) * root))
+    if not np.float32(dt) <= dt_max:
+        raise ArgumentError(f"CFL unstable (centered): dt={dt} > dt_max={float(dt_max):.4g} "
                             f"(vmax={vmax}, dh={dh}, fd_order={fd_order})")
     warn_dispersion(vmin_disp, f0, dh, ppw_min)

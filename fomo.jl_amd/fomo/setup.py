@@ -15,6 +15,17 @@ def get_fd_coefficients(order: int) -> np.ndarray:
     return out[: order // 2].copy()
 
 
+def get_centered_d2(order: int):
+    """get_centered_d2, src/utils/FD_centered.jl:59-62: (c0, d2) as Float32 (raises for unsupported
+    orders)."""
+    c0 = np.zeros(1, np.float32)
+    d2 = np.zeros(5, np.float32)
+    rc = _lib.lib().fomo_centered_d2(int(order), c0.ctypes.data, d2.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"Unsupported FD order: {order} (centered d2)")
+    return np.float32(c0[0]), d2[: order // 2].copy()
+
+
 def ricker_wavelet(f0: float, dt: float, nt: int) -> np.ndarray:
     """ricker_wavelet, src/utils/ricker_wavelet.jl:6-12 (Float32, length nt)."""
     out = np.zeros(max(int(nt), 1), np.float32)

@@ -34,6 +34,7 @@ extern "C" {
 #define FOMO_ELASTIC2D 2
 #define FOMO_ACOUSTIC3D 3
 #define FOMO_ELASTIC3D 4
+#define FOMO_SCALAR2D 5  /* second-order constant-density scalar equation, u in three rotating buffers */
 
 #define FOMO_SRC_EXPLOSIVE 0 /* elastic: wavelet added to txx and tzz (reference behaviour) */
 #define FOMO_SRC_FZ 1        /* elastic: vertical force, wavelet added to vz after the velocity HABC */
@@ -62,6 +63,10 @@ int fomo_stream_copy(int64_t bytes, int32_t iters, double *ms_per_copy);
 /* Staggered FD coefficients for fd_order in {2,4,6,8,10}; writes fd_order/2 floats.
  * Replaces get_fd_coefficients, src/utils/FD_utils.jl:16-19.  Returns FOMO_EINVAL otherwise. */
 int fomo_fd_coefficients(int32_t fd_order, float *out);
+
+/* Centred second-derivative weights for fd_order in {2,4,6,8,10}: the centre weight to *c0 and
+ * fd_order/2 offset weights to d2.  Replaces get_centered_d2, src/utils/FD_centered.jl:59-62. */
+int fomo_centered_d2(int32_t fd_order, float *c0, float *d2);
 
 /* Ricker wavelet of length nt.  Replaces ricker_wavelet, src/utils/ricker_wavelet.jl:6-12. */
 int fomo_ricker_wavelet(float f0, float dt, int32_t nt, float *out);
@@ -126,6 +131,17 @@ int fomo_acoustic2d(const fomo_desc *d, const float *vp, const float *rho, const
 int fomo_elastic2d(const fomo_desc *d, const float *vp, const float *vs, const float *rho, const int32_t *sx,
                    const int32_t *sz, const float *wavelet, const int32_t *rx, const int32_t *rz, float *seis_vx,
                    float *seis_vz, float *snaps, double *kernel_time_s);
+
+/*
+ * scalar2d -- replaces Fomo.scalar2d (src/equations/scalar2d/scalar2d.jl:113-189): second-order
+ * constant-density leapfrog, u^{n+1} = 2u^n - u^{n-1} + c2 * lap (c2 = (v dt/dh)^2, centred
+ * weights of fomo_centered_d2), the deterministic Higdon blend on u (f = u^{n+1}, f_old = u^n,
+ * tau weights), raw-add injection and the u gather.  d->eq = FOMO_SCALAR2D, HABC only, no
+ * snapshots.  wavelet: n_src x nt, already scaled as source_scale asks (scalar2d.jl:139-146).
+ * Output seis_u (n_rec x nt).
+ */
+int fomo_scalar2d(const fomo_desc *d, const float *vp, const int32_t *sx, const int32_t *sz, const float *wavelet,
+                  const int32_t *rx, const int32_t *rz, float *seis_u, double *kernel_time_s);
 
 /*
  * acoustic3d -- replaces Fomo.acoustic3d (src/equations/acoustic3d/acoustic3d.jl:31-129).

@@ -506,6 +506,97 @@ int orc_acoustic2d(int nxi, int nzi, const float *vp, const float *rho, float dh
   return 0;
 }
 
+/* Centred second-derivative weights, src/utils/FD_centered.jl:31-47: f'' ~ (c0 f[i] +
+ * sum_l d2_l (f[i+l] + f[i-l])) / h^2; Float64 ratios rounded to Float32 as Julia's Float32[...]
+ * literal rounds them. */
+static const double D2_TABLE[5][5] = {
+    {1.0, 0, 0, 0, 0},
+    {4.0 / 3.0, -1.0 / 12.0, 0, 0, 0},
+    {3.0 / 2.0, -3.0 / 20.0, 1.0 / 90.0, 0, 0},
+    {8.0 / 5.0, -1.0 / 5.0, 8.0 / 315.0, -1.0 / 560.0, 0},
+    {5.0 / 3.0, -5.0 / 21.0, 5.0 / 126.0, -5.0 / 1008.0, 1.0 / 3150.0},
+};
+static const double D2_CENTER[5] = {-2.0, -5.0 / 2.0, -49.0 / 18.0, -205.0 / 72.0, -5269.0 / 1800.0};
+
+int orc_centered_d2(int order, float *c0, float *d2) {
+  if (order < 2 || order > 10 || (order & 1)) return -1;
+  int N = order / 2;
+  *c0 = (float)D2_CENTER[N - 1];
+  for (int l = 0; l < N; ++l) d2[l] = (float)D2_TABLE[N - 1][l];
+  return N;
+}
+
+/*
+ * scalar2d (src/equations/scalar2d/scalar2d.jl:33-81, step order of _scalar2d_loop! at :58-81;
+ * the reference's own CPU step is test/verify_scalar.jl:33-86):
+ *   a) u_next = (2 uc - u_prev) + c2 lap on M < i <= nx-M, M < j <= nz-M, lap = (2 c0) uc +
+ *      sum_l d2_l (((u[i+l] + u[i-l]) + u[j+l]) + u[j-l]); u_next = uc elsewhere;
+ *   b) two-pass Higdon on u_next with f_old = u^n and the tau weights (kernels_det.jl:22-132);
+ *   c) raw-add injection of wavelet[s, it] (already scaled by the caller as source_scale asks);
+ *   d) gather u_next.
+ * c2 = (vp_padded * Float32(dt/dh))^2.  final_state: u^nt (nx*nz).
+ */
+int orc_scalar2d(int nxi, int nzi, const float *vp, float dh, float dt, int nt, int nbc, int fd_order, float v_ref,
+                 int n_src, const int *sx, const int *sz, const float *wavelet, int n_rec, const int *rx,
+                 const int *rz, real *seis_u, real *final_state, int nthreads) {
+  set_threads(nthreads);
+  geom2d g;
+  if (geom2d_init(&g, nxi, nzi, nbc, fd_order, dt, dh, v_ref, 0)) return -1;
+  float c0f, d2f[5];
+  const int N = orc_centered_d2(fd_order, &c0f, d2f);
+  if (N < 0) return -1;
+  const real c0 = (real)c0f;
+  real d2[5];
+  for (int l = 0; l < N; ++l) d2[l] = (real)d2f[l];
+  const int M = N, nx = g.nx, nz = g.nz;
+  const size_t n = (size_t)nx * nz;
+  real *vpp = xcalloc(n, sizeof(real)), *c2 = xcalloc(n, sizeof(real)), *wtau = xcalloc(n, sizeof(real));
+  real *ua = xcalloc(n, sizeof(real)), *ub = xcalloc(n, sizeof(real)), *uc_ = xcalloc(n, sizeof(real));
+  long total = orc_frame_total(nx, nz, g.H);
+  real *scr = xcalloc((size_t)(total + 1), sizeof(real));
+  orc_pad2d(vp, nxi, nzi, g.pad, vpp);
+  for (size_t c = 0; c < n; ++c) {
+    real t = vpp[c] * g.dtx;
+    c2[c] = t * t;
+  }
+  orc_habc_weights2d(nx, nz, g.pad, 0.75, wtau);
+  real *u_prev = ua, *u_cur = ub, *u_next = uc_;
+  for (int it = 1; it <= nt; ++it) {
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j <= nz; ++j)
+      for (int i = 1; i <= nx; ++i) {
+#define U(a, b) u_cur[(size_t)((a)-1) + (size_t)((b)-1) * nx]
+        const size_t c = (size_t)(i - 1) + (size_t)(j - 1) * nx;
+        real uc = u_cur[c], un = uc;
+        if (i > M && i <= nx - M && j > M && j <= nz - M) {
+          real lap = (real)2 * c0 * uc;
+          for (int l = 1; l <= N; ++l) lap += d2[l - 1] * (((U(i + l, j) + U(i - l, j)) + U(i, j + l)) + U(i, j - l));
+          un = ((real)2 * uc - u_prev[c]) + c2[c] * lap;
+        }
+        u_next[c] = un;
+#undef U
+      }
+    real *fu[1] = {u_next}, *fuo[1] = {u_cur};
+    const real *wu[1] = {wtau};
+    habc_det(1, fu, fuo, wu, scr, g.q, g.qt, g.qxt, nx, nz, g.H);
+    for (int s = 0; s < n_src; ++s) {
+      size_t c = (size_t)(sx[s] + g.pad - 1) + (size_t)(sz[s] + g.pad - 1) * nx;
+      u_next[c] += (real)wavelet[(size_t)s + (size_t)n_src * (it - 1)];
+    }
+    for (int r = 0; r < n_rec; ++r) {
+      size_t c = (size_t)(rx[r] + g.pad - 1) + (size_t)(rz[r] + g.pad - 1) * nx;
+      if (seis_u) seis_u[(size_t)r + (size_t)n_rec * (it - 1)] = u_next[c];
+    }
+    real *t = u_prev;
+    u_prev = u_cur;
+    u_cur = u_next;
+    u_next = t;
+  }
+  if (final_state) memcpy(final_state, u_cur, n * sizeof(real));
+  free(vpp); free(c2); free(wtau); free(ua); free(ub); free(uc_); free(scr);
+  return 0;
+}
+
 /*
  * elastic2d, boundary = :habc, det two-pass path (elastic2d/loop_graph.jl:30-41 step order).
  *   source_type 0: explosive -- wavelet added to txx and tzz (elastic2d/fused_kernels.jl:147-159);

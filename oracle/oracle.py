@@ -53,6 +53,10 @@ def lib(precision: str = "f32") -> C.CDLL:
                                     _p, _p, _p, _p, _p, _p, _i]
         L.orc_elastic_params3d.restype = None
         L.orc_elastic_params3d.argtypes = [_p, _p, _p, _i, _i, _i, _p, _p, _p, _p, _p]
+        L.orc_scalar2d.restype = _i
+        L.orc_scalar2d.argtypes = [_i, _i, _p, _f, _f, _i, _i, _i, _f, _i, _p, _p, _p, _i, _p, _p, _p, _p, _i]
+        L.orc_centered_d2.restype = _i
+        L.orc_centered_d2.argtypes = [_i, _p, _p]
         L.orc_sponge_damp2d.restype = None
         L.orc_sponge_damp2d.argtypes = [_i, _i, _i, _f, _p]
         L.orc_fd_coeffs.restype = _i
@@ -268,6 +272,43 @@ def acoustic2d(vp, rho, dh, dt, nt, *, sx, sz, rx, rz, wavelet, nbc=50, fd_order
         shp = (nx + 2 * pad, nz + 2 * pad)
         n = shp[0] * shp[1]
         out["state"] = {k: fs[m * n:(m + 1) * n].reshape(shp, order="F") for m, k in enumerate(("p", "vx", "vz"))}
+    return out
+
+
+def centered_d2(order: int):
+    """(c0, d2) of src/utils/FD_centered.jl:59-62 as the oracle stores them (Float32)."""
+    c0 = np.zeros(1, np.float32)
+    d2 = np.zeros(5, np.float32)
+    n = lib("f32").orc_centered_d2(order, c0.ctypes.data, d2.ctypes.data)
+    if n < 0:
+        raise ValueError(f"Unsupported FD order: {order} (centered d2)")
+    return float(c0[0]), d2[:n].copy()
+
+
+def scalar2d(vp, dh, dt, nt, *, sx, sz, rx, rz, wavelet, nbc=50, fd_order=8, v_ref=None, precision="f32",
+             nthreads=0, final_state=False):
+    """scalar2d (src/equations/scalar2d/scalar2d.jl:33-81).  wavelet: (n_src, nt), already scaled as
+    source_scale asks.  Returns dict with seis_u (and state["u"] = u^nt when final_state)."""
+    vp = _f32F(vp)
+    nx, nz = vp.shape
+    if v_ref is None:
+        v_ref = float(vp[vp > 0].min())
+    sx, sz, rx, rz = _i32(sx), _i32(sz), _i32(rx), _i32(rz)
+    w = np.asfortranarray(np.asarray(wavelet, np.float32).reshape(len(sx), nt))
+    nrec = len(rx)
+    dt_ = _dtype(precision)
+    su = np.zeros((nrec, nt), dt_, order="F")
+    pad = nbc + fd_order // 2
+    shp = (nx + 2 * pad, nz + 2 * pad)
+    fs = np.zeros(shp[0] * shp[1], dt_) if final_state else None
+    rc = lib(precision).orc_scalar2d(nx, nz, vp.ctypes.data, dh, dt, nt, nbc, fd_order, v_ref, len(sx),
+                                     sx.ctypes.data, sz.ctypes.data, w.ctypes.data, nrec, rx.ctypes.data,
+                                     rz.ctypes.data, su.ctypes.data, _ptr(fs), nthreads)
+    if rc != 0:
+        raise ValueError(f"oracle scalar2d failed ({rc})")
+    out = dict(seis_u=su)
+    if final_state:
+        out["state"] = {"u": fs.reshape(shp, order="F")}
     return out
 
 
