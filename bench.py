@@ -150,6 +150,53 @@ def learned_bench(args):
                                  "reference's absorption experiment (301x201, 1500 steps)", **res}), flush=True)
 
 
+def batch_bench(args):
+    """Uneven multi-shot batch: a 4000x1000 seeded layered model (the 2000x800 model's rules), dh 10
+    m, dt 0.8 ms, fd8, nbc 40, 64 acoustic shots in one windowed plan.  Window widths are drawn
+    uniformly from 256..1536 and window starts uniformly over the model (SplitMix64, seeded); each
+    shot has its source at the window centre 5 cells deep and receivers on every second cell at
+    depth 5.  Timed with the autotuned strip length and with each fixed one, so the line shows
+    whether the autotuner picked the fastest.  Times --steps steps (the workload is 3000)."""
+    import numpy as np
+
+    import fomo
+    import fomo._lib as L
+    from fomo.api import Plan
+    from fomo.models import SplitMix64, layered_2d
+    nx, nz, dt, nbc = 4000, 1000, 8e-4, 40
+    nt = min(args.steps, 3000)
+    log("building the seeded 4000x1000 layered model")
+    vp, _, rho = layered_2d(nx, nz, seed=SEED)
+    w = fomo.ricker_wavelet(15.0, dt, nt)
+    rng = SplitMix64(SEED + 3)
+    shots = []
+    for _ in range(64):
+        wd = 256 + int(rng.uniform(0.0, 1281.0 - 1e-9))
+        x0 = 1 + int(rng.uniform(0.0, float(nx - wd + 1) - 1e-9))
+        rxl = np.arange(1, wd + 1, 2)
+        shots.append(dict(x0=x0, nx=wd, sx=[wd // 2], sz=[5], rx=rxl, rz=np.full(len(rxl), 5), wavelet=w[None]))
+    res = {}
+    for cfg in (0, 1, 2, 3):
+        log(f"64-shot plan, tile_cfg {cfg} (0: autotuned): setup, graph capture, {nt} timed steps")
+        plan = Plan(L.ACOUSTIC2D, vp, rho, 10.0, dt, nt, shots=shots, nbc=nbc, fd_order=8, tile_cfg=cfg)
+        cells = plan.padded_cells
+        plan.advance(min(20, nt))
+        plan.sync()
+        plan.reset()
+        ms = plan.timed_advance(nt)
+        plan.reset()
+        tv, tp, tr = plan.kernel_times(min(50, nt))
+        plan.close()
+        res[cfg] = {"ms_per_step": ms / nt, "cell_updates_per_s": cells * nt / (ms * 1e-3),
+                    "step_gbs": 48 * cells * nt / ms / 1e6, "vel_ms": tv, "vel_gbs": 28 * cells / tv / 1e6,
+                    "prs_ms": tp, "prs_gbs": 20 * cells / tp / 1e6, "step_rest_ms": tr}
+    out = {"batch": f"64 uneven windows (widths 256..1536) of a 4000x1000 model, acoustic fd8, nbc 40, dt 0.8 ms, "
+                    f"{nt} steps", "padded_cells": cells, "autotuned": res[0],
+           "fixed_strip_rows": {str(16 << (c - 1)): res[c] for c in (1, 2, 3)},
+           "widths": [s["nx"] for s in shots]}
+    print(json.dumps(out), flush=True)
+
+
 def stream_copy_gbs(nbytes=2 << 30, iters=20):
     """Practical HBM ceiling: float4 streaming copy of `nbytes` (read + write counted)."""
     import ctypes as C
@@ -216,10 +263,13 @@ def main():
     ap.add_argument("--stream-copy", action="store_true", help="only the HBM streaming-copy calibration")
     ap.add_argument("--learned", action="store_true", help="learned nbc-16 boundary vs plain HABC nbc 100 (cfg5)")
     ap.add_argument("--probe-tile", type=int, default=0, help="3D z-segment choice (0 default, k: 16<<(k-1) planes)")
+    ap.add_argument("--copy-mb", default="", help="with --stream-copy: comma-separated copy sizes in MB")
+    ap.add_argument("--batch", action="store_true",
+                    help="uneven 64-shot windowed acoustic batch on a 4000x1000 model (timed over --steps)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (shots and slabs would shard "
-              "across GPUs; see DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
+              "across GPUs).  Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     if args.probe:
@@ -231,7 +281,18 @@ def main():
     if args.learned:
         learned_bench(args)
         return
+    if args.batch:
+        batch_bench(args)
+        return
     if args.stream_copy:
+        if args.copy_mb:
+            res = []
+            for mb in (float(x) for x in args.copy_mb.split(",")):
+                nb = int(mb * 1e6) // 16 * 16
+                gbs, ms = stream_copy_gbs(nb, 200 if mb < 300 else 20)
+                res.append({"bytes": nb, "ms": ms, "gbs": gbs})
+            print(json.dumps({"stream_copy": res}), flush=True)
+            return
         gbs, ms = stream_copy_gbs(1 << 30, 4)
         print(json.dumps({"stream_copy": {"bytes": 1 << 30, "ms": ms, "gbs": gbs}}), flush=True)
         return
