@@ -51,6 +51,9 @@ struct PlanBase {
   hipError_t run_steps(int m, hipEvent_t* ev = nullptr);
   virtual int reset() = 0;
   virtual int read_seis(int shot, float* const* dst) = 0;
+  // Device address of one slot's recorded traces of a shot (n_rec x nt, receiver index fastest)
+  // and the shot's receiver count; fails for a slot the plan does not record.
+  virtual int seis_device(int shot, int slot, const float** ptr, int* n_rec) const = 0;
   virtual int read_field(int shot, int which, float* out) = 0;
   virtual int dims(int shot, int* nx, int* ny, int* nz, int* pad) const = 0;
   virtual int n_shots() const = 0;

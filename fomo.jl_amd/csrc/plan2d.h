@@ -49,6 +49,7 @@ struct Plan2D : PlanBase {
   bool gather_in_sweep() const override { return true; }
   int reset() override;
   int read_seis(int shot, float* const* dst) override;
+  int seis_device(int shot, int slot, const float** ptr, int* n_rec) const override;
   int read_field(int shot, int which, float* out) override;
   int dims(int shot, int* nx, int* ny, int* nz, int* pad) const override;
   int n_shots() const override { return (int)shots.size(); }

@@ -145,6 +145,14 @@ int Plan2D::read_seis(int shot, float* const* dst) {
   return FOMO_OK;
 }
 
+int Plan2D::seis_device(int shot, int slot, const float** ptr, int* n_rec) const {
+  if (shot < 0 || shot >= (int)shots.size()) return set_error(FOMO_EINVAL, "shot index out of range");
+  if (slot < 0 || slot >= nslots || !seis[slot]) return set_error(FOMO_EINVAL, "trace slot not recorded");
+  *ptr = seis[slot] + meta[shot].seis_off;
+  *n_rec = meta[shot].n_rec;
+  return FOMO_OK;
+}
+
 int Plan2D::decode_field(int shot, int which) {
   const Shot2D& sh = shots[shot];
   const size_t n = (size_t)sh.nx * sh.nz;

@@ -101,6 +101,14 @@ struct Plan3D : PlanBase {
     return FOMO_OK;
   }
 
+  int seis_device(int sh, int slot, const float** ptr, int* n_rec) const override {
+    if (sh != 0) return set_error(FOMO_EINVAL, "shot index out of range");
+    if (slot < 0 || slot >= nslots || !seis[slot]) return set_error(FOMO_EINVAL, "trace slot not recorded");
+    *ptr = seis[slot];
+    *n_rec = d.n_rec;
+    return FOMO_OK;
+  }
+
   int ensure_scratch(size_t n) {
     if (scratch_n >= n) return FOMO_OK;
     if (scratch) {

@@ -527,6 +527,29 @@ class Plan:
         _lib.check(_lib.lib().fomo_plan_read_shot_seis(self._h, shot, *ptrs))
         return outs
 
+    def trace_misfit(self, other=None, *, slot: int = 0, shot: int = 0, other_shot: int = 0, t0: int = 0,
+                     t1=None):
+        """FP64 sums over one trace slot, reduced on the GPU (fomo_trace_misfit): per receiver
+        num = sum_t (a - b)^2 and den = sum_t b^2 over samples t0 <= t < t1, where a is this plan's
+        shot and b the other plan's shot (b = 0 when other is None: num is then each trace's
+        energy).  Returns (num, den) as float64 arrays of the shot's receiver count."""
+        if not 0 <= shot < self.n_shots:
+            raise _lib.ArgumentError("shot index out of range")
+        n = self.n_rec[shot]
+        num = np.zeros(n, np.float64)
+        den = np.zeros(n, np.float64)
+        t1 = self.nt if t1 is None else int(t1)
+        ob = None if other is None else other._h
+        _lib.check(_lib.lib().fomo_trace_misfit(self._h, int(shot), ob, int(other_shot), int(slot), int(t0), t1,
+                                                num.ctypes.data, den.ctypes.data))
+        return num, den
+
+    def relative_residual(self, other, *, slot: int = 0, shot: int = 0, other_shot: int = 0):
+        """||a - b|| / ||b|| over the whole record of one slot (the reference's reflection measure,
+        test/verify_patch.jl:86-87), from the GPU's FP64 sums."""
+        num, den = self.trace_misfit(other, slot=slot, shot=shot, other_shot=other_shot)
+        return float(np.sqrt(num.sum() / den.sum())) if den.sum() > 0 else float("inf")
+
     def field(self, which: int, shot: int = 0) -> np.ndarray:
         out = np.zeros(self.padded[shot], np.float32, order="F")
         _lib.check(_lib.lib().fomo_plan_read_shot_field(self._h, shot, int(which), out.ctypes.data))

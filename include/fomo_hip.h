@@ -225,6 +225,15 @@ int fomo_plan_steps_done(const fomo_plan *p, int32_t *out);
 /* Copy seismograms to host (n_rec x nt each; NULL skips).  Slots: acoustic {p, vx, vz},
  * elastic2d {vx, vz, -}, acoustic3d {vx, vy, vz, p}. */
 int fomo_plan_read_seis(fomo_plan *p, float *s0, float *s1, float *s2, float *s3);
+/* FP64 per-receiver sums over one trace slot of two plans' recorded traces, time samples
+ * t0 <= t < t1: num[r] = sum_t (a[r,t] - b[r,t])^2 and den[r] = sum_t b[r,t]^2, with b = 0 when pb
+ * is NULL (then num[r] is the energy of trace r of a).  The shots must have equal receiver counts
+ * and nt.  Reduced on the GPU in a fixed order, so repeated calls give the same bits.  Replaces the
+ * host-side norms of the reference's residual checks: test/verify_patch.jl:86-87
+ * (||h - ref|| / ||ref||, the reflection measure) and the per-trace relative L2 of its solver
+ * comparisons.  Blocks. */
+int fomo_trace_misfit(fomo_plan *pa, int32_t shot_a, fomo_plan *pb, int32_t shot_b, int32_t slot, int32_t t0,
+                      int32_t t1, double *num, double *den);
 /* Copy the current padded field number `which` (order: acoustic2d p,vx,vz; elastic2d
  * vx,vz,txx,tzz,txz; acoustic3d p,vx,vy,vz) to host as Float32 (decoded from the storage
  * precision when reduced-precision storage is used). */

@@ -71,6 +71,16 @@ def test_validation_errors_raise_argument_error():
         fomo.elastic2d(vp, vp / 1.73, rho, 10.0, 1e-3, 100, 10.0, **kw, storage=7)
 
 
+def test_trace_misfit_rejects_null_arguments():
+    """The C-ABI's argument checks run before any device work."""
+    import ctypes as C
+
+    import fomo._lib as L
+    num, den = (C.c_double * 4)(), (C.c_double * 4)()
+    assert L.lib().fomo_trace_misfit(None, 0, None, 0, 0, 0, 1, num, den) == L.FOMO_EINVAL
+    assert "NULL" in L.lib().fomo_last_error().decode()
+
+
 def test_cfl_bound_matches_reference_formula():
     """dt_max = dh / (vmax sqrt(2) sum|a|): just below passes, just above raises."""
     from fomo.checks import check_numerics
