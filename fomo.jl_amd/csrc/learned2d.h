@@ -22,6 +22,16 @@ constexpr int kOffB3 = kOffW3 + kMlpOut * kMlpH;  // W3 [2][32]
 constexpr int kMlpParams = kOffB3 + kMlpOut;      // 3586
 constexpr float kCorrMax = 0.25f;
 
+// The inference kernel's copy of the weights, in MFMA B-fragment order (v_mfma_f32_32x32x16_bf16):
+// layer 1 as [5 k-blocks][64 lanes][8], layers 2 and 3 as [2][64][8] each, then the biases b1[32],
+// b2[32], b3[2].  Lane l holds output unit l%32 and inputs k = 16 kb + 8 (l/32) + 0..7; inputs
+// past 76 and layer-3 units past 2 hold zeros.  One 16-byte load per fragment and lane.
+constexpr int kFragW1 = 0;
+constexpr int kFragW2 = kFragW1 + 5 * 64 * 8;
+constexpr int kFragW3 = kFragW2 + 2 * 64 * 8;
+constexpr int kFragB = kFragW3 + 2 * 64 * 8;
+constexpr int kFragParams = kFragB + 2 * kMlpH + kMlpOut;
+
 // Inference tile: w x h frame cells from (x0, z0); w <= 64, w * h <= kInferTileCells and the
 // staged patch region (w + 4) x (h + 4) fits kInferStage cells.
 struct InferTile {
@@ -32,6 +42,7 @@ constexpr int kInferStage = 544;  // (64 + 4) x (4 + 4)
 
 struct MlpArgs {
   const void* w_bf16;   // kMlpParams BFloat16 weights
+  const void* w_frag;   // kFragParams BFloat16: the same weights in fragment order (inference)
   const int* cells;     // frame cells (padded index) of shot 0 (training)
   int n_cells;
   const InferTile* tiles;  // the same cells as inference tiles

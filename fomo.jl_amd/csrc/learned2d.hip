@@ -113,18 +113,30 @@ __device__ __forceinline__ void mlp_bwd(const float* W, const float* h1, const f
 // features of one cell (FP32, scaled as above), rounds them to BF16 into LDS, and the three layers
 // run as 32-cell x 32-unit tiles of v_mfma_f32_32x32x16_bf16 (K = 80 for layer 1, 76 features and
 // 4 zeros; K = 32 for layers 2 and 3; layer 3 has 2 live output columns).  Weights stay in
-// registers as BF16 B-fragments for the whole kernel; accumulation is FP32; the hidden activations
-// are rounded to BF16 between layers.  Fragment layouts (32x32x16): A lane l holds row l%32,
+// registers as BF16 B-fragments for the whole kernel (loaded from a copy kept in fragment order);
+// accumulation is FP32; the hidden activations are kept in FP32 in LDS and rounded to BF16 as they
+// are read into the next layer's A fragments.  Fragment layouts (32x32x16): A lane l holds row l%32,
 // k = 8*(l/32) + 0..7 of a 16-wide k block; B lane l holds column l%32, the same k; D register r
 // of lane l holds row 8*(r/4) + 4*(l/32) + r%4, column l%32.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 constexpr int kInferWaves = 4;
 constexpr int kInferCells = 64;         // cells per wave pass
+constexpr int kHidPitch = 40;           // words per cell row of the hidden-activation buffer
 
-__device__ __forceinline__ __bf16 wbf(const unsigned short* w, int e) {
-  const unsigned short u = G(w)[e];
-  return __builtin_bit_cast(__bf16, u);
+// A fragment of a hidden layer: 8 consecutive FP32 activations (32-byte aligned) rounded to BF16.
+__device__ __forceinline__ bf16x8 hid_frag(const float* src) {
+  const float4 lo = *(const float4*)src, hi = *(const float4*)(src + 4);
+  bf16x8 a;
+  a[0] = (__bf16)lo.x;
+  a[1] = (__bf16)lo.y;
+  a[2] = (__bf16)lo.z;
+  a[3] = (__bf16)lo.w;
+  a[4] = (__bf16)hi.x;
+  a[5] = (__bf16)hi.y;
+  a[6] = (__bf16)hi.z;
+  a[7] = (__bf16)hi.w;
+  return a;
 }
 
 // Feature k of a cell (k < 25 p, k < 50 vx, k < 75 vz at patch offset q = k mod 25, row-major
@@ -171,8 +183,9 @@ __global__ __launch_bounds__(64 * kInferWaves) void mlp_infer_kernel(const MlpAr
   // staged p, vx, vz of the tile's patch region; per-cell feature scales; hidden activations
   __shared__ float stg[3][kInferStage];
   __shared__ float scl[kInferTileCells][3];  // 1/max|p|, 1/max|v|, dist/H
-  __shared__ __bf16 hbuf[kInferWaves][kInferCells * kMlpH];
-  __shared__ unsigned short wsh[kMlpParams];
+  // hidden activations, FP32, one row per cell; the row pitch of 40 words puts the two half-waves'
+  // stores of a D register in disjoint banks and the 16-byte A-fragment reads conflict-free
+  __shared__ float hbuf[kInferWaves][kInferCells * kHidPitch];
   __shared__ float zout[kInferTileCells][kMlpOut];  // layer-3 outputs (pre-activation)
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
@@ -193,18 +206,21 @@ __global__ __launch_bounds__(64 * kInferWaves) void mlp_infer_kernel(const MlpAr
     sv[k][1] = Cvt<T>::ld(A.vx, g, A.dec[1]);
     sv[k][2] = Cvt<T>::ld(A.vz, g, A.dec[2]);
   }
-  // the weights, staged in LDS with one round of unconditional loads (a load under a
-  // lane-dependent condition would be waited for on its own)
-  {
-    constexpr int KW = (kMlpParams + 64 * kInferWaves - 1) / (64 * kInferWaves);
-    const unsigned short* w = (const unsigned short*)A.w_bf16;
-    unsigned short wr[KW];
+  // the weights as B fragments: one 16-byte load per fragment, in fragment order (kFragW1 ..), and
+  // the biases (layer 3 past its two units: a clamped load, then zero)
+  const bf16x8* wf = (const bf16x8*)A.w_frag;
+  bf16x8 b1[5], b2[2], b3[2];
 #pragma unroll
-    for (int k = 0; k < KW; ++k) wr[k] = G(w)[min(tid + k * 64 * kInferWaves, kMlpParams - 1)];
+  for (int kb = 0; kb < 5; ++kb) b1[kb] = wf[kFragW1 / 8 + kb * 64 + lane];
 #pragma unroll
-    for (int k = 0; k < KW; ++k)
-      if (tid + k * 64 * kInferWaves < kMlpParams) wsh[tid + k * 64 * kInferWaves] = wr[k];
+  for (int kb = 0; kb < 2; ++kb) {
+    b2[kb] = wf[kFragW2 / 8 + kb * 64 + lane];
+    b3[kb] = wf[kFragW3 / 8 + kb * 64 + lane];
   }
+  const __bf16* wb = (const __bf16*)A.w_frag + kFragB;
+  const float bias1 = (float)wb[col], bias2 = (float)wb[kMlpH + col];
+  const float b3v = (float)wb[2 * kMlpH + min(col, kMlpOut - 1)];
+  const float bias3 = col < kMlpOut ? b3v : 0.0f;
 #pragma unroll
   for (int k = 0; k < KS; ++k) {
     const int idx = tid + k * 64 * kInferWaves;
@@ -214,26 +230,6 @@ __global__ __launch_bounds__(64 * kInferWaves) void mlp_infer_kernel(const MlpAr
       stg[2][idx] = sv[k][2];
     }
   }
-  __syncthreads();
-  auto wl = [&](int e) { return __builtin_bit_cast(__bf16, wsh[e]); };
-  bf16x8 b1[5], b2[2], b3[2];
-#pragma unroll
-  for (int kb = 0; kb < 5; ++kb)
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int k = kb * 16 + half * 8 + j;
-      b1[kb][j] = k < kMlpIn ? wl(kOffW1 + col * kMlpIn + k) : (__bf16)0.0f;
-    }
-#pragma unroll
-  for (int kb = 0; kb < 2; ++kb)
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int k = kb * 16 + half * 8 + j;
-      b2[kb][j] = wl(kOffW2 + col * kMlpH + k);
-      b3[kb][j] = col < kMlpOut ? wl(kOffW3 + col * kMlpH + k) : (__bf16)0.0f;
-    }
-  const float bias1 = (float)wl(kOffB1 + col), bias2 = (float)wl(kOffB2 + col);
-  const float bias3 = col < kMlpOut ? (float)wl(kOffB3 + col) : 0.0f;
   __syncthreads();
   // 2. per-cell scales: max-abs of the 5x5 patches (p; vx and vz together) and the layer depth
   const int n_tc = tl.w * tl.h;
@@ -258,7 +254,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void mlp_infer_kernel(const MlpAr
     scl[tid][2] = dh;
   }
   __syncthreads();
-  __bf16* Hb = hbuf[wv];
+  float* Hf = hbuf[wv];
   // 3. layer 1 (features built from the staged region), per wave 64 cells in two 32-row groups
 #pragma unroll
   for (int grp = 0; grp < 2; ++grp) {
@@ -284,7 +280,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void mlp_infer_kernel(const MlpAr
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int row = 8 * (r / 4) + 4 * half + (r % 4);
-      Hb[(rb + row) * kMlpH + col] = (__bf16)fmaxf(acc[r] + bias1, 0.0f);
+      Hf[(rb + row) * kHidPitch + col] = fmaxf(acc[r] + bias1, 0.0f);
     }
   }
   __syncthreads();
@@ -296,7 +292,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void mlp_infer_kernel(const MlpAr
     for (int r = 0; r < 16; ++r) h2[grp][r] = 0.0f;
 #pragma unroll
     for (int kb = 0; kb < 2; ++kb) {
-      const bf16x8 a = *(const bf16x8*)&Hb[(rb + col) * kMlpH + kb * 16 + half * 8];
+      const bf16x8 a = hid_frag(Hf + (rb + col) * kHidPitch + kb * 16 + half * 8);
       h2[grp] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b2[kb], h2[grp], 0, 0, 0);
     }
   }
@@ -306,7 +302,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void mlp_infer_kernel(const MlpAr
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int row = 8 * (r / 4) + 4 * half + (r % 4);
-      Hb[(grp * 32 + row) * kMlpH + col] = (__bf16)fmaxf(h2[grp][r] + bias2, 0.0f);
+      Hf[(grp * 32 + row) * kHidPitch + col] = fmaxf(h2[grp][r] + bias2, 0.0f);
     }
   __syncthreads();
 #pragma unroll
@@ -317,7 +313,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void mlp_infer_kernel(const MlpAr
     for (int r = 0; r < 16; ++r) z[r] = 0.0f;
 #pragma unroll
     for (int kb = 0; kb < 2; ++kb) {
-      const bf16x8 a = *(const bf16x8*)&Hb[(rb + col) * kMlpH + kb * 16 + half * 8];
+      const bf16x8 a = hid_frag(Hf + (rb + col) * kHidPitch + kb * 16 + half * 8);
       z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b3[kb], z, 0, 0, 0);
     }
     if (col < kMlpOut) {

@@ -17,10 +17,38 @@ static Plan2D* acoustic_plan(fomo_plan* p) {
   return q;
 }
 
+// The BF16 weights rearranged into the inference kernel's B-fragment order (kFragW1 ..).
+static std::vector<__hip_bfloat16> weight_fragments(const std::vector<__hip_bfloat16>& w) {
+  std::vector<__hip_bfloat16> f(kFragParams, __float2bfloat16(0.0f));
+  for (int lane = 0; lane < 64; ++lane) {
+    const int col = lane & 31, half = lane >> 5;
+    for (int j = 0; j < 8; ++j) {
+      for (int kb = 0; kb < 5; ++kb) {
+        const int k = kb * 16 + half * 8 + j;
+        if (k < kMlpIn) f[kFragW1 + (kb * 64 + lane) * 8 + j] = w[kOffW1 + col * kMlpIn + k];
+      }
+      for (int kb = 0; kb < 2; ++kb) {
+        const int k = kb * 16 + half * 8 + j;
+        f[kFragW2 + (kb * 64 + lane) * 8 + j] = w[kOffW2 + col * kMlpH + k];
+        if (col < kMlpOut) f[kFragW3 + (kb * 64 + lane) * 8 + j] = w[kOffW3 + col * kMlpH + k];
+      }
+    }
+  }
+  for (int o = 0; o < kMlpH; ++o) {
+    f[kFragB + o] = w[kOffB1 + o];
+    f[kFragB + kMlpH + o] = w[kOffB2 + o];
+  }
+  for (int o = 0; o < kMlpOut; ++o) f[kFragB + 2 * kMlpH + o] = w[kOffB3 + o];
+  return f;
+}
+
 static int upload_weights(Plan2D* q, const float* w) {
   std::vector<__hip_bfloat16> h(kMlpParams);
   for (int e = 0; e < kMlpParams; ++e) h[e] = __float2bfloat16(w[e]);
+  const std::vector<__hip_bfloat16> f = weight_fragments(h);
   FOMO_HIP_TRY(hipMemcpyAsync(q->w_bf16, h.data(), kMlpParams * sizeof(__hip_bfloat16), hipMemcpyHostToDevice,
+                              q->stream));
+  FOMO_HIP_TRY(hipMemcpyAsync(q->w_frag, f.data(), kFragParams * sizeof(__hip_bfloat16), hipMemcpyHostToDevice,
                               q->stream));
   FOMO_HIP_TRY(hipStreamSynchronize(q->stream));
   return FOMO_OK;
@@ -80,6 +108,9 @@ extern "C" int fomo_learned_enable(fomo_plan* p, const float* weights, int32_t t
   __hip_bfloat16* wb = nullptr;
   if ((rc = q->alloc(&wb, kMlpParams))) return rc;
   q->w_bf16 = wb;
+  __hip_bfloat16* wf = nullptr;
+  if ((rc = q->alloc(&wf, kFragParams))) return rc;
+  q->w_frag = wf;
   for (int k = 0; k < 2; ++k)
     if ((rc = q->alloc(&q->alpha[k], n))) return rc;
   if (train) {

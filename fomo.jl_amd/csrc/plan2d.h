@@ -38,6 +38,7 @@ struct Plan2D : PlanBase {
   void* d_itiles = nullptr;  // InferTile list over the same cells
   int n_itiles = 0;
   void* w_bf16 = nullptr;
+  void* w_frag = nullptr;  // the same weights in MFMA fragment order
   float* alpha[2] = {nullptr, nullptr};
   float* sens[3] = {nullptr, nullptr, nullptr};
   float* partial = nullptr;

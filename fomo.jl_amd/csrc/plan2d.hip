@@ -17,13 +17,9 @@ namespace fomo {
 
 Plan2D::~Plan2D() = default;
 
-// Work list of one launch: frame tiles (64 x kFrameRows) over the four pad bands and interior
-// strips (64 x strip_rows) over the user's interior, for every shot, sorted by (shot, row,
-// column).  The list is split into 8 contiguous runs of about equal cost, one per XCD, and
-// interleaved so that block b (which the hardware places on XCD b % 8) takes the next item of
-// run b % 8.
-int Plan2D::make_work(int cfg) {
-  if (d_work[cfg]) return FOMO_OK;
+// Work items of a tile shape: frame tiles (64 x kFrameRows) over the four pad bands and interior
+// strips (64 x strip_rows) over the user's interior, for every shot, sorted by (shot, row, column).
+static std::vector<Work2D> work_items(const std::vector<Shot2D>& shots, const std::vector<ShotMeta>& meta, int cfg) {
   std::vector<Work2D> items;
   const int zs = strip_rows(cfg);
   auto add_rect = [&](int s, int kind, int x0, int x1, int z0, int z1, int th) {
@@ -45,6 +41,13 @@ int Plan2D::make_work(int cfg) {
     if (a.j0 != b.j0) return a.j0 < b.j0;
     return a.i0 < b.i0;
   });
+  return items;
+}
+
+// Launch order of a list: 8 contiguous runs of about equal cost, one per XCD, interleaved so that
+// block b (which the hardware places on XCD b % 8) takes the next item of run b % 8.  Slots past
+// the end of a shorter run hold an empty item (kind -1).
+static std::vector<Work2D> xcd_layout(const std::vector<Work2D>& items) {
   // frame cells cost about twice an interior cell (ring recompute, blend)
   std::vector<double> cost(items.size());
   double total = 0.0;
@@ -64,6 +67,12 @@ int Plan2D::make_work(int cfg) {
   std::vector<Work2D> flat(8 * len, Work2D{0, -1, 0, 0, 0, 0});
   for (int r = 0; r < 8; ++r)
     for (size_t k = 0; k < runs[r].size(); ++k) flat[k * 8 + r] = runs[r][k];
+  return flat;
+}
+
+int Plan2D::make_work(int cfg) {
+  if (d_work[cfg]) return FOMO_OK;
+  const std::vector<Work2D> flat = xcd_layout(work_items(shots, meta, cfg));
   n_work[cfg] = (int)flat.size();
   return upload(&d_work[cfg], flat.data(), flat.size());
 }
@@ -100,6 +109,7 @@ hipError_t Plan2D::enqueue_part(int part, int s, int koff) {
     // blend-weight corrections of every frame cell from the state at time n
     MlpArgs m{};
     m.w_bf16 = w_bf16;
+    m.w_frag = w_frag;
     m.cells = d_cells;
     m.n_cells = n_cells;
     m.tiles = (const InferTile*)d_itiles;

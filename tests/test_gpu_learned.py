@@ -90,6 +90,39 @@ def test_zero_correction_is_plain_habc_bitwise(gpu):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_learned_graph_equals_stream(gpu, storage):
+    """The inference runs on a side stream beside the interior strips and the frame tiles wait for
+    it (a fork and join, captured into the graph as parallel branches).  Graph replays and direct
+    launches give the same bits, for step counts ending inside and at the end of a replay, and the
+    trained network changes the result (so the corrections are read)."""
+    import fomo
+    import fomo._lib as L
+    from fomo.api import Plan
+    from fomo.learned import enable, load_weights
+    nx, nz = 220, 130
+    vp = np.full((nx, nz), 2200.0, np.float32)
+    vp[:, 60:] = 3000.0
+    rho = np.full((nx, nz), 2000.0, np.float32)
+    rx = np.arange(1, nx + 1, 2)
+    for nt in (37, 64):
+        w = fomo.ricker_wavelet(15.0, 8e-4, nt)
+        out = []
+        for graph, learned in ((True, True), (False, True), (True, False)):
+            p = Plan(L.ACOUSTIC2D, vp, rho, 10.0, 8e-4, nt, sx=[110], sz=[8], rx=rx, rz=np.full(len(rx), 4),
+                     wavelet=w[None], nbc=16, fd_order=8, use_graph=graph, storage=storage)
+            try:
+                if learned:
+                    enable(p, load_weights())
+                p.advance(nt)
+                out.append(p.seis(3) + [p.field(k) for k in range(3)])
+            finally:
+                p.close()
+        for a, b in zip(out[0], out[1]):
+            assert np.array_equal(a, b)
+        assert any(not np.array_equal(a, b) for a, b in zip(out[0], out[2]))
+
+
 def test_shipped_network_reduces_reflection(gpu):
     """The shipped weights, in the reference's absorption experiment (test/verify_patch.jl:76-91, its
     model, source and receivers; FD8): at nbc = 16 the residual against the boundary-free reference
