@@ -38,6 +38,25 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_threads():
+    t = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    return min(t, os.cpu_count() or t)
+
+
+def cpu_rate(run, cells, n_lo, n_hi, budget_s=8.0, n_max=3000):
+    """Host cell-updates/s of `run(n)` (seconds for an n-step oracle run).  Runs of n_lo and n_hi
+    steps estimate the time per step; then a run of n_lo + n steps, n sized to about budget_s of
+    stepping, is timed against the n_lo run, so that setup cancels.  Returns (rate, steps timed,
+    seconds of stepping timed)."""
+    t_a = run(n_lo)
+    t_b = run(n_hi)
+    per = max((t_b - t_a) / (n_hi - n_lo), 1e-9)
+    n = int(max(n_hi - n_lo, min(n_max, budget_s / per)))
+    t_c = run(n_lo + n)
+    per = max((t_c - t_a) / n, 1e-9)
+    return cells / per, n, per * n
+
+
 def cpu_baseline(vp, vs, rho, rx, rz, budget_s):
     """Oracle FP32 elastic2d on the host cores; per-step time from two runs of different length
     (so setup cancels)."""
@@ -146,6 +165,23 @@ def learned_bench(args):
         res[k]["reflection_db"] = v["reflection_db"]
         if k == "learned_nbc16_f16":
             res[k]["trace_rel_l2_vs_learned_nbc16"] = v["vs_learned_nbc16"]
+    if not args.no_cpu:
+        import oracle
+        oracle.lib("f32")
+        th = host_threads()
+        for name, nbc in (("habc_nbc100", 100), ("habc_nbc16", 16)):
+            def run(k, nbc=nbc):
+                wc = fomo.ricker_wavelet(15.0, dt, k)
+                t = time.perf_counter()
+                oracle.acoustic2d(vp, rho, 10.0, dt, k, sx=[nx // 2], sz=[50], rx=rx, rz=np.full(len(rx), 10),
+                                  wavelet=wc[None], nbc=nbc, fd_order=8, nthreads=th)
+                return time.perf_counter() - t
+            log(f"cpu baseline: oracle acoustic2d FP32 nbc {nbc} on {th} threads")
+            rate, ns, secs = cpu_rate(run, res[name]["padded_cells"], 4, 24, budget_s=5.0)
+            res[name]["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": th, "kind": "port",
+                                         "ms_per_step": res[name]["padded_cells"] / rate * 1e3,
+                                         "sample": f"oracle acoustic2d FP32 (C, -O3, OpenMP {th} threads), same grid, "
+                                                   f"{ns} steps timed against a 4-step run ({secs:.1f} s)"}
     print(json.dumps({"learned": f"acoustic2d {nx}x{nz}, fd8, dt 0.8 ms, {nt} steps; reflection from the "
                                  "reference's absorption experiment (301x201, 1500 steps)", **res}), flush=True)
 
@@ -194,6 +230,26 @@ def batch_bench(args):
                     f"{nt} steps", "padded_cells": cells, "autotuned": res[0],
            "fixed_strip_rows": {str(16 << (c - 1)): res[c] for c in (1, 2, 3)},
            "widths": [s["nx"] for s in shots]}
+    if not args.no_cpu:
+        import oracle
+        oracle.lib("f32")
+        th = host_threads()
+        sample = shots[:8]
+
+        def run(k):
+            wc = fomo.ricker_wavelet(15.0, dt, k)
+            t = time.perf_counter()
+            for s in sample:
+                x0, wd = s["x0"], s["nx"]
+                oracle.acoustic2d(vp[x0 - 1:x0 - 1 + wd], rho[x0 - 1:x0 - 1 + wd], 10.0, dt, k, sx=s["sx"], sz=s["sz"],
+                                  rx=s["rx"], rz=s["rz"], wavelet=wc[None], nbc=nbc, fd_order=8, nthreads=th)
+            return time.perf_counter() - t
+        cells8 = sum((s["nx"] + 2 * (nbc + 4)) * (nz + 2 * (nbc + 4)) for s in sample)
+        log(f"cpu baseline: oracle acoustic2d FP32 on {th} threads over the first 8 windows")
+        rate, ns, secs = cpu_rate(run, cells8, 10, 60)
+        out["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": th, "kind": "port",
+                               "sample": f"oracle acoustic2d FP32 (C, -O3, OpenMP {th} threads) on the first 8 windows "
+                                         f"run one after another, {ns} steps timed against 10-step runs ({secs:.1f} s)"}
     print(json.dumps(out), flush=True)
 
 
@@ -244,6 +300,22 @@ def probe3d(args):
     gbs, cms = stream_copy_gbs()
     res["stream_copy"] = {"bytes": 2 << 30, "ms": cms, "gbs": gbs}
     log(f"stream copy: {gbs:.0f} GB/s")
+    if not args.no_cpu:
+        import oracle
+        oracle.lib("f32")
+        th = host_threads()
+
+        def run(k):
+            wc = fomo.ricker_wavelet(12.0, 8e-4, k)
+            t = time.perf_counter()
+            oracle.acoustic3d(vp, rho, 10.0, 8e-4, k, sx=[n // 2], sy=[n // 2], sz=[1], rx=gx.ravel(), ry=gy.ravel(),
+                              rz=np.ones(gx.size, int), wavelet=wc[None], nbc=32, fd_order=8, nthreads=th)
+            return time.perf_counter() - t
+        log(f"cpu baseline: oracle acoustic3d FP32 on {th} threads")
+        rate, ns, secs = cpu_rate(run, res[next(iter(res))]["padded_cells"], 1, 3, budget_s=10.0)
+        res["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": th, "kind": "port",
+                               "sample": f"oracle acoustic3d FP32 (C, -O3, OpenMP {th} threads), same {n}^3 "
+                                         f"workload, {ns} steps timed against a 1-step run ({secs:.1f} s of stepping)"}
     print(json.dumps({"probe3d": f"{n}^3 interior, fd8, nbc 32", **res}), flush=True)
 
 
