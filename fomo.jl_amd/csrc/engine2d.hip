@@ -266,13 +266,16 @@ __device__ __forceinline__ void inject(const Shot2D& S, int mask, int c, int it,
   }
 }
 
-// Stencil data in a circular window of RS rows: row j lives in slot (j - jb) & (RS - 1).
-template <int RW, int RS>
-struct WinRd {
-  const float* t;
-  int oi, jb;
-  __device__ __forceinline__ float operator()(int k, int i, int j) const {
-    return t[k * RS * RW + ((j - jb) & (RS - 1)) * RW + (i - oi)];
+// Reader of the strip window for the update of row j.  The window is circular over RS rows (row
+// r in slot (r - jb) mod RS) and keeps a copy of its first 2N slots after the last one, so rows
+// j - N .. j + N lie in consecutive slots from the slot of row j - N: every stencil read is a
+// constant offset from one base per row (an LDS immediate), not a slot computed per read.
+template <int RW, int RSX, int N>
+struct StripRd {
+  const float* t;  // slot of row j - N, shifted so that column i is t[i]
+  int j;
+  __device__ __forceinline__ float operator()(int k, int i, int jj) const {
+    return t[k * RSX * RW + (jj - j + N) * RW + i];
   }
 };
 
@@ -438,12 +441,14 @@ __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, 
 // wave, four rows apart) through a circular LDS window of RS rows holding the stencil inputs.
 // While a group is computed, the stencil rows, old values and materials of the next group are in
 // flight (sixteen to twenty loads per lane), so each block waits on one round of loads per eight
-// rows.  The window is large enough that the rows being written never overlap the rows being
-// read, so one barrier per group suffices.
+// rows.  The window spans exactly the rows one group reads (GR + 2N) and the rows the next group
+// adds (GR), so the rows being written never overlap the rows being read and one barrier per
+// group suffices; RSX adds the copy of the first 2N slots (see StripRd).
 constexpr int kStripGroup = 8;
 template <int N>
 struct StripRing {
-  static constexpr int RS = (2 * kStripGroup + 2 * N) <= 32 ? 32 : 64;
+  static constexpr int RS = 2 * kStripGroup + 2 * N;
+  static constexpr int RSX = RS + 2 * N;
 };
 
 template <int PH, int N, class T>
@@ -451,7 +456,8 @@ __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D&
                                                float* lds, int it) {
   using P = Phys<PH, N, T>;
   constexpr int NF = P::NF, NS = P::NS, NM = P::NM;
-  constexpr int BX = kTileW, RW = BX + 2 * N, RS = StripRing<N>::RS, GR = kStripGroup, RPT = GR / 4;
+  constexpr int BX = kTileW, RW = BX + 2 * N, RS = StripRing<N>::RS, RSX = StripRing<N>::RSX;
+  constexpr int GR = kStripGroup, RPT = GR / 4;
   constexpr int W0 = GR + 2 * N, R0 = (W0 + 3) / 4;  // initial window rows, per thread row
   const int nx = S.nx, nz = S.nz;
   const float dtx = S.dtx;
@@ -494,11 +500,18 @@ __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D&
   auto store_row = [&](int j, const float (&ra)[NS], const float (&rb)[NS]) {
     const bool inj = j >= 0 && j < nz;
     const float za = (inj && ina) ? 1.0f : 0.0f, zb = (inj && inb) ? 1.0f : 0.0f;
-    const int slot = (j - jb) & (RS - 1);
+    const int slot = (int)((unsigned)(j - jb) % (unsigned)RS);  // j >= jb
+    const bool mirror = slot < 2 * N;  // uniform per wave (one thread row per wave)
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-      lds[k * RS * RW + slot * RW + tx] = ra[k] * za;
-      if (has_b) lds[k * RS * RW + slot * RW + BX + tx] = rb[k] * zb;
+      const float va = ra[k] * za, vb = rb[k] * zb;
+      float* row = lds + k * RSX * RW + slot * RW;
+      row[tx] = va;
+      if (has_b) row[BX + tx] = vb;
+      if (mirror) {
+        row[RS * RW + tx] = va;
+        if (has_b) row[RS * RW + BX + tx] = vb;
+      }
     }
   };
   const int i = i0 + tx;
@@ -538,7 +551,6 @@ __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D&
       if (ty + 4 * k < W0) store_row(jb + ty + 4 * k, ia[k], ib[k]);
   }
   __syncthreads();
-  const WinRd<RW, RS> rd{lds, oi, jb};
   for (int k = 0; k < n_grp; ++k) {
     const bool more = k + 1 < n_grp;
     if (more) issue(k + 1);
@@ -547,6 +559,7 @@ __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D&
       const int j = j0 + GR * k + ty + 4 * r;
       if (col_ok && j < jend) {
         float v[NF];
+        const StripRd<RW, RSX, N> rd{lds + (int)((unsigned)(j - N - jb) % (unsigned)RS) * RW - oi, j};  // j - N >= jb
         P::upd(rd, PH == kScal ? A.c0 : dtx, A.a, i, j, co[r], cm[r], v);
         const unsigned c = (unsigned)(i + j * nx);
         if (j >= src_lo && j <= src_hi) inject<NF>(S, mask, (int)c, it, A.nt, v);
@@ -593,7 +606,7 @@ __global__ __launch_bounds__(256) void sweep2d_kernel(const SweepArgs A) {
   using P = Phys<PH, N, T>;
   constexpr int NF = P::NF, NS = P::NS;
   constexpr int FT = NS * (kTileW + 2 * N + 2) * (kFrameRows + 2 * N + 2) + NF * (kTileW + 2) * (kFrameRows + 2);
-  constexpr int IS = NS * StripRing<N>::RS * (kTileW + 2 * N);
+  constexpr int IS = NS * StripRing<N>::RSX * (kTileW + 2 * N);
   __shared__ float lds[FT > IS ? FT : IS];
   const int it = *A.step + A.koff;
   if ((int)blockIdx.x >= A.n_work) {
