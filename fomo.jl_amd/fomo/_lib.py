@@ -58,6 +58,7 @@ SIGNATURES = {
     "fomo_version": (C.c_char_p, []),
     "fomo_device_count": (C.c_int, []),
     "fomo_stream_copy": (C.c_int, [C.c_int64, _i, C.POINTER(C.c_double)]),
+    "fomo_stream_probe": (C.c_int, [_i, _i, C.c_int64, _i, C.POINTER(C.c_double)]),
     "fomo_fd_coefficients": (C.c_int, [_i, _p]),
     "fomo_centered_d2": (C.c_int, [_i, _p, _p]),
     "fomo_scalar2d": (C.c_int, [C.POINTER(Desc), _p, _p, _p, _p, _p, _p, _p, C.POINTER(C.c_double)]),

@@ -57,6 +57,12 @@ int fomo_device_count(void);
  * `bytes` between two device buffers `iters` times with a float4 streaming kernel and writes the
  * average milliseconds per copy.  Copy bandwidth = 2 * bytes / time. */
 int fomo_stream_copy(int64_t bytes, int32_t iters, double *ms_per_copy);
+/* A sweep's access pattern without its stencil (no counterpart in the reference; a measurement
+ * helper for the roofline): n_read arrays read and n_write written, `bytes_per_array` each, one
+ * float per lane per access as the sweeps load, `iters` passes; writes the average milliseconds
+ * per pass.  Bandwidth = (n_read + n_write) * bytes_per_array / time.  1 <= n_write <= 4,
+ * n_write <= n_read <= 8. */
+int fomo_stream_probe(int32_t n_read, int32_t n_write, int64_t bytes_per_array, int32_t iters, double *ms_per_pass);
 
 /* ---- host-side setup helpers (no GPU needed) ---------------------------------------------- */
 
