@@ -106,21 +106,25 @@ def probe(args):
     nt = args.steps
     w = fomo.ricker_wavelet(15.0, 1e-3, nt)
     res = {}
+    storages = args.probe_storage2d.split(",")
     for eq, name, bv, bs in ((L.ACOUSTIC2D, "acoustic2d", 28, 20), (L.ELASTIC2D, "elastic2d", BYTES_VEL, BYTES_STR),
                              (L.SCALAR2D, "scalar2d", 16, 0)):
-        plan = Plan(eq, vp, rho, 10.0, 1e-3, nt, vs=vs if eq == L.ELASTIC2D else None, sx=[n // 2], sz=[n // 2],
-                    rx=[1, n], rz=[1, n], wavelet=w[None], nbc=50, fd_order=8)
-        cells = plan.padded_cells
-        plan.advance(10)
-        plan.sync()
-        ms = plan.timed_advance(nt)
-        tv, ts, tr = plan.kernel_times(max(10, nt // 4))
-        plan.close()
-        res[name] = {"padded_cells": cells, "ms_per_step": ms / nt, "gcell_per_s": cells * nt / ms / 1e6,
-                     "step_gbs": (bv + bs) * cells * nt / ms / 1e6,
-                     "vel_ms": tv, "vel_gbs": bv * cells / tv / 1e6, "prs_ms": ts,
-                     "prs_gbs": bs * cells / ts / 1e6 if bs else 0.0, "gather_ms": tr}
-        log(f"{name}: {res[name]}")
+        for st in storages:
+            eb = 1.0 if st == "f32" else 0.5  # 16-bit storage halves every array's bytes
+            plan = Plan(eq, vp, rho, 10.0, 1e-3, nt, vs=vs if eq == L.ELASTIC2D else None, sx=[n // 2], sz=[n // 2],
+                        rx=[1, n], rz=[1, n], wavelet=w[None], nbc=50, fd_order=8, storage=st)
+            cells = plan.padded_cells
+            plan.advance(10)
+            plan.sync()
+            ms = plan.timed_advance(nt)
+            tv, ts, tr = plan.kernel_times(max(10, nt // 4))
+            plan.close()
+            key = name if st == "f32" else f"{name}_{st}"
+            res[key] = {"padded_cells": cells, "ms_per_step": ms / nt, "gcell_per_s": cells * nt / ms / 1e6,
+                        "step_gbs": eb * (bv + bs) * cells * nt / ms / 1e6,
+                        "vel_ms": tv, "vel_gbs": eb * bv * cells / tv / 1e6, "prs_ms": ts,
+                        "prs_gbs": eb * bs * cells / ts / 1e6 if bs else 0.0, "gather_ms": tr}
+            log(f"{key}: {res[key]}")
     print(json.dumps({"probe": f"{n}x{n} interior, fd8, nbc 50", **res}), flush=True)
 
 
@@ -332,6 +336,7 @@ def main():
     ap.add_argument("--probe3d", action="store_true", help="acoustic3d 512^3 probe (f32/f16/bf16)")
     ap.add_argument("--probe-n3", type=int, default=512)
     ap.add_argument("--probe-storage", default="f32,f16,bf16", help="storages of the 3D probe")
+    ap.add_argument("--probe-storage2d", default="f32", help="storages of the 2D probe (f32,f16,bf16)")
     ap.add_argument("--stream-copy", action="store_true", help="only the HBM streaming-copy calibration")
     ap.add_argument("--learned", action="store_true", help="learned nbc-16 boundary vs plain HABC nbc 100 (cfg5)")
     ap.add_argument("--probe-tile", type=int, default=0, help="3D z-segment choice (0 default, k: 16<<(k-1) planes)")
