@@ -145,8 +145,10 @@ def learned_bench(args):
     rx = np.arange(1, nx + 1, 2)
     weights = learned.load_weights()
     res = {}
+    ladder = (20, 24, 28, 32, 40)  # plain HABC widths between 16 and 100, for the equal-reflection comparison
     rows = (("habc_nbc100", 100, False, "f32"), ("learned_nbc16", 16, True, "f32"), ("habc_nbc16", 16, False, "f32"),
-            ("learned_nbc16_f16", 16, True, "f16"), ("habc_nbc100_f16", 100, False, "f16"))
+            ("learned_nbc16_f16", 16, True, "f16"), ("habc_nbc100_f16", 100, False, "f16")) + tuple(
+                (f"habc_nbc{b}", b, False, "f32") for b in ladder)
     for name, nbc, use, storage in rows:
         p = Plan(L.ACOUSTIC2D, vp, rho, 10.0, dt, nt, sx=[nx // 2], sz=[50], rx=rx, rz=np.full(len(rx), 10),
                  wavelet=w[None, :nt], nbc=nbc, fd_order=8, storage=storage)
@@ -164,11 +166,20 @@ def learned_bench(args):
     refl = learned.reflection_residual(vp2, rho2, sx=sx, sz=sz, rx=rx2, rz=rz2, nt=1500, L=270, judge="learned_nbc16",
                                        boundaries=[("habc_nbc100", 100, None), ("learned_nbc16", 16, weights),
                                                    ("habc_nbc16", 16, None), ("learned_nbc16_f16", 16, weights, "f16"),
-                                                   ("habc_nbc100_f16", 100, None, "f16")])
+                                                   ("habc_nbc100_f16", 100, None, "f16")]
+                                       + [(f"habc_nbc{b}", b, None) for b in ladder])
     for k, v in refl.items():
         res[k]["reflection_db"] = v["reflection_db"]
         if k == "learned_nbc16_f16":
             res[k]["trace_rel_l2_vs_learned_nbc16"] = v["vs_learned_nbc16"]
+    # equal reflection: the narrowest plain HABC at least as absorbing as the learned boundary
+    target = res["learned_nbc16"]["reflection_db"]
+    match = next((f"habc_nbc{b}" for b in ladder + (100,) if res[f"habc_nbc{b}"]["reflection_db"] <= target), None)
+    if match:
+        res["equal_reflection"] = {"habc": match, "habc_db": res[match]["reflection_db"], "learned_db": target,
+                                   "learned_speedup": res[match]["ms_per_step"] / res["learned_nbc16"]["ms_per_step"],
+                                   "learned_f16_speedup": res[match]["ms_per_step"] /
+                                   res["learned_nbc16_f16"]["ms_per_step"]}
     if not args.no_cpu:
         import oracle
         oracle.lib("f32")
