@@ -203,7 +203,8 @@ def learned_bench(args):
 
 def batch_bench(args):
     """Uneven multi-shot batch: a 4000x1000 seeded layered model (the 2000x800 model's rules), dh 10
-    m, dt 0.8 ms, fd8, nbc 40, 64 acoustic shots in one windowed plan.  Window widths are drawn
+    m, dt 0.8 ms, fd8, nbc 40, 64 shots (acoustic, or elastic with --batch-eq elastic2d) in one
+    windowed plan.  Window widths are drawn
     uniformly from 256..1536 and window starts uniformly over the model (SplitMix64, seeded); each
     shot has its source at the window centre 5 cells deep and receivers on every second cell at
     depth 5.  Timed with the autotuned strip length and with each fixed one, so the line shows
@@ -217,7 +218,10 @@ def batch_bench(args):
     nx, nz, dt, nbc = 4000, 1000, 8e-4, 40
     nt = min(args.steps, 3000)
     log("building the seeded 4000x1000 layered model")
-    vp, _, rho = layered_2d(nx, nz, seed=SEED)
+    vp, vs, rho = layered_2d(nx, nz, seed=SEED)
+    elastic = args.batch_eq == "elastic2d"
+    eq2 = L.ELASTIC2D if elastic else L.ACOUSTIC2D
+    bv, bp = (BYTES_VEL, BYTES_STR) if elastic else (28, 20)
     w = fomo.ricker_wavelet(15.0, dt, nt)
     rng = SplitMix64(SEED + 3)
     shots = []
@@ -229,7 +233,8 @@ def batch_bench(args):
     res = {}
     for cfg in (0, 1, 2, 3):
         log(f"64-shot plan, tile_cfg {cfg} (0: autotuned): setup, graph capture, {nt} timed steps")
-        plan = Plan(L.ACOUSTIC2D, vp, rho, 10.0, dt, nt, shots=shots, nbc=nbc, fd_order=8, tile_cfg=cfg)
+        plan = Plan(eq2, vp, rho, 10.0, dt, nt, vs=vs if elastic else None, shots=shots, nbc=nbc, fd_order=8,
+                    tile_cfg=cfg)
         cells = plan.padded_cells
         plan.advance(min(20, nt))
         plan.sync()
@@ -239,9 +244,10 @@ def batch_bench(args):
         tv, tp, tr = plan.kernel_times(min(50, nt))
         plan.close()
         res[cfg] = {"ms_per_step": ms / nt, "cell_updates_per_s": cells * nt / (ms * 1e-3),
-                    "step_gbs": 48 * cells * nt / ms / 1e6, "vel_ms": tv, "vel_gbs": 28 * cells / tv / 1e6,
-                    "prs_ms": tp, "prs_gbs": 20 * cells / tp / 1e6, "step_rest_ms": tr}
-    out = {"batch": f"64 uneven windows (widths 256..1536) of a 4000x1000 model, acoustic fd8, nbc 40, dt 0.8 ms, "
+                    "step_gbs": (bv + bp) * cells * nt / ms / 1e6, "vel_ms": tv, "vel_gbs": bv * cells / tv / 1e6,
+                    "prs_ms": tp, "prs_gbs": bp * cells / tp / 1e6, "step_rest_ms": tr}
+    out = {"batch": f"64 uneven windows (widths 256..1536) of a 4000x1000 model, {args.batch_eq} fd8, nbc 40, "
+                    f"dt 0.8 ms, "
                     f"{nt} steps", "padded_cells": cells, "autotuned": res[0],
            "fixed_strip_rows": {str(16 << (c - 1)): res[c] for c in (1, 2, 3)},
            "widths": [s["nx"] for s in shots]}
@@ -256,14 +262,19 @@ def batch_bench(args):
             t = time.perf_counter()
             for s in sample:
                 x0, wd = s["x0"], s["nx"]
-                oracle.acoustic2d(vp[x0 - 1:x0 - 1 + wd], rho[x0 - 1:x0 - 1 + wd], 10.0, dt, k, sx=s["sx"], sz=s["sz"],
-                                  rx=s["rx"], rz=s["rz"], wavelet=wc[None], nbc=nbc, fd_order=8, nthreads=th)
+                win = slice(x0 - 1, x0 - 1 + wd)
+                kw = dict(sx=s["sx"], sz=s["sz"], rx=s["rx"], rz=s["rz"], wavelet=wc[None], nbc=nbc, fd_order=8,
+                          nthreads=th)
+                if elastic:
+                    oracle.elastic2d(vp[win], vs[win], rho[win], 10.0, dt, k, **kw)
+                else:
+                    oracle.acoustic2d(vp[win], rho[win], 10.0, dt, k, **kw)
             return time.perf_counter() - t
         cells8 = sum((s["nx"] + 2 * (nbc + 4)) * (nz + 2 * (nbc + 4)) for s in sample)
-        log(f"cpu baseline: oracle acoustic2d FP32 on {th} threads over the first 8 windows")
+        log(f"cpu baseline: oracle {args.batch_eq} FP32 on {th} threads over the first 8 windows")
         rate, ns, secs = cpu_rate(run, cells8, 10, 60)
         out["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": th, "kind": "port",
-                               "sample": f"oracle acoustic2d FP32 (C, -O3, OpenMP {th} threads) on the first 8 windows "
+                               "sample": f"oracle {args.batch_eq} FP32 (C, -O3, OpenMP {th} threads) on the first 8 windows "
                                          f"run one after another, {ns} steps timed against 10-step runs ({secs:.1f} s)"}
     print(json.dumps(out), flush=True)
 
@@ -279,8 +290,10 @@ def stream_copy_gbs(nbytes=2 << 30, iters=20):
 
 
 def probe3d(args):
-    """acoustic3d at the BASELINE 3D size (512^3 interior, nbc 32, fd8; 584^3 padded) for each
-    storage precision: per-kernel times and algorithmic GB/s (64 B/cell FP32, 32 B/cell 16-bit)."""
+    """acoustic3d (or elastic3d with --probe-eq elastic3d) at the 3D workload's size (512^3
+    interior, nbc 32, fd8; 584^3 padded) for each storage precision: per-kernel times and
+    algorithmic GB/s (acoustic 64 B/cell FP32 = 40 velocity + 24 pressure; elastic 140 B/cell =
+    60 velocity + 80 stress; 16-bit storage halves them)."""
     import numpy as np
 
     import fomo
@@ -290,6 +303,10 @@ def probe3d(args):
     n = args.probe_n3
     log(f"building the seeded {n}^3 layered model")
     vp, rho = layered_3d(n, n, n, seed=SEED)
+    elastic = args.probe_eq == "elastic3d"
+    vs = (vp / np.float32(1.73)).astype(np.float32) if elastic else None
+    eq3 = L.ELASTIC3D if elastic else L.ACOUSTIC3D
+    bv, bs = (15, 20) if elastic else (10, 6)  # 4-byte words per cell of the two sweeps
     nt = args.steps
     w = fomo.ricker_wavelet(12.0, 8e-4, nt)
     rx = np.arange(1, n + 1, 2)
@@ -299,7 +316,7 @@ def probe3d(args):
     for name in args.probe_storage.split(","):
         storage = codes[name]
         eb = 4 if storage == L.STORE_F32 else 2
-        plan = Plan(L.ACOUSTIC3D, vp, rho, 10.0, 8e-4, nt, sx=[n // 2], sy=[n // 2], sz=[1], rx=gx.ravel(),
+        plan = Plan(eq3, vp, rho, 10.0, 8e-4, nt, vs=vs, sx=[n // 2], sy=[n // 2], sz=[1], rx=gx.ravel(),
                     ry=gy.ravel(), rz=np.ones(gx.size, int), wavelet=w[None], nbc=32, fd_order=8, storage=storage,
                     tile_cfg=args.probe_tile)
         cells = plan.padded_cells
@@ -309,8 +326,9 @@ def probe3d(args):
         tv, tp, tr = plan.kernel_times(max(4, nt // 4))
         plan.close()
         res[name] = {"padded_cells": cells, "ms_per_step": ms / nt, "gcell_per_s": cells * nt / ms / 1e6,
-                     "step_gbs": 16 * eb * cells * nt / ms / 1e6, "vel_ms": tv, "vel_gbs": 10 * eb * cells / tv / 1e6,
-                     "prs_ms": tp, "prs_gbs": 6 * eb * cells / tp / 1e6, "gather_ms": tr}
+                     "step_gbs": (bv + bs) * eb * cells * nt / ms / 1e6, "vel_ms": tv,
+                     "vel_gbs": bv * eb * cells / tv / 1e6, "prs_ms": tp, "prs_gbs": bs * eb * cells / tp / 1e6,
+                     "gather_ms": tr}
         log(f"{name}: {res[name]}")
     gbs, cms = stream_copy_gbs()
     res["stream_copy"] = {"bytes": 2 << 30, "ms": cms, "gbs": gbs}
@@ -322,16 +340,20 @@ def probe3d(args):
 
         def run(k):
             wc = fomo.ricker_wavelet(12.0, 8e-4, k)
+            kw = dict(sx=[n // 2], sy=[n // 2], sz=[1], rx=gx.ravel(), ry=gy.ravel(), rz=np.ones(gx.size, int),
+                      wavelet=wc[None], nbc=32, fd_order=8, nthreads=th)
             t = time.perf_counter()
-            oracle.acoustic3d(vp, rho, 10.0, 8e-4, k, sx=[n // 2], sy=[n // 2], sz=[1], rx=gx.ravel(), ry=gy.ravel(),
-                              rz=np.ones(gx.size, int), wavelet=wc[None], nbc=32, fd_order=8, nthreads=th)
+            if elastic:
+                oracle.elastic3d(vp, vs, rho, 10.0, 8e-4, k, **kw)
+            else:
+                oracle.acoustic3d(vp, rho, 10.0, 8e-4, k, **kw)
             return time.perf_counter() - t
-        log(f"cpu baseline: oracle acoustic3d FP32 on {th} threads")
+        log(f"cpu baseline: oracle {args.probe_eq} FP32 on {th} threads")
         rate, ns, secs = cpu_rate(run, res[next(iter(res))]["padded_cells"], 1, 3, budget_s=10.0)
         res["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": th, "kind": "port",
-                               "sample": f"oracle acoustic3d FP32 (C, -O3, OpenMP {th} threads), same {n}^3 "
+                               "sample": f"oracle {args.probe_eq} FP32 (C, -O3, OpenMP {th} threads), same {n}^3 "
                                          f"workload, {ns} steps timed against a 1-step run ({secs:.1f} s of stepping)"}
-    print(json.dumps({"probe3d": f"{n}^3 interior, fd8, nbc 32", **res}), flush=True)
+    print(json.dumps({"probe3d": f"{args.probe_eq} {n}^3 interior, fd8, nbc 32", **res}), flush=True)
 
 
 def main():
@@ -348,6 +370,8 @@ def main():
     ap.add_argument("--probe-n3", type=int, default=512)
     ap.add_argument("--probe-storage", default="f32,f16,bf16", help="storages of the 3D probe")
     ap.add_argument("--probe-storage2d", default="f32", help="storages of the 2D probe (f32,f16,bf16)")
+    ap.add_argument("--probe-eq", default="acoustic3d", choices=["acoustic3d", "elastic3d"], help="3D probe equation")
+    ap.add_argument("--batch-eq", default="acoustic2d", choices=["acoustic2d", "elastic2d"], help="batch equation")
     ap.add_argument("--stream-copy", action="store_true", help="only the HBM streaming-copy calibration")
     ap.add_argument("--learned", action="store_true", help="learned nbc-16 boundary vs plain HABC nbc 100 (cfg5)")
     ap.add_argument("--probe-tile", type=int, default=0, help="3D z-segment choice (0 default, k: 16<<(k-1) planes)")
