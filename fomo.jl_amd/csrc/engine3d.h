@@ -42,7 +42,9 @@ struct Work3D {
 
 // Rows of a tile (thread rows x rows per thread): acoustic 8 x 2 (512-thread blocks; 2 rows per
 // thread keep the per-thread state small enough for 4 waves per SIMD, 16 rows halve the share of
-// y-halo reads), elastic 4 x 2.
+// y-halo reads), elastic 8 x 1 (its z-queues and prefetch registers per row are three times the
+// acoustic ones: one row per thread keeps it near 128 VGPRs, 3-4 waves per SIMD, where two rows
+// per thread needed 224-234 and ran 2 waves per SIMD, 9 % slower).
 constexpr int kRowsAc3 = 16, kRowsEl3 = 8;
 // Core tiles (user interior, no frame logic): acoustic 16 x 2 rows (1024-thread blocks, y-halo
 // reads a quarter of the plane), elastic as its frame tiles.

@@ -71,7 +71,7 @@ struct P3<kA3Prs> {
 };
 template <>
 struct P3<kE3Vel> {
-  static constexpr int NF = 3, NS = 5, NQ = 3, YPT = 2, NM = 3, TR = 4, TRC = 4;
+  static constexpr int NF = 3, NS = 5, NQ = 3, YPT = 1, NM = 3, TR = 8, TRC = 8;
   __device__ static constexpr int mat_id(int k) { return k; }  // bx by bz
   static constexpr bool kNew = false;
   __device__ static constexpr int out_f(int k) { return k; }  // vx vy vz
@@ -88,7 +88,7 @@ struct P3<kE3Vel> {
 };
 template <>
 struct P3<kE3Str> {
-  static constexpr int NF = 6, NS = 3, NQ = 3, YPT = 2, NM = 5, TR = 4, TRC = 4;
+  static constexpr int NF = 6, NS = 3, NQ = 3, YPT = 1, NM = 5, TR = 8, TRC = 8;
   __device__ static constexpr int mat_id(int k) { return 3 + k; }  // lam lam2mu mu_txy mu_txz mu_tyz
   static constexpr bool kNew = true;
   __device__ static constexpr int out_f(int k) { return 3 + k; }  // txx tyy tzz txy txz tyz
