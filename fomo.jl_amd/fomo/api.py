@@ -539,6 +539,8 @@ class Plan:
         num = np.zeros(n, np.float64)
         den = np.zeros(n, np.float64)
         t1 = self.nt if t1 is None else int(t1)
+        if other is not None and not getattr(other, "_h", None):
+            raise _lib.ArgumentError("the other plan is closed")
         ob = None if other is None else other._h
         _lib.check(_lib.lib().fomo_trace_misfit(self._h, int(shot), ob, int(other_shot), int(slot), int(t0), t1,
                                                 num.ctypes.data, den.ctypes.data))

@@ -126,6 +126,9 @@ def test_misfit_argument_errors(gpu):
             a.trace_misfit(None, t1=101)
         with pytest.raises(fomo.ArgumentError):
             a.trace_misfit(None, shot=1)
+        c.close()
+        with pytest.raises(fomo.ArgumentError):
+            a.trace_misfit(c)  # a closed plan is an error, not a missing one
     finally:
         for p in (a, b, c):
             p.close()

@@ -81,6 +81,19 @@ def test_trace_misfit_rejects_null_arguments():
     assert "NULL" in L.lib().fomo_last_error().decode()
 
 
+def test_stream_probe_rejects_bad_shapes():
+    """The calibration entry points check their arguments before touching a device."""
+    import ctypes as C
+
+    import fomo._lib as L
+    ms = C.c_double(0.0)
+    for nr, nw, nb, it in ((0, 1, 1024, 1), (9, 1, 1024, 1), (2, 3, 1024, 1), (5, 0, 1024, 1), (5, 5, 1024, 1),
+                           (5, 2, 2, 1), (5, 2, 1024, 0)):
+        assert L.lib().fomo_stream_probe(nr, nw, nb, it, C.byref(ms)) == L.FOMO_EINVAL, (nr, nw, nb, it)
+    assert L.lib().fomo_stream_probe(5, 2, 1024, 1, None) == L.FOMO_EINVAL
+    assert L.lib().fomo_stream_copy(8, 1, C.byref(ms)) == L.FOMO_EINVAL
+
+
 def test_cfl_bound_matches_reference_formula():
     """dt_max = dh / (vmax sqrt(2) sum|a|): just below passes, just above raises."""
     from fomo.checks import check_numerics
