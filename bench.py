@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark: elastic2d time steps on one MI355X, with roofline and CPU-baseline figures.
 
-Workload (the BASELINE plain-elastic2d configuration): interior 2000 x 800, dh 5 m, dt 0.4 ms,
+Workload (plain elastic2d, FP32): interior 2000 x 800, dh 5 m, dt 0.4 ms,
 fd_order 8, nbc 50 (padded 2108 x 908 = 1,914,064 cells), seeded random layered model (12
 jittered interfaces, vp U[1800,4500], vs = vp/1.73, Gardner rho, vacuum top row), vertical-force
 Ricker 20 Hz at (1000, 5), 2000 receivers at z = 5.  A step is one full time step (velocity sweep,
@@ -129,9 +129,10 @@ def probe(args):
 
 
 def learned_bench(args):
-    """cfg5: acoustic2d 2400x1200, dh 10 m, dt 0.8 ms, FD8, seeded layered model; per-step time of
-    plain HABC at nbc 100 and of the learned boundary at nbc 16 (MLP inference included), and the
-    reflection of both in the reference's absorption experiment (test/verify_patch.jl:76-91)."""
+    """Learned thin boundary: acoustic2d 2400x1200, dh 10 m, dt 0.8 ms, FD8, seeded layered model;
+    per-step time of plain HABC at nbc 100 and of the learned boundary at nbc 16 (MLP inference
+    included), and the reflection of both in the reference's absorption experiment
+    (test/verify_patch.jl:76-91)."""
     import numpy as np
 
     import fomo
@@ -373,7 +374,7 @@ def main():
     ap.add_argument("--probe-eq", default="acoustic3d", choices=["acoustic3d", "elastic3d"], help="3D probe equation")
     ap.add_argument("--batch-eq", default="acoustic2d", choices=["acoustic2d", "elastic2d"], help="batch equation")
     ap.add_argument("--stream-copy", action="store_true", help="only the HBM streaming-copy calibration")
-    ap.add_argument("--learned", action="store_true", help="learned nbc-16 boundary vs plain HABC nbc 100 (cfg5)")
+    ap.add_argument("--learned", action="store_true", help="learned nbc-16 boundary vs plain HABC nbc 100 (acoustic2d 2400x1200)")
     ap.add_argument("--probe-tile", type=int, default=0, help="3D z-segment choice (0 default, k: 16<<(k-1) planes)")
     ap.add_argument("--copy-mb", default="", help="with --stream-copy: comma-separated copy sizes in MB")
     ap.add_argument("--stream-probe", action="store_true",

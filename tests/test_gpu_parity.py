@@ -213,7 +213,7 @@ def test_mirror_symmetry_gpu(gpu):
     assert rel_sym(e.seis_vz) < 1e-5 and rel_asym_vx(e.seis_vx) < 1e-5
 
 
-def test_cfg1_acoustic_full_bitwise(gpu, orc):
+def test_acoustic_400x300_full_bitwise(gpu, orc):
     """Plain acoustic2d 400x300, dh 10, dt 1 ms, nt 2000, fd8, nbc 100, vp 2500 / rho 2000, vacuum
     top row, Ricker 15 Hz at (200,10), receivers x = 1:2:400 at z = 10 (p, vx, vz)."""
     import fomo
@@ -233,7 +233,7 @@ def test_cfg1_acoustic_full_bitwise(gpu, orc):
         assert_parity(getattr(res, k), ref[k])
 
 
-def test_cfg2_elastic_full_grid_prefix_bitwise(gpu, orc):
+def test_elastic_2000x800_full_grid_prefix_bitwise(gpu, orc):
     """Plain elastic2d at the full 2000x800 grid (dh 5, dt 0.4 ms, fd8, nbc 50, seeded layered
     model, vacuum top, vertical-force Ricker 20 Hz at (1000,5), receivers at every cell of z=5):
     the first 300 steps bitwise against the oracle; the full 4000 steps are checked for finite,

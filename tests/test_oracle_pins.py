@@ -65,7 +65,7 @@ def test_habc_coefficients_closed_form(orc):
         assert np.allclose(got, ex, rtol=1e-6, atol=0)
         got32 = orc.habc_coeffs(vref, dt, dh, "f32")
         assert np.allclose(got32, ex, rtol=2e-6, atol=0)
-    # cfg1: r = 0.25
+    # the 400x300 constant model (vp 2500, dh 10, dt 1 ms): r = 0.25
     q, qt, qxt = orc.habc_coeffs(2500, 1e-3, 10, "f64")
     assert abs(q - 5 / 11) < 1e-6 and abs(qt - (-7 / 11)) < 1e-6 and abs(qxt - (-9 / 11)) < 1e-7
 
