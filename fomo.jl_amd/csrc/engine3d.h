@@ -74,6 +74,11 @@ struct Gather3DArgs {
   int nt;
   const int* step;  // the state after step *step + koff goes into that column
   int koff;
+  // Snapshots, written by the same launch after every snap_every-th step (none when snap is
+  // NULL): the interior slice of field snap_f in plane snap_plane (0 xz, 1 xy, 2 yz) at padded
+  // index snap_idx, decoded to Float32, snapshot k at snap + k * (cells of one slice).
+  float* snap;
+  int snap_every, snap_n, snap_plane, snap_idx, snap_f;
 };
 
 // eq: FOMO_ACOUSTIC3D / FOMO_ELASTIC3D; storage: FOMO_STORE_F32 / F16 / BF16; phase 0 velocity,
@@ -83,7 +88,5 @@ hipError_t launch_sweep3d(int eq, int storage, int phase, int N, const Sweep3DAr
                           hipStream_t s);
 hipError_t launch_gather3d(int storage, const Gather3DArgs& a, hipStream_t s);
 hipError_t launch_decode3d(int storage, const void* src, float* dst, size_t n, float dec, hipStream_t s);
-hipError_t launch_slice3d(int storage, const void* src, float* dst, int nx, int ny, int nz, int pad, int plane,
-                          int idx, float dec, hipStream_t s);
 
 }  // namespace fomo

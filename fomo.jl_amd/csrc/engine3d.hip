@@ -494,18 +494,40 @@ __global__ __launch_bounds__((kTileW * kTR<PH, LAG>)) void sweep3d_kernel(const 
   march3d<PH, N, T, LAG>(A, W, lds);
 }
 
+// Interior cells of a slice: a_n x b_n with a fastest (xz: x, z; xy: x, y; yz: y, z).
+__device__ __forceinline__ void slice_dims(int plane, int ni, int nj, int nk, int& a_n, int& b_n) {
+  a_n = plane == 2 ? nj : ni;
+  b_n = plane == 1 ? nj : nk;
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void gather3d_kernel(const Gather3DArgs A) {
   const int it = *A.step + A.koff;
   const Shot3D& S = A.shot;
-  if (it < A.nt) {
-    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < A.n_rec; r += gridDim.x * blockDim.x) {
-      const size_t c = (size_t)G(A.rec_c)[r];
-      const size_t o = (size_t)r + (size_t)A.n_rec * (size_t)it;
-      for (int k = 0; k < A.nslots; ++k) {
-        const int f = A.slot_field[k];
-        G(A.seis[k])[o] = Cvt<T>::ld(S.fld[A.parity_out][f], c, S.fdec[f]);
-      }
+  if (it >= A.nt) return;
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < A.n_rec; r += gridDim.x * blockDim.x) {
+    const size_t c = (size_t)G(A.rec_c)[r];
+    const size_t o = (size_t)r + (size_t)A.n_rec * (size_t)it;
+    for (int k = 0; k < A.nslots; ++k) {
+      const int f = A.slot_field[k];
+      G(A.seis[k])[o] = Cvt<T>::ld(S.fld[A.parity_out][f], c, S.fdec[f]);
+    }
+  }
+  if (!A.snap) return;  // snap_every is set (>= 1) only with a snapshot buffer
+  const int ks = (it + 1) / A.snap_every;
+  if ((it + 1) % A.snap_every == 0 && ks <= A.snap_n) {
+    const int pad = S.pad, ni = S.nx - 2 * pad, nj = S.ny - 2 * pad, nk = S.nz - 2 * pad;
+    int a_n, b_n;
+    slice_dims(A.snap_plane, ni, nj, nk, a_n, b_n);
+    const size_t n = (size_t)a_n * b_n;
+    float* dst = A.snap + (size_t)(ks - 1) * n;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+      const int aa = (int)(e % a_n), bb = (int)(e / a_n);
+      const int i = A.snap_plane == 2 ? A.snap_idx : pad + aa;
+      const int j = A.snap_plane == 0 ? A.snap_idx : (A.snap_plane == 1 ? pad + bb : pad + aa);
+      const int k = A.snap_plane == 1 ? A.snap_idx : pad + bb;
+      const size_t c = (size_t)i + (size_t)j * S.nx + (size_t)k * S.nx * S.ny;
+      G(dst)[e] = Cvt<T>::ld(S.fld[A.parity_out][A.snap_f], c, S.fdec[A.snap_f]);
     }
   }
 }
@@ -517,29 +539,6 @@ __global__ void decode_kernel(const void* src, float* dst, size_t n, float dec) 
     dst[c] = Cvt<T>::ld(src, c, dec);
 }
 
-// Pack a slice of a padded field (interior part) into dst, decoded to Float32.
-// plane 0: xz at y index `idx`; 1: xy at z index `idx`; 2: yz at x index `idx` (padded index).
-template <class T>
-__global__ void slice_kernel(const void* src, float* dst, int nx, int ny, int nz, int pad, int plane, int idx,
-                             float dec) {
-  const int ni = nx - 2 * pad, nj = ny - 2 * pad, nk = nz - 2 * pad;
-  const int a_n = plane == 2 ? nj : ni;
-  const int b_n = plane == 1 ? nj : nk;
-  const size_t n = (size_t)a_n * b_n;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-    const int aa = (int)(e % a_n), bb = (int)(e / a_n);
-    int i, j, k;
-    if (plane == 0) {
-      i = pad + aa; j = idx; k = pad + bb;
-    } else if (plane == 1) {
-      i = pad + aa; j = pad + bb; k = idx;
-    } else {
-      i = idx; j = pad + aa; k = pad + bb;
-    }
-    dst[e] = Cvt<T>::ld(src, (size_t)i + (size_t)j * nx + (size_t)k * nx * ny, dec);
-  }
-}
-
 hipError_t launch_decode3d(int storage, const void* src, float* dst, size_t n, float dec, hipStream_t s) {
   const int blocks = (int)std::min<size_t>(4096, (n + 255) / 256);
   switch (storage) {
@@ -547,25 +546,6 @@ hipError_t launch_decode3d(int storage, const void* src, float* dst, size_t n, f
     case FOMO_STORE_F16: hipLaunchKernelGGL(decode_kernel<__half>, dim3(blocks), dim3(256), 0, s, src, dst, n, dec); break;
     case FOMO_STORE_BF16:
       hipLaunchKernelGGL(decode_kernel<__hip_bfloat16>, dim3(blocks), dim3(256), 0, s, src, dst, n, dec);
-      break;
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
-
-hipError_t launch_slice3d(int storage, const void* src, float* dst, int nx, int ny, int nz, int pad, int plane,
-                          int idx, float dec, hipStream_t s) {
-  const int blocks = 1024;
-  switch (storage) {
-    case FOMO_STORE_F32:
-      hipLaunchKernelGGL(slice_kernel<float>, dim3(blocks), dim3(256), 0, s, src, dst, nx, ny, nz, pad, plane, idx, dec);
-      break;
-    case FOMO_STORE_F16:
-      hipLaunchKernelGGL(slice_kernel<__half>, dim3(blocks), dim3(256), 0, s, src, dst, nx, ny, nz, pad, plane, idx, dec);
-      break;
-    case FOMO_STORE_BF16:
-      hipLaunchKernelGGL(slice_kernel<__hip_bfloat16>, dim3(blocks), dim3(256), 0, s, src, dst, nx, ny, nz, pad, plane,
-                         idx, dec);
       break;
     default: return hipErrorInvalidValue;
   }
@@ -612,7 +592,14 @@ hipError_t launch_sweep3d(int eq, int storage, int phase, int N, const Sweep3DAr
 }
 
 hipError_t launch_gather3d(int storage, const Gather3DArgs& a, hipStream_t s) {
-  const int blocks = a.n_rec > 0 ? std::min(1024, (a.n_rec + 255) / 256) : 1;
+  int blocks = a.n_rec > 0 ? std::min(1024, (a.n_rec + 255) / 256) : 1;
+  if (a.snap) {
+    const int pad = a.shot.pad;
+    const long long cells = a.snap_plane == 0 ? (long long)(a.shot.nx - 2 * pad) * (a.shot.nz - 2 * pad)
+                            : a.snap_plane == 1 ? (long long)(a.shot.nx - 2 * pad) * (a.shot.ny - 2 * pad)
+                                                : (long long)(a.shot.ny - 2 * pad) * (a.shot.nz - 2 * pad);
+    blocks = std::max(blocks, (int)std::min<long long>(512, (cells + 1023) / 1024));
+  }
   switch (storage) {
     case FOMO_STORE_F32: hipLaunchKernelGGL(gather3d_kernel<float>, dim3(blocks), dim3(256), 0, s, a); break;
     case FOMO_STORE_F16: hipLaunchKernelGGL(gather3d_kernel<__half>, dim3(blocks), dim3(256), 0, s, a); break;

@@ -43,8 +43,13 @@ struct Plan3D : PlanBase {
   long long* rec_c = nullptr;
   float* seis[4] = {nullptr};
   int nslots = 3;
-  float* scratch = nullptr;  // slice / decode staging
+  float* scratch = nullptr;  // decode staging
   size_t scratch_n = 0;
+  // Snapshots (single-shot runs with snap_interval > 0): written by the per-step gather launch
+  // into this device buffer, graph replays included, and downloaded once after the step loop.
+  float* snap = nullptr;
+  int snap_n = 0, snap_plane = 0, snap_idx = 0;  // snap_idx: padded index of the slice
+  size_t snap_cells = 0;                          // cells of one slice
 
   hipError_t enqueue_part(int part, int s, int koff) override {
     if (part == 2) {
@@ -63,6 +68,14 @@ struct Plan3D : PlanBase {
       g.nt = d.nt;
       g.step = step;
       g.koff = koff;
+      if (snap) {
+        g.snap = snap;
+        g.snap_every = d.snap_interval;
+        g.snap_n = snap_n;
+        g.snap_plane = snap_plane;
+        g.snap_idx = snap_idx;
+        g.snap_f = snap_f;
+      }
       return launch_gather3d(storage, g, stream);
     }
     Sweep3DArgs a{};
@@ -133,18 +146,6 @@ struct Plan3D : PlanBase {
     FOMO_HIP_TRY(launch_decode3d(storage, shot.fld[steps_done % 2][which], scratch, n, shot.fdec[which], stream));
     FOMO_HIP_TRY(hipMemcpyAsync(out, scratch, n * sizeof(float), hipMemcpyDeviceToHost, stream));
     FOMO_HIP_TRY(hipStreamSynchronize(stream));
-    return FOMO_OK;
-  }
-
-  // Interior slice of the snapshot field (acoustic p, elastic vz; plane 0 xz, 1 xy, 2 yz; idx1 =
-  // 1-based interior index) into host dst.
-  int snapshot(float* dst, int plane, int idx1) {
-    const size_t n = plane == 0 ? (size_t)nxi * nzi : (plane == 1 ? (size_t)nxi * nyi : (size_t)nyi * nzi);
-    int rc = ensure_scratch(n);
-    if (rc) return rc;
-    FOMO_HIP_TRY(launch_slice3d(storage, shot.fld[steps_done % 2][snap_f], scratch, shot.nx, shot.ny, shot.nz, pad,
-                                plane, idx1 - 1 + pad, shot.fdec[snap_f], stream));
-    FOMO_HIP_TRY(hipMemcpyAsync(dst, scratch, n * sizeof(float), hipMemcpyDeviceToHost, stream));
     return FOMO_OK;
   }
 
@@ -290,6 +291,14 @@ int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, con
   for (int r = 0; r < d->n_rec; ++r)
     if (rx[r] < 1 || rx[r] > d->nx || ry[r] < 1 || ry[r] > d->ny || rz[r] < 1 || rz[r] > d->nz)
       return set_error(FOMO_EINVAL, "receiver index out of range");
+  // snapshot slice: plane 0 xz at y, 1 xy at z, 2 yz at x (1-based interior index; 0 = middle)
+  const int snap_idx1 = d->snap_index > 0 ? d->snap_index
+                                          : (d->snap_plane == 0 ? d->ny / 2 : (d->snap_plane == 1 ? d->nz / 2 : d->nx / 2));
+  if (d->snap_interval > 0) {
+    if (d->snap_plane < 0 || d->snap_plane > 2) return set_error(FOMO_EINVAL, "snap_plane must be 0 (xz), 1 (xy) or 2 (yz)");
+    const int axis = d->snap_plane == 0 ? d->ny : (d->snap_plane == 1 ? d->nz : d->nx);
+    if (snap_idx1 < 1 || snap_idx1 > axis) return set_error(FOMO_EINVAL, "snap_index out of range");
+  }
   Plan3D* p = new Plan3D();
   auto bail = [&](int code) {
     delete p;
@@ -506,6 +515,14 @@ int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, con
       if ((rc = p->alloc(&p->seis[k], (size_t)d->n_rec * d->nt))) return bail(rc);
   }
   if ((rc = p->alloc(&p->step, 1))) return bail(rc);
+  if (d->snap_interval > 0 && d->nt / d->snap_interval > 0) {
+    p->snap_plane = d->snap_plane;
+    p->snap_idx = snap_idx1 - 1 + pad;
+    p->snap_cells = d->snap_plane == 0 ? (size_t)d->nx * d->nz
+                    : (d->snap_plane == 1 ? (size_t)d->nx * d->ny : (size_t)d->ny * d->nz);
+    p->snap_n = d->nt / d->snap_interval;
+    if ((rc = p->alloc(&p->snap, p->snap_cells * p->snap_n))) return bail(rc);
+  }
   const int kseg = d->tile_cfg > 0 ? 16 << (d->tile_cfg - 1) : 32;
   if ((rc = make_work3d(p, kseg, elastic ? kRowsEl3 : kRowsAc3, elastic ? kRowsEl3Core : kRowsAc3Core))) return bail(rc);
   if ((rc = p->reset())) return bail(rc);
@@ -536,19 +553,7 @@ static int run3d(const fomo_desc* d, const float* vp, const float* vs, const flo
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     return finish(set_error(FOMO_EHIP, "hipEventCreate failed"));
   (void)hipEventRecord(e0, p->stream);
-  if (d->snap_interval > 0 && snaps) {
-    const int plane = d->snap_plane;
-    const size_t per = plane == 0 ? (size_t)d->nx * d->nz : (plane == 1 ? (size_t)d->nx * d->ny : (size_t)d->ny * d->nz);
-    const int idx = d->snap_index > 0 ? d->snap_index : (plane == 0 ? d->ny / 2 : (plane == 1 ? d->nz / 2 : d->nx / 2));
-    const int ns = d->nt / d->snap_interval;
-    for (int k = 0; k < ns; ++k) {
-      if ((rc = p->advance(d->snap_interval)) || (rc = p->snapshot(snaps + (size_t)k * per, plane, idx)))
-        return finish(rc);
-    }
-    if ((rc = p->advance(d->nt - ns * d->snap_interval))) return finish(rc);
-  } else if ((rc = p->advance(d->nt))) {
-    return finish(rc);
-  }
+  if ((rc = p->advance(d->nt))) return finish(rc);
   (void)hipEventRecord(e1, p->stream);
   if (hipEventSynchronize(e1) != hipSuccess) return finish(set_error(FOMO_EHIP, "step loop failed"));
   float ms = 0.0f;
@@ -556,6 +561,10 @@ static int run3d(const fomo_desc* d, const float* vp, const float* vs, const flo
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (kernel_time_s) *kernel_time_s = ms * 1e-3;
+  if (snaps && p->snap) {
+    const hipError_t ce = hipMemcpy(snaps, p->snap, p->snap_cells * p->snap_n * sizeof(float), hipMemcpyDeviceToHost);
+    if (ce != hipSuccess) return finish(set_error(FOMO_EHIP, std::string("snapshot download: ") + hipGetErrorString(ce)));
+  }
   rc = p->read_seis(0, seis);
   return finish(rc);
 }

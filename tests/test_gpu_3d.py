@@ -92,6 +92,40 @@ def test_acoustic3d_snapshots(gpu, orc, plane):
     assert np.array_equal(r.snaps[-1], want)
 
 
+@pytest.mark.parametrize("si", [1, 3, 16])
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_acoustic3d_snapshots_inside_graph_replays(gpu, si, storage):
+    """Snapshots are written by the per-step gather launch, also inside 16-step graph replays:
+    every snapshot k equals the p a plan holds after (k+1)*si steps, and graph == stream."""
+    import fomo
+    import fomo._lib as L
+    from fomo.api import Plan
+    nx, ny, nz, nt = 30, 26, 24, 37
+    vp, rho = model3d(nx, ny, nz, 11)
+    sx, sy, sz, rx, ry, rz = geom3d(nx, ny, nz)
+    kw = dict(sx=sx, sy=sy, sz=sz, rx=rx, ry=ry, rz=rz, nbc=5, fd_order=8, snap_interval=si, snap_plane="xy",
+              snap_index=9, storage=storage)
+    a = fomo.acoustic3d(vp, rho, 10.0, 1e-3, nt, 15.0, **kw)
+    b = fomo.acoustic3d(vp, rho, 10.0, 1e-3, nt, 15.0, use_cuda_graph=False, **kw)
+    assert len(a.snaps) == nt // si
+    for x, y in zip(a.snaps, b.snaps):
+        assert np.array_equal(x, y)
+    w = fomo.ricker_wavelet(15.0, 1e-3, nt)
+    p = Plan(L.ACOUSTIC3D, vp, rho, 10.0, 1e-3, nt, sx=sx, sy=sy, sz=sz, rx=rx, ry=ry, rz=rz,
+             wavelet=np.stack([w] * len(sx)), nbc=5, fd_order=8, storage=storage)
+    try:
+        done = 0
+        pad = p.pad
+        for k in sorted({0, len(a.snaps) // 2, len(a.snaps) - 1}):
+            p.advance((k + 1) * si - done)
+            done = (k + 1) * si
+            f = p.field(0)[pad:-pad, pad:-pad, pad:-pad]
+            assert np.array_equal(a.snaps[k], f[:, :, 8]), k
+        assert np.abs(a.snaps[-1]).max() > 0
+    finally:
+        p.close()
+
+
 @pytest.mark.parametrize("storage,tol", [(1, 2e-2), (2, 1e-1)])
 def test_reduced_precision_storage_error(gpu, orc, storage, tol):
     """Float16 / BFloat16 storage with per-array scaling, FP32 arithmetic: per-trace relative L2
