@@ -377,6 +377,17 @@ static int validate_common(const fomo_desc* d) {
     if (d->eq != FOMO_ACOUSTIC2D && d->eq != FOMO_ELASTIC2D)
       return fail(FOMO_EUNSUP, "the sponge boundary is 2D only (as in the reference)");
   }
+  // The sweeps address cells with 32-bit offsets: a 2D field, or one z plane of a 3D field, may
+  // hold at most 2^30 padded cells (its Float32 byte offsets then stay below 2^32).
+  {
+    const long long pad = (long long)d->nbc + d->fd_order / 2;
+    const long long px = d->nx + 2 * pad, py = d->ny + 2 * pad, pz = d->nz + 2 * pad;
+    const bool is3d = d->eq == FOMO_ACOUSTIC3D || d->eq == FOMO_ELASTIC3D;
+    const long long cells = is3d ? px * py : px * pz;
+    if (cells > (1LL << 30))
+      return fail(FOMO_EUNSUP, std::string(is3d ? "3D plane" : "2D field") + " of " + std::to_string(cells) +
+                                   " padded cells exceeds the 2^30 cells that 32-bit cell offsets address");
+  }
   if (fomo_device_count() < 1) return fail(FOMO_EHIP, "no HIP device visible");
   return FOMO_OK;
 }

@@ -103,6 +103,9 @@ typedef struct fomo_desc {
   float sponge_factor;   /* Cerjan factor (reference default 0.015); used with FOMO_BOUNDARY_SPONGE */
   int32_t reserved[4];
 } fomo_desc;
+/* Size limit: cells are addressed with 32-bit offsets, so a padded 2D field (nx + 2 pad) x
+ * (nz + 2 pad), or one padded z plane of a 3D field, holds at most 2^30 cells; larger grids are
+ * refused with FOMO_EUNSUP (pad = nbc + fd_order / 2; for a batch, the whole model counts). */
 
 /* One shot of a batch: a window of the model (full depth) with its own sources and receivers.
  * The shot is simulated exactly as a solo run on the cropped model would be: padding, medium,

@@ -94,6 +94,23 @@ def test_stream_probe_rejects_bad_shapes():
     assert L.lib().fomo_stream_copy(8, 1, C.byref(ms)) == L.FOMO_EINVAL
 
 
+def test_grids_beyond_32bit_cell_offsets_are_rejected():
+    """A 2D field (or a 3D plane) above 2^30 padded cells is refused before any device work: the
+    sweeps address cells with 32-bit offsets."""
+    import ctypes as C
+
+    import fomo._lib as L
+    from fomo.api import _desc
+    h = C.c_void_p()
+    for eq, nx, ny, nz in ((L.ACOUSTIC2D, 33000, 1, 33000), (L.ELASTIC2D, 70000, 1, 16000),
+                           (L.ACOUSTIC3D, 33000, 33000, 8)):
+        d = _desc(eq, nx, nz, 10, 8, 10, 10.0, 1e-3, 2000.0, 0, 0, 0, True, ny=ny)
+        rc = L.lib().fomo_plan_create(C.byref(d), None, None, None, None, None, None, None, None, None, None,
+                                      C.byref(h))
+        assert rc == L.FOMO_EUNSUP, (eq, rc)
+        assert "2^30" in L.lib().fomo_last_error().decode()
+
+
 def test_cfl_bound_matches_reference_formula():
     """dt_max = dh / (vmax sqrt(2) sum|a|): just below passes, just above raises."""
     from fomo.checks import check_numerics
