@@ -1,0 +1,64 @@
+"""Size-independent properties at the workloads' full sizes (too large for the CPU oracle):
+
+* acoustic3d at 512^3 interior, nbc 32, FD8: x-mirror symmetry of p about a centred source on a
+  model that varies in z only (the reference's own symmetry check, test/verify_patch.jl:198-229,
+  relative < 1e-5), for as many steps as the wave needs to stay away from the x faces;
+* the 64-window uneven acoustic batch on the 4000x1000 layered model (nbc 40, FD8): shots of the
+  batch equal their solo runs on the cropped model, bit for bit.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_acoustic3d_512_mirror_symmetry(gpu):
+    import fomo
+    n, nt = 512, 60
+    z = np.arange(n, dtype=np.float32)
+    vz = (2000.0 + 2.0 * z).astype(np.float32)  # 2000 .. 3022 m/s, varying in z only
+    vp = np.asfortranarray(np.broadcast_to(vz[None, None, :], (n, n, n)).astype(np.float32))
+    rho = np.full((n, n, n), 2100.0, np.float32, order="F")
+    k = np.arange(1, 201)
+    rx = np.concatenate([256 - k, 256 + k])
+    ry = np.full(len(rx), 256)
+    rz = np.full(len(rx), 12)
+    r = fomo.acoustic3d(vp, rho, 10.0, 8e-4, nt, 15.0, sx=[256], sy=[256], sz=[4], rx=rx, ry=ry, rz=rz, nbc=32,
+                        fd_order=8, record_p=True, extended=True)
+    p = np.asarray(r.seis_p, np.float64)
+    left, right = p[:200], p[200:]
+    assert np.abs(p).max() > 0
+    rel = np.linalg.norm(left - right) / np.linalg.norm(left)
+    print(f"acoustic3d 512^3 x-mirror relative difference {rel:.2e}", flush=True)
+    assert rel < 1e-5
+
+
+def test_batch_64_windows_full_size_equals_solo(gpu):
+    import fomo
+    import fomo._lib as L
+    from fomo.api import Plan
+    from fomo.models import SplitMix64, layered_2d
+    nx, nz, dt, nbc, nt = 4000, 1000, 8e-4, 40, 40
+    vp, _, rho = layered_2d(nx, nz, seed=20260101)
+    w = fomo.ricker_wavelet(15.0, dt, nt)
+    rng = SplitMix64(20260104)
+    shots = []
+    for _ in range(64):
+        wd = 256 + int(rng.uniform(0.0, 1281.0 - 1e-9))
+        x0 = 1 + int(rng.uniform(0.0, float(nx - wd + 1) - 1e-9))
+        rxl = np.arange(1, wd + 1, 2)
+        shots.append(dict(x0=x0, nx=wd, sx=[wd // 2], sz=[5], rx=rxl, rz=np.full(len(rxl), 5), wavelet=w[None]))
+    p = Plan(L.ACOUSTIC2D, vp, rho, 10.0, dt, nt, shots=shots, nbc=nbc, fd_order=8)
+    try:
+        p.advance(nt)
+        for s in (0, 17, 40, 63):
+            sh = shots[s]
+            got = p.seis(3, shot=s)
+            win = slice(sh["x0"] - 1, sh["x0"] - 1 + sh["nx"])
+            solo = fomo.acoustic2d(vp[win], rho[win], 10.0, dt, nt, 15.0, sx=sh["sx"], sz=sh["sz"], rx=sh["rx"],
+                                   rz=sh["rz"], nbc=nbc, fd_order=8, wavelet=w)
+            for a, b in zip(got, (solo.seis_p, solo.seis_vx, solo.seis_vz)):
+                assert np.array_equal(a, b), s
+            assert np.abs(got[0]).max() > 0
+    finally:
+        p.close()
