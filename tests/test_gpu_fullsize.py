@@ -1,8 +1,9 @@
 """Size-independent properties at the workloads' full sizes (too large for the CPU oracle):
 
-* acoustic3d at 512^3 interior, nbc 32, FD8: x-mirror symmetry of p about a centred source on a
-  model that varies in z only (the reference's own symmetry check, test/verify_patch.jl:198-229,
-  relative < 1e-5), for as many steps as the wave needs to stay away from the x faces;
+* acoustic3d and elastic3d at 512^3 interior, nbc 32, FD8: x-mirror symmetry of p (vz) about a
+  centred source on a model that varies in z only (the reference's own symmetry check,
+  test/verify_patch.jl:198-229, relative < 1e-5), for as many steps as the wave needs to stay
+  away from the x faces;
 * the 64-window uneven acoustic batch on the 4000x1000 layered model (nbc 40, FD8): shots of the
   batch equal their solo runs on the cropped model, bit for bit.
 """
@@ -30,6 +31,27 @@ def test_acoustic3d_512_mirror_symmetry(gpu):
     assert np.abs(p).max() > 0
     rel = np.linalg.norm(left - right) / np.linalg.norm(left)
     print(f"acoustic3d 512^3 x-mirror relative difference {rel:.2e}", flush=True)
+    assert rel < 1e-5
+
+
+def test_elastic3d_512_mirror_symmetry(gpu):
+    """elastic3d at the same size: vz of an explosive source is x-symmetric (vz lives on integer x
+    points), on a model that varies in z only."""
+    import fomo
+    n, nt = 512, 40
+    z = np.arange(n, dtype=np.float32)
+    vz_ = (2200.0 + 2.0 * z).astype(np.float32)
+    vp = np.asfortranarray(np.broadcast_to(vz_[None, None, :], (n, n, n)).astype(np.float32))
+    vs = np.asfortranarray((vp / np.float32(1.73)).astype(np.float32))
+    rho = np.full((n, n, n), 2200.0, np.float32, order="F")
+    k = np.arange(1, 151)
+    rx = np.concatenate([256 - k, 256 + k])
+    r = fomo.elastic3d(vp, vs, rho, 10.0, 6e-4, nt, 15.0, sx=[256], sy=[256], sz=[6], rx=rx, ry=np.full(len(rx), 256),
+                       rz=np.full(len(rx), 12), nbc=32, fd_order=8)
+    v = np.asarray(r.seis_vz, np.float64)
+    assert np.abs(v).max() > 0
+    rel = np.linalg.norm(v[:150] - v[150:]) / np.linalg.norm(v[:150])
+    print(f"elastic3d 512^3 x-mirror relative difference (vz) {rel:.2e}", flush=True)
     assert rel < 1e-5
 
 
