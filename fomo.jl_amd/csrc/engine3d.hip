@@ -25,6 +25,7 @@
 //                                    values (the race-free form of src/boundary/habc/kernels_det.jl:5-10)
 //   injection ...................... src/acquisition/inject_source_3d.jl:5-17 (after the HABC)
 #include <algorithm>
+#include <type_traits>
 
 #include "engine3d.h"
 #include "storage.h"
@@ -486,12 +487,26 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
 }
 
 template <int PH, int N, class T, bool LAG>
-__global__ __launch_bounds__((kTileW * kTR<PH, LAG>)) void sweep3d_kernel(const Sweep3DArgs A) {
+__device__ __forceinline__ void sweep3d_body(const Sweep3DArgs& A) {
   __shared__ float lds[LAG ? Lds3<PH, N, LAG>::SIZE : Lds3<PH, N, LAG>::CORE];
   if ((int)blockIdx.x >= A.n_work) return;
   const Work3D W = A.work[blockIdx.x];
   if (W.kind < 0) return;
   march3d<PH, N, T, LAG>(A, W, lds);
+}
+
+// Float32 storage: the compiler's register budget.  16-bit storage: at most 128 VGPRs, 4 waves
+// per SIMD (the elastic stress core otherwise needs a few more and runs 3 waves per SIMD: its
+// 512^3 step took 7.47 ms against 6.60 ms with the cap; Float32 loses 1 % to the spill the cap
+// costs it, so it has none).
+template <int PH, int N, class T, bool LAG>
+__global__ __launch_bounds__((kTileW * kTR<PH, LAG>)) void sweep3d_kernel(const Sweep3DArgs A) {
+  sweep3d_body<PH, N, T, LAG>(A);
+}
+template <int PH, int N, class T, bool LAG>
+__global__ __launch_bounds__((kTileW * kTR<PH, LAG>)) __attribute__((amdgpu_waves_per_eu(4))) void sweep3d_kernel16(
+    const Sweep3DArgs A) {
+  sweep3d_body<PH, N, T, LAG>(A);
 }
 
 // Interior cells of a slice: a_n x b_n with a fastest (xz: x, z; xy: x, y; yz: y, z).
@@ -552,16 +567,24 @@ hipError_t launch_decode3d(int storage, const void* src, float* dst, size_t n, f
   return hipGetLastError();
 }
 
+template <int PH, int N, class T, bool LAG>
+static void launch_one(const Sweep3DArgs& a, hipStream_t s) {
+  const dim3 blk(kTileW, kTR<PH, LAG>);
+  if constexpr (std::is_same<T, float>::value)
+    hipLaunchKernelGGL((sweep3d_kernel<PH, N, T, LAG>), dim3(a.n_work), blk, 0, s, a);
+  else
+    hipLaunchKernelGGL((sweep3d_kernel16<PH, N, T, LAG>), dim3(a.n_work), blk, 0, s, a);
+}
+
 template <int PH, class T, bool LAG>
 static hipError_t launch_n(int N, const Sweep3DArgs& a, hipStream_t s) {
   if (a.n_work <= 0) return hipSuccess;
-  const dim3 blk(kTileW, kTR<PH, LAG>);
   switch (N) {
-    case 1: hipLaunchKernelGGL((sweep3d_kernel<PH, 1, T, LAG>), dim3(a.n_work), blk, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((sweep3d_kernel<PH, 2, T, LAG>), dim3(a.n_work), blk, 0, s, a); break;
-    case 3: hipLaunchKernelGGL((sweep3d_kernel<PH, 3, T, LAG>), dim3(a.n_work), blk, 0, s, a); break;
-    case 4: hipLaunchKernelGGL((sweep3d_kernel<PH, 4, T, LAG>), dim3(a.n_work), blk, 0, s, a); break;
-    case 5: hipLaunchKernelGGL((sweep3d_kernel<PH, 5, T, LAG>), dim3(a.n_work), blk, 0, s, a); break;
+    case 1: launch_one<PH, 1, T, LAG>(a, s); break;
+    case 2: launch_one<PH, 2, T, LAG>(a, s); break;
+    case 3: launch_one<PH, 3, T, LAG>(a, s); break;
+    case 4: launch_one<PH, 4, T, LAG>(a, s); break;
+    case 5: launch_one<PH, 5, T, LAG>(a, s); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
