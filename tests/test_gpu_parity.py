@@ -233,29 +233,26 @@ def test_acoustic_400x300_full_bitwise(gpu, orc):
         assert_parity(getattr(res, k), ref[k])
 
 
-def test_elastic_2000x800_full_grid_prefix_bitwise(gpu, orc):
-    """Plain elastic2d at the full 2000x800 grid (dh 5, dt 0.4 ms, fd8, nbc 50, seeded layered
-    model, vacuum top, vertical-force Ricker 20 Hz at (1000,5), receivers at every cell of z=5):
-    the first 300 steps bitwise against the oracle; the full 4000 steps are checked for finite,
-    bounded output."""
+def test_elastic_2000x800_full_run_bitwise(gpu, orc):
+    """Plain elastic2d, the bench workload in full: 2000x800 grid (dh 5, dt 0.4 ms, fd8, nbc 50,
+    seeded layered model, vacuum top, vertical-force Ricker 20 Hz at (1000,5), receivers at every
+    cell of z=5), all 4000 steps bitwise against the oracle (about 15 s of oracle time on 16 host
+    threads); the traces are finite and decay by the end of the record."""
     import fomo
     from fomo.models import layered_2d
-    nx, nz = 2000, 800
+    nx, nz, nt = 2000, 800, 4000
     vp, vs, rho = layered_2d(nx, nz, seed=20260101)
     rx = np.arange(1, nx + 1)
     rz = np.full(nx, 5)
     kw = dict(sx=[1000], sz=[5], rx=rx, rz=rz, nbc=50, fd_order=8)
-    nt = 300
     res = fomo.elastic2d(vp, vs, rho, 5.0, 4e-4, nt, 20.0, source_type="fz", **kw)
     w = fomo.ricker_wavelet(20.0, 4e-4, nt)
     ref = orc.elastic2d(vp, vs, rho, 5.0, 4e-4, nt, wavelet=w[None], source_type=1, **kw)
     for k in ("seis_vx", "seis_vz"):
         assert_parity(getattr(res, k), ref[k])
-    full = fomo.elastic2d(vp, vs, rho, 5.0, 4e-4, 4000, 20.0, source_type="fz", **kw)
-    assert np.all(np.isfinite(full.seis_vz)) and np.all(np.isfinite(full.seis_vx))
-    assert np.array_equal(full.seis_vz[:, :nt], res.seis_vz)
-    peak = np.abs(full.seis_vz).max()
-    assert np.abs(full.seis_vz[:, -200:]).max() < peak
+    assert np.all(np.isfinite(res.seis_vz)) and np.all(np.isfinite(res.seis_vx))
+    peak = np.abs(res.seis_vz).max()
+    assert np.abs(res.seis_vz[:, -200:]).max() < peak
 
 
 # ---------------------------------------------------------------------------------------------
