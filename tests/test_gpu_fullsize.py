@@ -4,6 +4,7 @@
   centred source on a model that varies in z only (the reference's own symmetry check,
   test/verify_patch.jl:198-229, relative < 1e-5), for as many steps as the wave needs to stay
   away from the x faces;
+* acoustic3d and elastic3d at 512^3: the first steps bitwise against the oracle;
 * the 64-window uneven acoustic batch on the 4000x1000 layered model (nbc 40, FD8): shots of the
   batch equal their solo runs on the cropped model, bit for bit.
 """
@@ -53,6 +54,35 @@ def test_elastic3d_512_mirror_symmetry(gpu):
     rel = np.linalg.norm(v[:150] - v[150:]) / np.linalg.norm(v[:150])
     print(f"elastic3d 512^3 x-mirror relative difference (vz) {rel:.2e}", flush=True)
     assert rel < 1e-5
+
+
+@pytest.mark.parametrize("eq,nt", [("acoustic3d", 40), ("elastic3d", 20)])
+def test_3d_512_prefix_bitwise_vs_oracle(gpu, orc, eq, nt):
+    """The 3D workload at its full size (512^3, nbc 32, FD8, seeded layered model with Gaussian
+    perturbations, 12 Hz Ricker at the top centre, receivers on a 64x64 surface grid) for its
+    first steps, bitwise against the oracle (about 20 s of oracle time on 16 host threads)."""
+    import fomo
+    from fomo.models import layered_3d
+    n = 512
+    vp, rho = layered_3d(n, n, n, seed=20260101)
+    g = np.arange(4, n + 1, 8)
+    gx, gy = np.meshgrid(g, g, indexing="ij")
+    rx, ry = gx.ravel(), gy.ravel()
+    rz = np.ones(rx.size, int)
+    w = fomo.ricker_wavelet(12.0, 8e-4, nt)
+    kw = dict(sx=[n // 2], sy=[n // 2], sz=[1], rx=rx, ry=ry, rz=rz, nbc=32, fd_order=8)
+    if eq == "acoustic3d":
+        r = fomo.acoustic3d(vp, rho, 10.0, 8e-4, nt, 12.0, **kw)
+        ref = orc.acoustic3d(vp, rho, 10.0, 8e-4, nt, wavelet=w[None], **kw)
+    else:
+        vs = (vp / np.float32(1.73)).astype(np.float32)
+        r = fomo.elastic3d(vp, vs, rho, 10.0, 8e-4, nt, 12.0, **kw)
+        ref = orc.elastic3d(vp, vs, rho, 10.0, 8e-4, nt, wavelet=w[None], **kw)
+    for k in ("seis_vx", "seis_vy", "seis_vz"):
+        got = getattr(r, k)
+        assert np.all(np.isfinite(got))
+        assert np.array_equal(got, ref[k]), (k, np.abs(got.astype(np.float64) - ref[k]).max())
+    assert np.abs(r.seis_vz).max() > 0
 
 
 def test_batch_64_windows_full_size_equals_solo(gpu):
