@@ -5,6 +5,7 @@
   test/verify_patch.jl:198-229, relative < 1e-5), for as many steps as the wave needs to stay
   away from the x faces;
 * acoustic3d and elastic3d at 512^3: the first steps bitwise against the oracle;
+* the learned boundary at 2400x1200: graph == stream; FP16 storage at 512^3 close to FP32;
 * the 64-window uneven acoustic batch on the 4000x1000 layered model (nbc 40, FD8): shots of the
   batch equal their solo runs on the cropped model, bit for bit.
 """
@@ -83,6 +84,61 @@ def test_3d_512_prefix_bitwise_vs_oracle(gpu, orc, eq, nt):
         assert np.all(np.isfinite(got))
         assert np.array_equal(got, ref[k]), (k, np.abs(got.astype(np.float64) - ref[k]).max())
     assert np.abs(r.seis_vz).max() > 0
+
+
+def test_learned_2400x1200_graph_equals_stream(gpu):
+    """The learned thin boundary at its workload's size (2400x1200, nbc 16, dt 0.8 ms, FD8): graph
+    replays (inference on the frame, then the sweeps) equal direct launches bit for bit, and the
+    network changes the result against plain HABC at the same width."""
+    import fomo
+    import fomo._lib as L
+    from fomo.api import Plan
+    from fomo.learned import enable, load_weights
+    from fomo.models import layered_2d
+    nx, nz, nt = 2400, 1200, 100
+    vp, _, rho = layered_2d(nx, nz, seed=20260101, vacuum_top=False)
+    w = fomo.ricker_wavelet(15.0, 8e-4, nt)
+    rx = np.arange(1, nx + 1, 2)
+    out = []
+    for graph, learned in ((True, True), (False, True), (True, False)):
+        p = Plan(L.ACOUSTIC2D, vp, rho, 10.0, 8e-4, nt, sx=[nx // 2], sz=[50], rx=rx, rz=np.full(len(rx), 10),
+                 wavelet=w[None], nbc=16, fd_order=8, use_graph=graph)
+        try:
+            if learned:
+                enable(p, load_weights())
+            p.advance(nt)
+            out.append(p.seis(3) + [p.field(0)])
+        finally:
+            p.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+    assert not np.array_equal(out[0][3], out[2][3])
+
+
+def test_acoustic3d_512_fp16_close_to_fp32(gpu):
+    """FP16 storage at the 3D workload's size (512^3, nbc 32): traces within 4 % (per-trace
+    relative L2 over the traces holding at least 1 % of the strongest one's energy) of FP32, the
+    bar the small-grid FP16 tests hold against FP64.  Measured: 0.2 % after 120 steps (2.8 % after
+    60, at the wave's onset, where the record is short)."""
+    import fomo
+    from fomo.models import layered_3d
+    n, nt = 512, 120
+    vp, rho = layered_3d(n, n, n, seed=20260101)
+    g = np.arange(4, n + 1, 8)
+    gx, gy = np.meshgrid(g, g, indexing="ij")
+    kw = dict(sx=[n // 2], sy=[n // 2], sz=[1], rx=gx.ravel(), ry=gy.ravel(), rz=np.ones(gx.size, int), nbc=32,
+              fd_order=8)
+    a = fomo.acoustic3d(vp, rho, 10.0, 8e-4, nt, 12.0, **kw)
+    b = fomo.acoustic3d(vp, rho, 10.0, 8e-4, nt, 12.0, storage="f16", **kw)
+    ref = np.asarray(a.seis_vz, np.float64)
+    got = np.asarray(b.seis_vz, np.float64)
+    den = np.linalg.norm(ref, axis=1)
+    ok = den > 1e-2 * den.max()
+    e = np.linalg.norm(got - ref, axis=1)[ok] / den[ok]
+    print(f"acoustic3d 512^3 FP16 vs FP32 vz: per-trace rel L2 median {np.median(e):.2e} max {e.max():.2e}",
+          flush=True)
+    assert np.all(np.isfinite(got)) and ok.sum() > 0
+    assert e.max() < 4e-2
 
 
 def test_batch_64_windows_full_size_equals_solo(gpu):
