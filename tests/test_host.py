@@ -111,6 +111,16 @@ def test_grids_beyond_32bit_cell_offsets_are_rejected():
         assert "2^30" in L.lib().fomo_last_error().decode()
 
 
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus N > 1 exits with a message before touching a device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2", "--steps", "1"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in r.stderr
+
+
 def test_cfl_bound_matches_reference_formula():
     """dt_max = dh / (vmax sqrt(2) sum|a|): just below passes, just above raises."""
     from fomo.checks import check_numerics
