@@ -1,8 +1,9 @@
 """CPU restatement of the learned-boundary MLP (features, forward, backward) in numpy.
 
 TEST INFRASTRUCTURE ONLY: the checker for the HIP kernels in fomo.jl_amd/csrc/learned2d.hip.
-The reference (Fomo.jl) has no learned boundary; this oracle is pinned by finite differences
-(tests/test_learned_cpu.py) rather than by reference outputs.
+The reference (Fomo.jl) has no learned boundary; this oracle is pinned by closed forms (chosen
+routes through the network, a unit spike for the patch layout, a nearest-neighbour search for the
+BFloat16 rounding) and finite differences in tests/test_learned_cpu.py, not by reference outputs.
 
 Parameter layout (3586 values): W1 [32][76], b1 [32], W2 [32][32], b2 [32], W3 [2][32], b3 [2].
 Outputs are blend-weight corrections corr = CORR_MAX * tanh(z).
