@@ -378,8 +378,10 @@ def main():
     ap.add_argument("--probe-tile", type=int, default=0, help="3D z-segment choice (0 default, k: 16<<(k-1) planes)")
     ap.add_argument("--copy-mb", default="", help="with --stream-copy: comma-separated copy sizes in MB")
     ap.add_argument("--stream-probe", action="store_true",
-                    help="the sweeps' array counts without stencils (5r2w, 4r1w, 7r2w, 8r3w, 3r1w) at 16 and 256 MB "
+                    help="the sweeps' array counts without stencils (5r2w, 4r1w, 7r2w, 8r3w, 3r1w) at --probe-mb MiB "
                          "per array")
+    ap.add_argument("--probe-mb", default="16,256",
+                    help="with --stream-probe: comma-separated MiB per array (7.3 is one 2108x908 FP32 field)")
     ap.add_argument("--batch", action="store_true",
                     help="uneven 64-shot windowed acoustic batch on a 4000x1000 model (timed over --steps)")
     args = ap.parse_args()
@@ -405,12 +407,12 @@ def main():
 
         import fomo._lib as L
         out = []
-        for mb in (16, 256):
+        for mb in (float(v) for v in args.probe_mb.split(",")):
             for nr, nw, what in ((5, 2, "acoustic velocity"), (4, 1, "acoustic pressure"), (7, 2, "elastic velocity"),
                                  (8, 3, "elastic stress"), (3, 1, "scalar")):
                 ms = C.c_double(0.0)
-                nb = mb << 20
-                L.check(L.lib().fomo_stream_probe(nr, nw, nb, 40 if mb < 64 else 10, C.byref(ms)))
+                nb = int(mb * (1 << 20)) // 16 * 16
+                L.check(L.lib().fomo_stream_probe(nr, nw, nb, max(10, min(200, int(640 / mb))), C.byref(ms)))
                 out.append({"pattern": f"{nr} read + {nw} written ({what})", "mb_per_array": mb, "ms": ms.value,
                             "gbs": (nr + nw) * nb / (ms.value * 1e-3) / 1e9})
                 log(f"{out[-1]}")
