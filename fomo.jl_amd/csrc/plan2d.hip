@@ -48,11 +48,15 @@ static std::vector<Work2D> work_items(const std::vector<Shot2D>& shots, const st
 // block b (which the hardware places on XCD b % 8) takes the next item of run b % 8.  Slots past
 // the end of a shorter run hold an empty item (kind -1).
 static std::vector<Work2D> xcd_layout(const std::vector<Work2D>& items) {
-  // frame cells cost about twice an interior cell (ring recompute, blend)
+  // A frame cell weighs 1.5 interior cells (ring recompute, blend).  Measured on elastic2d
+  // 2000x800 FD8: 1.5 gives the fastest step (34.4 us; 1.0: 36.4, 2.0: 34.9, 3.0: 35.8, 4.0: 38.2),
+  // and at 8192^2 and the 64-shot batch it matches 2.0.  FOMO_FRAME_COST overrides it.
+  double fc = 1.5;
+  if (const char* env = std::getenv("FOMO_FRAME_COST")) fc = std::max(0.5, std::atof(env));
   std::vector<double> cost(items.size());
   double total = 0.0;
   for (size_t k = 0; k < items.size(); ++k) {
-    cost[k] = (double)items[k].w * items[k].h * (items[k].kind == 0 ? 2.0 : 1.0);
+    cost[k] = (double)items[k].w * items[k].h * (items[k].kind == 0 ? fc : 1.0);
     total += cost[k];
   }
   std::vector<std::vector<Work2D>> runs(8);
