@@ -28,7 +28,7 @@ METRIC = "cell-updates/s & % HBM roofline, FD8 2D/3D; learned-ABC+FP16 vs HABC a
 # HBM-side bytes per launch of the two sweeps at this workload, measured with rocprofv3 --pmc
 # (FETCH_SIZE scaled x2.0 and WRITE_SIZE x1.0, the factors the same counters read on a streaming
 # copy of known size); profiles/elastic2d_2000x800_fd8_pmc_traffic.md holds the readings.
-PMC_TRAFFIC = {"velocity_sweep": 75.2e6, "stress_sweep": 89.1e6}
+PMC_TRAFFIC = {"velocity_sweep": 74.8e6, "stress_sweep": 88.9e6}
 HBM_PEAK_GBS = 8000.0
 NX, NZ, DH, DT, NT, F0, NBC, ORDER, SEED = 2000, 800, 5.0, 4e-4, 4000, 20.0, 50, 8, 20260101
 BYTES_VEL, BYTES_STR = 36, 44
