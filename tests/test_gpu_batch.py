@@ -108,3 +108,35 @@ def test_tile_shapes_agree_bitwise(gpu):
         assert np.array_equal(a, b)
     for a, b in zip(outs[0], outs[2]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("weight", ["0.5", "1.5", "4"])
+def test_frame_weight_of_the_layout_changes_no_bits(gpu, monkeypatch, weight):
+    """FOMO_FRAME_COST only reorders the work list across XCDs: an uneven elastic batch gives the
+    same bits under any weight as under the default."""
+    import fomo
+    import fomo._lib as L
+    from fomo.api import Plan
+    nx, nz, nt = 400, 150, 90
+    vp, vs, rho = model(nx, nz, 35)
+    w = fomo.ricker_wavelet(15.0, 1e-3, nt)
+    shots = []
+    for x0, wd in ((1, 400), (37, 150), (200, 201)):
+        rx = np.arange(1, wd + 1, 3)
+        shots.append(dict(x0=x0, nx=wd, sx=[wd // 2], sz=[10], rx=rx, rz=np.full(len(rx), 5), wavelet=w[None]))
+    outs = []
+    for env in (None, weight):
+        if env is None:
+            monkeypatch.delenv("FOMO_FRAME_COST", raising=False)
+        else:
+            monkeypatch.setenv("FOMO_FRAME_COST", env)
+        p = Plan(L.ELASTIC2D, vp, rho, 10.0, 1e-3, nt, vs=vs, shots=shots, nbc=20, fd_order=8, tile_cfg=2)
+        try:
+            p.advance(nt)
+            outs.append([a for s in range(len(shots)) for a in p.seis(2, shot=s)])
+        finally:
+            p.close()
+    monkeypatch.delenv("FOMO_FRAME_COST", raising=False)
+    assert np.abs(outs[0][0]).max() > 0
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
