@@ -52,7 +52,7 @@ static std::vector<Work2D> xcd_layout(const std::vector<Work2D>& items) {
   // 2000x800 FD8: 1.5 gives the fastest step (34.4 us; 1.0: 36.4, 2.0: 34.9, 3.0: 35.8, 4.0: 38.2),
   // and at 8192^2 and the 64-shot batch it matches 2.0.  FOMO_FRAME_COST overrides it.
   double fc = 1.5;
-  if (const char* env = std::getenv("FOMO_FRAME_COST")) fc = std::max(0.5, std::atof(env));
+  if (const char* env = std::getenv("FOMO_FRAME_COST")) fc = std::min(64.0, std::max(0.5, std::atof(env)));  // NaN -> 0.5
   std::vector<double> cost(items.size());
   double total = 0.0;
   for (size_t k = 0; k < items.size(); ++k) {
