@@ -107,10 +107,18 @@ def probe(args):
     w = fomo.ricker_wavelet(15.0, 1e-3, nt)
     res = {}
     storages = args.probe_storage2d.split(",")
-    for eq, name, bv, bs in ((L.ACOUSTIC2D, "acoustic2d", 28, 20), (L.ELASTIC2D, "elastic2d", BYTES_VEL, BYTES_STR),
-                             (L.SCALAR2D, "scalar2d", 16, 0)):
+    # (equation, name, FP32 bytes per cell of the two phases, of which material bytes)
+    for eq, name, bv, bs, mv, ms_ in ((L.ACOUSTIC2D, "acoustic2d", 28, 20, 8, 4),
+                                      (L.ELASTIC2D, "elastic2d", BYTES_VEL, BYTES_STR, 8, 12),
+                                      (L.SCALAR2D, "scalar2d", 16, 0, 4, 0)):
         for st in storages:
-            eb = 1.0 if st == "f32" else 0.5  # 16-bit storage halves every array's bytes
+            if st == "mat16" and eq == L.SCALAR2D:
+                continue  # the scalar sweep keeps a wavefield among its material inputs
+            if st == "mat16":  # Float16 materials halve only the material bytes
+                b_v, b_s, eb = bv - mv // 2, bs - ms_ // 2, 1.0
+            else:
+                b_v, b_s = bv, bs
+                eb = 1.0 if st == "f32" else 0.5  # 16-bit storage halves every array's bytes
             plan = Plan(eq, vp, rho, 10.0, 1e-3, nt, vs=vs if eq == L.ELASTIC2D else None, sx=[n // 2], sz=[n // 2],
                         rx=[1, n], rz=[1, n], wavelet=w[None], nbc=50, fd_order=8, storage=st)
             cells = plan.padded_cells
@@ -121,9 +129,9 @@ def probe(args):
             plan.close()
             key = name if st == "f32" else f"{name}_{st}"
             res[key] = {"padded_cells": cells, "ms_per_step": ms / nt, "gcell_per_s": cells * nt / ms / 1e6,
-                        "step_gbs": eb * (bv + bs) * cells * nt / ms / 1e6,
-                        "vel_ms": tv, "vel_gbs": eb * bv * cells / tv / 1e6, "prs_ms": ts,
-                        "prs_gbs": eb * bs * cells / ts / 1e6 if bs else 0.0, "gather_ms": tr}
+                        "step_gbs": eb * (b_v + b_s) * cells * nt / ms / 1e6,
+                        "vel_ms": tv, "vel_gbs": eb * b_v * cells / tv / 1e6, "prs_ms": ts,
+                        "prs_gbs": eb * b_s * cells / ts / 1e6 if b_s else 0.0, "gather_ms": tr}
             log(f"{key}: {res[key]}")
     print(json.dumps({"probe": f"{n}x{n} interior, fd8, nbc 50", **res}), flush=True)
 
@@ -370,7 +378,7 @@ def main():
     ap.add_argument("--probe3d", action="store_true", help="acoustic3d 512^3 probe (f32/f16/bf16)")
     ap.add_argument("--probe-n3", type=int, default=512)
     ap.add_argument("--probe-storage", default="f32,f16,bf16", help="storages of the 3D probe")
-    ap.add_argument("--probe-storage2d", default="f32", help="storages of the 2D probe (f32,f16,bf16)")
+    ap.add_argument("--probe-storage2d", default="f32", help="storages of the 2D probe (f32,f16,bf16,mat16)")
     ap.add_argument("--probe-eq", default="acoustic3d", choices=["acoustic3d", "elastic3d"], help="3D probe equation")
     ap.add_argument("--batch-eq", default="acoustic2d", choices=["acoustic2d", "elastic2d"], help="batch equation")
     ap.add_argument("--stream-copy", action="store_true", help="only the HBM streaming-copy calibration")

@@ -23,6 +23,8 @@
 //                                    from a device counter, as the reference's graph path does)
 #include <algorithm>
 
+#include <type_traits>
+
 #include "fomo_common.h"
 #include "storage.h"
 
@@ -234,13 +236,13 @@ struct Phys<kScal, N, T> {
   }
 };
 
-// Old values and materials of cell c (decoded).
-template <class P, class T>
+// Old values and materials of cell c (decoded).  Fields are stored as T, materials as TM.
+template <class P, class T, class TM>
 __device__ __forceinline__ void load_om(const PV& V, unsigned c, float* o, float* m) {
 #pragma unroll
   for (int f = 0; f < P::NF; ++f) o[f] = Cvt<T>::ld(V.old[f], c, V.odec[f]);
 #pragma unroll
-  for (int k = 0; k < P::NM; ++k) m[k] = Cvt<T>::ld(V.mat[k], c, V.mdec[k]);
+  for (int k = 0; k < P::NM; ++k) m[k] = Cvt<TM>::ld(V.mat[k], c, V.mdec[k]);
 }
 
 // Which output fields receive the raw-add source after the HABC.
@@ -284,7 +286,7 @@ struct StripRd {
 // plus a one-cell ring, and the old values each frame cell's Higdon term needs.  Pre-correction
 // values of the tile and ring then go to LDS, and each frame cell blends with its inward
 // neighbour's pre-correction value.
-template <int PH, int N, class T>
+template <int PH, int N, class T, class TM>
 __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, const Shot2D& S, const PV& V,
                                            float* lds, int it) {
   using P = Phys<PH, N, T>;
@@ -326,7 +328,7 @@ __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, 
     const int gi = i0 - 1 + cc, gj = j0 - 1 + r;
     const bool ok = idx < EW * EH && gi >= 0 && gi < nx && gj >= 0 && gj < nz && cc <= W.w + 1 && r <= W.h + 1;
     const unsigned g = ok ? (unsigned)(gi + gj * nx) : 0u;
-    load_om<P, T>(V, g, ro[k], rm[k]);
+    load_om<P, T, TM>(V, g, ro[k], rm[k]);
   }
   const int i = i0 + (int)threadIdx.x;
   const bool lane_ok = (int)threadIdx.x < W.w;
@@ -451,7 +453,7 @@ struct StripRing {
   static constexpr int RSX = RS + 2 * N;
 };
 
-template <int PH, int N, class T>
+template <int PH, int N, class T, class TM>
 __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D& W, const Shot2D& S, const PV& V,
                                                float* lds, int it) {
   using P = Phys<PH, N, T>;
@@ -532,7 +534,7 @@ __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D&
     for (int r = 0; r < RPT; ++r) {
       load_row(j0 + GR * k + N + ty + 4 * r, ra[r], rb[r]);
       const int j = j0 + GR * k + ty + 4 * r;
-      load_om<P, T>(V, (col_ok && j < jend) ? (unsigned)(i + j * nx) : 0u, no[r], nm[r]);
+      load_om<P, T, TM>(V, (col_ok && j < jend) ? (unsigned)(i + j * nx) : 0u, no[r], nm[r]);
     }
   };
   float co[RPT][NF], cm[RPT][NM];
@@ -544,7 +546,7 @@ __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D&
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
       const int j = j0 + ty + 4 * r;
-      load_om<P, T>(V, (col_ok && j < jend) ? (unsigned)(i + j * nx) : 0u, co[r], cm[r]);
+      load_om<P, T, TM>(V, (col_ok && j < jend) ? (unsigned)(i + j * nx) : 0u, co[r], cm[r]);
     }
 #pragma unroll
     for (int k = 0; k < R0; ++k)
@@ -601,8 +603,8 @@ __device__ __forceinline__ void gather_entry(const GatherArgs& g, int e, int col
 // The host lays the work list out per XCD: hardware hands block b to XCD b % 8, and the list
 // gives each XCD a contiguous, cost-balanced run of the grid so that halo rows stay in its L2.
 // Padding entries have kind -1.  Blocks past the work list record the previous step's receivers.
-template <int PH, int N, class T>
-__global__ __launch_bounds__(256) void sweep2d_kernel(const SweepArgs A) {
+template <int PH, int N, class T, class TM>
+__device__ __forceinline__ void sweep2d_body(const SweepArgs& A) {
   using P = Phys<PH, N, T>;
   constexpr int NF = P::NF, NS = P::NS;
   constexpr int FT = NS * (kTileW + 2 * N + 2) * (kFrameRows + 2 * N + 2) + NF * (kTileW + 2) * (kFrameRows + 2);
@@ -620,9 +622,21 @@ __global__ __launch_bounds__(256) void sweep2d_kernel(const SweepArgs A) {
   PV V;
   make_view<PH>(S, A.parity, it, V);
   if (W.kind == 0)
-    frame_tile<PH, N, T>(A, W, S, V, lds, it);
+    frame_tile<PH, N, T, TM>(A, W, S, V, lds, it);
   else
-    interior_strip<PH, N, T>(A, W, S, V, lds, it);
+    interior_strip<PH, N, T, TM>(A, W, S, V, lds, it);
+}
+
+template <int PH, int N, class T>
+__global__ __launch_bounds__(256) void sweep2d_kernel(const SweepArgs A) {
+  sweep2d_body<PH, N, T, T>(A);
+}
+
+// FP32 fields with FP16 materials (velocity-stress phases only: the scalar phase keeps a field
+// among its "materials").
+template <int PH, int N>
+__global__ __launch_bounds__(256) void sweep2d_mat16_kernel(const SweepArgs A) {
+  sweep2d_body<PH, N, float, __half>(A);
 }
 
 template <class T>
@@ -638,10 +652,16 @@ __global__ void decode2d_kernel(const void* src, float* dst, size_t n, float dec
 
 int strip_rows(int tile_cfg) { return tile_cfg == 0 ? 16 : (tile_cfg == 1 ? 32 : 64); }
 
+// Launch tag of the FP32-field, FP16-material instantiation.
+struct Mat16 {};
+
 template <int PH, int N, class T>
 static hipError_t launch_cfg(const SweepArgs& a, hipStream_t s) {
   static_assert(kTileW * 4 == kGatherBlock, "one receiver entry per thread of a gather block");
-  hipLaunchKernelGGL((sweep2d_kernel<PH, N, T>), dim3(a.n_work + a.n_gather), dim3(kTileW, 4), 0, s, a);
+  if constexpr (std::is_same<T, Mat16>::value)
+    hipLaunchKernelGGL((sweep2d_mat16_kernel<PH, N>), dim3(a.n_work + a.n_gather), dim3(kTileW, 4), 0, s, a);
+  else
+    hipLaunchKernelGGL((sweep2d_kernel<PH, N, T>), dim3(a.n_work + a.n_gather), dim3(kTileW, 4), 0, s, a);
   return hipGetLastError();
 }
 
@@ -664,7 +684,11 @@ static hipError_t launch_t2(int phase, int N, const SweepArgs& a, hipStream_t s)
     case kAcPrs: return launch_phase<kAcPrs, T>(N, a, s);
     case kElVel: return launch_phase<kElVel, T>(N, a, s);
     case kElStr: return launch_phase<kElStr, T>(N, a, s);
-    case kScal: return launch_phase<kScal, T>(N, a, s);
+    case kScal:
+      if constexpr (std::is_same<T, Mat16>::value)
+        return hipErrorInvalidValue;
+      else
+        return launch_phase<kScal, T>(N, a, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -677,6 +701,7 @@ hipError_t launch_sweep2d(int storage, int phase, int N, int tile_cfg, const Swe
     case FOMO_STORE_F32: return launch_t2<float>(phase, N, a, s);
     case FOMO_STORE_F16: return launch_t2<__half>(phase, N, a, s);
     case FOMO_STORE_BF16: return launch_t2<__hip_bfloat16>(phase, N, a, s);
+    case FOMO_STORE_F32_MAT16: return launch_t2<Mat16>(phase, N, a, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -14,7 +14,9 @@ struct ShotMeta {
 
 struct Plan2D : PlanBase {
   int N = 0, nfld = 0, nslots = 0, tile_cfg = 1;
-  int storage = FOMO_STORE_F32, elem = 4;  // storage of fields and materials
+  int storage = FOMO_STORE_F32, elem = 4;  // storage of the fields
+  int mat_storage = FOMO_STORE_F32;        // storage of the materials
+  int sweep_storage = FOMO_STORE_F32;      // the requested code, which selects the sweep kernels
   float* scratch = nullptr;                // Float32 staging of one decoded field
   size_t scratch_n = 0;
   float coef[kMaxN] = {0};  // staggered first-derivative (scalar: centred second-derivative) weights

@@ -108,7 +108,7 @@ hipError_t Plan2D::enqueue_part(int part, int s, int koff) {
     a.g = g;
     a.n_gather = (n_entries + kGatherBlock - 1) / kGatherBlock;
   }
-  if (part == 1) return launch_sweep2d(storage, d.eq == FOMO_ACOUSTIC2D ? kAcPrs : kElStr, N, tile_cfg, a, stream);
+  if (part == 1) return launch_sweep2d(sweep_storage, d.eq == FOMO_ACOUSTIC2D ? kAcPrs : kElStr, N, tile_cfg, a, stream);
   if (learned) {
     // blend-weight corrections of every frame cell from the state at time n
     MlpArgs m{};
@@ -132,7 +132,7 @@ hipError_t Plan2D::enqueue_part(int part, int s, int koff) {
     if (e != hipSuccess) return e;
   }
   const int ph = d.eq == FOMO_ACOUSTIC2D ? kAcVel : (d.eq == FOMO_ELASTIC2D ? kElVel : kScal);
-  return launch_sweep2d(storage, ph, N, tile_cfg, a, stream);
+  return launch_sweep2d(sweep_storage, ph, N, tile_cfg, a, stream);
 }
 
 int Plan2D::reset() {
@@ -299,11 +299,16 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
     if (rc) return rc;
     if (sh[s].n_src > 0 && !sh[s].wavelet) return set_error(FOMO_EINVAL, "wavelet must not be NULL");
   }
-  if (d->storage < FOMO_STORE_F32 || d->storage > FOMO_STORE_BF16) return set_error(FOMO_EINVAL, "unknown storage");
+  if (d->storage < FOMO_STORE_F32 || d->storage > FOMO_STORE_F32_MAT16) return set_error(FOMO_EINVAL, "unknown storage");
+  if (d->storage == FOMO_STORE_F32_MAT16 && scalar)
+    return set_error(FOMO_EINVAL, "Float16 materials need a velocity-stress equation (acoustic2d, elastic2d)");
   Plan2D* p = new Plan2D();
   p->d = *d;
-  p->storage = d->storage;
-  p->elem = d->storage == FOMO_STORE_F32 ? 4 : 2;
+  const bool mat16 = d->storage == FOMO_STORE_F32_MAT16;
+  p->storage = mat16 ? FOMO_STORE_F32 : d->storage;
+  p->mat_storage = mat16 ? FOMO_STORE_F16 : d->storage;
+  p->sweep_storage = d->storage;
+  p->elem = p->storage == FOMO_STORE_F32 ? 4 : 2;
   p->N = d->fd_order / 2;
   if (scalar)
     fomo_centered_d2(d->fd_order, &p->c0, p->coef);
@@ -424,8 +429,8 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
       // material m, stored with scale max |value| when 16-bit
       auto put = [&](int m, const std::vector<float>& h) -> int {
         const float mx = max_abs(h.data(), h.size());
-        x.mdec[m] = scaled && mx > 0.0f ? mx : 1.0f;
-        return upload_stored(p, p->storage, &dm[m], h.data(), h.size(), 1.0f / x.mdec[m]);
+        x.mdec[m] = p->mat_storage != FOMO_STORE_F32 && mx > 0.0f ? mx : 1.0f;
+        return upload_stored(p, p->mat_storage, &dm[m], h.data(), h.size(), 1.0f / x.mdec[m]);
       };
       if (scalar) {
         // c2 = (v * Float32(dt/dh))^2 on the edge-padded model, scalar2d.jl:126-128

@@ -15,7 +15,7 @@ LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "libfomo_hip.so"))
 FOMO_OK, FOMO_EINVAL, FOMO_EHIP, FOMO_ENOMEM, FOMO_EUNSUP = 0, 1, 2, 3, 4
 ACOUSTIC2D, ELASTIC2D, ACOUSTIC3D, ELASTIC3D, SCALAR2D = 1, 2, 3, 4, 5
 SRC_EXPLOSIVE, SRC_FZ = 0, 1
-STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
+STORE_F32, STORE_F16, STORE_BF16, STORE_F32_MAT16 = 0, 1, 2, 3
 BOUNDARY_HABC, BOUNDARY_SPONGE = 0, 1
 
 

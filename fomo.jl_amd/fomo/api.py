@@ -90,7 +90,8 @@ def acoustic2d(vp, rho, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_order=8, s
     (an extension: the Higdon frame with MLP-corrected blend weights, fomo.learned, trained for thin
     layers such as nbc = 16; `learned_weights` overrides the shipped network).
     `storage` (extension): :f32 (default, bit-identical to the reference), :f16 or :bf16 storage of
-    wavefields and materials with per-field scales; arithmetic stays Float32."""
+    wavefields and materials with per-field scales, or :mat16 (Float32 wavefields, Float16
+    materials); arithmetic stays Float32."""
     del verbose
     _common_checks(boundary, nbc, fd_order, nt, snap_interval, learned_ok=True)
     vp, rho = _f32F(vp), _f32F(rho)
@@ -329,7 +330,8 @@ Acoustic3DResult = namedtuple("Acoustic3DResult", ["seis_vx", "seis_vy", "seis_v
 Acoustic3DExtended = namedtuple("Acoustic3DExtended", ["seis_vx", "seis_vy", "seis_vz", "snaps", "seis_p", "stats"])
 _PLANES = {"xz": 0, "xy": 1, "yz": 2}
 _STORAGE = {"f32": _lib.STORE_F32, "float32": _lib.STORE_F32, "f16": _lib.STORE_F16, "float16": _lib.STORE_F16,
-            "bf16": _lib.STORE_BF16, "bfloat16": _lib.STORE_BF16}
+            "bf16": _lib.STORE_BF16, "bfloat16": _lib.STORE_BF16, "mat16": _lib.STORE_F32_MAT16,
+            "f32_mat16": _lib.STORE_F32_MAT16}
 
 
 def storage_code(storage) -> int:
@@ -339,7 +341,7 @@ def storage_code(storage) -> int:
         return int(storage)
     key = _sym(storage).lower()
     if key not in _STORAGE:
-        raise ArgumentError(f"storage must be :f32, :f16 or :bf16, got {storage}")
+        raise ArgumentError(f"storage must be :f32, :f16, :bf16 or :mat16, got {storage}")
     return _STORAGE[key]
 
 

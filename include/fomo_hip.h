@@ -45,6 +45,7 @@ extern "C" {
 #define FOMO_STORE_F32 0
 #define FOMO_STORE_F16 1
 #define FOMO_STORE_BF16 2
+#define FOMO_STORE_F32_MAT16 3 /* 2D acoustic/elastic: Float32 fields, Float16 materials */
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *fomo_last_error(void);
@@ -92,7 +93,7 @@ typedef struct fomo_desc {
   float v_ref;           /* HABC reference velocity */
   int32_t n_src, n_rec;
   int32_t source_type;   /* FOMO_SRC_* (elastic only) */
-  int32_t storage;       /* FOMO_STORE_* (acoustic3d and elastic3d; 2D is f32) */
+  int32_t storage;       /* FOMO_STORE_*; FOMO_STORE_F32_MAT16 for acoustic2d/elastic2d only */
   int32_t snap_interval; /* 0 = no snapshots */
   int32_t snap_plane;    /* 3D snapshots: 0 = xz, 1 = xy, 2 = yz */
   int32_t snap_index;    /* 3D snapshots: 1-based interior index of the slice */
