@@ -141,3 +141,18 @@ def test_seeded_models_are_reproducible():
     vp, vs, rho = a
     assert np.all(vp[:, 0] == 0) and np.all(rho[:, 0] == 0)
     assert 1800 <= vp[:, 1:].min() and vp.max() <= 4500
+
+
+def test_storage_names_map_to_the_descriptor_codes():
+    """storage = :f32 / :f16 / :bf16 / :mat16 (and their long names) map to FOMO_STORE_* codes 0-3;
+    anything else is an ArgumentError before any device work."""
+    import fomo
+    from fomo import _lib
+    from fomo.api import storage_code
+    want = {"f32": 0, "float32": 0, ":f16": 1, "float16": 1, "bf16": 2, "bfloat16": 2, "mat16": 3, "f32_mat16": 3}
+    for name, code in want.items():
+        assert storage_code(name) == code, name
+    assert _lib.STORE_F32_MAT16 == 3
+    for bad in ("f8", "half", 4, -1):
+        with pytest.raises(fomo.ArgumentError):
+            storage_code(bad)
