@@ -717,6 +717,23 @@ void orc_probe_acoustic_pressure(int nx, int nz, int fd_order, real dtx, real *p
   acoustic_pressure(p, vx, vz, kappa, a, N, dtx, dtx, nx, nz);
 }
 
+/* One elastic velocity sweep and one elastic stress sweep (the functions the elastic2d time loop
+ * calls) on a caller-given padded state, for the plane-wave and Hooke's-law known-answer tests. */
+void orc_probe_elastic_velocity(int nx, int nz, int fd_order, real dtx, real *vx, real *vz, const real *txx,
+                                const real *tzz, const real *txz, const real *bx, const real *bz) {
+  real a[5];
+  int N = load_coeffs(fd_order, a);
+  elastic_velocity(vx, vz, txx, tzz, txz, bx, bz, a, N, dtx, dtx, nx, nz);
+}
+
+void orc_probe_elastic_stress(int nx, int nz, int fd_order, real dtx, real *txx, real *tzz, real *txz,
+                              const real *vx, const real *vz, const real *lam, const real *lam2mu,
+                              const real *muxz) {
+  real a[5];
+  int N = load_coeffs(fd_order, a);
+  elastic_stress(txx, tzz, txz, vx, vz, lam, lam2mu, muxz, a, N, dtx, dtx, nx, nz);
+}
+
 void orc_probe_habc(int nx, int nz, int nbc, real q, real qt, real qxt, real *f, real *fo, const real *w) {
   long total = orc_frame_total(nx, nz, nbc);
   real *scr = xcalloc((size_t)total + 1, sizeof(real));

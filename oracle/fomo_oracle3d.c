@@ -139,6 +139,51 @@ static void backup3(real *fo, const real *f, int nx, int ny, int nz, int nbc) {
         if (strip3(i, j, k, nx, ny, nz, nbc)) fo[ID3(i, j, k)] = f[ID3(i, j, k)];
 }
 
+/* acoustic3d velocity sweep, update_velocity.jl:15-52: vx at (i-1/2,j,k), vy at (i,j-1/2,k), vz at
+ * (i,j,k+1/2); interior cells M < i,j,k <= n - M. */
+static void ac3_velocity(real *vx, real *vy, real *vz, const real *p, const real *bx, const real *by, const real *bz,
+                         const real *a, int N, real dtdh, int nx, int ny, int nz) {
+  int M = N;
+#pragma omp parallel for schedule(static)
+  for (int k = M + 1; k <= nz - M; ++k)
+    for (int j = M + 1; j <= ny - M; ++j)
+      for (int i = M + 1; i <= nx - M; ++i) {
+        size_t c = ID3(i, j, k);
+        size_t sy = (size_t)nx, sz = (size_t)nx * ny;
+        real dpdx = 0, dpdy = 0, dpdz = 0;
+        for (int l = 1; l <= N; ++l) {
+          real cl = a[l - 1];
+          dpdx += cl * (p[c + (l - 1)] - p[c - l]);
+          dpdy += cl * (p[c + (size_t)(l - 1) * sy] - p[c - (size_t)l * sy]);
+          dpdz += cl * (p[c + (size_t)l * sz] - p[c - (size_t)(l - 1) * sz]);
+        }
+        vx[c] += (bx[c] * dtdh) * dpdx;
+        vy[c] += (by[c] * dtdh) * dpdy;
+        vz[c] += (bz[c] * dtdh) * dpdz;
+      }
+}
+
+/* acoustic3d pressure sweep, update_pressure.jl:13-48. */
+static void ac3_pressure(real *p, const real *vx, const real *vy, const real *vz, const real *kappa, const real *a,
+                         int N, real dtdh, int nx, int ny, int nz) {
+  int M = N;
+#pragma omp parallel for schedule(static)
+  for (int k = M + 1; k <= nz - M; ++k)
+    for (int j = M + 1; j <= ny - M; ++j)
+      for (int i = M + 1; i <= nx - M; ++i) {
+        size_t c = ID3(i, j, k);
+        size_t sy = (size_t)nx, sz = (size_t)nx * ny;
+        real dvxdx = 0, dvydy = 0, dvzdz = 0;
+        for (int l = 1; l <= N; ++l) {
+          real cl = a[l - 1];
+          dvxdx += cl * (vx[c + l] - vx[c - (l - 1)]);
+          dvydy += cl * (vy[c + (size_t)l * sy] - vy[c - (size_t)(l - 1) * sy]);
+          dvzdz += cl * (vz[c + (size_t)(l - 1) * sz] - vz[c - (size_t)l * sz]);
+        }
+        p[c] += (kappa[c] * dtdh) * ((dvxdx + dvydy) + dvzdz);
+      }
+}
+
 /*
  * acoustic3d (boundary HABC) -- inputs vp, rho: nxi*nyi*nzi (x fastest); 1-based interior
  * indices; wavelet n_src x nt column-major.  Outputs seis_{vx,vy,vz,p} n_rec x nt (any may be
@@ -183,23 +228,7 @@ int orc_acoustic3d(int nxi, int nyi, int nzi, const float *vp, const float *rho,
     backup3(vxo, vx, nx, ny, nz, H);
     backup3(vyo, vy, nx, ny, nz, H);
     backup3(vzo, vz, nx, ny, nz, H);
-#pragma omp parallel for schedule(static)
-    for (int k = M + 1; k <= nz - M; ++k)
-      for (int j = M + 1; j <= ny - M; ++j)
-        for (int i = M + 1; i <= nx - M; ++i) {
-          size_t c = ID3(i, j, k);
-          size_t sy = (size_t)nx, sz = (size_t)nx * ny;
-          real dpdx = 0, dpdy = 0, dpdz = 0;
-          for (int l = 1; l <= N; ++l) {
-            real cl = a[l - 1];
-            dpdx += cl * (p[c + (l - 1)] - p[c - l]);
-            dpdy += cl * (p[c + (size_t)(l - 1) * sy] - p[c - (size_t)l * sy]);
-            dpdz += cl * (p[c + (size_t)l * sz] - p[c - (size_t)(l - 1) * sz]);
-          }
-          vx[c] += (bx[c] * dtdh) * dpdx;
-          vy[c] += (by[c] * dtdh) * dpdy;
-          vz[c] += (bz[c] * dtdh) * dpdz;
-        }
+    ac3_velocity(vx, vy, vz, p, bx, by, bz, a, N, dtdh, nx, ny, nz);
     memcpy(fpre, vx, n * sizeof(real));
     habc3d(vx, fpre, vxo, nx, ny, nz, H, pad, 0.0, q, qt, qxt);
     memcpy(fpre, vy, n * sizeof(real));
@@ -208,21 +237,7 @@ int orc_acoustic3d(int nxi, int nyi, int nzi, const float *vp, const float *rho,
     habc3d(vz, fpre, vzo, nx, ny, nz, H, pad, 0.5, q, qt, qxt);
 
     backup3(fo, p, nx, ny, nz, H);
-#pragma omp parallel for schedule(static)
-    for (int k = M + 1; k <= nz - M; ++k)
-      for (int j = M + 1; j <= ny - M; ++j)
-        for (int i = M + 1; i <= nx - M; ++i) {
-          size_t c = ID3(i, j, k);
-          size_t sy = (size_t)nx, sz = (size_t)nx * ny;
-          real dvxdx = 0, dvydy = 0, dvzdz = 0;
-          for (int l = 1; l <= N; ++l) {
-            real cl = a[l - 1];
-            dvxdx += cl * (vx[c + l] - vx[c - (l - 1)]);
-            dvydy += cl * (vy[c + (size_t)l * sy] - vy[c - (size_t)(l - 1) * sy]);
-            dvzdz += cl * (vz[c + (size_t)(l - 1) * sz] - vz[c - (size_t)l * sz]);
-          }
-          p[c] += (kappa[c] * dtdh) * ((dvxdx + dvydy) + dvzdz);
-        }
+    ac3_pressure(p, vx, vy, vz, kappa, a, N, dtdh, nx, ny, nz);
     memcpy(fpre, p, n * sizeof(real));
     habc3d(p, fpre, fo, nx, ny, nz, H, pad, 0.75, q, qt, qxt);
 
@@ -305,6 +320,73 @@ void orc_elastic_params3d(const real *vp, const real *vs, const real *rho, int n
   free(mu);
 }
 
+/* elastic3d velocity sweep, update_velocity.jl:17-84 (F = vx, vy, vz, txx, tyy, tzz, txy, txz, tyz). */
+static void el3_velocity(real *const *F, const real *bx, const real *by, const real *bz, const real *a, int N,
+                         real dtdh, int nx, int ny, int nz) {
+  int M = N;
+  const size_t sy_ = (size_t)nx, sz_ = (size_t)nx * ny;
+  real *vx = F[0], *vy = F[1], *vz = F[2];
+  const real *txx = F[3], *tyy = F[4], *tzz = F[5], *txy = F[6], *txz = F[7], *tyz = F[8];
+#pragma omp parallel for schedule(static)
+  for (int k = M + 1; k <= nz - M; ++k)
+    for (int j = M + 1; j <= ny - M; ++j)
+      for (int i = M + 1; i <= nx - M; ++i) {
+        size_t c = ID3(i, j, k);
+        real dtxxdx = 0, dtxydy = 0, dtxzdz = 0, dtxydx = 0, dtyydy = 0, dtyzdz = 0, dtxzdx = 0, dtyzdy = 0,
+             dtzzdz = 0;
+        for (int l = 1; l <= N; ++l) {
+          real cl = a[l - 1];
+          dtxxdx += cl * (txx[c + (l - 1)] - txx[c - l]);
+          dtyydy += cl * (tyy[c + (l - 1) * sy_] - tyy[c - l * sy_]);
+          dtzzdz += cl * (tzz[c + l * sz_] - tzz[c - (l - 1) * sz_]);
+          dtxydy += cl * (txy[c + l * sy_] - txy[c - (l - 1) * sy_]);
+          dtxydx += cl * (txy[c + l] - txy[c - (l - 1)]);
+          dtxzdz += cl * (txz[c + (l - 1) * sz_] - txz[c - l * sz_]);
+          dtxzdx += cl * (txz[c + l] - txz[c - (l - 1)]);
+          dtyzdz += cl * (tyz[c + (l - 1) * sz_] - tyz[c - l * sz_]);
+          dtyzdy += cl * (tyz[c + l * sy_] - tyz[c - (l - 1) * sy_]);
+        }
+        vx[c] += (bx[c] * dtdh) * ((dtxxdx + dtxydy) + dtxzdz);
+        vy[c] += (by[c] * dtdh) * ((dtxydx + dtyydy) + dtyzdz);
+        vz[c] += (bz[c] * dtdh) * ((dtxzdx + dtyzdy) + dtzzdz);
+      }
+}
+
+/* elastic3d stress sweep, update_stress.jl:21-94. */
+static void el3_stress(real *const *F, const real *lam, const real *l2m, const real *mxy, const real *mxz,
+                       const real *myz, const real *a, int N, real dtdh, int nx, int ny, int nz) {
+  int M = N;
+  const size_t sy_ = (size_t)nx, sz_ = (size_t)nx * ny;
+  const real *vx = F[0], *vy = F[1], *vz = F[2];
+  real *txx = F[3], *tyy = F[4], *tzz = F[5], *txy = F[6], *txz = F[7], *tyz = F[8];
+#pragma omp parallel for schedule(static)
+  for (int k = M + 1; k <= nz - M; ++k)
+    for (int j = M + 1; j <= ny - M; ++j)
+      for (int i = M + 1; i <= nx - M; ++i) {
+        size_t c = ID3(i, j, k);
+        real dvxdx = 0, dvydy = 0, dvzdz = 0, dvxdy = 0, dvydx = 0, dvxdz = 0, dvzdx = 0, dvydz = 0, dvzdy = 0;
+        for (int l = 1; l <= N; ++l) {
+          real cl = a[l - 1];
+          dvxdx += cl * (vx[c + l] - vx[c - (l - 1)]);
+          dvydy += cl * (vy[c + l * sy_] - vy[c - (l - 1) * sy_]);
+          dvzdz += cl * (vz[c + (l - 1) * sz_] - vz[c - l * sz_]);
+          dvxdy += cl * (vx[c + (l - 1) * sy_] - vx[c - l * sy_]);
+          dvydx += cl * (vy[c + (l - 1)] - vy[c - l]);
+          dvxdz += cl * (vx[c + l * sz_] - vx[c - (l - 1) * sz_]);
+          dvzdx += cl * (vz[c + (l - 1)] - vz[c - l]);
+          dvydz += cl * (vy[c + l * sz_] - vy[c - (l - 1) * sz_]);
+          dvzdy += cl * (vz[c + (l - 1) * sy_] - vz[c - l * sy_]);
+        }
+        real lv = lam[c], l2 = l2m[c];
+        txx[c] += ((l2 * dvxdx + lv * dvydy) + lv * dvzdz) * dtdh;
+        tyy[c] += ((lv * dvxdx + l2 * dvydy) + lv * dvzdz) * dtdh;
+        tzz[c] += ((lv * dvxdx + lv * dvydy) + l2 * dvzdz) * dtdh;
+        txy[c] += (mxy[c] * (dvxdy + dvydx)) * dtdh;
+        txz[c] += (mxz[c] * (dvxdz + dvzdx)) * dtdh;
+        tyz[c] += (myz[c] * (dvydz + dvzdy)) * dtdh;
+      }
+}
+
 /*
  * elastic3d (boundary HABC) -- inputs vp, vs, rho: nxi*nyi*nzi; outputs seis_{vx,vy,vz} n_rec x nt
  * (any may be NULL); final_state (optional): vx, vy, vz, txx, tyy, tzz, txy, txz, tyz.
@@ -324,7 +406,6 @@ int orc_elastic3d(int nxi, int nyi, int nzi, const float *vp, const float *vs, c
   int M = N, pad = nbc + M, H = pad - 1;
   int nx = nxi + 2 * pad, ny = nyi + 2 * pad, nz = nzi + 2 * pad;
   size_t n = (size_t)nx * ny * nz;
-  const size_t sy_ = (size_t)nx, sz_ = (size_t)nx * ny;
   real *arr[32];
   for (int q = 0; q < 29; ++q) {
     arr[q] = calloc(n, sizeof(real));
@@ -338,8 +419,7 @@ int orc_elastic3d(int nxi, int nyi, int nzi, const float *vp, const float *vs, c
     F[f] = arr[11 + f];
     FO[f] = arr[20 + f];
   }
-  real *vx = F[0], *vy = F[1], *vz = F[2], *txx = F[3], *tyy = F[4], *tzz = F[5], *txy = F[6], *txz = F[7],
-       *tyz = F[8];
+  real *vx = F[0], *vy = F[1], *vz = F[2];
   orc_pad3d(vp, nxi, nyi, nzi, pad, vpp);
   orc_pad3d(vs, nxi, nyi, nzi, pad, vsp);
   orc_pad3d(rho, nxi, nyi, nzi, pad, rhop);
@@ -353,61 +433,14 @@ int orc_elastic3d(int nxi, int nyi, int nzi, const float *vp, const float *vs, c
   const double voff[3] = {0.0, 0.25, 0.5};
   for (int it = 1; it <= nt; ++it) {
     for (int f = 0; f < 3; ++f) backup3(FO[f], F[f], nx, ny, nz, H);
-#pragma omp parallel for schedule(static)
-    for (int k = M + 1; k <= nz - M; ++k)
-      for (int j = M + 1; j <= ny - M; ++j)
-        for (int i = M + 1; i <= nx - M; ++i) {
-          size_t c = ID3(i, j, k);
-          real dtxxdx = 0, dtxydy = 0, dtxzdz = 0, dtxydx = 0, dtyydy = 0, dtyzdz = 0, dtxzdx = 0, dtyzdy = 0,
-               dtzzdz = 0;
-          for (int l = 1; l <= N; ++l) {
-            real cl = a[l - 1];
-            dtxxdx += cl * (txx[c + (l - 1)] - txx[c - l]);
-            dtyydy += cl * (tyy[c + (l - 1) * sy_] - tyy[c - l * sy_]);
-            dtzzdz += cl * (tzz[c + l * sz_] - tzz[c - (l - 1) * sz_]);
-            dtxydy += cl * (txy[c + l * sy_] - txy[c - (l - 1) * sy_]);
-            dtxydx += cl * (txy[c + l] - txy[c - (l - 1)]);
-            dtxzdz += cl * (txz[c + (l - 1) * sz_] - txz[c - l * sz_]);
-            dtxzdx += cl * (txz[c + l] - txz[c - (l - 1)]);
-            dtyzdz += cl * (tyz[c + (l - 1) * sz_] - tyz[c - l * sz_]);
-            dtyzdy += cl * (tyz[c + l * sy_] - tyz[c - (l - 1) * sy_]);
-          }
-          vx[c] += (bx[c] * dtdh) * ((dtxxdx + dtxydy) + dtxzdz);
-          vy[c] += (by[c] * dtdh) * ((dtxydx + dtyydy) + dtyzdz);
-          vz[c] += (bz[c] * dtdh) * ((dtxzdx + dtyzdy) + dtzzdz);
-        }
+    el3_velocity(F, bx, by, bz, a, N, dtdh, nx, ny, nz);
     for (int f = 0; f < 3; ++f) {
       memcpy(fpre, F[f], n * sizeof(real));
       habc3d(F[f], fpre, FO[f], nx, ny, nz, H, pad, voff[f], q, qt, qxt);
     }
 
     for (int f = 3; f < 9; ++f) backup3(FO[f], F[f], nx, ny, nz, H);
-#pragma omp parallel for schedule(static)
-    for (int k = M + 1; k <= nz - M; ++k)
-      for (int j = M + 1; j <= ny - M; ++j)
-        for (int i = M + 1; i <= nx - M; ++i) {
-          size_t c = ID3(i, j, k);
-          real dvxdx = 0, dvydy = 0, dvzdz = 0, dvxdy = 0, dvydx = 0, dvxdz = 0, dvzdx = 0, dvydz = 0, dvzdy = 0;
-          for (int l = 1; l <= N; ++l) {
-            real cl = a[l - 1];
-            dvxdx += cl * (vx[c + l] - vx[c - (l - 1)]);
-            dvydy += cl * (vy[c + l * sy_] - vy[c - (l - 1) * sy_]);
-            dvzdz += cl * (vz[c + (l - 1) * sz_] - vz[c - l * sz_]);
-            dvxdy += cl * (vx[c + (l - 1) * sy_] - vx[c - l * sy_]);
-            dvydx += cl * (vy[c + (l - 1)] - vy[c - l]);
-            dvxdz += cl * (vx[c + l * sz_] - vx[c - (l - 1) * sz_]);
-            dvzdx += cl * (vz[c + (l - 1)] - vz[c - l]);
-            dvydz += cl * (vy[c + l * sz_] - vy[c - (l - 1) * sz_]);
-            dvzdy += cl * (vz[c + (l - 1) * sy_] - vz[c - l * sy_]);
-          }
-          real lv = lam[c], l2 = l2m[c];
-          txx[c] += ((l2 * dvxdx + lv * dvydy) + lv * dvzdz) * dtdh;
-          tyy[c] += ((lv * dvxdx + l2 * dvydy) + lv * dvzdz) * dtdh;
-          tzz[c] += ((lv * dvxdx + lv * dvydy) + l2 * dvzdz) * dtdh;
-          txy[c] += (mxy[c] * (dvxdy + dvydx)) * dtdh;
-          txz[c] += (mxz[c] * (dvxdz + dvzdx)) * dtdh;
-          tyz[c] += (myz[c] * (dvydz + dvzdy)) * dtdh;
-        }
+    el3_stress(F, lam, l2m, mxy, mxz, myz, a, N, dtdh, nx, ny, nz);
     for (int f = 3; f < 9; ++f) {
       memcpy(fpre, F[f], n * sizeof(real));
       habc3d(F[f], fpre, FO[f], nx, ny, nz, H, pad, 0.75, q, qt, qxt);
@@ -427,4 +460,42 @@ int orc_elastic3d(int nxi, int nyi, int nzi, const float *vp, const float *vs, c
     for (int f = 0; f < 9; ++f) memcpy(final_state + (size_t)f * n, F[f], n * sizeof(real));
   for (int q2 = 0; q2 < 29; ++q2) free(arr[q2]);
   return 0;
+}
+
+/* Single sweeps on a caller-given padded state (the functions the 3D time loops call), for the
+ * plane-wave known-answer tests.  F: vx, vy, vz, txx, tyy, tzz, txy, txz, tyz (9 arrays). */
+void orc_probe_acoustic3d_velocity(int nx, int ny, int nz, int fd_order, real dtdh, real *vx, real *vy, real *vz,
+                                   const real *p, const real *bx, const real *by, const real *bz) {
+  float af[5];
+  real a[5];
+  int N = orc_fd_coeffs(fd_order, af);
+  for (int l = 0; l < N; ++l) a[l] = (real)af[l];
+  ac3_velocity(vx, vy, vz, p, bx, by, bz, a, N, dtdh, nx, ny, nz);
+}
+
+void orc_probe_acoustic3d_pressure(int nx, int ny, int nz, int fd_order, real dtdh, real *p, const real *vx,
+                                   const real *vy, const real *vz, const real *kappa) {
+  float af[5];
+  real a[5];
+  int N = orc_fd_coeffs(fd_order, af);
+  for (int l = 0; l < N; ++l) a[l] = (real)af[l];
+  ac3_pressure(p, vx, vy, vz, kappa, a, N, dtdh, nx, ny, nz);
+}
+
+void orc_probe_elastic3d_velocity(int nx, int ny, int nz, int fd_order, real dtdh, real *const *F, const real *bx,
+                                  const real *by, const real *bz) {
+  float af[5];
+  real a[5];
+  int N = orc_fd_coeffs(fd_order, af);
+  for (int l = 0; l < N; ++l) a[l] = (real)af[l];
+  el3_velocity(F, bx, by, bz, a, N, dtdh, nx, ny, nz);
+}
+
+void orc_probe_elastic3d_stress(int nx, int ny, int nz, int fd_order, real dtdh, real *const *F, const real *lam,
+                                const real *l2m, const real *mxy, const real *mxz, const real *myz) {
+  float af[5];
+  real a[5];
+  int N = orc_fd_coeffs(fd_order, af);
+  for (int l = 0; l < N; ++l) a[l] = (real)af[l];
+  el3_stress(F, lam, l2m, mxy, mxz, myz, a, N, dtdh, nx, ny, nz);
 }

@@ -79,6 +79,18 @@ def lib(precision: str = "f32") -> C.CDLL:
         L.orc_probe_acoustic_velocity.argtypes = [_i, _i, _i, real, _p, _p, _p, _p, _p]
         L.orc_probe_acoustic_pressure.restype = None
         L.orc_probe_acoustic_pressure.argtypes = [_i, _i, _i, real, _p, _p, _p, _p]
+        L.orc_probe_elastic_velocity.restype = None
+        L.orc_probe_elastic_velocity.argtypes = [_i, _i, _i, real] + [_p] * 7
+        L.orc_probe_elastic_stress.restype = None
+        L.orc_probe_elastic_stress.argtypes = [_i, _i, _i, real] + [_p] * 8
+        L.orc_probe_acoustic3d_velocity.restype = None
+        L.orc_probe_acoustic3d_velocity.argtypes = [_i, _i, _i, _i, real] + [_p] * 7
+        L.orc_probe_acoustic3d_pressure.restype = None
+        L.orc_probe_acoustic3d_pressure.argtypes = [_i, _i, _i, _i, real] + [_p] * 5
+        L.orc_probe_elastic3d_velocity.restype = None
+        L.orc_probe_elastic3d_velocity.argtypes = [_i, _i, _i, _i, real] + [_p] * 4
+        L.orc_probe_elastic3d_stress.restype = None
+        L.orc_probe_elastic3d_stress.argtypes = [_i, _i, _i, _i, real] + [_p] * 6
         L.orc_probe_habc.restype = None
         L.orc_probe_habc.argtypes = [_i, _i, _i, real, real, real, _p, _p, _p]
         L.orc_max_threads.restype = _i
@@ -174,6 +186,64 @@ def probe_acoustic_pressure(p, vx, vz, kappa, fd_order: int, dtx: float, precisi
     lib(precision).orc_probe_acoustic_pressure(p.shape[0], p.shape[1], fd_order, dtx, p.ctypes.data,
                                                vx.ctypes.data, vz.ctypes.data, kappa.ctypes.data)
     return p
+
+
+def _fcopy(arrays, precision):
+    dt = _dtype(precision)
+    return [np.array(x, dtype=dt, order="F", copy=True) for x in arrays]
+
+
+def probe_elastic_velocity(vx, vz, txx, tzz, txz, bx, bz, fd_order: int, dtx: float, precision: str = "f64"):
+    """One elastic2d velocity sweep on a given padded (nx, nz) state; returns new (vx, vz)."""
+    vx, vz, txx, tzz, txz, bx, bz = _fcopy((vx, vz, txx, tzz, txz, bx, bz), precision)
+    lib(precision).orc_probe_elastic_velocity(vx.shape[0], vx.shape[1], fd_order, dtx,
+                                              *(a.ctypes.data for a in (vx, vz, txx, tzz, txz, bx, bz)))
+    return vx, vz
+
+
+def probe_elastic_stress(txx, tzz, txz, vx, vz, lam, lam2mu, muxz, fd_order: int, dtx: float,
+                         precision: str = "f64"):
+    """One elastic2d stress sweep on a given padded state; returns new (txx, tzz, txz)."""
+    arrs = _fcopy((txx, tzz, txz, vx, vz, lam, lam2mu, muxz), precision)
+    lib(precision).orc_probe_elastic_stress(arrs[0].shape[0], arrs[0].shape[1], fd_order, dtx,
+                                            *(a.ctypes.data for a in arrs))
+    return arrs[0], arrs[1], arrs[2]
+
+
+def probe_acoustic3d_velocity(vx, vy, vz, p, bx, by, bz, fd_order: int, dtdh: float, precision: str = "f64"):
+    """One acoustic3d velocity sweep on a given padded (nx, ny, nz) state; returns (vx, vy, vz)."""
+    arrs = _fcopy((vx, vy, vz, p, bx, by, bz), precision)
+    lib(precision).orc_probe_acoustic3d_velocity(*arrs[0].shape, fd_order, dtdh, *(a.ctypes.data for a in arrs))
+    return arrs[0], arrs[1], arrs[2]
+
+
+def probe_acoustic3d_pressure(p, vx, vy, vz, kappa, fd_order: int, dtdh: float, precision: str = "f64"):
+    """One acoustic3d pressure sweep on a given padded state; returns p."""
+    arrs = _fcopy((p, vx, vy, vz, kappa), precision)
+    lib(precision).orc_probe_acoustic3d_pressure(*arrs[0].shape, fd_order, dtdh, *(a.ctypes.data for a in arrs))
+    return arrs[0]
+
+
+def _fields9(F, precision):
+    arrs = _fcopy(F, precision)
+    ptrs = (C.c_void_p * 9)(*(a.ctypes.data for a in arrs))
+    return arrs, ptrs
+
+
+def probe_elastic3d_velocity(F, bx, by, bz, fd_order: int, dtdh: float, precision: str = "f64"):
+    """One elastic3d velocity sweep.  F: (vx, vy, vz, txx, tyy, tzz, txy, txz, tyz); returns new F."""
+    arrs, ptrs = _fields9(F, precision)
+    m = _fcopy((bx, by, bz), precision)
+    lib(precision).orc_probe_elastic3d_velocity(*arrs[0].shape, fd_order, dtdh, ptrs, *(a.ctypes.data for a in m))
+    return arrs
+
+
+def probe_elastic3d_stress(F, lam, lam2mu, mxy, mxz, myz, fd_order: int, dtdh: float, precision: str = "f64"):
+    """One elastic3d stress sweep.  F as in probe_elastic3d_velocity; returns new F."""
+    arrs, ptrs = _fields9(F, precision)
+    m = _fcopy((lam, lam2mu, mxy, mxz, myz), precision)
+    lib(precision).orc_probe_elastic3d_stress(*arrs[0].shape, fd_order, dtdh, ptrs, *(a.ctypes.data for a in m))
+    return arrs
 
 
 def probe_habc(f, fo, w, nbc: int, q: float, qt: float, qxt: float, precision: str = "f32"):
