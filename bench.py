@@ -1,22 +1,32 @@
 #!/usr/bin/env python3
-"""Benchmark: elastic2d time steps on one MI355X, with roofline and CPU-baseline figures.
+"""Benchmark: acoustic3d time steps on one MI355X, with roofline and CPU-baseline figures.
 
-Workload (plain elastic2d, FP32): interior 2000 x 800, dh 5 m, dt 0.4 ms,
-fd_order 8, nbc 50 (padded 2108 x 908 = 1,914,064 cells), seeded random layered model (12
-jittered interfaces, vp U[1800,4500], vs = vp/1.73, Gardner rho, vacuum top row), vertical-force
-Ricker 20 Hz at (1000, 5), 2000 receivers at z = 5.  A step is one full time step (velocity sweep,
-stress sweep, gather) replayed from the captured hipGraph.
+Headline workload (acoustic3d, FP32 storage and arithmetic): interior 512^3, dh 10 m, dt 0.8 ms,
+fd_order 8, nbc 32 (padded 584^3 = 199,176,704 cells), seeded layered 3D model (12 jittered
+interfaces, vp U[1800,4500], Gaussian perturbations of sigma 30 cells and +-5 %, Gardner rho),
+12 Hz Ricker at the top centre (256, 256, 1), 256 x 256 surface receivers (every second cell,
+depth 1) recording vx, vy, vz.  A step is one full time step (velocity phase, pressure phase,
+receiver gather) replayed from the captured hipGraph; the workload has 1500 steps.
 
 value = padded cell-updates per second = N_pad * steps / t, with t the hipEvent time of exactly
-`steps` steps (inputs resident in HBM).  roofline: the dominant kernel's algorithmic bytes per
-launch (velocity sweep 36 B/cell, stress sweep 44 B/cell, FP32) over its average launch time,
-measured with hipEvents on the plan's stream.  cpu_baseline: the C oracle (-O3, OpenMP) on a
-bounded number of steps of the same workload.
+`steps` steps (inputs resident in HBM).  roofline: the dominant launch of a step (the largest
+per-launch time, each launch timed alone with hipEvents on the plan's stream) at its algorithmic
+bytes (the reference's per-phase reads and writes: velocity 40 B, pressure 24 B per cell, FP32)
+over its average launch time.  traffic: HBM bytes per launch of that kernel from two rocprofv3
+--pmc runs of this script (FETCH_SIZE, then WRITE_SIZE) in child processes, each scaled by what
+the same counter reads on a streaming copy of known size in the same run.  cpu_baseline: the C
+oracle (-O3, OpenMP on the host cores) stepping the same 512^3 workload for a bounded number of
+steps (the setup cancels between two runs of different length).
 """
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -25,13 +35,14 @@ for p in (os.path.join(ROOT, "fomo.jl_amd"), os.path.join(ROOT, "oracle")):
         sys.path.insert(0, p)
 
 METRIC = "cell-updates/s & % HBM roofline, FD8 2D/3D; learned-ABC+FP16 vs HABC at equal error"
-# HBM-side bytes per launch of the two sweeps at this workload, measured with rocprofv3 --pmc
-# (FETCH_SIZE scaled x2.0 and WRITE_SIZE x1.0, the factors the same counters read on a streaming
-# copy of known size); profiles/elastic2d_2000x800_fd8_pmc_traffic.md holds the readings.
-PMC_TRAFFIC = {"velocity_sweep": 74.8e6, "stress_sweep": 88.9e6}
 HBM_PEAK_GBS = 8000.0
-NX, NZ, DH, DT, NT, F0, NBC, ORDER, SEED = 2000, 800, 5.0, 4e-4, 4000, 20.0, 50, 8, 20260101
+SEED = 20260101
+# the 3D workload
+N3, DH3, DT3, NT3, F03, NBC3 = 512, 10.0, 8e-4, 1500, 12.0, 32
+# the 2D elastic workload (an extra line in the output)
+NX, NZ, DH, DT, NT, F0, NBC, ORDER = 2000, 800, 5.0, 4e-4, 4000, 20.0, 50, 8
 BYTES_VEL, BYTES_STR = 36, 44
+COPY_BYTES = 1 << 30
 
 
 def log(msg):
@@ -57,37 +68,191 @@ def cpu_rate(run, cells, n_lo, n_hi, budget_s=8.0, n_max=3000):
     return cells / per, n, per * n
 
 
-def cpu_baseline(vp, vs, rho, rx, rz, budget_s):
-    """Oracle FP32 elastic2d on the host cores; per-step time from two runs of different length
-    (so setup cancels)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def model3d():
+    from fomo.models import layered_3d
+    log(f"building the seeded {N3}^3 layered model")
+    return layered_3d(N3, N3, N3, seed=SEED)
+
+
+def geometry3d():
     import numpy as np
+    rx = np.arange(1, N3 + 1, 2)
+    gx, gy = np.meshgrid(rx, rx, indexing="ij")
+    return dict(sx=[N3 // 2], sy=[N3 // 2], sz=[1], rx=gx.ravel(), ry=gy.ravel(), rz=np.ones(gx.size, int))
+
+
+def plan3d(vp, rho, nt, storage="f32", use_graph=True):
+    import fomo
+    import fomo._lib as L
+    from fomo.api import Plan
+    w = fomo.ricker_wavelet(F03, DT3, nt)
+    return Plan(L.ACOUSTIC3D, vp, rho, DH3, DT3, nt, wavelet=w[None], nbc=NBC3, fd_order=8, storage=storage,
+                use_graph=use_graph, **geometry3d())
+
+
+def cpu_baseline3d(vp, rho, budget_s):
+    """Oracle FP32 acoustic3d on the host cores over the full 512^3 workload: a 1-step run and a
+    (1 + n)-step run, n sized to about budget_s of stepping; their difference is n steps."""
     import oracle
-
     oracle.lib("f32")
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-    threads = min(threads, os.cpu_count() or threads)
-    w = None
+    th = host_threads()
+    geo = geometry3d()
 
-    def run(n):
-        nonlocal w
-        from fomo import ricker_wavelet
-        w = ricker_wavelet(F0, DT, n)[None]
+    def run(k):
+        w = oracle.ricker(F03, DT3, k)[None]
         t = time.perf_counter()
-        oracle.elastic2d(vp, vs, rho, DH, DT, n, sx=[1000], sz=[5], rx=rx, rz=rz, wavelet=w, nbc=NBC,
-                         fd_order=ORDER, source_type=1, nthreads=threads)
+        oracle.acoustic3d(vp, rho, DH3, DT3, k, wavelet=w, nbc=NBC3, fd_order=8, nthreads=th, **geo)
         return time.perf_counter() - t
 
-    t_a = run(4)
-    t_b = run(24)
-    per_step = max((t_b - t_a) / 20.0, 1e-6)
-    n = int(max(20, min(4000, budget_s / per_step)))
-    t_c = run(4 + n)
-    per_step = (t_c - t_a) / n
-    n_pad = (NX + 2 * (NBC + ORDER // 2)) * (NZ + 2 * (NBC + ORDER // 2))
-    _ = np
-    return {"value": n_pad / per_step, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"oracle elastic2d FP32 (C, -O3, OpenMP {threads} threads), same 2000x800 workload, "
-                      f"{n} steps timed (minus a 4-step run for setup), {n * per_step:.1f} s of CPU work"}
+    t1 = run(1)
+    t2 = run(3)
+    per = max((t2 - t1) / 2.0, 1e-3)
+    n = int(max(2, min(60, budget_s / per)))
+    t3 = run(1 + n)
+    per = (t3 - t1) / n
+    n_pad = (N3 + 2 * (NBC3 + 4)) ** 3
+    return {"value": n_pad / per, "unit": "cell-updates/s", "cores": th, "kind": "port", "cpu": cpu_model(),
+            "sample": f"oracle acoustic3d FP32 (C, -O3, OpenMP {th} threads on '{cpu_model()}'), the same "
+                      f"{N3}^3 workload and receivers, {n} steps timed as a (1+{n})-step run minus a 1-step run "
+                      f"({n * per:.1f} s of stepping)"}
+
+
+# ---------------------------------------------------------------------------------------------
+# HBM traffic per launch: rocprofv3 --pmc in child processes
+
+def traffic_child(args):
+    """Run under rocprofv3 --pmc: the calibration copy first, then the 3D plan's launches, each
+    launched twice through the per-launch timer (the last 2 x launches dispatches)."""
+    import ctypes as C
+
+    import numpy as np
+
+    import fomo._lib as L
+    ms = C.c_double(0.0)
+    L.check(L.lib().fomo_stream_copy(COPY_BYTES, 2, C.byref(ms)))
+    vp = np.load(os.path.join(args.traffic_child, "vp.npy"), mmap_mode="r")
+    rho = np.load(os.path.join(args.traffic_child, "rho.npy"), mmap_mode="r")
+    plan = plan3d(np.asfortranarray(vp), np.asfortranarray(rho), 4, storage=args.traffic_storage, use_graph=False)
+    plan.advance(2)
+    plan.sync()
+    info = plan.launches()
+    plan.launch_times(1)
+    plan.close()
+    print(json.dumps({"launches": [x["name"] for x in info]}), flush=True)
+
+
+def _dispatches(outdir):
+    """{dispatch id: (kernel name, summed counter value)} from rocprofv3's counter_collection.csv."""
+    rows = {}
+    for path in glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                d = int(r["Dispatch_Id"])
+                name, v = rows.get(d, (r["Kernel_Name"], 0.0))
+                rows[d] = (name, v + float(r["Counter_Value"]))
+    return rows
+
+
+def measure_traffic(vp, rho, storage="f32", timeout_s=300):
+    """HBM bytes per launch of each launch of a step: FETCH_SIZE and WRITE_SIZE (KiB per dispatch)
+    from two rocprofv3 --pmc runs of traffic_child, scaled by the readings on the streaming copy
+    (COPY_BYTES read and written per dispatch).  Returns None when rocprofv3 is unavailable or a
+    run fails.  Must run before this process touches the GPU (the children use it)."""
+    import numpy as np
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        log("rocprofv3 not found: traffic not measured")
+        return None
+    work = tempfile.mkdtemp(prefix="fomo_traffic_", dir="/tmp")
+    try:
+        np.save(os.path.join(work, "vp.npy"), np.ascontiguousarray(vp.ravel(order="F")).reshape(vp.shape, order="F"))
+        np.save(os.path.join(work, "rho.npy"), np.ascontiguousarray(rho.ravel(order="F")).reshape(rho.shape, order="F"))
+        env = dict(os.environ, TMPDIR="/tmp")
+        res = {}
+        names = None
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            out = os.path.join(work, counter)
+            cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "run", "--", sys.executable,
+                   os.path.abspath(__file__), "--traffic-child", work, "--traffic-storage", storage]
+            log(f"traffic: rocprofv3 --pmc {counter} (child process)")
+            try:
+                cp = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True, timeout=timeout_s)
+            except subprocess.TimeoutExpired:
+                log(f"traffic: {counter} run timed out")
+                return None
+            lines = [ln for ln in cp.stdout.splitlines() if ln.startswith("{")]
+            if cp.returncode != 0 or not lines:
+                log(f"traffic: {counter} run failed ({cp.returncode}): {cp.stderr[-400:]}")
+                return None
+            names = json.loads(lines[-1])["launches"]
+            disp = _dispatches(out)
+            ids = sorted(disp)
+            copy = [disp[d][1] for d in ids if "stream_copy" in disp[d][0]]
+            if len(copy) < 2 or len(ids) < 2 * len(names):
+                log(f"traffic: {counter}: unexpected dispatch list ({len(ids)} dispatches)")
+                return None
+            tail = ids[-2 * len(names):]
+            per = []
+            for k in range(len(names)):
+                (n0, v0), (n1, v1) = disp[tail[2 * k]], disp[tail[2 * k + 1]]
+                if n0 != n1:
+                    log(f"traffic: dispatch pairing mismatch ({n0} / {n1})")
+                    return None
+                per.append((n0, 0.5 * (v0 + v1)))
+            copy_kib = sum(copy) / len(copy)
+            res[counter] = {"scale": COPY_BYTES / (copy_kib * 1024.0), "copy_kib": copy_kib, "per_launch": per}
+        out = []
+        for k, name in enumerate(names):
+            kern, fk = res["FETCH_SIZE"]["per_launch"][k]
+            _, wk = res["WRITE_SIZE"]["per_launch"][k]
+            rd = fk * 1024.0 * res["FETCH_SIZE"]["scale"]
+            wr = wk * 1024.0 * res["WRITE_SIZE"]["scale"]
+            out.append({"launch": name, "kernel": kern, "fetch_kib": fk, "write_kib": wk, "read_bytes": rd,
+                        "written_bytes": wr, "bytes": rd + wr})
+        return {"launches": out, "fetch_scale": res["FETCH_SIZE"]["scale"], "write_scale": res["WRITE_SIZE"]["scale"],
+                "copy_fetch_kib": res["FETCH_SIZE"]["copy_kib"], "copy_write_kib": res["WRITE_SIZE"]["copy_kib"]}
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def elastic2d_line(args):
+    """The elastic2d 2000x800 workload (dh 5 m, dt 0.4 ms, FD8, nbc 50, seeded layered model with a
+    vacuum top row, 20 Hz vertical force at (1000, 5), 2000 receivers at z = 5), timed over its
+    4000 steps: a secondary figure beside the 3D headline (its 153 MB per step stays in the
+    256 MB Infinity Cache, so it is not an HBM figure)."""
+    import numpy as np
+
+    import fomo
+    import fomo._lib as L
+    from fomo.api import Plan
+    from fomo.models import layered_2d
+    vp, vs, rho = layered_2d(NX, NZ, seed=SEED)
+    rx = np.arange(1, NX + 1)
+    w = fomo.ricker_wavelet(F0, DT, NT)
+    plan = Plan(L.ELASTIC2D, vp, rho, DH, DT, NT, vs=vs, sx=[1000], sz=[5], rx=rx, rz=np.full(NX, 5),
+                wavelet=w[None], nbc=NBC, fd_order=ORDER, source_type=L.SRC_FZ, use_graph=True)
+    n_pad = plan.padded_cells
+    plan.advance(200)
+    plan.sync()
+    plan.reset()
+    ms = plan.timed_advance(NT)
+    lt = plan.launch_times(50)
+    plan.close()
+    return {"workload": f"elastic2d {NX}x{NZ}, fd8, nbc {NBC}, {NT} steps", "padded_cells": n_pad,
+            "cell_updates_per_s": n_pad * NT / (ms * 1e-3), "ms_per_step": ms / NT,
+            "launch_ms": {"velocity_sweep": lt[0], "stress_sweep": lt[1]},
+            "stress_sweep_gbs": BYTES_STR * n_pad / (lt[1] * 1e-3) / 1e9}
 
 
 def probe(args):
@@ -368,11 +533,15 @@ def probe3d(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=NT)
-    ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--kernel-steps", type=int, default=200, help="steps for per-kernel event timing")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--steps", type=int, default=NT3)
+    ap.add_argument("--warmup", type=int, default=32)
+    ap.add_argument("--launch-reps", type=int, default=10, help="back-to-back launches per per-launch timing")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc traffic children")
+    ap.add_argument("--no-extras", action="store_true", help="skip the FP16/BF16 and elastic2d extra lines")
+    ap.add_argument("--traffic-child", default="", help=argparse.SUPPRESS)
+    ap.add_argument("--traffic-storage", default="f32", help=argparse.SUPPRESS)
     ap.add_argument("--probe", action="store_true", help="HBM-bound 2D probe instead of the headline line")
     ap.add_argument("--probe-n", type=int, default=8192)
     ap.add_argument("--probe3d", action="store_true", help="acoustic3d 512^3 probe (f32/f16/bf16)")
@@ -398,6 +567,9 @@ def main():
               "across GPUs).  Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    if args.traffic_child:
+        traffic_child(args)
+        return
     if args.probe:
         probe(args)
         return
@@ -439,23 +611,22 @@ def main():
         print(json.dumps({"stream_copy": {"bytes": 1 << 30, "ms": ms, "gbs": gbs}}), flush=True)
         return
 
+    headline(args)
+
+
+def headline(args):
     import numpy as np
 
-    import fomo
-    import fomo._lib as L
-    from fomo.api import Plan
-    from fomo.models import layered_2d, model_digest
+    from fomo.models import model_digest
 
-    log("building the seeded 2000x800 layered model")
-    vp, vs, rho = layered_2d(NX, NZ, seed=SEED)
-    rx = np.arange(1, NX + 1)
-    rz = np.full(NX, 5)
-    w = fomo.ricker_wavelet(F0, DT, NT)
+    vp, rho = model3d()
+    traffic = None
+    if not args.no_traffic:
+        traffic = measure_traffic(vp, rho)  # before this process touches the GPU
     log("creating the plan (setup, upload, graph capture)")
-    plan = Plan(L.ELASTIC2D, vp, rho, DH, DT, NT, vs=vs, sx=[1000], sz=[5], rx=rx, rz=rz, wavelet=w[None],
-                nbc=NBC, fd_order=ORDER, source_type=L.SRC_FZ, use_graph=True)
+    plan = plan3d(vp, rho, NT3)
     n_pad = plan.padded_cells
-    pnx, pnz = plan.padded[0]
+    pnx, pny, pnz = plan.padded[0]
     log(f"warmup {args.warmup} steps")
     plan.advance(args.warmup)
     plan.sync()
@@ -463,21 +634,41 @@ def main():
     plan.sync()
     log(f"timing {args.steps} steps")
     ms = plan.timed_advance(args.steps)
-    seis = plan.seis(2)
-    finite = bool(np.all(np.isfinite(seis[1])))
+    seis = plan.seis(3)
+    finite = bool(all(np.all(np.isfinite(x)) for x in seis))
+    peak_vz = float(np.abs(seis[2]).max())
     log(f"{args.steps} steps in {ms:.2f} ms; seismograms finite: {finite}")
-    plan.reset()
-    pre = max(0, min(args.warmup, NT - args.kernel_steps))
-    plan.advance(pre)  # a live wavefield for the per-kernel timing
-    kt = plan.kernel_times(args.kernel_steps)
+    # per-launch times on a live wavefield (the launches are idempotent: the state does not move)
+    info = plan.launches()
+    lt = plan.launch_times(args.launch_reps)
     plan.close()
-    t_vel, t_str, t_rec = kt
-    dom_name, dom_ms, dom_bytes = ("stress_sweep", t_str, BYTES_STR) if t_str >= t_vel else \
-        ("velocity_sweep", t_vel, BYTES_VEL)
-    achieved = dom_bytes * n_pad / (dom_ms * 1e-3) / 1e9
-    log(f"kernel ms: velocity {t_vel:.4f}, stress {t_str:.4f}, gather {t_rec:.4f}")
-
+    launches = []
+    for k, (li, t) in enumerate(zip(info, lt)):
+        gbs = li["bytes_per_cell"] * li["cells"] / (t * 1e-3) / 1e9
+        launches.append({**li, "ms": t, "algorithmic_gbs": gbs})
+        log(f"launch {k} {li['name']}: {t:.4f} ms, {gbs:.0f} GB/s algorithmic")
+    dom = max(range(len(lt)), key=lambda k: lt[k])
+    d = launches[dom]
+    alg_bytes = d["bytes_per_cell"] * d["cells"]
+    achieved = alg_bytes / (d["ms"] * 1e-3) / 1e9
+    bytes_step = 64 * n_pad
     value = n_pad * args.steps / (ms * 1e-3)
+    roof = {"bound": "hbm", "kernel": d["name"], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes_per_launch": alg_bytes,
+            "bytes_per_cell": d["bytes_per_cell"], "cells_per_launch": d["cells"], "launch_ms": d["ms"]}
+    if traffic:
+        t = traffic["launches"][dom]
+        roof["traffic"] = t["bytes"]
+        roof["traffic_detail"] = {"read_bytes": t["read_bytes"], "written_bytes": t["written_bytes"],
+                                  "fetch_kib": t["fetch_kib"], "write_kib": t["write_kib"],
+                                  "fetch_scale": traffic["fetch_scale"], "write_scale": traffic["write_scale"],
+                                  "copy_fetch_kib": traffic["copy_fetch_kib"],
+                                  "copy_write_kib": traffic["copy_write_kib"], "copy_bytes": COPY_BYTES,
+                                  "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in child runs of this "
+                                            "script, scaled by the same counters on a 1 GiB streaming copy",
+                                  "traffic_over_algorithmic": t["bytes"] / alg_bytes}
+        for l, tr in zip(launches, traffic["launches"]):
+            l["traffic_bytes"] = tr["bytes"]
     out = {
         "metric": METRIC,
         "value": value,
@@ -492,32 +683,39 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"elastic2d {NX}x{NZ} interior, padded {pnx}x{pnz}, fd{ORDER}, "
-                        f"nbc {NBC}, dh {DH} m, dt {DT * 1e3} ms, vertical-force Ricker {F0} Hz, {NX} receivers",
+            "workload": f"acoustic3d {N3}^3 interior, padded {pnx}x{pny}x{pnz}, fd8, nbc {NBC3}, dh {DH3} m, "
+                        f"dt {DT3 * 1e3} ms, Ricker {F03} Hz at the top centre, 256x256 surface receivers "
+                        f"(vx, vy, vz), FP32 storage",
             "padded_cells": n_pad,
-            "interior_cell_updates_per_s": NX * NZ * args.steps / (ms * 1e-3),
-            "model": f"seeded layered (SplitMix64 seed {SEED}, digest {model_digest(vp, vs, rho)})",
-            "hbm_gbs_algorithmic_per_step": (BYTES_VEL + BYTES_STR) * n_pad / (ms * 1e-3 / args.steps) / 1e9,
-            "kernel_ms": {"velocity_sweep": t_vel, "stress_sweep": t_str, "gather": t_rec},
+            "interior_cell_updates_per_s": N3 ** 3 * args.steps / (ms * 1e-3),
+            "model": f"seeded layered 3D with Gaussian perturbations (SplitMix64 seed {SEED}, digest "
+                     f"{model_digest(vp, rho)})",
+            "step_algorithmic_gbs": bytes_step / (ms * 1e-3 / args.steps) / 1e9,
+            "step_frac_of_peak": bytes_step / (ms * 1e-3 / args.steps) / 1e9 / HBM_PEAK_GBS,
+            "launches": launches,
             "outputs_finite": finite,
+            "peak_abs_vz": peak_vz,
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dom_name,
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": PMC_TRAFFIC[dom_name],
-            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, calibrated on a "
-                            "streaming copy; profiles/elastic2d_2000x800_fd8_pmc_traffic.md)",
-            "algorithmic_bytes_per_launch": dom_bytes * n_pad,
-            "bytes_per_cell": dom_bytes,
-        },
+        "roofline": roof,
     }
+    if not args.no_extras:
+        ex = {}
+        for st in ("f16", "bf16"):
+            log(f"extra: the same workload with {st} storage")
+            p16 = plan3d(vp, rho, NT3, storage=st)
+            p16.advance(16)
+            p16.sync()
+            p16.reset()
+            m16 = p16.timed_advance(args.steps)
+            p16.close()
+            ex[f"acoustic3d_{st}"] = {"ms_per_step": m16 / args.steps, "cell_updates_per_s": n_pad * args.steps /
+                                      (m16 * 1e-3), "speedup_vs_f32": ms / m16}
+        log("extra: elastic2d 2000x800")
+        ex["elastic2d_2000x800"] = elastic2d_line(args)
+        out["extras"] = ex
     if not args.no_cpu:
-        log("timing the CPU baseline (oracle)")
-        out["cpu_baseline"] = cpu_baseline(vp, vs, rho, rx, rz, args.cpu_seconds)
+        log("timing the CPU baseline (oracle acoustic3d at 512^3)")
+        out["cpu_baseline"] = cpu_baseline3d(vp, rho, args.cpu_seconds)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

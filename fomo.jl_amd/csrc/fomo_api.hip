@@ -514,6 +514,47 @@ extern "C" int fomo_plan_kernel_times(fomo_plan* p, int32_t n, double* out3) {
   return FOMO_OK;
 }
 
+extern "C" int fomo_plan_launch_count(const fomo_plan* p, int32_t* n) {
+  if (!p || !p->impl || !n) return fail(FOMO_EINVAL, "NULL argument");
+  *n = p->impl->n_launches();
+  return FOMO_OK;
+}
+
+extern "C" int fomo_plan_launch_info(const fomo_plan* p, int32_t k, const char** name, int64_t* cells,
+                                     int32_t* bytes_per_cell) {
+  if (!p || !p->impl) return fail(FOMO_EINVAL, "NULL plan");
+  if (k < 0 || k >= p->impl->n_launches()) return fail(FOMO_EINVAL, "launch index out of range");
+  const PlanBase::LaunchInfo li = p->impl->launch_info(k);
+  if (name) *name = li.name;
+  if (cells) *cells = li.cells;
+  if (bytes_per_cell) *bytes_per_cell = li.bytes_per_cell;
+  return FOMO_OK;
+}
+
+extern "C" int fomo_plan_launch_times(fomo_plan* p, int32_t n, double* ms) {
+  PLAN_OR_FAIL(p);
+  if (!ms) return fail(FOMO_EINVAL, "NULL argument");
+  if (n < 1) return fail(FOMO_EINVAL, "repeat count must be >= 1");
+  PlanBase* b = p->impl;
+  hipEvent_t ev[2];
+  for (int k = 0; k < 2; ++k) HIP_TRY(hipEventCreate(&ev[k]));
+  const int s = (int)(b->steps_done % 2);
+  int rc = FOMO_OK;
+  for (int k = 0; k < b->n_launches() && rc == FOMO_OK; ++k) {
+    hipError_t e = b->enqueue_launch(k, s);  // warm
+    if (e == hipSuccess) e = hipEventRecord(ev[0], b->stream);
+    for (int t = 0; t < n && e == hipSuccess; ++t) e = b->enqueue_launch(k, s);
+    if (e == hipSuccess) e = hipEventRecord(ev[1], b->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+    float t_ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t_ms, ev[0], ev[1]);
+    if (e != hipSuccess) rc = fail(FOMO_EHIP, std::string("launch timing: ") + hipGetErrorString(e));
+    ms[k] = t_ms / n;
+  }
+  for (int k = 0; k < 2; ++k) (void)hipEventDestroy(ev[k]);
+  return rc;
+}
+
 extern "C" int fomo_plan_sync(fomo_plan* p) {
   PLAN_OR_FAIL(p);
   HIP_TRY(hipStreamSynchronize(p->impl->stream));

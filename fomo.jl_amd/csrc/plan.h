@@ -57,6 +57,17 @@ struct PlanBase {
   virtual int read_field(int shot, int which, float* out) = 0;
   virtual int dims(int shot, int* nx, int* ny, int* nz, int* pad) const = 0;
   virtual int n_shots() const = 0;
+  // The kernel launches of one step, each enqueueable alone for per-kernel timing: launch k from
+  // state parity s (idempotent, like a repeated part), its name, the cells it updates and the
+  // algorithmic bytes per cell of the reference's per-phase traffic (16-bit storage: half).
+  struct LaunchInfo {
+    const char* name;
+    long long cells;
+    int bytes_per_cell;
+  };
+  virtual int n_launches() const = 0;
+  virtual LaunchInfo launch_info(int k) const = 0;
+  virtual hipError_t enqueue_launch(int k, int s) = 0;
 
   // Advance n steps (graph replays where they fit), then record the latest state's receivers when
   // the sweeps record them one step late.

@@ -56,6 +56,9 @@ struct Plan2D : PlanBase {
   int read_field(int shot, int which, float* out) override;
   int dims(int shot, int* nx, int* ny, int* nz, int* pad) const override;
   int n_shots() const override { return (int)shots.size(); }
+  int n_launches() const override { return d.eq == FOMO_SCALAR2D ? 1 : 2; }
+  LaunchInfo launch_info(int k) const override;
+  hipError_t enqueue_launch(int k, int s) override { return enqueue_part(k, s, 0); }
   int make_work(int cfg);
   // Decode the current field `which` of a shot into `scratch` (asynchronous on the stream).
   int decode_field(int shot, int which);

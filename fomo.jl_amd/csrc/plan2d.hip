@@ -135,6 +135,26 @@ hipError_t Plan2D::enqueue_part(int part, int s, int koff) {
   return launch_sweep2d(sweep_storage, ph, N, tile_cfg, a, stream);
 }
 
+PlanBase::LaunchInfo Plan2D::launch_info(int k) const {
+  long long cells = 0;
+  for (const auto& sh : shots) cells += (long long)sh.nx * sh.nz;
+  // words per cell read + written: acoustic velocity p,vx,vz,bx,bz -> vx,vz (7) and pressure
+  // vx,vz,p,kappa -> p (5); elastic velocity 9, stress 11; scalar u^n,u^(n-1),c2 -> u^(n+1) (4)
+  int words = 0;
+  const char* name = "";
+  if (d.eq == FOMO_ACOUSTIC2D) {
+    words = k == 0 ? 7 : 5;
+    name = k == 0 ? "acoustic2d velocity sweep" : "acoustic2d pressure sweep";
+  } else if (d.eq == FOMO_ELASTIC2D) {
+    words = k == 0 ? 9 : 11;
+    name = k == 0 ? "elastic2d velocity sweep" : "elastic2d stress sweep";
+  } else {
+    words = 4;
+    name = "scalar2d sweep";
+  }
+  return {name, cells, words * elem};
+}
+
 int Plan2D::reset() {
   for (auto& sh : shots) {
     const size_t n = (size_t)sh.nx * sh.nz;

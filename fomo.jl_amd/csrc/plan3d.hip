@@ -87,9 +87,9 @@ struct Plan3D : PlanBase {
     a.nt = d.nt;
     Sweep3DArgs f = a;
     a.work = d_work[0];
-    a.n_work = n_work[0];
+    a.n_work = only == 1 ? 0 : n_work[0];
     f.work = d_work[1];
-    f.n_work = n_work[1];
+    f.n_work = only == 0 ? 0 : n_work[1];
     return launch_sweep3d(d.eq, storage, part, N, a, f, stream);
   }
 
@@ -158,6 +158,27 @@ struct Plan3D : PlanBase {
     return FOMO_OK;
   }
   int n_shots() const override { return 1; }
+  // Launches of a step: velocity core, velocity frame, pressure/stress core, pressure/stress frame.
+  int only = -1;  // enqueue_part launches only the core (0) or the frame (1) items when >= 0
+  long long cells_of[2] = {0, 0};
+  int n_launches() const override { return 4; }
+  LaunchInfo launch_info(int k) const override {
+    static const char* names[2][4] = {
+        {"acoustic3d velocity core", "acoustic3d velocity frame", "acoustic3d pressure core",
+         "acoustic3d pressure frame"},
+        {"elastic3d velocity core", "elastic3d velocity frame", "elastic3d stress core", "elastic3d stress frame"}};
+    const bool ac = d.eq == FOMO_ACOUSTIC3D;
+    // words per cell read + written: acoustic velocity p,vx,vy,vz,bx,by,bz -> vx,vy,vz (10),
+    // pressure vx,vy,vz,p,kappa -> p (6); elastic velocity 15, stress 20
+    const int words = ac ? (k < 2 ? 10 : 6) : (k < 2 ? 15 : 20);
+    return {names[ac ? 0 : 1][k], cells_of[k & 1], words * elem};
+  }
+  hipError_t enqueue_launch(int k, int s) override {
+    only = k & 1;
+    const hipError_t e = enqueue_part(k >> 1, s, 0);
+    only = -1;
+    return e;
+  }
 };
 
 // Work lists over the cells 1..n-2 of each axis (the outer ring is never written).  The x-y
@@ -525,6 +546,8 @@ int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, con
   }
   const int kseg = d->tile_cfg > 0 ? 16 << (d->tile_cfg - 1) : 32;
   if ((rc = make_work3d(p, kseg, elastic ? kRowsEl3 : kRowsAc3, elastic ? kRowsEl3Core : kRowsAc3Core))) return bail(rc);
+  p->cells_of[0] = (long long)p->nxi * p->nyi * p->nzi;
+  p->cells_of[1] = (long long)p->shot.nx * p->shot.ny * p->shot.nz - p->cells_of[0];
   if ((rc = p->reset())) return bail(rc);
   if (d->use_graph && (rc = p->build_graph())) return bail(rc);
   e = hipStreamSynchronize(p->stream);

@@ -523,6 +523,25 @@ class Plan:
         _lib.check(_lib.lib().fomo_plan_kernel_times(self._h, int(n), out))
         return out[0], out[1], out[2]
 
+    def launches(self) -> list[dict]:
+        """The kernel launches of one step: name, cells updated, algorithmic bytes per cell."""
+        L = _lib.lib()
+        n = C.c_int32(0)
+        _lib.check(L.fomo_plan_launch_count(self._h, C.byref(n)))
+        out = []
+        for k in range(n.value):
+            name, cells, bpc = C.c_char_p(), C.c_int64(0), C.c_int32(0)
+            _lib.check(L.fomo_plan_launch_info(self._h, k, C.byref(name), C.byref(cells), C.byref(bpc)))
+            out.append({"name": name.value.decode(), "cells": cells.value, "bytes_per_cell": bpc.value})
+        return out
+
+    def launch_times(self, n: int) -> list[float]:
+        """Average ms per launch of each launch of a step, each repeated n times back to back."""
+        k = len(self.launches())
+        ms = (C.c_double * k)()
+        _lib.check(_lib.lib().fomo_plan_launch_times(self._h, int(n), ms))
+        return list(ms)
+
     def seis(self, nslots: int = 3, shot: int = 0):
         outs = [np.zeros((self.n_rec[shot], self.nt), np.float32, order="F") for _ in range(nslots)]
         ptrs = [o.ctypes.data for o in outs] + [None] * (4 - nslots)

@@ -230,6 +230,16 @@ int fomo_plan_sync(fomo_plan *p);
  * gather, step-counter updates, gaps between launches), to out3.  The state advances by n steps.
  * Blocks. */
 int fomo_plan_kernel_times(fomo_plan *p, int32_t n, double *out3);
+/* Per-launch timing (a measurement helper with no counterpart in the reference): the kernel
+ * launches of one step (2D: velocity sweep, pressure/stress sweep; scalar2d: one sweep; 3D:
+ * velocity core, velocity frame, pressure/stress core, pressure/stress frame), with each one's
+ * name, the cells it updates and the algorithmic bytes per cell of its phase (the reference's
+ * per-phase reads and writes; 16-bit storage halves them).  fomo_plan_launch_times launches each
+ * one n times back to back from the current state (idempotent) between hipEvents on the plan's
+ * stream and writes the average milliseconds per launch, ms[k] for k < count.  Blocks. */
+int fomo_plan_launch_count(const fomo_plan *p, int32_t *n);
+int fomo_plan_launch_info(const fomo_plan *p, int32_t k, const char **name, int64_t *cells, int32_t *bytes_per_cell);
+int fomo_plan_launch_times(fomo_plan *p, int32_t n, double *ms);
 /* Steps taken since the last reset. */
 int fomo_plan_steps_done(const fomo_plan *p, int32_t *out);
 /* Copy seismograms to host (n_rec x nt each; NULL skips).  Slots: acoustic {p, vx, vz},
