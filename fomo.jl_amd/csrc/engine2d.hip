@@ -434,7 +434,7 @@ __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, 
       }
     }
 #pragma unroll
-    for (int f = 0; f < NF; ++f) Cvt<T>::st(V.out[f], c, v[f], V.oenc[f]);
+    for (int f = 0; f < NF; ++f) st_field<T>(V.out[f], c, v[f], V.oenc[f], S.ovf);
   }
 }
 
@@ -566,7 +566,7 @@ __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D&
         const unsigned c = (unsigned)(i + j * nx);
         if (j >= src_lo && j <= src_hi) inject<NF>(S, mask, (int)c, it, A.nt, v);
 #pragma unroll
-        for (int f = 0; f < NF; ++f) Cvt<T>::st(V.out[f], c, v[f], V.oenc[f]);
+        for (int f = 0; f < NF; ++f) st_field<T>(V.out[f], c, v[f], V.oenc[f], S.ovf);
         if constexpr (SF >= 0) {
           if (snap_k) G(snap_k)[(size_t)(i - pad) + (size_t)(j - pad) * nxi] = round_trip<T>(v[SF], V.oenc[SF], V.odec[SF]);
         }

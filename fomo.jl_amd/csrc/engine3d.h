@@ -28,6 +28,7 @@ struct Shot3D {
   int n_src;
   const long long* src_c;
   const float* wav;
+  unsigned* ovf;  // 16-bit storage: set to 1 when a stored value overflowed (see st_field)
 };
 
 // A column tile marched down z.  Output cells: x in [i0, i0+w), y in [j0, j0+hy), z in

@@ -389,7 +389,7 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
         for (int f = 0; f < NF; ++f) {
           float v = preN[r][f];
           if (P::inj(f) && src_here) inject3d(S, (size_t)co + (size_t)t * sz, it, A.nt, v);
-          Cvt<T>::st(plane_of<T>(S.fld[o][P::out_f(f)], t, sz), co, v, S.fenc[P::out_f(f)]);
+          st_field<T>(plane_of<T>(S.fld[o][P::out_f(f)], t, sz), co, v, S.fenc[P::out_f(f)], S.ovf);
         }
       }
     } else {
@@ -455,7 +455,7 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
             v = wt * v + ((1.0f - wt) * sum) / (float)nd;
           }
           // no injection here: sources are validated against the user's interior (core items)
-          Cvt<T>::st(plane_of<T>(S.fld[o][F], kk, sz), co, v, S.fenc[F]);
+          st_field<T>(plane_of<T>(S.fld[o][F], kk, sz), co, v, S.fenc[F], S.ovf);
         }
       }
     }

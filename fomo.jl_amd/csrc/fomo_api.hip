@@ -315,6 +315,19 @@ int PlanBase::build_graph() {
   return FOMO_OK;
 }
 
+int PlanBase::check_overflow() {
+  FOMO_HIP_TRY(hipStreamSynchronize(stream));
+  if (!ovf) return FOMO_OK;
+  unsigned h = 0;
+  FOMO_HIP_TRY(hipMemcpy(&h, ovf, sizeof(unsigned), hipMemcpyDeviceToHost));
+  if (h)
+    return set_error(FOMO_EOVERFLOW,
+                     "16-bit storage overflow: a wavefield value exceeded the storage format's range since the last "
+                     "reset (stored saturated); the results are invalid.  Use Float32 storage or a smaller source "
+                     "amplitude");
+  return FOMO_OK;
+}
+
 hipError_t PlanBase::run_steps(int m, hipEvent_t* ev) {
   hipError_t e = hipSuccess;
   const bool per_step_gather = !gather_in_sweep();
@@ -478,7 +491,7 @@ extern "C" int fomo_plan_timed_advance(fomo_plan* p, int32_t n, double* ms) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (ms) *ms = t;
-  return rc;
+  return rc ? rc : p->impl->check_overflow();
 }
 
 extern "C" int fomo_plan_kernel_times(fomo_plan* p, int32_t n, double* out3) {
@@ -512,6 +525,21 @@ extern "C" int fomo_plan_kernel_times(fomo_plan* p, int32_t n, double* out3) {
   if (rc) return rc;
   out3[2] = std::max(0.0, (double)ms / n - out3[0] - out3[1]);
   return FOMO_OK;
+}
+
+extern "C" int fomo_plan_read_snaps(fomo_plan* p, int32_t shot, int32_t* count, float* out) {
+  PLAN_OR_FAIL(p);
+  const float* src = nullptr;
+  int n = 0;
+  size_t cells = 0;
+  int rc = p->impl->snap_buffer(shot, &src, &n, &cells);
+  if (rc) return rc;
+  if (count) *count = n;
+  if (out && n) {
+    HIP_TRY(hipStreamSynchronize(p->impl->stream));
+    HIP_TRY(hipMemcpy(out, src, (size_t)n * cells * sizeof(float), hipMemcpyDeviceToHost));
+  }
+  return p->impl->check_overflow();
 }
 
 extern "C" int fomo_plan_launch_count(const fomo_plan* p, int32_t* n) {
@@ -557,8 +585,7 @@ extern "C" int fomo_plan_launch_times(fomo_plan* p, int32_t n, double* ms) {
 
 extern "C" int fomo_plan_sync(fomo_plan* p) {
   PLAN_OR_FAIL(p);
-  HIP_TRY(hipStreamSynchronize(p->impl->stream));
-  return FOMO_OK;
+  return p->impl->check_overflow();
 }
 
 extern "C" int fomo_plan_steps_done(const fomo_plan* p, int32_t* out) {
@@ -576,7 +603,8 @@ extern "C" int fomo_plan_n_shots(const fomo_plan* p, int32_t* out) {
 extern "C" int fomo_plan_read_shot_seis(fomo_plan* p, int32_t shot, float* s0, float* s1, float* s2, float* s3) {
   PLAN_OR_FAIL(p);
   float* dst[4] = {s0, s1, s2, s3};
-  return p->impl->read_seis(shot, dst);
+  const int rc = p->impl->read_seis(shot, dst);
+  return rc ? rc : p->impl->check_overflow();
 }
 
 extern "C" int fomo_plan_read_seis(fomo_plan* p, float* s0, float* s1, float* s2, float* s3) {
@@ -586,7 +614,8 @@ extern "C" int fomo_plan_read_seis(fomo_plan* p, float* s0, float* s1, float* s2
 extern "C" int fomo_plan_read_shot_field(fomo_plan* p, int32_t shot, int32_t which, float* out) {
   PLAN_OR_FAIL(p);
   if (!out) return fail(FOMO_EINVAL, "NULL argument");
-  return p->impl->read_field(shot, which, out);
+  const int rc = p->impl->read_field(shot, which, out);
+  return rc ? rc : p->impl->check_overflow();
 }
 
 extern "C" int fomo_plan_read_field(fomo_plan* p, int32_t which, float* out) {

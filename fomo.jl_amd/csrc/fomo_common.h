@@ -66,6 +66,7 @@ struct Shot2D {
   float* snap;
   int snap_every, snap_n;
   int nbuf;             // 2: ping-pong fields; 3: scalar u^{k+1}, u^k, u^{k-1} rotation
+  unsigned* ovf;        // 16-bit storage: set to 1 when a stored value overflowed (see st_field)
 };
 
 // One unit of sweep work.  kind 0: a frame tile (at most kTileW x kFrameRows cells of the

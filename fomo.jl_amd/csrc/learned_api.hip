@@ -155,8 +155,9 @@ extern "C" int fomo_learned_train_step(fomo_plan* student, fomo_plan* teacher, i
   Plan2D* t = acoustic_plan(teacher);
   if (!q || !q->training) return set_error(FOMO_EINVAL, "student plan is not in training mode");
   if (!t || t->shots.size() != 1) return set_error(FOMO_EINVAL, "teacher must be a single-shot acoustic2d plan");
-  if (q->storage != FOMO_STORE_F32 || t->storage != FOMO_STORE_F32)
-    return set_error(FOMO_EINVAL, "training needs Float32 storage in the student and teacher plans");
+  if (q->sweep_storage != FOMO_STORE_F32 || t->sweep_storage != FOMO_STORE_F32)
+    return set_error(FOMO_EINVAL,
+                     "training needs Float32 storage of fields and materials in the student and teacher plans");
   const Shot2D& S = q->shots[0];
   const Shot2D& T = t->shots[0];
   if (off_x < 0 || off_z < 0 || S.nx + off_x > T.nx || S.nz + off_z > T.nz)
@@ -199,6 +200,10 @@ extern "C" int fomo_learned_train_step(fomo_plan* student, fomo_plan* teacher, i
   a.inv_scale_v = 1.0f / scale_v;
   a.partial = q->partial;
   FOMO_HIP_TRY(launch_mlp_train(a, q->train_blocks, q->stream));
+  // The teacher's next steps overwrite the buffer this launch reads (T.fld[to] is the input of
+  // its step after next): its stream waits for the training launch.
+  FOMO_HIP_TRY(hipEventRecord(ev, q->stream));
+  FOMO_HIP_TRY(hipStreamWaitEvent(t->stream, ev, 0));
   (void)hipEventDestroy(ev);
   return FOMO_OK;
 }

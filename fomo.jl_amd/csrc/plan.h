@@ -34,6 +34,7 @@ struct PlanBase {
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
   int* step = nullptr;
+  unsigned* ovf = nullptr;  // 16-bit storage overflow flag written by the sweeps (st_field)
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   long steps_done = 0;
@@ -55,6 +56,8 @@ struct PlanBase {
   // and the shot's receiver count; fails for a slot the plan does not record.
   virtual int seis_device(int shot, int slot, const float** ptr, int* n_rec) const = 0;
   virtual int read_field(int shot, int which, float* out) = 0;
+  // Snapshot buffer of a shot (count x cells of one snapshot, Float32); *count = 0 without one.
+  virtual int snap_buffer(int shot, const float** ptr, int* count, size_t* cells) const = 0;
   virtual int dims(int shot, int* nx, int* ny, int* nz, int* pad) const = 0;
   virtual int n_shots() const = 0;
   // The kernel launches of one step, each enqueueable alone for per-kernel timing: launch k from
@@ -73,6 +76,8 @@ struct PlanBase {
   // the sweeps record them one step late.
   int advance(int n);
   int build_graph();
+  // Waits for the stream; FOMO_EOVERFLOW when a 16-bit store overflowed since the last reset.
+  int check_overflow();
   template <class T>
   int alloc(T** ptr, size_t count) {
     void* v = nullptr;

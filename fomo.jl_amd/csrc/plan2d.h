@@ -54,6 +54,13 @@ struct Plan2D : PlanBase {
   int read_seis(int shot, float* const* dst) override;
   int seis_device(int shot, int slot, const float** ptr, int* n_rec) const override;
   int read_field(int shot, int which, float* out) override;
+  int snap_buffer(int shot, const float** ptr, int* count, size_t* cells) const override {
+    if (shot < 0 || shot >= (int)shots.size()) return set_error(FOMO_EINVAL, "shot index out of range");
+    *ptr = shots[shot].snap;
+    *count = shots[shot].snap ? shots[shot].snap_n : 0;
+    *cells = (size_t)meta[shot].nxi * meta[shot].nzi;
+    return FOMO_OK;
+  }
   int dims(int shot, int* nx, int* ny, int* nz, int* pad) const override;
   int n_shots() const override { return (int)shots.size(); }
   int n_launches() const override { return d.eq == FOMO_SCALAR2D ? 1 : 2; }

@@ -164,7 +164,13 @@ int Plan2D::reset() {
   }
   for (int k = 0; k < nslots; ++k)
     if (seis_total) FOMO_HIP_TRY(hipMemsetAsync(seis[k], 0, (size_t)seis_total * sizeof(float), stream));
+  // snapshots written before the reset (autotuning, an earlier run) must not survive it
+  for (size_t k = 0; k < shots.size(); ++k)
+    if (shots[k].snap)
+      FOMO_HIP_TRY(hipMemsetAsync(shots[k].snap, 0,
+                                  (size_t)shots[k].snap_n * meta[k].nxi * meta[k].nzi * sizeof(float), stream));
   FOMO_HIP_TRY(hipMemsetAsync(step, 0, sizeof(int), stream));
+  if (ovf) FOMO_HIP_TRY(hipMemsetAsync(ovf, 0, sizeof(unsigned), stream));
   steps_done = 0;
   return FOMO_OK;
 }
@@ -239,13 +245,13 @@ int Plan2D::dims(int shot, int* nx, int* ny, int* nz, int* pad) const {
 }
 
 // Times a few direct-launch steps per tile shape and keeps the fastest.  The choice is cached
-// per (equation, order, shot count, padded size) for the life of the process; FOMO_TILE_CFG
-// forces a shape.
+// per (equation, order, sweep storage, boundary, learned, shot count, padded size) for the life of
+// the process; FOMO_TILE_CFG forces a shape.
 int Plan2D::autotune() {
-  static std::map<std::tuple<int, int, int, long long>, int> cache;
+  static std::map<std::tuple<int, int, int, int, int, int, long long>, int> cache;
   long long cells = 0;
   for (auto& sh : shots) cells += (long long)sh.nx * sh.nz;
-  const auto key = std::make_tuple(d.eq, N, (int)shots.size(), cells);
+  const auto key = std::make_tuple(d.eq, N, sweep_storage, d.boundary, learned ? 1 : 0, (int)shots.size(), cells);
   if (const char* env = std::getenv("FOMO_TILE_CFG")) {
     const int c = std::atoi(env);
     if (c >= 0 && c < kNumTileCfgs) {
@@ -359,9 +365,11 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
   long long seis_off = 0;
   std::vector<int> rs, rcell, rstride;
   std::vector<long long> rout;
+  if ((rc = p->alloc(&p->ovf, 1))) return bail(rc);
   for (int s = 0; s < n_shots; ++s) {
     const fomo_shot& S = sh[s];
     Shot2D x{};
+    x.ovf = p->ovf;
     ShotMeta m{};
     m.nxi = S.nx;
     m.nzi = d->nz;

@@ -100,6 +100,8 @@ struct Plan3D : PlanBase {
     const size_t ns = (size_t)d.n_rec * d.nt;
     for (int k = 0; k < nslots; ++k)
       if (ns) FOMO_HIP_TRY(hipMemsetAsync(seis[k], 0, ns * sizeof(float), stream));
+    if (snap) FOMO_HIP_TRY(hipMemsetAsync(snap, 0, (size_t)snap_n * snap_cells * sizeof(float), stream));
+    if (ovf) FOMO_HIP_TRY(hipMemsetAsync(ovf, 0, sizeof(unsigned), stream));
     FOMO_HIP_TRY(hipMemsetAsync(step, 0, sizeof(int), stream));
     steps_done = 0;
     return FOMO_OK;
@@ -158,6 +160,13 @@ struct Plan3D : PlanBase {
     return FOMO_OK;
   }
   int n_shots() const override { return 1; }
+  int snap_buffer(int sh, const float** ptr, int* count, size_t* cells) const override {
+    if (sh != 0) return set_error(FOMO_EINVAL, "shot index out of range");
+    *ptr = snap;
+    *count = snap ? snap_n : 0;
+    *cells = snap_cells;
+    return FOMO_OK;
+  }
   // Launches of a step: velocity core, velocity frame, pressure/stress core, pressure/stress frame.
   int only = -1;  // enqueue_part launches only the core (0) or the frame (1) items when >= 0
   long long cells_of[2] = {0, 0};
@@ -535,7 +544,8 @@ int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, con
     for (int k = 0; k < p->nslots; ++k)
       if ((rc = p->alloc(&p->seis[k], (size_t)d->n_rec * d->nt))) return bail(rc);
   }
-  if ((rc = p->alloc(&p->step, 1))) return bail(rc);
+  if ((rc = p->alloc(&p->step, 1)) || (rc = p->alloc(&p->ovf, 1))) return bail(rc);
+  p->shot.ovf = p->ovf;
   if (d->snap_interval > 0 && d->nt / d->snap_interval > 0) {
     p->snap_plane = d->snap_plane;
     p->snap_idx = snap_idx1 - 1 + pad;
@@ -589,6 +599,7 @@ static int run3d(const fomo_desc* d, const float* vp, const float* vs, const flo
     if (ce != hipSuccess) return finish(set_error(FOMO_EHIP, std::string("snapshot download: ") + hipGetErrorString(ce)));
   }
   rc = p->read_seis(0, seis);
+  if (!rc) rc = p->check_overflow();
   return finish(rc);
 }
 

@@ -6,6 +6,8 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "fomo_common.h"
 
 namespace fomo {
@@ -76,6 +78,27 @@ struct Cvt<__hip_bfloat16> {
     gput<unsigned short, NT>(p, c, __bfloat16_as_ushort(__float2bfloat16(v * enc)));
   }
 };
+
+// Store of a wavefield value.  Float32: as is.  16-bit: as Cvt<T>::st, except that a value the
+// format cannot hold (the rounded result is an infinity or a NaN: overflow of v * enc) is stored
+// as the largest finite value of its sign and *flag is set to 1, which the host reports as a
+// storage-overflow error.  Runs without overflow store exactly the bits of Cvt<T>::st.
+template <class T, class I>
+__device__ __forceinline__ void st_field(void* p, I c, float v, float enc, unsigned* flag) {
+  if constexpr (std::is_same<T, float>::value) {
+    (void)flag;
+    gput<float>(p, c, v);
+  } else {
+    constexpr bool kHalf = std::is_same<T, __half>::value;
+    unsigned short b = kHalf ? __half_as_ushort(__float2half_rn(v * enc)) : __bfloat16_as_ushort(__float2bfloat16(v * enc));
+    constexpr unsigned short kExp = kHalf ? 0x7C00 : 0x7F80, kMaxFinite = kHalf ? 0x7BFF : 0x7F7F;
+    if ((b & kExp) == kExp) {
+      b = (unsigned short)((b & 0x8000) | kMaxFinite);
+      if (flag) G(flag)[0] = 1u;  // every writer stores the same value
+    }
+    gput<unsigned short>(p, c, b);
+  }
+}
 
 template <>
 __device__ __forceinline__ float round_trip<float>(float v, float, float) {

@@ -12,7 +12,7 @@ import threading
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "libfomo_hip.so"))
 
-FOMO_OK, FOMO_EINVAL, FOMO_EHIP, FOMO_ENOMEM, FOMO_EUNSUP = 0, 1, 2, 3, 4
+FOMO_OK, FOMO_EINVAL, FOMO_EHIP, FOMO_ENOMEM, FOMO_EUNSUP, FOMO_EOVERFLOW = 0, 1, 2, 3, 4, 5
 ACOUSTIC2D, ELASTIC2D, ACOUSTIC3D, ELASTIC3D, SCALAR2D = 1, 2, 3, 4, 5
 SRC_EXPLOSIVE, SRC_FZ = 0, 1
 STORE_F32, STORE_F16, STORE_BF16, STORE_F32_MAT16 = 0, 1, 2, 3
@@ -25,6 +25,10 @@ class ArgumentError(ValueError):
 
 class FomoError(RuntimeError):
     """HIP runtime / allocation failure inside the engine."""
+
+
+class StorageOverflowError(FomoError):
+    """16-bit storage: a wavefield value overflowed the storage format (FOMO_EOVERFLOW)."""
 
 
 class Desc(C.Structure):
@@ -89,6 +93,7 @@ SIGNATURES = {
     "fomo_plan_sync": (C.c_int, [_p]),
     "fomo_plan_kernel_times": (C.c_int, [_p, _i, C.POINTER(C.c_double)]),
     "fomo_plan_steps_done": (C.c_int, [_p, C.POINTER(_i)]),
+    "fomo_plan_read_snaps": (C.c_int, [_p, _i, C.POINTER(_i), _p]),
     "fomo_plan_launch_count": (C.c_int, [_p, C.POINTER(_i)]),
     "fomo_plan_launch_info": (C.c_int, [_p, _i, C.POINTER(C.c_char_p), C.POINTER(C.c_int64), C.POINTER(_i)]),
     "fomo_plan_launch_times": (C.c_int, [_p, _i, _p]),
@@ -132,4 +137,6 @@ def check(rc: int) -> None:
     msg = lib().fomo_last_error().decode(errors="replace")
     if rc == FOMO_EINVAL:
         raise ArgumentError(msg)
+    if rc == FOMO_EOVERFLOW:
+        raise StorageOverflowError(msg)
     raise FomoError(f"fomo error {rc}: {msg}")

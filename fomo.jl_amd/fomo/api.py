@@ -439,7 +439,7 @@ class Plan:
 
     def __init__(self, eq, vp, rho, dh, dt, nt, *, sx=None, sz=None, rx=None, rz=None, wavelet=None, vs=None,
                  nbc=50, fd_order=8, v_ref=None, source_type=0, use_graph=True, sy=None, ry=None, storage=0,
-                 record_p=False, tile_cfg=0, shots=None):
+                 record_p=False, tile_cfg=0, shots=None, snap_interval=0):
         vp, rho = _f32F(vp), _f32F(rho)
         self._keep = [vp, rho]
         if vs is not None:
@@ -459,7 +459,7 @@ class Plan:
             wm = np.asfortranarray(np.asarray(wavelet, np.float32).reshape(len(sx), nt))
             self._keep += [sx, sz, rx, rz, sy, ry, wm]
             self.n_rec = [len(rx)]
-            d = _desc(eq, nx, nz, nbc, fd_order, nt, dh, dt, v_ref, len(sx), len(rx), 0, use_graph,
+            d = _desc(eq, nx, nz, nbc, fd_order, nt, dh, dt, v_ref, len(sx), len(rx), snap_interval, use_graph,
                       source_type=source_type, ny=ny, storage=storage_code(storage))
             d.record_p = 1 if record_p else 0
             d.tile_cfg = tile_cfg
@@ -522,6 +522,20 @@ class Plan:
         out = (C.c_double * 3)()
         _lib.check(_lib.lib().fomo_plan_kernel_times(self._h, int(n), out))
         return out[0], out[1], out[2]
+
+    def snaps(self, shot: int = 0) -> np.ndarray:
+        """All snapshot slots of a 2D shot, (count, nx * nz) Float32 interior cells, x fastest;
+        slots of steps not yet taken since the last reset are zero."""
+        shp = self.padded[shot]
+        if len(shp) != 2:
+            raise ArgumentError("Plan.snaps reads 2D plans; 3D slices come from acoustic3d / elastic3d")
+        n = C.c_int32(0)
+        _lib.check(_lib.lib().fomo_plan_read_snaps(self._h, shot, C.byref(n), None))
+        cells = (shp[0] - 2 * self.pads[shot]) * (shp[1] - 2 * self.pads[shot])
+        out = np.zeros((n.value, cells), np.float32)
+        if n.value:
+            _lib.check(_lib.lib().fomo_plan_read_snaps(self._h, shot, C.byref(n), out.ctypes.data))
+        return out
 
     def launches(self) -> list[dict]:
         """The kernel launches of one step: name, cells updated, algorithmic bytes per cell."""

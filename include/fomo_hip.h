@@ -28,6 +28,9 @@ extern "C" {
 #define FOMO_EHIP 2    /* HIP runtime error */
 #define FOMO_ENOMEM 3  /* device or host allocation failed */
 #define FOMO_EUNSUP 4  /* combination not supported by this build */
+#define FOMO_EOVERFLOW 5 /* 16-bit storage: a wavefield value overflowed the storage format since the last
+                          * reset (stored saturated to the largest finite value); reported by the calls
+                          * that wait for the plan's stream (sync, timed advance, reads, one-shot runs) */
 
 /* ---- equation / option enums -------------------------------------------------------------- */
 #define FOMO_ACOUSTIC2D 1
@@ -258,6 +261,11 @@ int fomo_trace_misfit(fomo_plan *pa, int32_t shot_a, fomo_plan *pb, int32_t shot
  * vx,vz,txx,tzz,txz; acoustic3d p,vx,vy,vz) to host as Float32 (decoded from the storage
  * precision when reduced-precision storage is used). */
 int fomo_plan_read_field(fomo_plan *p, int32_t which, float *out);
+/* Snapshots of a plan created with d->snap_interval > 0 (2D single-shot and 3D plans): writes the
+ * number of snapshot slots to *count and, when out is not NULL, copies every slot (count x the
+ * interior cells of one snapshot, as the one-shot entry points return them).  Slots of steps not
+ * yet taken since the last reset are zero. */
+int fomo_plan_read_snaps(fomo_plan *p, int32_t shot, int32_t *count, float *out);
 /* Padded grid dims and pad width. */
 int fomo_plan_dims(const fomo_plan *p, int32_t *nx, int32_t *ny, int32_t *nz, int32_t *pad);
 int fomo_plan_destroy(fomo_plan *p);

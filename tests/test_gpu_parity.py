@@ -133,6 +133,34 @@ def test_snapshots_match_oracle(gpu, orc):
         assert np.array_equal(a, b)
 
 
+def test_snapshots_after_partial_advance(gpu, orc):
+    """A plan whose tile shape is autotuned (steps run before the reset that starts the user's
+    run): after a partial advance, the slots reached equal the oracle and the others are zero."""
+    import fomo
+    import fomo._lib as L
+    from fomo.api import Plan
+    nx, nz, nt, si = 72, 54, 100, 10
+    vp, vs, rho = small_model(nx, nz, seed=6)
+    sx, sz, rx, rz = geometry(nx, nz)
+    w = fomo.ricker_wavelet(15.0, 1e-3, nt)
+    ref = orc.acoustic2d(vp, rho, 10.0, 1e-3, nt, snap_interval=si, wavelet=np.stack([w, w]), sx=sx, sz=sz, rx=rx,
+                         rz=rz, nbc=8, fd_order=8)
+    p = Plan(L.ACOUSTIC2D, vp, rho, 10.0, 1e-3, nt, sx=sx, sz=sz, rx=rx, rz=rz, wavelet=np.stack([w, w]), nbc=8,
+             fd_order=8, tile_cfg=0, snap_interval=si)
+    try:
+        p.advance(35)
+        got = p.snaps()
+        assert got.shape == (nt // si, nx * nz)
+        for k in range(3):
+            assert np.array_equal(got[k].reshape(nz, nx).T, ref["snaps"][k])
+        assert not np.any(got[3:])
+        p.reset()
+        p.advance(9)
+        assert not np.any(p.snaps())
+    finally:
+        p.close()
+
+
 @pytest.mark.parametrize("si,nt", [(1, 37), (7, 100), (16, 64)])
 def test_snapshots_inside_graph_replays(gpu, orc, si, nt):
     """Snapshots are written by the sweeps on the device, also inside 16-step graph replays and

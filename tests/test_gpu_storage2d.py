@@ -214,3 +214,38 @@ def test_mat16_rejected_where_fields_and_materials_share_buffers(gpu):
     with pytest.raises(fomo.ArgumentError):
         fomo.acoustic3d(v3, np.full((n, n, n), 2000.0, np.float32), DH, DT, 20, F0, sx=[12], sy=[12], sz=[4],
                         rx=[5], ry=[5], rz=[3], nbc=8, storage="mat16")
+
+
+def test_fp16_overflow_is_reported(gpu):
+    """16-bit overflow is loud.  On the reference's absorption-experiment model, plain HABC at
+    nbc 100 grows without bound in the bottom frame after about 700 steps (the oracle shows the
+    same growth in FP32, where it stays finite for 1500 steps); with Float16 storage the growth
+    leaves the format's range.  The overflowing stores saturate and set the plan's flag, and the
+    next call that waits for the plan raises StorageOverflowError; a reset clears the flag, and
+    the one-shot API raises the same error.  Float32 storage runs the same case without error."""
+    import fomo
+    import fomo._lib as L
+    from fomo import learned
+    from fomo.api import Plan
+    vp, rho, sx, sz, rx, rz = learned.patch_geometry()
+    nt = 1500
+    w = fomo.ricker_wavelet(10.0, 1e-3, nt)
+    p = Plan(L.ACOUSTIC2D, vp, rho, 10.0, 1e-3, nt, sx=[sx], sz=[sz], rx=rx, rz=rz, wavelet=w[None], nbc=100,
+             fd_order=8, storage="f16")
+    try:
+        p.advance(300)
+        p.sync()  # early steps stay in range
+        p.advance(nt - 300)
+        with pytest.raises(L.StorageOverflowError, match="overflow"):
+            p.sync()
+        with pytest.raises(L.StorageOverflowError):
+            p.seis(3)
+        p.reset()
+        p.advance(300)
+        p.sync()
+    finally:
+        p.close()
+    with pytest.raises(L.StorageOverflowError):
+        fomo.acoustic2d(vp, rho, 10.0, 1e-3, nt, 10.0, sx=[sx], sz=[sz], rx=rx, rz=rz, nbc=100, storage="f16")
+    r = fomo.acoustic2d(vp, rho, 10.0, 1e-3, nt, 10.0, sx=[sx], sz=[sz], rx=rx, rz=rz, nbc=100)
+    assert np.all(np.isfinite(r.seis_p))
