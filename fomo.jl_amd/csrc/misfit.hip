@@ -8,11 +8,21 @@
 // chunks of each receiver in chunk order.  The result is a pure function of the traces.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "plan.h"
 
 namespace fomo {
+
+// Scratch of fomo_trace_misfit_noalloc: the partial sums (2 doubles per receiver and time chunk)
+// followed by the per-receiver results (2 doubles per receiver) of one call, at most
+// kScratchDoubles doubles (16 MiB).  Each GPU the library runs on holds its own copy of this
+// array until the process ends.  Host code never passes the array's name to a kernel: it uses
+// the device address hipGetSymbolAddress returns for the current device (scratch_on_device).
+constexpr size_t kScratchDoubles = (size_t)1 << 21;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
 namespace {
 
 constexpr int kRecPerBlock = 64;  // receivers of one block (one wave row)
@@ -62,13 +72,37 @@ __global__ void trace_final_kernel(const double* __restrict__ part, int n_rec, i
   out[2 * r + 1] = den;
 }
 
-}  // namespace
-}  // namespace fomo
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {nullptr};
 
-using namespace fomo;
+// Device address of `scratch` on the current device, looked up once per device.
+int scratch_on_device(double** out) {
+  int dev = 0;
+  FOMO_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return set_error(FOMO_EUNSUP, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return set_error(FOMO_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double*)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return FOMO_OK;
+}
 
-extern "C" int fomo_trace_misfit(fomo_plan* pa, int32_t shot_a, fomo_plan* pb, int32_t shot_b, int32_t slot,
-                                 int32_t t0, int32_t t1, double* num, double* den) {
+struct MisfitJob {
+  const float* da = nullptr;
+  const float* db = nullptr;
+  int na = 0, t0 = 0, t1 = 0, n_chunks = 0, chunk = 0;
+  size_t n_part = 0;  // doubles of partial sums (the results follow them)
+};
+
+// Validates the arguments and resolves the traces; job.na == 0 or t1 == t0 needs no kernel.
+int prepare_misfit(fomo_plan* pa, int32_t shot_a, fomo_plan* pb, int32_t shot_b, int32_t slot, int32_t t0,
+                   int32_t t1, double* num, double* den, MisfitJob& job) {
   if (!pa || !pa->impl) return set_error(FOMO_EINVAL, "plan is NULL");
   if (pb && !pb->impl) return set_error(FOMO_EINVAL, "second plan is NULL");
   if (!num || !den) return set_error(FOMO_EINVAL, "NULL output");
@@ -77,42 +111,66 @@ extern "C" int fomo_trace_misfit(fomo_plan* pa, int32_t shot_a, fomo_plan* pb, i
   const int nt = A->d.nt;
   if (B && B->d.nt != nt) return set_error(FOMO_EINVAL, "plans record different numbers of time samples");
   if (t0 < 0 || t1 > nt || t0 > t1) return set_error(FOMO_EINVAL, "time window outside [0, nt]");
-  const float* da = nullptr;
-  const float* db = nullptr;
   int na = 0, nb = 0;
-  int rc = A->seis_device(shot_a, slot, &da, &na);
+  int rc = A->seis_device(shot_a, slot, &job.da, &na);
   if (rc) return rc;
   if (B) {
-    rc = B->seis_device(shot_b, slot, &db, &nb);
+    rc = B->seis_device(shot_b, slot, &job.db, &nb);
     if (rc) return rc;
     if (nb != na) return set_error(FOMO_EINVAL, "shots have different receiver counts");
   }
+  job.na = na;
+  job.t0 = t0;
+  job.t1 = t1;
+  if (na == 0 || t1 == t0) return FOMO_OK;
+  const int len = t1 - t0;
+  int n_chunks = (len + kMinChunkLen - 1) / kMinChunkLen;
+  if (n_chunks > kMaxChunks) n_chunks = kMaxChunks;
+  job.chunk = (len + n_chunks - 1) / n_chunks;
+  job.n_chunks = (len + job.chunk - 1) / job.chunk;
+  job.n_part = (size_t)job.n_chunks * na * 2;
+  return FOMO_OK;
+}
+
+// Both passes on A's stream into work (partials) and work + n_part (results).
+hipError_t enqueue_misfit(const MisfitJob& j, double* work, hipStream_t s) {
+  hipLaunchKernelGGL(trace_partial_kernel, dim3((j.na + kRecPerBlock - 1) / kRecPerBlock, j.n_chunks),
+                     dim3(kRecPerBlock, kTRows), 0, s, j.da, j.db, j.na, j.t0, j.t1, j.chunk, work);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(trace_final_kernel, dim3((j.na + 255) / 256), dim3(256), 0, s, work, j.na, j.n_chunks,
+                     work + j.n_part);
+  return hipGetLastError();
+}
+
+}  // namespace
+}  // namespace fomo
+
+using namespace fomo;
+
+extern "C" int fomo_trace_misfit(fomo_plan* pa, int32_t shot_a, fomo_plan* pb, int32_t shot_b, int32_t slot,
+                                 int32_t t0, int32_t t1, double* num, double* den) {
+  MisfitJob j;
+  int rc = prepare_misfit(pa, shot_a, pb, shot_b, slot, t0, t1, num, den, j);
+  if (rc) return rc;
+  const int na = j.na;
   if (na == 0) return FOMO_OK;
   if (t1 == t0) {
     for (int r = 0; r < na; ++r) num[r] = den[r] = 0.0;
     return FOMO_OK;
   }
-  const int len = t1 - t0;
-  int n_chunks = (len + kMinChunkLen - 1) / kMinChunkLen;
-  if (n_chunks > kMaxChunks) n_chunks = kMaxChunks;
-  const int chunk = (len + n_chunks - 1) / n_chunks;
-  n_chunks = (len + chunk - 1) / chunk;
-  if (B) FOMO_HIP_TRY(hipStreamSynchronize(B->stream));  // b's traces are complete before a's stream reads them
+  PlanBase* A = pa->impl;
+  if (pb) FOMO_HIP_TRY(hipStreamSynchronize(pb->impl->stream));  // b's traces are complete before a's stream reads them
   double* work = nullptr;
-  const size_t n_part = (size_t)n_chunks * na * 2;
-  FOMO_HIP_TRY(hipMalloc(&work, (n_part + (size_t)na * 2) * sizeof(double)));
-  double* out = work + n_part;
+  FOMO_HIP_TRY(hipMalloc(&work, (j.n_part + (size_t)na * 2) * sizeof(double)));
+  double* out = work + j.n_part;
   auto fail = [&](hipError_t e, const char* what) {
     (void)hipStreamSynchronize(A->stream);
     (void)hipFree(work);
     return set_error(FOMO_EHIP, std::string(what) + ": " + hipGetErrorString(e));
   };
-  hipLaunchKernelGGL(trace_partial_kernel, dim3((na + kRecPerBlock - 1) / kRecPerBlock, n_chunks),
-                     dim3(kRecPerBlock, kTRows), 0, A->stream, da, db, na, t0, t1, chunk, work);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(e, "trace_partial_kernel");
-  hipLaunchKernelGGL(trace_final_kernel, dim3((na + 255) / 256), dim3(256), 0, A->stream, work, na, n_chunks, out);
-  if ((e = hipGetLastError()) != hipSuccess) return fail(e, "trace_final_kernel");
+  hipError_t e = enqueue_misfit(j, work, A->stream);
+  if (e != hipSuccess) return fail(e, "trace misfit kernels");
   double* h = nullptr;
   if ((e = hipHostMalloc((void**)&h, (size_t)na * 2 * sizeof(double))) != hipSuccess) return fail(e, "hipHostMalloc");
   e = hipMemcpyAsync(h, out, (size_t)na * 2 * sizeof(double), hipMemcpyDeviceToHost, A->stream);
@@ -125,5 +183,39 @@ extern "C" int fomo_trace_misfit(fomo_plan* pa, int32_t shot_a, fomo_plan* pb, i
   (void)hipHostFree(h);
   if (e != hipSuccess) return fail(e, "trace misfit download");
   FOMO_HIP_TRY(hipFree(work));
+  return FOMO_OK;
+}
+
+// The same sums with the library's static device scratch instead of allocations: no hipMalloc,
+// hipFree or pinned host buffer.  Calls that overlap on one GPU share its scratch and race; each
+// call overwrites every scratch element it reads (pass 1 writes every partial of every chunk and
+// receiver before pass 2 reads them), so nothing needs clearing between calls.
+extern "C" int fomo_trace_misfit_noalloc(fomo_plan* pa, int32_t shot_a, fomo_plan* pb, int32_t shot_b, int32_t slot,
+                                         int32_t t0, int32_t t1, double* num, double* den) {
+  MisfitJob j;
+  int rc = prepare_misfit(pa, shot_a, pb, shot_b, slot, t0, t1, num, den, j);
+  if (rc) return rc;
+  const int na = j.na;
+  if (na == 0) return FOMO_OK;
+  if (t1 == t0) {
+    for (int r = 0; r < na; ++r) num[r] = den[r] = 0.0;
+    return FOMO_OK;
+  }
+  const size_t need = j.n_part + (size_t)na * 2;
+  if (need > kScratchDoubles)
+    return set_error(FOMO_EINVAL, "trace misfit without allocation: (time chunks + 1) x 2 x receivers = " +
+                                      std::to_string(need) + " doubles exceeds the static scratch of " +
+                                      std::to_string(kScratchDoubles) + " (chunks = min(64, ceil(samples / 256)))");
+  PlanBase* A = pa->impl;
+  double* work = nullptr;
+  if ((rc = scratch_on_device(&work))) return rc;
+  if (pb) FOMO_HIP_TRY(hipStreamSynchronize(pb->impl->stream));
+  FOMO_HIP_TRY(enqueue_misfit(j, work, A->stream));
+  // the results are interleaved (num, den) per receiver: one copy each into the caller's arrays
+  FOMO_HIP_TRY(hipMemcpy2DAsync(num, sizeof(double), work + j.n_part, 2 * sizeof(double), sizeof(double), na,
+                                hipMemcpyDeviceToHost, A->stream));
+  FOMO_HIP_TRY(hipMemcpy2DAsync(den, sizeof(double), work + j.n_part + 1, 2 * sizeof(double), sizeof(double), na,
+                                hipMemcpyDeviceToHost, A->stream));
+  FOMO_HIP_TRY(hipStreamSynchronize(A->stream));
   return FOMO_OK;
 }

@@ -85,6 +85,7 @@ SIGNATURES = {
     "fomo_plan_n_shots": (C.c_int, [_p, C.POINTER(_i)]),
     "fomo_plan_read_shot_seis": (C.c_int, [_p, _i, _p, _p, _p, _p]),
     "fomo_trace_misfit": (C.c_int, [_p, _i, _p, _i, _i, _i, _i, _p, _p]),
+    "fomo_trace_misfit_noalloc": (C.c_int, [_p, _i, _p, _i, _i, _i, _i, _p, _p]),
     "fomo_plan_read_shot_field": (C.c_int, [_p, _i, _i, _p]),
     "fomo_plan_shot_dims": (C.c_int, [_p, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)]),
     "fomo_plan_reset": (C.c_int, [_p]),

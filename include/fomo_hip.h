@@ -257,6 +257,16 @@ int fomo_plan_read_seis(fomo_plan *p, float *s0, float *s1, float *s2, float *s3
  * comparisons.  Blocks. */
 int fomo_trace_misfit(fomo_plan *pa, int32_t shot_a, fomo_plan *pb, int32_t shot_b, int32_t slot, int32_t t0,
                       int32_t t1, double *num, double *den);
+/* fomo_trace_misfit without allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device or pinned memory (tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and bits as fomo_trace_misfit; the partial sums live in a static device
+ * array of 2^21 doubles (16 MiB, held by every GPU the library runs on until the process ends), so
+ * (chunks + 1) x 2 x n_rec must not exceed 2^21, with chunks = min(64, ceil((t1 - t0) / 256)) (for
+ * example 65,536 receivers over 4000 samples); larger calls fail with FOMO_EINVAL.  The results are
+ * copied straight into num and den.  Calls that overlap on one GPU share its array and race.
+ * Blocks. */
+int fomo_trace_misfit_noalloc(fomo_plan *pa, int32_t shot_a, fomo_plan *pb, int32_t shot_b, int32_t slot,
+                              int32_t t0, int32_t t1, double *num, double *den);
 /* Copy the current padded field number `which` (order: acoustic2d p,vx,vz; elastic2d
  * vx,vz,txx,tzz,txz; acoustic3d p,vx,vy,vz) to host as Float32 (decoded from the storage
  * precision when reduced-precision storage is used). */
