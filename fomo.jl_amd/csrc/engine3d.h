@@ -51,6 +51,16 @@ constexpr int kRowsAc3 = 16, kRowsEl3 = 8;
 // reads a quarter of the plane), elastic as its frame tiles.
 constexpr int kRowsAc3Core = 32, kRowsEl3Core = 8;
 
+// Fused acoustic3d interior tiles (engine3d_fused.hip): 64 x kFTY cells in x-y, 512 threads.
+constexpr int kFTY = 16;
+
+// Raw-add source injection into a cell of plane-step `it` (src/acquisition/inject_source_3d.jl:5-17).
+__device__ __forceinline__ void inject3d(const Shot3D& S, size_t c, int it, int nt, float& v) {
+  if (it >= nt) return;
+  for (int q = 0; q < S.n_src; ++q)
+    if ((size_t)G(S.src_c)[q] == c) v += G(S.wav)[(size_t)q + (size_t)S.n_src * (size_t)it];
+}
+
 // The shot descriptor travels by value in the kernel arguments (constant, so its pointers and
 // scales stay in scalar registers or are reloaded from the argument segment).
 struct Sweep3DArgs {
@@ -88,6 +98,9 @@ struct Gather3DArgs {
 hipError_t launch_sweep3d(int eq, int storage, int phase, int N, const Sweep3DArgs& core, const Sweep3DArgs& frame,
                           hipStream_t s);
 hipError_t launch_gather3d(int storage, const Gather3DArgs& a, hipStream_t s);
+// acoustic3d: the whole step (new velocities, then new pressure) of the interior tiles in a.work;
+// the frame velocity launch runs before it and the frame pressure launch after it.
+hipError_t launch_fused3d(int storage, int N, const Sweep3DArgs& a, hipStream_t s);
 hipError_t launch_decode3d(int storage, const void* src, float* dst, size_t n, float dec, hipStream_t s);
 
 }  // namespace fomo

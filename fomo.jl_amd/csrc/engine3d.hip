@@ -124,12 +124,6 @@ struct Lds3 {
   static constexpr int CORE = 2 * STG, SIZE = 2 * STG + 4 * SLOT;
 };
 
-__device__ __forceinline__ void inject3d(const Shot3D& S, size_t c, int it, int nt, float& v) {
-  if (it >= nt) return;
-  for (int q = 0; q < S.n_src; ++q)
-    if ((size_t)G(S.src_c)[q] == c) v += G(S.wav)[(size_t)q + (size_t)S.n_src * (size_t)it];
-}
-
 // LAG = true: frame march (one plane behind, ring tiles, Higdon blend, bounds-checked loads).
 // LAG = false: core march over the user's interior (no frame cell anywhere in the item): every
 // cell is updated and stored at once, and every stencil read lies inside the grid.

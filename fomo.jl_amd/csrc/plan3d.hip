@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <thread>
@@ -40,6 +41,11 @@ struct Plan3D : PlanBase {
   Shot3D shot{};
   Work3D* d_work[2] = {nullptr, nullptr};  // core items, frame items
   int n_work[2] = {0, 0};
+  // acoustic3d: the interior's two phases in one launch (engine3d_fused.hip) over these items,
+  // between the frame velocity and frame pressure launches; the core lists above are then unused
+  bool fused = false;
+  Work3D* d_fused = nullptr;
+  int n_fused = 0;
   long long* rec_c = nullptr;
   float* seis[4] = {nullptr};
   int nslots = 3;
@@ -86,6 +92,18 @@ struct Plan3D : PlanBase {
     a.koff = koff;
     a.nt = d.nt;
     Sweep3DArgs f = a;
+    if (fused) {
+      // part 0: frame velocities, then the interior's whole step; part 1: frame pressure
+      f.work = d_work[1];
+      f.n_work = only == 0 ? 0 : n_work[1];  // only: 0 the interior launch, 1 the frame launch
+      a.work = d_work[0];
+      a.n_work = 0;
+      hipError_t e = launch_sweep3d(d.eq, storage, part, N, a, f, stream);
+      if (e != hipSuccess || part != 0 || only == 1) return e;
+      a.work = d_fused;
+      a.n_work = n_fused;
+      return launch_fused3d(storage, N, a, stream);
+    }
     a.work = d_work[0];
     a.n_work = only == 1 ? 0 : n_work[0];
     f.work = d_work[1];
@@ -167,11 +185,18 @@ struct Plan3D : PlanBase {
     *cells = snap_cells;
     return FOMO_OK;
   }
-  // Launches of a step: velocity core, velocity frame, pressure/stress core, pressure/stress frame.
+  // Launches of a step: velocity core, velocity frame, pressure/stress core, pressure/stress frame;
+  // fused: velocity frame, interior step, pressure frame.
   int only = -1;  // enqueue_part launches only the core (0) or the frame (1) items when >= 0
   long long cells_of[2] = {0, 0};
-  int n_launches() const override { return 4; }
+  int n_launches() const override { return fused ? 3 : 4; }
   LaunchInfo launch_info(int k) const override {
+    if (fused) {
+      // the interior launch is credited with both phases' reads and writes (10 + 6 words per cell)
+      if (k == 0) return {"acoustic3d velocity frame", cells_of[1], 10 * elem};
+      if (k == 1) return {"acoustic3d interior step (velocity + pressure)", cells_of[0], 16 * elem};
+      return {"acoustic3d pressure frame", cells_of[1], 6 * elem};
+    }
     static const char* names[2][4] = {
         {"acoustic3d velocity core", "acoustic3d velocity frame", "acoustic3d pressure core",
          "acoustic3d pressure frame"},
@@ -183,8 +208,14 @@ struct Plan3D : PlanBase {
     return {names[ac ? 0 : 1][k], cells_of[k & 1], words * elem};
   }
   hipError_t enqueue_launch(int k, int s) override {
-    only = k & 1;
-    const hipError_t e = enqueue_part(k >> 1, s, 0);
+    hipError_t e;
+    if (fused) {
+      only = k == 1 ? 0 : 1;
+      e = enqueue_part(k == 2 ? 1 : 0, s, 0);
+    } else {
+      only = k & 1;
+      e = enqueue_part(k >> 1, s, 0);
+    }
     only = -1;
     return e;
   }
@@ -221,11 +252,15 @@ static int upload_work(Plan3D* p, int which, std::vector<Work3D>& items, int row
   for (int r = 0; r < 8; ++r)
     for (size_t q = 0; q < runs[r].size(); ++q) flat[q * 8 + r] = runs[r][q];
   if (flat.size() > (size_t)0x7fffffff) return set_error(FOMO_EUNSUP, "grid too large for one launch");
+  if (which == 2) {
+    p->n_fused = (int)flat.size();
+    return p->upload(&p->d_fused, flat.data(), flat.size());
+  }
   p->n_work[which] = (int)flat.size();
   return p->upload(&p->d_work[which], flat.data(), flat.size());
 }
 
-static int make_work3d(Plan3D* p, int kseg, int rows, int core_rows) {
+static int make_work3d(Plan3D* p, int kseg, int rows, int core_rows, int kseg_fused) {
   const int nx = p->shot.nx, ny = p->shot.ny, nz = p->shot.nz, pad = p->pad;
   std::vector<Work3D> core_fp, ring_fp;
   // rx / ry: the rectangle holds x / y frame cells (then the tile computes a ring in that direction)
@@ -268,6 +303,18 @@ static int make_work3d(Plan3D* p, int kseg, int rows, int core_rows) {
   segs(1, nz - 1, ring_fp, frame);
   int rc = upload_work(p, 0, core, core_rows);
   if (!rc) rc = upload_work(p, 1, frame, rows);
+  if (!rc && p->fused) {
+    // interior tiles of the fused launch: 64 x kFTY footprints over z-segments of about kseg_fused planes
+    std::vector<Work3D> fp, items;
+    for (int j = pad; j < ny - pad; j += kFTY)
+      for (int i = pad; i < nx - pad; i += kTileW)
+        fp.push_back(Work3D{1, i, j, 0, std::min(kTileW, nx - pad - i), std::min(kFTY, ny - pad - j), 0, 0});
+    const int keep = kseg;
+    kseg = kseg_fused;
+    segs(pad, nz - pad, fp, items);
+    kseg = keep;
+    rc = upload_work(p, 2, items, kFTY);
+  }
   return rc;
 }
 
@@ -555,7 +602,22 @@ int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, con
     if ((rc = p->alloc(&p->snap, p->snap_cells * p->snap_n))) return bail(rc);
   }
   const int kseg = d->tile_cfg > 0 ? 16 << (d->tile_cfg - 1) : 32;
-  if ((rc = make_work3d(p, kseg, elastic ? kRowsEl3 : kRowsAc3, elastic ? kRowsEl3Core : kRowsAc3Core))) return bail(rc);
+  // acoustic3d runs the interior's step in one launch unless FOMO_3D_UNFUSED is set (the two-sweep
+  // form, kept for comparison); its z-segments are longer (each pays 2N - 1 extra levels of vz)
+  p->fused = !elastic && !std::getenv("FOMO_3D_UNFUSED");
+  // Fused interior z-segments: one block per CU fits (1024 threads, 60 KB of LDS), so the columns
+  // are cut into the fewest segments that still give every CU a block; each segment pays 2N - 1
+  // extra levels of vz, and tiles that march together share their halos through L2.
+  // FOMO_FUSED_KSEG overrides the segment length.
+  int kseg_fused = d->tile_cfg > 0 ? 32 << (d->tile_cfg - 1) : 0;
+  if (const char* env = std::getenv("FOMO_FUSED_KSEG")) kseg_fused = std::max(8, std::atoi(env));
+  if (kseg_fused == 0) {
+    const long long cols = (long long)((p->nxi + kTileW - 1) / kTileW) * ((p->nyi + kFTY - 1) / kFTY);
+    const long long nseg = std::max(1LL, (256 + cols - 1) / std::max(1LL, cols));
+    kseg_fused = (int)std::max(8LL, (p->nzi + nseg - 1) / nseg);
+  }
+  if ((rc = make_work3d(p, kseg, elastic ? kRowsEl3 : kRowsAc3, elastic ? kRowsEl3Core : kRowsAc3Core, kseg_fused)))
+    return bail(rc);
   p->cells_of[0] = (long long)p->nxi * p->nyi * p->nzi;
   p->cells_of[1] = (long long)p->shot.nx * p->shot.ny * p->shot.nz - p->cells_of[0];
   if ((rc = p->reset())) return bail(rc);

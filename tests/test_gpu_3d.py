@@ -280,3 +280,37 @@ def test_elastic3d_reduced_precision_storage_error(gpu, orc, storage, tol, tol_v
     for key, (med, mx, glob, ew) in errs.items():
         t = tol_vy if key == "seis_vy" else tol
         assert mx < t and glob < t, (key, mx, glob)
+
+
+@pytest.mark.parametrize("order,storage", [(2, "f32"), (4, "f32"), (8, "f32"), (8, "f16"), (10, "bf16")])
+def test_acoustic3d_fused_interior_equals_two_sweeps(gpu, order, storage, monkeypatch):
+    """The one-launch interior step (velocity then pressure, new velocities kept on chip) gives the
+    bits of the two separate sweeps (FOMO_3D_UNFUSED), for every FD order's halo widths, for 16-bit
+    storage, and for interior sizes that leave partial tiles in x (50 = 64 - 14), y (37 = 2 x 16 +
+    5) and z (segments of 32 planes over 45); the launch list names the fused launch."""
+    import fomo
+    import fomo._lib as L
+    from fomo.api import Plan
+    nx, ny, nz, nt = 50, 37, 45, 24
+    vp, rho = model3d(nx, ny, nz, 12)
+    geo = geom3d(nx, ny, nz)
+    w = fomo.ricker_wavelet(15.0, 1e-3, nt)
+    sx, sy, sz, rx, ry, rz = geo
+    outs = []
+    for unfused in (False, True):
+        if unfused:
+            monkeypatch.setenv("FOMO_3D_UNFUSED", "1")
+        else:
+            monkeypatch.delenv("FOMO_3D_UNFUSED", raising=False)
+        p = Plan(L.ACOUSTIC3D, vp, rho, 10.0, 1e-3, nt, sx=sx, sy=sy, sz=sz, rx=rx, ry=ry, rz=rz,
+                 wavelet=np.stack([w] * len(sx)), nbc=6, fd_order=order, storage=storage, record_p=True, tile_cfg=1)
+        try:
+            names = [x["name"] for x in p.launches()]
+            assert any("interior step" in n for n in names) != unfused, names
+            p.advance(nt)
+            outs.append(p.seis(4) + [p.field(k) for k in range(4)])
+        finally:
+            p.close()
+    for x, y in zip(outs[0], outs[1]):
+        assert np.array_equal(x, y)
+    assert np.abs(outs[0][4]).max() > 0
