@@ -92,13 +92,13 @@ def geometry3d():
     return dict(sx=[N3 // 2], sy=[N3 // 2], sz=[1], rx=gx.ravel(), ry=gy.ravel(), rz=np.ones(gx.size, int))
 
 
-def plan3d(vp, rho, nt, storage="f32", use_graph=True):
+def plan3d(vp, rho, nt, storage="f32", use_graph=True, tile_cfg=0):
     import fomo
     import fomo._lib as L
     from fomo.api import Plan
     w = fomo.ricker_wavelet(F03, DT3, nt)
     return Plan(L.ACOUSTIC3D, vp, rho, DH3, DT3, nt, wavelet=w[None], nbc=NBC3, fd_order=8, storage=storage,
-                use_graph=use_graph, **geometry3d())
+                use_graph=use_graph, tile_cfg=tile_cfg, **geometry3d())
 
 
 def cpu_baseline3d(vp, rho, budget_s):
@@ -540,6 +540,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc traffic children")
     ap.add_argument("--no-extras", action="store_true", help="skip the FP16/BF16 and elastic2d extra lines")
+    ap.add_argument("--tile", type=int, default=0, help="3D z-segment choice of the headline plan (0: default)")
     ap.add_argument("--traffic-child", default="", help=argparse.SUPPRESS)
     ap.add_argument("--traffic-storage", default="f32", help=argparse.SUPPRESS)
     ap.add_argument("--probe", action="store_true", help="HBM-bound 2D probe instead of the headline line")
@@ -624,7 +625,7 @@ def headline(args):
     if not args.no_traffic:
         traffic = measure_traffic(vp, rho)  # before this process touches the GPU
     log("creating the plan (setup, upload, graph capture)")
-    plan = plan3d(vp, rho, NT3)
+    plan = plan3d(vp, rho, NT3, tile_cfg=args.tile)
     n_pad = plan.padded_cells
     pnx, pny, pnz = plan.padded[0]
     log(f"warmup {args.warmup} steps")
