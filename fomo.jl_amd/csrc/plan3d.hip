@@ -298,8 +298,15 @@ static int make_work3d(Plan3D* p, int kseg, int rows, int core_rows, int kseg_fu
   };
   std::vector<Work3D> core, frame;
   segs(pad, nz - pad, core_fp, core);
-  segs(1, pad, slab_fp, frame);
-  segs(nz - pad, nz - 1, slab_fp, frame);
+  {
+    // the top and bottom slabs are pad - 1 planes deep: one segment each (a second one would pay
+    // the z-queue prologue again for a few planes)
+    const int keep = kseg;
+    kseg = std::max(kseg, pad);
+    segs(1, pad, slab_fp, frame);
+    segs(nz - pad, nz - 1, slab_fp, frame);
+    kseg = keep;
+  }
   segs(1, nz - 1, ring_fp, frame);
   int rc = upload_work(p, 0, core, core_rows);
   if (!rc) rc = upload_work(p, 1, frame, rows);
@@ -445,6 +452,9 @@ int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, con
   std::vector<float> rhop;
   pad3(rho, rhop);
   const int mb = elastic ? 0 : 1;  // index of bx among the materials
+  // acoustic3d runs the interior's step in one launch unless FOMO_3D_UNFUSED is set (the two-sweep
+  // form, kept for comparison)
+  p->fused = !elastic && !std::getenv("FOMO_3D_UNFUSED");
   if (!elastic) {
     // kappa = rho * vp^2 (acoustic3d/medium.jl:40-62)
     std::vector<float> vpp, kappa(n);
@@ -602,9 +612,6 @@ int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, con
     if ((rc = p->alloc(&p->snap, p->snap_cells * p->snap_n))) return bail(rc);
   }
   const int kseg = d->tile_cfg > 0 ? 16 << (d->tile_cfg - 1) : 32;
-  // acoustic3d runs the interior's step in one launch unless FOMO_3D_UNFUSED is set (the two-sweep
-  // form, kept for comparison); its z-segments are longer (each pays 2N - 1 extra levels of vz)
-  p->fused = !elastic && !std::getenv("FOMO_3D_UNFUSED");
   // Fused interior z-segments: one block per CU fits (1024 threads, 60 KB of LDS), so the columns
   // are cut into the fewest segments that still give every CU a block; each segment pays 2N - 1
   // extra levels of vz, and tiles that march together share their halos through L2.

@@ -93,10 +93,12 @@ def _fd_rows(w, g, d, epss):
 def test_train_step_gradient_matches_finite_differences(gpu):
     """Central differences of the reported one-step loss against the accumulated gradient, along
     random directions (the weights as stored, BFloat16).  The perturbations move the loss by only
-    ~1e-5 of its value (most of it comes from frame cells the weights barely change), but the sum
-    is formed in a fixed order, so the differences are clean.  Output layer: within 5 %.  Hidden layers:
-    the step's inference rounds activations to BFloat16, which quantises their change, so within
-    15 % at larger steps."""
+    ~1e-5 of its value (most of it comes from frame cells the weights barely change); the loss is a
+    Float32 sum of per-block partials formed in a fixed order, whose rounding (~1e-23 here against
+    differences of ~1e-21) and the O(eps^2) term of the central difference leave a few per cent, so
+    the bar is 10 % for the output layer.  Hidden layers: the step's inference also rounds
+    activations to BFloat16, which quantises their change; 20 %.  A wrong sign, factor or missing
+    term in the composition of the sensitivities fails by far more."""
     from fomo.learned import N_H, N_IN, load_weights
     w = load_weights().astype(np.float32)
     g, l0 = _train(w, 1)
@@ -110,7 +112,7 @@ def test_train_step_gradient_matches_finite_differences(gpu):
     out_rows = _fd_rows(w, g, d_out, (0.01, 0.03))
     hid_rows = _fd_rows(w, g, d_hid, (0.03, 0.06))
     print({"loss": l0, "output layer": out_rows, "hidden layers": hid_rows})
-    for rows, tol in ((out_rows, 0.05), (hid_rows, 0.15)):
+    for rows, tol in ((out_rows, 0.10), (hid_rows, 0.20)):
         for _, act, pred in rows:
             assert pred != 0.0, rows
             assert abs(act - pred) <= tol * abs(pred), rows
