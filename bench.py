@@ -291,12 +291,15 @@ def probe(args):
             plan.sync()
             ms = plan.timed_advance(nt)
             tv, ts, tr = plan.kernel_times(max(10, nt // 4))
+            li, lt = plan.launches(), plan.launch_times(max(10, nt // 4))
             plan.close()
             key = name if st == "f32" else f"{name}_{st}"
             res[key] = {"padded_cells": cells, "ms_per_step": ms / nt, "gcell_per_s": cells * nt / ms / 1e6,
                         "step_gbs": eb * (b_v + b_s) * cells * nt / ms / 1e6,
                         "vel_ms": tv, "vel_gbs": eb * b_v * cells / tv / 1e6, "prs_ms": ts,
-                        "prs_gbs": eb * b_s * cells / ts / 1e6 if b_s else 0.0, "gather_ms": tr}
+                        "prs_gbs": eb * b_s * cells / ts / 1e6 if b_s else 0.0, "gather_ms": tr,
+                        "launches": [{"name": x["name"], "ms": t, "gbs": x["cells"] * x["bytes_per_cell"] / t / 1e6}
+                                     for x, t in zip(li, lt)]}
             log(f"{key}: {res[key]}")
     print(json.dumps({"probe": f"{n}x{n} interior, fd8, nbc 50", **res}), flush=True)
 

@@ -585,6 +585,257 @@ __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D&
   }
 }
 
+// Fused acoustic interior strip: the whole step (new vx, vz, then new p) of a strip of
+// 257 - 2N columns by W.h rows in one pass down its rows.  The separate sweeps move 48 B per cell
+// (read p, vx, vz, bx, bz, write vx, vz; read vx, vz, p, kappa, write p); here each of p, vx, vz,
+// bx, bz, kappa is read and each of vx, vz, p written once (36 B), plus halos shared through L2.
+//
+// Lane k of the 256 owns column x0 + k, x0 = i0 - (2N - 1), and keeps that column's p rows
+// tau - LP .. tau + N and new vz rows tau - LP - N .. tau in registers, so the z-derivatives of
+// the velocity and pressure updates need no shared memory.  At row tau the block
+//   A. computes new vz(tau) (lanes of the strip's columns use it), stages p(tau) with its x-halo
+//      into one of two LDS rows, and computes new p at row t = tau - LP on the strip's columns
+//      from the vx ring, its vz queue and its p queue;
+//   B. after one barrier, computes new vx(tau) on the 256 columns i0 - N + 1 .. i0 + W + N - 1
+//      the pressure's x-stencil reads, into a ring of LP + 1 LDS rows.
+// The pressure row lags the velocity row by LP = max(N - 1, 2): its z-stencil reaches new vz(t +
+// N - 1), and new vx(t), written in phase B of row t, must be behind a barrier (the one of row
+// t + 1) before phase A of row t + 2 reads it.  New velocities of cells in the user's interior
+// are recomputed from the old state (buffers [s], read-only during the launch); those of frame
+// cells are read from the frame velocity launch, which runs first and writes only frame cells
+// (buffers [1-s]; this launch writes only interior cells).  Rows are the same for every lane, so
+// row bounds, row offsets and the choice of source buffer are scalar.  Every load of a row is
+// unconditional (rows past the grid read clamped cells, zeroed where the separate sweeps' windows
+// hold zeros), so the number of loads in flight is the same on every path and the wait before a
+// use counts only the older loads.  Values are the separate sweeps' expressions in their order
+// (and, with 16-bit storage, the stored values the pressure sweep would read back), so the result
+// is bit-identical to them.
+template <int N>
+struct MarchGeom {
+  static constexpr int NT = 256;                      // lanes
+  static constexpr int HX = 2 * N - 1;                // lanes left of the strip's first column
+  static constexpr int W = NT + 1 - 2 * N;            // columns of a strip
+  static constexpr int SW = NT + 2 * N - 1;           // staged p row: columns x0 .. x0 + SW - 1
+  static constexpr int LP = N - 1 > 2 ? N - 1 : 2;    // pressure row = velocity row - LP
+  static constexpr int NQ = LP + N + 1;               // p queue: rows tau - LP .. tau + N
+  static constexpr int NZ = LP + N + 1;               // vz queue: rows tau - LP - N .. tau
+  static constexpr int NR = LP + 1;                   // rows of the vx ring
+  // rows of loads in flight: at 8192^2 FD8 the interior launch takes 0.561 / 0.557 / 0.561 ms
+  // with 2 / 3 / 4 rows in Float32 and 0.521 / 0.545 / 0.603 ms in Float16
+  static constexpr int D = 2;
+  static constexpr int LDS = 2 * SW + NR * NT;
+};
+
+int fused2d_width(int N) { return 257 - 2 * N; }
+
+template <int N, class T, class TM>
+__device__ __forceinline__ void march_strip(const SweepArgs& A, const Work2D& W, const Shot2D& S, float* lds,
+                                            int it) {
+  using Gm = MarchGeom<N>;
+  constexpr int NT = Gm::NT, HX = Gm::HX, SW = Gm::SW, LP = Gm::LP, NQ = Gm::NQ, NZ = Gm::NZ, NR = Gm::NR,
+                D = Gm::D;
+  const int nx = S.nx, nz = S.nz, pad = S.H + 1;
+  const float dtx = S.dtx;
+  const int s = A.parity, o = 1 - s;
+  const int i0 = W.i0, j0 = W.j0, h = W.h, w = W.w;
+  const int k = threadIdx.x;
+  const int x0 = i0 - HX;
+  const int cp = x0 + k;                      // this lane's p / vz / pressure column
+  const int cv = x0 + N + k;                  // this lane's vx column
+  const int ce = x0 + NT + min(k, HX - 1);    // extra staged p column (lanes k < HX)
+  auto clampx = [=](int i) { return min(max(i, 0), nx - 1); };
+  auto rowc = [=](int j) { return min(max(j, 0), nz - 1); };
+  auto in_z = [=](int j) { return j >= 0 && j < nz; };
+  auto core_z = [=](int j) { return j >= pad && j < nz - pad; };
+  const bool inp = cp >= 0 && cp < nx, inv = cv >= 0 && cv < nx, ine = ce >= 0 && ce < nx;
+  const int cpc = clampx(cp), cvc = clampx(cv), cec = clampx(ce);
+  const bool own = k >= HX && k - HX < w;     // a column of the strip
+  const bool core_v = cv >= pad && cv < nx - pad;
+  const bool store_v = cv >= i0 && cv < i0 + w;
+  const void* const p_s = S.fld[s][0];
+  const void* const vz_s = S.fld[s][2];
+  const void* const vz_o = S.fld[o][2];
+  const void* const vx_src = core_v ? S.fld[s][1] : S.fld[o][1];
+  void* const p_w = S.fld[o][0];
+  void* const vx_w = S.fld[o][1];
+  void* const vz_w = S.fld[o][2];
+  auto ld = [=](const void* base, int f, int j, int i) -> float {
+    return Cvt<T>::ld(base, (unsigned)(j * nx) + (unsigned)i, S.fdec[f]);
+  };
+  auto ldm = [=](int m, int j, int i) -> float {
+    return Cvt<TM>::ld(S.mat[m], (unsigned)(j * nx) + (unsigned)i, S.mdec[m]);
+  };
+  float a[N];
+#pragma unroll
+  for (int l = 0; l < N; ++l) a[l] = A.a[l];
+
+  // Sources in this lane's column within the strip, scanned once before the loop: the rows and
+  // amplitudes of this step of at most two (the host keeps the separate sweeps for plans with more
+  // sources in one column), in source order, so that sources sharing a cell add in that order.
+  int srow0 = -1, srow1 = -1;
+  float sw0 = 0.0f, sw1 = 0.0f;
+  if (it < A.nt && own) {
+    int q2 = 0;
+    for (int q = 0; q < S.n_src; ++q) {
+      const int cs = G(S.src_c)[q];
+      const int js = cs / nx, is = cs - js * nx;
+      if (is == cp && js >= j0 && js < j0 + h) {
+        const float wq = G(S.wav)[(size_t)q + (size_t)S.n_src * (size_t)it];
+        if (q2 == 0) {
+          srow0 = js;
+          sw0 = wq;
+        } else {
+          srow1 = js;
+          sw1 = wq;
+        }
+        ++q2;
+      }
+    }
+  }
+  // snapshot of this step (p), written by the pressure rows
+  const int nxi = nx - 2 * pad;
+  float* snap_k = nullptr;
+  if (S.snap && (it + 1) % S.snap_every == 0 && (it + 1) / S.snap_every <= S.snap_n)
+    snap_k = S.snap + (size_t)((it + 1) / S.snap_every - 1) * (size_t)nxi * (size_t)(nz - 2 * pad);
+
+  float* const stg = lds;            // two staged p rows (row tau in stg + (tau & 1) * SW)
+  float* const vxr = lds + 2 * SW;   // new vx of row r in vxr + (r mod NR) * NT
+
+  // loads of row rho: p(rho + N), extra p(rho), old / frame-final vx(rho) and bx(rho) at the vx
+  // column, old / frame-final vz(rho) and bz(rho), kappa(rho - LP)
+  struct Pre {
+    float p, pe, vxo, bx, vzo, bz, kap;
+  };
+  auto prefetch = [=](int rho, Pre& P) __attribute__((always_inline)) {
+    const int rp = rowc(rho + N), r0 = rowc(rho), rk = rowc(rho - LP);
+    P.p = ld(p_s, 0, rp, cpc);
+    P.pe = ld(p_s, 0, r0, cec);
+    P.vxo = ld(vx_src, 1, r0, cvc);
+    P.bx = ldm(1, r0, cvc);
+    P.vzo = ld(core_z(rho) ? vz_s : vz_o, 2, r0, cpc);
+    P.bz = ldm(2, r0, cpc);
+    P.kap = ldm(0, rk, cpc);
+  };
+
+  const int tau0 = j0 - N;
+  const int nit = (h + N + LP + D - 1) / D;   // rows tau0 .. tau0 + h + N + LP - 1, in steps of D
+  Pre pre[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) prefetch(tau0 + d, pre[d]);
+  // p queue before the first row: Q[i] = p(tau0 - 1 - LP + i), i >= 1
+  float Q[NQ], Z[NZ];
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) Q[i] = 0.0f;
+#pragma unroll
+  for (int i = 0; i < NZ; ++i) Z[i] = 0.0f;
+  {
+    float q0[NQ - 1];
+#pragma unroll
+    for (int i = 1; i < NQ; ++i) q0[i - 1] = ld(p_s, 0, rowc(tau0 - 1 - LP + i), cpc);
+#pragma unroll
+    for (int i = 1; i < NQ; ++i) Q[i] = (inp && in_z(tau0 - 1 - LP + i)) ? q0[i - 1] : 0.0f;
+  }
+
+  for (int g = 0; g < nit; ++g) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int tau = tau0 + g * D + d;
+      Pre& P = pre[d];
+      // A. p queue, new vz(tau), staged p(tau), new p(tau - LP)
+#pragma unroll
+      for (int i = 0; i < NQ - 1; ++i) Q[i] = Q[i + 1];
+      Q[NQ - 1] = (inp && in_z(tau + N)) ? P.p : 0.0f;
+      float vz = in_z(tau) ? P.vzo : 0.0f;
+      if (core_z(tau)) {
+        float dz = 0.0f;
+#pragma unroll
+        for (int l = 1; l <= N; ++l) dz += a[l - 1] * (Q[LP + l] - Q[LP - l + 1]);
+        vz = P.vzo + (P.bz * dtx) * dz;
+        if (own && tau >= j0 && tau < j0 + h)
+          st_field<T>(vz_w, (unsigned)(tau * nx) + (unsigned)cp, vz, S.fenc[2], S.ovf);
+        vz = round_trip<T>(vz, S.fenc[2], S.fdec[2]);
+      }
+#pragma unroll
+      for (int i = 0; i < NZ - 1; ++i) Z[i] = Z[i + 1];
+      Z[NZ - 1] = vz;
+      float* const sb = stg + (tau & 1) * SW;
+      sb[k] = Q[LP];
+      if (k < HX) sb[NT + k] = (ine && in_z(tau)) ? P.pe : 0.0f;
+      const int t = tau - LP;
+      if (t >= j0 && t < j0 + h && own) {
+        const float* VX = vxr + (t % NR) * NT + k - N;
+        float dvxdx = 0.0f, dvzdz = 0.0f;
+#pragma unroll
+        for (int l = 1; l <= N; ++l) {
+          const float cl = a[l - 1];
+          dvxdx += cl * (VX[l] - VX[1 - l]);
+          dvzdz += cl * (Z[N + l - 1] - Z[N - l]);
+        }
+        float v = Q[0] + P.kap * (dvxdx * dtx + dvzdz * dtx);
+        if (t == srow0) v += sw0;
+        if (t == srow1) v += sw1;
+        st_field<T>(p_w, (unsigned)(t * nx) + (unsigned)cp, v, S.fenc[0], S.ovf);
+        if (snap_k) G(snap_k)[(size_t)(cp - pad) + (size_t)(t - pad) * nxi] = round_trip<T>(v, S.fenc[0], S.fdec[0]);
+      }
+      __syncthreads();
+      // B. new vx(tau) on the 256 vx columns
+      if (tau >= j0 && tau < j0 + h) {
+        const float* PS = sb + k + N;
+        float v = inv ? P.vxo : 0.0f;
+        if (core_v) {
+          float dx = 0.0f;
+#pragma unroll
+          for (int l = 1; l <= N; ++l) dx += a[l - 1] * (PS[l - 1] - PS[-l]);
+          v = P.vxo + (P.bx * dtx) * dx;
+          if (store_v) st_field<T>(vx_w, (unsigned)(tau * nx) + (unsigned)cv, v, S.fenc[1], S.ovf);
+          v = round_trip<T>(v, S.fenc[1], S.fdec[1]);
+        }
+        vxr[(tau % NR) * NT + k] = v;
+      }
+      prefetch(tau + D, P);
+    }
+  }
+}
+
+template <int N, class T, class TM>
+__global__ __launch_bounds__(256) void fused2d_kernel(const SweepArgs A) {
+  __shared__ float lds[MarchGeom<N>::LDS];
+  if ((int)blockIdx.x >= A.n_work) return;
+  const Work2D W = A.work[blockIdx.x];
+  if (W.kind < 0) return;
+  const Shot2D& S = A.shots[W.shot];
+  march_strip<N, T, TM>(A, W, S, lds, *A.step + A.koff);
+}
+
+template <int N, class T, class TM>
+static hipError_t launch_fused2d_n(const SweepArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((fused2d_kernel<N, T, TM>), dim3(a.n_work), dim3(MarchGeom<N>::NT), 0, s, a);
+  return hipGetLastError();
+}
+
+template <class T, class TM>
+static hipError_t launch_fused2d_t(int N, const SweepArgs& a, hipStream_t s) {
+  switch (N) {
+    case 1: return launch_fused2d_n<1, T, TM>(a, s);
+    case 2: return launch_fused2d_n<2, T, TM>(a, s);
+    case 3: return launch_fused2d_n<3, T, TM>(a, s);
+    case 4: return launch_fused2d_n<4, T, TM>(a, s);
+    case 5: return launch_fused2d_n<5, T, TM>(a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_fused2d(int storage, int N, const SweepArgs& a, hipStream_t s) {
+  if (a.n_work <= 0) return hipSuccess;
+  switch (storage) {
+    case FOMO_STORE_F32: return launch_fused2d_t<float, float>(N, a, s);
+    case FOMO_STORE_F16: return launch_fused2d_t<__half, __half>(N, a, s);
+    case FOMO_STORE_BF16: return launch_fused2d_t<__hip_bfloat16, __hip_bfloat16>(N, a, s);
+    case FOMO_STORE_F32_MAT16: return launch_fused2d_t<float, __half>(N, a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 // Receiver entry e: the value of every slot's field in buffer [parity] into column col
 // (src/acquisition/device_indexed.jl:100-134).
 template <class T>

@@ -24,8 +24,15 @@ struct Plan2D : PlanBase {
   std::vector<Shot2D> shots;
   std::vector<ShotMeta> meta;
   Shot2D* d_shots = nullptr;
-  Work2D* d_work[kNumTileCfgs] = {nullptr};
-  int n_work[kNumTileCfgs] = {0};
+  Work2D* d_work[kNumFusedCfgs > kNumTileCfgs ? kNumFusedCfgs : kNumTileCfgs] = {nullptr};
+  int n_work[kNumFusedCfgs > kNumTileCfgs ? kNumFusedCfgs : kNumTileCfgs] = {0};
+  // acoustic2d: the interior's whole step in one launch (launch_fused2d).  d_work[cfg] then lists
+  // the interior strips of that launch (fused_rows(cfg) rows each) and d_frame the frame tiles,
+  // which the velocity and pressure launches take alone.
+  bool fused = false;
+  Work2D* d_frame = nullptr;
+  int n_frame = 0;
+  int only = -1;  // enqueue_part: only the frame launch (1) or only the fused launch (0) when >= 0
   int n_entries = 0;
   int* rec_shot = nullptr;
   int* rec_c = nullptr;
@@ -63,10 +70,19 @@ struct Plan2D : PlanBase {
   }
   int dims(int shot, int* nx, int* ny, int* nz, int* pad) const override;
   int n_shots() const override { return (int)shots.size(); }
-  int n_launches() const override { return d.eq == FOMO_SCALAR2D ? 1 : 2; }
+  // Launches of a step: velocity, pressure/stress (scalar: one); fused: velocity frame, interior
+  // step, pressure frame.
+  int n_launches() const override { return fused ? 3 : (d.eq == FOMO_SCALAR2D ? 1 : 2); }
   LaunchInfo launch_info(int k) const override;
-  hipError_t enqueue_launch(int k, int s) override { return enqueue_part(k, s, 0); }
+  hipError_t enqueue_launch(int k, int s) override {
+    if (!fused) return enqueue_part(k, s, 0);
+    only = k == 1 ? 0 : 1;
+    const hipError_t e = enqueue_part(k == 2 ? 1 : 0, s, 0);
+    only = -1;
+    return e;
+  }
   int make_work(int cfg);
+  int n_cfgs() const { return fused ? kNumFusedCfgs : kNumTileCfgs; }
   // Decode the current field `which` of a shot into `scratch` (asynchronous on the stream).
   int decode_field(int shot, int which);
   // Buffer holding the current state of field `which` of a shot.

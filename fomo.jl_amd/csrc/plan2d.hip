@@ -74,8 +74,44 @@ static std::vector<Work2D> xcd_layout(const std::vector<Work2D>& items) {
   return flat;
 }
 
+// Rows of one interior strip of the fused launch: 64, 96, 128, 176 or 256 (FOMO_FUSED2D_ROWS
+// forces a length, rounded up to a multiple of 8).  Each strip recomputes velocities on about 2N
+// rows past its ends and re-reads as many rows of its inputs, so longer strips move fewer bytes;
+// the block count also decides how evenly the last round of blocks fills the chip.  At 8192^2 and
+// FD8 (33 columns of 249-column strips) the interior launch takes 0.566, 0.588, 0.559 and 0.603 ms
+// at 96, 128, 176 and 256 rows: the autotuner times them all.
+static int fused_rows(int cfg) {
+  if (const char* env = std::getenv("FOMO_FUSED2D_ROWS")) {
+    const int r = std::atoi(env);
+    if (r > 0) return std::min(4096, (r + 7) / 8 * 8);
+  }
+  static const int rows[kNumFusedCfgs] = {64, 96, 128, 176, 256};
+  return rows[cfg];
+}
+
 int Plan2D::make_work(int cfg) {
   if (d_work[cfg]) return FOMO_OK;
+  if (fused) {
+    std::vector<Work2D> frame, inner;
+    for (const Work2D& w : work_items(shots, meta, 0))
+      if (w.kind == 0) frame.push_back(w);
+    const int rows = fused_rows(cfg), cols = fused2d_width(N);
+    for (size_t s = 0; s < shots.size(); ++s) {
+      const int pad = meta[s].pad, xe = pad + meta[s].nxi, ze = pad + meta[s].nzi;
+      for (int j = pad; j < ze; j += rows)
+        for (int i = pad; i < xe; i += cols)
+          inner.push_back(Work2D{(int)s, 1, i, j, std::min(cols, xe - i), std::min(rows, ze - j)});
+    }
+    if (!d_frame) {
+      const std::vector<Work2D> ff = xcd_layout(frame);
+      n_frame = (int)ff.size();
+      const int rc = upload(&d_frame, ff.data(), ff.size());
+      if (rc) return rc;
+    }
+    const std::vector<Work2D> fi = xcd_layout(inner);
+    n_work[cfg] = (int)fi.size();
+    return upload(&d_work[cfg], fi.data(), fi.size());
+  }
   const std::vector<Work2D> flat = xcd_layout(work_items(shots, meta, cfg));
   n_work[cfg] = (int)flat.size();
   return upload(&d_work[cfg], flat.data(), flat.size());
@@ -108,8 +144,16 @@ hipError_t Plan2D::enqueue_part(int part, int s, int koff) {
     a.g = g;
     a.n_gather = (n_entries + kGatherBlock - 1) / kGatherBlock;
   }
-  if (part == 1) return launch_sweep2d(sweep_storage, d.eq == FOMO_ACOUSTIC2D ? kAcPrs : kElStr, N, tile_cfg, a, stream);
-  if (learned) {
+  if (fused) {
+    // part 0: the frame's velocities (and the receivers of the previous step), then the interior's
+    // whole step; part 1: the frame's pressure
+    a.work = d_frame;
+    a.n_work = n_frame;
+    if (part == 1) return only == 0 ? hipSuccess : launch_sweep2d(sweep_storage, kAcPrs, N, tile_cfg, a, stream);
+  } else if (part == 1) {
+    return launch_sweep2d(sweep_storage, d.eq == FOMO_ACOUSTIC2D ? kAcPrs : kElStr, N, tile_cfg, a, stream);
+  }
+  if (learned && only != 0) {
     // blend-weight corrections of every frame cell from the state at time n
     MlpArgs m{};
     m.w_bf16 = w_bf16;
@@ -132,7 +176,15 @@ hipError_t Plan2D::enqueue_part(int part, int s, int koff) {
     if (e != hipSuccess) return e;
   }
   const int ph = d.eq == FOMO_ACOUSTIC2D ? kAcVel : (d.eq == FOMO_ELASTIC2D ? kElVel : kScal);
-  return launch_sweep2d(sweep_storage, ph, N, tile_cfg, a, stream);
+  if (!fused) return launch_sweep2d(sweep_storage, ph, N, tile_cfg, a, stream);
+  if (only != 0) {
+    const hipError_t e = launch_sweep2d(sweep_storage, ph, N, tile_cfg, a, stream);
+    if (e != hipSuccess || only == 1) return e;
+  }
+  a.work = d_work[tile_cfg];
+  a.n_work = n_work[tile_cfg];
+  a.n_gather = 0;
+  return launch_fused2d(sweep_storage, N, a, stream);
 }
 
 PlanBase::LaunchInfo Plan2D::launch_info(int k) const {
@@ -142,6 +194,14 @@ PlanBase::LaunchInfo Plan2D::launch_info(int k) const {
   // vx,vz,p,kappa -> p (5); elastic velocity 9, stress 11; scalar u^n,u^(n-1),c2 -> u^(n+1) (4)
   int words = 0;
   const char* name = "";
+  if (fused) {
+    // the interior launch is credited with both phases' reads and writes (7 + 5 words per cell)
+    long long inner = 0;
+    for (const auto& m : meta) inner += (long long)m.nxi * m.nzi;
+    if (k == 0) return {"acoustic2d velocity frame", cells - inner, 7 * elem};
+    if (k == 1) return {"acoustic2d interior step (velocity + pressure)", inner, 12 * elem};
+    return {"acoustic2d pressure frame", cells - inner, 5 * elem};
+  }
   if (d.eq == FOMO_ACOUSTIC2D) {
     words = k == 0 ? 7 : 5;
     name = k == 0 ? "acoustic2d velocity sweep" : "acoustic2d pressure sweep";
@@ -251,10 +311,11 @@ int Plan2D::autotune() {
   static std::map<std::tuple<int, int, int, int, int, int, long long>, int> cache;
   long long cells = 0;
   for (auto& sh : shots) cells += (long long)sh.nx * sh.nz;
-  const auto key = std::make_tuple(d.eq, N, sweep_storage, d.boundary, learned ? 1 : 0, (int)shots.size(), cells);
+  const auto key = std::make_tuple(d.eq, N, sweep_storage, d.boundary, (learned ? 1 : 0) + (fused ? 2 : 0),
+                                   (int)shots.size(), cells);
   if (const char* env = std::getenv("FOMO_TILE_CFG")) {
     const int c = std::atoi(env);
-    if (c >= 0 && c < kNumTileCfgs) {
+    if (c >= 0 && c < n_cfgs()) {
       tile_cfg = c;
       return make_work(c);
     }
@@ -269,15 +330,16 @@ int Plan2D::autotune() {
   FOMO_HIP_TRY(hipEventCreate(&e1));
   // Every strip length is timed in two interleaved rounds (so that no length is favoured by
   // running first or last) over about 2^24 cell-updates each, and keeps its faster round.
-  float best_ms[kNumTileCfgs];
-  for (int c = 0; c < kNumTileCfgs; ++c) {
+  const int nc = n_cfgs();
+  float best_ms[kNumFusedCfgs > kNumTileCfgs ? kNumFusedCfgs : kNumTileCfgs];
+  for (int c = 0; c < nc; ++c) {
     best_ms[c] = 1e30f;
     int rc = make_work(c);
     if (rc) return rc;
   }
   const int reps = (int)std::max<long long>(4, std::min<long long>(32, (1LL << 24) / std::max(cells, 1LL)));
   for (int round = 0; round < 2; ++round)
-    for (int c = 0; c < kNumTileCfgs; ++c) {
+    for (int c = 0; c < nc; ++c) {
       tile_cfg = c;
       FOMO_HIP_TRY(run_steps(2));
       FOMO_HIP_TRY(hipEventRecord(e0, stream));
@@ -289,7 +351,7 @@ int Plan2D::autotune() {
       best_ms[c] = std::min(best_ms[c], ms);
     }
   int best_cfg = 0;
-  for (int c = 1; c < kNumTileCfgs; ++c)
+  for (int c = 1; c < nc; ++c)
     if (best_ms[c] < best_ms[best_cfg]) best_cfg = c;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -334,6 +396,8 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
   p->storage = mat16 ? FOMO_STORE_F32 : d->storage;
   p->mat_storage = mat16 ? FOMO_STORE_F16 : d->storage;
   p->sweep_storage = d->storage;
+  // acoustic2d fuses the interior's two phases (FOMO_2D_UNFUSED=1 keeps the separate sweeps)
+  p->fused = d->eq == FOMO_ACOUSTIC2D && !std::getenv("FOMO_2D_UNFUSED");
   p->elem = p->storage == FOMO_STORE_F32 ? 4 : 2;
   p->N = d->fd_order / 2;
   if (scalar)
@@ -504,6 +568,10 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
     {
       std::vector<int> sc(std::max(S.n_src, 1), -1);
       for (int q = 0; q < S.n_src; ++q) sc[q] = (S.sx[q] - 1 + pad) + (S.sz[q] - 1 + pad) * x.nx;
+      // the fused interior launch holds at most two sources per column of a strip in registers
+      std::map<int, int> per_column;
+      for (int q = 0; q < S.n_src; ++q)
+        if (++per_column[S.sx[q]] > 2) p->fused = false;
       int* dsc = nullptr;
       if ((rc = p->upload(&dsc, sc.data(), sc.size()))) return bail(rc);
       x.src_c = dsc;
@@ -542,7 +610,7 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
   if ((rc = p->upload(&p->d_shots, p->shots.data(), p->shots.size()))) return bail(rc);
   if ((rc = p->alloc(&p->step, 1))) return bail(rc);
   if ((rc = p->reset())) return bail(rc);
-  if (d->tile_cfg > 0 && d->tile_cfg <= kNumTileCfgs) {
+  if (d->tile_cfg > 0 && d->tile_cfg <= p->n_cfgs()) {
     p->tile_cfg = d->tile_cfg - 1;
     if ((rc = p->make_work(p->tile_cfg))) return bail(rc);
   } else if ((rc = p->autotune())) {
