@@ -333,8 +333,10 @@ def learned_bench(args):
             learned.enable(p, weights)
         p.advance(min(20, nt))
         p.reset()
-        ms = p.timed_advance(nt)
+        ms, overflowed = p.timed_advance_status(nt)
         res[name] = {"padded_cells": p.padded_cells, "ms_per_step": ms / nt}
+        if overflowed:  # 16-bit storage saturated: the time stands, the wavefields are invalid
+            res[name]["storage_overflow"] = True
         p.close()
         log(f"{name}: {res[name]}")
     for name in ("learned_nbc16", "learned_nbc16_f16"):
@@ -346,12 +348,16 @@ def learned_bench(args):
                                                    ("habc_nbc100_f16", 100, None, "f16")]
                                        + [(f"habc_nbc{b}", b, None) for b in ladder])
     for k, v in refl.items():
+        if v.get("storage_overflow"):
+            res[k]["reflection_storage_overflow"] = True
+            continue
         res[k]["reflection_db"] = v["reflection_db"]
-        if k == "learned_nbc16_f16":
+        if k == "learned_nbc16_f16" and "vs_learned_nbc16" in v:
             res[k]["trace_rel_l2_vs_learned_nbc16"] = v["vs_learned_nbc16"]
     # equal reflection: the narrowest plain HABC at least as absorbing as the learned boundary
     target = res["learned_nbc16"]["reflection_db"]
-    match = next((f"habc_nbc{b}" for b in ladder + (100,) if res[f"habc_nbc{b}"]["reflection_db"] <= target), None)
+    match = next((f"habc_nbc{b}" for b in ladder + (100,)
+                  if res[f"habc_nbc{b}"].get("reflection_db", float("inf")) <= target), None)
     if match:
         res["equal_reflection"] = {"habc": match, "habc_db": res[match]["reflection_db"], "learned_db": target,
                                    "learned_speedup": res[match]["ms_per_step"] / res["learned_nbc16"]["ms_per_step"],
@@ -408,6 +414,7 @@ def batch_bench(args):
         rxl = np.arange(1, wd + 1, 2)
         shots.append(dict(x0=x0, nx=wd, sx=[wd // 2], sz=[5], rx=rxl, rz=np.full(len(rxl), 5), wavelet=w[None]))
     res = {}
+    fused_family = not elastic and not os.environ.get("FOMO_2D_UNFUSED")
     for cfg in (0, 1, 2, 3):
         log(f"64-shot plan, tile_cfg {cfg} (0: autotuned): setup, graph capture, {nt} timed steps")
         plan = Plan(eq2, vp, rho, 10.0, dt, nt, vs=vs if elastic else None, shots=shots, nbc=nbc, fd_order=8,
@@ -426,7 +433,8 @@ def batch_bench(args):
     out = {"batch": f"64 uneven windows (widths 256..1536) of a 4000x1000 model, {args.batch_eq} fd8, nbc 40, "
                     f"dt 0.8 ms, "
                     f"{nt} steps", "padded_cells": cells, "autotuned": res[0],
-           "fixed_strip_rows": {str(16 << (c - 1)): res[c] for c in (1, 2, 3)},
+           # a forced tile shape of an acoustic2d plan is one of the fused interior's strip lengths
+           "fixed_strip_rows": {str(r): res[c] for c, r in zip((1, 2, 3), (64, 96, 128) if fused_family else (16, 32, 64))},
            "widths": [s["nx"] for s in shots]}
     if not args.no_cpu:
         import oracle

@@ -24,12 +24,15 @@ struct Plan2D : PlanBase {
   std::vector<Shot2D> shots;
   std::vector<ShotMeta> meta;
   Shot2D* d_shots = nullptr;
-  Work2D* d_work[kNumFusedCfgs > kNumTileCfgs ? kNumFusedCfgs : kNumTileCfgs] = {nullptr};
-  int n_work[kNumFusedCfgs > kNumTileCfgs ? kNumFusedCfgs : kNumTileCfgs] = {0};
-  // acoustic2d: the interior's whole step in one launch (launch_fused2d).  d_work[cfg] then lists
-  // the interior strips of that launch (fused_rows(cfg) rows each) and d_frame the frame tiles,
-  // which the velocity and pressure launches take alone.
-  bool fused = false;
+  Work2D* d_work[kNumTileCfgs] = {nullptr};
+  int n_work[kNumTileCfgs] = {0};
+  // acoustic2d: the interior's whole step in one launch (launch_fused2d) over d_fwork[cfg]
+  // (interior strips of fused_rows(cfg) rows), the velocity and pressure launches over the frame
+  // tiles alone (d_frame).  fusable: the plan allows it (acoustic2d, at most two sources per
+  // column); fused: the current tile shape is one of the fused family.
+  bool fusable = false, fused = false;
+  Work2D* d_fwork[kNumFusedCfgs] = {nullptr};
+  int n_fwork[kNumFusedCfgs] = {0};
   Work2D* d_frame = nullptr;
   int n_frame = 0;
   int only = -1;  // enqueue_part: only the frame launch (1) or only the fused launch (0) when >= 0

@@ -90,7 +90,7 @@ static int fused_rows(int cfg) {
 }
 
 int Plan2D::make_work(int cfg) {
-  if (d_work[cfg]) return FOMO_OK;
+  if (fused ? d_fwork[cfg] != nullptr : d_work[cfg] != nullptr) return FOMO_OK;
   if (fused) {
     std::vector<Work2D> frame, inner;
     for (const Work2D& w : work_items(shots, meta, 0))
@@ -109,8 +109,8 @@ int Plan2D::make_work(int cfg) {
       if (rc) return rc;
     }
     const std::vector<Work2D> fi = xcd_layout(inner);
-    n_work[cfg] = (int)fi.size();
-    return upload(&d_work[cfg], fi.data(), fi.size());
+    n_fwork[cfg] = (int)fi.size();
+    return upload(&d_fwork[cfg], fi.data(), fi.size());
   }
   const std::vector<Work2D> flat = xcd_layout(work_items(shots, meta, cfg));
   n_work[cfg] = (int)flat.size();
@@ -181,8 +181,8 @@ hipError_t Plan2D::enqueue_part(int part, int s, int koff) {
     const hipError_t e = launch_sweep2d(sweep_storage, ph, N, tile_cfg, a, stream);
     if (e != hipSuccess || only == 1) return e;
   }
-  a.work = d_work[tile_cfg];
-  a.n_work = n_work[tile_cfg];
+  a.work = d_fwork[tile_cfg];
+  a.n_work = n_fwork[tile_cfg];
   a.n_gather = 0;
   return launch_fused2d(sweep_storage, N, a, stream);
 }
@@ -304,15 +304,23 @@ int Plan2D::dims(int shot, int* nx, int* ny, int* nz, int* pad) const {
   return FOMO_OK;
 }
 
-// Times a few direct-launch steps per tile shape and keeps the fastest.  The choice is cached
-// per (equation, order, sweep storage, boundary, learned, shot count, padded size) for the life of
-// the process; FOMO_TILE_CFG forces a shape.
+// Times a few direct-launch steps per tile shape and keeps the fastest.  An acoustic2d plan that
+// allows the fused interior launch times both families: the separate sweeps' strips (16, 32, 64
+// rows) and the fused interior's (64 .. 256 rows); FOMO_2D_FUSED=1 / FOMO_2D_UNFUSED=1 keep one.
+// (At 8192^2 the fused family is 12 % faster; on the 64-window batch, whose narrow windows leave
+// lanes of the 249-column fused strips idle, the separate sweeps are 1 % faster.)  The choice is
+// cached per (equation, order, sweep storage, boundary, learned, families, shot count, padded size)
+// for the life of the process; FOMO_TILE_CFG forces a shape of the current family.
 int Plan2D::autotune() {
-  static std::map<std::tuple<int, int, int, int, int, int, long long>, int> cache;
+  static std::map<std::tuple<int, int, int, int, int, int, long long>, std::pair<bool, int>> cache;
   long long cells = 0;
   for (auto& sh : shots) cells += (long long)sh.nx * sh.nz;
-  const auto key = std::make_tuple(d.eq, N, sweep_storage, d.boundary, (learned ? 1 : 0) + (fused ? 2 : 0),
-                                   (int)shots.size(), cells);
+  const bool only_fused = fusable && std::getenv("FOMO_2D_FUSED");
+  const bool try_fused = fusable && !std::getenv("FOMO_2D_UNFUSED");
+  const bool try_sweeps = !only_fused || !try_fused;
+  const int fam = (try_fused ? 1 : 0) + (try_sweeps ? 2 : 0);
+  const auto key = std::make_tuple(d.eq, N, sweep_storage, d.boundary, (learned ? 1 : 0) + 2 * fam, (int)shots.size(),
+                                   cells);
   if (const char* env = std::getenv("FOMO_TILE_CFG")) {
     const int c = std::atoi(env);
     if (c >= 0 && c < n_cfgs()) {
@@ -322,25 +330,31 @@ int Plan2D::autotune() {
   }
   auto it = cache.find(key);
   if (it != cache.end()) {
-    tile_cfg = it->second;
+    fused = it->second.first;
+    tile_cfg = it->second.second;
     return make_work(tile_cfg);
+  }
+  std::vector<std::pair<bool, int>> cand;
+  if (try_sweeps)
+    for (int c = 0; c < kNumTileCfgs; ++c) cand.push_back({false, c});
+  if (try_fused)
+    for (int c = 0; c < kNumFusedCfgs; ++c) cand.push_back({true, c});
+  for (const auto& c : cand) {
+    fused = c.first;
+    const int rc = make_work(c.second);
+    if (rc) return rc;
   }
   hipEvent_t e0, e1;
   FOMO_HIP_TRY(hipEventCreate(&e0));
   FOMO_HIP_TRY(hipEventCreate(&e1));
-  // Every strip length is timed in two interleaved rounds (so that no length is favoured by
-  // running first or last) over about 2^24 cell-updates each, and keeps its faster round.
-  const int nc = n_cfgs();
-  float best_ms[kNumFusedCfgs > kNumTileCfgs ? kNumFusedCfgs : kNumTileCfgs];
-  for (int c = 0; c < nc; ++c) {
-    best_ms[c] = 1e30f;
-    int rc = make_work(c);
-    if (rc) return rc;
-  }
+  // Every shape is timed in two interleaved rounds (so that no shape is favoured by running first
+  // or last) over about 2^24 cell-updates each, and keeps its faster round.
+  std::vector<float> best_ms(cand.size(), 1e30f);
   const int reps = (int)std::max<long long>(4, std::min<long long>(32, (1LL << 24) / std::max(cells, 1LL)));
   for (int round = 0; round < 2; ++round)
-    for (int c = 0; c < nc; ++c) {
-      tile_cfg = c;
+    for (size_t c = 0; c < cand.size(); ++c) {
+      fused = cand[c].first;
+      tile_cfg = cand[c].second;
       FOMO_HIP_TRY(run_steps(2));
       FOMO_HIP_TRY(hipEventRecord(e0, stream));
       FOMO_HIP_TRY(run_steps(reps));
@@ -350,13 +364,14 @@ int Plan2D::autotune() {
       FOMO_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
       best_ms[c] = std::min(best_ms[c], ms);
     }
-  int best_cfg = 0;
-  for (int c = 1; c < nc; ++c)
-    if (best_ms[c] < best_ms[best_cfg]) best_cfg = c;
+  size_t best = 0;
+  for (size_t c = 1; c < cand.size(); ++c)
+    if (best_ms[c] < best_ms[best]) best = c;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  tile_cfg = best_cfg;
-  cache[key] = best_cfg;
+  fused = cand[best].first;
+  tile_cfg = cand[best].second;
+  cache[key] = cand[best];
   return reset();
 }
 
@@ -396,8 +411,9 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
   p->storage = mat16 ? FOMO_STORE_F32 : d->storage;
   p->mat_storage = mat16 ? FOMO_STORE_F16 : d->storage;
   p->sweep_storage = d->storage;
-  // acoustic2d fuses the interior's two phases (FOMO_2D_UNFUSED=1 keeps the separate sweeps)
-  p->fused = d->eq == FOMO_ACOUSTIC2D && !std::getenv("FOMO_2D_UNFUSED");
+  // acoustic2d may fuse the interior's two phases (the autotuner decides; a forced tile shape is
+  // one of the fused family unless FOMO_2D_UNFUSED=1)
+  p->fusable = d->eq == FOMO_ACOUSTIC2D;
   p->elem = p->storage == FOMO_STORE_F32 ? 4 : 2;
   p->N = d->fd_order / 2;
   if (scalar)
@@ -571,7 +587,7 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
       // the fused interior launch holds at most two sources per column of a strip in registers
       std::map<int, int> per_column;
       for (int q = 0; q < S.n_src; ++q)
-        if (++per_column[S.sx[q]] > 2) p->fused = false;
+        if (++per_column[S.sx[q]] > 2) p->fusable = false;
       int* dsc = nullptr;
       if ((rc = p->upload(&dsc, sc.data(), sc.size()))) return bail(rc);
       x.src_c = dsc;
@@ -610,6 +626,7 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
   if ((rc = p->upload(&p->d_shots, p->shots.data(), p->shots.size()))) return bail(rc);
   if ((rc = p->alloc(&p->step, 1))) return bail(rc);
   if ((rc = p->reset())) return bail(rc);
+  p->fused = p->fusable && !std::getenv("FOMO_2D_UNFUSED");
   if (d->tile_cfg > 0 && d->tile_cfg <= p->n_cfgs()) {
     p->tile_cfg = d->tile_cfg - 1;
     if ((rc = p->make_work(p->tile_cfg))) return bail(rc);

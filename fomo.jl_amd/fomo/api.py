@@ -512,6 +512,16 @@ class Plan:
         _lib.check(_lib.lib().fomo_plan_timed_advance(self._h, int(n), C.byref(ms)))
         return ms.value
 
+    def timed_advance_status(self, n: int) -> tuple[float, bool]:
+        """timed_advance that reports a 16-bit storage overflow (FOMO_EOVERFLOW: some stored value
+        saturated, so the wavefields are invalid) instead of raising: (ms, overflowed)."""
+        ms = C.c_double(0.0)
+        rc = _lib.lib().fomo_plan_timed_advance(self._h, int(n), C.byref(ms))
+        if rc == _lib.FOMO_EOVERFLOW:
+            return ms.value, True
+        _lib.check(rc)
+        return ms.value, False
+
     def sync(self):
         _lib.check(_lib.lib().fomo_plan_sync(self._h))
 

@@ -181,7 +181,9 @@ def reflection_residual(vp, rho, *, sx, sz, rx, rz, nt, boundaries, dh=10.0, dt=
     own boundary arrive after the record ends), also given as 20 log10(resid) dB.  `boundaries` is a
     list of (name, nbc, weights-or-None[, storage]).  Also reports per-trace relative L2 against u_ref and against
     the boundary named `judge`, and the time of the step loop.  The norms are FP64 sums formed on the
-    GPU (Plan.trace_misfit); the traces are not downloaded."""
+    GPU (Plan.trace_misfit); the traces are not downloaded.  A 16-bit run whose stored values
+    overflowed (saturated, so its traces are invalid) is reported as {"storage_overflow": True,
+    "ms": ...} without residuals."""
     from .setup import ricker_wavelet
     wav = ricker_wavelet(f0, dt, nt)
     if L is None:
@@ -194,7 +196,8 @@ def reflection_residual(vp, rho, *, sx, sz, rx, rz, nt, boundaries, dh=10.0, dt=
         try:
             if weights is not None:
                 enable(p, weights)
-            return p, p.timed_advance(nt)
+            ms, overflowed = p.timed_advance_status(nt)
+            return p, ms, overflowed
         except Exception:
             p.close()
             raise
@@ -208,11 +211,14 @@ def reflection_residual(vp, rho, *, sx, sz, rx, rz, nt, boundaries, dh=10.0, dt=
     ref = None
     out = {}
     try:
-        ref, _ = run(edge_extend(vp, L), edge_extend(rho, L), 20, sx + L, sz + L, rx + L, rz + L,
-                     v_ref=float(vp.min()))
+        ref, _, _ = run(edge_extend(vp, L), edge_extend(rho, L), 20, sx + L, sz + L, rx + L, rz + L,
+                        v_ref=float(vp.min()))
         for b in boundaries:
             plans[b[0]] = run(vp, rho, b[1], sx, sz, rx, rz, b[2], storage=b[3] if len(b) > 3 else "f32")
-        for name, (p, ms) in plans.items():
+        for name, (p, ms, overflowed) in plans.items():
+            if overflowed:
+                out[name] = {"storage_overflow": True, "ms": ms, "padded_cells": p.padded_cells}
+                continue
             # seis_p (slot 0) against the edge-extended run, summed in FP64 on the GPU
             num, den = p.trace_misfit(ref)
             resid = float(math.sqrt(num.sum() / den.sum()))
@@ -220,12 +226,12 @@ def reflection_residual(vp, rho, *, sx, sz, rx, rz, nt, boundaries, dh=10.0, dt=
             rec = {"resid": resid, "reflection_db": 20.0 * math.log10(max(resid, 1e-300)),
                    "trace_rel_l2_median": med, "trace_rel_l2_max": mx, "ms": ms,
                    "padded_cells": p.padded_cells, "shots_per_s": 1000.0 / ms}
-            if judge in plans and name != judge:
+            if judge in plans and name != judge and not plans[judge][2]:
                 mj, xj = trace_errors(*p.trace_misfit(plans[judge][0]))
                 rec["vs_" + judge] = {"trace_rel_l2_median": mj, "trace_rel_l2_max": xj}
             out[name] = rec
     finally:
-        for p, _ in plans.values():
+        for p, _, _ in plans.values():
             p.close()
         if ref is not None:
             ref.close()

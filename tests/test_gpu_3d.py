@@ -285,7 +285,8 @@ def test_elastic3d_reduced_precision_storage_error(gpu, orc, storage, tol, tol_v
 @pytest.mark.parametrize("order,storage", [(2, "f32"), (4, "f32"), (8, "f32"), (8, "f16"), (10, "bf16")])
 def test_acoustic3d_fused_interior_equals_two_sweeps(gpu, order, storage, monkeypatch):
     """The one-launch interior step (velocity then pressure, new velocities kept on chip) gives the
-    bits of the two separate sweeps (FOMO_3D_UNFUSED), for every FD order's halo widths, for 16-bit
+    bits of the two separate sweeps (FOMO_3D_UNFUSED; FOMO_3D_FUSED forces it on 16-bit storage,
+    whose default is the separate sweeps), for every FD order's halo widths, for 16-bit
     storage, and for interior sizes that leave partial tiles in x (50 = 64 - 14), y (37 = 2 x 16 +
     5) and z (segments of 32 planes over 45); the launch list names the fused launch."""
     import fomo
@@ -300,8 +301,10 @@ def test_acoustic3d_fused_interior_equals_two_sweeps(gpu, order, storage, monkey
     for unfused in (False, True):
         if unfused:
             monkeypatch.setenv("FOMO_3D_UNFUSED", "1")
+            monkeypatch.delenv("FOMO_3D_FUSED", raising=False)
         else:
             monkeypatch.delenv("FOMO_3D_UNFUSED", raising=False)
+            monkeypatch.setenv("FOMO_3D_FUSED", "1")  # 16-bit plans run the separate sweeps by default
         p = Plan(L.ACOUSTIC3D, vp, rho, 10.0, 1e-3, nt, sx=sx, sy=sy, sz=sz, rx=rx, ry=ry, rz=rz,
                  wavelet=np.stack([w] * len(sx)), nbc=6, fd_order=order, storage=storage, record_p=True, tile_cfg=1)
         try:
