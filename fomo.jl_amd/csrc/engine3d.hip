@@ -149,14 +149,18 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
   const bool i_upd = i >= N && i < nx - N;
 
   // Loads of field f at (ii, jj) of plane kk: scalar plane base + 32-bit in-plane offset.  Frame
-  // tiles reach outside the grid (halos, the ring); there the cell is clamped to the grid instead
-  // of the load being skipped: a load under a per-lane condition is waited for in its own branch,
-  // and values of cells outside the grid enter no stored result (those cells are not stored, and
-  // every stencil of an update cell stays inside the grid).
+  // tiles reach outside the grid (halos, the ring) and, where the band is narrower than the tile,
+  // past what their cells need; there the cell is clamped to the columns and rows the tile's
+  // outputs, ring and stencils use (inside the grid) instead of the load being skipped: a load
+  // under a per-lane condition is waited for in its own branch.  A clamped load repeats a cell the
+  // tile loads anyway, so it costs no HBM traffic, and its value enters no stored result (those
+  // cells are not stored, and every stencil of an output or ring cell stays inside the range).
+  const int cx_lo = LAG ? max(pi0 - N, 0) : 0, cx_hi = LAG ? min(W.i0 + W.w + rx - 1 + N, nx - 1) : nx - 1;
+  const int cy_lo = LAG ? max(pj0 - N, 0) : 0, cy_hi = LAG ? min(W.j0 + W.hy + ry - 1 + N, ny - 1) : ny - 1;
   auto ld = [&](int buf, int f, int ii, int jj, int kk) -> float {
     if constexpr (LAG) {
-      ii = min(max(ii, 0), nx - 1);
-      jj = min(max(jj, 0), ny - 1);
+      ii = min(max(ii, cx_lo), cx_hi);
+      jj = min(max(jj, cy_lo), cy_hi);
       kk = min(max(kk, 0), nz - 1);
     }
     const unsigned off = (unsigned)(ii + jj * nx);
@@ -207,7 +211,7 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
       const int j = pj0 + ty + TR * r;
       // cells outside the grid load a clamped cell (see ld); materials of cells outside the update
       // region are loaded too and not used
-      const int ic = LAG ? min(max(i, 0), nx - 1) : i, jc = LAG ? min(max(j, 0), ny - 1) : j;
+      const int ic = LAG ? min(max(i, cx_lo), cx_hi) : i, jc = LAG ? min(max(j, cy_lo), cy_hi) : j;
       const int tc = LAG ? min(max(tp, 0), nz - 1) : tp;
       const unsigned co = (unsigned)(ic + jc * nx);
 #pragma unroll
