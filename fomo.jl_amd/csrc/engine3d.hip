@@ -161,8 +161,8 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
     if constexpr (LAG) {
       ii = min(max(ii, cx_lo), cx_hi);
       jj = min(max(jj, cy_lo), cy_hi);
-      kk = min(max(kk, 0), nz - 1);
     }
+    kk = min(max(kk, 0), nz - 1);  // uniform: scalar instructions (the loads of a level past an item's last)
     const unsigned off = (unsigned)(ii + jj * nx);
     return Cvt<T>::ld(plane_of<T>(S.fld[buf][f], kk, sz), off, S.fdec[f]);
   };
@@ -181,18 +181,33 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
 
   // Registers prefetched one plane ahead: own-column values of staged fields that have no queue,
   // x-halo values (lanes tx < N), y-halo rows, old values of the outputs and the materials.
-  float sv[NS][YPT], hx[NS][YPT][2], hy[NS][HYR], oldF[YPT][NF], matF[YPT][NM], qn[NQ][YPT];
-  auto prefetch_stage = [&](int tp) {
+  // NSET register sets: level t uses set (t - t0) % NSET, refilled with level t + NSET as soon as
+  // it is consumed.  The 16-bit acoustic frame march keeps two planes of loads in flight: it runs
+  // at 4 waves per SIMD under its 128-VGPR cap either way, and the second set took the 512^3 FP16
+  // step from 3.13 to 2.65 ms.  In Float32 the second set needs 142 VGPRs (one 512-thread block
+  // per CU) or a spilling cap, and both were slower (3.16 -> 3.25 ms).
+  constexpr int NSET = LAG && (PH == kA3Vel || PH == kA3Prs) && !std::is_same<T, float>::value ? 2 : 1;
+  // Every level issues its next loads (past the item's last level too), so that every path issues
+  // the same loads and the compiler's wait before a use counts only the older ones instead of
+  // draining all of them (512^3: elastic3d FP16 6.94 -> 6.13 ms with the two-set acoustic frames
+  // above, acoustic3d FP16 3.13 -> 2.61 ms).  Two phases keep their loads under `t < t_end`:
+  // issued unconditionally they held a few more registers and lost a wave per SIMD (the elastic
+  // stress frame 124 -> 129 VGPRs in FP32, spills in FP16; the FP32 acoustic pressure frame
+  // 80 -> 81 VGPRs, 6 -> 5 waves, 0.49 -> 0.54 ms).
+  constexpr bool kAll = PH != kE3Str && !(PH == kA3Prs && LAG && std::is_same<T, float>::value);
+  float sv[NSET][NS][YPT], hx[NSET][NS][YPT][2], hy[NSET][NS][HYR], oldF[NSET][YPT][NF], matF[NSET][YPT][NM],
+      qn[NSET][NQ][YPT];
+  auto prefetch_stage = [&](int tp, int st) {
 #pragma unroll
     for (int sp = 0; sp < NS; ++sp) {
       const int f = P::st_f(sp);
 #pragma unroll
       for (int r = 0; r < YPT; ++r) {
         const int j = pj0 + ty + TR * r;
-        if (P::st_q(sp) < 0) sv[sp][r] = ld(sb, f, i, j, tp);
+        if (P::st_q(sp) < 0) sv[st][sp][r] = ld(sb, f, i, j, tp);
         if (P::st_xh(sp)) {  // used by lanes tx < N; the others load a clamped cell
-          hx[sp][r][0] = ld(sb, f, max(pi0 - N + tx, 0), j, tp);
-          hx[sp][r][1] = ld(sb, f, min(pi0 + kTileW + tx, nx - 1), j, tp);
+          hx[st][sp][r][0] = ld(sb, f, max(pi0 - N + tx, 0), j, tp);
+          hx[st][sp][r][1] = ld(sb, f, min(pi0 + kTileW + tx, nx - 1), j, tp);
         }
       }
       if (P::st_yh(sp)) {
@@ -200,26 +215,35 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
         for (int m = 0; m < HYR; ++m) {
           const int hr = ty + TR * m;
           const int jl = hr < N ? hr - N : ROWS + (hr - N);  // used by rows hr < 2N
-          hy[sp][m] = ld(sb, f, min(i, nx - 1), min(max(pj0 + jl, 0), ny - 1), tp);
+          hy[st][sp][m] = ld(sb, f, min(i, nx - 1), min(max(pj0 + jl, 0), ny - 1), tp);
         }
       }
     }
   };
-  auto prefetch_cells = [&](int tp) {
+  auto prefetch_cells = [&](int tp, int st) {
 #pragma unroll
     for (int r = 0; r < YPT; ++r) {
       const int j = pj0 + ty + TR * r;
       // cells outside the grid load a clamped cell (see ld); materials of cells outside the update
       // region are loaded too and not used
       const int ic = LAG ? min(max(i, cx_lo), cx_hi) : i, jc = LAG ? min(max(j, cy_lo), cy_hi) : j;
-      const int tc = LAG ? min(max(tp, 0), nz - 1) : tp;
+      const int tc = min(max(tp, 0), nz - 1);
       const unsigned co = (unsigned)(ic + jc * nx);
 #pragma unroll
       for (int f = 0; f < NF; ++f)
-        oldF[r][f] = Cvt<T>::ld(plane_of<T>(S.fld[s][P::out_f(f)], tc, sz), co, S.fdec[P::out_f(f)]);
+        oldF[st][r][f] = Cvt<T>::ld(plane_of<T>(S.fld[s][P::out_f(f)], tc, sz), co, S.fdec[P::out_f(f)]);
 #pragma unroll
       for (int k = 0; k < NM; ++k)
-        matF[r][k] = Cvt<T>::ld(plane_of<T>(S.mat[P::mat_id(k)], tc, sz), co, S.mdec[P::mat_id(k)]);
+        matF[st][r][k] = Cvt<T>::ld(plane_of<T>(S.mat[P::mat_id(k)], tc, sz), co, S.mdec[P::mat_id(k)]);
+    }
+  };
+  // the queue entry that completes the window of level t + 1 (consumed at the end of level t)
+  auto prefetch_q = [&](int t, int st) {
+#pragma unroll
+    for (int qq = 0; qq < NQ; ++qq) {
+      const int lo = P::q_back(qq) ? -N : -N + 1;
+#pragma unroll
+      for (int r = 0; r < YPT; ++r) qn[st][qq][r] = ld(sb, P::q_f(qq), i, pj0 + ty + TR * r, t + 1 + lo + 2 * N - 1);
     }
   };
 
@@ -243,10 +267,15 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
       }
     }
   }
-  prefetch_stage(t0);
-  prefetch_cells(t0);
   const int t_end = LAG ? W.k0 + W.hz : W.k0 + W.hz - 1;
-  for (int t = t0; t <= t_end; ++t) {
+  // Loads of levels past t_end are issued too (planes clamped into the grid).
+#pragma unroll
+  for (int st = 0; st < NSET; ++st) {
+    prefetch_stage(t0 + st, st);
+    prefetch_cells(t0 + st, st);
+    if constexpr (NSET == 2) prefetch_q(t0 + st, st);
+  }
+  auto level = [&](const int t, const int st) __attribute__((always_inline)) {
     float* stg = lds + (t & 1) * L::STG;
     // 1. stage the x/y-stencil planes of level t from the prefetched registers
 #pragma unroll
@@ -256,11 +285,11 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
       for (int r = 0; r < YPT; ++r) {
         const int jl = ty + TR * r;
         const int qs = P::st_q(sp) < 0 ? 0 : P::st_q(sp);
-        const float v = P::st_q(sp) >= 0 ? q[qs][r][P::q_back(qs) ? N : N - 1] : sv[sp][r];
+        const float v = P::st_q(sp) >= 0 ? q[qs][r][P::q_back(qs) ? N : N - 1] : sv[st][sp][r];
         pl[(jl + N) * PW + tx + N] = v;
         if (P::st_xh(sp) && tx < N) {
-          pl[(jl + N) * PW + tx] = hx[sp][r][0];
-          pl[(jl + N) * PW + kTileW + N + tx] = hx[sp][r][1];
+          pl[(jl + N) * PW + tx] = hx[st][sp][r][0];
+          pl[(jl + N) * PW + kTileW + N + tx] = hx[st][sp][r][1];
         }
       }
       if (P::st_yh(sp)) {
@@ -269,20 +298,16 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
           const int hr = ty + TR * m;
           if (hr < 2 * N) {
             const int jl = hr < N ? hr - N : ROWS + (hr - N);
-            pl[(jl + N) * PW + tx + N] = hy[sp][m];
+            pl[(jl + N) * PW + tx + N] = hy[st][sp][m];
           }
         }
       }
     }
-    // 2. issue the staging loads of level t+1 (and the queue entry that completes its window)
-    if (t < t_end) {
-      prefetch_stage(t + 1);
-#pragma unroll
-      for (int qq = 0; qq < NQ; ++qq) {
-        const int lo = P::q_back(qq) ? -N : -N + 1;
-#pragma unroll
-        for (int r = 0; r < YPT; ++r) qn[qq][r] = ld(sb, P::q_f(qq), i, pj0 + ty + TR * r, t + 1 + lo + 2 * N - 1);
-      }
+    // 2. issue the staging loads of level t + NSET into the set just consumed (one set: also the
+    // queue entry this level consumes at its end; two sets: that entry was loaded two levels ago)
+    if (kAll || t < t_end) {
+      prefetch_stage(t + NSET, st);
+      if constexpr (NSET == 1) prefetch_q(t, st);
     }
     __syncthreads();
 
@@ -303,9 +328,9 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
           dy += cl * (P0[(l - 1) * PW] - P0[-l * PW]);
           dz += cl * (q[0][r][N - 1 + l] - q[0][r][N - l]);
         }
-        preN[r][0] = oldF[r][0] + (matF[r][0] * dtdh) * dx;
-        preN[r][1] = oldF[r][1] + (matF[r][1] * dtdh) * dy;
-        preN[r][2] = oldF[r][2] + (matF[r][2] * dtdh) * dz;
+        preN[r][0] = oldF[st][r][0] + (matF[st][r][0] * dtdh) * dx;
+        preN[r][1] = oldF[st][r][1] + (matF[st][r][1] * dtdh) * dy;
+        preN[r][2] = oldF[st][r][2] + (matF[st][r][2] * dtdh) * dz;
       } else if constexpr (PH == kA3Prs) {
         const float* VX = stg + row;
         const float* VY = stg + PR * PW + row;
@@ -317,7 +342,7 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
           dy += cl * (VY[l * PW] - VY[-(l - 1) * PW]);
           dz += cl * (q[0][r][N + l - 1] - q[0][r][N - l]);
         }
-        preN[r][0] = oldF[r][0] + (matF[r][0] * dtdh) * ((dx + dy) + dz);
+        preN[r][0] = oldF[st][r][0] + (matF[st][r][0] * dtdh) * ((dx + dy) + dz);
       } else if constexpr (PH == kE3Vel) {
         const float* TXX = stg + row;
         const float* TXY = stg + PR * PW + row;
@@ -339,9 +364,9 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
           dtyzdz += cl * (q[2][r][N + l - 1] - q[2][r][N - l]);
           dtyzdy += cl * (TYZ[l * PW] - TYZ[-(l - 1) * PW]);
         }
-        preN[r][0] = oldF[r][0] + (matF[r][0] * dtdh) * ((dtxxdx + dtxydy) + dtxzdz);
-        preN[r][1] = oldF[r][1] + (matF[r][1] * dtdh) * ((dtxydx + dtyydy) + dtyzdz);
-        preN[r][2] = oldF[r][2] + (matF[r][2] * dtdh) * ((dtxzdx + dtyzdy) + dtzzdz);
+        preN[r][0] = oldF[st][r][0] + (matF[st][r][0] * dtdh) * ((dtxxdx + dtxydy) + dtxzdz);
+        preN[r][1] = oldF[st][r][1] + (matF[st][r][1] * dtdh) * ((dtxydx + dtyydy) + dtyzdz);
+        preN[r][2] = oldF[st][r][2] + (matF[st][r][2] * dtdh) * ((dtxzdx + dtyzdy) + dtzzdz);
       } else {
         const float* VX = stg + row;
         const float* VY = stg + PR * PW + row;
@@ -361,22 +386,22 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
           dvydz += cl * (q[1][r][N - 1 + l] - q[1][r][N - l]);
           dvzdy += cl * (VZ[(l - 1) * PW] - VZ[-l * PW]);
         }
-        const float lv = matF[r][0], l2 = matF[r][1];
-        preN[r][0] = oldF[r][0] + ((l2 * dvxdx + lv * dvydy) + lv * dvzdz) * dtdh;
-        preN[r][1] = oldF[r][1] + ((lv * dvxdx + l2 * dvydy) + lv * dvzdz) * dtdh;
-        preN[r][2] = oldF[r][2] + ((lv * dvxdx + lv * dvydy) + l2 * dvzdz) * dtdh;
-        preN[r][3] = oldF[r][3] + (matF[r][2] * (dvxdy + dvydx)) * dtdh;
-        preN[r][4] = oldF[r][4] + (matF[r][3] * (dvxdz + dvzdx)) * dtdh;
-        preN[r][5] = oldF[r][5] + (matF[r][4] * (dvydz + dvzdy)) * dtdh;
+        const float lv = matF[st][r][0], l2 = matF[st][r][1];
+        preN[r][0] = oldF[st][r][0] + ((l2 * dvxdx + lv * dvydy) + lv * dvzdz) * dtdh;
+        preN[r][1] = oldF[st][r][1] + ((lv * dvxdx + l2 * dvydy) + lv * dvzdz) * dtdh;
+        preN[r][2] = oldF[st][r][2] + ((lv * dvxdx + lv * dvydy) + l2 * dvzdz) * dtdh;
+        preN[r][3] = oldF[st][r][3] + (matF[st][r][2] * (dvxdy + dvydx)) * dtdh;
+        preN[r][4] = oldF[st][r][4] + (matF[st][r][3] * (dvxdz + dvzdx)) * dtdh;
+        preN[r][5] = oldF[st][r][5] + (matF[st][r][4] * (dvydz + dvzdy)) * dtdh;
       }
       if (!upd) {
 #pragma unroll
-        for (int f = 0; f < NF; ++f) preN[r][f] = oldF[r][f];
+        for (int f = 0; f < NF; ++f) preN[r][f] = oldF[st][r][f];
       }
     }
     if constexpr (!LAG) {
       // core march: no frame cell, store at once (after the next plane's loads are issued)
-      if (t < t_end) prefetch_cells(t + 1);
+      if (kAll || t < t_end) prefetch_cells(t + NSET, st);
       const bool col_out = tx < W.w;
 #pragma unroll
       for (int r = 0; r < YPT; ++r) {
@@ -401,11 +426,11 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
         for (int f = 0; f < NF; ++f) {
           const int e = (f * ROWS + ty + TR * r) * kTileW + tx;
           sp_[e] = preN[r][f];
-          so_[e] = oldF[r][f];
-          oldT[r][f] = oldF[r][f];
+          so_[e] = oldF[st][r][f];
+          oldT[r][f] = oldF[st][r][f];
         }
     }
-    if (t < t_end) prefetch_cells(t + 1);
+    if (kAll || t < t_end) prefetch_cells(t + NSET, st);
 
     // 4. finalise plane t-1 (its rings were written before this level's barrier): Higdon blend of frame cells, injection, store
     if (t > W.k0) {
@@ -479,8 +504,15 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
       for (int r = 0; r < YPT; ++r) {
 #pragma unroll
         for (int m = 0; m < 2 * N - 1; ++m) q[qq][r][m] = q[qq][r][m + 1];
-        q[qq][r][2 * N - 1] = qn[qq][r];
+        q[qq][r][2 * N - 1] = qn[st][qq][r];
       }
+    if constexpr (NSET == 2) prefetch_q(t + NSET, st);
+  };
+  for (int t = t0; t <= t_end; t += NSET) {
+    level(t, 0);
+    if constexpr (NSET == 2) {
+      if (t + 1 <= t_end) level(t + 1, 1);
+    }
   }
 }
 
@@ -502,7 +534,7 @@ __global__ __launch_bounds__((kTileW * kTR<PH, LAG>)) void sweep3d_kernel(const 
   sweep3d_body<PH, N, T, LAG>(A);
 }
 template <int PH, int N, class T, bool LAG>
-__global__ __launch_bounds__((kTileW * kTR<PH, LAG>)) __attribute__((amdgpu_waves_per_eu(4))) void sweep3d_kernel16(
+__global__ __launch_bounds__((kTileW * kTR<PH, LAG>)) __attribute__((amdgpu_waves_per_eu(4))) void sweep3d_kernel_w4(
     const Sweep3DArgs A) {
   sweep3d_body<PH, N, T, LAG>(A);
 }
@@ -571,7 +603,7 @@ static void launch_one(const Sweep3DArgs& a, hipStream_t s) {
   if constexpr (std::is_same<T, float>::value)
     hipLaunchKernelGGL((sweep3d_kernel<PH, N, T, LAG>), dim3(a.n_work), blk, 0, s, a);
   else
-    hipLaunchKernelGGL((sweep3d_kernel16<PH, N, T, LAG>), dim3(a.n_work), blk, 0, s, a);
+    hipLaunchKernelGGL((sweep3d_kernel_w4<PH, N, T, LAG>), dim3(a.n_work), blk, 0, s, a);
 }
 
 template <int PH, class T, bool LAG>
