@@ -18,6 +18,8 @@
 // Float32 updates:
 //   acoustic velocity / pressure ... src/equations/acoustic2d/fused_kernels.jl:47-61, :101-112
 //   elastic velocity / stress ...... src/equations/elastic2d/fused_kernels.jl:39-57, :105-121
+//   coupled P-S potentials ......... src/equations/coupled2d/update_fields.jl:27-144 (the
+//                                    interior's position half-step rides in its velocity strips)
 //   Higdon value ................... src/boundary/habc/kernels_det.jl:22-57
 //   injection / gather ............. src/acquisition/device_indexed.jl:28-134 (step index read
 //                                    from a device counter, as the reference's graph path does)
@@ -35,8 +37,8 @@ struct PV {
   const void* sten[3];
   const void* old[3];
   void* out[3];
-  const void* mat[3];
-  float sdec[3], odec[3], oenc[3], mdec[3];  // scales of the stored arrays above
+  const void* mat[8];
+  float sdec[3], odec[3], oenc[3], mdec[8];  // scales of the stored arrays above
   gp<const float> lut[3];
   gp<const float> alpha[3];  // learned corrections of the distance-table weight (NULL: table only)
   gp<float> sens[3];         // f_pde - S at frame cells while training (NULL: not recorded)
@@ -102,6 +104,21 @@ __device__ __forceinline__ void make_view(const Shot2D& S, int s, int it, PV& V)
       mat(f, f);
       V.lut[f] = G(S.wlut + 3 * L);
     }
+  } else if (PH == kCpVel) {
+    // P, S through the stencils; dP/dt, dS/dt updated (vx weights); the eight medium arrays
+    sten(0, s, 0);
+    sten(1, s, 1);
+    old_out(0, 2);
+    old_out(1, 3);
+    for (int m = 0; m < 8; ++m) mat(m, m);
+    V.lut[0] = V.lut[1] = G(S.wlut);
+  } else if (PH == kCpPos) {
+    // the new dP/dt, dS/dt at the cell; P, S updated (tau weights)
+    sten(0, o, 2);
+    sten(1, o, 3);
+    old_out(0, 0);
+    old_out(1, 1);
+    V.lut[0] = V.lut[1] = G(S.wlut + 3 * L);
   } else {
     // scalar leapfrog, step it: u^it is read (stencil and old value), u^{it-1} enters as the second
     // per-cell input, u^{it+1} is written; buffers rotate by it mod 3
@@ -139,7 +156,7 @@ struct Phys;
 
 template <int N, class T>
 struct Phys<kAcVel, N, T> {
-  static constexpr int NF = 2, NS = 1, NM = 2;
+  static constexpr int NF = 2, NS = 1, NM = 2, HALO = N;
   template <class R>
   __device__ __forceinline__ static void upd(const R& rd, float dtx, const float* a, int i, int j,
                                              const float* o, const float* m, float* v) {
@@ -157,7 +174,7 @@ struct Phys<kAcVel, N, T> {
 
 template <int N, class T>
 struct Phys<kAcPrs, N, T> {
-  static constexpr int NF = 1, NS = 2, NM = 1;
+  static constexpr int NF = 1, NS = 2, NM = 1, HALO = N;
   template <class R>
   __device__ __forceinline__ static void upd(const R& rd, float dtx, const float* a, int i, int j,
                                              const float* o, const float* m, float* v) {
@@ -174,7 +191,7 @@ struct Phys<kAcPrs, N, T> {
 
 template <int N, class T>
 struct Phys<kElVel, N, T> {
-  static constexpr int NF = 2, NS = 3, NM = 2;
+  static constexpr int NF = 2, NS = 3, NM = 2, HALO = N;
   template <class R>
   __device__ __forceinline__ static void upd(const R& rd, float dtx, const float* a, int i, int j,
                                              const float* o, const float* m, float* v) {
@@ -195,7 +212,7 @@ struct Phys<kElVel, N, T> {
 
 template <int N, class T>
 struct Phys<kElStr, N, T> {
-  static constexpr int NF = 3, NS = 2, NM = 3;
+  static constexpr int NF = 3, NS = 2, NM = 3, HALO = N;
   template <class R>
   __device__ __forceinline__ static void upd(const R& rd, float dtx, const float* a, int i, int j,
                                              const float* o, const float* m, float* v) {
@@ -221,7 +238,7 @@ struct Phys<kElStr, N, T> {
 // u[j+l]) + u[j-l]); o[0] = u^n at the cell, m = {c2, u^{n-1}}; c0 is passed in dtx's place.
 template <int N, class T>
 struct Phys<kScal, N, T> {
-  static constexpr int NF = 1, NS = 1, NM = 2;
+  static constexpr int NF = 1, NS = 1, NM = 2, HALO = N;
   template <class R>
   __device__ __forceinline__ static void upd(const R& rd, float c0, const float* a, int i, int j,
                                              const float* o, const float* m, float* v) {
@@ -235,6 +252,85 @@ struct Phys<kScal, N, T> {
     v[0] = (2.0f * uc - m[1]) + m[0] * lap;
   }
 };
+
+// Per-shot step constants read once per block: Float32(dt/dh), and the coupled equations' dt, 1/dh
+// and (1/dh)^2.
+struct UpdC {
+  float dtx, dt, ih, ih2;
+  __device__ __forceinline__ explicit UpdC(const Shot2D& S) : dtx(S.dtx), dt(S.dt), ih(S.ih), ih2(S.ih2) {}
+};
+
+// Coupled P-S potentials, "velocity" half-step (src/equations/coupled2d/update_fields.jl:27-112):
+// centred first derivatives (weights A.a) and Laplacians (A.a2, centre A.c0) of P and S, then
+//   rhs_P = P lap_alpha + 2 (grad alpha . grad P) - 2 P lap_beta + 2 (dbeta/dx dS/dz - dbeta/dz dS/dx)
+//           + alpha lap P,
+//   rhs_S = 2 (grad beta . grad S) + 2 (dbeta/dz dP/dx - dbeta/dx dP/dz) + beta lap S,
+// dP/dt += dt rhs_P, dS/dt += dt rhs_S, each sum in the reference's left-to-right order.
+// m = {alpha, beta, dalpha/dx, dalpha/dz, dbeta/dx, dbeta/dz, lap alpha, lap beta}.
+template <int N, class T>
+struct Phys<kCpVel, N, T> {
+  static constexpr int NF = 2, NS = 2, NM = 8, HALO = N;
+  template <class R>
+  __device__ __forceinline__ static void upd(const R& rd, const SweepArgs& A, const UpdC& K, int i, int j,
+                                             const float* o, const float* m, float* v) {
+    const float c0 = A.c0;
+    const float Pc = rd(0, i, j), Sc = rd(1, i, j);
+    float dPdx = 0.0f, dPdz = 0.0f, lPx = c0 * Pc, lPz = c0 * Pc;
+    float dSdx = 0.0f, dSdz = 0.0f, lSx = c0 * Sc, lSz = c0 * Sc;
+#pragma unroll
+    for (int l = 1; l <= N; ++l) {
+      const float c1 = A.a[l - 1], c2 = A.a2[l - 1];
+      const float pr = rd(0, i + l, j), pl = rd(0, i - l, j), pd = rd(0, i, j + l), pu = rd(0, i, j - l);
+      const float sr = rd(1, i + l, j), sl = rd(1, i - l, j), sd = rd(1, i, j + l), su = rd(1, i, j - l);
+      dPdx += c1 * (pr - pl);
+      dPdz += c1 * (pd - pu);
+      lPx += c2 * (pr + pl);
+      lPz += c2 * (pd + pu);
+      dSdx += c1 * (sr - sl);
+      dSdz += c1 * (sd - su);
+      lSx += c2 * (sr + sl);
+      lSz += c2 * (sd + su);
+    }
+    const float ih = K.ih, ih2 = K.ih2, dt = K.dt;
+    dPdx = dPdx * ih;
+    dPdz = dPdz * ih;
+    dSdx = dSdx * ih;
+    dSdz = dSdz * ih;
+    const float lapP = (lPx + lPz) * ih2, lapS = (lSx + lSz) * ih2;
+    const float rP = (((Pc * m[6] + 2.0f * (m[2] * dPdx + m[3] * dPdz)) - (2.0f * Pc) * m[7]) +
+                      2.0f * (m[4] * dSdz - m[5] * dSdx)) +
+                     m[0] * lapP;
+    const float rS = (2.0f * (m[4] * dSdx + m[5] * dSdz) + 2.0f * (m[5] * dPdx - m[4] * dPdz)) + m[1] * lapS;
+    v[0] = o[0] + dt * rP;
+    v[1] = o[1] + dt * rS;
+  }
+};
+
+// Coupled "position" half-step (update_fields.jl:124-144): P += dt dP/dt, S += dt dS/dt with the
+// new rates at the cell itself (no stencil: the staged tile is the one-cell ring).
+template <int N, class T>
+struct Phys<kCpPos, N, T> {
+  static constexpr int NF = 2, NS = 2, NM = 0, HALO = 0;
+  template <class R>
+  __device__ __forceinline__ static void upd(const R& rd, const SweepArgs& A, const UpdC& K, int i, int j,
+                                             const float* o, const float* m, float* v) {
+    (void)A;
+    (void)m;
+    v[0] = o[0] + K.dt * rd(0, i, j);
+    v[1] = o[1] + K.dt * rd(1, i, j);
+  }
+};
+
+// FD update of phase P at (i, j): the coupled phases take their weights and steps from the
+// launch and the shot, the others from (dtx or c0, a).
+template <int PH, class P, class R>
+__device__ __forceinline__ void upd_any(const R& rd, const SweepArgs& A, const UpdC& K, int i, int j,
+                                        const float* o, const float* m, float* v) {
+  if constexpr (PH == kCpVel || PH == kCpPos)
+    P::upd(rd, A, K, i, j, o, m, v);
+  else
+    P::upd(rd, PH == kScal ? A.c0 : K.dtx, A.a, i, j, o, m, v);
+}
 
 // Old values and materials of cell c (decoded).  Fields are stored as T, materials as TM.
 template <class P, class T, class TM>
@@ -252,6 +348,7 @@ __device__ __forceinline__ int inj_mask(const Shot2D& S) {
   if (PH == kElVel) return S.src_fz ? 2 : 0;
   if (PH == kElStr) return S.src_fz ? 0 : 3;
   if (PH == kScal) return 1;
+  if (PH == kCpVel) return 1;  // into the new P, which the interior strips of this phase compute
   return 0;
 }
 
@@ -290,20 +387,24 @@ template <int PH, int N, class T, class TM>
 __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, const Shot2D& S, const PV& V,
                                            float* lds, int it) {
   using P = Phys<PH, N, T>;
-  constexpr int NF = P::NF, NS = P::NS, NM = P::NM;
+  constexpr int NF = P::NF, NS = P::NS, NM = P::NM, NMA = NM > 0 ? NM : 1, HN = P::HALO;
   constexpr int BX = kTileW, BZ = kFrameRows, ZPT = BZ / 4, NTH = BX * 4;
-  constexpr int TW = BX + 2 * N + 2, TH = BZ + 2 * N + 2;
+  constexpr int TW = BX + 2 * HN + 2, TH = BZ + 2 * HN + 2;
   constexpr int EW = BX + 2, EH = BZ + 2;
   constexpr int KS = (TW * TH + NTH - 1) / NTH;  // staged cells per thread
   constexpr int KR = (EW * EH + NTH - 1) / NTH;  // tile + ring cells per thread
   float* tile = lds;
   float* pre_s = lds + NS * TW * TH;
   const int nx = S.nx, nz = S.nz, H = S.H;
-  const float dtx = S.dtx, q = S.q, qt = S.qt, qxt = S.qxt;
+  const UpdC K(S);
+  // Higdon coefficients: the coupled S fields have their own reference velocity
+  constexpr bool two_sets = PH == kCpVel || PH == kCpPos;
+  const float qa = S.q, qta = S.qt, qxta = S.qxt;
+  const float qb = two_sets ? S.q2 : qa, qtb = two_sets ? S.qt2 : qta, qxtb = two_sets ? S.qxt2 : qxta;
   const float* damp = S.damp;
   const int i0 = W.i0, j0 = W.j0;
   const int tid = threadIdx.x + threadIdx.y * BX;
-  const int oi = i0 - N - 1, oj = j0 - N - 1;
+  const int oi = i0 - HN - 1, oj = j0 - HN - 1;
   // 1. issue every load
   float st[KS][NS];
 #pragma unroll
@@ -320,7 +421,7 @@ __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, 
 #pragma unroll
     for (int f = 0; f < NS; ++f) st[k][f] = Cvt<T>::ld(V.sten[f], g, V.sdec[f]) * zm;
   }
-  float ro[KR][NF], rm[KR][NM];
+  float ro[KR][NF], rm[KR][NMA];
 #pragma unroll
   for (int k = 0; k < KR; ++k) {
     const int idx = tid + k * NTH;
@@ -382,7 +483,7 @@ __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, 
 #pragma unroll
       for (int f = 0; f < NF; ++f) v[f] = ro[k][f];
     } else {
-      P::upd(rd, PH == kScal ? A.c0 : dtx, A.a, gi, gj, ro[k], rm[k], v);
+      upd_any<PH, P>(rd, A, K, gi, gj, ro[k], rm[k], v);
     }
 #pragma unroll
     for (int f = 0; f < NF; ++f) pre_s[f * EW * EH + idx] = v[f];
@@ -413,6 +514,8 @@ __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, 
       const int di = Lf ? 1 : -1, dj = Tf ? 1 : -1;
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
+        const bool sb = two_sets && f == 1;
+        const float q = sb ? qb : qa, qt = sb ? qtb : qta, qxt = sb ? qxtb : qxta;
         const float foc = fo[k][f];
         float sx = 0.0f, sz = 0.0f;
         if (inx) {
@@ -457,12 +560,12 @@ template <int PH, int N, class T, class TM>
 __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D& W, const Shot2D& S, const PV& V,
                                                float* lds, int it) {
   using P = Phys<PH, N, T>;
-  constexpr int NF = P::NF, NS = P::NS, NM = P::NM;
+  constexpr int NF = P::NF, NS = P::NS, NM = P::NM, NMA = NM > 0 ? NM : 1;
   constexpr int BX = kTileW, RW = BX + 2 * N, RS = StripRing<N>::RS, RSX = StripRing<N>::RSX;
   constexpr int GR = kStripGroup, RPT = GR / 4;
   constexpr int W0 = GR + 2 * N, R0 = (W0 + 3) / 4;  // initial window rows, per thread row
   const int nx = S.nx, nz = S.nz;
-  const float dtx = S.dtx;
+  const UpdC K(S);
   const int i0 = W.i0, j0 = W.j0, jend = W.j0 + W.h;
   const int tx = threadIdx.x, ty = threadIdx.y;
   // Rows of this strip where this lane's column holds a source, scanned before any other load so
@@ -523,12 +626,12 @@ __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D&
   // phase): the interior cells are exactly the strips' cells.
   constexpr int SF = PH == kAcPrs ? 0 : (PH == kElVel ? 1 : -1);
   const int pad = S.H + 1, nxi = nx - 2 * pad;
-  float* snap_k = nullptr;  // snapshot k of this step, or none
-  if (SF >= 0 && S.snap && (it + 1) % S.snap_every == 0 && (it + 1) / S.snap_every <= S.snap_n)
+  float* snap_k = nullptr;  // snapshot k of this step (coupled: of P; of S at the same offset from snap2)
+  if ((SF >= 0 || PH == kCpVel) && S.snap && (it + 1) % S.snap_every == 0 && (it + 1) / S.snap_every <= S.snap_n)
     snap_k = S.snap + (size_t)((it + 1) / S.snap_every - 1) * (size_t)nxi * (size_t)(nz - 2 * pad);
   // Loads of group k: the window rows it adds (j0 + GR*k + N + ty + 4r) and the old values and
   // materials of its rows j0 + GR*k + ty + 4r.
-  float ra[RPT][NS], rb[RPT][NS], no[RPT][NF], nm[RPT][NM];
+  float ra[RPT][NS], rb[RPT][NS], no[RPT][NF], nm[RPT][NMA];
   auto issue = [&](int k) {
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
@@ -537,7 +640,7 @@ __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D&
       load_om<P, T, TM>(V, (col_ok && j < jend) ? (unsigned)(i + j * nx) : 0u, no[r], nm[r]);
     }
   };
-  float co[RPT][NF], cm[RPT][NM];
+  float co[RPT][NF], cm[RPT][NMA];
   {
     // initial window (rows j0-N .. j0+GR-1+N) and the old values / materials of group 0
     float ia[R0][NS], ib[R0][NS];
@@ -562,9 +665,25 @@ __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D&
       if (col_ok && j < jend) {
         float v[NF];
         const StripRd<RW, RSX, N> rd{lds + (int)((unsigned)(j - N - jb) % (unsigned)RS) * RW - oi, j};  // j - N >= jb
-        P::upd(rd, PH == kScal ? A.c0 : dtx, A.a, i, j, co[r], cm[r], v);
+        upd_any<PH, P>(rd, A, K, i, j, co[r], cm[r], v);
         const unsigned c = (unsigned)(i + j * nx);
-        if (j >= src_lo && j <= src_hi) inject<NF>(S, mask, (int)c, it, A.nt, v);
+        if constexpr (PH == kCpVel) {
+          // The position half-step of this interior cell needs only its own new rates (interior
+          // cells take no Higdon blend): P += dt dP/dt, S += dt dS/dt, then the source into P.
+          const int s = A.parity, o = 1 - s;
+          float pn = rd(0, i, j) + K.dt * v[0];
+          const float sn = rd(1, i, j) + K.dt * v[1];
+          if (j >= src_lo && j <= src_hi) inject<1>(S, 1, (int)c, it, A.nt, &pn);
+          st_field<T>(S.fld[o][0], c, pn, S.fenc[0], S.ovf);
+          st_field<T>(S.fld[o][1], c, sn, S.fenc[1], S.ovf);
+          if (snap_k) {
+            const size_t so = (size_t)(i - pad) + (size_t)(j - pad) * nxi;
+            G(snap_k)[so] = pn;
+            G(snap_k + (S.snap2 - S.snap))[so] = sn;
+          }
+        } else {
+          if (j >= src_lo && j <= src_hi) inject<NF>(S, mask, (int)c, it, A.nt, v);
+        }
 #pragma unroll
         for (int f = 0; f < NF; ++f) st_field<T>(V.out[f], c, v[f], V.oenc[f], S.ovf);
         if constexpr (SF >= 0) {
@@ -857,8 +976,8 @@ __device__ __forceinline__ void gather_entry(const GatherArgs& g, int e, int col
 template <int PH, int N, class T, class TM>
 __device__ __forceinline__ void sweep2d_body(const SweepArgs& A) {
   using P = Phys<PH, N, T>;
-  constexpr int NF = P::NF, NS = P::NS;
-  constexpr int FT = NS * (kTileW + 2 * N + 2) * (kFrameRows + 2 * N + 2) + NF * (kTileW + 2) * (kFrameRows + 2);
+  constexpr int NF = P::NF, NS = P::NS, HN = P::HALO;
+  constexpr int FT = NS * (kTileW + 2 * HN + 2) * (kFrameRows + 2 * HN + 2) + NF * (kTileW + 2) * (kFrameRows + 2);
   constexpr int IS = NS * StripRing<N>::RSX * (kTileW + 2 * N);
   __shared__ float lds[FT > IS ? FT : IS];
   const int it = *A.step + A.koff;
@@ -940,6 +1059,16 @@ static hipError_t launch_t2(int phase, int N, const SweepArgs& a, hipStream_t s)
         return hipErrorInvalidValue;
       else
         return launch_phase<kScal, T>(N, a, s);
+    case kCpVel:  // the coupled equations run with Float32 storage only
+      if constexpr (std::is_same<T, float>::value)
+        return launch_phase<kCpVel, T>(N, a, s);
+      else
+        return hipErrorInvalidValue;
+    case kCpPos:
+      if constexpr (std::is_same<T, float>::value)
+        return launch_phase<kCpPos, T>(N, a, s);
+      else
+        return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }
 }

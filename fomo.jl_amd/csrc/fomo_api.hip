@@ -84,6 +84,25 @@ extern "C" int fomo_centered_d2(int32_t fd_order, float* c0, float* d2) {
   return FOMO_OK;
 }
 
+// Centred first-derivative weights, src/utils/FD_centered.jl:14-20 (Float64 ratios rounded to
+// Float32, as Julia's Float32[...] literal rounds them).
+static const double kD1[5][5] = {
+    {1.0 / 2.0, 0, 0, 0, 0},
+    {2.0 / 3.0, -1.0 / 12.0, 0, 0, 0},
+    {3.0 / 4.0, -3.0 / 20.0, 1.0 / 60.0, 0, 0},
+    {4.0 / 5.0, -1.0 / 5.0, 4.0 / 105.0, -1.0 / 280.0, 0},
+    {5.0 / 6.0, -5.0 / 21.0, 5.0 / 84.0, -5.0 / 504.0, 1.0 / 1260.0},
+};
+
+extern "C" int fomo_centered_d1(int32_t fd_order, float* d1) {
+  if (fd_order < 2 || fd_order > 10 || (fd_order & 1))
+    return fail(FOMO_EINVAL, "Unsupported FD order: " + std::to_string(fd_order) + " (centered d1)");
+  if (!d1) return fail(FOMO_EINVAL, "NULL argument");
+  const int N = fd_order / 2;
+  for (int l = 0; l < N; ++l) d1[l] = (float)kD1[N - 1][l];
+  return FOMO_OK;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Ricker wavelet, src/utils/ricker_wavelet.jl:6-12:
 //   t = (0:nt-1) .* dt; tau = t .- 1.5f0/f0; u = pi*f0 .* tau; w = (1 - 2u^2) exp(-u^2).
@@ -411,19 +430,21 @@ extern "C" int fomo_batch_create(const fomo_desc* d, const float* vp, const floa
   *out = nullptr;
   int rc = validate_common(d);
   if (rc) return rc;
-  if (d->eq != FOMO_ACOUSTIC2D && d->eq != FOMO_ELASTIC2D)
-    return fail(FOMO_EUNSUP, "batches are 2D (acoustic2d, elastic2d)");
+  if (d->eq != FOMO_ACOUSTIC2D && d->eq != FOMO_ELASTIC2D && d->eq != FOMO_COUPLED2D)
+    return fail(FOMO_EUNSUP, "batches are 2D (acoustic2d, elastic2d, coupled2d)");
   Plan2D* p = nullptr;
   if ((rc = create_plan2d(d, vp, vs, rho, n_shots, shots, &p))) return rc;
   *out = new fomo_plan{p};
   return FOMO_OK;
 }
 
-// A single-shot 2D plan of any 2D equation (acoustic2d, elastic2d, scalar2d).
+// A single-shot 2D plan of any 2D equation (acoustic2d, elastic2d, scalar2d, coupled2d).
 static int create_single2d(const fomo_desc* d, const float* vp, const float* vs, const float* rho, const fomo_shot* s,
                            fomo_plan** out) {
-  if (d->eq == FOMO_SCALAR2D) {
-    if (d->boundary != FOMO_BOUNDARY_HABC) return fail(FOMO_EUNSUP, "scalar2d uses the HABC (as in the reference)");
+  if (d->eq == FOMO_SCALAR2D || d->eq == FOMO_COUPLED2D) {
+    if (d->boundary != FOMO_BOUNDARY_HABC)
+      return fail(FOMO_EUNSUP, std::string(d->eq == FOMO_SCALAR2D ? "scalar2d" : "coupled2d") +
+                                   " uses the HABC (as in the reference)");
     Plan2D* p = nullptr;
     const int rc = create_plan2d(d, vp, vs, rho, 1, s, &p);
     if (rc) return rc;
@@ -446,7 +467,7 @@ extern "C" int fomo_plan_create(const fomo_desc* d, const float* vp, const float
   if (rc) return rc;
   if (d->eq == FOMO_ACOUSTIC3D || d->eq == FOMO_ELASTIC3D)
     return fomo_plan_create3d(d, vp, vs, rho, sx, sy, sz, wavelet, rx, ry, rz, out);
-  if (d->eq != FOMO_ACOUSTIC2D && d->eq != FOMO_ELASTIC2D && d->eq != FOMO_SCALAR2D)
+  if (d->eq != FOMO_ACOUSTIC2D && d->eq != FOMO_ELASTIC2D && d->eq != FOMO_SCALAR2D && d->eq != FOMO_COUPLED2D)
     return fail(FOMO_EINVAL, "unknown equation " + std::to_string(d->eq));
   fomo_shot s{};
   s.x0 = 1;
@@ -710,6 +731,30 @@ extern "C" int fomo_scalar2d(const fomo_desc* d, const float* vp, const int32_t*
   if (!d || d->eq != FOMO_SCALAR2D) return fail(FOMO_EINVAL, "descriptor must have eq = FOMO_SCALAR2D");
   if (d->snap_interval != 0) return fail(FOMO_EINVAL, "scalar2d takes no snapshots (as in the reference)");
   return run_2d(d, vp, nullptr, nullptr, sx, sz, wavelet, rx, rz, seis_u, nullptr, nullptr, nullptr, kernel_time_s);
+}
+
+extern "C" int fomo_coupled2d(const fomo_desc* d, const float* vp, const float* vs, const int32_t* sx,
+                              const int32_t* sz, const float* wavelet, const int32_t* rx, const int32_t* rz,
+                              float* seis_P, float* seis_S, float* snaps_P, float* snaps_S, double* kernel_time_s) {
+  if (!d || d->eq != FOMO_COUPLED2D) return fail(FOMO_EINVAL, "descriptor eq must be FOMO_COUPLED2D");
+  fomo_plan* p = nullptr;
+  int rc = fomo_plan_create(d, vp, vs, nullptr, sx, nullptr, sz, wavelet, rx, nullptr, rz, &p);
+  if (rc) return rc;
+  rc = timed_run(p, d, nullptr, kernel_time_s);
+  if (!rc) rc = fomo_plan_read_seis(p, seis_P, seis_S, nullptr, nullptr);
+  if (!rc && d->snap_interval > 0 && (snaps_P || snaps_S)) {
+    // the plan's snapshot slots: every P snapshot, then every S snapshot
+    const Shot2D& sh = static_cast<Plan2D*>(p->impl)->shots[0];
+    const size_t per = (size_t)sh.snap_n * d->nx * d->nz;
+    if (per && !sh.snap) rc = fail(FOMO_EHIP, "snapshot buffer missing");
+    hipError_t e = hipSuccess;
+    if (!rc && per && snaps_P) e = hipMemcpy(snaps_P, sh.snap, per * sizeof(float), hipMemcpyDeviceToHost);
+    if (!rc && per && snaps_S && e == hipSuccess)
+      e = hipMemcpy(snaps_S, sh.snap2, per * sizeof(float), hipMemcpyDeviceToHost);
+    if (!rc && e != hipSuccess) rc = fail(FOMO_EHIP, std::string("snapshots: ") + hipGetErrorString(e));
+  }
+  fomo_plan_destroy(p);
+  return rc;
 }
 
 // Reference batch form: every shot spans the whole model and shares the receivers and wavelet.

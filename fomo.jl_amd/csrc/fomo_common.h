@@ -32,7 +32,9 @@ __device__ __forceinline__ gp<T> G(T* p) {
 }
 
 // Sweep phases.
-enum Phase : int { kAcVel = 0, kAcPrs = 1, kElVel = 2, kElStr = 3, kScal = 4 };
+// kCpVel / kCpPos: the coupled P-S potential equations' "velocity" (dP/dt, dS/dt) and "position"
+// (P, S) half-steps.
+enum Phase : int { kAcVel = 0, kAcPrs = 1, kElVel = 2, kElStr = 3, kScal = 4, kCpVel = 5, kCpPos = 6 };
 
 // One shot of a 2D run (device-resident table entry).  Velocity-stress equations keep each field
 // in two buffers (ping-pong, fld[s][f]); the second-order scalar equation keeps u in three buffers
@@ -41,12 +43,16 @@ enum Phase : int { kAcVel = 0, kAcPrs = 1, kElVel = 2, kElStr = 3, kScal = 4 };
 // Fields and materials are stored as the plan's storage type (Float32, Float16 or BFloat16); a
 // 16-bit value v is stored as RN(v * enc) and read back as float(stored) * dec.
 struct Shot2D {
-  void* fld[2][5];      // ping-pong fields. acoustic: p, vx, vz; elastic: vx, vz, txx, tzz, txz
-  const void* mat[5];   // acoustic: kappa, bx, bz; elastic: lam, lam2mu, muxz, bx, bz
-  float fdec[5], fenc[5], mdec[5];
+  void* fld[2][5];      // ping-pong fields. acoustic: p, vx, vz; elastic: vx, vz, txx, tzz, txz;
+                        // coupled: P, S, dP/dt, dS/dt
+  const void* mat[8];   // acoustic: kappa, bx, bz; elastic: lam, lam2mu, muxz, bx, bz; coupled: alpha,
+                        // beta, d alpha/dx, d alpha/dz, d beta/dx, d beta/dz, lap alpha, lap beta
+  float fdec[5], fenc[5], mdec[8];
   const float* wlut;    // HABC weights by distance to the edge: 4 tables of H+1 (vx, vz, vy, tau)
   float dtx;            // Float32(dt/dh)
-  float q, qt, qxt;     // Higdon coefficients
+  float q, qt, qxt;     // Higdon coefficients (coupled: of the P fields)
+  float q2, qt2, qxt2;  // coupled: Higdon coefficients of the S fields
+  float dt, ih, ih2;    // coupled: dt, 1/dh and (1/dh)^2 of the velocity half-step
   int nx, nz;           // padded dims
   int H;                // HABC frame width (pad - 1)
   int n_src;
@@ -64,6 +70,7 @@ struct Shot2D {
   // Snapshots (NULL: none): after every snap_every-th step, the interior of p (acoustic) or vz
   // (elastic) as Float32 into snap[k] (x fastest), k = steps / snap_every - 1 < snap_n.
   float* snap;
+  float* snap2;         // coupled: snapshots of S (snap holds P)
   int snap_every, snap_n;
   int nbuf;             // 2: ping-pong fields; 3: scalar u^{k+1}, u^k, u^{k-1} rotation
   unsigned* ovf;        // 16-bit storage: set to 1 when a stored value overflowed (see st_field)
@@ -97,11 +104,12 @@ struct SweepArgs {
   const Work2D* work;
   int n_work;
   int parity;           // state parity s: read buffers [s], write buffers [1-s]
-  float a[kMaxN];       // staggered FD coefficients
+  float a[kMaxN];       // staggered FD coefficients (coupled: centred first-derivative weights)
+  float a2[kMaxN];      // coupled: centred second-derivative weights
   const int* step;      // device step counter
   int koff;             // offset of this launch's step from the counter
   int nt;
-  float c0;             // scalar equation: centre coefficient of the second derivative
+  float c0;             // scalar and coupled equations: centre coefficient of the second derivative
   // Velocity phase only: blocks n_work .. n_work + n_gather - 1 record the state held in buffer
   // [s] (the state after step it - 1) into column it - 1.
   GatherArgs g;

@@ -19,8 +19,13 @@ struct Plan2D : PlanBase {
   int sweep_storage = FOMO_STORE_F32;      // the requested code, which selects the sweep kernels
   float* scratch = nullptr;                // Float32 staging of one decoded field
   size_t scratch_n = 0;
-  float coef[kMaxN] = {0};  // staggered first-derivative (scalar: centred second-derivative) weights
-  float c0 = 0.0f;           // scalar: centre weight of the second derivative
+  float coef[kMaxN] = {0};  // staggered first-derivative (scalar: centred second-derivative; coupled:
+                            // centred first-derivative) weights
+  float coef2[kMaxN] = {0}; // coupled: centred second-derivative weights
+  float c0 = 0.0f;           // scalar, coupled: centre weight of the second derivative
+  // coupled2d: the velocity launch covers the frame tiles and the interior strips (whose cells also
+  // take their position half-step there); the position launch covers the frame tiles (d_frame).
+  bool coupled = false;
   std::vector<Shot2D> shots;
   std::vector<ShotMeta> meta;
   Shot2D* d_shots = nullptr;
@@ -67,7 +72,8 @@ struct Plan2D : PlanBase {
   int snap_buffer(int shot, const float** ptr, int* count, size_t* cells) const override {
     if (shot < 0 || shot >= (int)shots.size()) return set_error(FOMO_EINVAL, "shot index out of range");
     *ptr = shots[shot].snap;
-    *count = shots[shot].snap ? shots[shot].snap_n : 0;
+    // coupled: the P slots, then the S slots (snap2 follows snap in one allocation)
+    *count = shots[shot].snap ? shots[shot].snap_n * (coupled ? 2 : 1) : 0;
     *cells = (size_t)meta[shot].nxi * meta[shot].nzi;
     return FOMO_OK;
   }
@@ -95,5 +101,8 @@ struct Plan2D : PlanBase {
 
 int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const float* rho, int n_shots,
                   const fomo_shot* shots, Plan2D** out);
+// The coupled2d medium on the device (coupled2d.hip): 8 new plan allocations in out.
+int coupled_medium2d(PlanBase* p, const std::vector<float>& vpp, const std::vector<float>& vsp, int nx, int nz,
+                     int fd_order, float dh, double sigma, float* out[8]);
 
 }  // namespace fomo

@@ -91,6 +91,15 @@ static int fused_rows(int cfg) {
 
 int Plan2D::make_work(int cfg) {
   if (fused ? d_fwork[cfg] != nullptr : d_work[cfg] != nullptr) return FOMO_OK;
+  if (coupled && !d_frame) {
+    std::vector<Work2D> frame;
+    for (const Work2D& w : work_items(shots, meta, 0))
+      if (w.kind == 0) frame.push_back(w);
+    const std::vector<Work2D> ff = xcd_layout(frame);
+    n_frame = (int)ff.size();
+    const int rc = upload(&d_frame, ff.data(), ff.size());
+    if (rc) return rc;
+  }
   if (fused) {
     std::vector<Work2D> frame, inner;
     for (const Work2D& w : work_items(shots, meta, 0))
@@ -140,6 +149,7 @@ hipError_t Plan2D::enqueue_part(int part, int s, int koff) {
   a.koff = koff;
   a.nt = d.nt;
   a.c0 = c0;
+  for (int l = 0; l < N; ++l) a.a2[l] = coef2[l];
   if (part == 0) {
     a.g = g;
     a.n_gather = (n_entries + kGatherBlock - 1) / kGatherBlock;
@@ -150,6 +160,10 @@ hipError_t Plan2D::enqueue_part(int part, int s, int koff) {
     a.work = d_frame;
     a.n_work = n_frame;
     if (part == 1) return only == 0 ? hipSuccess : launch_sweep2d(sweep_storage, kAcPrs, N, tile_cfg, a, stream);
+  } else if (part == 1 && coupled) {
+    a.work = d_frame;  // the interior's position half-steps ran in the velocity launch
+    a.n_work = n_frame;
+    return launch_sweep2d(sweep_storage, kCpPos, N, tile_cfg, a, stream);
   } else if (part == 1) {
     return launch_sweep2d(sweep_storage, d.eq == FOMO_ACOUSTIC2D ? kAcPrs : kElStr, N, tile_cfg, a, stream);
   }
@@ -175,7 +189,7 @@ hipError_t Plan2D::enqueue_part(int part, int s, int koff) {
     const hipError_t e = launch_mlp_forward(m, stream);
     if (e != hipSuccess) return e;
   }
-  const int ph = d.eq == FOMO_ACOUSTIC2D ? kAcVel : (d.eq == FOMO_ELASTIC2D ? kElVel : kScal);
+  const int ph = d.eq == FOMO_ACOUSTIC2D ? kAcVel : (d.eq == FOMO_ELASTIC2D ? kElVel : (coupled ? kCpVel : kScal));
   if (!fused) return launch_sweep2d(sweep_storage, ph, N, tile_cfg, a, stream);
   if (only != 0) {
     const hipError_t e = launch_sweep2d(sweep_storage, ph, N, tile_cfg, a, stream);
@@ -201,6 +215,15 @@ PlanBase::LaunchInfo Plan2D::launch_info(int k) const {
     if (k == 0) return {"acoustic2d velocity frame", cells - inner, 7 * elem};
     if (k == 1) return {"acoustic2d interior step (velocity + pressure)", inner, 12 * elem};
     return {"acoustic2d pressure frame", cells - inner, 5 * elem};
+  }
+  if (coupled) {
+    // velocity half-step: P, S, dP/dt, dS/dt and the eight medium arrays -> dP/dt, dS/dt (14 words);
+    // position half-step: P, S, dP/dt, dS/dt -> P, S (6 words).  The velocity launch also takes
+    // the interior's position half-steps; only its velocity words are credited to it.
+    long long inner = 0;
+    for (const auto& m : meta) inner += (long long)m.nxi * m.nzi;
+    if (k == 0) return {"coupled2d velocity sweep (interior position half-step not credited)", cells, 14 * elem};
+    return {"coupled2d position frame", cells - inner, 6 * elem};
   }
   if (d.eq == FOMO_ACOUSTIC2D) {
     words = k == 0 ? 7 : 5;
@@ -228,7 +251,9 @@ int Plan2D::reset() {
   for (size_t k = 0; k < shots.size(); ++k)
     if (shots[k].snap)
       FOMO_HIP_TRY(hipMemsetAsync(shots[k].snap, 0,
-                                  (size_t)shots[k].snap_n * meta[k].nxi * meta[k].nzi * sizeof(float), stream));
+                                  (size_t)shots[k].snap_n * (coupled ? 2 : 1) * meta[k].nxi * meta[k].nzi *
+                                      sizeof(float),
+                                  stream));
   FOMO_HIP_TRY(hipMemsetAsync(step, 0, sizeof(int), stream));
   if (ovf) FOMO_HIP_TRY(hipMemsetAsync(ovf, 0, sizeof(unsigned), stream));
   steps_done = 0;
@@ -387,11 +412,11 @@ static int check_pts(const char* what, int n, const int32_t* x, const int32_t* z
 int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const float* rho, int n_shots,
                   const fomo_shot* sh, Plan2D** out) {
   *out = nullptr;
-  if (d->eq != FOMO_ACOUSTIC2D && d->eq != FOMO_ELASTIC2D && d->eq != FOMO_SCALAR2D)
+  if (d->eq != FOMO_ACOUSTIC2D && d->eq != FOMO_ELASTIC2D && d->eq != FOMO_SCALAR2D && d->eq != FOMO_COUPLED2D)
     return set_error(FOMO_EINVAL, "not a 2D equation");
-  const bool scalar = d->eq == FOMO_SCALAR2D;
+  const bool scalar = d->eq == FOMO_SCALAR2D, coupled = d->eq == FOMO_COUPLED2D;
   if (n_shots < 1) return set_error(FOMO_EINVAL, "at least one shot is required");
-  if (!vp || (!scalar && !rho) || (d->eq == FOMO_ELASTIC2D && !vs))
+  if (!vp || (!scalar && !coupled && !rho) || ((d->eq == FOMO_ELASTIC2D || coupled) && !vs))
     return set_error(FOMO_EINVAL, "model arrays must not be NULL");
   if (!sh) return set_error(FOMO_EINVAL, "shot table is NULL");
   for (int s = 0; s < n_shots; ++s) {
@@ -405,6 +430,9 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
   if (d->storage < FOMO_STORE_F32 || d->storage > FOMO_STORE_F32_MAT16) return set_error(FOMO_EINVAL, "unknown storage");
   if (d->storage == FOMO_STORE_F32_MAT16 && scalar)
     return set_error(FOMO_EINVAL, "Float16 materials need a velocity-stress equation (acoustic2d, elastic2d)");
+  if (coupled && d->storage != FOMO_STORE_F32) return set_error(FOMO_EUNSUP, "coupled2d runs with Float32 storage");
+  if (coupled && d->boundary != FOMO_BOUNDARY_HABC)
+    return set_error(FOMO_EUNSUP, "coupled2d uses the HABC (as in the reference)");
   Plan2D* p = new Plan2D();
   p->d = *d;
   const bool mat16 = d->storage == FOMO_STORE_F32_MAT16;
@@ -416,11 +444,17 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
   p->fusable = d->eq == FOMO_ACOUSTIC2D;
   p->elem = p->storage == FOMO_STORE_F32 ? 4 : 2;
   p->N = d->fd_order / 2;
-  if (scalar)
+  p->coupled = coupled;
+  if (scalar) {
     fomo_centered_d2(d->fd_order, &p->c0, p->coef);
-  else
+  } else if (coupled) {
+    fomo_centered_d1(d->fd_order, p->coef);
+    fomo_centered_d2(d->fd_order, &p->c0, p->coef2);
+  } else {
     fomo_fd_coefficients(d->fd_order, p->coef);
-  p->nfld = d->eq == FOMO_ELASTIC2D ? 5 : 3;  // scalar: the three u buffers
+  }
+  // scalar: the three u buffers; coupled: P, S, dP/dt, dS/dt
+  p->nfld = d->eq == FOMO_ELASTIC2D ? 5 : (coupled ? 4 : 3);
   p->nslots = d->eq == FOMO_ACOUSTIC2D ? 3 : (scalar ? 1 : 2);
   auto bail = [&](int code) {
     delete p;
@@ -462,11 +496,14 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
     x.dtx = dtx;
     x.damp = d_damp;
     if (d->snap_interval > 0 && n_shots == 1 && d->nt / d->snap_interval > 0) {
-      // snapshots of the interior, filled by the sweeps (no host copy inside the step loop)
+      // snapshots of the interior, filled by the sweeps (no host copy inside the step loop);
+      // coupled: the P slots, then the S slots
       x.snap_every = d->snap_interval;
       x.snap_n = d->nt / d->snap_interval;
-      if ((rc = p->alloc(&x.snap, (size_t)x.snap_n * S.nx * d->nz))) return bail(rc);
-      if (hipMemset(x.snap, 0, (size_t)x.snap_n * S.nx * d->nz * sizeof(float)) != hipSuccess)
+      const size_t per = (size_t)x.snap_n * S.nx * d->nz, tot = per * (coupled ? 2 : 1);
+      if ((rc = p->alloc(&x.snap, tot))) return bail(rc);
+      if (coupled) x.snap2 = x.snap + per;
+      if (hipMemset(x.snap, 0, tot * sizeof(float)) != hipSuccess)
         return bail(set_error(FOMO_EHIP, "snapshot buffer: hipMemset failed"));
     }
     const size_t n = (size_t)x.nx * x.nz;
@@ -480,9 +517,29 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
     }
     float hc[3];
     fomo_habc_coefficients(v_ref, d->dt, d->dh, hc);
-    x.q = hc[0];
-    x.qt = hc[1];
-    x.qxt = hc[2];
+    x.q = x.q2 = hc[0];
+    x.qt = x.qt2 = hc[1];
+    x.qxt = x.qxt2 = hc[2];
+    // window's vs, for the coupled S fields' reference velocity (coupled2d.jl:66)
+    float v_ref_s = 0.0f;
+    if (coupled) {
+      v_ref_s = d->v_ref_s;
+      if (!(v_ref_s > 0.0f)) {
+        std::vector<float> crop((size_t)S.nx * d->nz);
+        for (int j = 0; j < d->nz; ++j)
+          std::memcpy(&crop[(size_t)j * S.nx], vs + (size_t)(S.x0 - 1) + (size_t)j * d->nx, S.nx * sizeof(float));
+        v_ref_s = min_positive(crop.data(), crop.size());
+        if (!(v_ref_s > 0.0f)) return bail(set_error(FOMO_EINVAL, "vs has no positive entry in shot window"));
+      }
+      fomo_habc_coefficients(v_ref_s, d->dt, d->dh, hc);
+      x.q2 = hc[0];
+      x.qt2 = hc[1];
+      x.qxt2 = hc[2];
+      // update_fields.jl:161-162: ih = 1/dh, ih2 = ih*ih
+      x.dt = d->dt;
+      x.ih = 1.0f / d->dh;
+      x.ih2 = x.ih * x.ih;
+    }
     // storage scales.  Sources into p or stresses: p / stresses in units of the shot's peak source
     // amplitude, velocities in that over the smallest impedance rho*vp of the window (cells with
     // vp, rho > 0).  Vertical force into vz: velocities in units of the peak amplitude, stresses
@@ -494,7 +551,7 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
       for (int j = 0; j < d->nz; ++j)
         for (int i = 0; i < S.nx; ++i) {
           const size_t c = (size_t)(S.x0 - 1 + i) + (size_t)j * d->nx;
-          if (!scalar && vp[c] > 0.0f && rho[c] > 0.0f) {
+          if (!scalar && !coupled && vp[c] > 0.0f && rho[c] > 0.0f) {
             zmin = std::min(zmin, rho[c] * vp[c]);
             zmax = std::max(zmax, rho[c] * vp[c]);
           }
@@ -521,7 +578,7 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
     auto mit = medium_of.find(key);
     if (mit != medium_of.end()) {
       const Shot2D& o = p->shots[mit->second];
-      for (int k = 0; k < 5; ++k) {
+      for (int k = 0; k < 8; ++k) {
         x.mat[k] = o.mat[k];
         x.mdec[k] = o.mdec[k];
       }
@@ -529,18 +586,28 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
     } else {
       std::vector<float> vpp, rhop, bx, bz;
       pad2d(vp, S.nx, d->nz, pad, S.x0 - 1, d->nx, vpp);
-      if (!scalar) {
+      if (!scalar && !coupled) {
         pad2d(rho, S.nx, d->nz, pad, S.x0 - 1, d->nx, rhop);
         buoyancy2d(rhop, x.nx, x.nz, bx, bz);
       }
-      void* dm[5] = {nullptr};
+      void* dm[8] = {nullptr};
       // material m, stored with scale max |value| when 16-bit
       auto put = [&](int m, const std::vector<float>& h) -> int {
         const float mx = max_abs(h.data(), h.size());
         x.mdec[m] = p->mat_storage != FOMO_STORE_F32 && mx > 0.0f ? mx : 1.0f;
         return upload_stored(p, p->mat_storage, &dm[m], h.data(), h.size(), 1.0f / x.mdec[m]);
       };
-      if (scalar) {
+      if (coupled) {
+        // alpha, beta and the smoothed-medium derivatives, built on the device (coupled2d.hip)
+        std::vector<float> vsp;
+        pad2d(vs, S.nx, d->nz, pad, S.x0 - 1, d->nx, vsp);
+        float* cm[8] = {nullptr};
+        if ((rc = coupled_medium2d(p, vpp, vsp, x.nx, x.nz, d->fd_order, d->dh, d->smooth_sigma, cm))) return bail(rc);
+        for (int k = 0; k < 8; ++k) {
+          dm[k] = cm[k];
+          x.mdec[k] = 1.0f;
+        }
+      } else if (scalar) {
         // c2 = (v * Float32(dt/dh))^2 on the edge-padded model, scalar2d.jl:126-128
         std::vector<float> c2(n);
         for (size_t c = 0; c < n; ++c) {
@@ -559,7 +626,7 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
         if ((rc = put(0, lam)) || (rc = put(1, l2m)) || (rc = put(2, muxz)) || (rc = put(3, bx)) || (rc = put(4, bz)))
           return bail(rc);
       }
-      for (int k = 0; k < 5; ++k) x.mat[k] = dm[k];
+      for (int k = 0; k < 8; ++k) x.mat[k] = dm[k];
       std::vector<float> lut(4 * (size_t)(H + 1));
       weight_lut(H, pad, 0.0, lut.data());
       weight_lut(H, pad, 0.5, lut.data() + (H + 1));

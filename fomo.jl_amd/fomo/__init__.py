@@ -4,11 +4,12 @@ Python mirror of the reference's public API over the native library libfomo_hip.
 no CPU fallback: every solver call runs on the GPU through the C-ABI in include/fomo_hip.h.
 """
 from ._lib import ArgumentError, FomoError  # noqa: F401
-from .api import (Plan, acoustic2d, acoustic2d_batch, acoustic3d, elastic2d, elastic2d_batch,  # noqa: F401
-                  elastic3d, scalar2d)
-from .setup import get_centered_d2, get_fd_coefficients, habc_coefficients, ricker_wavelet  # noqa: F401
+from .api import (Plan, acoustic2d, acoustic2d_batch, acoustic3d, coupled2d, elastic2d,  # noqa: F401
+                  elastic2d_batch, elastic3d, scalar2d)
+from .setup import (get_centered_d1, get_centered_d2, get_fd_coefficients, habc_coefficients,  # noqa: F401
+                    ricker_wavelet)
 from .traces import trace_norm  # noqa: F401
 
 __all__ = ["acoustic2d", "elastic2d", "acoustic2d_batch", "elastic2d_batch", "acoustic3d", "elastic3d",
-           "scalar2d", "ricker_wavelet", "trace_norm",
-           "get_fd_coefficients", "get_centered_d2", "habc_coefficients", "Plan", "ArgumentError", "FomoError"]
+           "scalar2d", "coupled2d", "ricker_wavelet", "trace_norm",
+           "get_fd_coefficients", "get_centered_d1", "get_centered_d2", "habc_coefficients", "Plan", "ArgumentError", "FomoError"]

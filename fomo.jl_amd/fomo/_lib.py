@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "libfomo_hip.so"))
 
 FOMO_OK, FOMO_EINVAL, FOMO_EHIP, FOMO_ENOMEM, FOMO_EUNSUP, FOMO_EOVERFLOW = 0, 1, 2, 3, 4, 5
-ACOUSTIC2D, ELASTIC2D, ACOUSTIC3D, ELASTIC3D, SCALAR2D = 1, 2, 3, 4, 5
+ACOUSTIC2D, ELASTIC2D, ACOUSTIC3D, ELASTIC3D, SCALAR2D, COUPLED2D = 1, 2, 3, 4, 5, 6
 SRC_EXPLOSIVE, SRC_FZ = 0, 1
 STORE_F32, STORE_F16, STORE_BF16, STORE_F32_MAT16 = 0, 1, 2, 3
 BOUNDARY_HABC, BOUNDARY_SPONGE = 0, 1
@@ -40,7 +40,8 @@ class Desc(C.Structure):
         ("n_src", C.c_int32), ("n_rec", C.c_int32), ("source_type", C.c_int32), ("storage", C.c_int32),
         ("snap_interval", C.c_int32), ("snap_plane", C.c_int32), ("snap_index", C.c_int32),
         ("use_graph", C.c_int32), ("record_p", C.c_int32), ("tile_cfg", C.c_int32),
-        ("boundary", C.c_int32), ("sponge_factor", C.c_float), ("reserved", C.c_int32 * 4),
+        ("boundary", C.c_int32), ("sponge_factor", C.c_float), ("smooth_sigma", C.c_double),
+        ("v_ref_s", C.c_float), ("reserved", C.c_int32 * 1),
     ]
 
 
@@ -65,6 +66,9 @@ SIGNATURES = {
     "fomo_stream_probe": (C.c_int, [_i, _i, C.c_int64, _i, C.POINTER(C.c_double)]),
     "fomo_fd_coefficients": (C.c_int, [_i, _p]),
     "fomo_centered_d2": (C.c_int, [_i, _p, _p]),
+    "fomo_centered_d1": (C.c_int, [_i, _p]),
+    "fomo_coupled2d": (C.c_int, [C.POINTER(Desc), _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p,
+                                 C.POINTER(C.c_double)]),
     "fomo_scalar2d": (C.c_int, [C.POINTER(Desc), _p, _p, _p, _p, _p, _p, _p, C.POINTER(C.c_double)]),
     "fomo_ricker_wavelet": (C.c_int, [_f, _f, _i, _p]),
     "fomo_habc_coefficients": (C.c_int, [_f, _f, _f, _p]),

@@ -3,6 +3,7 @@
 Same names, keywords, defaults, validation and result fields as the reference:
   acoustic2d  src/equations/acoustic2d/acoustic2d.jl:47-192
   elastic2d   src/equations/elastic2d/elastic2d.jl:45-163
+  coupled2d   src/equations/coupled2d/coupled2d.jl:53-183
 Models are numpy arrays indexed [i, j] (i along x) like the reference's Julia matrices; source
 and receiver indices are 1-based.  Seismograms come back as (n_rec, nt) arrays, snapshots as a list
 of (nx, nz) arrays.
@@ -23,6 +24,7 @@ Stats = namedtuple("Stats", ["kernel_time_s"])
 Acoustic2DResult = namedtuple("Acoustic2DResult", ["seis_p", "seis_vx", "seis_vz", "snaps", "stats"])
 Elastic2DResult = namedtuple("Elastic2DResult", ["seis_vx", "seis_vz", "snaps", "stats"])
 Scalar2DResult = namedtuple("Scalar2DResult", ["seis_u", "stats"])
+Coupled2DResult = namedtuple("Coupled2DResult", ["seis_P", "seis_S", "snaps_P", "snaps_S", "stats"])
 
 
 def _sym(x) -> str:
@@ -245,6 +247,58 @@ def scalar2d(vp, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_order=8, v_ref=No
     return Scalar2DResult(su, Stats(float(t.value)))
 
 
+def coupled2d(vp, vs, dh, dt, nt, f0, *, sx, sz, rx, rz, nbc=50, fd_order=8, snap_interval=0, v_ref_p=None,
+              v_ref_s=None, smooth_sigma=3.0, wavelet=None, verbose=True, use_cuda_graph=True):
+    """coupled2d(vp, vs, dh, dt, nt, f0; sx, sz, rx, rz, ...) -> (seis_P, seis_S, snaps_P, snaps_S, stats).
+
+    src/equations/coupled2d/coupled2d.jl:53-183: the coupled P-S potential equations (constant
+    density) as a velocity-position split with a Higdon blend after each half-step.  v_ref_p /
+    v_ref_s (defaults min(vp > 0) / min(vs > 0)) set the P and S fields' Higdon coefficients;
+    smooth_sigma (grid points, 0 disables) smooths alpha = vp^2 and beta = vs^2 for the coupling
+    terms.  The wavelet (Ricker(f0) by default) is added raw to P.  Extension: `use_cuda_graph`."""
+    del verbose
+    if nbc < 0 or nt < 0 or snap_interval < 0:
+        raise ArgumentError("nbc, nt and snap_interval must be >= 0")
+    if fd_order not in (2, 4, 6, 8, 10):
+        raise ValueError(f"Unsupported FD order: {fd_order} (centered d1)")
+    vp, vs = _f32F(vp), _f32F(vs)
+    if vp.ndim != 2 or vp.shape != vs.shape:
+        raise ArgumentError("vp and vs must be matrices of the same size")
+    nx, nz = vp.shape
+    check_geometry(nx, nz, sx, sz, rx, rz)
+    vpmin = _default_vref(vp)
+    vs_pos = vs[vs > np.float32(0)]
+    vmin_disp = float(np.float32(vs_pos.min())) if vs_pos.size else vpmin
+    check_numerics_centered(float(vp.max()), vmin_disp, dh, dt, f0, fd_order)
+    v_ref_p = vpmin if v_ref_p is None else float(np.float32(v_ref_p))
+    if v_ref_s is None:
+        if vs_pos.size == 0:
+            raise ArgumentError("vs has no positive entry (v_ref_s defaults to min(vs > 0))")
+        v_ref_s = float(np.float32(vs_pos.min()))
+    else:
+        v_ref_s = float(np.float32(v_ref_s))
+    sx, sz, rx, rz = _i32(sx), _i32(sz), _i32(rx), _i32(rz)
+    if len(sx) != len(sz) or len(rx) != len(rz):
+        raise ArgumentError("source / receiver coordinate vectors differ in length")
+    wm = wavelet_matrix(wavelet, f0, dt, nt, len(sx), False)
+    n_rec = len(rx)
+    d = _desc(_lib.COUPLED2D, nx, nz, nbc, fd_order, nt, dh, dt, v_ref_p, len(sx), n_rec, snap_interval,
+              use_cuda_graph)
+    d.v_ref_s = v_ref_s
+    d.smooth_sigma = float(smooth_sigma)
+    sP, sS = (np.zeros((n_rec, nt), np.float32, order="F") for _ in range(2))
+    nsnap = nt // snap_interval if snap_interval > 0 else 0
+    snP = np.zeros((nsnap, nz, nx), np.float32) if nsnap else None
+    snS = np.zeros((nsnap, nz, nx), np.float32) if nsnap else None
+    t = C.c_double(0.0)
+    ptr = lambda a: None if a is None else a.ctypes.data  # noqa: E731
+    _lib.check(_lib.lib().fomo_coupled2d(C.byref(d), vp.ctypes.data, vs.ctypes.data, sx.ctypes.data,
+                                         sz.ctypes.data, wm.ctypes.data, rx.ctypes.data, rz.ctypes.data,
+                                         sP.ctypes.data, sS.ctypes.data, ptr(snP), ptr(snS), C.byref(t)))
+    return Coupled2DResult(sP, sS, [s.T for s in snP] if snP is not None else [],
+                           [s.T for s in snS] if snS is not None else [], Stats(float(t.value)))
+
+
 Acoustic2DBatchResult = namedtuple("Acoustic2DBatchResult", ["seis_p", "seis_vx", "seis_vz", "stats"])
 Elastic2DBatchResult = namedtuple("Elastic2DBatchResult", ["seis_vx", "seis_vz", "stats"])
 BatchStats = namedtuple("BatchStats", ["kernel_time_s", "n_shots"])
@@ -439,8 +493,9 @@ class Plan:
 
     def __init__(self, eq, vp, rho, dh, dt, nt, *, sx=None, sz=None, rx=None, rz=None, wavelet=None, vs=None,
                  nbc=50, fd_order=8, v_ref=None, source_type=0, use_graph=True, sy=None, ry=None, storage=0,
-                 record_p=False, tile_cfg=0, shots=None, snap_interval=0):
-        vp, rho = _f32F(vp), _f32F(rho)
+                 record_p=False, tile_cfg=0, shots=None, snap_interval=0, v_ref_s=None, smooth_sigma=3.0):
+        vp = _f32F(vp)
+        rho = None if rho is None else _f32F(rho)  # scalar2d and coupled2d take no density
         self._keep = [vp, rho]
         if vs is not None:
             vs = _f32F(vs)
@@ -463,7 +518,9 @@ class Plan:
                       source_type=source_type, ny=ny, storage=storage_code(storage))
             d.record_p = 1 if record_p else 0
             d.tile_cfg = tile_cfg
-            _lib.check(_lib.lib().fomo_plan_create(C.byref(d), vp.ctypes.data, ptr(vs), rho.ctypes.data,
+            d.v_ref_s = 0.0 if v_ref_s is None else float(np.float32(v_ref_s))
+            d.smooth_sigma = float(smooth_sigma)
+            _lib.check(_lib.lib().fomo_plan_create(C.byref(d), vp.ctypes.data, ptr(vs), ptr(rho),
                                                    sx.ctypes.data, ptr(sy), sz.ctypes.data, wm.ctypes.data,
                                                    rx.ctypes.data, ptr(ry), rz.ctypes.data, C.byref(h)))
         else:
@@ -483,7 +540,9 @@ class Plan:
             d = _desc(eq, nx, nz, nbc, fd_order, nt, dh, dt, 0.0, 0, 0, 0, use_graph, source_type=source_type,
                       storage=storage_code(storage))
             d.tile_cfg = tile_cfg
-            _lib.check(_lib.lib().fomo_batch_create(C.byref(d), vp.ctypes.data, ptr(vs), rho.ctypes.data,
+            d.v_ref_s = 0.0 if v_ref_s is None else float(np.float32(v_ref_s))
+            d.smooth_sigma = float(smooth_sigma)
+            _lib.check(_lib.lib().fomo_batch_create(C.byref(d), vp.ctypes.data, ptr(vs), ptr(rho),
                                                     len(shots), table, C.byref(h)))
         self._h = h
         n = C.c_int32()

@@ -26,6 +26,15 @@ def get_centered_d2(order: int):
     return np.float32(c0[0]), d2[: order // 2].copy()
 
 
+def get_centered_d1(order: int) -> np.ndarray:
+    """get_centered_d1, src/utils/FD_centered.jl:48-51 (Float32; raises for unsupported orders)."""
+    d1 = np.zeros(5, np.float32)
+    rc = _lib.lib().fomo_centered_d1(int(order), d1.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"Unsupported FD order: {order} (centered d1)")
+    return d1[: order // 2].copy()
+
+
 def ricker_wavelet(f0: float, dt: float, nt: int) -> np.ndarray:
     """ricker_wavelet, src/utils/ricker_wavelet.jl:6-12 (Float32, length nt)."""
     out = np.zeros(max(int(nt), 1), np.float32)

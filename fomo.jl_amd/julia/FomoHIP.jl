@@ -10,6 +10,7 @@
 #   acoustic2d        src/equations/acoustic2d/acoustic2d.jl:47-64, 187-191  -> NamedTuple
 #   elastic2d         src/equations/elastic2d/elastic2d.jl:45-163            -> NamedTuple
 #   scalar2d          src/equations/scalar2d/scalar2d.jl:104-189             -> NamedTuple
+#   coupled2d         src/equations/coupled2d/coupled2d.jl:53-183            -> NamedTuple
 #   acoustic2d_batch  src/equations/acoustic2d/batch.jl:180-261              -> NamedTuple
 #   elastic2d_batch   src/equations/elastic2d/batch.jl:197-261               -> NamedTuple
 #   acoustic3d        src/equations/acoustic3d/acoustic3d.jl:31-129          -> plain 4-tuple
@@ -35,12 +36,14 @@ const FOMO_ELASTIC2D = Int32(2)
 const FOMO_ACOUSTIC3D = Int32(3)
 const FOMO_ELASTIC3D = Int32(4)
 const FOMO_SCALAR2D = Int32(5)
+const FOMO_COUPLED2D = Int32(6)
 const FOMO_BOUNDARY_HABC = Int32(0)
 const FOMO_BOUNDARY_SPONGE = Int32(1)
 const STORAGE_CODES = Dict(:f32 => Int32(0), :f16 => Int32(1), :bf16 => Int32(2), :mat16 => Int32(3))
 const SNAP_PLANES = Dict(:xz => Int32(0), :xy => Int32(1), :yz => Int32(2))
 
-# struct fomo_desc, field by field (every field is 4 bytes, so the Julia and C layouts agree)
+# struct fomo_desc, field by field (4-byte fields, then one Float64 at an 8-byte-aligned offset, so
+# the Julia and C layouts agree)
 struct FomoDesc
     eq::Int32
     nx::Int32
@@ -64,7 +67,9 @@ struct FomoDesc
     tile_cfg::Int32
     boundary::Int32
     sponge_factor::Float32
-    reserved::NTuple{4,Int32}
+    smooth_sigma::Float64
+    v_ref_s::Float32
+    reserved::NTuple{1,Int32}
 end
 
 # struct fomo_shot (one window of a windowed batch plan, fomo_batch_create)
@@ -83,14 +88,14 @@ end
 
 function _desc(eq, nx, ny, nz, nbc, fd_order, nt, dh, dt, v_ref, n_src, n_rec; source_type=0, storage=:f32,
                snap_interval=0, snap_plane=:xz, snap_index=0, use_graph=true, boundary=:habc,
-               sponge_factor=0.015f0)
+               sponge_factor=0.015f0, smooth_sigma=0.0, v_ref_s=0.0f0)
     haskey(STORAGE_CODES, storage) || throw(ArgumentError("storage must be :f32, :f16, :bf16 or :mat16, got $storage"))
     return FomoDesc(Int32(eq), Int32(nx), Int32(ny), Int32(nz), Int32(nbc), Int32(fd_order), Int32(nt),
                     Float32(dh), Float32(dt), Float32(v_ref), Int32(n_src), Int32(n_rec), Int32(source_type),
                     STORAGE_CODES[storage], Int32(snap_interval), SNAP_PLANES[snap_plane], Int32(snap_index),
                     Int32(use_graph), Int32(0), Int32(0),
                     boundary === :sponge ? FOMO_BOUNDARY_SPONGE : FOMO_BOUNDARY_HABC, Float32(sponge_factor),
-                    (Int32(0), Int32(0), Int32(0), Int32(0)))
+                    Float64(smooth_sigma), Float32(v_ref_s), (Int32(0),))
 end
 
 last_error() = unsafe_string(ccall((:fomo_last_error, LIB), Cstring, ()))
@@ -214,6 +219,44 @@ function scalar2d(vp::AbstractMatrix{Float32}, dh::Float32, dt::Float32, nt::Int
                  d, Matrix{Float32}(vp), _i32(sx), _i32(sz), wm, _i32(rx), _i32(rz), seis_u, t))
     verbose && @info "Scalar complete! GPU time: $(round(t[], digits=3))s"
     return (seis_u=seis_u, stats=(kernel_time_s=t[],))
+end
+
+function coupled2d(vp::Matrix{Float32}, vs::Matrix{Float32}, dh::Float32, dt::Float32, nt::Int, f0::Float32;
+                   sx::Vector{Int}, sz::Vector{Int}, rx::AbstractVector{Int}, rz::AbstractVector{Int},
+                   nbc::Int=50, fd_order::Int=8, snap_interval::Int=0,
+                   v_ref_p::Float32=Float32(minimum(x for x in vp if x > 0.0f0)),
+                   v_ref_s::Float32=Float32(minimum(x for x in vs if x > 0.0f0)),
+                   smooth_sigma::Float64=3.0,
+                   wavelet::Union{Nothing,AbstractVector{<:Real}}=nothing,
+                   verbose::Bool=true)
+    nx, nz = size(vp)
+    _check_geometry(nx, nz, sx, sz, rx, rz)
+    vmin_disp = any(x -> x > 0.0f0, vs) ? minimum(x for x in vs if x > 0.0f0) : minimum(x for x in vp if x > 0.0f0)
+    _check_numerics_centered(maximum(vp), vmin_disp, dh, dt, f0, fd_order)
+    get_centered_d1(fd_order)
+    get_centered_d2(fd_order)
+    w = isnothing(wavelet) ? ricker_wavelet(f0, dt, nt) : Float32.(wavelet)
+    length(w) == nt || throw(ArgumentError("wavelet 长度 $(length(w)) ≠ nt=$nt"))
+    wm = repeat(reshape(w, 1, nt), length(sx), 1)
+    d = _desc(FOMO_COUPLED2D, nx, 1, nz, nbc, fd_order, nt, dh, dt, v_ref_p, length(sx), length(rx);
+              snap_interval=snap_interval, smooth_sigma=smooth_sigma, v_ref_s=v_ref_s)
+    nrec = length(rx)
+    seis_P, seis_S = (zeros(Float32, nrec, nt) for _ in 1:2)
+    nsnap = snap_interval > 0 ? nt ÷ snap_interval : 0
+    sbP, sbS = _snapbuf(nx * nz, nsnap), _snapbuf(nx * nz, nsnap)
+    t = Ref{Float64}(0.0)
+    verbose && @info "Starting coupled2d simulation... (nx=$nx, nz=$nz, nt=$nt)"
+    GC.@preserve sbP sbS begin
+        _check(ccall((:fomo_coupled2d, LIB), Cint,
+                     (Ref{FomoDesc}, Ptr{Float32}, Ptr{Float32}, Ptr{Int32}, Ptr{Int32}, Ptr{Float32}, Ptr{Int32},
+                      Ptr{Int32}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}, Ref{Float64}),
+                     d, vp, vs, _i32(sx), _i32(sz), wm, _i32(rx), _i32(rz), seis_P, seis_S, _snapptr(sbP),
+                     _snapptr(sbS), t))
+    end
+    verbose && @info "Simulation complete! GPU time: $(round(t[], digits=3))s"
+    snaps_P = Matrix{Float32}[reshape(sbP[:, k], nx, nz) for k in 1:nsnap]
+    snaps_S = Matrix{Float32}[reshape(sbS[:, k], nx, nz) for k in 1:nsnap]
+    return (seis_P=seis_P, seis_S=seis_S, snaps_P=snaps_P, snaps_S=snaps_S, stats=(kernel_time_s=t[],))
 end
 
 # batch.jl:193-199: a vector means one source per shot; a matrix is (n_shots, n_src_per_shot).  The

@@ -38,6 +38,7 @@ extern "C" {
 #define FOMO_ACOUSTIC3D 3
 #define FOMO_ELASTIC3D 4
 #define FOMO_SCALAR2D 5  /* second-order constant-density scalar equation, u in three rotating buffers */
+#define FOMO_COUPLED2D 6 /* coupled P-S potential equations (constant density), velocity-position split */
 
 #define FOMO_SRC_EXPLOSIVE 0 /* elastic: wavelet added to txx and tzz (reference behaviour) */
 #define FOMO_SRC_FZ 1        /* elastic: vertical force, wavelet added to vz after the velocity HABC */
@@ -78,6 +79,10 @@ int fomo_fd_coefficients(int32_t fd_order, float *out);
  * fd_order/2 offset weights to d2.  Replaces get_centered_d2, src/utils/FD_centered.jl:59-62. */
 int fomo_centered_d2(int32_t fd_order, float *c0, float *d2);
 
+/* Centred first-derivative weights for fd_order in {2,4,6,8,10}: fd_order/2 floats to d1.
+ * Replaces get_centered_d1, src/utils/FD_centered.jl:48-51. */
+int fomo_centered_d1(int32_t fd_order, float *d1);
+
 /* Ricker wavelet of length nt.  Replaces ricker_wavelet, src/utils/ricker_wavelet.jl:6-12. */
 int fomo_ricker_wavelet(float f0, float dt, int32_t nt, float *out);
 
@@ -108,7 +113,12 @@ typedef struct fomo_desc {
                             interior 32 << (k-1)) */
   int32_t boundary;      /* FOMO_BOUNDARY_HABC or FOMO_BOUNDARY_SPONGE (2D) */
   float sponge_factor;   /* Cerjan factor (reference default 0.015); used with FOMO_BOUNDARY_SPONGE */
-  int32_t reserved[4];
+  double smooth_sigma;   /* coupled2d: Gaussian sigma in grid points of the smoothed alpha = vp^2, beta = vs^2
+                            whose derivatives enter the coupling terms (the reference's default is 3.0;
+                            <= 0: unsmoothed) */
+  float v_ref_s;         /* coupled2d: HABC reference velocity of the S and dS/dt fields (<= 0: min(vs > 0));
+                            v_ref is the P fields' */
+  int32_t reserved[1];
 } fomo_desc;
 /* Size limit: cells are addressed with 32-bit offsets, so a padded 2D field (nx + 2 pad) x
  * (nz + 2 pad), or one padded z plane of a 3D field, holds at most 2^30 cells; larger grids are
@@ -158,6 +168,24 @@ int fomo_elastic2d(const fomo_desc *d, const float *vp, const float *vs, const f
  */
 int fomo_scalar2d(const fomo_desc *d, const float *vp, const int32_t *sx, const int32_t *sz, const float *wavelet,
                   const int32_t *rx, const int32_t *rz, float *seis_u, double *kernel_time_s);
+
+/*
+ * coupled2d -- replaces Fomo.coupled2d (src/equations/coupled2d/coupled2d.jl:53-183): the coupled
+ * P-S potential equations of constant-density elastic media as a velocity-position split (the
+ * "velocity" half-step dP/dt += dt rhs_P, dS/dt += dt rhs_S with centred fd_order weights on the
+ * regular grid, then P += dt dP/dt, S += dt dS/dt), a Higdon blend after each half-step (P fields
+ * with d->v_ref, S fields with d->v_ref_s; vx weights after the velocity half-step, tau weights
+ * after the position half-step), raw-add injection into P and the P, S gather.  The medium is
+ * alpha = vp^2 and beta = vs^2 with the derivatives of their Gaussian-smoothed copies
+ * (d->smooth_sigma), as medium.jl:50-117 builds it.  The Higdon blend is the deterministic two-pass
+ * form (the reference applies its in-place single-field kernel, src/boundary/habc/kernels.jl:
+ * 122-190).  d->eq = FOMO_COUPLED2D, HABC, Float32 storage.  vp, vs: nx*nz.  wavelet: n_src x nt.
+ * Outputs (n_rec x nt, either may be NULL): seis_P, seis_S.  snaps_P, snaps_S:
+ * (nt/snap_interval) x nx*nz interior P and S, or NULL.
+ */
+int fomo_coupled2d(const fomo_desc *d, const float *vp, const float *vs, const int32_t *sx, const int32_t *sz,
+                   const float *wavelet, const int32_t *rx, const int32_t *rz, float *seis_P, float *seis_S,
+                   float *snaps_P, float *snaps_S, double *kernel_time_s);
 
 /*
  * acoustic3d -- replaces Fomo.acoustic3d (src/equations/acoustic3d/acoustic3d.jl:31-129).
@@ -249,7 +277,7 @@ int fomo_plan_launch_times(fomo_plan *p, int32_t n, double *ms);
 /* Steps taken since the last reset. */
 int fomo_plan_steps_done(const fomo_plan *p, int32_t *out);
 /* Copy seismograms to host (n_rec x nt each; NULL skips).  Slots: acoustic {p, vx, vz},
- * elastic2d {vx, vz, -}, acoustic3d {vx, vy, vz, p}. */
+ * elastic2d {vx, vz, -}, scalar2d {u}, coupled2d {P, S}, acoustic3d {vx, vy, vz, p}. */
 int fomo_plan_read_seis(fomo_plan *p, float *s0, float *s1, float *s2, float *s3);
 /* FP64 per-receiver sums over one trace slot of two plans' recorded traces, time samples
  * t0 <= t < t1: num[r] = sum_t (a[r,t] - b[r,t])^2 and den[r] = sum_t b[r,t]^2, with b = 0 when pb
@@ -271,12 +299,13 @@ int fomo_trace_misfit(fomo_plan *pa, int32_t shot_a, fomo_plan *pb, int32_t shot
 int fomo_trace_misfit_noalloc(fomo_plan *pa, int32_t shot_a, fomo_plan *pb, int32_t shot_b, int32_t slot,
                               int32_t t0, int32_t t1, double *num, double *den);
 /* Copy the current padded field number `which` (order: acoustic2d p,vx,vz; elastic2d
- * vx,vz,txx,tzz,txz; acoustic3d p,vx,vy,vz) to host as Float32 (decoded from the storage
+ * vx,vz,txx,tzz,txz; coupled2d P,S,dP/dt,dS/dt; acoustic3d p,vx,vy,vz) to host as Float32 (decoded from the storage
  * precision when reduced-precision storage is used). */
 int fomo_plan_read_field(fomo_plan *p, int32_t which, float *out);
 /* Snapshots of a plan created with d->snap_interval > 0 (2D single-shot and 3D plans): writes the
  * number of snapshot slots to *count and, when out is not NULL, copies every slot (count x the
- * interior cells of one snapshot, as the one-shot entry points return them).  Slots of steps not
+ * interior cells of one snapshot, as the one-shot entry points return them; coupled2d: the
+ * nt/snap_interval slots of P, then as many of S).  Slots of steps not
  * yet taken since the last reset are zero. */
 int fomo_plan_read_snaps(fomo_plan *p, int32_t shot, int32_t *count, float *out);
 /* Padded grid dims and pad width. */

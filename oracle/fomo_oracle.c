@@ -742,3 +742,326 @@ void orc_probe_habc(int nx, int nz, int nbc, real q, real qt, real qxt, real *f,
   habc_det(1, ff, ffo, ww, scr, q, qt, qxt, nx, nz, nbc);
   free(scr);
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* coupled2d: the coupled P-S potential solver (src/equations/coupled2d/).                      */
+
+/* Centred first-derivative weights, src/utils/FD_centered.jl:14-20: f' ~ sum_l d1_l (f[i+l] -
+ * f[i-l]) / h; Float64 ratios rounded to Float32 as Julia's Float32[...] literal rounds them. */
+static const double D1_TABLE[5][5] = {
+    {1.0 / 2.0, 0, 0, 0, 0},
+    {2.0 / 3.0, -1.0 / 12.0, 0, 0, 0},
+    {3.0 / 4.0, -3.0 / 20.0, 1.0 / 60.0, 0, 0},
+    {4.0 / 5.0, -1.0 / 5.0, 4.0 / 105.0, -1.0 / 280.0, 0},
+    {5.0 / 6.0, -5.0 / 21.0, 5.0 / 84.0, -5.0 / 504.0, 1.0 / 1260.0},
+};
+
+int orc_centered_d1(int order, float *d1) {
+  if (order < 2 || order > 10 || (order & 1)) return -1;
+  int N = order / 2;
+  for (int l = 0; l < N; ++l) d1[l] = (float)D1_TABLE[N - 1][l];
+  return N;
+}
+
+/* medium.jl:118-150 (_gaussian_smooth): separable Gaussian of radius ceil(3 sigma), taps
+ * exp(-0.5 (k/sigma)^2) in Float64 rounded to the field type, normalised by their sum (summed
+ * first-to-last, the order of Julia's sequential reduction for so few terms), x pass then z pass,
+ * indices clamped to the grid.  tmp: nx*nz scratch. */
+static void coupled_smooth(const real *f, real *out, real *tmp, int nx, int nz, double sigma) {
+  int R = (int)ceil(3.0 * sigma);
+  int nk = 2 * R + 1;
+  real *ker = xcalloc((size_t)nk, sizeof(real));
+  for (int k = -R; k <= R; ++k) {
+    double t = (double)k / sigma;
+    ker[k + R] = (real)exp(-0.5 * (t * t));
+  }
+  real s = ker[0];
+  if (nk > 1) {
+    s = ker[0] + ker[1];
+    for (int k = 2; k < nk; ++k) s += ker[k];
+  }
+  for (int k = 0; k < nk; ++k) ker[k] = ker[k] / s;
+#pragma omp parallel for schedule(static)
+  for (int j = 1; j <= nz; ++j)
+    for (int i = 1; i <= nx; ++i) {
+      real v = (real)0;
+      for (int k = -R; k <= R; ++k) {
+        int ii = i + k < 1 ? 1 : (i + k > nx ? nx : i + k);
+        v += ker[k + R] * f[(size_t)(ii - 1) + (size_t)(j - 1) * nx];
+      }
+      tmp[(size_t)(i - 1) + (size_t)(j - 1) * nx] = v;
+    }
+#pragma omp parallel for schedule(static)
+  for (int j = 1; j <= nz; ++j)
+    for (int i = 1; i <= nx; ++i) {
+      real v = (real)0;
+      for (int k = -R; k <= R; ++k) {
+        int jj = j + k < 1 ? 1 : (j + k > nz ? nz : j + k);
+        v += ker[k + R] * tmp[(size_t)(i - 1) + (size_t)(jj - 1) * nx];
+      }
+      out[(size_t)(i - 1) + (size_t)(j - 1) * nx] = v;
+    }
+  free(ker);
+}
+
+/* medium.jl:156-193: x-gradient on M < i <= nx-M (every j), z-gradient on M < j <= nz-M (every
+ * i), Laplacian on both ranges, each divided by h (gradients) or scaled by 1/(h h) (Laplacian);
+ * zero elsewhere. */
+static void coupled_derivs(const real *f, const real *d1, const real *d2, real c0, int N, real h, int nx, int nz,
+                           real *gx, real *gz, real *lap) {
+  const real ih2 = (real)1 / (h * h);
+#define FF(a, b) f[(size_t)((a)-1) + (size_t)((b)-1) * nx]
+#pragma omp parallel for schedule(static)
+  for (int j = 1; j <= nz; ++j)
+    for (int i = 1; i <= nx; ++i) {
+      const size_t c = (size_t)(i - 1) + (size_t)(j - 1) * nx;
+      const int ix = i > N && i <= nx - N, iz = j > N && j <= nz - N;
+      real vx = 0, vz = 0, vl = 0;
+      if (ix) {
+        real v = 0;
+        for (int l = 1; l <= N; ++l) v += d1[l - 1] * (FF(i + l, j) - FF(i - l, j));
+        vx = v / h;
+      }
+      if (iz) {
+        real v = 0;
+        for (int l = 1; l <= N; ++l) v += d1[l - 1] * (FF(i, j + l) - FF(i, j - l));
+        vz = v / h;
+      }
+      if (ix && iz) {
+        real ax = c0 * FF(i, j), az = c0 * FF(i, j);
+        for (int l = 1; l <= N; ++l) ax += d2[l - 1] * (FF(i + l, j) + FF(i - l, j));
+        for (int l = 1; l <= N; ++l) az += d2[l - 1] * (FF(i, j + l) + FF(i, j - l));
+        vl = (ax + az) * ih2;
+      }
+      gx[c] = vx;
+      gz[c] = vz;
+      lap[c] = vl;
+    }
+#undef FF
+}
+
+/* init_coupled_medium, medium.jl:50-117, on the padded grid (nx, nz).  out: 8 arrays of nx*nz in
+ * the order alpha = vp^2, beta = vs^2 (raw), then d alpha/dx, d alpha/dz, d beta/dx, d beta/dz,
+ * lap alpha, lap beta of the smoothed alpha, beta (sigma <= 0: unsmoothed). */
+static int coupled_medium(const real *vpp, const real *vsp, int nx, int nz, int fd_order, float dh, double sigma,
+                          real *out) {
+  float d1f[5], d2f[5], c0f;
+  int N = orc_centered_d1(fd_order, d1f);
+  if (N < 0 || orc_centered_d2(fd_order, &c0f, d2f) < 0) return -1;
+  real d1[5], d2[5];
+  for (int l = 0; l < N; ++l) {
+    d1[l] = (real)d1f[l];
+    d2[l] = (real)d2f[l];
+  }
+  const size_t n = (size_t)nx * nz;
+  real *alpha = out, *beta = out + n;
+  for (size_t c = 0; c < n; ++c) {
+    alpha[c] = vpp[c] * vpp[c];
+    beta[c] = vsp[c] * vsp[c];
+  }
+  real *as = xcalloc(n, sizeof(real)), *bs = xcalloc(n, sizeof(real)), *tmp = xcalloc(n, sizeof(real));
+  if (sigma > 0) {
+    coupled_smooth(alpha, as, tmp, nx, nz, sigma);
+    coupled_smooth(beta, bs, tmp, nx, nz, sigma);
+  } else {
+    memcpy(as, alpha, n * sizeof(real));
+    memcpy(bs, beta, n * sizeof(real));
+  }
+  const real h = (real)dh;
+  coupled_derivs(as, d1, d2, (real)c0f, N, h, nx, nz, out + 2 * n, out + 3 * n, out + 6 * n);
+  coupled_derivs(bs, d1, d2, (real)c0f, N, h, nx, nz, out + 4 * n, out + 5 * n, out + 7 * n);
+  free(as); free(bs); free(tmp);
+  return 0;
+}
+
+int orc_coupled_medium(int nxi, int nzi, const float *vp, const float *vs, float dh, int nbc, int fd_order,
+                       double sigma, real *out) {
+  if (fd_order < 2 || fd_order > 10 || (fd_order & 1)) return -1;
+  const int pad = nbc + fd_order / 2, nx = nxi + 2 * pad, nz = nzi + 2 * pad;
+  const size_t n = (size_t)nx * nz;
+  real *vpp = xcalloc(n, sizeof(real)), *vsp = xcalloc(n, sizeof(real));
+  orc_pad2d(vp, nxi, nzi, pad, vpp);
+  orc_pad2d(vs, nxi, nzi, pad, vsp);
+  int rc = coupled_medium(vpp, vsp, nx, nz, fd_order, dh, sigma, out);
+  free(vpp); free(vsp);
+  return rc;
+}
+
+/* update_fields.jl:27-112: the "velocity" half-step dPdt += dt rhs_P, dSdt += dt rhs_S on
+ * M < i <= nx-M, M < j <= nz-M (in place: it reads only P, S and the cell's own rates).
+ * med: the 8 arrays of coupled_medium. */
+static void coupled_velocity(real *dPdt, real *dSdt, const real *P, const real *S, const real *med, const real *d1,
+                             const real *d2, real c0, int N, real dt, real ih, real ih2, int nx, int nz) {
+  const size_t n = (size_t)nx * nz;
+  const real *al = med, *be = med + n, *dax = med + 2 * n, *daz = med + 3 * n, *dbx = med + 4 * n,
+             *dbz = med + 5 * n, *la = med + 6 * n, *lb = med + 7 * n;
+#define PP(a, b) P[(size_t)((a)-1) + (size_t)((b)-1) * nx]
+#define SS(a, b) S[(size_t)((a)-1) + (size_t)((b)-1) * nx]
+#pragma omp parallel for schedule(static)
+  for (int j = N + 1; j <= nz - N; ++j)
+    for (int i = N + 1; i <= nx - N; ++i) {
+      const size_t c = (size_t)(i - 1) + (size_t)(j - 1) * nx;
+      real dPdx = 0, dPdz = 0, lPx = c0 * PP(i, j), lPz = c0 * PP(i, j);
+      real dSdx = 0, dSdz = 0, lSx = c0 * SS(i, j), lSz = c0 * SS(i, j);
+      for (int l = 1; l <= N; ++l) {
+        const real c1 = d1[l - 1], c2 = d2[l - 1];
+        dPdx += c1 * (PP(i + l, j) - PP(i - l, j));
+        dPdz += c1 * (PP(i, j + l) - PP(i, j - l));
+        lPx += c2 * (PP(i + l, j) + PP(i - l, j));
+        lPz += c2 * (PP(i, j + l) + PP(i, j - l));
+        dSdx += c1 * (SS(i + l, j) - SS(i - l, j));
+        dSdz += c1 * (SS(i, j + l) - SS(i, j - l));
+        lSx += c2 * (SS(i + l, j) + SS(i - l, j));
+        lSz += c2 * (SS(i, j + l) + SS(i, j - l));
+      }
+      dPdx = dPdx * ih;
+      dPdz = dPdz * ih;
+      dSdx = dSdx * ih;
+      dSdz = dSdz * ih;
+      const real lapP = (lPx + lPz) * ih2, lapS = (lSx + lSz) * ih2;
+      const real Pv = PP(i, j);
+      const real rP = (((Pv * la[c] + (real)2 * (dax[c] * dPdx + daz[c] * dPdz)) - ((real)2 * Pv) * lb[c]) +
+                       (real)2 * (dbx[c] * dSdz - dbz[c] * dSdx)) +
+                      al[c] * lapP;
+      const real rS = ((real)2 * (dbx[c] * dSdx + dbz[c] * dSdz) + (real)2 * (dbz[c] * dPdx - dbx[c] * dPdz)) +
+                      be[c] * lapS;
+      dPdt[c] = dPdt[c] + dt * rP;
+      dSdt[c] = dSdt[c] + dt * rS;
+    }
+#undef PP
+#undef SS
+}
+
+/* update_fields.jl:124-144: the "position" half-step P += dt dPdt, S += dt dSdt on the same
+ * region. */
+static void coupled_position(real *P, real *S, const real *dPdt, const real *dSdt, real dt, int N, int nx, int nz) {
+#pragma omp parallel for schedule(static)
+  for (int j = N + 1; j <= nz - N; ++j)
+    for (int i = N + 1; i <= nx - N; ++i) {
+      const size_t c = (size_t)(i - 1) + (size_t)(j - 1) * nx;
+      P[c] = P[c] + dt * dPdt[c];
+      S[c] = S[c] + dt * dSdt[c];
+    }
+}
+
+/* One velocity half-step and one position half-step on a caller-given padded state (no HABC,
+ * no source), for the known-answer tests of the update equations. */
+int orc_probe_coupled_velocity(int nx, int nz, int fd_order, float dt, float dh, real *dPdt, real *dSdt,
+                               const real *P, const real *S, const real *med) {
+  float d1f[5], d2f[5], c0f;
+  int N = orc_centered_d1(fd_order, d1f);
+  if (N < 0 || orc_centered_d2(fd_order, &c0f, d2f) < 0) return -1;
+  real d1[5], d2[5];
+  for (int l = 0; l < N; ++l) {
+    d1[l] = (real)d1f[l];
+    d2[l] = (real)d2f[l];
+  }
+  const real ih = (real)1 / (real)dh;
+  coupled_velocity(dPdt, dSdt, P, S, med, d1, d2, (real)c0f, N, (real)dt, ih, ih * ih, nx, nz);
+  return 0;
+}
+
+void orc_probe_coupled_position(int nx, int nz, int fd_order, float dt, real *P, real *S, const real *dPdt,
+                                const real *dSdt) {
+  coupled_position(P, S, dPdt, dSdt, (real)dt, fd_order / 2, nx, nz);
+}
+
+/*
+ * coupled2d (src/equations/coupled2d/coupled2d.jl:53-260; step order of _coupled2d_loop! at
+ * :197-260):
+ *   A) backup dPdt, dSdt; velocity half-step; Higdon on dPdt (P coefficients, v_ref_p) and on
+ *      dSdt (S coefficients, v_ref_s), both with the vx weights;
+ *   B) backup P, S; position half-step; Higdon on P and S with the tau weights;
+ *   C) raw add of wavelet[s, it] into P;  D) gather P and S;  E) snapshots of P and S.
+ * The reference applies its in-place (racy) single-field Higdon kernel here
+ * (src/boundary/habc/kernels.jl:122-190); this restatement uses the deterministic two-pass form
+ * (every read is the pre-correction value), the form src/boundary/habc/kernels_det.jl:5-10
+ * states as the intended semantics.
+ *   outputs seis_P, seis_S (n_rec x nt); snaps_P, snaps_S (n_snap x nxi*nzi); final_state
+ *   4*nx*nz (P, S, dPdt, dSdt).
+ */
+int orc_coupled2d(int nxi, int nzi, const float *vp, const float *vs, float dh, float dt, int nt, int nbc,
+                  int fd_order, float v_ref_p, float v_ref_s, double sigma, int n_src, const int *sx, const int *sz,
+                  const float *wavelet, int n_rec, const int *rx, const int *rz, int snap_interval, real *seis_P,
+                  real *seis_S, real *snaps_P, real *snaps_S, real *final_state, int nthreads) {
+  set_threads(nthreads);
+  geom2d g;
+  if (geom2d_init(&g, nxi, nzi, nbc, fd_order, dt, dh, v_ref_p, 0)) return -1;
+  float d1f[5], d2f[5], c0f;
+  const int N = orc_centered_d1(fd_order, d1f);
+  if (N < 0 || orc_centered_d2(fd_order, &c0f, d2f) < 0) return -1;
+  real d1[5], d2[5];
+  for (int l = 0; l < N; ++l) {
+    d1[l] = (real)d1f[l];
+    d2[l] = (real)d2f[l];
+  }
+  real qs[3];
+  orc_habc_coeffs(v_ref_s, dt, dh, qs);
+  const int nx = g.nx, nz = g.nz;
+  const size_t n = (size_t)nx * nz;
+  real *med = xcalloc(8 * n, sizeof(real));
+  if (orc_coupled_medium(nxi, nzi, vp, vs, dh, nbc, fd_order, sigma, med)) {
+    free(med);
+    return -1;
+  }
+  real *wvx = xcalloc(n, sizeof(real)), *wtau = xcalloc(n, sizeof(real));
+  orc_habc_weights2d(nx, nz, g.pad, 0.0, wvx);
+  orc_habc_weights2d(nx, nz, g.pad, 0.75, wtau);
+  real *P = xcalloc(n, sizeof(real)), *S = xcalloc(n, sizeof(real));
+  real *dP = xcalloc(n, sizeof(real)), *dS = xcalloc(n, sizeof(real));
+  real *Po = xcalloc(n, sizeof(real)), *So = xcalloc(n, sizeof(real));
+  real *dPo = xcalloc(n, sizeof(real)), *dSo = xcalloc(n, sizeof(real));
+  long total = orc_frame_total(nx, nz, g.H);
+  real *scr = xcalloc((size_t)(total + 1), sizeof(real));
+  const real rdt = (real)dt, ih = (real)1 / (real)dh;
+  int snap_idx = 0;
+  for (int it = 1; it <= nt; ++it) {
+    real *fv[2] = {dP, dS}, *fvo[2] = {dPo, dSo};
+    strip_backup(2, fv, fvo, nx, nz, g.H);
+    coupled_velocity(dP, dS, P, S, med, d1, d2, (real)c0f, N, rdt, ih, ih * ih, nx, nz);
+    {
+      real *f1[1] = {dP}, *f1o[1] = {dPo}, *f2[1] = {dS}, *f2o[1] = {dSo};
+      const real *w1[1] = {wvx};
+      habc_det(1, f1, f1o, w1, scr, g.q, g.qt, g.qxt, nx, nz, g.H);
+      habc_det(1, f2, f2o, w1, scr, qs[0], qs[1], qs[2], nx, nz, g.H);
+    }
+    real *fp[2] = {P, S}, *fpo[2] = {Po, So};
+    strip_backup(2, fp, fpo, nx, nz, g.H);
+    coupled_position(P, S, dP, dS, rdt, N, nx, nz);
+    {
+      real *f1[1] = {P}, *f1o[1] = {Po}, *f2[1] = {S}, *f2o[1] = {So};
+      const real *w1[1] = {wtau};
+      habc_det(1, f1, f1o, w1, scr, g.q, g.qt, g.qxt, nx, nz, g.H);
+      habc_det(1, f2, f2o, w1, scr, qs[0], qs[1], qs[2], nx, nz, g.H);
+    }
+    for (int s = 0; s < n_src; ++s) {
+      size_t c = (size_t)(sx[s] + g.pad - 1) + (size_t)(sz[s] + g.pad - 1) * nx;
+      P[c] += (real)wavelet[(size_t)s + (size_t)n_src * (it - 1)];
+    }
+    for (int r = 0; r < n_rec; ++r) {
+      size_t c = (size_t)(rx[r] + g.pad - 1) + (size_t)(rz[r] + g.pad - 1) * nx;
+      size_t o = (size_t)r + (size_t)n_rec * (it - 1);
+      if (seis_P) seis_P[o] = P[c];
+      if (seis_S) seis_S[o] = S[c];
+    }
+    if (snap_interval > 0 && it % snap_interval == 0) {
+      for (int j = 1; j <= nzi; ++j)
+        for (int i = 1; i <= nxi; ++i) {
+          size_t c = (size_t)(i + g.pad - 1) + (size_t)(j + g.pad - 1) * nx;
+          size_t o = (size_t)snap_idx * nxi * nzi + (size_t)(i - 1) + (size_t)(j - 1) * nxi;
+          if (snaps_P) snaps_P[o] = P[c];
+          if (snaps_S) snaps_S[o] = S[c];
+        }
+      ++snap_idx;
+    }
+  }
+  if (final_state) {
+    memcpy(final_state, P, n * sizeof(real));
+    memcpy(final_state + n, S, n * sizeof(real));
+    memcpy(final_state + 2 * n, dP, n * sizeof(real));
+    memcpy(final_state + 3 * n, dS, n * sizeof(real));
+  }
+  free(med); free(wvx); free(wtau); free(P); free(S); free(dP); free(dS);
+  free(Po); free(So); free(dPo); free(dSo); free(scr);
+  return 0;
+}

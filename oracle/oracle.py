@@ -94,6 +94,17 @@ def lib(precision: str = "f32") -> C.CDLL:
         L.orc_probe_habc.restype = None
         L.orc_probe_habc.argtypes = [_i, _i, _i, real, real, real, _p, _p, _p]
         L.orc_max_threads.restype = _i
+        L.orc_centered_d1.restype = _i
+        L.orc_centered_d1.argtypes = [_i, _p]
+        L.orc_coupled_medium.restype = _i
+        L.orc_coupled_medium.argtypes = [_i, _i, _p, _p, _f, _i, _i, C.c_double, _p]
+        L.orc_probe_coupled_velocity.restype = _i
+        L.orc_probe_coupled_velocity.argtypes = [_i, _i, _i, _f, _f] + [_p] * 5
+        L.orc_probe_coupled_position.restype = None
+        L.orc_probe_coupled_position.argtypes = [_i, _i, _i, _f] + [_p] * 4
+        L.orc_coupled2d.restype = _i
+        L.orc_coupled2d.argtypes = [_i, _i, _p, _p, _f, _f, _i, _i, _i, _f, _f, C.c_double, _i, _p, _p, _p, _i,
+                                    _p, _p, _i, _p, _p, _p, _p, _p, _i]
         _LIBS[precision] = L
     return _LIBS[precision]
 
@@ -379,6 +390,95 @@ def scalar2d(vp, dh, dt, nt, *, sx, sz, rx, rz, wavelet, nbc=50, fd_order=8, v_r
     out = dict(seis_u=su)
     if final_state:
         out["state"] = {"u": fs.reshape(shp, order="F")}
+    return out
+
+
+def centered_d1(order: int) -> np.ndarray:
+    """d1 of src/utils/FD_centered.jl:48-51 as the oracle stores them (Float32)."""
+    d1 = np.zeros(5, np.float32)
+    n = lib("f32").orc_centered_d1(order, d1.ctypes.data)
+    if n < 0:
+        raise ValueError(f"Unsupported FD order: {order} (centered d1)")
+    return d1[:n].copy()
+
+
+COUPLED_MEDIUM = ("alpha", "beta", "dalpha_dx", "dalpha_dz", "dbeta_dx", "dbeta_dz", "lap_alpha", "lap_beta")
+
+
+def coupled_medium(vp, vs, dh, nbc, fd_order, smooth_sigma=3.0, precision="f32"):
+    """init_coupled_medium (src/equations/coupled2d/medium.jl:50-117) on the padded grid: a dict of
+    the eight (nx+2pad, nz+2pad) arrays named as in COUPLED_MEDIUM."""
+    vp, vs = _f32F(vp), _f32F(vs)
+    nx, nz = vp.shape
+    pad = nbc + fd_order // 2
+    shp = (nx + 2 * pad, nz + 2 * pad)
+    n = shp[0] * shp[1]
+    out = np.zeros(8 * n, _dtype(precision))
+    rc = lib(precision).orc_coupled_medium(nx, nz, vp.ctypes.data, vs.ctypes.data, dh, nbc, fd_order,
+                                           float(smooth_sigma), out.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"oracle coupled_medium failed ({rc})")
+    return {k: out[m * n:(m + 1) * n].reshape(shp, order="F") for m, k in enumerate(COUPLED_MEDIUM)}
+
+
+def probe_coupled_velocity(dPdt, dSdt, P, S, med, fd_order, dt, dh, precision="f64"):
+    """One velocity half-step (update_fields.jl:27-112) on a given padded state; med: a dict as
+    coupled_medium returns.  Returns the new (dPdt, dSdt)."""
+    dt_ = _dtype(precision)
+    a, b, p, s = (np.asfortranarray(np.asarray(x, dt_)).copy(order="F") for x in (dPdt, dSdt, P, S))
+    m = np.concatenate([np.asarray(med[k], dt_).ravel(order="F") for k in COUPLED_MEDIUM])
+    nx, nz = a.shape
+    rc = lib(precision).orc_probe_coupled_velocity(nx, nz, fd_order, dt, dh, a.ctypes.data, b.ctypes.data,
+                                                   p.ctypes.data, s.ctypes.data, m.ctypes.data)
+    if rc != 0:
+        raise ValueError("oracle probe_coupled_velocity failed")
+    return a, b
+
+
+def probe_coupled_position(P, S, dPdt, dSdt, fd_order, dt, precision="f64"):
+    """One position half-step (update_fields.jl:124-144).  Returns the new (P, S)."""
+    dt_ = _dtype(precision)
+    p, s, a, b = (np.asfortranarray(np.asarray(x, dt_)).copy(order="F") for x in (P, S, dPdt, dSdt))
+    nx, nz = p.shape
+    lib(precision).orc_probe_coupled_position(nx, nz, fd_order, dt, p.ctypes.data, s.ctypes.data, a.ctypes.data,
+                                              b.ctypes.data)
+    return p, s
+
+
+def coupled2d(vp, vs, dh, dt, nt, *, sx, sz, rx, rz, wavelet, nbc=50, fd_order=8, snap_interval=0, v_ref_p=None,
+              v_ref_s=None, smooth_sigma=3.0, precision="f32", nthreads=0, final_state=False):
+    """coupled2d (src/equations/coupled2d/coupled2d.jl:53-260) with the deterministic two-pass
+    Higdon boundary.  wavelet: (n_src, nt).  Returns dict seis_P, seis_S, snaps_P, snaps_S (and
+    state P, S, dPdt, dSdt when final_state)."""
+    vp, vs = _f32F(vp), _f32F(vs)
+    nx, nz = vp.shape
+    if v_ref_p is None:
+        v_ref_p = float(vp[vp > 0].min())
+    if v_ref_s is None:
+        v_ref_s = float(vs[vs > 0].min())
+    sx, sz, rx, rz = _i32(sx), _i32(sz), _i32(rx), _i32(rz)
+    w = np.asfortranarray(np.asarray(wavelet, np.float32).reshape(len(sx), nt))
+    nrec = len(rx)
+    dt_ = _dtype(precision)
+    sP, sS = (np.zeros((nrec, nt), dt_, order="F") for _ in range(2))
+    nsnap = nt // snap_interval if snap_interval > 0 else 0
+    snP = np.zeros((nsnap, nz, nx), dt_) if nsnap else None
+    snS = np.zeros((nsnap, nz, nx), dt_) if nsnap else None
+    pad = nbc + fd_order // 2
+    shp = (nx + 2 * pad, nz + 2 * pad)
+    n = shp[0] * shp[1]
+    fs = np.zeros(4 * n, dt_) if final_state else None
+    rc = lib(precision).orc_coupled2d(nx, nz, vp.ctypes.data, vs.ctypes.data, dh, dt, nt, nbc, fd_order, v_ref_p,
+                                      v_ref_s, float(smooth_sigma), len(sx), sx.ctypes.data, sz.ctypes.data,
+                                      w.ctypes.data, nrec, rx.ctypes.data, rz.ctypes.data, snap_interval,
+                                      sP.ctypes.data, sS.ctypes.data, _ptr(snP), _ptr(snS), _ptr(fs), nthreads)
+    if rc != 0:
+        raise ValueError(f"oracle coupled2d failed ({rc})")
+    out = dict(seis_P=sP, seis_S=sS, snaps_P=[s.T for s in snP] if snP is not None else [],
+               snaps_S=[s.T for s in snS] if snS is not None else [])
+    if final_state:
+        out["state"] = {k: fs[m * n:(m + 1) * n].reshape(shp, order="F")
+                        for m, k in enumerate(("P", "S", "dPdt", "dSdt"))}
     return out
 
 

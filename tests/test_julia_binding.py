@@ -15,8 +15,8 @@ JL = os.path.join(ROOT, "fomo.jl_amd", "julia", "FomoHIP.jl")
 HDR = os.path.join(ROOT, "include", "fomo_hip.h")
 
 JL_SIZE = {"Int32": 4, "Float32": 4, "Float64": 8}
-JL_TO_C = {"Int32": "int32_t", "Float32": "float", "Ptr{Int32}": "const int32_t *", "Ptr{Float32}": "const float *",
-           "Ptr{Float64}": "double *"}
+JL_TO_C = {"Int32": "int32_t", "Float32": "float", "Float64": "double", "Ptr{Int32}": "const int32_t *",
+           "Ptr{Float32}": "const float *", "Ptr{Float64}": "double *"}
 
 
 def _strip_comments(text):
@@ -145,8 +145,9 @@ def test_every_ccall_matches_the_header_and_the_library():
     hdr = header_functions()
     calls = list(re.finditer(r"ccall\(\(:(\w+), LIB\), (Cint|Cstring),", src))
     names = {m.group(1) for m in calls}
-    assert {"fomo_acoustic2d", "fomo_elastic2d", "fomo_scalar2d", "fomo_acoustic2d_batch", "fomo_elastic2d_batch",
-            "fomo_acoustic3d", "fomo_elastic3d", "fomo_last_error", "fomo_trace_misfit"} <= names
+    assert {"fomo_acoustic2d", "fomo_elastic2d", "fomo_scalar2d", "fomo_coupled2d", "fomo_acoustic2d_batch",
+            "fomo_elastic2d_batch", "fomo_acoustic3d", "fomo_elastic3d", "fomo_last_error",
+            "fomo_trace_misfit"} <= names
     so = C.CDLL(L.LIB_PATH)
     for m in calls:
         name = m.group(1)
@@ -168,6 +169,9 @@ def test_every_ccall_matches_the_header_and_the_library():
       "wavelet": "nothing", "scale_source_by_dt": "false", "use_cuda_graph": "true", "verbose": "true"}),
     ("scalar2d", "vp::AbstractMatrix{Float32}, dh::Float32, dt::Float32, nt::Int, f0::Float32",
      {"nbc": "50", "fd_order": "8", "wavelet": "nothing", "source_scale": ":v2dt2", "verbose": "true"}),
+    ("coupled2d", "vp::Matrix{Float32}, vs::Matrix{Float32}, dh::Float32, dt::Float32, nt::Int, f0::Float32",
+     {"nbc": "50", "fd_order": "8", "snap_interval": "0", "smooth_sigma": "3.0", "wavelet": "nothing",
+      "verbose": "true"}),
     ("acoustic2d_batch", "vp::AbstractMatrix{Float32}, rho::AbstractMatrix{Float32}, dh::Float32, dt::Float32,",
      {"nbc": "50", "fd_order": "8", "wavelet": "nothing", "scale_source_by_dt": "false", "verbose": "true"}),
     ("elastic2d_batch", "vp::AbstractMatrix{Float32}, vs::AbstractMatrix{Float32}, rho::AbstractMatrix{Float32},",
@@ -191,6 +195,7 @@ def test_public_signatures_keep_the_reference_contract(fn, positional, kw):
     want = {"acoustic2d": "(seis_p=seis_p, seis_vx=seis_vx, seis_vz=seis_vz, snaps=snaps, stats=(kernel_time_s=t[],))",
             "elastic2d": "(seis_vx=seis_vx, seis_vz=seis_vz, snaps=snaps, stats=(kernel_time_s=t[],))",
             "scalar2d": "(seis_u=seis_u, stats=(kernel_time_s=t[],))",
+            "coupled2d": "(seis_P=seis_P, seis_S=seis_S, snaps_P=snaps_P, snaps_S=snaps_S, stats=(kernel_time_s=t[],))",
             "acoustic2d_batch": "(seis_p=seis_p, seis_vx=seis_vx, seis_vz=seis_vz, stats=(kernel_time_s=t[], n_shots=ns))",
             "elastic2d_batch": "(seis_vx=seis_vx, seis_vz=seis_vz, stats=(kernel_time_s=t[], n_shots=ns))",
             "acoustic3d": "seis_vx, seis_vy, seis_vz, snaps",
