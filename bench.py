@@ -257,8 +257,11 @@ def elastic2d_line(args):
 
 def probe(args):
     """HBM-bound 2D probe (not the headline line): homogeneous 8192 x 8192 interior, fd8, nbc 50,
-    acoustic2d, elastic2d and scalar2d (one sweep per step: read u^n, u^(n-1), c2, write u^(n+1) =
-    16 B/cell); prints per-kernel times and algorithmic GB/s."""
+    acoustic2d, elastic2d, scalar2d (one sweep per step: read u^n, u^(n-1), c2, write u^(n+1) =
+    16 B/cell) and coupled2d (velocity half-step: P, S, dP/dt, dS/dt and eight medium arrays in,
+    dP/dt, dS/dt out = 56 B/cell; position half-step: 24 B/cell; its velocity launch also takes the
+    interior's position half-steps, so only the step figure credits them); prints per-kernel times
+    and algorithmic GB/s."""
     import numpy as np
 
     import fomo
@@ -273,19 +276,25 @@ def probe(args):
     res = {}
     storages = args.probe_storage2d.split(",")
     # (equation, name, FP32 bytes per cell of the two phases, of which material bytes)
-    for eq, name, bv, bs, mv, ms_ in ((L.ACOUSTIC2D, "acoustic2d", 28, 20, 8, 4),
-                                      (L.ELASTIC2D, "elastic2d", BYTES_VEL, BYTES_STR, 8, 12),
-                                      (L.SCALAR2D, "scalar2d", 16, 0, 4, 0)):
+    eqs = [(L.ACOUSTIC2D, "acoustic2d", 28, 20, 8, 4), (L.ELASTIC2D, "elastic2d", BYTES_VEL, BYTES_STR, 8, 12),
+           (L.SCALAR2D, "scalar2d", 16, 0, 4, 0), (L.COUPLED2D, "coupled2d", 56, 24, 32, 0)]
+    only = set(args.probe_eq2d.split(",")) if args.probe_eq2d else None
+    for eq, name, bv, bs, mv, ms_ in eqs:
+        if only and name not in only:
+            continue
         for st in storages:
             if st == "mat16" and eq == L.SCALAR2D:
                 continue  # the scalar sweep keeps a wavefield among its material inputs
+            if st != "f32" and eq == L.COUPLED2D:
+                continue  # the coupled equations run with Float32 storage
             if st == "mat16":  # Float16 materials halve only the material bytes
                 b_v, b_s, eb = bv - mv // 2, bs - ms_ // 2, 1.0
             else:
                 b_v, b_s = bv, bs
                 eb = 1.0 if st == "f32" else 0.5  # 16-bit storage halves every array's bytes
-            plan = Plan(eq, vp, rho, 10.0, 1e-3, nt, vs=vs if eq == L.ELASTIC2D else None, sx=[n // 2], sz=[n // 2],
-                        rx=[1, n], rz=[1, n], wavelet=w[None], nbc=50, fd_order=8, storage=st)
+            plan = Plan(eq, vp, rho, 10.0, 1e-3, nt, vs=vs if eq in (L.ELASTIC2D, L.COUPLED2D) else None,
+                        sx=[n // 2], sz=[n // 2], rx=[1, n], rz=[1, n], wavelet=w[None], nbc=50, fd_order=8,
+                        storage=st)
             cells = plan.padded_cells
             plan.advance(10)
             plan.sync()
@@ -560,6 +569,8 @@ def main():
     ap.add_argument("--probe-n3", type=int, default=512)
     ap.add_argument("--probe-storage", default="f32,f16,bf16", help="storages of the 3D probe")
     ap.add_argument("--probe-storage2d", default="f32", help="storages of the 2D probe (f32,f16,bf16,mat16)")
+    ap.add_argument("--probe-eq2d", default="", help="equations of the 2D probe (default all: acoustic2d,elastic2d,"
+                    "scalar2d,coupled2d)")
     ap.add_argument("--probe-eq", default="acoustic3d", choices=["acoustic3d", "elastic3d"], help="3D probe equation")
     ap.add_argument("--batch-eq", default="acoustic2d", choices=["acoustic2d", "elastic2d"], help="batch equation")
     ap.add_argument("--stream-copy", action="store_true", help="only the HBM streaming-copy calibration")

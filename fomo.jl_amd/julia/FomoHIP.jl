@@ -4,7 +4,7 @@
 # A drop-in for the CUDA.jl solver bodies: a Fomo.jl maintainer includes this file inside the Fomo
 # module (so the reference's own _check_geometry / _check_numerics / _check_numerics_centered
 # (src/utils/checks.jl), ricker_wavelet (src/utils/ricker_wavelet.jl) and get_fd_coefficients /
-# get_centered_d2 (src/utils/FD_utils.jl, FD_centered.jl) are in scope) and exports these
+# get_centered_d1 / get_centered_d2 (src/utils/FD_utils.jl, FD_centered.jl) are in scope) and exports these
 # functions in place of the CUDA.jl ones (src/Fomo.jl:107-117).  Names, positional arguments,
 # keywords, defaults, validation messages and returns are the reference's:
 #   acoustic2d        src/equations/acoustic2d/acoustic2d.jl:47-64, 187-191  -> NamedTuple
