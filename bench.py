@@ -143,13 +143,25 @@ def traffic_child(args):
     L.check(L.lib().fomo_stream_copy(COPY_BYTES, 2, C.byref(ms)))
     vp = np.load(os.path.join(args.traffic_child, "vp.npy"), mmap_mode="r")
     rho = np.load(os.path.join(args.traffic_child, "rho.npy"), mmap_mode="r")
-    plan = plan3d(np.asfortranarray(vp), np.asfortranarray(rho), 4, storage=args.traffic_storage, use_graph=False)
+    plan = plan3d(np.asfortranarray(vp), np.asfortranarray(rho), 4, storage=args.traffic_storage, use_graph=False,
+                  tile_cfg=args.tile)
     plan.advance(2)
     plan.sync()
     info = plan.launches()
     plan.launch_times(1)
     plan.close()
     print(json.dumps({"launches": [x["name"] for x in info]}), flush=True)
+
+
+def tune_child(args):
+    """Create the headline plan once, without a profiler, so that the 3D autotuner times its layouts
+    and records the choice in the tuning file this process shares with its parent and siblings."""
+    import numpy as np
+    vp = np.load(os.path.join(args.tune_child, "vp.npy"), mmap_mode="r")
+    rho = np.load(os.path.join(args.tune_child, "rho.npy"), mmap_mode="r")
+    plan = plan3d(np.asfortranarray(vp), np.asfortranarray(rho), 4, tile_cfg=args.tile)
+    print(json.dumps({"tuning": plan.tuning()}), flush=True)
+    plan.close()
 
 
 def _dispatches(outdir):
@@ -164,7 +176,7 @@ def _dispatches(outdir):
     return rows
 
 
-def measure_traffic(vp, rho, storage="f32", timeout_s=300):
+def measure_traffic(vp, rho, storage="f32", timeout_s=300, tile=0):
     """HBM bytes per launch of each launch of a step: FETCH_SIZE and WRITE_SIZE (KiB per dispatch)
     from two rocprofv3 --pmc runs of traffic_child, scaled by the readings on the streaming copy
     (COPY_BYTES read and written per dispatch).  Returns None when rocprofv3 is unavailable or a
@@ -179,12 +191,23 @@ def measure_traffic(vp, rho, storage="f32", timeout_s=300):
         np.save(os.path.join(work, "vp.npy"), np.ascontiguousarray(vp.ravel(order="F")).reshape(vp.shape, order="F"))
         np.save(os.path.join(work, "rho.npy"), np.ascontiguousarray(rho.ravel(order="F")).reshape(rho.shape, order="F"))
         env = dict(os.environ, TMPDIR="/tmp")
+        # the layout is chosen once, outside the profiler; the counter runs read it from the file
+        log("traffic: tuning the layout (child process)")
+        try:
+            cp = subprocess.run([sys.executable, os.path.abspath(__file__), "--tune-child", work, "--tile",
+                                 str(tile)], cwd="/tmp", env=env, capture_output=True, text=True, timeout=timeout_s)
+        except subprocess.TimeoutExpired:
+            log("traffic: tuning run timed out")
+            return None
+        if cp.returncode != 0:
+            log(f"traffic: tuning run failed ({cp.returncode}): {cp.stderr[-400:]}")
+            return None
         res = {}
         names = None
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
             out = os.path.join(work, counter)
             cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "run", "--", sys.executable,
-                   os.path.abspath(__file__), "--traffic-child", work, "--traffic-storage", storage]
+                   os.path.abspath(__file__), "--traffic-child", work, "--traffic-storage", storage, "--tile", str(tile)]
             log(f"traffic: rocprofv3 --pmc {counter} (child process)")
             try:
                 cp = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True, timeout=timeout_s)
@@ -562,6 +585,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the FP16/BF16 and elastic2d extra lines")
     ap.add_argument("--tile", type=int, default=0, help="3D z-segment choice of the headline plan (0: default)")
     ap.add_argument("--traffic-child", default="", help=argparse.SUPPRESS)
+    ap.add_argument("--tune-child", default="", help=argparse.SUPPRESS)
     ap.add_argument("--traffic-storage", default="f32", help=argparse.SUPPRESS)
     ap.add_argument("--probe", action="store_true", help="HBM-bound 2D probe instead of the headline line")
     ap.add_argument("--probe-n", type=int, default=8192)
@@ -592,6 +616,9 @@ def main():
 
     if args.traffic_child:
         traffic_child(args)
+        return
+    if args.tune_child:
+        tune_child(args)
         return
     if args.probe:
         probe(args)
@@ -643,11 +670,17 @@ def headline(args):
     from fomo.models import model_digest
 
     vp, rho = model3d()
+    # This process and its children share one tuning file, so the counter runs measure the layout
+    # the timed run uses (a fresh file per bench run unless FOMO_TUNE_CACHE names one).
+    if not os.environ.get("FOMO_TUNE_CACHE"):
+        os.environ["FOMO_TUNE_CACHE"] = os.path.join(tempfile.mkdtemp(prefix="fomo_tune_", dir="/tmp"), "tune.txt")
     traffic = None
     if not args.no_traffic:
-        traffic = measure_traffic(vp, rho)  # before this process touches the GPU
+        traffic = measure_traffic(vp, rho, tile=args.tile)  # before this process touches the GPU
     log("creating the plan (setup, upload, graph capture)")
     plan = plan3d(vp, rho, NT3, tile_cfg=args.tile)
+    layout = plan.tuning()
+    log(f"layout: {layout}")
     n_pad = plan.padded_cells
     pnx, pny, pnz = plan.padded[0]
     log(f"warmup {args.warmup} steps")
@@ -716,6 +749,7 @@ def headline(args):
             "step_algorithmic_gbs": bytes_step / (ms * 1e-3 / args.steps) / 1e9,
             "step_frac_of_peak": bytes_step / (ms * 1e-3 / args.steps) / 1e9 / HBM_PEAK_GBS,
             "launches": launches,
+            "layout": layout,
             "outputs_finite": finite,
             "peak_abs_vz": peak_vz,
         },

@@ -604,6 +604,15 @@ extern "C" int fomo_plan_launch_times(fomo_plan* p, int32_t n, double* ms) {
   return rc;
 }
 
+extern "C" int fomo_plan_tuning(const fomo_plan* p, int32_t* out4) {
+  if (!p || !p->impl || !out4) return fail(FOMO_EINVAL, "NULL argument");
+  out4[0] = p->impl->tune_family;
+  out4[1] = p->impl->tune_shape;
+  out4[2] = p->impl->tune_extent;
+  out4[3] = p->impl->tune_source;
+  return FOMO_OK;
+}
+
 extern "C" int fomo_plan_sync(fomo_plan* p) {
   PLAN_OR_FAIL(p);
   return p->impl->check_overflow();

@@ -38,6 +38,11 @@ struct PlanBase {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   long steps_done = 0;
+  // The work layout in use (fomo_plan_tuning): family (0 separate sweeps, 1 a fused interior
+  // launch), shape index, its extent (2D: strip rows; 3D: z-planes of a segment of the fused
+  // interior, else of the sweeps), and where the choice came from (0 timed by this plan, 1 this
+  // process's tuning cache, 2 the tuning file, 3 fixed by the descriptor or the environment).
+  int tune_family = 0, tune_shape = 0, tune_extent = 0, tune_source = 3;
 
   virtual ~PlanBase();
   // Enqueue one part of the step with index *step + koff from state parity s: 0 velocity phase,

@@ -91,6 +91,7 @@ struct Plan2D : PlanBase {
     return e;
   }
   int make_work(int cfg);
+  void note_tuning(int source);  // record the layout in use (PlanBase::tune_*)
   int n_cfgs() const { return fused ? kNumFusedCfgs : kNumTileCfgs; }
   // Decode the current field `which` of a shot into `scratch` (asynchronous on the stream).
   int decode_field(int shot, int which);

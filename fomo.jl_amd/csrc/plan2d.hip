@@ -12,6 +12,7 @@
 #include "plan.h"
 #include "plan2d.h"
 #include "learned2d.h"
+#include "tune.h"
 
 namespace fomo {
 
@@ -329,35 +330,55 @@ int Plan2D::dims(int shot, int* nx, int* ny, int* nz, int* pad) const {
   return FOMO_OK;
 }
 
+void Plan2D::note_tuning(int source) {
+  tune_family = fused ? 1 : 0;
+  tune_shape = tile_cfg;
+  tune_extent = fused ? fused_rows(tile_cfg) : strip_rows(tile_cfg);
+  tune_source = source;
+}
+
 // Times a few direct-launch steps per tile shape and keeps the fastest.  An acoustic2d plan that
 // allows the fused interior launch times both families: the separate sweeps' strips (16, 32, 64
 // rows) and the fused interior's (64 .. 256 rows); FOMO_2D_FUSED=1 / FOMO_2D_UNFUSED=1 keep one.
 // (At 8192^2 the fused family is 12 % faster; on the 64-window batch, whose narrow windows leave
 // lanes of the 249-column fused strips idle, the separate sweeps are 1 % faster.)  The choice is
-// cached per (equation, order, sweep storage, boundary, learned, families, shot count, padded size)
-// for the life of the process; FOMO_TILE_CFG forces a shape of the current family.
+// kept per (device and library build, equation, order, sweep storage, boundary, learned, families,
+// shot count, padded size and window widths), in this process and in the tuning file (tune.h), so a
+// later process with the same problem reuses it; FOMO_TILE_CFG forces a shape of the current family.
 int Plan2D::autotune() {
-  static std::map<std::tuple<int, int, int, int, int, int, long long>, std::pair<bool, int>> cache;
   long long cells = 0;
-  for (auto& sh : shots) cells += (long long)sh.nx * sh.nz;
+  unsigned long long widths = 1469598103934665603ull;  // FNV-1a over the shots' padded widths
+  for (auto& sh : shots) {
+    cells += (long long)sh.nx * sh.nz;
+    widths = (widths ^ (unsigned long long)sh.nx) * 1099511628211ull;
+  }
   const bool only_fused = fusable && std::getenv("FOMO_2D_FUSED");
   const bool try_fused = fusable && !std::getenv("FOMO_2D_UNFUSED");
   const bool try_sweeps = !only_fused || !try_fused;
   const int fam = (try_fused ? 1 : 0) + (try_sweeps ? 2 : 0);
-  const auto key = std::make_tuple(d.eq, N, sweep_storage, d.boundary, (learned ? 1 : 0) + 2 * fam, (int)shots.size(),
-                                   cells);
   if (const char* env = std::getenv("FOMO_TILE_CFG")) {
     const int c = std::atoi(env);
     if (c >= 0 && c < n_cfgs()) {
       tile_cfg = c;
+      note_tuning(3);
       return make_work(c);
     }
   }
-  auto it = cache.find(key);
-  if (it != cache.end()) {
-    fused = it->second.first;
-    tile_cfg = it->second.second;
-    return make_work(tile_cfg);
+  const std::string key = tune_device_tag() + " 2d eq=" + std::to_string(d.eq) + " N=" + std::to_string(N) +
+                          " storage=" + std::to_string(sweep_storage) + " boundary=" + std::to_string(d.boundary) +
+                          " learned=" + std::to_string(learned ? 1 : 0) + " families=" + std::to_string(fam) +
+                          " shots=" + std::to_string(shots.size()) + " cells=" + std::to_string(cells) +
+                          " widths=" + std::to_string(widths);
+  std::vector<int> rec;
+  if (tune_lookup(key, rec) && rec.size() == 2 && (rec[0] == 0 || rec[0] == 1)) {
+    const bool f = rec[0] == 1;
+    const int c = rec[1];
+    if ((f ? try_fused : try_sweeps) && c >= 0 && c < (f ? kNumFusedCfgs : kNumTileCfgs)) {
+      fused = f;
+      tile_cfg = c;
+      note_tuning(tune_last_source());
+      return make_work(tile_cfg);
+    }
   }
   std::vector<std::pair<bool, int>> cand;
   if (try_sweeps)
@@ -396,7 +417,8 @@ int Plan2D::autotune() {
   (void)hipEventDestroy(e1);
   fused = cand[best].first;
   tile_cfg = cand[best].second;
-  cache[key] = cand[best];
+  tune_store(key, {fused ? 1 : 0, tile_cfg});
+  note_tuning(0);
   return reset();
 }
 
@@ -696,6 +718,7 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
   p->fused = p->fusable && !std::getenv("FOMO_2D_UNFUSED");
   if (d->tile_cfg > 0 && d->tile_cfg <= p->n_cfgs()) {
     p->tile_cfg = d->tile_cfg - 1;
+    p->note_tuning(3);
     if ((rc = p->make_work(p->tile_cfg))) return bail(rc);
   } else if ((rc = p->autotune())) {
     return bail(rc);

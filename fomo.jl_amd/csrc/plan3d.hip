@@ -15,6 +15,7 @@
 
 #include "engine3d.h"
 #include "plan.h"
+#include "tune.h"
 
 namespace fomo {
 
@@ -358,6 +359,8 @@ static int upload_any(Plan3D* p, void** dst, const std::vector<float>& h, float 
 
 using namespace fomo;
 
+static int layout3d(Plan3D* p, const fomo_desc* d, int rows, int core_rows);
+
 int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, const float* rho, const int32_t* sx,
                        const int32_t* sy, const int32_t* sz, const float* wavelet, const int32_t* rx,
                        const int32_t* ry, const int32_t* rz, fomo_plan** out) {
@@ -614,28 +617,145 @@ int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, con
     p->snap_n = d->nt / d->snap_interval;
     if ((rc = p->alloc(&p->snap, p->snap_cells * p->snap_n))) return bail(rc);
   }
-  const int kseg = d->tile_cfg > 0 ? 16 << (d->tile_cfg - 1) : 32;
-  // Fused interior z-segments: one block per CU fits (1024 threads, 60 KB of LDS), so the columns
-  // are cut into the fewest segments that still give every CU a block; each segment pays 2N - 1
-  // extra levels of vz, and tiles that march together share their halos through L2.
-  // FOMO_FUSED_KSEG overrides the segment length.
-  int kseg_fused = d->tile_cfg > 0 ? 32 << (d->tile_cfg - 1) : 0;
-  if (const char* env = std::getenv("FOMO_FUSED_KSEG")) kseg_fused = std::max(8, std::atoi(env));
-  if (kseg_fused == 0) {
-    const long long cols = (long long)((p->nxi + kTileW - 1) / kTileW) * ((p->nyi + kFTY - 1) / kFTY);
-    const long long nseg = std::max(1LL, (256 + cols - 1) / std::max(1LL, cols));
-    kseg_fused = (int)std::max(8LL, (p->nzi + nseg - 1) / nseg);
-  }
-  if ((rc = make_work3d(p, kseg, elastic ? kRowsEl3 : kRowsAc3, elastic ? kRowsEl3Core : kRowsAc3Core, kseg_fused)))
-    return bail(rc);
+  const int rows = elastic ? kRowsEl3 : kRowsAc3, core_rows = elastic ? kRowsEl3Core : kRowsAc3Core;
   p->cells_of[0] = (long long)p->nxi * p->nyi * p->nzi;
   p->cells_of[1] = (long long)p->shot.nx * p->shot.ny * p->shot.nz - p->cells_of[0];
   if ((rc = p->reset())) return bail(rc);
+  if ((rc = layout3d(p, d, rows, core_rows))) return bail(rc);
   if (d->use_graph && (rc = p->build_graph())) return bail(rc);
   e = hipStreamSynchronize(p->stream);
   if (e != hipSuccess) return bail(set_error(FOMO_EHIP, std::string("setup: ") + hipGetErrorString(e)));
   *out = new fomo_plan{p};
   return FOMO_OK;
+}
+
+// Fused interior z-segments: one block per CU fits (1024 threads, 60 KB of LDS), so the columns are
+// cut into the fewest segments that still give every CU a block; each segment pays 2N - 1 extra
+// levels of vz, and tiles that march together share their halos through L2.
+static int fused_kseg_default(const Plan3D* p) {
+  const long long cols = (long long)((p->nxi + kTileW - 1) / kTileW) * ((p->nyi + kFTY - 1) / kFTY);
+  const long long nseg = std::max(1LL, (256 + cols - 1) / std::max(1LL, cols));
+  return (int)std::max(8LL, (p->nzi + nseg - 1) / nseg);
+}
+
+// The work layout of a 3D plan.  d->tile_cfg = k > 0 fixes the z-segments (sweeps and frame
+// 16 << (k-1) planes, fused interior 32 << (k-1)); FOMO_FUSED_KSEG fixes the fused interior's.
+// Otherwise the candidates are timed on the plan's own grid, as the 2D autotuner does (two
+// interleaved rounds of direct-launch steps, each candidate's faster round kept): sweep and frame
+// segments of 16, 32 and 64 planes, and, where the fused interior launch is allowed (acoustic3d,
+// Float32 storage), that family with its default segment (fused_kseg_default), half and a quarter
+// of it, against the separate sweeps (FOMO_3D_FUSED=1 keeps the fused family only).  The choice is kept per (device and library build,
+// equation, order, storage, padded dims, families) in this process and in the tuning file
+// (tune.h).  Segment lengths change no bits.
+static int layout3d(Plan3D* p, const fomo_desc* d, int rows, int core_rows) {
+  struct Cand {
+    bool fused;
+    int kcfg, kseg, kseg_fused;
+    Work3D* work[2];
+    int n_work[2];
+    Work3D* fw;
+    int n_fw;
+  };
+  const bool may_fuse = p->fused;
+  auto build = [&](Cand& c) -> int {
+    p->fused = c.fused;
+    const int rc = make_work3d(p, c.kseg, rows, core_rows, c.kseg_fused);
+    if (rc) return rc;
+    c.work[0] = p->d_work[0];
+    c.work[1] = p->d_work[1];
+    c.n_work[0] = p->n_work[0];
+    c.n_work[1] = p->n_work[1];
+    c.fw = c.fused ? p->d_fused : nullptr;
+    c.n_fw = c.fused ? p->n_fused : 0;
+    return FOMO_OK;
+  };
+  auto select = [&](const Cand& c) {
+    p->fused = c.fused;
+    p->d_work[0] = c.work[0];
+    p->d_work[1] = c.work[1];
+    p->n_work[0] = c.n_work[0];
+    p->n_work[1] = c.n_work[1];
+    p->d_fused = c.fw;
+    p->n_fused = c.n_fw;
+    p->tune_family = c.fused ? 1 : 0;
+    p->tune_shape = c.kcfg;
+    p->tune_extent = c.fused ? c.kseg_fused : c.kseg;
+  };
+  const int kf_default = fused_kseg_default(p);
+  const char* kf_env = std::getenv("FOMO_FUSED_KSEG");
+  if (d->tile_cfg > 0 || kf_env) {
+    Cand c{};
+    c.fused = may_fuse;
+    c.kcfg = d->tile_cfg > 0 ? d->tile_cfg - 1 : 1;
+    c.kseg = 16 << c.kcfg;
+    c.kseg_fused = kf_env ? std::max(8, std::atoi(kf_env)) : (d->tile_cfg > 0 ? 32 << (d->tile_cfg - 1) : kf_default);
+    const int rc = build(c);
+    if (rc) return rc;
+    select(c);
+    p->tune_source = 3;
+    return FOMO_OK;
+  }
+  std::vector<Cand> cand;
+  const bool only_fused = may_fuse && std::getenv("FOMO_3D_FUSED") != nullptr;
+  if (!only_fused)
+    for (int k = 0; k < 3; ++k) cand.push_back(Cand{false, k, 16 << k, 0, {nullptr, nullptr}, {0, 0}, nullptr, 0});
+  if (may_fuse) {
+    std::vector<int> kfs;
+    for (int div : {1, 2, 4}) {
+      const int v = std::max(8, kf_default / div);
+      if (std::find(kfs.begin(), kfs.end(), v) == kfs.end()) kfs.push_back(v);
+    }
+    for (int k = 0; k < 3; ++k)
+      for (int kf : kfs) cand.push_back(Cand{true, k, 16 << k, kf, {nullptr, nullptr}, {0, 0}, nullptr, 0});
+  }
+  const std::string key = tune_device_tag() + " 3d eq=" + std::to_string(d->eq) + " N=" + std::to_string(p->N) +
+                          " storage=" + std::to_string(p->storage) + " dims=" + std::to_string(p->shot.nx) + "x" +
+                          std::to_string(p->shot.ny) + "x" + std::to_string(p->shot.nz) +
+                          " pad=" + std::to_string(p->pad) + " fuse=" + std::to_string(may_fuse ? (only_fused ? 2 : 1) : 0);
+  std::vector<int> rec;
+  if (tune_lookup(key, rec) && rec.size() == 3) {
+    for (Cand& c : cand)
+      if ((c.fused ? 1 : 0) == rec[0] && c.kcfg == rec[1] && (!c.fused || c.kseg_fused == rec[2])) {
+        const int rc = build(c);
+        if (rc) return rc;
+        select(c);
+        p->tune_source = tune_last_source();
+        return FOMO_OK;
+      }
+  }
+  for (Cand& c : cand) {
+    const int rc = build(c);
+    if (rc) return rc;
+  }
+  hipEvent_t e0, e1;
+  FOMO_HIP_TRY(hipEventCreate(&e0));
+  FOMO_HIP_TRY(hipEventCreate(&e1));
+  const long long cells = (long long)p->shot.nx * p->shot.ny * p->shot.nz;
+  const int reps = (int)std::max<long long>(2, std::min<long long>(16, (1LL << 24) / std::max(cells, 1LL)));
+  std::vector<float> best_ms(cand.size(), 1e30f);
+  hipError_t e = hipSuccess;
+  for (int round = 0; round < 2 && e == hipSuccess; ++round)
+    for (size_t k = 0; k < cand.size() && e == hipSuccess; ++k) {
+      select(cand[k]);
+      e = p->run_steps(1);
+      if (e == hipSuccess) e = hipEventRecord(e0, p->stream);
+      if (e == hipSuccess) e = p->run_steps(reps);
+      if (e == hipSuccess) e = hipEventRecord(e1, p->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.0f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      best_ms[k] = std::min(best_ms[k], ms);
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return set_error(FOMO_EHIP, std::string("3D autotune: ") + hipGetErrorString(e));
+  size_t best = 0;
+  for (size_t k = 1; k < cand.size(); ++k)
+    if (best_ms[k] < best_ms[best]) best = k;
+  select(cand[best]);
+  p->tune_source = 0;
+  tune_store(key, {cand[best].fused ? 1 : 0, cand[best].kcfg, cand[best].fused ? cand[best].kseg_fused : 0});
+  return p->reset();
 }
 
 // Shared one-shot driver of acoustic3d and elastic3d: warm-up step, reset, timed loop with

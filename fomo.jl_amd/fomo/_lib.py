@@ -98,6 +98,7 @@ SIGNATURES = {
     "fomo_plan_sync": (C.c_int, [_p]),
     "fomo_plan_kernel_times": (C.c_int, [_p, _i, C.POINTER(C.c_double)]),
     "fomo_plan_steps_done": (C.c_int, [_p, C.POINTER(_i)]),
+    "fomo_plan_tuning": (C.c_int, [_p, _p]),
     "fomo_plan_read_snaps": (C.c_int, [_p, _i, C.POINTER(_i), _p]),
     "fomo_plan_launch_count": (C.c_int, [_p, C.POINTER(_i)]),
     "fomo_plan_launch_info": (C.c_int, [_p, _i, C.POINTER(C.c_char_p), C.POINTER(C.c_int64), C.POINTER(_i)]),

@@ -606,6 +606,15 @@ class Plan:
             _lib.check(_lib.lib().fomo_plan_read_snaps(self._h, shot, C.byref(n), out.ctypes.data))
         return out
 
+    def tuning(self) -> dict:
+        """The work layout in use (fomo_plan_tuning): family ("sweeps" or "fused"), shape index,
+        extent (2D strip rows; 3D z-planes per segment) and source ("timed", "process", "file",
+        "fixed")."""
+        out = (C.c_int32 * 4)()
+        _lib.check(_lib.lib().fomo_plan_tuning(self._h, out))
+        return {"family": ("sweeps", "fused")[out[0]], "shape": out[1], "extent": out[2],
+                "source": ("timed", "process", "file", "fixed")[out[3]]}
+
     def launches(self) -> list[dict]:
         """The kernel launches of one step: name, cells updated, algorithmic bytes per cell."""
         L = _lib.lib()

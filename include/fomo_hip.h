@@ -274,6 +274,15 @@ int fomo_plan_kernel_times(fomo_plan *p, int32_t n, double *out3);
 int fomo_plan_launch_count(const fomo_plan *p, int32_t *n);
 int fomo_plan_launch_info(const fomo_plan *p, int32_t k, const char **name, int64_t *cells, int32_t *bytes_per_cell);
 int fomo_plan_launch_times(fomo_plan *p, int32_t n, double *ms);
+/* The work layout a plan runs with (no counterpart in the reference; the autotuner's result):
+ * out4[0] family (0 separate sweeps, 1 a fused interior launch), out4[1] shape index (the
+ * descriptor's tile_cfg - 1), out4[2] its extent (2D: rows of an interior strip; 3D: z-planes of a
+ * segment of the fused interior launch, else of the sweeps), out4[3] where it came from (0 timed
+ * by this plan, 1 an earlier plan of this process, 2 the tuning file, 3 fixed by d->tile_cfg or
+ * the environment).  Choices are kept per device, library build and problem in the file named by
+ * FOMO_TUNE_CACHE (default $XDG_CACHE_HOME/fomo_hip/tune.txt or ~/.cache/fomo_hip/tune.txt;
+ * "off" disables the file).  Layouts change no bits. */
+int fomo_plan_tuning(const fomo_plan *p, int32_t *out4);
 /* Steps taken since the last reset. */
 int fomo_plan_steps_done(const fomo_plan *p, int32_t *out);
 /* Copy seismograms to host (n_rec x nt each; NULL skips).  Slots: acoustic {p, vx, vz},

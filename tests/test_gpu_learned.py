@@ -92,10 +92,10 @@ def test_zero_correction_is_plain_habc_bitwise(gpu):
 
 @pytest.mark.parametrize("storage", ["f32", "f16"])
 def test_learned_graph_equals_stream(gpu, storage):
-    """The inference runs on a side stream beside the interior strips and the frame tiles wait for
-    it (a fork and join, captured into the graph as parallel branches).  Graph replays and direct
-    launches give the same bits, for step counts ending inside and at the end of a replay, and the
-    trained network changes the result (so the corrections are read)."""
+    """The inference launch runs on the plan's stream before the frame tiles that read its blend
+    weights, inside the captured graph as in direct launches.  Graph replays and direct launches
+    give the same bits, for step counts ending inside and at the end of a replay, and the trained
+    network changes the result (so the corrections are read)."""
     import fomo
     import fomo._lib as L
     from fomo.api import Plan
