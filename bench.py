@@ -541,11 +541,15 @@ def probe3d(args):
         plan.sync()
         ms = plan.timed_advance(nt)
         tv, tp, tr = plan.kernel_times(max(4, nt // 4))
+        li, lt = plan.launches(), plan.launch_times(max(4, nt // 4))
+        layout = plan.tuning()
         plan.close()
         res[name] = {"padded_cells": cells, "ms_per_step": ms / nt, "gcell_per_s": cells * nt / ms / 1e6,
                      "step_gbs": (bv + bs) * eb * cells * nt / ms / 1e6, "vel_ms": tv,
                      "vel_gbs": bv * eb * cells / tv / 1e6, "prs_ms": tp, "prs_gbs": bs * eb * cells / tp / 1e6,
-                     "gather_ms": tr}
+                     "gather_ms": tr, "layout": layout,
+                     "launches": [{"name": x["name"], "ms": t, "gbs": x["cells"] * x["bytes_per_cell"] / t / 1e6}
+                                  for x, t in zip(li, lt)]}
         log(f"{name}: {res[name]}")
     gbs, cms = stream_copy_gbs()
     res["stream_copy"] = {"bytes": 2 << 30, "ms": cms, "gbs": gbs}
