@@ -17,14 +17,30 @@ namespace fomo {
 // value v is stored as RN(v * enc) and read back as float(stored) * dec, one pair per array.
 constexpr int kMaxF3 = 9, kMaxM3 = 8;
 
+// Row layout of every 3D array: cell (i, j, k) of the padded nx x ny x nz grid sits at element
+// i + j*px + k*px*ny from the array's base.  The pitch px is a multiple of 128 bytes and the base
+// lies xoff elements into the allocation, so that column `pad` (the first interior column) of
+// every row starts on a 128-byte boundary: the interior's tiles of 64 columns then read and write
+// whole cache lines (512^3 FD8, nbc 32: the fused interior launch takes 1.66 ms with rows at the
+// unaligned 584-cell pitch and 1.38 ms aligned, elastic3d's step 8.64 and 8.06 ms).
+struct RowPitch {
+  int xoff, px;
+};
+inline RowPitch row_pitch3d(int nx, int pad, int elem) {
+  const int a = 128 / elem;
+  const int xoff = (a - pad % a) % a;
+  return {xoff, (nx + xoff + a - 1) / a * a};
+}
+
 struct Shot3D {
-  void* fld[2][kMaxF3];
+  void* fld[2][kMaxF3];  // array bases (see RowPitch)
   const void* mat[kMaxM3];
   float fdec[kMaxF3], fenc[kMaxF3];
   float mdec[kMaxM3];
   const float* wlut;  // HABC weight tables by distance: vx, vz, vy, tau (H+1 each)
   float dtdh, q, qt, qxt;
   int nx, ny, nz, H, pad;
+  int px;  // row pitch in elements
   int n_src;
   const long long* src_c;
   const float* wav;
@@ -101,6 +117,8 @@ hipError_t launch_gather3d(int storage, const Gather3DArgs& a, hipStream_t s);
 // acoustic3d: the whole step (new velocities, then new pressure) of the interior tiles in a.work;
 // the frame velocity launch runs before it and the frame pressure launch after it.
 hipError_t launch_fused3d(int storage, int N, const Sweep3DArgs& a, hipStream_t s);
-hipError_t launch_decode3d(int storage, const void* src, float* dst, size_t n, float dec, hipStream_t s);
+// Float32 copy of a stored array without its row pitch (rows of nx values, px apart at src).
+hipError_t launch_decode3d(int storage, const void* src, float* dst, int nx, size_t rows, int px, float dec,
+                           hipStream_t s);
 
 }  // namespace fomo

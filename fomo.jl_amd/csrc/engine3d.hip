@@ -1,7 +1,8 @@
 // engine3d.hip -- 3D staggered-grid sweeps (acoustic3d, elastic3d) for gfx950 with the
 // deterministic Higdon boundary fused in.
 //
-// Padded cube, x fastest, then y, then z; cell (i,j,k) at i + j*nx + k*nx*ny.  One launch per
+// Padded cube, x fastest, then y, then z; cell (i,j,k) at i + j*px + k*px*ny (px: the row pitch,
+// a multiple of 128 bytes with the interior's first column aligned).  One launch per
 // phase over a work list of column tiles (64 x rows cells in x-y) that march down z.  A tile
 // keeps, per thread, the fields read through z-stencils in register queues, stages the planes
 // read through x/y stencils in LDS with their halos, and reads every other array once.
@@ -135,8 +136,8 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
   constexpr int TR = kTR<PH, LAG>, PW = L::PW, PR = L::PR, HYR = (2 * N + TR - 1) / TR;
   float* ring = lds + 2 * L::STG;  // pre-correction values of plane t in slot t % 2, old values in slot 2 + t % 2
   const Shot3D& S = A.shot;
-  const int nx = S.nx, ny = S.ny, nz = S.nz, H = S.H;
-  const size_t sz = (size_t)nx * ny;
+  const int nx = S.nx, ny = S.ny, nz = S.nz, H = S.H, px = S.px;
+  const size_t sz = (size_t)px * ny;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int s = A.parity, o = 1 - s;
   const int sb = P::kNew ? o : s;
@@ -163,7 +164,7 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
       jj = min(max(jj, cy_lo), cy_hi);
     }
     kk = min(max(kk, 0), nz - 1);  // uniform: scalar instructions (the loads of a level past an item's last)
-    const unsigned off = (unsigned)(ii + jj * nx);
+    const unsigned off = (unsigned)(ii + jj * px);
     return Cvt<T>::ld(plane_of<T>(S.fld[buf][f], kk, sz), off, S.fdec[f]);
   };
 
@@ -228,7 +229,7 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
       // region are loaded too and not used
       const int ic = LAG ? min(max(i, cx_lo), cx_hi) : i, jc = LAG ? min(max(j, cy_lo), cy_hi) : j;
       const int tc = min(max(tp, 0), nz - 1);
-      const unsigned co = (unsigned)(ic + jc * nx);
+      const unsigned co = (unsigned)(ic + jc * px);
 #pragma unroll
       for (int f = 0; f < NF; ++f)
         oldF[st][r][f] = Cvt<T>::ld(plane_of<T>(S.fld[s][P::out_f(f)], tc, sz), co, S.fdec[P::out_f(f)]);
@@ -260,7 +261,7 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
       for (int q = 0; q < S.n_src; ++q) {
         const long long cs = (long long)G(S.src_c)[q];
         const long long ks = cs / (long long)sz, rem = cs - ks * (long long)sz;
-        const int js = (int)(rem / nx), is = (int)(rem - (long long)js * nx);
+        const int js = (int)(rem / px), is = (int)(rem - (long long)js * px);
         if (is != i || ks < W.k0 || ks >= W.k0 + W.hz) continue;
 #pragma unroll
         for (int r = 0; r < YPT; ++r) src_here = src_here || js == pj0 + ty + TR * r;
@@ -407,7 +408,7 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
       for (int r = 0; r < YPT; ++r) {
         const int jl = ty + TR * r;
         if (!col_out || jl >= W.hy) continue;
-        const unsigned co = (unsigned)(i + (pj0 + jl) * nx);
+        const unsigned co = (unsigned)(i + (pj0 + jl) * px);
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
           float v = preN[r][f];
@@ -444,7 +445,7 @@ __device__ __forceinline__ void march3d(const Sweep3DArgs& A, const Work3D& W, f
       for (int r = 0; r < YPT; ++r) {
         const int jl = ty + TR * r, j = pj0 + jl;
         if (!col_out || jl < ry || jl >= ry + W.hy) continue;
-        const unsigned co = (unsigned)(i + j * nx);
+        const unsigned co = (unsigned)(i + j * px);
         const bool jf = j <= H, jb = j >= ny - 1 - H;
         const int dist = min(min(min(i, nx - 1 - i), min(j, ny - 1 - j)), min(kk, nz - 1 - kk));
         const bool frame = dist <= H;
@@ -571,26 +572,36 @@ __global__ __launch_bounds__(256) void gather3d_kernel(const Gather3DArgs A) {
       const int i = A.snap_plane == 2 ? A.snap_idx : pad + aa;
       const int j = A.snap_plane == 0 ? A.snap_idx : (A.snap_plane == 1 ? pad + bb : pad + aa);
       const int k = A.snap_plane == 1 ? A.snap_idx : pad + bb;
-      const size_t c = (size_t)i + (size_t)j * S.nx + (size_t)k * S.nx * S.ny;
+      const size_t c = (size_t)i + (size_t)j * S.px + (size_t)k * S.px * S.ny;
       G(dst)[e] = Cvt<T>::ld(S.fld[A.parity_out][A.snap_f], c, S.fdec[A.snap_f]);
     }
   }
 }
 
-// Decode a stored array to Float32 (dst[c] = value of cell c).
+// Decode a stored array to Float32 without its pitch: dst[i + nx*r] = value at i + px*r of the
+// array base, for rows r < rows.
 template <class T>
-__global__ void decode_kernel(const void* src, float* dst, size_t n, float dec) {
-  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x)
-    dst[c] = Cvt<T>::ld(src, c, dec);
+__global__ void decode_kernel(const void* src, float* dst, int nx, size_t rows, int px, float dec) {
+  const size_t n = (size_t)nx * rows;
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = c / (size_t)nx, i = c - r * (size_t)nx;
+    dst[c] = Cvt<T>::ld(src, i + r * (size_t)px, dec);
+  }
 }
 
-hipError_t launch_decode3d(int storage, const void* src, float* dst, size_t n, float dec, hipStream_t s) {
+hipError_t launch_decode3d(int storage, const void* src, float* dst, int nx, size_t rows, int px, float dec,
+                           hipStream_t s) {
+  const size_t n = (size_t)nx * rows;
   const int blocks = (int)std::min<size_t>(4096, (n + 255) / 256);
   switch (storage) {
-    case FOMO_STORE_F32: hipLaunchKernelGGL(decode_kernel<float>, dim3(blocks), dim3(256), 0, s, src, dst, n, dec); break;
-    case FOMO_STORE_F16: hipLaunchKernelGGL(decode_kernel<__half>, dim3(blocks), dim3(256), 0, s, src, dst, n, dec); break;
+    case FOMO_STORE_F32:
+      hipLaunchKernelGGL(decode_kernel<float>, dim3(blocks), dim3(256), 0, s, src, dst, nx, rows, px, dec);
+      break;
+    case FOMO_STORE_F16:
+      hipLaunchKernelGGL(decode_kernel<__half>, dim3(blocks), dim3(256), 0, s, src, dst, nx, rows, px, dec);
+      break;
     case FOMO_STORE_BF16:
-      hipLaunchKernelGGL(decode_kernel<__hip_bfloat16>, dim3(blocks), dim3(256), 0, s, src, dst, n, dec);
+      hipLaunchKernelGGL(decode_kernel<__hip_bfloat16>, dim3(blocks), dim3(256), 0, s, src, dst, nx, rows, px, dec);
       break;
     default: return hipErrorInvalidValue;
   }

@@ -51,8 +51,8 @@ __device__ __forceinline__ void fused3d_body(const Sweep3DArgs& A, const Work3D&
   constexpr int TY = Gm::TY, PH = Gm::PH, PSW = Gm::PSW, VXW = Gm::VXW;
   constexpr int POWN = Gm::POWN, PVX = Gm::PVX, PVY = Gm::PVY, PPH = Gm::PPH;
   const Shot3D& S = A.shot;
-  const int nx = S.nx, ny = S.ny, nz = S.nz, pad = S.pad;
-  const size_t sz = (size_t)nx * ny;
+  const int nx = S.nx, ny = S.ny, nz = S.nz, pad = S.pad, px = S.px;
+  const size_t sz = (size_t)px * ny;
   const int tid = threadIdx.x + kTileW * threadIdx.y;
   const int s = A.parity, o = 1 - s;
   const int i0 = W.i0, j0 = W.j0, k0 = W.k0, w = W.w, hy = W.hy, hz = W.hz;
@@ -90,7 +90,7 @@ __device__ __forceinline__ void fused3d_body(const Sweep3DArgs& A, const Work3D&
   const float fenc[4] = {S.fenc[0], S.fenc[1], S.fenc[2], S.fenc[3]};
   const float mdec[4] = {S.mdec[0], S.mdec[1], S.mdec[2], S.mdec[3]};
   auto ldp = [&](const void* base, float dec, int i, int j, int k) -> float {
-    return Cvt<T>::ld(plane_of<T>(base, k, sz), (unsigned)(i + j * nx), dec);
+    return Cvt<T>::ld(plane_of<T>(base, k, sz), (unsigned)(i + j * px), dec);
   };
   auto ldf = [&](bool old, int f, int i, int j, int k) -> float {
     return ldp(old ? old_f[f] : (const void*)new_f[f], fdec[f], i, j, k);
@@ -114,7 +114,7 @@ __device__ __forceinline__ void fused3d_body(const Sweep3DArgs& A, const Work3D&
     for (int q = 0; q < S.n_src; ++q) {
       const long long cs = (long long)G(S.src_c)[q];
       const long long ks = cs / (long long)sz, rem = cs - ks * (long long)sz;
-      const int js = (int)(rem / nx), is = (int)(rem - (long long)js * nx);
+      const int js = (int)(rem / px), is = (int)(rem - (long long)js * px);
       if (ks < k0 || ks >= k0 + hz) continue;
 #pragma unroll
       for (int r = 0; r < POWN; ++r) src_here = src_here || (own_ok[r] && is == oi[r] && js == oj[r]);
@@ -222,7 +222,7 @@ __device__ __forceinline__ void fused3d_body(const Sweep3DArgs& A, const Work3D&
         for (int l = 1; l <= N; ++l) dz += a[l - 1] * (pq[r][N - 1 + l] - pq[r][N - l]);
         v = P.vz[r] + (P.bz[r] * dtdh) * dz;
       }
-      if (outp && own_ok[r]) st_field<T>(plane_of<T>(new_f[3], tau, sz), (unsigned)(oi[r] + oj[r] * nx), v, fenc[3], S.ovf);
+      if (outp && own_ok[r]) st_field<T>(plane_of<T>(new_f[3], tau, sz), (unsigned)(oi[r] + oj[r] * px), v, fenc[3], S.ovf);
       // the pressure update reads what the separate sweeps would read back: the stored value
       if (cz) v = round_trip<T>(v, fenc[3], fdec[3]);
 #pragma unroll
@@ -246,7 +246,7 @@ __device__ __forceinline__ void fused3d_body(const Sweep3DArgs& A, const Work3D&
           for (int l = 1; l <= N; ++l) dx += a[l - 1] * (P0[l - 1] - P0[-l]);
           v = P.vx[q] + (P.bx[q] * dtdh) * dx;
           if (c >= 0 && c < w && r < hy)
-            st_field<T>(plane_of<T>(new_f[1], tau, sz), (unsigned)(i0 + c + (j0 + r) * nx), v, fenc[1], S.ovf);
+            st_field<T>(plane_of<T>(new_f[1], tau, sz), (unsigned)(i0 + c + (j0 + r) * px), v, fenc[1], S.ovf);
           v = round_trip<T>(v, fenc[1], fdec[1]);
         }
         rx_[r * VXW + cc] = v;
@@ -264,7 +264,7 @@ __device__ __forceinline__ void fused3d_body(const Sweep3DArgs& A, const Work3D&
           for (int l = 1; l <= N; ++l) dy += a[l - 1] * (P0[(l - 1) * PSW] - P0[-l * PSW]);
           v = P.vy[q] + (P.by[q] * dtdh) * dy;
           if (r >= 0 && r < hy && c < w)
-            st_field<T>(plane_of<T>(new_f[2], tau, sz), (unsigned)(i0 + c + (j0 + r) * nx), v, fenc[2], S.ovf);
+            st_field<T>(plane_of<T>(new_f[2], tau, sz), (unsigned)(i0 + c + (j0 + r) * px), v, fenc[2], S.ovf);
           v = round_trip<T>(v, fenc[2], fdec[2]);
         }
         ry_[rr * kTileW + c] = v;
@@ -290,7 +290,7 @@ __device__ __forceinline__ void fused3d_body(const Sweep3DArgs& A, const Work3D&
           dz += cl * (vzq[r][N + l - 1] - vzq[r][N - l]);
         }
         float v = pq[r][0] + (P.kap[r] * dtdh) * ((dx + dy) + dz);
-        const unsigned co = (unsigned)(oi[r] + oj[r] * nx);
+        const unsigned co = (unsigned)(oi[r] + oj[r] * px);
         if (src_here) inject3d(S, (size_t)co + (size_t)t * sz, it, A.nt, v);
         if (own_ok[r]) st_field<T>(plane_of<T>(new_f[0], t, sz), co, v, fenc[0], S.ovf);
       }

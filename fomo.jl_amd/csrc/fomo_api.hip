@@ -19,6 +19,7 @@
 
 #include "../../include/fomo_hip.h"
 #include "fomo_common.h"
+#include "engine3d.h"
 #include "plan.h"
 #include "plan2d.h"
 
@@ -409,12 +410,16 @@ static int validate_common(const fomo_desc* d) {
     if (d->eq != FOMO_ACOUSTIC2D && d->eq != FOMO_ELASTIC2D)
       return fail(FOMO_EUNSUP, "the sponge boundary is 2D only (as in the reference)");
   }
-  // The sweeps address cells with 32-bit offsets: a 2D field, or one z plane of a 3D field, may
-  // hold at most 2^30 padded cells (its Float32 byte offsets then stay below 2^32).
+  // The sweeps address cells with 32-bit offsets: a 2D field, or one z plane of a 3D field (rows
+  // at their pitch), may hold at most 2^30 padded cells (its Float32 byte offsets then stay below
+  // 2^32).
   {
     const long long pad = (long long)d->nbc + d->fd_order / 2;
-    const long long px = d->nx + 2 * pad, py = d->ny + 2 * pad, pz = d->nz + 2 * pad;
+    long long px = d->nx + 2 * pad;
+    const long long py = d->ny + 2 * pad, pz = d->nz + 2 * pad;
     const bool is3d = d->eq == FOMO_ACOUSTIC3D || d->eq == FOMO_ELASTIC3D;
+    if (is3d && px < (1LL << 30))
+      px = row_pitch3d((int)px, (int)pad, d->storage == FOMO_STORE_F32 ? 4 : 2).px;
     const long long cells = is3d ? px * py : px * pz;
     if (cells > (1LL << 30))
       return fail(FOMO_EUNSUP, std::string(is3d ? "3D plane" : "2D field") + " of " + std::to_string(cells) +

@@ -40,6 +40,8 @@ struct Plan3D : PlanBase {
   int snap_f = 0;   // field of the snapshots: acoustic p, elastic vz
   float coef[kMaxN] = {0};
   Shot3D shot{};
+  int xoff = 0;        // elements from an array's allocation to its base (row_pitch3d)
+  size_t alloc_n = 0;  // elements of each array's allocation: shot.px * ny * nz
   Work3D* d_work[2] = {nullptr, nullptr};  // core items, frame items
   int n_work[2] = {0, 0};
   // acoustic3d: the interior's two phases in one launch (engine3d_fused.hip) over these items,
@@ -113,9 +115,9 @@ struct Plan3D : PlanBase {
   }
 
   int reset() override {
-    const size_t n = (size_t)shot.nx * shot.ny * shot.nz;
     for (int b = 0; b < 2; ++b)
-      for (int f = 0; f < nf; ++f) FOMO_HIP_TRY(hipMemsetAsync(shot.fld[b][f], 0, n * elem, stream));
+      for (int f = 0; f < nf; ++f)
+        FOMO_HIP_TRY(hipMemsetAsync((char*)shot.fld[b][f] - (size_t)xoff * elem, 0, alloc_n * elem, stream));
     const size_t ns = (size_t)d.n_rec * d.nt;
     for (int k = 0; k < nslots; ++k)
       if (ns) FOMO_HIP_TRY(hipMemsetAsync(seis[k], 0, ns * sizeof(float), stream));
@@ -164,7 +166,8 @@ struct Plan3D : PlanBase {
     const size_t n = (size_t)shot.nx * shot.ny * shot.nz;
     int rc = ensure_scratch(n);
     if (rc) return rc;
-    FOMO_HIP_TRY(launch_decode3d(storage, shot.fld[steps_done % 2][which], scratch, n, shot.fdec[which], stream));
+    FOMO_HIP_TRY(launch_decode3d(storage, shot.fld[steps_done % 2][which], scratch, shot.nx, (size_t)shot.ny * shot.nz,
+                                 shot.px, shot.fdec[which], stream));
     FOMO_HIP_TRY(hipMemcpyAsync(out, scratch, n * sizeof(float), hipMemcpyDeviceToHost, stream));
     FOMO_HIP_TRY(hipStreamSynchronize(stream));
     return FOMO_OK;
@@ -326,26 +329,30 @@ static int make_work3d(Plan3D* p, int kseg, int rows, int core_rows, int kseg_fu
   return rc;
 }
 
+// Upload a padded array (rows of nx values, x fastest) as the storage type (16-bit: RN(v * enc))
+// in the arrays' row layout; *dst receives the array base.  The pitch padding holds zeros.
 template <class T>
 static int upload_stored(Plan3D* p, void** dst, const std::vector<float>& h, float enc) {
-  if (p->storage == FOMO_STORE_F32) {
-    float* d = nullptr;
-    int rc = p->upload(&d, h.data(), h.size());
-    *dst = d;
-    return rc;
-  }
-  std::vector<T> tmp(h.size());
-  parallel_for((long long)h.size(), [&](long long a, long long b) {
-    for (long long c = a; c < b; ++c) {
-      if constexpr (std::is_same<T, __half>::value)
-        tmp[c] = __float2half_rn(h[c] * enc);
-      else
-        tmp[c] = __float2bfloat16(h[c] * enc);
+  const Shot3D& S = p->shot;
+  const long long rows = (long long)S.ny * S.nz;
+  std::vector<T> tmp(p->alloc_n);
+  parallel_for(rows, [&](long long a, long long b) {
+    for (long long r = a; r < b; ++r) {
+      const float* in = h.data() + (size_t)r * S.nx;
+      T* out = tmp.data() + p->xoff + (size_t)r * S.px;
+      for (int i = 0; i < S.nx; ++i) {
+        if constexpr (std::is_same<T, float>::value)
+          out[i] = in[i];
+        else if constexpr (std::is_same<T, __half>::value)
+          out[i] = __float2half_rn(in[i] * enc);
+        else
+          out[i] = __float2bfloat16(in[i] * enc);
+      }
     }
   });
   T* d = nullptr;
   int rc = p->upload(&d, tmp.data(), tmp.size());
-  *dst = d;
+  *dst = d ? d + p->xoff : nullptr;
   return rc;
 }
 
@@ -409,6 +416,12 @@ int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, con
   S.nz = d->nz + 2 * pad;
   S.H = pad - 1;
   S.pad = pad;
+  {
+    const RowPitch rp = row_pitch3d(S.nx, pad, p->elem);
+    S.px = rp.px;
+    p->xoff = rp.xoff;
+    p->alloc_n = (size_t)S.px * S.ny * S.nz;
+  }
   S.dtdh = d->dt / d->dh;
   hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
   if (e != hipSuccess) return bail(set_error(FOMO_EHIP, std::string("stream: ") + hipGetErrorString(e)));
@@ -572,8 +585,8 @@ int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, con
   for (int b = 0; b < 2; ++b)
     for (int f = 0; f < p->nf; ++f) {
       char* ptr = nullptr;
-      if ((rc = p->alloc(&ptr, n * p->elem))) return bail(rc);
-      S.fld[b][f] = ptr;
+      if ((rc = p->alloc(&ptr, p->alloc_n * p->elem))) return bail(rc);
+      S.fld[b][f] = ptr + (size_t)p->xoff * p->elem;
     }
   {
     std::vector<float> lut(4 * (size_t)(S.H + 1));
@@ -588,9 +601,10 @@ int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, con
   {
     S.n_src = d->n_src;
     std::vector<long long> sc(std::max(d->n_src, 1), -1);
+    // cells as element offsets from the array base (row pitch S.px)
+    const long long dy = S.px, dz = (long long)S.px * S.ny;
     for (int q = 0; q < d->n_src; ++q)
-      sc[q] = (long long)(sx[q] - 1 + pad) + (long long)(sy[q] - 1 + pad) * (long long)sy_ +
-              (long long)(sz[q] - 1 + pad) * (long long)sz_;
+      sc[q] = (long long)(sx[q] - 1 + pad) + (long long)(sy[q] - 1 + pad) * dy + (long long)(sz[q] - 1 + pad) * dz;
     long long* dsc = nullptr;
     if ((rc = p->upload(&dsc, sc.data(), sc.size()))) return bail(rc);
     S.src_c = dsc;
@@ -600,8 +614,7 @@ int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, con
     S.wav = dw;
     std::vector<long long> rcell(std::max(d->n_rec, 1), 0);
     for (int r = 0; r < d->n_rec; ++r)
-      rcell[r] = (long long)(rx[r] - 1 + pad) + (long long)(ry[r] - 1 + pad) * (long long)sy_ +
-                 (long long)(rz[r] - 1 + pad) * (long long)sz_;
+      rcell[r] = (long long)(rx[r] - 1 + pad) + (long long)(ry[r] - 1 + pad) * dy + (long long)(rz[r] - 1 + pad) * dz;
     if ((rc = p->upload(&p->rec_c, rcell.data(), rcell.size()))) return bail(rc);
     p->nslots = d->record_p ? 4 : 3;
     for (int k = 0; k < p->nslots; ++k)
