@@ -308,8 +308,10 @@ int fomo_trace_misfit(fomo_plan *pa, int32_t shot_a, fomo_plan *pb, int32_t shot
 int fomo_trace_misfit_noalloc(fomo_plan *pa, int32_t shot_a, fomo_plan *pb, int32_t shot_b, int32_t slot,
                               int32_t t0, int32_t t1, double *num, double *den);
 /* Copy the current padded field number `which` (order: acoustic2d p,vx,vz; elastic2d
- * vx,vz,txx,tzz,txz; coupled2d P,S,dP/dt,dS/dt; acoustic3d p,vx,vy,vz) to host as Float32 (decoded from the storage
- * precision when reduced-precision storage is used). */
+ * vx,vz,txx,tzz,txz; coupled2d P,S,dP/dt,dS/dt; acoustic3d p,vx,vy,vz; elastic3d
+ * vx,vy,vz,txx,tyy,tzz,txy,txz,tyz) to host as a dense padded Float32 array, x fastest (decoded from
+ * the storage precision when reduced-precision storage is used; 3D plans keep rows at a 128-byte
+ * pitch on the device, which the copy drops). */
 int fomo_plan_read_field(fomo_plan *p, int32_t which, float *out);
 /* Snapshots of a plan created with d->snap_interval > 0 (2D single-shot and 3D plans): writes the
  * number of snapshot slots to *count and, when out is not NULL, copies every slot (count x the
