@@ -317,3 +317,57 @@ def test_acoustic3d_fused_interior_equals_two_sweeps(gpu, order, storage, monkey
     for x, y in zip(outs[0], outs[1]):
         assert np.array_equal(x, y)
     assert np.abs(outs[0][4]).max() > 0
+
+
+# ---------------------------------------------------------------------------------------------
+# row layout
+
+
+@pytest.mark.parametrize("nbc,nx", [(28, 40), (11, 49), (13, 52), (3, 62)])
+def test_row_layouts_bitwise_vs_oracle(gpu, orc, nbc, nx):
+    """3D arrays sit in rows of a 128-byte pitch whose base is offset so that the interior's first
+    column starts a line.  Pads that put that column at different positions of a line (offsets 0,
+    17, 15 and 25 floats), a grid whose offset rows fill their pitch exactly (nbc 11, nx 49) and
+    small pads all give the oracle's bits in traces and final fields, through the fused interior
+    and through the separate sweeps (FD8), and for elastic3d (FD4)."""
+    import fomo
+    ny, nz, nt = 30, 34, 40
+    vp, rho = model3d(nx, ny, nz, 21)
+    geo = geom3d(nx, ny, nz)
+    w = fomo.ricker_wavelet(15.0, 1e-3, nt)
+    sx, sy, sz, rx, ry, rz = geo
+    print(f"row layout: nx {nx} nbc {nbc}", flush=True)
+    ref = orc.acoustic3d(vp, rho, 10.0, 1e-3, nt, sx=sx, sy=sy, sz=sz, rx=rx, ry=ry, rz=rz,
+                         wavelet=np.stack([w, w]), nbc=nbc, fd_order=8, final_state=True)
+    seis, state = run_plan(vp, rho, nt, w, geo, nbc, 8)
+    for got, key in zip(seis, ("seis_vx", "seis_vy", "seis_vz", "seis_p")):
+        assert np.array_equal(got, ref[key]), key
+    for got, key in zip(state, ("p", "vx", "vy", "vz")):
+        assert np.array_equal(got, ref["state"][key]), key
+    vs = (vp / 1.75).astype(np.float32)
+    ref = orc.elastic3d(vp, vs, rho, 10.0, 1e-3, nt, sx=sx, sy=sy, sz=sz, rx=rx, ry=ry, rz=rz,
+                        wavelet=np.stack([w, w]), nbc=nbc, fd_order=4, final_state=True)
+    seis, state = run_plan_el(vp, vs, rho, nt, w, geo, nbc, 4)
+    for got, key in zip(seis, ("seis_vx", "seis_vy", "seis_vz")):
+        assert np.array_equal(got, ref[key]), key
+    for got, key in zip(state, E3_FIELDS):
+        assert np.array_equal(got, ref["state"][key]), key
+
+
+@pytest.mark.parametrize("storage", [1, 2])
+def test_row_layout_16bit_traces_equal_field_reads(gpu, storage):
+    """With 16-bit storage the rows are padded to 64 elements (128 bytes) and the base offset
+    differs from Float32's: the receiver gather (cells addressed through the pitch) and the
+    decoded field read (which drops the pitch) return the same values at the receivers."""
+    import fomo
+    nx, ny, nz, nt, nbc = 45, 31, 33, 30, 13
+    vp, rho = model3d(nx, ny, nz, 22)
+    geo = geom3d(nx, ny, nz)
+    w = fomo.ricker_wavelet(15.0, 1e-3, nt)
+    seis, state = run_plan(vp, rho, nt, w, geo, nbc, 8, storage=storage)
+    sx, sy, sz, rx, ry, rz = geo
+    pad = nbc + 4
+    for k, f in ((0, 1), (1, 2), (2, 3), (3, 0)):  # vx, vy, vz, p traces against fields vx, vy, vz, p
+        at = state[f][rx - 1 + pad, ry - 1 + pad, rz - 1 + pad]
+        assert np.abs(seis[k][:, -1]).max() > 0
+        assert np.array_equal(seis[k][:, -1], at), k
