@@ -336,6 +336,58 @@ def probe(args):
     print(json.dumps({"probe": f"{n}x{n} interior, fd8, nbc 50", **res}), flush=True)
 
 
+def learned_line(nt=300):
+    """The learned thin boundary beside plain HABC, for the default line: step times on the learned
+    workload's 2400x1200 grid (dh 10 m, dt 0.8 ms, FD8, seeded layered model; the first `nt` of
+    its 3000 steps) for the learned boundary at nbc 16 (FP32 and FP16 storage) and plain HABC at
+    nbc 16, 32 and 100, and the reflection of each in the reference's absorption experiment
+    (test/verify_patch.jl:76-91, 301x201, 1500 steps).  `bench.py --learned` adds the full width
+    ladder and the CPU baseline."""
+    import numpy as np
+
+    import fomo
+    import fomo._lib as L
+    from fomo import learned
+    from fomo.api import Plan
+    from fomo.models import layered_2d
+    nx, nz, dt = 2400, 1200, 8e-4
+    vp, _, rho = layered_2d(nx, nz, seed=SEED, vacuum_top=False)
+    w = fomo.ricker_wavelet(15.0, dt, nt)
+    rx = np.arange(1, nx + 1, 2)
+    weights = learned.load_weights()
+    rows = (("learned_nbc16", 16, weights, "f32"), ("learned_nbc16_f16", 16, weights, "f16"),
+            ("habc_nbc16", 16, None, "f32"), ("habc_nbc32", 32, None, "f32"), ("habc_nbc100", 100, None, "f32"))
+    res = {}
+    for name, nbc, wts, storage in rows:
+        p = Plan(L.ACOUSTIC2D, vp, rho, 10.0, dt, nt, sx=[nx // 2], sz=[50], rx=rx, rz=np.full(len(rx), 10),
+                 wavelet=w[None], nbc=nbc, fd_order=8, storage=storage)
+        try:
+            if wts is not None:
+                learned.enable(p, wts)
+            p.advance(min(20, nt))
+            p.reset()
+            ms = p.timed_advance(nt)
+        finally:
+            p.close()
+        res[name] = {"ms_per_step": ms / nt}
+    vp2, rho2, sx, sz, rx2, rz2 = learned.patch_geometry()
+    refl = learned.reflection_residual(vp2, rho2, sx=sx, sz=sz, rx=rx2, rz=rz2, nt=1500, L=270, judge="learned_nbc16",
+                                       boundaries=[(name, nbc, wts, storage) for name, nbc, wts, storage in rows])
+    for k, v in refl.items():
+        if k in res:
+            if v.get("storage_overflow"):
+                res[k]["reflection_storage_overflow"] = True
+            else:
+                res[k]["reflection_db"] = v["reflection_db"]
+    t = res["learned_nbc16"]["ms_per_step"]
+    return {"workload": f"acoustic2d 2400x1200, fd8, dt 0.8 ms, first {nt} of 3000 steps; reflection from the "
+                        "reference's absorption experiment (301x201, 1500 steps)",
+            **res,
+            "learned_speedup_vs_habc_nbc100": res["habc_nbc100"]["ms_per_step"] / t,
+            "learned_speedup_vs_habc_nbc32": res["habc_nbc32"]["ms_per_step"] / t,
+            "learned_f16_speedup_vs_habc_nbc32": res["habc_nbc32"]["ms_per_step"] / res["learned_nbc16_f16"]["ms_per_step"]}
+
+
 def learned_bench(args):
     """Learned thin boundary: acoustic2d 2400x1200, dh 10 m, dt 0.8 ms, FD8, seeded layered model;
     per-step time of plain HABC at nbc 100 and of the learned boundary at nbc 16 (MLP inference
@@ -773,6 +825,8 @@ def headline(args):
                                       (m16 * 1e-3), "speedup_vs_f32": ms / m16}
         log("extra: elastic2d 2000x800")
         ex["elastic2d_2000x800"] = elastic2d_line(args)
+        log("extra: learned thin boundary against plain HABC")
+        ex["learned_2400x1200"] = learned_line()
         out["extras"] = ex
     if not args.no_cpu:
         log("timing the CPU baseline (oracle acoustic3d at 512^3)")
