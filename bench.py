@@ -270,12 +270,14 @@ def elastic2d_line(args):
     plan.sync()
     plan.reset()
     ms = plan.timed_advance(NT)
-    lt = plan.launch_times(50)
+    li, lt = plan.launches(), plan.launch_times(50)
+    layout = plan.tuning()
     plan.close()
+    # the autotuner picks the two sweeps or the fused interior step (frame launches beside it)
     return {"workload": f"elastic2d {NX}x{NZ}, fd8, nbc {NBC}, {NT} steps", "padded_cells": n_pad,
-            "cell_updates_per_s": n_pad * NT / (ms * 1e-3), "ms_per_step": ms / NT,
-            "launch_ms": {"velocity_sweep": lt[0], "stress_sweep": lt[1]},
-            "stress_sweep_gbs": BYTES_STR * n_pad / (lt[1] * 1e-3) / 1e9}
+            "cell_updates_per_s": n_pad * NT / (ms * 1e-3), "ms_per_step": ms / NT, "layout": layout,
+            "launch_ms": {x["name"]: t for x, t in zip(li, lt)},
+            "step_gbs": (BYTES_VEL + BYTES_STR) * n_pad * NT / (ms * 1e-3) / 1e9}
 
 
 def probe(args):
@@ -498,7 +500,7 @@ def batch_bench(args):
         rxl = np.arange(1, wd + 1, 2)
         shots.append(dict(x0=x0, nx=wd, sx=[wd // 2], sz=[5], rx=rxl, rz=np.full(len(rxl), 5), wavelet=w[None]))
     res = {}
-    fused_family = not elastic and not os.environ.get("FOMO_2D_UNFUSED")
+    fused_family = not os.environ.get("FOMO_2D_UNFUSED")
     for cfg in (0, 1, 2, 3):
         log(f"64-shot plan, tile_cfg {cfg} (0: autotuned): setup, graph capture, {nt} timed steps")
         plan = Plan(eq2, vp, rho, 10.0, dt, nt, vs=vs if elastic else None, shots=shots, nbc=nbc, fd_order=8,
@@ -517,7 +519,7 @@ def batch_bench(args):
     out = {"batch": f"64 uneven windows (widths 256..1536) of a 4000x1000 model, {args.batch_eq} fd8, nbc 40, "
                     f"dt 0.8 ms, "
                     f"{nt} steps", "padded_cells": cells, "autotuned": res[0],
-           # a forced tile shape of an acoustic2d plan is one of the fused interior's strip lengths
+           # a forced tile shape of an acoustic2d or elastic2d plan is one of the fused interior's strip lengths
            "fixed_strip_rows": {str(r): res[c] for c, r in zip((1, 2, 3), (64, 96, 128) if fused_family else (16, 32, 64))},
            "widths": [s["nx"] for s in shots]}
     if not args.no_cpu:

@@ -745,7 +745,7 @@ struct MarchGeom {
   static constexpr int LDS = 2 * SW + NR * NT;
 };
 
-int fused2d_width(int N) { return 257 - 2 * N; }
+int fused2d_width(int eq, int N) { return eq == FOMO_ELASTIC2D ? 256 - 2 * N : 257 - 2 * N; }
 
 template <int N, class T, class TM>
 __device__ __forceinline__ void march_strip(const SweepArgs& A, const Work2D& W, const Shot2D& S, float* lds,
@@ -916,6 +916,299 @@ __device__ __forceinline__ void march_strip(const SweepArgs& A, const Work2D& W,
   }
 }
 
+// Fused elastic interior strip: the whole step (new vx, vz, then new txx, tzz, txz) of a strip of
+// 256 - 2N columns by W.h rows in one pass down its rows.  The separate sweeps move 80 B per cell
+// (velocity: read txx, tzz, txz, vx, vz, bx, bz, write vx, vz; stress: read vx, vz, txx, tzz, txz,
+// lam, lam2mu, muxz, write txx, tzz, txz); here each of the ten arrays is read and each of the
+// five fields written once (60 B), plus halos shared through L2.
+//
+// Lane k of the 256 owns column c = i0 - N + k; the strip's columns are lanes N .. N+w-1.  Each lane
+// reads each stress of its column once, into register queues that reach from the stress row
+// t = tau - LP (its old value) to the velocities' z-stencils (txz rows up to tau + N - 1, tzz up
+// to tau + N), and keeps its column's new vx and vz of the last 2N + 1 rows (the stresses'
+// z-stencils).  At row tau the block
+//   A. stages txx(tau) and txz(tau) from the queues (plus 2N halo columns) into one of two LDS
+//      rows, and computes the new stresses of row t on the strip's columns from the velocity rings
+//      (x) and queues (z);
+//   B. after one barrier, computes new vx(tau), vz(tau) on all 256 columns (the stresses' x-stencils
+//      reach N columns past the strip) into rings of LP + 1 LDS rows.
+// The stress row lags by LP = N + 1: its z-stencil reaches new vx(t + N), computed in phase B of
+// row t + N and in the lane's queue from row t + N + 1 on; and the rings' row t, written in phase
+// B of row t, is behind the barrier of row t + 1 before phase A of row t + 2 reads it.  As in the
+// acoustic march, velocities of cells in the user's interior are recomputed from the old state
+// (buffers [s]) and those of frame cells read from the frame velocity launch, which runs first
+// (buffers [1-s]); every value is the separate sweeps' expression in their order
+// (src/equations/elastic2d/fused_kernels.jl:21-124), and 16-bit storage rounds the velocities as
+// the stress sweep would read them back, so the result is bit-identical to the separate sweeps.
+template <int N>
+struct MarchGeomEl {
+  static constexpr int NT = 256;
+  static constexpr int W = NT - 2 * N;       // columns of a strip
+  static constexpr int SW = NT + 2 * N;      // staged row: columns c(0) - N .. c(255) + N
+  static constexpr int LP = N + 1;           // stress row = velocity row - LP
+  static constexpr int NXX = LP + 1;         // txx rows t .. tau
+  static constexpr int NXZ = 2 * N + 1;      // txz rows t .. tau + N - 1
+  static constexpr int NZZ = 2 * N + 2;      // tzz rows t .. tau + N
+  static constexpr int NV = 2 * N + 1;       // new vx / vz queues: rows tau - NV .. tau - 1 at phase A
+  static constexpr int NR = LP + 1;          // rows of the velocity rings
+  static constexpr int D = 2;                // rows of loads in flight
+  static constexpr int LDS = 2 * 2 * SW + 2 * NR * NT;
+};
+
+template <int N, class T, class TM>
+__device__ __forceinline__ void march_strip_el(const SweepArgs& A, const Work2D& W, const Shot2D& S, float* lds,
+                                               int it) {
+  using Gm = MarchGeomEl<N>;
+  constexpr int NT = Gm::NT, SW = Gm::SW, LP = Gm::LP, NXX = Gm::NXX, NXZ = Gm::NXZ, NZZ = Gm::NZZ, NV = Gm::NV,
+                NR = Gm::NR, D = Gm::D;
+  const int nx = S.nx, nz = S.nz, pad = S.H + 1;
+  const float dtx = S.dtx;
+  const int s = A.parity, o = 1 - s;
+  const int i0 = W.i0, j0 = W.j0, h = W.h, w = W.w;
+  const int k = threadIdx.x;
+  const int c0 = i0 - N;
+  const int c = c0 + k;                       // this lane's column
+  // halo column staged by lanes k < 2N (slot k: columns left of lane 0; slot NT + k: right of lane
+  // 255); the other lanes repeat lane 2N - 1's cell (one cache line) and stage nothing
+  const int ke = min(k, 2 * N - 1);
+  const int ce = ke < N ? c0 - N + ke : c0 + NT + (ke - N);
+  const int se = ke < N ? ke : NT + ke;
+  auto clampx = [=](int i) { return min(max(i, 0), nx - 1); };
+  auto rowc = [=](int j) { return min(max(j, 0), nz - 1); };
+  auto in_z = [=](int j) { return j >= 0 && j < nz; };
+  auto core_z = [=](int j) { return j >= pad && j < nz - pad; };
+  const bool inc = c >= 0 && c < nx, ine = ce >= 0 && ce < nx;
+  const int cc = clampx(c), cce = clampx(ce);
+  const bool own = k >= N && k - N < w;
+  const bool core_x = c >= pad && c < nx - pad;
+  // array bases and scales, read from the shot table once: old state [s], this launch's outputs and
+  // the frame velocity launch's results [o], moduli and buoyancies
+  const void* const vx_s = S.fld[s][0];
+  const void* const vz_s = S.fld[s][1];
+  const void* const xx_s = S.fld[s][2];
+  const void* const zz_s = S.fld[s][3];
+  const void* const xz_s = S.fld[s][4];
+  void* const vx_o = S.fld[o][0];
+  void* const vz_o = S.fld[o][1];
+  void* const xx_o = S.fld[o][2];
+  void* const zz_o = S.fld[o][3];
+  void* const xz_o = S.fld[o][4];
+  const void* const m_lam = S.mat[0];
+  const void* const m_l2m = S.mat[1];
+  const void* const m_mu = S.mat[2];
+  const void* const m_bx = S.mat[3];
+  const void* const m_bz = S.mat[4];
+  const float fdec[5] = {S.fdec[0], S.fdec[1], S.fdec[2], S.fdec[3], S.fdec[4]};
+  const float fenc[5] = {S.fenc[0], S.fenc[1], S.fenc[2], S.fenc[3], S.fenc[4]};
+  const float mdec[5] = {S.mdec[0], S.mdec[1], S.mdec[2], S.mdec[3], S.mdec[4]};
+  unsigned* const ovf = S.ovf;
+  auto ld = [=](const void* base, int f, int j, int i) -> float {
+    return Cvt<T>::ld(base, (unsigned)(j * nx) + (unsigned)i, fdec[f]);
+  };
+  auto ldm = [=](const void* base, int m, int j, int i) -> float {
+    return Cvt<TM>::ld(base, (unsigned)(j * nx) + (unsigned)i, mdec[m]);
+  };
+  float a[N];
+#pragma unroll
+  for (int l = 0; l < N; ++l) a[l] = A.a[l];
+
+  // Sources in this lane's column (at most two per column, in source order, so that sources
+  // sharing a cell add in that order): a vertical force goes into vz on every lane that computes
+  // that column's velocity, an explosive source into txx and tzz of the strip's own cells.
+  int srow0 = -1, srow1 = -1;
+  float sw0 = 0.0f, sw1 = 0.0f;
+  if (it < A.nt && (S.src_fz || own)) {
+    int q2 = 0;
+    for (int q = 0; q < S.n_src; ++q) {
+      const int csrc = G(S.src_c)[q];
+      const int js = csrc / nx, is = csrc - js * nx;
+      if (is == c) {
+        const float wq = G(S.wav)[(size_t)q + (size_t)S.n_src * (size_t)it];
+        if (q2 == 0) {
+          srow0 = js;
+          sw0 = wq;
+        } else if (q2 == 1) {
+          srow1 = js;
+          sw1 = wq;
+        }
+        ++q2;
+      }
+    }
+  }
+  const bool force = S.src_fz != 0;
+  // snapshot of this step (vz), written by the velocity rows
+  const int nxi = nx - 2 * pad;
+  float* snap_k = nullptr;
+  if (S.snap && (it + 1) % S.snap_every == 0 && (it + 1) / S.snap_every <= S.snap_n)
+    snap_k = S.snap + (size_t)((it + 1) / S.snap_every - 1) * (size_t)nxi * (size_t)(nz - 2 * pad);
+
+  float* const stg = lds;                    // two staged rows of txx and txz: stg + (tau & 1) * 2 * SW
+  float* const vxr = lds + 4 * SW;           // new vx of row r at vxr + (r mod NR) * NT
+  float* const vzr = vxr + NR * NT;          // new vz likewise
+
+  // loads of row rho: txx(c, rho) and the halo column's txx, txz at rho; txz(c, rho + N - 1) and
+  // tzz(c, rho + N) for the queues; old (interior) or frame-final vx, vz and bx, bz at (c, rho);
+  // the moduli at (c, rho - LP)
+  struct Pre {
+    float xx, xxe, xze, qxz, qzz, vxo, vzo, bx, bz, lam, l2m, mu;
+  };
+  auto prefetch = [=](int rho, Pre& P) __attribute__((always_inline)) {
+    const int r0 = rowc(rho), rq = rowc(rho + N - 1), rz = rowc(rho + N), rt = rowc(rho - LP);
+    P.xx = ld(xx_s, 2, r0, cc);
+    P.xxe = ld(xx_s, 2, r0, cce);
+    P.xze = ld(xz_s, 4, r0, cce);
+    P.qxz = ld(xz_s, 4, rq, cc);
+    P.qzz = ld(zz_s, 3, rz, cc);
+    const bool cz = core_x && core_z(rho);
+    P.vxo = ld(cz ? vx_s : (const void*)vx_o, 0, r0, cc);
+    P.vzo = ld(cz ? vz_s : (const void*)vz_o, 1, r0, cc);
+    P.bx = ldm(m_bx, 3, r0, cc);
+    P.bz = ldm(m_bz, 4, r0, cc);
+    P.lam = ldm(m_lam, 0, rt, cc);
+    P.l2m = ldm(m_l2m, 1, rt, cc);
+    P.mu = ldm(m_mu, 2, rt, cc);
+  };
+
+  const int tau0 = j0 - N;
+  const int nit = (h + N + LP + D - 1) / D;  // rows tau0 .. tau0 + h + N + LP - 1, in steps of D
+  Pre pre[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) prefetch(tau0 + d, pre[d]);
+  // Queues at phase A of row tau (after that row's shift): Qxx[i] = txx(tau - LP + i), Qxz[i] =
+  // txz(tau - LP + i), Qzz[i] = tzz(tau - LP + i); QVX / QVZ hold new vx / vz of rows tau - NV ..
+  // tau - 1.  Before the first row they hold the rows one earlier; txx rows before tau0 only ever
+  // serve stress rows before the strip and start as zeros.
+  float Qxx[NXX], Qxz[NXZ], Qzz[NZZ], QVX[NV], QVZ[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) QVX[i] = QVZ[i] = 0.0f;
+#pragma unroll
+  for (int i = 0; i < NXX; ++i) Qxx[i] = 0.0f;
+  Qxz[0] = Qzz[0] = 0.0f;
+  {
+    float q0[NXZ - 1], q1[NZZ - 1];
+#pragma unroll
+    for (int i = 1; i < NXZ; ++i) q0[i - 1] = ld(xz_s, 4, rowc(tau0 - LP - 1 + i), cc);
+#pragma unroll
+    for (int i = 1; i < NZZ; ++i) q1[i - 1] = ld(zz_s, 3, rowc(tau0 - LP - 1 + i), cc);
+#pragma unroll
+    for (int i = 1; i < NXZ; ++i) Qxz[i] = (inc && in_z(tau0 - LP - 1 + i)) ? q0[i - 1] : 0.0f;
+#pragma unroll
+    for (int i = 1; i < NZZ; ++i) Qzz[i] = (inc && in_z(tau0 - LP - 1 + i)) ? q1[i - 1] : 0.0f;
+  }
+
+  for (int g = 0; g < nit; ++g) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int tau = tau0 + g * D + d;
+      Pre& P = pre[d];
+      // A. queues, staged txx / txz of row tau, new stresses of row t = tau - LP
+      const bool iz = in_z(tau);
+#pragma unroll
+      for (int i = 0; i < NXX - 1; ++i) Qxx[i] = Qxx[i + 1];
+#pragma unroll
+      for (int i = 0; i < NXZ - 1; ++i) Qxz[i] = Qxz[i + 1];
+#pragma unroll
+      for (int i = 0; i < NZZ - 1; ++i) Qzz[i] = Qzz[i + 1];
+      Qxx[NXX - 1] = (inc && iz) ? P.xx : 0.0f;
+      Qxz[NXZ - 1] = (inc && in_z(tau + N - 1)) ? P.qxz : 0.0f;
+      Qzz[NZZ - 1] = (inc && in_z(tau + N)) ? P.qzz : 0.0f;
+      float* const sb = stg + (tau & 1) * 2 * SW;
+      sb[k + N] = Qxx[LP];        // txx(c, tau)
+      sb[SW + k + N] = Qxz[LP];   // txz(c, tau)
+      if (k < 2 * N) {
+        sb[se] = (ine && iz) ? P.xxe : 0.0f;
+        sb[SW + se] = (ine && iz) ? P.xze : 0.0f;
+      }
+      const int t = tau - LP;
+      if (t >= j0 && t < j0 + h && own) {
+        const float* VX = vxr + (t % NR) * NT + k;
+        const float* VZ = vzr + (t % NR) * NT + k;
+        float dvxdx = 0.0f, dvzdz = 0.0f, dvxdz = 0.0f, dvzdx = 0.0f;
+#pragma unroll
+        for (int l = 1; l <= N; ++l) {
+          const float cl = a[l - 1];
+          dvxdx += cl * (VX[l] - VX[1 - l]);
+          dvzdz += cl * (QVZ[N - 1 + l] - QVZ[N - l]);
+          dvxdz += cl * (QVX[N + l] - QVX[N + 1 - l]);
+          dvzdx += cl * (VZ[l - 1] - VZ[-l]);
+        }
+        const float dt = dtx;
+        float txx = Qxx[0] + ((P.l2m * dvxdx) * dt + (P.lam * dvzdz) * dt);
+        float tzz = Qzz[0] + ((P.lam * dvxdx) * dt + (P.l2m * dvzdz) * dt);
+        const float txz = Qxz[0] + P.mu * (dvxdz * dt + dvzdx * dt);
+        if (!force) {
+          if (t == srow0) {
+            txx += sw0;
+            tzz += sw0;
+          }
+          if (t == srow1) {
+            txx += sw1;
+            tzz += sw1;
+          }
+        }
+        const unsigned co = (unsigned)(t * nx) + (unsigned)c;
+        st_field<T>(xx_o, co, txx, fenc[2], ovf);
+        st_field<T>(zz_o, co, tzz, fenc[3], ovf);
+        st_field<T>(xz_o, co, txz, fenc[4], ovf);
+      }
+      __syncthreads();
+      // B. new vx(tau), vz(tau) on the 256 columns, into the rings and the lane's queues
+      {
+        float vx = inc && iz ? P.vxo : 0.0f, vz = inc && iz ? P.vzo : 0.0f;
+        if (core_x && core_z(tau)) {
+          const float* XX = sb + k + N;
+          const float* XZ = sb + SW + k + N;
+          float dtxxdx = 0.0f, dtxzdz = 0.0f, dtxzdx = 0.0f, dtzzdz = 0.0f;
+#pragma unroll
+          for (int l = 1; l <= N; ++l) {
+            const float cl = a[l - 1];
+            dtxxdx += cl * (XX[l - 1] - XX[-l]);
+            dtxzdz += cl * (Qxz[N + l] - Qxz[N + 1 - l]);
+            dtxzdx += cl * (XZ[l] - XZ[1 - l]);
+            dtzzdz += cl * (Qzz[N + 1 + l] - Qzz[N + 2 - l]);
+          }
+          const float dt = dtx;
+          vx = P.vxo + P.bx * (dt * dtxxdx + dt * dtxzdz);
+          vz = P.vzo + P.bz * (dt * dtxzdx + dt * dtzzdz);
+          if (force) {
+            if (tau == srow0) vz += sw0;
+            if (tau == srow1) vz += sw1;
+          }
+          if (own && tau >= j0 && tau < j0 + h) {
+            const unsigned co = (unsigned)(tau * nx) + (unsigned)c;
+            st_field<T>(vx_o, co, vx, fenc[0], ovf);
+            st_field<T>(vz_o, co, vz, fenc[1], ovf);
+          }
+          vx = round_trip<T>(vx, fenc[0], fdec[0]);
+          vz = round_trip<T>(vz, fenc[1], fdec[1]);
+          if (snap_k && own && tau >= j0 && tau < j0 + h)
+            G(snap_k)[(size_t)(c - pad) + (size_t)(tau - pad) * nxi] = vz;
+        }
+        vxr[(tau % NR) * NT + k] = vx;
+        vzr[(tau % NR) * NT + k] = vz;
+#pragma unroll
+        for (int i = 0; i < NV - 1; ++i) {
+          QVX[i] = QVX[i + 1];
+          QVZ[i] = QVZ[i + 1];
+        }
+        QVX[NV - 1] = vx;
+        QVZ[NV - 1] = vz;
+      }
+      prefetch(tau + D, P);
+    }
+  }
+}
+
+template <int N, class T, class TM>
+__global__ __launch_bounds__(256) void fused2d_el_kernel(const SweepArgs A) {
+  __shared__ float lds[MarchGeomEl<N>::LDS];
+  if ((int)blockIdx.x >= A.n_work) return;
+  const Work2D W = A.work[blockIdx.x];
+  if (W.kind < 0) return;
+  const Shot2D& S = A.shots[W.shot];
+  march_strip_el<N, T, TM>(A, W, S, lds, *A.step + A.koff);
+}
+
 template <int N, class T, class TM>
 __global__ __launch_bounds__(256) void fused2d_kernel(const SweepArgs A) {
   __shared__ float lds[MarchGeom<N>::LDS];
@@ -932,25 +1225,33 @@ static hipError_t launch_fused2d_n(const SweepArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <int N, class T, class TM>
+static hipError_t launch_fused2d_el_n(const SweepArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((fused2d_el_kernel<N, T, TM>), dim3(a.n_work), dim3(MarchGeomEl<N>::NT), 0, s, a);
+  return hipGetLastError();
+}
+
 template <class T, class TM>
-static hipError_t launch_fused2d_t(int N, const SweepArgs& a, hipStream_t s) {
+static hipError_t launch_fused2d_t(bool el, int N, const SweepArgs& a, hipStream_t s) {
   switch (N) {
-    case 1: return launch_fused2d_n<1, T, TM>(a, s);
-    case 2: return launch_fused2d_n<2, T, TM>(a, s);
-    case 3: return launch_fused2d_n<3, T, TM>(a, s);
-    case 4: return launch_fused2d_n<4, T, TM>(a, s);
-    case 5: return launch_fused2d_n<5, T, TM>(a, s);
+    case 1: return el ? launch_fused2d_el_n<1, T, TM>(a, s) : launch_fused2d_n<1, T, TM>(a, s);
+    case 2: return el ? launch_fused2d_el_n<2, T, TM>(a, s) : launch_fused2d_n<2, T, TM>(a, s);
+    case 3: return el ? launch_fused2d_el_n<3, T, TM>(a, s) : launch_fused2d_n<3, T, TM>(a, s);
+    case 4: return el ? launch_fused2d_el_n<4, T, TM>(a, s) : launch_fused2d_n<4, T, TM>(a, s);
+    case 5: return el ? launch_fused2d_el_n<5, T, TM>(a, s) : launch_fused2d_n<5, T, TM>(a, s);
     default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t launch_fused2d(int storage, int N, const SweepArgs& a, hipStream_t s) {
+hipError_t launch_fused2d(int storage, int eq, int N, const SweepArgs& a, hipStream_t s) {
   if (a.n_work <= 0) return hipSuccess;
+  if (eq != FOMO_ACOUSTIC2D && eq != FOMO_ELASTIC2D) return hipErrorInvalidValue;
+  const bool el = eq == FOMO_ELASTIC2D;
   switch (storage) {
-    case FOMO_STORE_F32: return launch_fused2d_t<float, float>(N, a, s);
-    case FOMO_STORE_F16: return launch_fused2d_t<__half, __half>(N, a, s);
-    case FOMO_STORE_BF16: return launch_fused2d_t<__hip_bfloat16, __hip_bfloat16>(N, a, s);
-    case FOMO_STORE_F32_MAT16: return launch_fused2d_t<float, __half>(N, a, s);
+    case FOMO_STORE_F32: return launch_fused2d_t<float, float>(el, N, a, s);
+    case FOMO_STORE_F16: return launch_fused2d_t<__half, __half>(el, N, a, s);
+    case FOMO_STORE_BF16: return launch_fused2d_t<__hip_bfloat16, __hip_bfloat16>(el, N, a, s);
+    case FOMO_STORE_F32_MAT16: return launch_fused2d_t<float, __half>(el, N, a, s);
     default: return hipErrorInvalidValue;
   }
 }

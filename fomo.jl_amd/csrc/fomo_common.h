@@ -119,11 +119,11 @@ struct SweepArgs {
 // Launchers (defined in engine2d.hip).  tile_cfg selects the interior strip length; storage is
 // FOMO_STORE_F32 / F16 / BF16.
 hipError_t launch_sweep2d(int storage, int phase, int N, int tile_cfg, const SweepArgs& a, hipStream_t s);
-// acoustic2d: the whole step (new vx, vz, then new p) of the interior strips in a.work (kind 1,
-// at most fused2d_width(N) columns each); the frame velocity launch runs before it and the frame
-// pressure launch after it.
-hipError_t launch_fused2d(int storage, int N, const SweepArgs& a, hipStream_t s);
-int fused2d_width(int N);
+// acoustic2d / elastic2d (eq): the whole step (new velocities, then new pressure or stresses) of
+// the interior strips in a.work (kind 1, at most fused2d_width(eq, N) columns each); the frame
+// velocity launch runs before it and the frame pressure / stress launch after it.
+hipError_t launch_fused2d(int storage, int eq, int N, const SweepArgs& a, hipStream_t s);
+int fused2d_width(int eq, int N);
 // Record the state held in buffer [parity] into column (*step + koff - 1).
 hipError_t launch_gather2d(int storage, const GatherArgs& g, const int* step, int koff, int parity, hipStream_t s);
 // Decode n stored values (storage type, scale dec) to Float32.
@@ -132,6 +132,6 @@ int strip_rows(int tile_cfg);   // rows (z) of one interior strip
 constexpr int kTileW = 64;      // tile / strip width (x): one wave64 row
 constexpr int kFrameRows = 8;   // rows of a frame tile
 constexpr int kNumTileCfgs = 3;
-constexpr int kNumFusedCfgs = 5;  // strip lengths of the fused acoustic2d interior launch
+constexpr int kNumFusedCfgs = 5;  // strip lengths of the fused acoustic2d / elastic2d interior launch
 
 }  // namespace fomo

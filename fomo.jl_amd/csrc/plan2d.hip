@@ -79,8 +79,8 @@ static std::vector<Work2D> xcd_layout(const std::vector<Work2D>& items) {
 // forces a length, rounded up to a multiple of 8).  Each strip recomputes velocities on about 2N
 // rows past its ends and re-reads as many rows of its inputs, so longer strips move fewer bytes;
 // the block count also decides how evenly the last round of blocks fills the chip.  At 8192^2 and
-// FD8 (33 columns of 249-column strips) the interior launch takes 0.566, 0.588, 0.559 and 0.603 ms
-// at 96, 128, 176 and 256 rows: the autotuner times them all.
+// FD8 (33 columns of 249-column strips) the acoustic2d interior launch takes 0.566, 0.588, 0.559
+// and 0.603 ms at 96, 128, 176 and 256 rows: the autotuner times them all.
 static int fused_rows(int cfg) {
   if (const char* env = std::getenv("FOMO_FUSED2D_ROWS")) {
     const int r = std::atoi(env);
@@ -105,7 +105,7 @@ int Plan2D::make_work(int cfg) {
     std::vector<Work2D> frame, inner;
     for (const Work2D& w : work_items(shots, meta, 0))
       if (w.kind == 0) frame.push_back(w);
-    const int rows = fused_rows(cfg), cols = fused2d_width(N);
+    const int rows = fused_rows(cfg), cols = fused2d_width(d.eq, N);
     for (size_t s = 0; s < shots.size(); ++s) {
       const int pad = meta[s].pad, xe = pad + meta[s].nxi, ze = pad + meta[s].nzi;
       for (int j = pad; j < ze; j += rows)
@@ -157,10 +157,12 @@ hipError_t Plan2D::enqueue_part(int part, int s, int koff) {
   }
   if (fused) {
     // part 0: the frame's velocities (and the receivers of the previous step), then the interior's
-    // whole step; part 1: the frame's pressure
+    // whole step; part 1: the frame's pressure or stresses
     a.work = d_frame;
     a.n_work = n_frame;
-    if (part == 1) return only == 0 ? hipSuccess : launch_sweep2d(sweep_storage, kAcPrs, N, tile_cfg, a, stream);
+    if (part == 1)
+      return only == 0 ? hipSuccess
+                       : launch_sweep2d(sweep_storage, d.eq == FOMO_ACOUSTIC2D ? kAcPrs : kElStr, N, tile_cfg, a, stream);
   } else if (part == 1 && coupled) {
     a.work = d_frame;  // the interior's position half-steps ran in the velocity launch
     a.n_work = n_frame;
@@ -199,7 +201,7 @@ hipError_t Plan2D::enqueue_part(int part, int s, int koff) {
   a.work = d_fwork[tile_cfg];
   a.n_work = n_fwork[tile_cfg];
   a.n_gather = 0;
-  return launch_fused2d(sweep_storage, N, a, stream);
+  return launch_fused2d(sweep_storage, d.eq, N, a, stream);
 }
 
 PlanBase::LaunchInfo Plan2D::launch_info(int k) const {
@@ -210,9 +212,15 @@ PlanBase::LaunchInfo Plan2D::launch_info(int k) const {
   int words = 0;
   const char* name = "";
   if (fused) {
-    // the interior launch is credited with both phases' reads and writes (7 + 5 words per cell)
+    // the interior launch is credited with both phases' reads and writes (acoustic 7 + 5 words per
+    // cell, elastic 9 + 11)
     long long inner = 0;
     for (const auto& m : meta) inner += (long long)m.nxi * m.nzi;
+    if (d.eq == FOMO_ELASTIC2D) {
+      if (k == 0) return {"elastic2d velocity frame", cells - inner, 9 * elem};
+      if (k == 1) return {"elastic2d interior step (velocity + stress)", inner, 20 * elem};
+      return {"elastic2d stress frame", cells - inner, 11 * elem};
+    }
     if (k == 0) return {"acoustic2d velocity frame", cells - inner, 7 * elem};
     if (k == 1) return {"acoustic2d interior step (velocity + pressure)", inner, 12 * elem};
     return {"acoustic2d pressure frame", cells - inner, 5 * elem};
@@ -461,9 +469,9 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
   p->storage = mat16 ? FOMO_STORE_F32 : d->storage;
   p->mat_storage = mat16 ? FOMO_STORE_F16 : d->storage;
   p->sweep_storage = d->storage;
-  // acoustic2d may fuse the interior's two phases (the autotuner decides; a forced tile shape is
-  // one of the fused family unless FOMO_2D_UNFUSED=1)
-  p->fusable = d->eq == FOMO_ACOUSTIC2D;
+  // acoustic2d and elastic2d may fuse the interior's two phases (the autotuner decides; a forced
+  // tile shape is one of the fused family unless FOMO_2D_UNFUSED=1)
+  p->fusable = d->eq == FOMO_ACOUSTIC2D || d->eq == FOMO_ELASTIC2D;
   p->elem = p->storage == FOMO_STORE_F32 ? 4 : 2;
   p->N = d->fd_order / 2;
   p->coupled = coupled;
