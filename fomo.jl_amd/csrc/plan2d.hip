@@ -108,9 +108,13 @@ int Plan2D::make_work(int cfg) {
     const int rows = fused_rows(cfg), cols = fused2d_width(d.eq, N);
     for (size_t s = 0; s < shots.size(); ++s) {
       const int pad = meta[s].pad, xe = pad + meta[s].nxi, ze = pad + meta[s].nzi;
+      // strips of about equal width: a window of 256 interior columns runs as two of 128 rather
+      // than one of the full width and a sliver, whose block would march every row for a few columns
+      const int nstrip = std::max(1, (meta[s].nxi + cols - 1) / cols);
+      const int cw = (meta[s].nxi + nstrip - 1) / nstrip;
       for (int j = pad; j < ze; j += rows)
-        for (int i = pad; i < xe; i += cols)
-          inner.push_back(Work2D{(int)s, 1, i, j, std::min(cols, xe - i), std::min(rows, ze - j)});
+        for (int i = pad; i < xe; i += cw)
+          inner.push_back(Work2D{(int)s, 1, i, j, std::min(cw, xe - i), std::min(rows, ze - j)});
     }
     if (!d_frame) {
       const std::vector<Work2D> ff = xcd_layout(frame);

@@ -39,10 +39,14 @@ def _solo(monkeypatch, unfused, rows, nx, nz, order, storage, nbc, nt=60, learne
     vp, rho = _model(nx, nz, 21)
     w = fomo.ricker_wavelet(18.0, 8e-4, nt)
     rx = np.arange(1, nx + 1, 3)
-    # sources at an interior corner cell, inside, and in the thin band next to the frame
-    sx, sz = [1, nx // 2, nx - 2], [1, nz // 3, nz - 3]
+    # sources at an interior corner cell, inside, in the thin band next to the frame and on each
+    # side of the seam of the first two strips (strips of at most 257 - 2N columns, of about equal
+    # width)
+    n = -(-nx // (257 - order))
+    edge = -(-nx // n)
+    sx, sz = [1, nx // 2, nx - 2, min(edge, nx - 1), min(edge + 1, nx)], [1, nz // 3, nz - 3, nz // 4, nz // 5]
     p = Plan(L.ACOUSTIC2D, vp, rho, 10.0, 8e-4, nt, sx=sx, sz=sz, rx=rx, rz=np.full(len(rx), 2),
-             wavelet=np.stack([w, 0.5 * w, -w]), nbc=nbc, fd_order=order, storage=storage, tile_cfg=1,
+             wavelet=np.stack([w, 0.5 * w, -w, 0.75 * w, -0.6 * w]), nbc=nbc, fd_order=order, storage=storage, tile_cfg=1,
              snap_interval=7, use_graph=True)
     try:
         if learned is not None:
@@ -68,7 +72,8 @@ def _same(a, b):
 @pytest.mark.parametrize("order,storage", [(2, "f32"), (4, "f32"), (6, "f32"), (8, "f32"), (10, "f32"), (8, "f16"),
                                            (8, "bf16"), (8, "mat16"), (10, "bf16"), (2, "f16")])
 def test_fused_equals_two_sweeps(gpu, monkeypatch, order, storage):
-    """560 x 171: strips of 257 - 2N columns (FD8: 249 + 249 + 62) by 24 rows (a partial last one)."""
+    """560 x 171: strips of at most 257 - 2N columns, of about equal width (187 + 187 + 186 at every
+    order), by 24 rows (a partial last one)."""
     a = _solo(monkeypatch, False, 24, 560, 171, order, storage, nbc=12)
     b = _solo(monkeypatch, True, None, 560, 171, order, storage, nbc=12)
     _same(a, b)
@@ -78,9 +83,9 @@ def test_fused_equals_two_sweeps(gpu, monkeypatch, order, storage):
 def test_fused_strip_lengths_and_thin_frame(gpu, monkeypatch, rows):
     """Strips of 8 rows (shorter than the 2N - 1 = 9 rows of velocities recomputed past each end at
     FD10) and one strip per column (4096 rows), with nbc = 0 (the frame is the N-cell stencil pad),
-    over 260 columns (a 247-column strip and a 13-column one)."""
-    a = _solo(monkeypatch, False, rows, 260, 203, 10, "f32", nbc=0)
-    b = _solo(monkeypatch, True, None, 260, 203, 10, "f32", nbc=0)
+    over 494 columns: two strips of the full 247 columns."""
+    a = _solo(monkeypatch, False, rows, 494, 203, 10, "f32", nbc=0)
+    b = _solo(monkeypatch, True, None, 494, 203, 10, "f32", nbc=0)
     _same(a, b)
 
 

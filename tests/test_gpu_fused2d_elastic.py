@@ -33,12 +33,19 @@ def _set(monkeypatch, unfused, rows):
         monkeypatch.delenv("FOMO_FUSED2D_ROWS", raising=False)
 
 
+def _strip_width(nx, order):
+    """Columns of the fused launch's strips over an interior of nx columns: at most 256 - 2N, the
+    strips of about equal width."""
+    n = -(-nx // (256 - order))
+    return -(-nx // n)
+
+
 def _geometry(nx, nz, order):
     """Sources at an interior corner cell, inside, in the thin band next to the frame, two in one
-    column (they add in source order) and one on the column where the first two strips meet (its
-    velocity is recomputed by the neighbouring strip's halo lanes)."""
-    edge = 256 - order  # columns of the first strip (1-based: the next strip starts at edge + 1)
-    sx = [1, nx // 2, nx - 2, nx // 2, min(edge, nx - 1), min(edge + 1, nx)]
+    column (they add in source order) and one on each side of the seam of the first two strips
+    (a velocity there is recomputed by the neighbouring strip's halo lanes)."""
+    edge = _strip_width(nx, order)  # columns of the first strip (1-based: the next strip starts at edge + 1)
+    sx = [1, 2 * nx // 5, nx - 2, 2 * nx // 5, min(edge, nx - 1), min(edge + 1, nx)]
     sz = [1, nz // 3, nz - 3, nz // 2, nz // 4, nz // 5]
     return sx, sz
 
@@ -78,7 +85,8 @@ def _same(a, b):
 @pytest.mark.parametrize("order,storage", [(2, "f32"), (4, "f32"), (6, "f32"), (8, "f32"), (10, "f32"), (8, "f16"),
                                            (8, "bf16"), (8, "mat16"), (2, "f16")])
 def test_fused_elastic_equals_two_sweeps(gpu, monkeypatch, order, storage, src):
-    """560 x 171: strips of 256 - 2N columns (FD8: 248 + 248 + 64) by 24 rows (a partial last one)."""
+    """560 x 171: strips of at most 256 - 2N columns, of about equal width (187 + 187 + 186 at every
+    order), by 24 rows (a partial last one)."""
     import fomo._lib as L
     st = L.SRC_EXPLOSIVE if src == 0 else L.SRC_FZ
     print(f"fused elastic2d order {order} storage {storage} source {src}", flush=True)
@@ -123,11 +131,11 @@ def test_fused_elastic_bitwise_vs_oracle(gpu, orc, monkeypatch, src):
 @pytest.mark.parametrize("rows", [8, 4096])
 def test_fused_elastic_strip_lengths_and_thin_frame(gpu, monkeypatch, rows):
     """Strips of 8 rows (shorter than the rows of velocities recomputed past each end at FD10) and
-    one strip per column (4096 rows), with nbc = 0, over 260 columns (a 246-column strip and a
-    14-column one); vertical-force sources."""
+    one strip per column (4096 rows), with nbc = 0, over 492 columns: two strips of the full 246
+    columns; vertical-force sources."""
     import fomo._lib as L
-    a = _solo(monkeypatch, False, rows, 260, 203, 10, "f32", nbc=0, src=L.SRC_FZ)
-    b = _solo(monkeypatch, True, None, 260, 203, 10, "f32", nbc=0, src=L.SRC_FZ)
+    a = _solo(monkeypatch, False, rows, 492, 203, 10, "f32", nbc=0, src=L.SRC_FZ)
+    b = _solo(monkeypatch, True, None, 492, 203, 10, "f32", nbc=0, src=L.SRC_FZ)
     _same(a, b)
 
 
