@@ -468,10 +468,10 @@ int fomo_plan_create3d(const fomo_desc* d, const float* vp, const float* vs, con
   std::vector<float> rhop;
   pad3(rho, rhop);
   const int mb = elastic ? 0 : 1;  // index of bx among the materials
-  // acoustic3d with Float32 storage runs the interior's step in one launch (512^3 FD8: 3.22 against
+  // acoustic3d with Float32 storage runs the interior's step in one launch (512^3 FD8: 3.12 against
   // 3.59 ms per step).  With 16-bit storage the fused loop's decodes, encodes and round trips make
-  // it issue-bound (3.83 against 3.13 ms), so the separate sweeps run; FOMO_3D_FUSED=1 forces the
-  // fused form, FOMO_3D_UNFUSED=1 the separate sweeps.
+  // it issue-bound (interior 2.29 ms against 1.08 ms for the two 16-bit core sweeps), so the
+  // separate sweeps run; FOMO_3D_FUSED=1 forces the fused form, FOMO_3D_UNFUSED=1 the sweeps.
   p->fused = !elastic && !std::getenv("FOMO_3D_UNFUSED") &&
              (d->storage == FOMO_STORE_F32 || std::getenv("FOMO_3D_FUSED") != nullptr);
   if (!elastic) {
