@@ -89,6 +89,16 @@ def test_fused_strip_lengths_and_thin_frame(gpu, monkeypatch, rows):
     _same(a, b)
 
 
+@pytest.mark.parametrize("nbc", [0, 3])
+def test_fused_tiny_grid(gpu, monkeypatch, nbc):
+    """7 x 5 cells at FD10: one strip of 7 columns in a 256-lane block, 5 rows (fewer than the
+    rows of velocities recomputed past each end), every source cell within the stencil's reach of
+    the frame."""
+    a = _solo(monkeypatch, False, 8, 7, 5, 10, "f32", nbc=nbc, nt=30)
+    b = _solo(monkeypatch, True, None, 7, 5, 10, "f32", nbc=nbc, nt=30)
+    _same(a, b)
+
+
 def test_fused_batch_of_uneven_windows(gpu, monkeypatch):
     """Three windows of different widths (one, two and one strips in x) in one batch."""
     import fomo

@@ -139,6 +139,18 @@ def test_fused_elastic_strip_lengths_and_thin_frame(gpu, monkeypatch, rows):
     _same(a, b)
 
 
+@pytest.mark.parametrize("nbc", [0, 3])
+@pytest.mark.parametrize("src", [0, 1])
+def test_fused_elastic_tiny_grid(gpu, monkeypatch, nbc, src):
+    """7 x 5 cells at FD10: one strip of 7 columns in a 256-lane block, 5 rows (fewer than the
+    rows of velocities recomputed past each end), two sources in one column."""
+    import fomo._lib as L
+    st = L.SRC_EXPLOSIVE if src == 0 else L.SRC_FZ
+    a = _solo(monkeypatch, False, 8, 7, 5, 10, "f32", nbc=nbc, src=st, nt=30)
+    b = _solo(monkeypatch, True, None, 7, 5, 10, "f32", nbc=nbc, src=st, nt=30)
+    _same(a, b)
+
+
 def test_fused_elastic_batch_of_uneven_windows(gpu, monkeypatch):
     """Three windows of different widths (one, two and one strips in x) in one batch."""
     import fomo
