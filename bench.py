@@ -520,7 +520,7 @@ def batch_bench(args):
                     f"dt 0.8 ms, "
                     f"{nt} steps", "padded_cells": cells, "autotuned": res[0],
            # a forced tile shape of an acoustic2d or elastic2d plan is one of the fused interior's strip lengths
-           "fixed_strip_rows": {str(r): res[c] for c, r in zip((1, 2, 3), (64, 96, 128) if fused_family else (16, 32, 64))},
+           "fixed_strip_rows": {str(r): res[c] for c, r in zip((1, 2, 3), (24, 32, 48) if fused_family else (16, 32, 64))},
            "widths": [s["nx"] for s in shots]}
     if not args.no_cpu:
         import oracle

@@ -719,7 +719,13 @@ __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D&
 //      the pressure's x-stencil reads, into a ring of LP + 1 LDS rows.
 // The pressure row lags the velocity row by LP = max(N - 1, 2): its z-stencil reaches new vz(t +
 // N - 1), and new vx(t), written in phase B of row t, must be behind a barrier (the one of row
-// t + 1) before phase A of row t + 2 reads it.  New velocities of cells in the user's interior
+// t + 1) before phase A of row t + 2 reads it.
+// Strips of odd row bands march up instead (UP: rows in decreasing order, the pressure row t =
+// tau + LP with LP = max(N, 2), since its z-stencil now trails at new vz(t - N); p queue rows
+// tau - N + 1 .. tau + LP, vz queue rows tau .. tau + LP + N - 1).  Vertically adjacent strips
+// then reach the rows they both read at the same time, at the start or at the end of their
+// marches, so the second read finds them in the caches (the Infinity Cache, going by the L2 miss
+// counts, which do not fall) instead of HBM.  New velocities of cells in the user's interior
 // are recomputed from the old state (buffers [s], read-only during the launch); those of frame
 // cells are read from the frame velocity launch, which runs first and writes only frame cells
 // (buffers [1-s]; this launch writes only interior cells).  Rows are the same for every lane, so
@@ -729,15 +735,16 @@ __device__ __forceinline__ void interior_strip(const SweepArgs& A, const Work2D&
 // use counts only the older loads.  Values are the separate sweeps' expressions in their order
 // (and, with 16-bit storage, the stored values the pressure sweep would read back), so the result
 // is bit-identical to them.
-template <int N>
+template <int N, bool UP = false>
 struct MarchGeom {
   static constexpr int NT = 256;                      // lanes
   static constexpr int HX = 2 * N - 1;                // lanes left of the strip's first column
   static constexpr int W = NT + 1 - 2 * N;            // columns of a strip
   static constexpr int SW = NT + 2 * N - 1;           // staged p row: columns x0 .. x0 + SW - 1
-  static constexpr int LP = N - 1 > 2 ? N - 1 : 2;    // pressure row = velocity row - LP
-  static constexpr int NQ = LP + N + 1;               // p queue: rows tau - LP .. tau + N
-  static constexpr int NZ = LP + N + 1;               // vz queue: rows tau - LP - N .. tau
+  // pressure row = velocity row - LP (down) or + LP (up)
+  static constexpr int LP = UP ? (N > 2 ? N : 2) : (N - 1 > 2 ? N - 1 : 2);
+  static constexpr int NQ = UP ? LP + N : LP + N + 1;  // p queue: rows tau - LP .. tau + N (up: tau + LP .. tau - N + 1)
+  static constexpr int NZ = UP ? LP + N : LP + N + 1;  // vz queue: rows tau - LP - N .. tau (up: tau + LP + N - 1 .. tau)
   static constexpr int NR = LP + 1;                   // rows of the vx ring
   // rows of loads in flight: at 8192^2 FD8 the interior launch takes 0.561 / 0.557 / 0.561 ms
   // with 2 / 3 / 4 rows in Float32 and 0.521 / 0.545 / 0.603 ms in Float16
@@ -747,10 +754,10 @@ struct MarchGeom {
 
 int fused2d_width(int eq, int N) { return eq == FOMO_ELASTIC2D ? 256 - 2 * N : 257 - 2 * N; }
 
-template <int N, class T, class TM>
+template <int N, class T, class TM, bool UP>
 __device__ __forceinline__ void march_strip(const SweepArgs& A, const Work2D& W, const Shot2D& S, float* lds,
                                             int it) {
-  using Gm = MarchGeom<N>;
+  using Gm = MarchGeom<N, UP>;
   constexpr int NT = Gm::NT, HX = Gm::HX, SW = Gm::SW, LP = Gm::LP, NQ = Gm::NQ, NZ = Gm::NZ, NR = Gm::NR,
                 D = Gm::D;
   const int nx = S.nx, nz = S.nz, pad = S.H + 1;
@@ -820,13 +827,18 @@ __device__ __forceinline__ void march_strip(const SweepArgs& A, const Work2D& W,
   float* const stg = lds;            // two staged p rows (row tau in stg + (tau & 1) * SW)
   float* const vxr = lds + 2 * SW;   // new vx of row r in vxr + (r mod NR) * NT
 
-  // loads of row rho: p(rho + N), extra p(rho), old / frame-final vx(rho) and bx(rho) at the vx
-  // column, old / frame-final vz(rho) and bz(rho), kappa(rho - LP)
+  // Row offsets from the velocity row tau: DIR the march step, TOFF the pressure row, LEAD the p row
+  // that enters the queue.  Queue slots in march order (oldest first): qi / zi of a row offset.
+  constexpr int DIR = UP ? -1 : 1, TOFF = UP ? LP : -LP, LEAD = UP ? 1 - N : N;
+  auto qi = [](int off) constexpr { return UP ? LP - off : off + LP; };
+  auto zi = [](int off) constexpr { return UP ? LP + N - 1 - off : off + LP + N; };
+  // loads of row rho: p(rho + LEAD), extra p(rho), old / frame-final vx(rho) and bx(rho) at the vx
+  // column, old / frame-final vz(rho) and bz(rho), kappa(rho + TOFF)
   struct Pre {
     float p, pe, vxo, bx, vzo, bz, kap;
   };
   auto prefetch = [=](int rho, Pre& P) __attribute__((always_inline)) {
-    const int rp = rowc(rho + N), r0 = rowc(rho), rk = rowc(rho - LP);
+    const int rp = rowc(rho + LEAD), r0 = rowc(rho), rk = rowc(rho + TOFF);
     P.p = ld(p_s, 0, rp, cpc);
     P.pe = ld(p_s, 0, r0, cec);
     P.vxo = ld(vx_src, 1, r0, cvc);
@@ -836,12 +848,14 @@ __device__ __forceinline__ void march_strip(const SweepArgs& A, const Work2D& W,
     P.kap = ldm(0, rk, cpc);
   };
 
-  const int tau0 = j0 - N;
-  const int nit = (h + N + LP + D - 1) / D;   // rows tau0 .. tau0 + h + N + LP - 1, in steps of D
+  // rows j0 - N .. j0 + h - 1 + LP down, or j0 + h + N - 2 .. j0 - LP up (the new vz rows the
+  // pressure rows j0 .. j0 + h - 1 read, and the velocity rows of those pressure rows)
+  const int tau0 = UP ? j0 + h + N - 2 : j0 - N;
+  const int nit = (h + N + LP - (UP ? 1 : 0) + D - 1) / D;
   Pre pre[D];
 #pragma unroll
-  for (int d = 0; d < D; ++d) prefetch(tau0 + d, pre[d]);
-  // p queue before the first row: Q[i] = p(tau0 - 1 - LP + i), i >= 1
+  for (int d = 0; d < D; ++d) prefetch(tau0 + DIR * d, pre[d]);
+  // p queue before the first row: slot i >= 1 holds p at row offset (qi^-1)(i) from tau0 - DIR
   float Q[NQ], Z[NZ];
 #pragma unroll
   for (int i = 0; i < NQ; ++i) Q[i] = 0.0f;
@@ -850,25 +864,25 @@ __device__ __forceinline__ void march_strip(const SweepArgs& A, const Work2D& W,
   {
     float q0[NQ - 1];
 #pragma unroll
-    for (int i = 1; i < NQ; ++i) q0[i - 1] = ld(p_s, 0, rowc(tau0 - 1 - LP + i), cpc);
+    for (int i = 1; i < NQ; ++i) q0[i - 1] = ld(p_s, 0, rowc(tau0 - DIR + (UP ? LP - i : i - LP)), cpc);
 #pragma unroll
-    for (int i = 1; i < NQ; ++i) Q[i] = (inp && in_z(tau0 - 1 - LP + i)) ? q0[i - 1] : 0.0f;
+    for (int i = 1; i < NQ; ++i) Q[i] = (inp && in_z(tau0 - DIR + (UP ? LP - i : i - LP))) ? q0[i - 1] : 0.0f;
   }
 
   for (int g = 0; g < nit; ++g) {
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const int tau = tau0 + g * D + d;
+      const int tau = tau0 + DIR * (g * D + d);
       Pre& P = pre[d];
-      // A. p queue, new vz(tau), staged p(tau), new p(tau - LP)
+      // A. p queue, new vz(tau), staged p(tau), new p(tau + TOFF)
 #pragma unroll
       for (int i = 0; i < NQ - 1; ++i) Q[i] = Q[i + 1];
-      Q[NQ - 1] = (inp && in_z(tau + N)) ? P.p : 0.0f;
+      Q[NQ - 1] = (inp && in_z(tau + LEAD)) ? P.p : 0.0f;
       float vz = in_z(tau) ? P.vzo : 0.0f;
       if (core_z(tau)) {
         float dz = 0.0f;
 #pragma unroll
-        for (int l = 1; l <= N; ++l) dz += a[l - 1] * (Q[LP + l] - Q[LP - l + 1]);
+        for (int l = 1; l <= N; ++l) dz += a[l - 1] * (Q[qi(l)] - Q[qi(1 - l)]);
         vz = P.vzo + (P.bz * dtx) * dz;
         if (own && tau >= j0 && tau < j0 + h)
           st_field<T>(vz_w, (unsigned)(tau * nx) + (unsigned)cp, vz, S.fenc[2], S.ovf);
@@ -878,9 +892,9 @@ __device__ __forceinline__ void march_strip(const SweepArgs& A, const Work2D& W,
       for (int i = 0; i < NZ - 1; ++i) Z[i] = Z[i + 1];
       Z[NZ - 1] = vz;
       float* const sb = stg + (tau & 1) * SW;
-      sb[k] = Q[LP];
+      sb[k] = Q[qi(0)];
       if (k < HX) sb[NT + k] = (ine && in_z(tau)) ? P.pe : 0.0f;
-      const int t = tau - LP;
+      const int t = tau + TOFF;
       if (t >= j0 && t < j0 + h && own) {
         const float* VX = vxr + (t % NR) * NT + k - N;
         float dvxdx = 0.0f, dvzdz = 0.0f;
@@ -888,9 +902,9 @@ __device__ __forceinline__ void march_strip(const SweepArgs& A, const Work2D& W,
         for (int l = 1; l <= N; ++l) {
           const float cl = a[l - 1];
           dvxdx += cl * (VX[l] - VX[1 - l]);
-          dvzdz += cl * (Z[N + l - 1] - Z[N - l]);
+          dvzdz += cl * (Z[zi(TOFF + l - 1)] - Z[zi(TOFF - l)]);
         }
-        float v = Q[0] + P.kap * (dvxdx * dtx + dvzdz * dtx);
+        float v = Q[qi(TOFF)] + P.kap * (dvxdx * dtx + dvzdz * dtx);
         if (t == srow0) v += sw0;
         if (t == srow1) v += sw1;
         st_field<T>(p_w, (unsigned)(t * nx) + (unsigned)cp, v, S.fenc[0], S.ovf);
@@ -911,7 +925,7 @@ __device__ __forceinline__ void march_strip(const SweepArgs& A, const Work2D& W,
         }
         vxr[(tau % NR) * NT + k] = v;
       }
-      prefetch(tau + D, P);
+      prefetch(tau + DIR * D, P);
     }
   }
 }
@@ -934,31 +948,35 @@ __device__ __forceinline__ void march_strip(const SweepArgs& A, const Work2D& W,
 //      reach N columns past the strip) into rings of LP + 1 LDS rows.
 // The stress row lags by LP = N + 1: its z-stencil reaches new vx(t + N), computed in phase B of
 // row t + N and in the lane's queue from row t + N + 1 on; and the rings' row t, written in phase
-// B of row t, is behind the barrier of row t + 1 before phase A of row t + 2 reads it.  As in the
+// B of row t, is behind the barrier of row t + 1 before phase A of row t + 2 reads it.  Strips of
+// odd row bands march up (UP: the stress row t = tau + LP, whose z-stencil trails at new vz(t - N);
+// txz queue rows tau + LP .. tau - N, tzz tau + LP .. tau - N + 1, new velocities tau + 2N + 1 ..
+// tau + 1), so that vertically adjacent strips read their shared rows at the same time and the
+// second read finds them in the caches.  As in the
 // acoustic march, velocities of cells in the user's interior are recomputed from the old state
 // (buffers [s]) and those of frame cells read from the frame velocity launch, which runs first
 // (buffers [1-s]); every value is the separate sweeps' expression in their order
 // (src/equations/elastic2d/fused_kernels.jl:21-124), and 16-bit storage rounds the velocities as
 // the stress sweep would read them back, so the result is bit-identical to the separate sweeps.
-template <int N>
+template <int N, bool UP = false>
 struct MarchGeomEl {
   static constexpr int NT = 256;
   static constexpr int W = NT - 2 * N;       // columns of a strip
   static constexpr int SW = NT + 2 * N;      // staged row: columns c(0) - N .. c(255) + N
-  static constexpr int LP = N + 1;           // stress row = velocity row - LP
+  static constexpr int LP = N + 1;           // stress row = velocity row - LP (up: + LP)
   static constexpr int NXX = LP + 1;         // txx rows t .. tau
-  static constexpr int NXZ = 2 * N + 1;      // txz rows t .. tau + N - 1
-  static constexpr int NZZ = 2 * N + 2;      // tzz rows t .. tau + N
-  static constexpr int NV = 2 * N + 1;       // new vx / vz queues: rows tau - NV .. tau - 1 at phase A
+  static constexpr int NXZ = UP ? 2 * N + 2 : 2 * N + 1;  // txz rows t .. tau + N - 1 (up: t .. tau - N)
+  static constexpr int NZZ = UP ? 2 * N + 1 : 2 * N + 2;  // tzz rows t .. tau + N (up: t .. tau - N + 1)
+  static constexpr int NV = 2 * N + 1;       // new vx / vz queues: rows tau - NV .. tau - 1 at phase A (up: tau + NV .. tau + 1)
   static constexpr int NR = LP + 1;          // rows of the velocity rings
   static constexpr int D = 2;                // rows of loads in flight
   static constexpr int LDS = 2 * 2 * SW + 2 * NR * NT;
 };
 
-template <int N, class T, class TM>
+template <int N, class T, class TM, bool UP>
 __device__ __forceinline__ void march_strip_el(const SweepArgs& A, const Work2D& W, const Shot2D& S, float* lds,
                                                int it) {
-  using Gm = MarchGeomEl<N>;
+  using Gm = MarchGeomEl<N, UP>;
   constexpr int NT = Gm::NT, SW = Gm::SW, LP = Gm::LP, NXX = Gm::NXX, NXZ = Gm::NXZ, NZZ = Gm::NZZ, NV = Gm::NV,
                 NR = Gm::NR, D = Gm::D;
   const int nx = S.nx, nz = S.nz, pad = S.H + 1;
@@ -1046,14 +1064,20 @@ __device__ __forceinline__ void march_strip_el(const SweepArgs& A, const Work2D&
   float* const vxr = lds + 4 * SW;           // new vx of row r at vxr + (r mod NR) * NT
   float* const vzr = vxr + NR * NT;          // new vz likewise
 
-  // loads of row rho: txx(c, rho) and the halo column's txx, txz at rho; txz(c, rho + N - 1) and
-  // tzz(c, rho + N) for the queues; old (interior) or frame-final vx, vz and bx, bz at (c, rho);
-  // the moduli at (c, rho - LP)
+  // Row offsets from the velocity row tau: DIR the march step, TOFF the stress row, LXZ / LZZ the
+  // txz / tzz rows that enter their queues.  Queue slots in march order (oldest first): sq / sv of
+  // a row offset (stress queues, new-velocity queues).
+  constexpr int DIR = UP ? -1 : 1, TOFF = UP ? LP : -LP, LXZ = UP ? -N : N - 1, LZZ = UP ? 1 - N : N;
+  auto sq = [](int off) constexpr { return UP ? LP - off : off + LP; };
+  auto sv = [](int off) constexpr { return UP ? NV - off : off + NV; };
+  // loads of row rho: txx(c, rho) and the halo column's txx, txz at rho; txz(c, rho + LXZ) and
+  // tzz(c, rho + LZZ) for the queues; old (interior) or frame-final vx, vz and bx, bz at (c, rho);
+  // the moduli at (c, rho + TOFF)
   struct Pre {
     float xx, xxe, xze, qxz, qzz, vxo, vzo, bx, bz, lam, l2m, mu;
   };
   auto prefetch = [=](int rho, Pre& P) __attribute__((always_inline)) {
-    const int r0 = rowc(rho), rq = rowc(rho + N - 1), rz = rowc(rho + N), rt = rowc(rho - LP);
+    const int r0 = rowc(rho), rq = rowc(rho + LXZ), rz = rowc(rho + LZZ), rt = rowc(rho + TOFF);
     P.xx = ld(xx_s, 2, r0, cc);
     P.xxe = ld(xx_s, 2, r0, cce);
     P.xze = ld(xz_s, 4, r0, cce);
@@ -1069,15 +1093,17 @@ __device__ __forceinline__ void march_strip_el(const SweepArgs& A, const Work2D&
     P.mu = ldm(m_mu, 2, rt, cc);
   };
 
-  const int tau0 = j0 - N;
-  const int nit = (h + N + LP + D - 1) / D;  // rows tau0 .. tau0 + h + N + LP - 1, in steps of D
+  // rows j0 - N .. j0 + h + N down, or j0 + h + N - 1 .. j0 - N - 1 up: the velocity rows the stress
+  // rows j0 .. j0 + h - 1 read, and the velocity rows of those stress rows
+  const int tau0 = UP ? j0 + h + N - 1 : j0 - N;
+  const int nit = (h + N + LP + D - 1) / D;
   Pre pre[D];
 #pragma unroll
-  for (int d = 0; d < D; ++d) prefetch(tau0 + d, pre[d]);
-  // Queues at phase A of row tau (after that row's shift): Qxx[i] = txx(tau - LP + i), Qxz[i] =
-  // txz(tau - LP + i), Qzz[i] = tzz(tau - LP + i); QVX / QVZ hold new vx / vz of rows tau - NV ..
-  // tau - 1.  Before the first row they hold the rows one earlier; txx rows before tau0 only ever
-  // serve stress rows before the strip and start as zeros.
+  for (int d = 0; d < D; ++d) prefetch(tau0 + DIR * d, pre[d]);
+  // Queues at phase A of row tau (after that row's shift): Qxx[sq(o)] = txx(tau + o), Qxz[sq(o)] =
+  // txz(tau + o), Qzz[sq(o)] = tzz(tau + o); QVX[sv(o)] / QVZ[sv(o)] hold new vx / vz of row
+  // tau + o.  Before the first row they hold the rows one earlier; txx rows before tau0 only ever
+  // serve stress rows outside the strip and start as zeros.
   float Qxx[NXX], Qxz[NXZ], Qzz[NZZ], QVX[NV], QVZ[NV];
 #pragma unroll
   for (int i = 0; i < NV; ++i) QVX[i] = QVZ[i] = 0.0f;
@@ -1086,22 +1112,24 @@ __device__ __forceinline__ void march_strip_el(const SweepArgs& A, const Work2D&
   Qxz[0] = Qzz[0] = 0.0f;
   {
     float q0[NXZ - 1], q1[NZZ - 1];
+    // slot i >= 1 holds the row at offset (sq^-1)(i) from tau0 - DIR
+    auto row_of = [&](int i) { return tau0 - DIR + (UP ? LP - i : i - LP); };
 #pragma unroll
-    for (int i = 1; i < NXZ; ++i) q0[i - 1] = ld(xz_s, 4, rowc(tau0 - LP - 1 + i), cc);
+    for (int i = 1; i < NXZ; ++i) q0[i - 1] = ld(xz_s, 4, rowc(row_of(i)), cc);
 #pragma unroll
-    for (int i = 1; i < NZZ; ++i) q1[i - 1] = ld(zz_s, 3, rowc(tau0 - LP - 1 + i), cc);
+    for (int i = 1; i < NZZ; ++i) q1[i - 1] = ld(zz_s, 3, rowc(row_of(i)), cc);
 #pragma unroll
-    for (int i = 1; i < NXZ; ++i) Qxz[i] = (inc && in_z(tau0 - LP - 1 + i)) ? q0[i - 1] : 0.0f;
+    for (int i = 1; i < NXZ; ++i) Qxz[i] = (inc && in_z(row_of(i))) ? q0[i - 1] : 0.0f;
 #pragma unroll
-    for (int i = 1; i < NZZ; ++i) Qzz[i] = (inc && in_z(tau0 - LP - 1 + i)) ? q1[i - 1] : 0.0f;
+    for (int i = 1; i < NZZ; ++i) Qzz[i] = (inc && in_z(row_of(i))) ? q1[i - 1] : 0.0f;
   }
 
   for (int g = 0; g < nit; ++g) {
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const int tau = tau0 + g * D + d;
+      const int tau = tau0 + DIR * (g * D + d);
       Pre& P = pre[d];
-      // A. queues, staged txx / txz of row tau, new stresses of row t = tau - LP
+      // A. queues, staged txx / txz of row tau, new stresses of row t = tau + TOFF
       const bool iz = in_z(tau);
 #pragma unroll
       for (int i = 0; i < NXX - 1; ++i) Qxx[i] = Qxx[i + 1];
@@ -1110,16 +1138,16 @@ __device__ __forceinline__ void march_strip_el(const SweepArgs& A, const Work2D&
 #pragma unroll
       for (int i = 0; i < NZZ - 1; ++i) Qzz[i] = Qzz[i + 1];
       Qxx[NXX - 1] = (inc && iz) ? P.xx : 0.0f;
-      Qxz[NXZ - 1] = (inc && in_z(tau + N - 1)) ? P.qxz : 0.0f;
-      Qzz[NZZ - 1] = (inc && in_z(tau + N)) ? P.qzz : 0.0f;
+      Qxz[NXZ - 1] = (inc && in_z(tau + LXZ)) ? P.qxz : 0.0f;
+      Qzz[NZZ - 1] = (inc && in_z(tau + LZZ)) ? P.qzz : 0.0f;
       float* const sb = stg + (tau & 1) * 2 * SW;
-      sb[k + N] = Qxx[LP];        // txx(c, tau)
-      sb[SW + k + N] = Qxz[LP];   // txz(c, tau)
+      sb[k + N] = Qxx[sq(0)];        // txx(c, tau)
+      sb[SW + k + N] = Qxz[sq(0)];   // txz(c, tau)
       if (k < 2 * N) {
         sb[se] = (ine && iz) ? P.xxe : 0.0f;
         sb[SW + se] = (ine && iz) ? P.xze : 0.0f;
       }
-      const int t = tau - LP;
+      const int t = tau + TOFF;
       if (t >= j0 && t < j0 + h && own) {
         const float* VX = vxr + (t % NR) * NT + k;
         const float* VZ = vzr + (t % NR) * NT + k;
@@ -1128,14 +1156,14 @@ __device__ __forceinline__ void march_strip_el(const SweepArgs& A, const Work2D&
         for (int l = 1; l <= N; ++l) {
           const float cl = a[l - 1];
           dvxdx += cl * (VX[l] - VX[1 - l]);
-          dvzdz += cl * (QVZ[N - 1 + l] - QVZ[N - l]);
-          dvxdz += cl * (QVX[N + l] - QVX[N + 1 - l]);
+          dvzdz += cl * (QVZ[sv(TOFF + l - 1)] - QVZ[sv(TOFF - l)]);
+          dvxdz += cl * (QVX[sv(TOFF + l)] - QVX[sv(TOFF + 1 - l)]);
           dvzdx += cl * (VZ[l - 1] - VZ[-l]);
         }
         const float dt = dtx;
-        float txx = Qxx[0] + ((P.l2m * dvxdx) * dt + (P.lam * dvzdz) * dt);
-        float tzz = Qzz[0] + ((P.lam * dvxdx) * dt + (P.l2m * dvzdz) * dt);
-        const float txz = Qxz[0] + P.mu * (dvxdz * dt + dvzdx * dt);
+        float txx = Qxx[sq(TOFF)] + ((P.l2m * dvxdx) * dt + (P.lam * dvzdz) * dt);
+        float tzz = Qzz[sq(TOFF)] + ((P.lam * dvxdx) * dt + (P.l2m * dvzdz) * dt);
+        const float txz = Qxz[sq(TOFF)] + P.mu * (dvxdz * dt + dvzdx * dt);
         if (!force) {
           if (t == srow0) {
             txx += sw0;
@@ -1163,9 +1191,9 @@ __device__ __forceinline__ void march_strip_el(const SweepArgs& A, const Work2D&
           for (int l = 1; l <= N; ++l) {
             const float cl = a[l - 1];
             dtxxdx += cl * (XX[l - 1] - XX[-l]);
-            dtxzdz += cl * (Qxz[N + l] - Qxz[N + 1 - l]);
+            dtxzdz += cl * (Qxz[sq(l - 1)] - Qxz[sq(-l)]);
             dtxzdx += cl * (XZ[l] - XZ[1 - l]);
-            dtzzdz += cl * (Qzz[N + 1 + l] - Qzz[N + 2 - l]);
+            dtzzdz += cl * (Qzz[sq(l)] - Qzz[sq(1 - l)]);
           }
           const float dt = dtx;
           vx = P.vxo + P.bx * (dt * dtxxdx + dt * dtxzdz);
@@ -1184,8 +1212,10 @@ __device__ __forceinline__ void march_strip_el(const SweepArgs& A, const Work2D&
           if (snap_k && own && tau >= j0 && tau < j0 + h)
             G(snap_k)[(size_t)(c - pad) + (size_t)(tau - pad) * nxi] = vz;
         }
-        vxr[(tau % NR) * NT + k] = vx;
-        vzr[(tau % NR) * NT + k] = vz;
+        // an up strip's last row can be -1 (nbc = 0, j0 = N): its ring slot is NR - 1
+        const int rs = (tau + NR) % NR;
+        vxr[rs * NT + k] = vx;
+        vzr[rs * NT + k] = vz;
 #pragma unroll
         for (int i = 0; i < NV - 1; ++i) {
           QVX[i] = QVX[i + 1];
@@ -1194,29 +1224,37 @@ __device__ __forceinline__ void march_strip_el(const SweepArgs& A, const Work2D&
         QVX[NV - 1] = vx;
         QVZ[NV - 1] = vz;
       }
-      prefetch(tau + D, P);
+      prefetch(tau + DIR * D, P);
     }
   }
 }
 
+// Work items of kind 2 march up (see march_strip_el), those of kind 1 down.
 template <int N, class T, class TM>
 __global__ __launch_bounds__(256) void fused2d_el_kernel(const SweepArgs A) {
-  __shared__ float lds[MarchGeomEl<N>::LDS];
+  __shared__ float lds[std::max(MarchGeomEl<N, false>::LDS, MarchGeomEl<N, true>::LDS)];
   if ((int)blockIdx.x >= A.n_work) return;
   const Work2D W = A.work[blockIdx.x];
   if (W.kind < 0) return;
   const Shot2D& S = A.shots[W.shot];
-  march_strip_el<N, T, TM>(A, W, S, lds, *A.step + A.koff);
+  if (W.kind == 2)
+    march_strip_el<N, T, TM, true>(A, W, S, lds, *A.step + A.koff);
+  else
+    march_strip_el<N, T, TM, false>(A, W, S, lds, *A.step + A.koff);
 }
 
+// Work items of kind 2 march up (see march_strip), those of kind 1 down.
 template <int N, class T, class TM>
 __global__ __launch_bounds__(256) void fused2d_kernel(const SweepArgs A) {
-  __shared__ float lds[MarchGeom<N>::LDS];
+  __shared__ float lds[std::max(MarchGeom<N, false>::LDS, MarchGeom<N, true>::LDS)];
   if ((int)blockIdx.x >= A.n_work) return;
   const Work2D W = A.work[blockIdx.x];
   if (W.kind < 0) return;
   const Shot2D& S = A.shots[W.shot];
-  march_strip<N, T, TM>(A, W, S, lds, *A.step + A.koff);
+  if (W.kind == 2)
+    march_strip<N, T, TM, true>(A, W, S, lds, *A.step + A.koff);
+  else
+    march_strip<N, T, TM, false>(A, W, S, lds, *A.step + A.koff);
 }
 
 template <int N, class T, class TM>

@@ -75,18 +75,21 @@ static std::vector<Work2D> xcd_layout(const std::vector<Work2D>& items) {
   return flat;
 }
 
-// Rows of one interior strip of the fused launch: 64, 96, 128, 176 or 256 (FOMO_FUSED2D_ROWS
-// forces a length, rounded up to a multiple of 8).  Each strip recomputes velocities on about 2N
-// rows past its ends and re-reads as many rows of its inputs, so longer strips move fewer bytes;
-// the block count also decides how evenly the last round of blocks fills the chip.  At 8192^2 and
-// FD8 (33 columns of 249-column strips) the acoustic2d interior launch takes 0.566, 0.588, 0.559
-// and 0.603 ms at 96, 128, 176 and 256 rows: the autotuner times them all.
+// Rows of one interior strip of the fused launch: 24, 32, 48, 96 or 176 (FOMO_FUSED2D_ROWS forces
+// a length, rounded up to a multiple of 8).  Each strip recomputes velocities on about 2N rows
+// past its ends and reads as many rows of its inputs again.  Strips alternate their march
+// direction by row band, so that two strips read those rows at the same time and the second read
+// finds them in the caches; short strips then cost little: at 8192^2 FD8 the acoustic2d step
+// takes 0.576 / 0.575 / 0.578 / 0.582 ms at 16 / 24 / 32 / 40 rows, against 0.601 ms with every
+// strip marching down at its best length (176), and elastic2d 1.009 / 1.002 / 1.013 / 1.045 ms at
+// 24 / 32 / 48 / 96 rows against 1.042 ms (16-bit storage: fastest at 32-48 rows).  The autotuner
+// times every length.
 static int fused_rows(int cfg) {
   if (const char* env = std::getenv("FOMO_FUSED2D_ROWS")) {
     const int r = std::atoi(env);
     if (r > 0) return std::min(4096, (r + 7) / 8 * 8);
   }
-  static const int rows[kNumFusedCfgs] = {64, 96, 128, 176, 256};
+  static const int rows[kNumFusedCfgs] = {24, 32, 48, 96, 176};
   return rows[cfg];
 }
 
@@ -112,9 +115,11 @@ int Plan2D::make_work(int cfg) {
       // than one of the full width and a sliver, whose block would march every row for a few columns
       const int nstrip = std::max(1, (meta[s].nxi + cols - 1) / cols);
       const int cw = (meta[s].nxi + nstrip - 1) / nstrip;
-      for (int j = pad; j < ze; j += rows)
+      // strips of odd row bands march up (kind 2), so that vertically adjacent strips read the rows
+      // they share at the same time
+      for (int j = pad, band = 0; j < ze; j += rows, ++band)
         for (int i = pad; i < xe; i += cw)
-          inner.push_back(Work2D{(int)s, 1, i, j, std::min(cw, xe - i), std::min(rows, ze - j)});
+          inner.push_back(Work2D{(int)s, (band & 1) ? 2 : 1, i, j, std::min(cw, xe - i), std::min(rows, ze - j)});
     }
     if (!d_frame) {
       const std::vector<Work2D> ff = xcd_layout(frame);

@@ -108,8 +108,8 @@ typedef struct fomo_desc {
   int32_t use_graph;     /* 1: one hipGraph launch per pair of steps; 0: direct launches */
   int32_t record_p;      /* acoustic3d: also gather p (extension) */
   int32_t tile_cfg;      /* 0: autotune the tile shape; k > 0: fixed shape k-1 (2D sweeps: interior
-                            strips of 16, 32, 64 rows; acoustic2d / elastic2d fused interior: 64, 96,
-                            128, 176, 256 rows; 3D: z-segments of 16 << (k-1) planes, of the fused acoustic3d
+                            strips of 16, 32, 64 rows; acoustic2d / elastic2d fused interior: 24, 32,
+                            48, 96, 176 rows; 3D: z-segments of 16 << (k-1) planes, of the fused acoustic3d
                             interior 32 << (k-1)) */
   int32_t boundary;      /* FOMO_BOUNDARY_HABC or FOMO_BOUNDARY_SPONGE (2D) */
   float sponge_factor;   /* Cerjan factor (reference default 0.015); used with FOMO_BOUNDARY_SPONGE */

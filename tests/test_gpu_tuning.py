@@ -100,7 +100,7 @@ def test_off_keeps_no_file_and_fixed_shapes_are_not_tuned(gpu, tmp_path, monkeyp
     assert not any(tmp_path.iterdir())
     # a forced elastic2d shape is one of the fused interior's strip lengths, or of the two sweeps'
     t, _ = _plan2d(tile_cfg=2)
-    assert (t["source"], t["family"], t["shape"], t["extent"]) == ("fixed", "fused", 1, 96), t
+    assert (t["source"], t["family"], t["shape"], t["extent"]) == ("fixed", "fused", 1, 32), t
     monkeypatch.setenv("FOMO_2D_UNFUSED", "1")
     t, _ = _plan2d(tile_cfg=2)
     assert (t["source"], t["family"], t["shape"], t["extent"]) == ("fixed", "sweeps", 1, 32), t
