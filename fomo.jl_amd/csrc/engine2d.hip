@@ -382,13 +382,15 @@ struct StripRd {
 // is issued at once: the stencil fields with an (N+1) halo, old values and materials of the tile
 // plus a one-cell ring, and the old values each frame cell's Higdon term needs.  Pre-correction
 // values of the tile and ring then go to LDS, and each frame cell blends with its inward
-// neighbour's pre-correction value.
-template <int PH, int N, class T, class TM>
+// neighbour's pre-correction value.  BZ rows of a tile, TY thread rows of the block (64 x TY
+// threads, BZ / TY rows per thread).
+template <int PH, int N, class T, class TM, int BZ = kFrameRows, int TY = 4>
 __device__ __forceinline__ void frame_tile(const SweepArgs& A, const Work2D& W, const Shot2D& S, const PV& V,
                                            float* lds, int it) {
   using P = Phys<PH, N, T>;
   constexpr int NF = P::NF, NS = P::NS, NM = P::NM, NMA = NM > 0 ? NM : 1, HN = P::HALO;
-  constexpr int BX = kTileW, BZ = kFrameRows, ZPT = BZ / 4, NTH = BX * 4;
+  constexpr int BX = kTileW, ZPT = BZ / TY, NTH = BX * TY;
+  static_assert(BZ % TY == 0, "whole rows per thread");
   constexpr int TW = BX + 2 * HN + 2, TH = BZ + 2 * HN + 2;
   constexpr int EW = BX + 2, EH = BZ + 2;
   constexpr int KS = (TW * TH + NTH - 1) / NTH;  // staged cells per thread
@@ -1348,6 +1350,29 @@ __global__ __launch_bounds__(256) void sweep2d_mat16_kernel(const SweepArgs A) {
   sweep2d_body<PH, N, float, __half>(A);
 }
 
+// The frame launches of the fused families: frame tiles of kFrameRowsF rows in blocks of 64 x 8
+// threads (the same two rows per thread as the 8-row tiles of sweep2d_kernel, whose interior strips
+// set its block shape), so that a round of loads covers twice the cells and the staged halo rows
+// weigh half as much; blocks past the work list record the previous step's receivers.
+template <int PH, int N, class T, class TM>
+__global__ __launch_bounds__(kTileW * 8) void frame2d_kernel(const SweepArgs A) {
+  using P = Phys<PH, N, T>;
+  constexpr int NF = P::NF, NS = P::NS, HN = P::HALO;
+  __shared__ float lds[NS * (kTileW + 2 * HN + 2) * (kFrameRowsF + 2 * HN + 2) + NF * (kTileW + 2) * (kFrameRowsF + 2)];
+  const int it = *A.step + A.koff;
+  if ((int)blockIdx.x >= A.n_work) {
+    const int t = (int)(threadIdx.x + threadIdx.y * kTileW);
+    if (t < kGatherBlock) gather_entry<T>(A.g, ((int)blockIdx.x - A.n_work) * kGatherBlock + t, it - 1, A.parity);
+    return;
+  }
+  const Work2D W = A.work[blockIdx.x];
+  if (W.kind < 0) return;
+  const Shot2D& S = A.shots[W.shot];
+  PV V;
+  make_view<PH>(S, A.parity, it, V);
+  frame_tile<PH, N, T, TM, kFrameRowsF, 8>(A, W, S, V, lds, it);
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void gather2d_kernel(const GatherArgs g, const int* step, int koff, int parity) {
   gather_entry<T>(g, (int)(blockIdx.x * blockDim.x + threadIdx.x), *step + koff - 1, parity);
@@ -1421,6 +1446,41 @@ hipError_t launch_sweep2d(int storage, int phase, int N, int tile_cfg, const Swe
     case FOMO_STORE_F16: return launch_t2<__half>(phase, N, a, s);
     case FOMO_STORE_BF16: return launch_t2<__hip_bfloat16>(phase, N, a, s);
     case FOMO_STORE_F32_MAT16: return launch_t2<Mat16>(phase, N, a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int PH, class T, class TM>
+static hipError_t launch_frame_n(int N, const SweepArgs& a, hipStream_t s) {
+  const dim3 grid(a.n_work + a.n_gather), blk(kTileW, 8);
+  switch (N) {
+    case 1: hipLaunchKernelGGL((frame2d_kernel<PH, 1, T, TM>), grid, blk, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((frame2d_kernel<PH, 2, T, TM>), grid, blk, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((frame2d_kernel<PH, 3, T, TM>), grid, blk, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((frame2d_kernel<PH, 4, T, TM>), grid, blk, 0, s, a); break;
+    case 5: hipLaunchKernelGGL((frame2d_kernel<PH, 5, T, TM>), grid, blk, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <class T, class TM>
+static hipError_t launch_frame_t(int phase, int N, const SweepArgs& a, hipStream_t s) {
+  switch (phase) {
+    case kAcVel: return launch_frame_n<kAcVel, T, TM>(N, a, s);
+    case kAcPrs: return launch_frame_n<kAcPrs, T, TM>(N, a, s);
+    case kElVel: return launch_frame_n<kElVel, T, TM>(N, a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_frame2d(int storage, int phase, int N, const SweepArgs& a, hipStream_t s) {
+  if (a.n_work + a.n_gather <= 0) return hipSuccess;
+  switch (storage) {
+    case FOMO_STORE_F32: return launch_frame_t<float, float>(phase, N, a, s);
+    case FOMO_STORE_F16: return launch_frame_t<__half, __half>(phase, N, a, s);
+    case FOMO_STORE_BF16: return launch_frame_t<__hip_bfloat16, __hip_bfloat16>(phase, N, a, s);
+    case FOMO_STORE_F32_MAT16: return launch_frame_t<float, __half>(phase, N, a, s);
     default: return hipErrorInvalidValue;
   }
 }

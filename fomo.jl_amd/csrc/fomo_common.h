@@ -123,6 +123,10 @@ hipError_t launch_sweep2d(int storage, int phase, int N, int tile_cfg, const Swe
 // the interior strips in a.work (kind 1, at most fused2d_width(eq, N) columns each); the frame
 // velocity launch runs before it and the frame pressure / stress launch after it.
 hipError_t launch_fused2d(int storage, int eq, int N, const SweepArgs& a, hipStream_t s);
+// The frame launches beside it (phases kAcVel / kAcPrs / kElVel; elastic2d's stress frame runs
+// launch_sweep2d over 8-row tiles): the frame tiles in a.work, of kFrameRowsF rows, and in the
+// velocity phase the gather blocks.
+hipError_t launch_frame2d(int storage, int phase, int N, const SweepArgs& a, hipStream_t s);
 int fused2d_width(int eq, int N);
 // Record the state held in buffer [parity] into column (*step + koff - 1).
 hipError_t launch_gather2d(int storage, const GatherArgs& g, const int* step, int koff, int parity, hipStream_t s);
@@ -131,6 +135,7 @@ hipError_t launch_decode(int storage, const void* src, float* dst, size_t n, flo
 int strip_rows(int tile_cfg);   // rows (z) of one interior strip
 constexpr int kTileW = 64;      // tile / strip width (x): one wave64 row
 constexpr int kFrameRows = 8;   // rows of a frame tile
+constexpr int kFrameRowsF = 16; // rows of a frame tile of the fused families' frame launches
 constexpr int kNumTileCfgs = 3;
 constexpr int kNumFusedCfgs = 5;  // strip lengths of the fused acoustic2d / elastic2d interior launch
 

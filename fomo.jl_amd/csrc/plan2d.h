@@ -31,15 +31,19 @@ struct Plan2D : PlanBase {
   Shot2D* d_shots = nullptr;
   Work2D* d_work[kNumTileCfgs] = {nullptr};
   int n_work[kNumTileCfgs] = {0};
-  // acoustic2d: the interior's whole step in one launch (launch_fused2d) over d_fwork[cfg]
-  // (interior strips of fused_rows(cfg) rows), the velocity and pressure launches over the frame
-  // tiles alone (d_frame).  fusable: the plan allows it (acoustic2d, at most two sources per
-  // column); fused: the current tile shape is one of the fused family.
+  // acoustic2d / elastic2d: the interior's whole step in one launch (launch_fused2d) over
+  // d_fwork[cfg] (interior strips of fused_rows(cfg) rows), the frame launches over the frame tiles
+  // alone: d_frame, tiles of kFrameRowsF rows for launch_frame2d; elastic2d's stress frame d_frame8,
+  // tiles of kFrameRows rows for launch_sweep2d (its 512-thread frame blocks run fewer waves per
+  // CU: 8192^2 FD8 25.4 -> 28.4 us).  fusable: the plan allows it (at most two sources per column);
+  // fused: the current tile shape is one of the fused family.
   bool fusable = false, fused = false;
   Work2D* d_fwork[kNumFusedCfgs] = {nullptr};
   int n_fwork[kNumFusedCfgs] = {0};
   Work2D* d_frame = nullptr;
   int n_frame = 0;
+  Work2D* d_frame8 = nullptr;
+  int n_frame8 = 0;
   int only = -1;  // enqueue_part: only the frame launch (1) or only the fused launch (0) when >= 0
   int n_entries = 0;
   int* rec_shot = nullptr;

@@ -20,7 +20,8 @@ Plan2D::~Plan2D() = default;
 
 // Work items of a tile shape: frame tiles (64 x kFrameRows) over the four pad bands and interior
 // strips (64 x strip_rows) over the user's interior, for every shot, sorted by (shot, row, column).
-static std::vector<Work2D> work_items(const std::vector<Shot2D>& shots, const std::vector<ShotMeta>& meta, int cfg) {
+static std::vector<Work2D> work_items(const std::vector<Shot2D>& shots, const std::vector<ShotMeta>& meta, int cfg,
+                                      int frame_rows = kFrameRows) {
   std::vector<Work2D> items;
   const int zs = strip_rows(cfg);
   auto add_rect = [&](int s, int kind, int x0, int x1, int z0, int z1, int th) {
@@ -31,11 +32,11 @@ static std::vector<Work2D> work_items(const std::vector<Shot2D>& shots, const st
   for (size_t s = 0; s < shots.size(); ++s) {
     const int nx = shots[s].nx, nz = shots[s].nz, pad = meta[s].pad;
     const int xe = pad + meta[s].nxi, ze = pad + meta[s].nzi;
-    add_rect((int)s, 0, 0, nx, 0, pad, kFrameRows);       // top band
-    add_rect((int)s, 0, 0, pad, pad, ze, kFrameRows);     // left band
+    add_rect((int)s, 0, 0, nx, 0, pad, frame_rows);       // top band
+    add_rect((int)s, 0, 0, pad, pad, ze, frame_rows);     // left band
     add_rect((int)s, 1, pad, xe, pad, ze, zs);            // interior
-    add_rect((int)s, 0, xe, nx, pad, ze, kFrameRows);     // right band
-    add_rect((int)s, 0, 0, nx, ze, nz, kFrameRows);       // bottom band
+    add_rect((int)s, 0, xe, nx, pad, ze, frame_rows);     // right band
+    add_rect((int)s, 0, 0, nx, ze, nz, frame_rows);       // bottom band
   }
   std::stable_sort(items.begin(), items.end(), [](const Work2D& a, const Work2D& b) {
     if (a.shot != b.shot) return a.shot < b.shot;
@@ -106,7 +107,7 @@ int Plan2D::make_work(int cfg) {
   }
   if (fused) {
     std::vector<Work2D> frame, inner;
-    for (const Work2D& w : work_items(shots, meta, 0))
+    for (const Work2D& w : work_items(shots, meta, 0, kFrameRowsF))
       if (w.kind == 0) frame.push_back(w);
     const int rows = fused_rows(cfg), cols = fused2d_width(d.eq, N);
     for (size_t s = 0; s < shots.size(); ++s) {
@@ -125,6 +126,15 @@ int Plan2D::make_work(int cfg) {
       const std::vector<Work2D> ff = xcd_layout(frame);
       n_frame = (int)ff.size();
       const int rc = upload(&d_frame, ff.data(), ff.size());
+      if (rc) return rc;
+    }
+    if (d.eq == FOMO_ELASTIC2D && !d_frame8) {
+      std::vector<Work2D> frame8;
+      for (const Work2D& w : work_items(shots, meta, 0))
+        if (w.kind == 0) frame8.push_back(w);
+      const std::vector<Work2D> ff = xcd_layout(frame8);
+      n_frame8 = (int)ff.size();
+      const int rc = upload(&d_frame8, ff.data(), ff.size());
       if (rc) return rc;
     }
     const std::vector<Work2D> fi = xcd_layout(inner);
@@ -169,9 +179,13 @@ hipError_t Plan2D::enqueue_part(int part, int s, int koff) {
     // whole step; part 1: the frame's pressure or stresses
     a.work = d_frame;
     a.n_work = n_frame;
-    if (part == 1)
-      return only == 0 ? hipSuccess
-                       : launch_sweep2d(sweep_storage, d.eq == FOMO_ACOUSTIC2D ? kAcPrs : kElStr, N, tile_cfg, a, stream);
+    if (part == 1 && only == 0) return hipSuccess;
+    if (part == 1 && d.eq == FOMO_ELASTIC2D) {
+      a.work = d_frame8;
+      a.n_work = n_frame8;
+      return launch_sweep2d(sweep_storage, kElStr, N, tile_cfg, a, stream);
+    }
+    if (part == 1) return launch_frame2d(sweep_storage, kAcPrs, N, a, stream);
   } else if (part == 1 && coupled) {
     a.work = d_frame;  // the interior's position half-steps ran in the velocity launch
     a.n_work = n_frame;
@@ -204,7 +218,7 @@ hipError_t Plan2D::enqueue_part(int part, int s, int koff) {
   const int ph = d.eq == FOMO_ACOUSTIC2D ? kAcVel : (d.eq == FOMO_ELASTIC2D ? kElVel : (coupled ? kCpVel : kScal));
   if (!fused) return launch_sweep2d(sweep_storage, ph, N, tile_cfg, a, stream);
   if (only != 0) {
-    const hipError_t e = launch_sweep2d(sweep_storage, ph, N, tile_cfg, a, stream);
+    const hipError_t e = launch_frame2d(sweep_storage, ph, N, a, stream);
     if (e != hipSuccess || only == 1) return e;
   }
   a.work = d_fwork[tile_cfg];
