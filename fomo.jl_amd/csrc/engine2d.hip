@@ -754,7 +754,7 @@ struct MarchGeom {
   static constexpr int LDS = 2 * SW + NR * NT;
 };
 
-int fused2d_width(int eq, int N) { return eq == FOMO_ELASTIC2D ? 256 - 2 * N : 257 - 2 * N; }
+int fused2d_width(int eq, int N) { return eq == FOMO_ACOUSTIC2D ? 257 - 2 * N : 256 - 2 * N; }
 
 template <int N, class T, class TM, bool UP>
 __device__ __forceinline__ void march_strip(const SweepArgs& A, const Work2D& W, const Shot2D& S, float* lds,
@@ -1231,6 +1231,133 @@ __device__ __forceinline__ void march_strip_el(const SweepArgs& A, const Work2D&
   }
 }
 
+// Scalar interior strip (scalar2d leapfrog, u^{it+1} = (2 u^it - u^{it-1}) + c2 lap u^it with the
+// separate sweep's expression order, src/equations/scalar2d/scalar2d.jl): one pass down (or, kind
+// 2, up) the rows of a strip of 256 - 2N columns.  Lane k owns column c = i0 - N + k, the strip's
+// columns are lanes N .. N+w-1, so every x-neighbour of a strip cell is another lane's column and
+// nothing past the 256 columns is staged.  Each lane keeps u^it of its column over rows tau - N ..
+// tau + N in a register queue (the z-stencil), stages u^it(tau) in one of two LDS rows (the
+// x-stencil) and, after one barrier, writes u^{it+1}(tau): the update reads only the state before
+// the step, so there is no lag and one barrier per row.  Sources (at most two per column, in
+// source order) add to u^{it+1} as in the sweep, 16-bit storage as there too; like the sweep (and
+// the reference's scalar2d) it records no snapshots.
+template <int N, class T, bool UP>
+__device__ __forceinline__ void march_strip_sc(const SweepArgs& A, const Work2D& W, const Shot2D& S, float* lds,
+                                               int it) {
+  constexpr int NT = 256, NQ = 2 * N + 1, D = 2;
+  constexpr int DIR = UP ? -1 : 1, LEAD = UP ? -N : N;
+  auto qi = [](int off) constexpr { return UP ? N - off : N + off; };  // queue slot of row tau + off
+  const int nx = S.nx, nz = S.nz;
+  const int i0 = W.i0, j0 = W.j0, h = W.h, w = W.w;
+  const int k = threadIdx.x;
+  const int c = i0 - N + k;
+  const bool inc = c >= 0 && c < nx;
+  const int cc = min(max(c, 0), nx - 1);
+  const bool own = k >= N && k - N < w;
+  auto rowc = [=](int j) { return min(max(j, 0), nz - 1); };
+  auto in_z = [=](int j) { return j >= 0 && j < nz; };
+  const int cur = it % 3, prv = (it + 2) % 3, nxt = (it + 1) % 3;
+  const void* const u_s = S.fld[0][cur];
+  const void* const up_s = S.fld[0][prv];
+  void* const u_o = S.fld[0][nxt];
+  const void* const c2_s = S.mat[0];
+  const float dec = S.fdec[0], enc = S.fenc[0], mdec = S.mdec[0];
+  unsigned* const ovf = S.ovf;
+  const float c0 = A.c0;
+  auto ld = [=](const void* base, int j, int i) -> float {
+    return Cvt<T>::ld(base, (unsigned)(j * nx) + (unsigned)i, dec);
+  };
+  float a[N];
+#pragma unroll
+  for (int l = 0; l < N; ++l) a[l] = A.a[l];
+  int srow0 = -1, srow1 = -1;
+  float sw0 = 0.0f, sw1 = 0.0f;
+  if (it < A.nt && own) {
+    int q2 = 0;
+    for (int q = 0; q < S.n_src; ++q) {
+      const int cs = G(S.src_c)[q];
+      const int js = cs / nx, is = cs - js * nx;
+      if (is == c && js >= j0 && js < j0 + h) {
+        const float wq = G(S.wav)[(size_t)q + (size_t)S.n_src * (size_t)it];
+        if (q2 == 0) {
+          srow0 = js;
+          sw0 = wq;
+        } else {
+          srow1 = js;
+          sw1 = wq;
+        }
+        ++q2;
+      }
+    }
+  }
+  // loads of row rho: u^it(rho + LEAD) for the queue, u^{it-1}(rho) and c2(rho)
+  struct Pre {
+    float u, up, c2;
+  };
+  auto prefetch = [=](int rho, Pre& P) __attribute__((always_inline)) {
+    P.u = ld(u_s, rowc(rho + LEAD), cc);
+    P.up = ld(up_s, rowc(rho), cc);
+    P.c2 = Cvt<T>::ld(c2_s, (unsigned)(rowc(rho) * nx) + (unsigned)cc, mdec);
+  };
+  const int tau0 = UP ? j0 + h - 1 : j0;
+  const int nit = (h + D - 1) / D;
+  Pre pre[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) prefetch(tau0 + DIR * d, pre[d]);
+  // queue before the first row: slot i >= 1 holds row tau0 - DIR + (UP ? N - i : i - N)
+  float Q[NQ];
+  Q[0] = 0.0f;
+  {
+    float q0[NQ - 1];
+#pragma unroll
+    for (int i = 1; i < NQ; ++i) q0[i - 1] = ld(u_s, rowc(tau0 - DIR + (UP ? N - i : i - N)), cc);
+#pragma unroll
+    for (int i = 1; i < NQ; ++i) Q[i] = (inc && in_z(tau0 - DIR + (UP ? N - i : i - N))) ? q0[i - 1] : 0.0f;
+  }
+  for (int g = 0; g < nit; ++g) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int tau = tau0 + DIR * (g * D + d);
+      Pre& P = pre[d];
+      if (g * D + d < h) {
+#pragma unroll
+        for (int i = 0; i < NQ - 1; ++i) Q[i] = Q[i + 1];
+        Q[NQ - 1] = (inc && in_z(tau + LEAD)) ? P.u : 0.0f;
+        float* const sb = lds + (tau & 1) * NT;
+        sb[k] = Q[qi(0)];
+        __syncthreads();
+        if (own) {
+          const float uc = Q[qi(0)];
+          float lap = (2.0f * c0) * uc;
+#pragma unroll
+          for (int l = 1; l <= N; ++l) {
+            const float cl = a[l - 1];
+            lap += cl * (((sb[k + l] + sb[k - l]) + Q[qi(l)]) + Q[qi(-l)]);
+          }
+          float v = (2.0f * uc - P.up) + P.c2 * lap;
+          if (tau == srow0) v += sw0;
+          if (tau == srow1) v += sw1;
+          st_field<T>(u_o, (unsigned)(tau * nx) + (unsigned)c, v, enc, ovf);
+        }
+      }
+      prefetch(tau + DIR * D, P);
+    }
+  }
+}
+
+template <int N, class T>
+__global__ __launch_bounds__(256) void fused2d_sc_kernel(const SweepArgs A) {
+  __shared__ float lds[2 * 256];
+  if ((int)blockIdx.x >= A.n_work) return;
+  const Work2D W = A.work[blockIdx.x];
+  if (W.kind < 0) return;
+  const Shot2D& S = A.shots[W.shot];
+  if (W.kind == 2)
+    march_strip_sc<N, T, true>(A, W, S, lds, *A.step + A.koff);
+  else
+    march_strip_sc<N, T, false>(A, W, S, lds, *A.step + A.koff);
+}
+
 // Work items of kind 2 march up (see march_strip_el), those of kind 1 down.
 template <int N, class T, class TM>
 __global__ __launch_bounds__(256) void fused2d_el_kernel(const SweepArgs A) {
@@ -1283,8 +1410,30 @@ static hipError_t launch_fused2d_t(bool el, int N, const SweepArgs& a, hipStream
   }
 }
 
+template <class T>
+static hipError_t launch_fused2d_sc(int N, const SweepArgs& a, hipStream_t s) {
+  const dim3 grid(a.n_work), blk(256);
+  switch (N) {
+    case 1: hipLaunchKernelGGL((fused2d_sc_kernel<1, T>), grid, blk, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((fused2d_sc_kernel<2, T>), grid, blk, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((fused2d_sc_kernel<3, T>), grid, blk, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((fused2d_sc_kernel<4, T>), grid, blk, 0, s, a); break;
+    case 5: hipLaunchKernelGGL((fused2d_sc_kernel<5, T>), grid, blk, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_fused2d(int storage, int eq, int N, const SweepArgs& a, hipStream_t s) {
   if (a.n_work <= 0) return hipSuccess;
+  if (eq == FOMO_SCALAR2D) {
+    switch (storage) {
+      case FOMO_STORE_F32: return launch_fused2d_sc<float>(N, a, s);
+      case FOMO_STORE_F16: return launch_fused2d_sc<__half>(N, a, s);
+      case FOMO_STORE_BF16: return launch_fused2d_sc<__hip_bfloat16>(N, a, s);
+      default: return hipErrorInvalidValue;  // the scalar keeps a wavefield among its materials
+    }
+  }
   if (eq != FOMO_ACOUSTIC2D && eq != FOMO_ELASTIC2D) return hipErrorInvalidValue;
   const bool el = eq == FOMO_ELASTIC2D;
   switch (storage) {
@@ -1470,6 +1619,11 @@ static hipError_t launch_frame_t(int phase, int N, const SweepArgs& a, hipStream
     case kAcVel: return launch_frame_n<kAcVel, T, TM>(N, a, s);
     case kAcPrs: return launch_frame_n<kAcPrs, T, TM>(N, a, s);
     case kElVel: return launch_frame_n<kElVel, T, TM>(N, a, s);
+    case kScal:
+      if constexpr (std::is_same<T, TM>::value)
+        return launch_frame_n<kScal, T, TM>(N, a, s);
+      else
+        return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }
 }

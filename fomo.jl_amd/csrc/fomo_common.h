@@ -120,12 +120,13 @@ struct SweepArgs {
 // FOMO_STORE_F32 / F16 / BF16.
 hipError_t launch_sweep2d(int storage, int phase, int N, int tile_cfg, const SweepArgs& a, hipStream_t s);
 // acoustic2d / elastic2d (eq): the whole step (new velocities, then new pressure or stresses) of
-// the interior strips in a.work (kind 1, at most fused2d_width(eq, N) columns each); the frame
-// velocity launch runs before it and the frame pressure / stress launch after it.
+// the interior strips in a.work (kind 1 marching down, 2 up; at most fused2d_width(eq, N) columns
+// each); the frame velocity launch runs before it and the frame pressure / stress launch after it.
+// scalar2d: the interior's leapfrog step; its frame launch (with the gather blocks) runs before it.
 hipError_t launch_fused2d(int storage, int eq, int N, const SweepArgs& a, hipStream_t s);
-// The frame launches beside it (phases kAcVel / kAcPrs / kElVel; elastic2d's stress frame runs
-// launch_sweep2d over 8-row tiles): the frame tiles in a.work, of kFrameRowsF rows, and in the
-// velocity phase the gather blocks.
+// The frame launches beside it (phases kAcVel / kAcPrs / kElVel / kScal; elastic2d's stress frame
+// runs launch_sweep2d over 8-row tiles): the frame tiles in a.work, of kFrameRowsF rows, and in the
+// first phase of a step the gather blocks.
 hipError_t launch_frame2d(int storage, int phase, int N, const SweepArgs& a, hipStream_t s);
 int fused2d_width(int eq, int N);
 // Record the state held in buffer [parity] into column (*step + koff - 1).
@@ -137,6 +138,6 @@ constexpr int kTileW = 64;      // tile / strip width (x): one wave64 row
 constexpr int kFrameRows = 8;   // rows of a frame tile
 constexpr int kFrameRowsF = 16; // rows of a frame tile of the fused families' frame launches
 constexpr int kNumTileCfgs = 3;
-constexpr int kNumFusedCfgs = 5;  // strip lengths of the fused acoustic2d / elastic2d interior launch
+constexpr int kNumFusedCfgs = 5;  // strip lengths of the fused acoustic2d / elastic2d / scalar2d interior launch
 
 }  // namespace fomo

@@ -85,7 +85,9 @@ struct Plan2D : PlanBase {
   int n_shots() const override { return (int)shots.size(); }
   // Launches of a step: velocity, pressure/stress (scalar: one); fused: velocity frame, interior
   // step, pressure frame.
-  int n_launches() const override { return fused ? 3 : (d.eq == FOMO_SCALAR2D ? 1 : 2); }
+  int n_launches() const override {
+    return fused ? (d.eq == FOMO_SCALAR2D ? 2 : 3) : (d.eq == FOMO_SCALAR2D ? 1 : 2);
+  }
   LaunchInfo launch_info(int k) const override;
   hipError_t enqueue_launch(int k, int s) override {
     if (!fused) return enqueue_part(k, s, 0);

@@ -239,6 +239,10 @@ PlanBase::LaunchInfo Plan2D::launch_info(int k) const {
     // cell, elastic 9 + 11)
     long long inner = 0;
     for (const auto& m : meta) inner += (long long)m.nxi * m.nzi;
+    if (d.eq == FOMO_SCALAR2D) {
+      if (k == 0) return {"scalar2d frame", cells - inner, 4 * elem};
+      return {"scalar2d interior step", inner, 4 * elem};
+    }
     if (d.eq == FOMO_ELASTIC2D) {
       if (k == 0) return {"elastic2d velocity frame", cells - inner, 9 * elem};
       if (k == 1) return {"elastic2d interior step (velocity + stress)", inner, 20 * elem};
@@ -492,9 +496,10 @@ int create_plan2d(const fomo_desc* d, const float* vp, const float* vs, const fl
   p->storage = mat16 ? FOMO_STORE_F32 : d->storage;
   p->mat_storage = mat16 ? FOMO_STORE_F16 : d->storage;
   p->sweep_storage = d->storage;
-  // acoustic2d and elastic2d may fuse the interior's two phases (the autotuner decides; a forced
-  // tile shape is one of the fused family unless FOMO_2D_UNFUSED=1)
-  p->fusable = d->eq == FOMO_ACOUSTIC2D || d->eq == FOMO_ELASTIC2D;
+  // acoustic2d and elastic2d may fuse the interior's two phases, scalar2d run its interior as a
+  // row march (the autotuner decides; a forced tile shape is one of that family unless
+  // FOMO_2D_UNFUSED=1)
+  p->fusable = d->eq == FOMO_ACOUSTIC2D || d->eq == FOMO_ELASTIC2D || d->eq == FOMO_SCALAR2D;
   p->elem = p->storage == FOMO_STORE_F32 ? 4 : 2;
   p->N = d->fd_order / 2;
   p->coupled = coupled;
